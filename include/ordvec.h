@@ -1,0 +1,252 @@
+/*
+ * ordvec C ABI for the MI355X engine.
+ *
+ * Part 1 (ABI v1) is the drop-in boundary: the same symbols, struct layouts
+ * (128 / 128 / 24 / 184 bytes), status codes, capability bits and validation
+ * order as the reference's ordvec-ffi (ordvec-ffi/include/ordvec.h:1-260,
+ * implementation ordvec-ffi/src/lib.rs:600-998).  A C or cgo caller of the
+ * reference links against libordvec_amd.so instead and recompiles nothing.
+ *
+ * Part 2 (additive, ORDVEC_ABI_EXT_VERSION) exposes what ABI v1 cannot
+ * express: building indexes, batched and two-stage search, the .ovsb/.ovfs
+ * formats and device-resident (HBM) buffers.  Every entry point names the
+ * reference function whose semantics it carries.
+ *
+ * Threading: every function may be called from any thread.  Calls are
+ * serialised on one HIP stream per process; index handles may be shared for
+ * concurrent searches but must not be freed while in use.
+ */
+#ifndef ORDVEC_H
+#define ORDVEC_H
+
+#include <stdint.h>
+
+#define ORDVEC_ABI_VERSION 1
+#define ORDVEC_ABI_EXT_VERSION 1
+
+#define ORDVEC_CAP_FULL_SEARCH (1 << 0)
+#define ORDVEC_CAP_SUBSET_SEARCH (1 << 1)
+#define ORDVEC_CAP_STATS (1 << 2)
+#define ORDVEC_CAP_ID_EQUALS_ROW_ID (1 << 3)
+
+#define ORDVEC_SEARCH_FLAG_NONE 0
+
+typedef uint32_t ordvec_status_t;
+typedef uint32_t ordvec_index_kind_t;
+
+#define ORDVEC_STATUS_OK 0
+#define ORDVEC_STATUS_NULL_POINTER 1
+#define ORDVEC_STATUS_BAD_ARGUMENT 2
+#define ORDVEC_STATUS_BAD_STRUCT_SIZE 3
+#define ORDVEC_STATUS_UNSUPPORTED_FORMAT 4
+#define ORDVEC_STATUS_CORRUPT_INDEX 5
+#define ORDVEC_STATUS_IO 6
+#define ORDVEC_STATUS_DIM_MISMATCH 7
+#define ORDVEC_STATUS_NONFINITE_QUERY 8
+#define ORDVEC_STATUS_ROW_ID_OUT_OF_RANGE 9
+#define ORDVEC_STATUS_BUFFER_TOO_SMALL 10
+#define ORDVEC_STATUS_UNSUPPORTED_OPERATION 11
+#define ORDVEC_STATUS_PANIC 12
+#define ORDVEC_STATUS_INTERNAL 13
+
+#define ORDVEC_INDEX_KIND_UNKNOWN 0
+#define ORDVEC_INDEX_KIND_RANK_QUANT 1
+#define ORDVEC_INDEX_KIND_BITMAP 2
+
+typedef struct ordvec_index_t ordvec_index_t;
+
+/* ordvec-ffi/src/lib.rs:56-70 (128 bytes). */
+typedef struct {
+    uint64_t struct_size;
+    uint32_t kind;
+    uint32_t format_version;
+    uint64_t dim;
+    uint32_t bit_width;
+    uint32_t n_top;
+    uint64_t vector_count;
+    uint64_t bytes_per_vec;
+    uint64_t source_file_size_bytes;
+    uint64_t capabilities;
+    uint64_t reserved[8];
+} ordvec_index_info_t;
+
+/* ordvec-ffi/src/lib.rs:72-86 (128 bytes).  candidate_rows are global row
+ * ids, may be unsorted and may repeat; duplicates are scored separately. */
+typedef struct {
+    uint64_t struct_size;
+    const float *query;
+    uint64_t dim;
+    uint64_t k;
+    const uint32_t *candidate_rows;
+    uint64_t candidate_count;
+    uint64_t flags;
+    uint64_t user_tag;
+    uint64_t reserved[8];
+} ordvec_search_params_t;
+
+/* ordvec-ffi/src/lib.rs:97-124 (184 bytes).  This engine also fills the
+ * fields ABI v1 reserved: prepare_ns (host validation and upload), score_ns
+ * and select_ns (GPU time of the scoring and selection kernels) and
+ * bytes_read (algorithmic HBM bytes the kernels read). */
+typedef struct {
+    uint64_t struct_size;
+    uint32_t abi_version;
+    uint32_t kind;
+    uint64_t dim;
+    uint32_t bit_width;
+    uint32_t n_top;
+    uint64_t k;
+    uint64_t user_tag;
+    uint64_t vector_count;
+    uint64_t candidate_count;
+    uint64_t returned_count;
+    uint64_t total_ns;
+    uint64_t prepare_ns;
+    uint64_t score_ns;
+    uint64_t select_ns;
+    uint64_t vectors_scored;
+    uint64_t bytes_read;
+    uint64_t reserved[8];
+} ordvec_search_stats_t;
+
+/* ordvec-ffi/src/lib.rs:88-95 (24 bytes). */
+typedef struct {
+    uint64_t row_id;
+    uint64_t id;
+    float score;
+    uint32_t reserved;
+} ordvec_hit_t;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Part 1: ABI v1 (ordvec-ffi/include/ordvec.h:142-254) ---------------- */
+
+/* ordvec-ffi/src/lib.rs:600-603 */
+uint32_t ordvec_abi_version(void);
+/* ordvec-ffi/src/lib.rs:605-608 */
+const char *ordvec_version_string(void);
+/* ordvec-ffi/src/lib.rs:610-628 */
+const char *ordvec_status_name(ordvec_status_t status);
+/* ordvec-ffi/src/lib.rs:630-633: thread-local, cleared on success. */
+const char *ordvec_last_error(void);
+/* ordvec-ffi/src/lib.rs:635-638: also selects and initialises the GPU. */
+ordvec_status_t ordvec_init(void);
+/* ordvec-ffi/src/lib.rs:640-653 */
+void ordvec_index_info_init(ordvec_index_info_t *info);
+/* ordvec-ffi/src/lib.rs:655-668 */
+void ordvec_search_params_init(ordvec_search_params_t *params);
+/* ordvec-ffi/src/lib.rs:670-683 */
+void ordvec_search_stats_init(ordvec_search_stats_t *stats);
+/* ordvec-ffi/src/lib.rs:685-776: .ovrq loads into HBM; .ovbm, .ovr, .ovsb and
+ * .ovfs return UNSUPPORTED_FORMAT here (use the Part 2 loaders for .ovsb/.ovfs). */
+ordvec_status_t ordvec_index_load(const char *path, uint64_t flags, ordvec_index_t **out);
+/* ordvec-ffi/src/lib.rs:778-840 */
+ordvec_status_t ordvec_index_probe(const char *path, uint64_t flags, ordvec_index_info_t *info_out);
+/* ordvec-ffi/src/lib.rs:842-875 */
+ordvec_status_t ordvec_index_info(const ordvec_index_t *index, ordvec_index_info_t *info_out);
+/* ordvec-ffi/src/lib.rs:877-892 */
+void ordvec_index_free(ordvec_index_t *index);
+/* ordvec-ffi/src/lib.rs:894-998: synchronous single-query search. */
+ordvec_status_t ordvec_index_search(const ordvec_index_t *index, const ordvec_search_params_t *params,
+                                    ordvec_hit_t *hits_out, uint64_t hits_capacity, uint64_t *returned_out,
+                                    ordvec_search_stats_t *stats_out);
+
+/* ---- Part 2: additive extension ------------------------------------------ */
+
+/* Flags for Part 2 calls (bitwise OR). */
+#define ORDVEC_FLAG_NONE 0
+/* Array arguments are device (HBM) pointers; work is enqueued on the library
+ * stream and the call returns without waiting for the GPU (a non-finite check
+ * on device inputs still synchronises unless ORDVEC_FLAG_ASSUME_FINITE). */
+#define ORDVEC_FLAG_DEVICE_PTRS 1
+/* Caller guarantees inputs are finite; skips the NaN/Inf scan. */
+#define ORDVEC_FLAG_ASSUME_FINITE 2
+
+/* Engine limits of this build. */
+#define ORDVEC_MAX_TOPK 1024
+#define ORDVEC_MAX_STAGE1_M 4096
+#define ORDVEC_MAX_ENCODE_DIM 4096
+#define ORDVEC_MAX_SIGN_DIM 8192
+
+typedef struct ordvec_rankquant_t ordvec_rankquant_t;
+typedef struct ordvec_sign_bitmap_t ordvec_sign_bitmap_t;
+typedef struct ordvec_fastscan_t ordvec_fastscan_t;
+
+/* Stream used by every call (a hipStream_t; NULL = the default stream).  The
+ * previous stream is synchronised first. */
+ordvec_status_t ordvec_set_stream(void *hip_stream);
+/* Wait for all work the library enqueued. */
+ordvec_status_t ordvec_synchronize(void);
+
+/* RankQuant: src/quant.rs:438-491 new, 587-614 add, 713-847 search_asymmetric. */
+ordvec_status_t ordvec_rankquant_new(uint64_t dim, uint32_t bits, ordvec_rankquant_t **out);
+ordvec_status_t ordvec_rankquant_add(ordvec_rankquant_t *index, const float *vectors, uint64_t n, uint64_t flags);
+ordvec_status_t ordvec_rankquant_info(const ordvec_rankquant_t *index, uint64_t *dim, uint32_t *bits,
+                                      uint64_t *n_vectors);
+/* Outputs are nq x min(k, n) (row-major), like SearchResults (src/lib.rs:314). */
+ordvec_status_t ordvec_rankquant_search_asymmetric(const ordvec_rankquant_t *index, const float *queries,
+                                                   uint64_t nq, uint64_t k, float *out_scores, int64_t *out_ids,
+                                                   uint64_t flags);
+/* src/quant.rs:1048-1083: one query, returns min(k, n_rows) hits in *returned. */
+ordvec_status_t ordvec_rankquant_search_subset(const ordvec_rankquant_t *index, const float *query,
+                                               const uint32_t *rows, uint64_t n_rows, uint64_t k,
+                                               float *out_scores, int64_t *out_ids, uint64_t *returned,
+                                               uint64_t flags);
+/* src/quant.rs:1336-1394: CSR rows (offsets has nq+1 entries), outputs
+ * nq x min(k, n), underfull rows padded with -inf / -1. */
+ordvec_status_t ordvec_rankquant_search_subset_batched(const ordvec_rankquant_t *index, const float *queries,
+                                                       uint64_t nq, const uint64_t *offsets,
+                                                       const uint32_t *candidates, uint64_t n_candidates,
+                                                       uint64_t k, float *out_scores, int64_t *out_ids,
+                                                       uint64_t flags);
+/* Copy the packed codes (n x dim*bits/8, .ovrq payload order). */
+ordvec_status_t ordvec_rankquant_codes(const ordvec_rankquant_t *index, uint8_t *out, uint64_t capacity,
+                                       uint64_t flags);
+/* src/quant.rs:892-1017: .ovrq persistence. */
+ordvec_status_t ordvec_rankquant_write(const ordvec_rankquant_t *index, const char *path);
+ordvec_status_t ordvec_rankquant_load(const char *path, ordvec_rankquant_t **out);
+void ordvec_rankquant_free(ordvec_rankquant_t *index);
+
+/* SignBitmap: src/sign_bitmap.rs:143-200 new/add, 330-483 top-m candidates. */
+ordvec_status_t ordvec_sign_bitmap_new(uint64_t dim, ordvec_sign_bitmap_t **out);
+ordvec_status_t ordvec_sign_bitmap_add(ordvec_sign_bitmap_t *index, const float *vectors, uint64_t n,
+                                       uint64_t flags);
+ordvec_status_t ordvec_sign_bitmap_info(const ordvec_sign_bitmap_t *index, uint64_t *dim, uint64_t *n_vectors);
+/* Exact top-m by (hamming asc, id asc), rows sorted: out is nq x min(m, n);
+ * out_hamming (optional) receives the matching distances. */
+ordvec_status_t ordvec_sign_bitmap_top_m(const ordvec_sign_bitmap_t *index, const float *queries, uint64_t nq,
+                                         uint64_t m, uint32_t *out_candidates, uint32_t *out_hamming,
+                                         uint64_t flags);
+/* Copy the bitmaps row-major (n x dim/64 u64, .ovsb payload order). */
+ordvec_status_t ordvec_sign_bitmap_words(const ordvec_sign_bitmap_t *index, uint64_t *out, uint64_t capacity,
+                                         uint64_t flags);
+/* src/sign_bitmap.rs:593-660: .ovsb persistence. */
+ordvec_status_t ordvec_sign_bitmap_write(const ordvec_sign_bitmap_t *index, const char *path);
+ordvec_status_t ordvec_sign_bitmap_load(const char *path, ordvec_sign_bitmap_t **out);
+void ordvec_sign_bitmap_free(ordvec_sign_bitmap_t *index);
+
+/* Two-stage search in one call: SignBitmap::top_m_candidates_batched_serial_csr
+ * followed by RankQuant::search_asymmetric_subset_batched_serial_into
+ * (src/sign_bitmap.rs:453-483, src/quant.rs:1336-1394).  Both indexes must
+ * hold the same documents.  Outputs nq x min(k, n). */
+ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t *sign, const ordvec_rankquant_t *rq,
+                                        const float *queries, uint64_t nq, uint64_t m, uint64_t k,
+                                        float *out_scores, int64_t *out_ids, uint64_t flags);
+
+/* RankQuantFastscan b=2: src/fastscan.rs:571-705. */
+ordvec_status_t ordvec_fastscan_new(uint64_t dim, ordvec_fastscan_t **out);
+ordvec_status_t ordvec_fastscan_add(ordvec_fastscan_t *index, const float *vectors, uint64_t n, uint64_t flags);
+ordvec_status_t ordvec_fastscan_search(const ordvec_fastscan_t *index, const float *queries, uint64_t nq,
+                                       uint64_t k, float *out_scores, int64_t *out_ids, uint64_t flags);
+ordvec_status_t ordvec_fastscan_info(const ordvec_fastscan_t *index, uint64_t *dim, uint64_t *n_vectors);
+ordvec_status_t ordvec_fastscan_write(const ordvec_fastscan_t *index, const char *path);
+ordvec_status_t ordvec_fastscan_load(const char *path, ordvec_fastscan_t **out);
+void ordvec_fastscan_free(ordvec_fastscan_t *index);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORDVEC_H */

@@ -1,0 +1,32 @@
+"""MI355X-native ordinal retrieval (ordvec's RankQuant / SignBitmap /
+FastScan hot path) behind ordvec's C ABI.
+
+Importing the package loads ``ordvec_amd/lib/libordvec_amd.so``; there is no
+other compute path.
+"""
+from ._native import OrdvecError, load_library  # noqa: F401
+
+load_library()
+
+from .index import (  # noqa: E402,F401
+    MAX_DIM,
+    MAX_SIGN_BITMAP_DIM,
+    MAX_VECTORS,
+    RankQuant,
+    RankQuantFastscan,
+    SignBitmap,
+    synchronize,
+    two_stage_search,
+)
+
+__all__ = [
+    "RankQuant",
+    "SignBitmap",
+    "RankQuantFastscan",
+    "two_stage_search",
+    "synchronize",
+    "OrdvecError",
+    "MAX_DIM",
+    "MAX_SIGN_BITMAP_DIM",
+    "MAX_VECTORS",
+]

@@ -1,0 +1,184 @@
+"""ctypes binding to ``libordvec_amd.so`` (the ordvec C ABI, ``include/ordvec.h``).
+
+The library is the only compute path: if it is missing this module raises at
+import time instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libordvec_amd.so")
+
+# Status codes (include/ordvec.h).
+OK = 0
+NULL_POINTER = 1
+BAD_ARGUMENT = 2
+BAD_STRUCT_SIZE = 3
+UNSUPPORTED_FORMAT = 4
+CORRUPT_INDEX = 5
+IO = 6
+DIM_MISMATCH = 7
+NONFINITE_QUERY = 8
+ROW_ID_OUT_OF_RANGE = 9
+BUFFER_TOO_SMALL = 10
+UNSUPPORTED_OPERATION = 11
+PANIC = 12
+INTERNAL = 13
+
+FLAG_DEVICE_PTRS = 1
+FLAG_ASSUME_FINITE = 2
+
+MAX_TOPK = 1024
+MAX_STAGE1_M = 4096
+MAX_ENCODE_DIM = 4096
+MAX_SIGN_DIM = 8192
+
+
+class OrdvecError(RuntimeError):
+    """A non-OK status crossing the C ABI."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+class OrdvecValueError(OrdvecError, ValueError):
+    pass
+
+
+class OrdvecIOError(OrdvecError, OSError):
+    pass
+
+
+class OrdvecUnsupported(OrdvecError, NotImplementedError):
+    pass
+
+
+class IndexInfo(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint64), ("kind", C.c_uint32), ("format_version", C.c_uint32),
+        ("dim", C.c_uint64), ("bit_width", C.c_uint32), ("n_top", C.c_uint32),
+        ("vector_count", C.c_uint64), ("bytes_per_vec", C.c_uint64),
+        ("source_file_size_bytes", C.c_uint64), ("capabilities", C.c_uint64),
+        ("reserved", C.c_uint64 * 8),
+    ]
+
+
+class SearchParams(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint64), ("query", C.POINTER(C.c_float)), ("dim", C.c_uint64),
+        ("k", C.c_uint64), ("candidate_rows", C.POINTER(C.c_uint32)), ("candidate_count", C.c_uint64),
+        ("flags", C.c_uint64), ("user_tag", C.c_uint64), ("reserved", C.c_uint64 * 8),
+    ]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint64), ("abi_version", C.c_uint32), ("kind", C.c_uint32),
+        ("dim", C.c_uint64), ("bit_width", C.c_uint32), ("n_top", C.c_uint32), ("k", C.c_uint64),
+        ("user_tag", C.c_uint64), ("vector_count", C.c_uint64), ("candidate_count", C.c_uint64),
+        ("returned_count", C.c_uint64), ("total_ns", C.c_uint64), ("prepare_ns", C.c_uint64),
+        ("score_ns", C.c_uint64), ("select_ns", C.c_uint64), ("vectors_scored", C.c_uint64),
+        ("bytes_read", C.c_uint64), ("reserved", C.c_uint64 * 8),
+    ]
+
+
+class Hit(C.Structure):
+    _fields_ = [("row_id", C.c_uint64), ("id", C.c_uint64), ("score", C.c_float), ("reserved", C.c_uint32)]
+
+
+P = C.c_void_p
+U64 = C.c_uint64
+U32 = C.c_uint32
+ST = C.c_uint32
+
+# name -> (restype, argtypes); every symbol include/ordvec.h declares.
+SIGNATURES = {
+    "ordvec_abi_version": (U32, []),
+    "ordvec_version_string": (C.c_char_p, []),
+    "ordvec_status_name": (C.c_char_p, [U32]),
+    "ordvec_last_error": (C.c_char_p, []),
+    "ordvec_init": (ST, []),
+    "ordvec_index_info_init": (None, [P]),
+    "ordvec_search_params_init": (None, [P]),
+    "ordvec_search_stats_init": (None, [P]),
+    "ordvec_index_load": (ST, [C.c_char_p, U64, C.POINTER(P)]),
+    "ordvec_index_probe": (ST, [C.c_char_p, U64, P]),
+    "ordvec_index_info": (ST, [P, P]),
+    "ordvec_index_free": (None, [P]),
+    "ordvec_index_search": (ST, [P, P, P, U64, P, P]),
+    "ordvec_set_stream": (ST, [P]),
+    "ordvec_synchronize": (ST, []),
+    "ordvec_rankquant_new": (ST, [U64, U32, C.POINTER(P)]),
+    "ordvec_rankquant_add": (ST, [P, P, U64, U64]),
+    "ordvec_rankquant_info": (ST, [P, P, P, P]),
+    "ordvec_rankquant_search_asymmetric": (ST, [P, P, U64, U64, P, P, U64]),
+    "ordvec_rankquant_search_subset": (ST, [P, P, P, U64, U64, P, P, P, U64]),
+    "ordvec_rankquant_search_subset_batched": (ST, [P, P, U64, P, P, U64, U64, P, P, U64]),
+    "ordvec_rankquant_codes": (ST, [P, P, U64, U64]),
+    "ordvec_rankquant_write": (ST, [P, C.c_char_p]),
+    "ordvec_rankquant_load": (ST, [C.c_char_p, C.POINTER(P)]),
+    "ordvec_rankquant_free": (None, [P]),
+    "ordvec_sign_bitmap_new": (ST, [U64, C.POINTER(P)]),
+    "ordvec_sign_bitmap_add": (ST, [P, P, U64, U64]),
+    "ordvec_sign_bitmap_info": (ST, [P, P, P]),
+    "ordvec_sign_bitmap_top_m": (ST, [P, P, U64, U64, P, P, U64]),
+    "ordvec_sign_bitmap_words": (ST, [P, P, U64, U64]),
+    "ordvec_sign_bitmap_write": (ST, [P, C.c_char_p]),
+    "ordvec_sign_bitmap_load": (ST, [C.c_char_p, C.POINTER(P)]),
+    "ordvec_sign_bitmap_free": (None, [P]),
+    "ordvec_two_stage_search": (ST, [P, P, P, U64, U64, U64, P, P, U64]),
+    "ordvec_fastscan_new": (ST, [U64, C.POINTER(P)]),
+    "ordvec_fastscan_add": (ST, [P, P, U64, U64]),
+    "ordvec_fastscan_search": (ST, [P, P, U64, U64, P, P, U64]),
+    "ordvec_fastscan_info": (ST, [P, P, P]),
+    "ordvec_fastscan_write": (ST, [P, C.c_char_p]),
+    "ordvec_fastscan_load": (ST, [C.c_char_p, C.POINTER(P)]),
+    "ordvec_fastscan_free": (None, [P]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load and declare the native library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"ordvec_amd native library not found at {path}; build it with "
+            "`python -m ordvec_amd._build` (hipcc, gfx950)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def lib():
+    return _lib if _lib is not None else load_library()
+
+
+def last_error() -> str:
+    msg = lib().ordvec_last_error()
+    return msg.decode("utf-8", "replace") if msg else ""
+
+
+def check(status: int) -> None:
+    if status == OK:
+        return
+    msg = last_error()
+    name = lib().ordvec_status_name(status).decode()
+    text = f"{name}: {msg}"
+    if status in (NULL_POINTER, BAD_ARGUMENT, BAD_STRUCT_SIZE, DIM_MISMATCH, NONFINITE_QUERY,
+                  ROW_ID_OUT_OF_RANGE, BUFFER_TOO_SMALL):
+        raise OrdvecValueError(status, text)
+    if status in (UNSUPPORTED_FORMAT, CORRUPT_INDEX, IO):
+        raise OrdvecIOError(status, text)
+    if status == UNSUPPORTED_OPERATION:
+        raise OrdvecUnsupported(status, text)
+    raise OrdvecError(status, text)

@@ -1,0 +1,1441 @@
+// Host side of the engine: the ordvec C ABI (v1 drop-in plus the additive
+// extension), index handles resident in HBM, persisted formats, validation
+// and the per-call device pipelines.  Errors cross the ABI as status codes
+// with a thread-local message, mirroring ordvec-ffi/src/lib.rs:195-239.
+#include "ordvec.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace ovx {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Errors (ordvec-ffi/src/lib.rs:195-239)
+// ---------------------------------------------------------------------------
+thread_local std::string g_last_error;
+
+template <typename F>
+ordvec_status_t boundary(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return ORDVEC_STATUS_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return ORDVEC_STATUS_PANIC;
+    } catch (...) {
+        g_last_error = "unknown C++ exception crossing ordvec C ABI";
+        return ORDVEC_STATUS_PANIC;
+    }
+}
+
+[[noreturn]] void fail(uint32_t status, const std::string& msg) { throw Error(status, msg); }
+
+// Limits shared with the reference loaders (src/rank_io.rs:124-138,268-286).
+constexpr uint64_t kMaxDim = 65535;
+constexpr uint64_t kMaxSignDim = 1ull << 24;
+constexpr uint64_t kMaxVectors = 64ull * 1024 * 1024;
+constexpr uint64_t kMaxPayload = 128ull * 1024 * 1024 * 1024;
+
+// ---------------------------------------------------------------------------
+// Engine: one stream, a grow-only workspace and staging area, one mutex.
+// ---------------------------------------------------------------------------
+struct Engine {
+    std::recursive_mutex mu;
+    hipStream_t stream = nullptr;
+    bool inited = false;
+    void* ws = nullptr;
+    size_t ws_size = 0;
+    uint32_t* d_flag = nullptr;
+
+    void init() {
+        if (inited) return;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) fail(kInternal, "no HIP device is visible");
+        OVX_HIP(hipMalloc(&d_flag, 256));
+        inited = true;
+    }
+
+    void* workspace(size_t bytes) {
+        init();
+        if (bytes > ws_size) {
+            if (ws) {
+                OVX_HIP(hipStreamSynchronize(stream));
+                OVX_HIP(hipFree(ws));
+                ws = nullptr;
+            }
+            size_t want = std::max(bytes, ws_size + ws_size / 2);
+            if (hipMalloc(&ws, want) != hipSuccess) {
+                (void)hipGetLastError();
+                ws_size = 0;
+                if (hipMalloc(&ws, bytes) != hipSuccess) {
+                    (void)hipGetLastError();
+                    ws = nullptr;
+                    fail(kInternal, "device workspace allocation of " + std::to_string(bytes) + " bytes failed");
+                }
+                want = bytes;
+            }
+            ws_size = want;
+        }
+        return ws;
+    }
+};
+
+Engine& eng() {
+    static Engine* e = new Engine();
+    return *e;
+}
+
+struct Carver {
+    char* base;
+    size_t off = 0;
+    template <typename T>
+    T* take(uint64_t count) {
+        T* p = reinterpret_cast<T*>(base + off);
+        off += (count * sizeof(T) + 255) & ~(size_t)255;
+        return p;
+    }
+};
+
+size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+bool host_all_finite(const float* x, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++)
+        if (!std::isfinite(x[i])) return false;
+    return true;
+}
+
+// Device finite check (synchronises); returns true when every value is finite.
+bool device_all_finite(const float* x, uint64_t n) {
+    Engine& E = eng();
+    OVX_HIP(hipMemsetAsync(E.d_flag, 0, 4, E.stream));
+    launch_check_finite(x, n, E.d_flag, E.stream);
+    uint32_t flag = 0;
+    OVX_HIP(hipMemcpyAsync(&flag, E.d_flag, 4, hipMemcpyDeviceToHost, E.stream));
+    OVX_HIP(hipStreamSynchronize(E.stream));
+    return flag == 0;
+}
+
+void check_finite(const float* x, uint64_t n, uint64_t flags, uint32_t status, const char* what) {
+    if (flags & ORDVEC_FLAG_ASSUME_FINITE) return;
+    bool ok = (flags & ORDVEC_FLAG_DEVICE_PTRS) ? device_all_finite(x, n) : host_all_finite(x, n);
+    if (!ok) fail(status, std::string(what) + " contains NaN or infinity");
+}
+
+void h2d(void* dst, const void* src, size_t bytes) {
+    if (bytes) OVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, eng().stream));
+}
+void d2h(void* dst, const void* src, size_t bytes) {
+    if (bytes) OVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, eng().stream));
+}
+void sync() { OVX_HIP(hipStreamSynchronize(eng().stream)); }
+
+// ---------------------------------------------------------------------------
+// Index handles
+// ---------------------------------------------------------------------------
+struct RqIndex {
+    uint32_t dim = 0, bits = 0, bpv = 0;
+    uint64_t n = 0, cap = 0;
+    uint8_t* codes = nullptr;
+    ~RqIndex() {
+        if (codes) (void)hipFree(codes);
+    }
+    void reserve(uint64_t want) {
+        if (want <= cap) return;
+        uint64_t nc = std::max<uint64_t>(want, cap + cap / 2);
+        uint8_t* p = nullptr;
+        if (hipMalloc(&p, std::max<uint64_t>(1, nc * bpv)) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(kInternal, "HBM allocation for RankQuant codes failed");
+        }
+        if (codes) {
+            OVX_HIP(hipMemcpyAsync(p, codes, n * bpv, hipMemcpyDeviceToDevice, eng().stream));
+            sync();
+            OVX_HIP(hipFree(codes));
+        }
+        codes = p;
+        cap = nc;
+    }
+};
+
+struct SbIndex {
+    uint32_t dim = 0, qpv = 0;
+    uint64_t n = 0, cap_blocks = 0;
+    uint64_t* words = nullptr;  // interleaved [block][word][64]
+    ~SbIndex() {
+        if (words) (void)hipFree(words);
+    }
+    uint64_t block_words() const { return (uint64_t)qpv * 64; }
+    void reserve(uint64_t want_docs) {
+        uint64_t want = (want_docs + 63) / 64;
+        if (want <= cap_blocks) return;
+        uint64_t nc = std::max<uint64_t>(want, cap_blocks + cap_blocks / 2);
+        uint64_t* p = nullptr;
+        if (hipMalloc(&p, std::max<uint64_t>(8, nc * block_words() * 8)) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(kInternal, "HBM allocation for SignBitmap words failed");
+        }
+        OVX_HIP(hipMemsetAsync(p, 0, nc * block_words() * 8, eng().stream));
+        if (words) {
+            OVX_HIP(hipMemcpyAsync(p, words, cap_blocks * block_words() * 8, hipMemcpyDeviceToDevice, eng().stream));
+            sync();
+            OVX_HIP(hipFree(words));
+        }
+        words = p;
+        cap_blocks = nc;
+    }
+};
+
+struct FsIndex {
+    uint32_t dim = 0;
+    uint64_t n = 0;
+    uint8_t* packed = nullptr;
+    ~FsIndex() {
+        if (packed) (void)hipFree(packed);
+    }
+    uint64_t payload() const { return ((n + 31) / 32) * (uint64_t)(dim / 2) * 32; }
+};
+
+struct IndexHandle {
+    RqIndex* rq = nullptr;
+    uint64_t source_file_size_bytes = 0;
+    ~IndexHandle() { delete rq; }
+};
+
+// ---------------------------------------------------------------------------
+// Parameter validation (src/quant.rs:375-420, src/sign_bitmap.rs:115-139,
+// src/fastscan.rs:571-593)
+// ---------------------------------------------------------------------------
+void validate_rq_params(uint64_t dim, uint32_t bits) {
+    if (bits != 1 && bits != 2 && bits != 4)
+        fail(kBadArgument, "bits must be 1, 2, or 4 (b=8 is not persistable and not offered on the GPU engine)");
+    if (dim < 2) fail(kBadArgument, "dim must be >= 2");
+    if (dim > kMaxDim) fail(kBadArgument, "dim must fit in u16");
+    uint32_t cpb = 8 / bits;
+    if (dim % cpb) fail(kBadArgument, "dim must be a multiple of " + std::to_string(cpb) + " for bits = " + std::to_string(bits));
+    uint32_t nb = 1u << bits;
+    if (dim % nb)
+        fail(kBadArgument, "dim must be divisible by 2^bits = " + std::to_string(nb) +
+                               " so every bucket receives exactly dim / 2^bits rank entries");
+}
+
+void validate_sign_dim(uint64_t dim) {
+    if (dim == 0) fail(kBadArgument, "dim must be > 0");
+    if (dim % 64) fail(kBadArgument, "dim must be a multiple of 64");
+    if (dim > kMaxSignDim) fail(kBadArgument, "dim must be <= MAX_SIGN_BITMAP_DIM (= 16777216)");
+}
+
+void check_new_count(uint64_t cur, uint64_t add) {
+    if (cur + add > kMaxVectors)
+        fail(kBadArgument, "index would exceed MAX_VECTORS (" + std::to_string(kMaxVectors) + "); had " +
+                               std::to_string(cur) + ", adding " + std::to_string(add));
+}
+
+// ---------------------------------------------------------------------------
+// Encode paths (upload in chunks, encode on the GPU into the resident index)
+// ---------------------------------------------------------------------------
+constexpr uint64_t kChunkBytes = 256ull << 20;
+
+template <typename Fn>
+void for_each_chunk(const float* vectors, uint64_t n, uint32_t dim, uint64_t flags, Fn&& fn) {
+    if (n == 0) return;
+    if (flags & ORDVEC_FLAG_DEVICE_PTRS) {
+        fn(vectors, (uint64_t)0, n);
+        return;
+    }
+    uint64_t rows = std::max<uint64_t>(1, kChunkBytes / ((uint64_t)dim * 4));
+    float* stage = static_cast<float*>(eng().workspace(rows * dim * 4));
+    for (uint64_t r0 = 0; r0 < n; r0 += rows) {
+        uint64_t cnt = std::min(rows, n - r0);
+        h2d(stage, vectors + r0 * dim, cnt * dim * 4);
+        fn(stage, r0, cnt);
+        sync();
+    }
+}
+
+void rq_add(RqIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
+    if (ix->dim > kEncodeMaxDim)
+        fail(kUnsupportedOperation, "GPU rank encode supports dim <= " + std::to_string(kEncodeMaxDim));
+    check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
+    check_new_count(ix->n, n);
+    ix->reserve(ix->n + n);
+    uint64_t base = ix->n;
+    for_each_chunk(vectors, n, ix->dim, flags, [&](const float* dv, uint64_t r0, uint64_t cnt) {
+        launch_rank_encode(dv, cnt, ix->dim, ix->bits, ix->codes + (base + r0) * ix->bpv, eng().stream);
+    });
+    ix->n += n;
+    if (!(flags & ORDVEC_FLAG_DEVICE_PTRS)) sync();
+}
+
+void sb_add(SbIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
+    if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= " + std::to_string(kSignMaxDim));
+    check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
+    check_new_count(ix->n, n);
+    ix->reserve(ix->n + n);
+    uint64_t base = ix->n;
+    for_each_chunk(vectors, n, ix->dim, flags, [&](const float* dv, uint64_t r0, uint64_t cnt) {
+        launch_sign_encode(dv, cnt, ix->dim, ix->words, base + r0, true, eng().stream);
+    });
+    ix->n += n;
+    if (!(flags & ORDVEC_FLAG_DEVICE_PTRS)) sync();
+}
+
+// ---------------------------------------------------------------------------
+// Search pipelines
+// ---------------------------------------------------------------------------
+
+// Stage queries to the device if needed; returns a device pointer.
+const float* stage_queries(Carver& cv, const float* q, uint64_t count, uint64_t flags) {
+    if (flags & ORDVEC_FLAG_DEVICE_PTRS) return q;
+    float* d = cv.take<float>(count);
+    h2d(d, q, count * 4);
+    return d;
+}
+
+void rq_full_search(const RqIndex* ix, const float* queries, uint64_t nq, uint64_t k, float* out_s, int64_t* out_i,
+                    uint64_t flags) {
+    const uint64_t kk = std::min<uint64_t>(k, ix->n);
+    if (nq == 0 || kk == 0) return;
+    if (kk > kTopKMax) fail(kUnsupportedOperation, "k above " + std::to_string(kTopKMax) + " is not supported by the GPU engine");
+    const uint32_t d = ix->dim;
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    size_t bytes = al(nq * d * 4) * 2 + al(nq * 4) + fullscan_work_bytes(ix->n, nq, (uint32_t)kk) +
+                   (dev ? 0 : al(nq * kk * 4) + al(nq * kk * 8));
+    Carver cv{static_cast<char*>(eng().workspace(bytes))};
+    const float* dq = stage_queries(cv, queries, nq * d, flags);
+    float* qhat = cv.take<float>(nq * d);
+    float* off = cv.take<float>(nq);
+    void* work = cv.take<uint64_t>(fullscan_work_bytes(ix->n, nq, (uint32_t)kk) / 8);
+    float* ds = dev ? out_s : cv.take<float>(nq * kk);
+    int64_t* di = dev ? out_i : cv.take<int64_t>(nq * kk);
+    const int tier = select_tier(d, ix->bits);
+    const float inv_norm = rankquant_inv_norm(d, ix->bits);
+    hipStream_t s = eng().stream;
+    launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, s);
+    launch_fullscan_rq(ix->codes, ix->n, d, ix->bits, tier, inv_norm, qhat, off, nq, (uint32_t)kk, work, ds, di, s);
+    if (!dev) {
+        d2h(out_s, ds, nq * kk * 4);
+        d2h(out_i, di, nq * kk * 8);
+        sync();
+    }
+}
+
+// CSR (offsets != null) or uniform rows; outputs nq x out_k.
+void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, const uint64_t* offsets,
+                      const uint32_t* cands, uint64_t n_cands, uint64_t row_m, uint64_t out_k, float* out_s,
+                      int64_t* out_i, uint64_t flags) {
+    if (nq == 0 || out_k == 0) return;
+    if (out_k > kTopKMax) fail(kUnsupportedOperation, "k above " + std::to_string(kTopKMax) + " is not supported by the GPU engine");
+    const uint32_t d = ix->dim;
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    size_t bytes = al(nq * d * 4) * 2 + al(nq * 4) +
+                   (dev ? 0 : al((nq + 1) * 8) + al(n_cands * 4) + al(nq * out_k * 4) + al(nq * out_k * 8));
+    Carver cv{static_cast<char*>(eng().workspace(bytes))};
+    const float* dq = stage_queries(cv, queries, nq * d, flags);
+    float* qhat = cv.take<float>(nq * d);
+    float* off = cv.take<float>(nq);
+    const uint64_t* doff = offsets;
+    const uint32_t* dc = cands;
+    float* ds = out_s;
+    int64_t* di = out_i;
+    if (!dev) {
+        if (offsets) {
+            uint64_t* t = cv.take<uint64_t>(nq + 1);
+            h2d(t, offsets, (nq + 1) * 8);
+            doff = t;
+        }
+        uint32_t* tc = cv.take<uint32_t>(n_cands);
+        h2d(tc, cands, n_cands * 4);
+        dc = tc;
+        ds = cv.take<float>(nq * out_k);
+        di = cv.take<int64_t>(nq * out_k);
+    }
+    const int tier = select_tier(d, ix->bits);
+    const float inv_norm = rankquant_inv_norm(d, ix->bits);
+    hipStream_t s = eng().stream;
+    launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, s);
+    launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, doff, dc, row_m, ix->n, (uint32_t)out_k, ds, di, s);
+    if (!dev) {
+        d2h(out_s, ds, nq * out_k * 4);
+        d2h(out_i, di, nq * out_k * 8);
+        sync();
+    }
+}
+
+// Stage 1 into device buffers cand (nq x m_eff) and optional hamming.
+void sb_top_m_device(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t m_eff, uint32_t* cand,
+                     uint32_t* cand_h, Carver& cv) {
+    const uint32_t d = ix->dim;
+    uint64_t* qbits = cv.take<uint64_t>(nq * ix->qpv);
+    Stage1Plan plan = plan_stage1(ix->n, d, nq);
+    Stage1Work w{};
+    void* base = cv.take<char>(stage1_work_bytes(plan, nq, m_eff));
+    carve_stage1(plan, nq, m_eff, base, &w);
+    hipStream_t s = eng().stream;
+    launch_sign_encode(dq, nq, d, qbits, 0, false, s);
+    launch_stage1(ix->words, ix->n, d, qbits, nq, m_eff, plan, w, cand, cand_h, s);
+}
+
+size_t sb_top_m_bytes(const SbIndex* ix, uint64_t nq, uint64_t m_eff) {
+    Stage1Plan plan = plan_stage1(ix->n, ix->dim, nq);
+    return al(nq * ix->qpv * 8) + al(stage1_work_bytes(plan, nq, m_eff));
+}
+
+void check_m(uint64_t m_eff) {
+    if (m_eff > kStage1MaxM)
+        fail(kUnsupportedOperation, "stage-1 candidate count above " + std::to_string(kStage1MaxM) +
+                                        " is not supported by the GPU engine");
+}
+
+// ---------------------------------------------------------------------------
+// Persisted formats (src/rank_io.rs; docs/PERSISTED_FORMAT.md)
+// ---------------------------------------------------------------------------
+struct Reader {
+    std::vector<uint8_t> buf;
+    size_t pos = 0;
+    std::string label;
+    void need(size_t k, const char* field) {
+        if (pos + k > buf.size()) fail(kCorruptIndex, label + " header truncated while reading " + field);
+    }
+    uint8_t u8(const char* f) {
+        need(1, f);
+        return buf[pos++];
+    }
+    uint32_t u32(const char* f) {
+        need(4, f);
+        uint32_t v = (uint32_t)buf[pos] | ((uint32_t)buf[pos + 1] << 8) | ((uint32_t)buf[pos + 2] << 16) |
+                     ((uint32_t)buf[pos + 3] << 24);
+        pos += 4;
+        return v;
+    }
+    void version() {
+        uint8_t v = u8("version");
+        if (v != 1) fail(kCorruptIndex, "unsupported " + label + " version: " + std::to_string(v));
+    }
+    void payload_exact(uint64_t bytes) {
+        if (bytes > kMaxPayload) fail(kCorruptIndex, "payload " + std::to_string(bytes) + " B exceeds MAX_PAYLOAD");
+        uint64_t remaining = buf.size() - pos;
+        if (bytes > remaining)
+            fail(kCorruptIndex, label + " payload truncated: header declares " + std::to_string(bytes) +
+                                    " B but file has " + std::to_string(remaining) + " B remaining");
+        if (bytes < remaining)
+            fail(kCorruptIndex, label + " payload has trailing bytes: header declares " + std::to_string(bytes) +
+                                    " B but file has " + std::to_string(remaining) + " B remaining");
+    }
+};
+
+std::vector<uint8_t> read_file(const char* path, size_t limit = SIZE_MAX) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) fail(kIo, std::string("open index: ") + std::strerror(errno));
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    size_t want = std::min<size_t>((size_t)std::max<long>(sz, 0), limit);
+    std::vector<uint8_t> b(want);
+    size_t got = want ? std::fread(b.data(), 1, want, f) : 0;
+    std::fclose(f);
+    if (got != want) fail(kIo, "short read");
+    return b;
+}
+
+uint64_t file_size(const char* path) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) fail(kIo, std::string("open index: ") + std::strerror(errno));
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fclose(f);
+    return (uint64_t)std::max<long>(sz, 0);
+}
+
+void write_file(const char* path, const std::vector<uint8_t>& hdr, const void* payload, uint64_t bytes) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) fail(kIo, std::string("create: ") + std::strerror(errno));
+    bool ok = std::fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size();
+    if (ok && bytes) ok = std::fwrite(payload, 1, bytes, f) == bytes;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) fail(kIo, "write failed");
+}
+
+void put_u32(std::vector<uint8_t>& h, uint32_t v) {
+    for (int i = 0; i < 4; i++) h.push_back((uint8_t)(v >> (8 * i)));
+}
+
+bool magic_is(const std::vector<uint8_t>& b, const char* a, const char* legacy) {
+    if (b.size() < 4) return false;
+    return std::memcmp(b.data(), a, 4) == 0 || (legacy && std::memcmp(b.data(), legacy, 4) == 0);
+}
+
+// src/rank_io.rs:713-818: header, exact payload, per-row constant composition.
+struct OvrqParsed {
+    uint32_t bits, dim;
+    uint64_t n;
+    size_t payload_off;
+};
+
+OvrqParsed parse_ovrq_header(Reader& r, bool check_payload_len) {
+    r.label = "OVRQ";
+    r.pos = 4;
+    r.version();
+    uint8_t bits = r.u8("bits");
+    if (bits != 1 && bits != 2 && bits != 4)
+        fail(kCorruptIndex, "unsupported OVRQ bits: " + std::to_string(bits) + " (expected 1, 2, or 4)");
+    uint32_t dim = r.u32("dim");
+    if (dim < 2 || dim > kMaxDim) fail(kCorruptIndex, "dim " + std::to_string(dim) + " out of range [2, 65535]");
+    uint32_t nb = 1u << bits;
+    if (dim % nb)
+        fail(kCorruptIndex, "OVRQ dim " + std::to_string(dim) + " is not a multiple of 2^bits = " + std::to_string(nb) +
+                                "; constant-composition invariant violated");
+    if (dim % (8 / bits)) fail(kCorruptIndex, "OVRQ dim is not a multiple of codes_per_byte");
+    uint32_t n = r.u32("n_vectors");
+    if (n > kMaxVectors) fail(kCorruptIndex, "n_vectors " + std::to_string(n) + " exceeds MAX_VECTORS");
+    uint64_t payload = (uint64_t)n * (dim * bits / 8);
+    if (check_payload_len) r.payload_exact(payload);
+    return {bits, dim, n, r.pos};
+}
+
+RqIndex* load_ovrq(const char* path) {
+    Reader r;
+    r.buf = read_file(path);
+    if (!magic_is(r.buf, "OVRQ", "TVRQ")) fail(kCorruptIndex, "not an OVRQ/TVRQ (RankQuant) file: wrong magic");
+    OvrqParsed h = parse_ovrq_header(r, true);
+    const uint8_t* pay = r.buf.data() + h.payload_off;
+    const uint32_t bpv = h.dim * h.bits / 8, cpb = 8 / h.bits, nb = 1u << h.bits, per = h.dim / nb;
+    // Per-row constant composition (src/rank_io.rs:764-816), first bad row reported.
+    uint32_t lut[256][16];
+    std::memset(lut, 0, sizeof(lut));
+    for (uint32_t byte = 0; byte < 256; byte++)
+        for (uint32_t slot = 0; slot < cpb; slot++)
+            lut[byte][(byte >> ((cpb - 1 - slot) * h.bits)) & (nb - 1)]++;
+    for (uint64_t row = 0; row < h.n; row++) {
+        uint32_t hist[16] = {0};
+        const uint8_t* rp = pay + row * bpv;
+        for (uint32_t g = 0; g < bpv; g++)
+            for (uint32_t b = 0; b < nb; b++) hist[b] += lut[rp[g]][b];
+        for (uint32_t b = 0; b < nb; b++)
+            if (hist[b] != per)
+                fail(kCorruptIndex, "OVRQ row " + std::to_string(row) + " violates constant composition: bucket " +
+                                        std::to_string(b) + " has " + std::to_string(hist[b]) + " codes, expected " +
+                                        std::to_string(per) + " (= dim / 2^bits)");
+    }
+    RqIndex* ix = new RqIndex();
+    ix->dim = h.dim;
+    ix->bits = h.bits;
+    ix->bpv = bpv;
+    try {
+        ix->reserve(h.n);
+        h2d(ix->codes, pay, h.n * bpv);
+        sync();
+    } catch (...) {
+        delete ix;
+        throw;
+    }
+    ix->n = h.n;
+    return ix;
+}
+
+void write_ovrq(const RqIndex* ix, const char* path) {
+    std::vector<uint8_t> host(ix->n * ix->bpv);
+    d2h(host.data(), ix->codes, host.size());
+    sync();
+    std::vector<uint8_t> h = {'O', 'V', 'R', 'Q', 1, (uint8_t)ix->bits};
+    put_u32(h, ix->dim);
+    put_u32(h, (uint32_t)ix->n);
+    write_file(path, h, host.data(), host.size());
+}
+
+// Row-major host words <-> interleaved device layout.
+void sb_upload_rowmajor(SbIndex* ix, const uint64_t* rows, uint64_t n) {
+    const uint64_t nblk = (n + 63) / 64, bw = ix->block_words();
+    std::vector<uint64_t> inter(nblk * bw, 0);
+    for (uint64_t doc = 0; doc < n; doc++)
+        for (uint32_t w = 0; w < ix->qpv; w++) inter[(doc / 64) * bw + (uint64_t)w * 64 + (doc & 63)] = rows[doc * ix->qpv + w];
+    ix->reserve(n);
+    h2d(ix->words, inter.data(), inter.size() * 8);
+    sync();
+    ix->n = n;
+}
+
+std::vector<uint64_t> sb_download_rowmajor(const SbIndex* ix) {
+    const uint64_t nblk = (ix->n + 63) / 64, bw = ix->block_words();
+    std::vector<uint64_t> inter(nblk * bw);
+    d2h(inter.data(), ix->words, inter.size() * 8);
+    sync();
+    std::vector<uint64_t> rows(ix->n * ix->qpv);
+    for (uint64_t doc = 0; doc < ix->n; doc++)
+        for (uint32_t w = 0; w < ix->qpv; w++) rows[doc * ix->qpv + w] = inter[(doc / 64) * bw + (uint64_t)w * 64 + (doc & 63)];
+    return rows;
+}
+
+// src/rank_io.rs:1034-1075
+SbIndex* load_ovsb(const char* path) {
+    Reader r;
+    r.buf = read_file(path);
+    if (!magic_is(r.buf, "OVSB", "TVSB")) fail(kCorruptIndex, "not an OVSB/TVSB (SignBitmap) file: wrong magic");
+    r.label = "OVSB";
+    r.pos = 4;
+    r.version();
+    uint32_t dim = r.u32("dim");
+    if (dim < 64 || dim > kMaxSignDim) fail(kCorruptIndex, "OVSB dim " + std::to_string(dim) + " out of range [64, 16777216]");
+    if (dim % 64) fail(kCorruptIndex, "OVSB dim " + std::to_string(dim) + " is not a multiple of 64");
+    uint32_t n = r.u32("n_vectors");
+    if (n > kMaxVectors) fail(kCorruptIndex, "n_vectors exceeds MAX_VECTORS");
+    r.payload_exact((uint64_t)n * (dim / 64) * 8);
+    SbIndex* ix = new SbIndex();
+    ix->dim = dim;
+    ix->qpv = dim / 64;
+    std::vector<uint64_t> rows((uint64_t)n * ix->qpv);
+    std::memcpy(rows.data(), r.buf.data() + r.pos, rows.size() * 8);  // little-endian host
+    try {
+        sb_upload_rowmajor(ix, rows.data(), n);
+    } catch (...) {
+        delete ix;
+        throw;
+    }
+    return ix;
+}
+
+void write_ovsb(const SbIndex* ix, const char* path) {
+    std::vector<uint64_t> rows = sb_download_rowmajor(ix);
+    std::vector<uint8_t> h = {'O', 'V', 'S', 'B', 1};
+    put_u32(h, ix->dim);
+    put_u32(h, (uint32_t)ix->n);
+    write_file(path, h, rows.data(), rows.size() * 8);
+}
+
+// src/rank_io.rs:1077-1250 (.ovfs): header, nibble domain, b=2 composition,
+// zero tail padding.
+FsIndex* load_ovfs(const char* path) {
+    Reader r;
+    r.buf = read_file(path);
+    if (!magic_is(r.buf, "OVFS", nullptr)) fail(kCorruptIndex, "not an OVFS (RankQuantFastscan) file: wrong magic");
+    r.label = "OVFS";
+    r.pos = 4;
+    r.version();
+    uint32_t dim = r.u32("dim");
+    if (dim < 2 || dim > kMaxDim) fail(kCorruptIndex, "OVFS dim out of range");
+    if (dim % 4) fail(kCorruptIndex, "OVFS dim " + std::to_string(dim) + " is not a multiple of 4");
+    uint32_t n = r.u32("n_vectors");
+    if (n > kMaxVectors) fail(kCorruptIndex, "n_vectors exceeds MAX_VECTORS");
+    const uint64_t pairs = dim / 2, nblk = ((uint64_t)n + 31) / 32, bytes = nblk * pairs * 32;
+    r.payload_exact(bytes);
+    const uint8_t* pay = r.buf.data() + r.pos;
+    for (uint64_t b = 0; b < nblk; b++) {
+        uint64_t inb = std::min<uint64_t>(32, n - b * 32);
+        for (uint64_t l = 0; l < 32; l++) {
+            uint32_t hist[4] = {0, 0, 0, 0};
+            for (uint64_t p = 0; p < pairs; p++) {
+                uint8_t v = pay[b * pairs * 32 + p * 32 + l];
+                if (l >= inb) {
+                    if (v) fail(kCorruptIndex, "OVFS block-tail padding must be zero");
+                    continue;
+                }
+                if (v > 15) fail(kCorruptIndex, "OVFS byte is not a 4-bit FastScan code");
+                hist[v >> 2]++;
+                hist[v & 3]++;
+            }
+            if (l < inb)
+                for (int k = 0; k < 4; k++)
+                    if (hist[k] != dim / 4)
+                        fail(kCorruptIndex, "OVFS row " + std::to_string(b * 32 + l) + " violates b=2 constant composition");
+        }
+    }
+    FsIndex* ix = new FsIndex();
+    ix->dim = dim;
+    ix->n = n;
+    if (bytes) {
+        if (hipMalloc(&ix->packed, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            delete ix;
+            fail(kInternal, "HBM allocation for FastScan payload failed");
+        }
+        h2d(ix->packed, pay, bytes);
+        sync();
+    }
+    return ix;
+}
+
+void write_ovfs(const FsIndex* ix, const char* path) {
+    std::vector<uint8_t> host(ix->payload());
+    d2h(host.data(), ix->packed, host.size());
+    sync();
+    std::vector<uint8_t> h = {'O', 'V', 'F', 'S', 1};
+    put_u32(h, ix->dim);
+    put_u32(h, (uint32_t)ix->n);
+    write_file(path, h, host.data(), host.size());
+}
+
+// ---------------------------------------------------------------------------
+// ABI v1 helpers (ordvec-ffi/src/lib.rs:246-598)
+// ---------------------------------------------------------------------------
+ordvec_index_info_t default_info() {
+    ordvec_index_info_t i;
+    std::memset(&i, 0, sizeof(i));
+    i.struct_size = sizeof(ordvec_index_info_t);
+    return i;
+}
+
+ordvec_search_params_t default_params() {
+    ordvec_search_params_t p;
+    std::memset(&p, 0, sizeof(p));
+    p.struct_size = sizeof(ordvec_search_params_t);
+    return p;
+}
+
+ordvec_search_stats_t default_stats() {
+    ordvec_search_stats_t s;
+    std::memset(&s, 0, sizeof(s));
+    s.struct_size = sizeof(ordvec_search_stats_t);
+    s.abi_version = ORDVEC_ABI_VERSION;
+    return s;
+}
+
+void check_exact_size(uint64_t got, size_t expected, const char* what) {
+    if (got != expected)
+        fail(kBadStructSize, std::string(what) + ".struct_size must be exactly " + std::to_string(expected) + ", got " +
+                                 std::to_string(got));
+}
+
+constexpr uint64_t kAllCaps =
+    ORDVEC_CAP_FULL_SEARCH | ORDVEC_CAP_SUBSET_SEARCH | ORDVEC_CAP_STATS | ORDVEC_CAP_ID_EQUALS_ROW_ID;
+
+uint64_t now_ns() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+}  // namespace ovx
+
+using namespace ovx;
+
+struct ordvec_index_t {};
+struct ordvec_rankquant_t {};
+struct ordvec_sign_bitmap_t {};
+struct ordvec_fastscan_t {};
+
+static IndexHandle* H(const ordvec_index_t* p) { return reinterpret_cast<IndexHandle*>(const_cast<ordvec_index_t*>(p)); }
+static RqIndex* RQ(const ordvec_rankquant_t* p) { return reinterpret_cast<RqIndex*>(const_cast<ordvec_rankquant_t*>(p)); }
+static SbIndex* SB(const ordvec_sign_bitmap_t* p) { return reinterpret_cast<SbIndex*>(const_cast<ordvec_sign_bitmap_t*>(p)); }
+static FsIndex* FS(const ordvec_fastscan_t* p) { return reinterpret_cast<FsIndex*>(const_cast<ordvec_fastscan_t*>(p)); }
+
+#define LOCK std::lock_guard<std::recursive_mutex> _lk(eng().mu)
+#define NONNULL(p, name) \
+    if (!(p)) fail(kNullPointer, name " pointer is NULL")
+
+extern "C" {
+
+uint32_t ordvec_abi_version(void) { return ORDVEC_ABI_VERSION; }
+
+const char* ordvec_version_string(void) { return "0.6.0"; }
+
+const char* ordvec_status_name(ordvec_status_t s) {
+    switch (s) {
+        case 0: return "OK";
+        case 1: return "NULL_POINTER";
+        case 2: return "BAD_ARGUMENT";
+        case 3: return "BAD_STRUCT_SIZE";
+        case 4: return "UNSUPPORTED_FORMAT";
+        case 5: return "CORRUPT_INDEX";
+        case 6: return "IO";
+        case 7: return "DIM_MISMATCH";
+        case 8: return "NONFINITE_QUERY";
+        case 9: return "ROW_ID_OUT_OF_RANGE";
+        case 10: return "BUFFER_TOO_SMALL";
+        case 11: return "UNSUPPORTED_OPERATION";
+        case 12: return "PANIC";
+        case 13: return "INTERNAL";
+        default: return "UNKNOWN";
+    }
+}
+
+const char* ordvec_last_error(void) { return g_last_error.c_str(); }
+
+ordvec_status_t ordvec_init(void) {
+    return boundary([&] {
+        LOCK;
+        eng().init();
+    });
+}
+
+void ordvec_index_info_init(ordvec_index_info_t* info) {
+    if (info) *info = default_info();
+}
+void ordvec_search_params_init(ordvec_search_params_t* p) {
+    if (p) *p = default_params();
+}
+void ordvec_search_stats_init(ordvec_search_stats_t* s) {
+    if (s) *s = default_stats();
+}
+
+ordvec_status_t ordvec_index_load(const char* path, uint64_t flags, ordvec_index_t** out) {
+    return boundary([&] {
+        NONNULL(path, "path");
+        NONNULL(out, "out");
+        *out = nullptr;
+        if (flags != 0) fail(kBadArgument, "unknown load flags: " + std::to_string(flags));
+        std::vector<uint8_t> head = read_file(path, 4);
+        if (head.size() < 4) fail(kCorruptIndex, "read index magic: failed to fill whole buffer");
+        uint64_t fsz = file_size(path);
+        if (magic_is(head, "OVRQ", "TVRQ")) {
+            LOCK;
+            eng().init();
+            IndexHandle* h = new IndexHandle();
+            try {
+                h->rq = load_ovrq(path);
+            } catch (...) {
+                delete h;
+                throw;
+            }
+            h->source_file_size_bytes = fsz;
+            *out = reinterpret_cast<ordvec_index_t*>(h);
+        } else if (magic_is(head, "OVBM", "TVBM")) {
+            fail(kUnsupportedFormat, "Bitmap (.ovbm) indexes are not served by the GPU engine in this version");
+        } else if (magic_is(head, "OVR1", "TVR1") || magic_is(head, "OVSB", "TVSB")) {
+            fail(kUnsupportedFormat, "ABI v1 supports only RankQuant and Bitmap indexes");
+        } else if (magic_is(head, "OVFS", nullptr)) {
+            fail(kUnsupportedFormat, "ABI v1 does not support RankQuantFastscan indexes (tracked in #232)");
+        } else {
+            fail(kCorruptIndex, "unrecognized ordvec index magic");
+        }
+    });
+}
+
+ordvec_status_t ordvec_index_probe(const char* path, uint64_t flags, ordvec_index_info_t* info_out) {
+    return boundary([&] {
+        NONNULL(path, "path");
+        NONNULL(info_out, "info_out");
+        if (flags != 0) fail(kBadArgument, "unknown probe flags: " + std::to_string(flags));
+        check_exact_size(info_out->struct_size, sizeof(ordvec_index_info_t), "ordvec_index_info_t");
+        uint64_t fsz = file_size(path);
+        Reader r;
+        r.buf = read_file(path, 64);
+        if (r.buf.size() < 4) fail(kCorruptIndex, "probe index metadata: ordvec index header truncated while reading magic");
+        ordvec_index_info_t info = default_info();
+        info.format_version = 1;
+        info.source_file_size_bytes = fsz;
+        info.capabilities = kAllCaps;
+        // Header fields are parsed from the head; the exact-length rule uses the real size.
+        auto exact = [&](uint64_t header, uint64_t payload, const char* label) {
+            if (payload > kMaxPayload) fail(kCorruptIndex, "payload exceeds MAX_PAYLOAD");
+            uint64_t remaining = fsz >= header ? fsz - header : 0;
+            if (payload > remaining)
+                fail(kCorruptIndex, std::string("probe index metadata: ") + label + " payload truncated: header declares " +
+                                        std::to_string(payload) + " B but file has " + std::to_string(remaining) + " B remaining");
+            if (payload < remaining)
+                fail(kCorruptIndex, std::string("probe index metadata: ") + label + " payload has trailing bytes: header declares " +
+                                        std::to_string(payload) + " B but file has " + std::to_string(remaining) + " B remaining");
+        };
+        if (magic_is(r.buf, "OVRQ", "TVRQ")) {
+            OvrqParsed h = parse_ovrq_header(r, false);
+            exact(14, h.n * (uint64_t)(h.dim * h.bits / 8), "OVRQ");
+            info.kind = ORDVEC_INDEX_KIND_RANK_QUANT;
+            info.dim = h.dim;
+            info.bit_width = h.bits;
+            info.vector_count = h.n;
+            info.bytes_per_vec = h.dim * h.bits / 8;
+        } else if (magic_is(r.buf, "OVBM", "TVBM")) {
+            r.label = "OVBM";
+            r.pos = 4;
+            r.version();
+            uint32_t dim = r.u32("dim");
+            if (dim < 2 || dim > kMaxDim) fail(kCorruptIndex, "dim out of range [2, 65535]");
+            if (dim % 64) fail(kCorruptIndex, "OVBM dim " + std::to_string(dim) + " is not a multiple of 64");
+            uint32_t n_top = r.u32("n_top");
+            if (n_top == 0 || n_top >= dim) fail(kCorruptIndex, "OVBM n_top must satisfy 0 < n_top < dim");
+            uint32_t n = r.u32("n_vectors");
+            if (n > kMaxVectors) fail(kCorruptIndex, "n_vectors exceeds MAX_VECTORS");
+            exact(17, (uint64_t)n * (dim / 64) * 8, "OVBM");
+            info.kind = ORDVEC_INDEX_KIND_BITMAP;
+            info.dim = dim;
+            info.n_top = n_top;
+            info.vector_count = n;
+            info.bytes_per_vec = dim / 8;
+        } else if (magic_is(r.buf, "OVR1", "TVR1") || magic_is(r.buf, "OVSB", "TVSB")) {
+            fail(kUnsupportedFormat, "ABI v1 supports metadata probes only for RankQuant and Bitmap indexes");
+        } else if (magic_is(r.buf, "OVFS", nullptr)) {
+            fail(kCorruptIndex, "probe index metadata: OVFS (RankQuantFastscan) metadata probing is not supported in this version");
+        } else {
+            fail(kCorruptIndex, "probe index metadata: unknown ordvec index magic");
+        }
+        *info_out = info;
+    });
+}
+
+ordvec_status_t ordvec_index_info(const ordvec_index_t* index, ordvec_index_info_t* info_out) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        NONNULL(info_out, "info_out");
+        check_exact_size(info_out->struct_size, sizeof(ordvec_index_info_t), "ordvec_index_info_t");
+        const IndexHandle* h = H(index);
+        ordvec_index_info_t info = default_info();
+        info.kind = ORDVEC_INDEX_KIND_RANK_QUANT;
+        info.format_version = 1;
+        info.dim = h->rq->dim;
+        info.bit_width = h->rq->bits;
+        info.vector_count = h->rq->n;
+        info.bytes_per_vec = h->rq->bpv;
+        info.source_file_size_bytes = h->source_file_size_bytes;
+        info.capabilities = kAllCaps;
+        *info_out = info;
+    });
+}
+
+void ordvec_index_free(ordvec_index_t* index) {
+    if (!index) return;
+    LOCK;
+    delete H(index);
+}
+
+ordvec_status_t ordvec_index_search(const ordvec_index_t* index, const ordvec_search_params_t* params,
+                                    ordvec_hit_t* hits_out, uint64_t hits_capacity, uint64_t* returned_out,
+                                    ordvec_search_stats_t* stats_out) {
+    return boundary([&] {
+        const uint64_t t0 = now_ns();
+        NONNULL(index, "index");
+        NONNULL(params, "params");
+        const IndexHandle* h = H(index);
+        const RqIndex* ix = h->rq;
+        // Validation order: ordvec-ffi/src/lib.rs:444-568.
+        check_exact_size(params->struct_size, sizeof(ordvec_search_params_t), "ordvec_search_params_t");
+        if (params->flags != ORDVEC_SEARCH_FLAG_NONE) fail(kBadArgument, "unknown search flags: " + std::to_string(params->flags));
+        for (uint64_t r : params->reserved)
+            if (r) fail(kBadArgument, "ordvec_search_params_t.reserved fields must be zero");
+        NONNULL(returned_out, "returned_out");
+        if (stats_out) check_exact_size(stats_out->struct_size, sizeof(ordvec_search_stats_t), "ordvec_search_stats_t");
+        NONNULL(params->query, "query");
+        if (params->dim != ix->dim)
+            fail(kDimMismatch, "query dim " + std::to_string(params->dim) + " does not match index dim " + std::to_string(ix->dim));
+        if (!host_all_finite(params->query, ix->dim)) fail(kNonfiniteQuery, "query contains NaN or infinity");
+        const uint32_t* rows = nullptr;
+        uint64_t count = params->candidate_count;
+        if (count == 0 && params->candidate_rows) fail(kBadArgument, "candidate_count is zero but candidate_rows is non-NULL");
+        if (count != 0 && !params->candidate_rows) fail(kNullPointer, "candidate_rows pointer is NULL but candidate_count is nonzero");
+        if (count) {
+            rows = params->candidate_rows;
+            for (uint64_t i = 0; i < count; i++)
+                if (rows[i] >= ix->n)
+                    fail(kRowIdOutOfRange, "candidate row " + std::to_string(rows[i]) + " is out of range for " +
+                                               std::to_string(ix->n) + " vectors");
+        }
+        const uint64_t space = rows ? count : ix->n;
+        const uint64_t required = std::min<uint64_t>(params->k, space);
+        if (required > 0) {
+            NONNULL(hits_out, "hits_out");
+            if (hits_capacity < required)
+                fail(kBufferTooSmall, "hits_capacity " + std::to_string(hits_capacity) + " is smaller than required " +
+                                          std::to_string(required));
+        }
+        uint64_t t_prep = now_ns(), t_score = t_prep, t_sel = t_prep;
+        if (required > 0) {
+            // search_asymmetric_subset asserts rows <= n_vectors (src/quant.rs:1057-1063).
+            if (rows && count > ix->n)
+                throw std::runtime_error("search_asymmetric_subset: candidate list length " + std::to_string(count) +
+                                         " exceeds n_vectors " + std::to_string(ix->n) + "; deduplicate repeated ids before calling");
+            std::vector<float> sc(required);
+            std::vector<int64_t> id(required);
+            {
+                LOCK;
+                eng().init();
+                t_prep = now_ns();
+                if (rows) {
+                    uint64_t off[2] = {0, count};
+                    rq_subset_search(ix, params->query, 1, off, rows, count, 0, required, sc.data(), id.data(), 0);
+                } else {
+                    rq_full_search(ix, params->query, 1, required, sc.data(), id.data(), 0);
+                }
+                t_score = now_ns();
+            }
+            for (uint64_t j = 0; j < required; j++) {
+                if (id[j] < 0) fail(kInternal, "engine returned a negative row id");
+                hits_out[j].row_id = (uint64_t)id[j];
+                hits_out[j].id = (uint64_t)id[j];
+                hits_out[j].score = sc[j];
+                hits_out[j].reserved = 0;
+            }
+            t_sel = now_ns();
+        }
+        *returned_out = required;
+        if (stats_out) {
+            ordvec_search_stats_t st = default_stats();
+            st.kind = ORDVEC_INDEX_KIND_RANK_QUANT;
+            st.dim = ix->dim;
+            st.bit_width = ix->bits;
+            st.k = params->k;
+            st.user_tag = params->user_tag;
+            st.vector_count = ix->n;
+            st.candidate_count = space;
+            st.returned_count = required;
+            st.total_ns = now_ns() - t0;
+            st.prepare_ns = t_prep - t0;
+            st.score_ns = t_score - t_prep;
+            st.select_ns = t_sel - t_score;
+            st.vectors_scored = space;
+            st.bytes_read = required ? space * ix->bpv + (uint64_t)ix->dim * 4 : 0;
+            *stats_out = st;
+        }
+    });
+}
+
+// ---- Part 2 -----------------------------------------------------------------
+
+ordvec_status_t ordvec_set_stream(void* hip_stream) {
+    return boundary([&] {
+        LOCK;
+        eng().init();
+        if (eng().stream == static_cast<hipStream_t>(hip_stream)) return;
+        OVX_HIP(hipStreamSynchronize(eng().stream));
+        eng().stream = static_cast<hipStream_t>(hip_stream);
+    });
+}
+
+ordvec_status_t ordvec_synchronize(void) {
+    return boundary([&] {
+        LOCK;
+        eng().init();
+        sync();
+    });
+}
+
+ordvec_status_t ordvec_rankquant_new(uint64_t dim, uint32_t bits, ordvec_rankquant_t** out) {
+    return boundary([&] {
+        NONNULL(out, "out");
+        *out = nullptr;
+        validate_rq_params(dim, bits);
+        LOCK;
+        eng().init();
+        RqIndex* ix = new RqIndex();
+        ix->dim = (uint32_t)dim;
+        ix->bits = bits;
+        ix->bpv = (uint32_t)(dim * bits / 8);
+        *out = reinterpret_cast<ordvec_rankquant_t*>(ix);
+    });
+}
+
+ordvec_status_t ordvec_rankquant_add(ordvec_rankquant_t* index, const float* vectors, uint64_t n, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        if (n == 0) return;
+        NONNULL(vectors, "vectors");
+        LOCK;
+        rq_add(RQ(index), vectors, n, flags);
+    });
+}
+
+ordvec_status_t ordvec_rankquant_info(const ordvec_rankquant_t* index, uint64_t* dim, uint32_t* bits, uint64_t* n) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        if (dim) *dim = RQ(index)->dim;
+        if (bits) *bits = RQ(index)->bits;
+        if (n) *n = RQ(index)->n;
+    });
+}
+
+ordvec_status_t ordvec_rankquant_search_asymmetric(const ordvec_rankquant_t* index, const float* queries, uint64_t nq,
+                                                   uint64_t k, float* out_scores, int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const RqIndex* ix = RQ(index);
+        if (nq == 0 || std::min<uint64_t>(k, ix->n) == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        LOCK;
+        check_finite(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
+        rq_full_search(ix, queries, nq, k, out_scores, out_ids, flags);
+    });
+}
+
+ordvec_status_t ordvec_rankquant_search_subset(const ordvec_rankquant_t* index, const float* query, const uint32_t* rows,
+                                               uint64_t n_rows, uint64_t k, float* out_scores, int64_t* out_ids,
+                                               uint64_t* returned, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const RqIndex* ix = RQ(index);
+        NONNULL(query, "query");
+        if (n_rows > ix->n)
+            fail(kBadArgument, "search_asymmetric_subset: candidate list length " + std::to_string(n_rows) +
+                                   " exceeds n_vectors " + std::to_string(ix->n) + "; deduplicate repeated ids before calling");
+        const uint64_t out_k = std::min(k, n_rows);
+        if (returned) *returned = out_k;
+        if (out_k == 0) return;
+        NONNULL(rows, "rows");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        LOCK;
+        check_finite(query, ix->dim, flags, kNonfiniteQuery, "query");
+        if (!(flags & ORDVEC_FLAG_DEVICE_PTRS))
+            for (uint64_t i = 0; i < n_rows; i++)
+                if (rows[i] >= ix->n) fail(kRowIdOutOfRange, "search_asymmetric_subset: candidate id out of range");
+        uint64_t off[2] = {0, n_rows};
+        if (flags & ORDVEC_FLAG_DEVICE_PTRS) {
+            rq_subset_search(ix, query, 1, nullptr, rows, n_rows, n_rows, out_k, out_scores, out_ids, flags);
+        } else {
+            rq_subset_search(ix, query, 1, off, rows, n_rows, 0, out_k, out_scores, out_ids, flags);
+        }
+    });
+}
+
+ordvec_status_t ordvec_rankquant_search_subset_batched(const ordvec_rankquant_t* index, const float* queries,
+                                                       uint64_t nq, const uint64_t* offsets, const uint32_t* candidates,
+                                                       uint64_t n_candidates, uint64_t k, float* out_scores,
+                                                       int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const RqIndex* ix = RQ(index);
+        NONNULL(offsets, "offsets");
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        if (!dev) {
+            // validate_csr_batch (src/quant.rs:1088-1119)
+            if (offsets[0] != 0) fail(kBadArgument, "candidate_offsets[0] must be 0");
+            if (offsets[nq] != n_candidates) fail(kBadArgument, "candidate_offsets[nq] must equal candidates.len()");
+            for (uint64_t q = 0; q < nq; q++) {
+                if (offsets[q + 1] < offsets[q]) fail(kBadArgument, "candidate_offsets must be monotonic non-decreasing");
+                if (offsets[q + 1] - offsets[q] > ix->n)
+                    fail(kBadArgument, "per-row candidate count exceeds n_vectors " + std::to_string(ix->n));
+            }
+            for (uint64_t i = 0; i < n_candidates; i++)
+                if (candidates[i] >= ix->n)
+                    fail(kRowIdOutOfRange, "candidate id out of range (n_vectors " + std::to_string(ix->n) + ")");
+        }
+        const uint64_t out_k = std::min<uint64_t>(k, ix->n);
+        if (out_k == 0 || nq == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        if (n_candidates) NONNULL(candidates, "candidates");
+        LOCK;
+        check_finite(queries, nq * ix->dim, flags, kBadArgument, "queries");
+        rq_subset_search(ix, queries, nq, offsets, candidates, n_candidates, 0, out_k, out_scores, out_ids, flags);
+    });
+}
+
+ordvec_status_t ordvec_rankquant_codes(const ordvec_rankquant_t* index, uint8_t* out, uint64_t capacity, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const RqIndex* ix = RQ(index);
+        uint64_t bytes = ix->n * ix->bpv;
+        if (capacity < bytes) fail(kBufferTooSmall, "codes buffer too small");
+        if (!bytes) return;
+        NONNULL(out, "out");
+        LOCK;
+        if (flags & ORDVEC_FLAG_DEVICE_PTRS) {
+            OVX_HIP(hipMemcpyAsync(out, ix->codes, bytes, hipMemcpyDeviceToDevice, eng().stream));
+        } else {
+            d2h(out, ix->codes, bytes);
+            sync();
+        }
+    });
+}
+
+ordvec_status_t ordvec_rankquant_write(const ordvec_rankquant_t* index, const char* path) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        NONNULL(path, "path");
+        LOCK;
+        write_ovrq(RQ(index), path);
+    });
+}
+
+ordvec_status_t ordvec_rankquant_load(const char* path, ordvec_rankquant_t** out) {
+    return boundary([&] {
+        NONNULL(path, "path");
+        NONNULL(out, "out");
+        *out = nullptr;
+        LOCK;
+        eng().init();
+        *out = reinterpret_cast<ordvec_rankquant_t*>(load_ovrq(path));
+    });
+}
+
+void ordvec_rankquant_free(ordvec_rankquant_t* index) {
+    if (!index) return;
+    LOCK;
+    delete RQ(index);
+}
+
+ordvec_status_t ordvec_sign_bitmap_new(uint64_t dim, ordvec_sign_bitmap_t** out) {
+    return boundary([&] {
+        NONNULL(out, "out");
+        *out = nullptr;
+        validate_sign_dim(dim);
+        LOCK;
+        eng().init();
+        SbIndex* ix = new SbIndex();
+        ix->dim = (uint32_t)dim;
+        ix->qpv = (uint32_t)(dim / 64);
+        *out = reinterpret_cast<ordvec_sign_bitmap_t*>(ix);
+    });
+}
+
+ordvec_status_t ordvec_sign_bitmap_add(ordvec_sign_bitmap_t* index, const float* vectors, uint64_t n, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        if (n == 0) return;
+        NONNULL(vectors, "vectors");
+        LOCK;
+        sb_add(SB(index), vectors, n, flags);
+    });
+}
+
+ordvec_status_t ordvec_sign_bitmap_info(const ordvec_sign_bitmap_t* index, uint64_t* dim, uint64_t* n) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        if (dim) *dim = SB(index)->dim;
+        if (n) *n = SB(index)->n;
+    });
+}
+
+ordvec_status_t ordvec_sign_bitmap_top_m(const ordvec_sign_bitmap_t* index, const float* queries, uint64_t nq,
+                                         uint64_t m, uint32_t* out_candidates, uint32_t* out_hamming, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = SB(index);
+        const uint64_t m_eff = std::min<uint64_t>(m, ix->n);
+        if (nq == 0 || m_eff == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_candidates, "out_candidates");
+        check_m(m_eff);
+        if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
+        LOCK;
+        check_finite(queries, nq * ix->dim, flags, kBadArgument, "queries");
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        size_t bytes = al(nq * ix->dim * 4) + sb_top_m_bytes(ix, nq, m_eff) + (dev ? 0 : 2 * al(nq * m_eff * 4));
+        Carver cv{static_cast<char*>(eng().workspace(bytes))};
+        const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
+        uint32_t* dc = dev ? out_candidates : cv.take<uint32_t>(nq * m_eff);
+        uint32_t* dh = dev ? out_hamming : (out_hamming ? cv.take<uint32_t>(nq * m_eff) : nullptr);
+        sb_top_m_device(ix, dq, nq, m_eff, dc, dh, cv);
+        if (!dev) {
+            d2h(out_candidates, dc, nq * m_eff * 4);
+            if (out_hamming) d2h(out_hamming, dh, nq * m_eff * 4);
+            sync();
+        }
+    });
+}
+
+ordvec_status_t ordvec_sign_bitmap_words(const ordvec_sign_bitmap_t* index, uint64_t* out, uint64_t capacity,
+                                         uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = SB(index);
+        uint64_t cnt = ix->n * ix->qpv;
+        if (capacity < cnt) fail(kBufferTooSmall, "words buffer too small");
+        if (!cnt) return;
+        NONNULL(out, "out");
+        if (flags & ORDVEC_FLAG_DEVICE_PTRS) fail(kUnsupportedOperation, "row-major export is host-only");
+        LOCK;
+        std::vector<uint64_t> rows = sb_download_rowmajor(ix);
+        std::memcpy(out, rows.data(), cnt * 8);
+    });
+}
+
+ordvec_status_t ordvec_sign_bitmap_write(const ordvec_sign_bitmap_t* index, const char* path) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        NONNULL(path, "path");
+        LOCK;
+        write_ovsb(SB(index), path);
+    });
+}
+
+ordvec_status_t ordvec_sign_bitmap_load(const char* path, ordvec_sign_bitmap_t** out) {
+    return boundary([&] {
+        NONNULL(path, "path");
+        NONNULL(out, "out");
+        *out = nullptr;
+        LOCK;
+        eng().init();
+        *out = reinterpret_cast<ordvec_sign_bitmap_t*>(load_ovsb(path));
+    });
+}
+
+void ordvec_sign_bitmap_free(ordvec_sign_bitmap_t* index) {
+    if (!index) return;
+    LOCK;
+    delete SB(index);
+}
+
+ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const ordvec_rankquant_t* rq,
+                                        const float* queries, uint64_t nq, uint64_t m, uint64_t k, float* out_scores,
+                                        int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(sign, "sign");
+        NONNULL(rq, "rq");
+        const SbIndex* sb = SB(sign);
+        const RqIndex* ix = RQ(rq);
+        if (sb->dim != ix->dim) fail(kBadArgument, "sign_probe.dim must match RankQuant dim " + std::to_string(ix->dim));
+        if (sb->n != ix->n) fail(kBadArgument, "sign_probe.len must match RankQuant len " + std::to_string(ix->n));
+        const uint64_t out_k = std::min<uint64_t>(k, ix->n);
+        const uint64_t m_eff = std::min<uint64_t>(m, ix->n);
+        if (nq == 0 || out_k == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        check_m(m_eff);
+        if (out_k > kTopKMax) fail(kUnsupportedOperation, "k above 1024 is not supported by the GPU engine");
+        if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
+        LOCK;
+        check_finite(queries, nq * ix->dim, flags, kBadArgument, "queries");
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        const uint32_t d = ix->dim;
+        size_t bytes = al(nq * d * 4) * 2 + al(nq * 4) + al(nq * m_eff * 4) + sb_top_m_bytes(sb, nq, m_eff) +
+                       (dev ? 0 : al(nq * out_k * 4) + al(nq * out_k * 8));
+        Carver cv{static_cast<char*>(eng().workspace(bytes))};
+        const float* dq = stage_queries(cv, queries, nq * d, flags);
+        float* qhat = cv.take<float>(nq * d);
+        float* off = cv.take<float>(nq);
+        uint32_t* cand = cv.take<uint32_t>(nq * m_eff);
+        float* ds = dev ? out_scores : cv.take<float>(nq * out_k);
+        int64_t* di = dev ? out_ids : cv.take<int64_t>(nq * out_k);
+        const int tier = select_tier(d, ix->bits);
+        const float inv_norm = rankquant_inv_norm(d, ix->bits);
+        hipStream_t s = eng().stream;
+        launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, s);
+        if (m_eff) sb_top_m_device(sb, dq, nq, m_eff, cand, nullptr, cv);
+        launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, (uint32_t)out_k, ds, di, s);
+        if (!dev) {
+            d2h(out_scores, ds, nq * out_k * 4);
+            d2h(out_ids, di, nq * out_k * 8);
+            sync();
+        }
+    });
+}
+
+ordvec_status_t ordvec_fastscan_new(uint64_t dim, ordvec_fastscan_t** out) {
+    return boundary([&] {
+        NONNULL(out, "out");
+        *out = nullptr;
+        if (dim < 2) fail(kBadArgument, "FastScan b=2: dim must be >= 2");
+        if (dim > kMaxDim) fail(kBadArgument, "FastScan b=2: dim must fit in u16");
+        if (dim % 4) fail(kBadArgument, "FastScan b=2: dim " + std::to_string(dim) + " must be divisible by 4");
+        LOCK;
+        eng().init();
+        FsIndex* ix = new FsIndex();
+        ix->dim = (uint32_t)dim;
+        *out = reinterpret_cast<ordvec_fastscan_t*>(ix);
+    });
+}
+
+ordvec_status_t ordvec_fastscan_add(ordvec_fastscan_t* index, const float* vectors, uint64_t n, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        FsIndex* ix = FS(index);
+        if (n == 0) return;
+        NONNULL(vectors, "vectors");
+        if (ix->n != 0)
+            fail(kBadArgument, "FastScan v1: incremental add() not supported (block-32 layout has tail-padding semantics "
+                               "that don't compose); construct a new RankQuantFastscan instead");
+        if (ix->dim > kEncodeMaxDim) fail(kUnsupportedOperation, "GPU rank encode supports dim <= 4096");
+        LOCK;
+        check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
+        check_new_count(0, n);
+        RqIndex tmp;
+        tmp.dim = ix->dim;
+        tmp.bits = 2;
+        tmp.bpv = ix->dim / 4;
+        rq_add(&tmp, vectors, n, flags | ORDVEC_FLAG_ASSUME_FINITE);
+        ix->n = n;
+        uint64_t bytes = ix->payload();
+        if (hipMalloc(&ix->packed, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            ix->n = 0;
+            fail(kInternal, "HBM allocation for FastScan payload failed");
+        }
+        OVX_HIP(hipMemsetAsync(ix->packed, 0, bytes, eng().stream));
+        launch_fastscan_pack(tmp.codes, n, ix->dim, ix->packed, eng().stream);
+        sync();
+    });
+}
+
+ordvec_status_t ordvec_fastscan_search(const ordvec_fastscan_t* index, const float* queries, uint64_t nq, uint64_t k,
+                                       float* out_scores, int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const FsIndex* ix = FS(index);
+        const uint64_t kk = std::min<uint64_t>(k, ix->n);
+        if (nq == 0 || kk == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        if (kk > kTopKMax) fail(kUnsupportedOperation, "k above 1024 is not supported by the GPU engine");
+        if (ix->dim > 4096) fail(kUnsupportedOperation, "FastScan LUT staging supports dim <= 4096");
+        LOCK;
+        check_finite(queries, nq * ix->dim, flags, kBadArgument, "queries");
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        const uint32_t d = ix->dim;
+        size_t wk = fullscan_work_bytes(ix->n, nq, (uint32_t)kk);
+        size_t bytes = al(nq * d * 4) * 2 + al(nq * 4) + al(nq * (d / 2) * 16) + al(nq * 8) + wk +
+                       (dev ? 0 : al(nq * kk * 4) + al(nq * kk * 8));
+        Carver cv{static_cast<char*>(eng().workspace(bytes))};
+        const float* dq = stage_queries(cv, queries, nq * d, flags);
+        float* qhat = cv.take<float>(nq * d);
+        float* off = cv.take<float>(nq);
+        uint8_t* lut = cv.take<uint8_t>(nq * (d / 2) * 16);
+        float* bq = cv.take<float>(nq * 2);
+        void* work = cv.take<char>(wk);
+        float* ds = dev ? out_scores : cv.take<float>(nq * kk);
+        int64_t* di = dev ? out_ids : cv.take<int64_t>(nq * kk);
+        const float inv_norm = rankquant_inv_norm(d, 2);
+        hipStream_t s = eng().stream;
+        launch_prepare_queries(dq, nq, d, 2, 0, inv_norm, qhat, off, s);
+        launch_fastscan_prepare(qhat, nq, d, lut, bq, s);
+        launch_fullscan_fs(ix->packed, ix->n, d, lut, bq, inv_norm, nq, (uint32_t)kk, work, ds, di, s);
+        if (!dev) {
+            d2h(out_scores, ds, nq * kk * 4);
+            d2h(out_ids, di, nq * kk * 8);
+            sync();
+        }
+    });
+}
+
+ordvec_status_t ordvec_fastscan_info(const ordvec_fastscan_t* index, uint64_t* dim, uint64_t* n) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        if (dim) *dim = FS(index)->dim;
+        if (n) *n = FS(index)->n;
+    });
+}
+
+ordvec_status_t ordvec_fastscan_write(const ordvec_fastscan_t* index, const char* path) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        NONNULL(path, "path");
+        LOCK;
+        write_ovfs(FS(index), path);
+    });
+}
+
+ordvec_status_t ordvec_fastscan_load(const char* path, ordvec_fastscan_t** out) {
+    return boundary([&] {
+        NONNULL(path, "path");
+        NONNULL(out, "out");
+        *out = nullptr;
+        LOCK;
+        eng().init();
+        *out = reinterpret_cast<ordvec_fastscan_t*>(load_ovfs(path));
+    });
+}
+
+void ordvec_fastscan_free(ordvec_fastscan_t* index) {
+    if (!index) return;
+    LOCK;
+    delete FS(index);
+}
+
+}  // extern "C"

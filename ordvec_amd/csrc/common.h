@@ -1,0 +1,117 @@
+// Shared declarations for the MI355X ordinal-retrieval engine.
+//
+// Layouts in HBM (all device-resident, uploaded once):
+//   RankQuant codes   row-major, n x bpv bytes, MSB-first packing
+//                     (bytes identical to the .ovrq payload, src/rank.rs:128-156)
+//   SignBitmap words  doc-interleaved: 64-doc blocks, [block][word][lane] u64,
+//                     so one wave's load of word w is 512 contiguous bytes
+//   FastScan payload  block-32 [block][pair][lane] nibble bytes
+//                     (identical to the .ovfs payload, src/fastscan.rs:87-110)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace ovx {
+
+// Status codes mirror ordvec-ffi (ordvec.h ORDVEC_STATUS_*).
+enum Status : uint32_t {
+    kOk = 0,
+    kNullPointer = 1,
+    kBadArgument = 2,
+    kBadStructSize = 3,
+    kUnsupportedFormat = 4,
+    kCorruptIndex = 5,
+    kIo = 6,
+    kDimMismatch = 7,
+    kNonfiniteQuery = 8,
+    kRowIdOutOfRange = 9,
+    kBufferTooSmall = 10,
+    kUnsupportedOperation = 11,
+    kPanic = 12,
+    kInternal = 13,
+};
+
+struct Error : std::runtime_error {
+    uint32_t status;
+    Error(uint32_t s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define OVX_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess)                                                              \
+            throw ::ovx::Error(::ovx::kInternal, std::string("HIP error: ") +              \
+                                                     hipGetErrorString(_e) + " at " #call); \
+    } while (0)
+
+// Compile-time engine limits (documented in the header and the design notes).
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kTopKMax = 1024;      // largest k a single block-level top-k keeps
+constexpr uint32_t kStage1MaxM = 4096;   // largest stage-1 candidate count per query
+constexpr uint32_t kTieCap = 4096;       // boundary-tier ids buffered per query before
+                                         // the ordered fallback scan is used
+constexpr uint32_t kEncodeMaxDim = 4096; // LDS rank sort capacity per row
+constexpr uint32_t kSignMaxDim = 8192;   // LDS Hamming histogram capacity
+
+// ---- kernel launchers (kernels.hip) ----------------------------------------
+
+struct QueryPrep {
+    float* qhat;     // [nq][d] L2-normalised queries
+    float* offset;   // [nq] TopK score offset (centre term for SIMD tiers, else 0)
+    uint64_t* qbits; // [nq][qpv] sign bits (may be null)
+};
+
+int select_tier(uint32_t d, uint32_t bits);
+float rankquant_inv_norm(uint32_t d, uint32_t bits);
+
+void launch_check_finite(const float* x, uint64_t count, uint32_t* flag, hipStream_t s);
+void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bits, int tier, float inv_norm,
+                            float* qhat, float* offset, hipStream_t s);
+void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, uint64_t first_doc, bool interleaved,
+                        hipStream_t s);
+void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, hipStream_t s);
+void launch_fastscan_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint8_t* packed, hipStream_t s);
+
+// Stage 1: exact top-m by (hamming asc, id asc) for nq queries over the
+// interleaved sign corpus.  Writes cand[nq][m_eff] (sorted) and, when
+// cand_h != null, the matching Hamming distances.
+struct Stage1Work {
+    uint16_t* H;          // [nq][n_pad]
+    uint16_t* hist_part;  // [nq][nseg][nbins]
+    uint32_t* thr;        // [nq][4]
+    uint32_t* counters;   // [nq][2]
+    uint64_t* below;      // [nq][m_eff]
+    uint32_t* ties;       // [nq][kTieCap]
+};
+struct Stage1Plan {
+    uint32_t qt, nseg, blocks_per_seg, nbins, hwords, lds_bytes, nseg3;
+    uint64_t n_pad;
+};
+Stage1Plan plan_stage1(uint64_t n, uint32_t d, uint64_t nq);
+size_t stage1_work_bytes(const Stage1Plan& p, uint64_t nq, uint64_t m_eff);
+void carve_stage1(const Stage1Plan& p, uint64_t nq, uint64_t m_eff, void* base, Stage1Work* w);
+void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t* qbits, uint64_t nq, uint64_t m_eff,
+                   const Stage1Plan& p, const Stage1Work& w, uint32_t* cand, uint32_t* cand_h, hipStream_t s);
+
+// Stage 2 / subset rerank over candidate rows.  Rows are CSR (offsets != null)
+// or uniform (row q = cands[q*row_m .. (q+1)*row_m)).  Output nq x out_k.
+void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat,
+                   const float* offset, uint64_t nq, const uint64_t* offsets, const uint32_t* cands, uint64_t row_m,
+                   uint64_t n_docs, uint32_t out_k, float* out_s, int64_t* out_i, hipStream_t s);
+
+// Full scans: per (query, segment) block top-k into partial keys, then merge.
+size_t fullscan_work_bytes(uint64_t n, uint64_t nq, uint32_t k);
+void launch_fullscan_rq(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t bits, int tier, float inv_norm,
+                        const float* qhat, const float* offset, uint64_t nq, uint32_t k, void* work, float* out_s,
+                        int64_t* out_i, hipStream_t s);
+void launch_fastscan_prepare(const float* qhat, uint64_t nq, uint32_t d, uint8_t* lut, float* bias_invq,
+                             hipStream_t s);
+void launch_fullscan_fs(const uint8_t* packed, uint64_t n, uint32_t d, const uint8_t* lut, const float* bias_invq,
+                        float inv_norm, uint64_t nq, uint32_t k, void* work, float* out_s, int64_t* out_i,
+                        hipStream_t s);
+
+}  // namespace ovx

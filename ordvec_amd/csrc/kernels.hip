@@ -1,0 +1,969 @@
+// CDNA4 (gfx950) kernels for the ordinal retrieval hot path.
+//
+// Exactness rules (the engine is bit-exact with the reference CPU path):
+//  * every f32 add/mul that the reference performs separately is written with
+//    __fadd_rn / __fmul_rn, every _mm*_fmadd_ps with __builtin_fmaf, and the
+//    file is compiled with -ffp-contract=off;
+//  * summation orders follow the reference tier that the same (dim, bits)
+//    selects on an AVX-512 host (src/quant.rs:300-329);
+//  * selections are total orders on integer keys, so ties resolve exactly as
+//    the reference's (score desc, id asc) / (hamming asc, id asc) contracts.
+#include "common.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace ovx {
+
+namespace {
+
+constexpr uint64_t kKeyMax = ~0ull;
+
+__device__ __forceinline__ uint32_t f32_ord(float x) {
+    uint32_t u = __float_as_uint(x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float ord_f32(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(u);
+}
+
+// Ascending key = (score desc under f32::total_cmp, id asc).
+__device__ __forceinline__ uint64_t score_key(float s, uint32_t id) {
+    return ((uint64_t)(~f32_ord(s)) << 32) | id;
+}
+
+__device__ __forceinline__ float key_score(uint64_t k) { return ord_f32(~(uint32_t)(k >> 32)); }
+
+__device__ __forceinline__ uint32_t next_pow2(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+template <typename T>
+__device__ void bitonic_sort(T* a, uint32_t P) {
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < P / 2; i += nt) {
+                uint32_t lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                uint32_t hi = lo + j;
+                bool asc = (lo & k) == 0;
+                T x = a[lo], y = a[hi];
+                if ((x > y) == asc) {
+                    a[lo] = y;
+                    a[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block-level streaming top-k on 64-bit keys (ascending = better).
+// keys[0, KMAX) hold the best seen so far (sorted after each flush); offers
+// that beat the current k-th key are appended to keys[KMAX, CAP).
+// ---------------------------------------------------------------------------
+constexpr uint32_t TK_CAP = 2048, TK_KMAX = kTopKMax, TK_BUF = TK_CAP - TK_KMAX;
+
+struct TopKShared {
+    uint64_t keys[TK_CAP];
+    uint64_t thresh;
+    uint32_t nbuf;
+};
+
+__device__ void tk_init(TopKShared& s) {
+    for (uint32_t i = threadIdx.x; i < TK_CAP; i += blockDim.x) s.keys[i] = kKeyMax;
+    if (threadIdx.x == 0) {
+        s.thresh = kKeyMax;
+        s.nbuf = 0;
+    }
+    __syncthreads();
+}
+
+__device__ void tk_flush(TopKShared& s, uint32_t k) {
+    bitonic_sort(s.keys, TK_CAP);
+    for (uint32_t i = TK_KMAX + threadIdx.x; i < TK_CAP; i += blockDim.x) s.keys[i] = kKeyMax;
+    if (threadIdx.x == 0) {
+        s.nbuf = 0;
+        s.thresh = s.keys[k - 1];
+    }
+    __syncthreads();
+}
+
+// Called by all threads between rounds: make room for up to `r` offers.
+__device__ void tk_reserve(TopKShared& s, uint32_t k, uint32_t r) {
+    __syncthreads();
+    if (s.nbuf + r > TK_BUF) tk_flush(s, k);
+}
+
+__device__ __forceinline__ void tk_offer(TopKShared& s, uint64_t key) {
+    if (key < s.thresh) {
+        uint32_t pos = atomicAdd(&s.nbuf, 1u);
+        s.keys[TK_KMAX + pos] = key;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Query preparation (src/util.rs:93-115, src/quant.rs:788-791)
+// ---------------------------------------------------------------------------
+
+__global__ void k_check_finite(const float* __restrict__ x, uint64_t count, uint32_t* flag) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    bool bad = false;
+    for (; i < count; i += stride) bad |= !isfinite(x[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+// One wave per query.  The reference sums sequentially (Rust f32 Sum folds
+// from -0.0), so lane 0 carries both running sums; all lanes write q_hat.
+__global__ __launch_bounds__(64) void k_prepare(const float* __restrict__ q, uint64_t nq, uint32_t d, int tier,
+                                                float centre, float inv_norm, float* __restrict__ qhat,
+                                                float* __restrict__ offset) {
+    const uint64_t qi = blockIdx.x;
+    const float* x = q + qi * d;
+    float* out = qhat + qi * d;
+    __shared__ float s_inv;
+    __shared__ int s_zero;
+    if (threadIdx.x == 0) {
+        float s = -0.0f;
+        for (uint32_t i = 0; i < d; i++) s = __fadd_rn(s, __fmul_rn(x[i], x[i]));
+        float norm = __fsqrt_rn(s);
+        s_zero = norm <= 1e-12f;
+        s_inv = __fdiv_rn(1.0f, norm);
+    }
+    __syncthreads();
+    const float inv = s_inv;
+    const int zero = s_zero;
+    for (uint32_t i = threadIdx.x; i < d; i += 64) out[i] = zero ? 0.0f : __fmul_rn(x[i], inv);
+    if (threadIdx.x == 0) {
+        float off = 0.0f;
+        if (tier != 0) {
+            float qs = -0.0f;
+            for (uint32_t i = 0; i < d; i++) qs = __fadd_rn(qs, zero ? 0.0f : __fmul_rn(x[i], inv));
+            off = __fmul_rn(__fmul_rn(-centre, qs), inv_norm);
+        }
+        offset[qi] = off;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Encode (src/sign_bitmap.rs:180-200, src/rank.rs:38-156)
+// ---------------------------------------------------------------------------
+
+// One wave per row: word w of the row is the 64-lane ballot of v[w*64+lane] > 0.
+__global__ __launch_bounds__(256) void k_sign_encode(const float* __restrict__ x, uint64_t n, uint32_t d,
+                                                     uint64_t* __restrict__ out, uint64_t first_doc,
+                                                     int interleaved) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const uint32_t qpv = d / 64;
+    const float* v = x + r * d;
+    const uint64_t doc = first_doc + r;
+    for (uint32_t w = 0; w < qpv; w++) {
+        uint64_t mask = __ballot(v[w * 64 + lane] > 0.0f);
+        if (lane == 0) {
+            uint64_t pos = interleaved ? (((doc >> 6) * qpv + w) << 6) + (doc & 63) : doc * qpv + w;
+            out[pos] = mask;
+        }
+    }
+}
+
+// One workgroup per row: LDS bitonic sort of (orderable value, index) keys,
+// scatter rank -> bucket = floor(rank * 2^bits / d), pack MSB-first.
+// -0.0 is canonicalised to +0.0 first: the reference compares with `==`
+// before total_cmp, so the two zeros tie and fall back to index order.
+__global__ __launch_bounds__(256) void k_rank_encode(const float* __restrict__ x, uint64_t n, uint32_t d,
+                                                     uint32_t bits, uint8_t* __restrict__ codes) {
+    extern __shared__ uint64_t smem[];
+    const uint32_t P = next_pow2(d);
+    uint64_t* keys = smem;
+    uint8_t* buck = reinterpret_cast<uint8_t*>(smem + P);
+    const uint32_t cpb = 8 / bits, bpv = d / cpb, nb = 1u << bits;
+    for (uint64_t r = blockIdx.x; r < n; r += gridDim.x) {
+        const float* v = x + r * d;
+        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+            if (i < d) {
+                float f = v[i];
+                if (f == 0.0f) f = 0.0f;
+                keys[i] = ((uint64_t)f32_ord(f) << 16) | i;
+            } else {
+                keys[i] = kKeyMax;
+            }
+        }
+        __syncthreads();
+        bitonic_sort(keys, P);
+        for (uint32_t p = threadIdx.x; p < d; p += blockDim.x) {
+            uint32_t idx = (uint32_t)(keys[p] & 0xffffu);
+            buck[idx] = (uint8_t)(((uint64_t)p * nb) / d);
+        }
+        __syncthreads();
+        uint8_t* out = codes + r * bpv;
+        for (uint32_t t = threadIdx.x; t < bpv; t += blockDim.x) {
+            uint32_t byte = 0;
+            for (uint32_t j = 0; j < cpb; j++) byte |= (uint32_t)buck[t * cpb + j] << ((cpb - 1 - j) * bits);
+            out[t] = (uint8_t)byte;
+        }
+        __syncthreads();
+    }
+}
+
+// src/fastscan.rs:87-110: out[b*(d/2)*32 + p*32 + lane] = code[2p]<<2 | code[2p+1]
+// from packed b=2 rows (tail lanes of the last block stay zero).
+__global__ void k_fastscan_pack(const uint8_t* __restrict__ codes, uint64_t n, uint32_t d, uint8_t* __restrict__ out) {
+    const uint32_t pairs = d / 2, bpv = d / 4;
+    uint64_t total = n * pairs;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t doc = t / pairs;
+        uint32_t p = (uint32_t)(t % pairs);
+        uint8_t byte = codes[doc * bpv + p / 2];
+        uint8_t nib = (p & 1) ? (byte & 0xf) : (byte >> 4);
+        out[(doc / 32) * pairs * 32 + (uint64_t)p * 32 + (doc & 31)] = nib;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 1: XOR-popcount scan + exact top-m (src/sign_bitmap.rs:231-483)
+// ---------------------------------------------------------------------------
+
+// K1: grid (query tiles, doc segments).  Each wave scores one 64-doc block per
+// step against QT queries: doc words are 512-byte coalesced loads, query
+// words are wave-uniform (scalar loads).  Writes H[q][doc] (u16) and a
+// per-(query, segment) Hamming histogram kept in LDS as packed 16-bit bins.
+template <int QT, int WCH>
+__global__ __launch_bounds__(256) void k_sign_scan(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
+                                                   uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
+                                                   uint32_t blocks_per_seg, uint64_t n_pad, uint16_t* __restrict__ H,
+                                                   uint16_t* __restrict__ hist_part, uint32_t nseg, uint32_t nbins,
+                                                   uint32_t hwords) {
+    extern __shared__ uint32_t lds_hist[];
+    const uint32_t q0 = blockIdx.x * QT;
+    const uint32_t tq = min((uint32_t)QT, nq - q0);
+    const uint32_t seg = blockIdx.y;
+    for (uint32_t i = threadIdx.x; i < QT * hwords; i += blockDim.x) lds_hist[i] = 0;
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t b_begin = seg * blocks_per_seg;
+    const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
+    const uint64_t* qb = qbits + (uint64_t)q0 * qpv;
+    for (uint32_t b = b_begin + wave; b < b_end; b += 4) {
+        uint32_t acc[QT];
+#pragma unroll
+        for (int i = 0; i < QT; i++) acc[i] = 0;
+        const uint64_t* base = words + (uint64_t)b * qpv * 64 + lane;
+        for (uint32_t w0 = 0; w0 < qpv; w0 += WCH) {
+            uint64_t dw[WCH];
+#pragma unroll
+            for (int j = 0; j < WCH; j++) dw[j] = base[(uint64_t)(w0 + j) * 64];
+#pragma unroll
+            for (int i = 0; i < QT; i++) {
+                if (i < (int)tq) {
+                    const uint64_t* qp = qb + (uint64_t)i * qpv + w0;
+#pragma unroll
+                    for (int j = 0; j < WCH; j++) acc[i] += (uint32_t)__popcll(dw[j] ^ qp[j]);
+                }
+            }
+        }
+        const uint64_t doc = (uint64_t)b * 64 + lane;
+        const bool valid = doc < n;
+#pragma unroll
+        for (int i = 0; i < QT; i++) {
+            if (i < (int)tq) {
+                H[(uint64_t)(q0 + i) * n_pad + doc] = (uint16_t)acc[i];
+                if (valid) atomicAdd(&lds_hist[i * hwords + (acc[i] >> 1)], 1u << ((acc[i] & 1) * 16));
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tq * nbins; i += blockDim.x) {
+        uint32_t qi = i / nbins, bin = i - qi * nbins;
+        uint32_t w = lds_hist[qi * hwords + (bin >> 1)];
+        hist_part[((uint64_t)(q0 + qi) * nseg + seg) * nbins + bin] = (uint16_t)((bin & 1) ? (w >> 16) : (w & 0xffffu));
+    }
+}
+
+// K2: per query, reduce segment histograms, find h* = smallest h with
+// count(hamming <= h) >= m.  thr = {h*, below = count(< h*), need, tie_total}.
+__global__ __launch_bounds__(256) void k_stage1_threshold(const uint16_t* __restrict__ hist_part, uint32_t nseg,
+                                                          uint32_t nbins, uint64_t m_eff, uint32_t* __restrict__ thr,
+                                                          uint32_t* __restrict__ counters) {
+    extern __shared__ uint32_t cnt[];
+    __shared__ uint32_t part[256];
+    const uint64_t q = blockIdx.x;
+    const uint16_t* hp = hist_part + q * nseg * nbins;
+    for (uint32_t bin = threadIdx.x; bin < nbins; bin += blockDim.x) {
+        uint32_t s = 0;
+        for (uint32_t g = 0; g < nseg; g++) s += hp[(uint64_t)g * nbins + bin];
+        cnt[bin] = s;
+    }
+    __syncthreads();
+    const uint32_t per = (nbins + blockDim.x - 1) / blockDim.x;
+    const uint32_t lo = min(nbins, threadIdx.x * per), hi = min(nbins, lo + per);
+    uint32_t local = 0;
+    for (uint32_t b = lo; b < hi; b++) local += cnt[b];
+    part[threadIdx.x] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t t = 0; t < blockDim.x; t++) {
+            uint32_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    uint64_t cum = part[threadIdx.x];
+    for (uint32_t b = lo; b < hi; b++) {
+        uint64_t c = cnt[b];
+        if (cum < m_eff && cum + c >= m_eff) {
+            thr[q * 4 + 0] = b;
+            thr[q * 4 + 1] = (uint32_t)cum;
+            thr[q * 4 + 2] = (uint32_t)(m_eff - cum);
+            thr[q * 4 + 3] = (uint32_t)c;
+        }
+        cum += c;
+    }
+    if (threadIdx.x == 0) {
+        counters[q * 2 + 0] = 0;
+        counters[q * 2 + 1] = 0;
+    }
+}
+
+// K3: stream H[q] in segments; keys below h* go to `below` (order fixed later
+// by the sort), boundary-tier ids to `ties` (capacity kTieCap).  Appends are
+// wave-aggregated: one LDS-free global atomic per wave and element slot.
+__global__ __launch_bounds__(256) void k_stage1_compact(const uint16_t* __restrict__ H, uint64_t n, uint64_t n_pad,
+                                                        uint64_t per_seg, const uint32_t* __restrict__ thr,
+                                                        uint32_t* __restrict__ counters, uint64_t m_eff,
+                                                        uint64_t* __restrict__ below, uint32_t* __restrict__ ties) {
+    const uint64_t q = blockIdx.y;
+    const uint32_t hstar = thr[q * 4 + 0];
+    const uint64_t begin = (uint64_t)blockIdx.x * per_seg;
+    const uint64_t end = min(n, begin + per_seg);
+    const uint16_t* row = H + q * n_pad;
+    uint64_t* bq = below + q * m_eff;
+    uint32_t* tq = ties + q * kTieCap;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint64_t base = begin + (uint64_t)threadIdx.x * 8; base < end + 0; base += (uint64_t)blockDim.x * 8) {
+        uint4 v = *reinterpret_cast<const uint4*>(row + base);
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            uint64_t idx = base + e;
+            uint32_t h = (w[e >> 1] >> ((e & 1) * 16)) & 0xffffu;
+            bool ok = idx < end;
+            bool is_below = ok && h < hstar;
+            bool is_tie = ok && h == hstar;
+            uint64_t mb = __ballot(is_below);
+            uint64_t mt = __ballot(is_tie);
+            if (mb) {
+                uint32_t basepos = 0;
+                if (lane == __ffsll((long long)mb) - 1) basepos = atomicAdd(&counters[q * 2 + 0], (uint32_t)__popcll(mb));
+                basepos = __shfl(basepos, __ffsll((long long)mb) - 1);
+                if (is_below) bq[basepos + __popcll(mb & lt_mask)] = ((uint64_t)h << 32) | idx;
+            }
+            if (mt) {
+                uint32_t basepos = 0;
+                if (lane == __ffsll((long long)mt) - 1) basepos = atomicAdd(&counters[q * 2 + 1], (uint32_t)__popcll(mt));
+                basepos = __shfl(basepos, __ffsll((long long)mt) - 1);
+                uint32_t pos = basepos + __popcll(mt & lt_mask);
+                if (is_tie && pos < kTieCap) tq[pos] = (uint32_t)idx;
+            }
+        }
+    }
+}
+
+// K4: per query, pick the `need` lowest boundary-tier ids (sorted buffer, or
+// an ordered scan of H when the tier overflowed kTieCap), merge with the
+// below-threshold keys and sort by (hamming, id).
+__global__ __launch_bounds__(256) void k_stage1_finalize(const uint16_t* __restrict__ H, uint64_t n, uint64_t n_pad,
+                                                         const uint32_t* __restrict__ thr,
+                                                         const uint64_t* __restrict__ below,
+                                                         const uint32_t* __restrict__ ties, uint64_t m_eff,
+                                                         uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_h) {
+    __shared__ uint64_t keys[kStage1MaxM];
+    __shared__ uint32_t tid_buf[kTieCap];
+    __shared__ uint32_t wave_cnt[4];
+    __shared__ uint32_t s_taken;
+    const uint64_t q = blockIdx.x;
+    const uint32_t hstar = thr[q * 4 + 0], nbelow = thr[q * 4 + 1], need = thr[q * 4 + 2], tie_total = thr[q * 4 + 3];
+    const uint64_t* bq = below + q * m_eff;
+    for (uint32_t i = threadIdx.x; i < nbelow; i += blockDim.x) keys[i] = bq[i];
+    if (need > 0) {
+        if (tie_total <= kTieCap) {
+            const uint32_t* tq = ties + q * kTieCap;
+            const uint32_t P = next_pow2(tie_total);
+            for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) tid_buf[i] = i < tie_total ? tq[i] : 0xffffffffu;
+            __syncthreads();
+            if (tie_total > need) bitonic_sort(tid_buf, P);
+        } else {
+            // Ordered scan: tie ids in ascending id order until `need` are taken.
+            const uint16_t* row = H + q * n_pad;
+            const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            if (threadIdx.x == 0) s_taken = 0;
+            __syncthreads();
+            for (uint64_t base = 0; base < n; base += blockDim.x) {
+                uint64_t idx = base + threadIdx.x;
+                bool is_tie = idx < n && row[idx] == hstar;
+                uint64_t mt = __ballot(is_tie);
+                if (lane == 0) wave_cnt[wave] = (uint32_t)__popcll(mt);
+                __syncthreads();
+                uint32_t before = s_taken;
+                for (uint32_t w = 0; w < wave; w++) before += wave_cnt[w];
+                uint32_t ord = before + (uint32_t)__popcll(mt & lt_mask);
+                if (is_tie && ord < need) tid_buf[ord] = (uint32_t)idx;
+                __syncthreads();
+                if (threadIdx.x == 0) s_taken += wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+                __syncthreads();
+                if (s_taken >= need) break;
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < need; j += blockDim.x)
+            keys[nbelow + j] = ((uint64_t)hstar << 32) | tid_buf[j];
+    }
+    const uint32_t P = next_pow2((uint32_t)m_eff);
+    for (uint32_t i = (uint32_t)m_eff + threadIdx.x; i < P; i += blockDim.x) keys[i] = kKeyMax;
+    __syncthreads();
+    bitonic_sort(keys, P);
+    for (uint32_t j = threadIdx.x; j < m_eff; j += blockDim.x) {
+        cand[q * m_eff + j] = (uint32_t)(keys[j] & 0xffffffffu);
+        if (cand_h) cand_h[q * m_eff + j] = (uint32_t)(keys[j] >> 32);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Asymmetric scoring (src/quant_kernels.rs), one helper per reference tier.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t code_at(const uint8_t* row, uint32_t p, uint32_t bits) {
+    const uint32_t cpb = 8 / bits;
+    const uint32_t shift = (cpb - 1 - p % cpb) * bits;
+    return (row[p / cpb] >> shift) & ((1u << bits) - 1);
+}
+
+// Scalar LUT tier (src/quant_kernels.rs:89-160): acc += q[p]*centre(code) in
+// coordinate order, one thread per document.
+__device__ __forceinline__ float score_scalar(const uint8_t* row, uint32_t d, uint32_t bits, const float* qv,
+                                              float half_span) {
+    float acc = 0.0f;
+    const uint32_t cpb = 8 / bits, bpv = d / cpb, mask = (1u << bits) - 1;
+    for (uint32_t g = 0; g < bpv; g++) {
+        uint32_t byte = row[g];
+        for (uint32_t j = 0; j < cpb; j++) {
+            uint32_t code = (byte >> ((cpb - 1 - j) * bits)) & mask;
+            float c = __fadd_rn((float)code, -half_span);
+            acc = __fadd_rn(acc, __fmul_rn(qv[g * cpb + j], c));
+        }
+    }
+    return acc;
+}
+
+// AVX-512 tier (src/quant_kernels.rs:406-581), one wave per document: lane
+// L = 16*i + l is accumulator i (chunks c = i mod 4), SIMD lane l; then the
+// (acc0+acc1)+(acc2+acc3) combine and the 8/4/2/1 halving tree.  Result is
+// valid in lane 0.
+__device__ __forceinline__ float score_avx512_wave(const uint8_t* row, uint32_t d, uint32_t bits, const float* qv) {
+    const uint32_t L = threadIdx.x & 63, i = L >> 4, l = L & 15;
+    float acc = 0.0f;
+    const uint32_t outer = d / 64;
+    for (uint32_t t = 0; t < outer; t++) {
+        uint32_t p = (i + 4 * t) * 16 + l;
+        acc = __builtin_fmaf((float)code_at(row, p, bits), qv[p], acc);
+    }
+    float a1 = __shfl(acc, (int)(L + 16) & 63);
+    float a2 = __shfl(acc, (int)(L + 32) & 63);
+    float a3 = __shfl(acc, (int)(L + 48) & 63);
+    float t16 = __fadd_rn(__fadd_rn(acc, a1), __fadd_rn(a2, a3));
+    float u = __fadd_rn(t16, __shfl(t16, (int)(L + 8) & 63));
+    float w = __fadd_rn(u, __shfl(u, (int)(L + 4) & 63));
+    float x = __fadd_rn(w, __shfl(w, (int)(L + 2) & 63));
+    return __fadd_rn(x, __shfl(x, (int)(L + 1) & 63));
+}
+
+// AVX2 tier (src/quant_kernels.rs:226-389), one wave per document using the
+// low 16 (b=2) or 8 (b=4) lanes, then horizontal_sum_avx2's order.
+__device__ __forceinline__ float score_avx2_wave(const uint8_t* row, uint32_t d, uint32_t bits, const float* qv) {
+    const uint32_t L = threadIdx.x & 63;
+    float acc = 0.0f;
+    if (bits == 2) {
+        if (L < 16)
+            for (uint32_t c = 0; c < d / 16; c++) {
+                uint32_t p = c * 16 + L;
+                acc = __builtin_fmaf((float)code_at(row, p, bits), qv[p], acc);
+            }
+        float t8 = __fadd_rn(acc, __shfl(acc, (int)(L + 8) & 63));
+        acc = t8;
+    } else {
+        if (L < 8)
+            for (uint32_t c = 0; c < d / 8; c++) {
+                uint32_t p = c * 8 + L;
+                acc = __builtin_fmaf((float)code_at(row, p, bits), qv[p], acc);
+            }
+    }
+    float s = __fadd_rn(acc, __shfl(acc, (int)(L + 4) & 63));
+    float s1 = __shfl(s, (int)(L + 1) & 63);
+    float s2 = __shfl(s, (int)(L + 2) & 63);
+    float s3 = __shfl(s, (int)(L + 3) & 63);
+    return __fadd_rn(__fadd_rn(s, s1), __fadd_rn(s2, s3));
+}
+
+constexpr uint32_t kQInLds = 8192;  // q_hat staged in LDS up to this dim
+
+// Stage 2 / subset rerank: one workgroup per query row.
+template <int TIER>
+__global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ codes, uint32_t d, uint32_t bits,
+                                                float inv_norm, const float* __restrict__ qhat,
+                                                const float* __restrict__ offset, const uint64_t* __restrict__ offsets,
+                                                const uint32_t* __restrict__ cands, uint64_t row_m, uint64_t n_docs,
+                                                uint32_t out_k, float* __restrict__ out_s, int64_t* __restrict__ out_i) {
+    __shared__ TopKShared tk;
+    __shared__ float qs[kQInLds];
+    const uint64_t q = blockIdx.x;
+    const uint64_t b = offsets ? offsets[q] : q * row_m;
+    const uint64_t e = offsets ? offsets[q + 1] : (q + 1) * row_m;
+    const uint64_t m = e - b;
+    float* os = out_s + q * out_k;
+    int64_t* oi = out_i + q * out_k;
+    if (m == 0) {
+        for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
+            os[j] = -INFINITY;
+            oi[j] = -1;
+        }
+        return;
+    }
+    const float* qg = qhat + q * d;
+    const float* qv = qg;
+    if (d <= kQInLds) {
+        for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) qs[i] = qg[i];
+        qv = qs;
+    }
+    const float off = offset[q];
+    const uint32_t bpv = d * bits / 8;
+    const float half_span = ((float)(1u << bits) - 1.0f) / 2.0f;
+    tk_init(tk);
+    const uint32_t kk = out_k;
+    if (TIER == 0) {
+        for (uint64_t r0 = 0; r0 < m; r0 += blockDim.x) {
+            tk_reserve(tk, kk, blockDim.x);
+            uint64_t j = r0 + threadIdx.x;
+            uint32_t id = j < m ? cands[b + j] : 0u;
+            if (j < m && id < n_docs) {
+                float s = __fadd_rn(__fmul_rn(score_scalar(codes + (uint64_t)id * bpv, d, bits, qv, half_span), inv_norm), off);
+                tk_offer(tk, score_key(s, id));
+            }
+        }
+    } else {
+        const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        constexpr uint32_t PER_WAVE = 8;
+        for (uint64_t r0 = 0; r0 < m; r0 += (uint64_t)nw * PER_WAVE) {
+            tk_reserve(tk, kk, nw * PER_WAVE);
+            for (uint32_t u = 0; u < PER_WAVE; u++) {
+                uint64_t j = r0 + (uint64_t)wave * PER_WAVE + u;
+                uint32_t id = j < m ? cands[b + j] : 0u;
+                if (j < m && id < n_docs) {
+                    const uint8_t* row = codes + (uint64_t)id * bpv;
+                    float raw = TIER == 2 ? score_avx512_wave(row, d, bits, qv) : score_avx2_wave(row, d, bits, qv);
+                    if ((threadIdx.x & 63) == 0) tk_offer(tk, score_key(__fadd_rn(__fmul_rn(raw, inv_norm), off), id));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    tk_flush(tk, kk);
+    for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
+        uint64_t key = tk.keys[j];
+        if (key == kKeyMax) {
+            os[j] = -INFINITY;
+            oi[j] = -1;
+        } else {
+            os[j] = key_score(key);
+            oi[j] = (int64_t)(key & 0xffffffffu);
+        }
+    }
+}
+
+// Full asymmetric scan (src/quant.rs:713-847): grid (segments, queries), each
+// workgroup keeps its segment's top-k; k_merge_topk combines segments.
+template <int TIER>
+__global__ __launch_bounds__(256) void k_fullscan_rq(const uint8_t* __restrict__ codes, uint64_t n, uint32_t d,
+                                                     uint32_t bits, float inv_norm, const float* __restrict__ qhat,
+                                                     const float* __restrict__ offset, uint64_t per_seg, uint32_t k,
+                                                     uint64_t* __restrict__ partial) {
+    __shared__ TopKShared tk;
+    __shared__ float qs[kQInLds];
+    const uint64_t q = blockIdx.y, seg = blockIdx.x, nseg = gridDim.x;
+    const uint64_t begin = seg * per_seg, end = min(n, begin + per_seg);
+    const float* qg = qhat + q * d;
+    const float* qv = qg;
+    if (d <= kQInLds) {
+        for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) qs[i] = qg[i];
+        qv = qs;
+    }
+    const float off = offset[q];
+    const uint32_t bpv = d * bits / 8;
+    const float half_span = ((float)(1u << bits) - 1.0f) / 2.0f;
+    tk_init(tk);
+    if (TIER == 0) {
+        for (uint64_t r0 = begin; r0 < end; r0 += blockDim.x) {
+            tk_reserve(tk, k, blockDim.x);
+            uint64_t di = r0 + threadIdx.x;
+            if (di < end) {
+                float s = __fadd_rn(__fmul_rn(score_scalar(codes + di * bpv, d, bits, qv, half_span), inv_norm), off);
+                tk_offer(tk, score_key(s, (uint32_t)di));
+            }
+        }
+    } else {
+        const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        constexpr uint32_t PER_WAVE = 8;
+        for (uint64_t r0 = begin; r0 < end; r0 += (uint64_t)nw * PER_WAVE) {
+            tk_reserve(tk, k, nw * PER_WAVE);
+            for (uint32_t u = 0; u < PER_WAVE; u++) {
+                uint64_t di = r0 + (uint64_t)wave * PER_WAVE + u;
+                if (di < end) {
+                    const uint8_t* row = codes + di * bpv;
+                    float raw = TIER == 2 ? score_avx512_wave(row, d, bits, qv) : score_avx2_wave(row, d, bits, qv);
+                    if ((threadIdx.x & 63) == 0) tk_offer(tk, score_key(__fadd_rn(__fmul_rn(raw, inv_norm), off), (uint32_t)di));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    tk_flush(tk, k);
+    uint64_t* out = partial + (q * nseg + seg) * k;
+    for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) out[j] = tk.keys[j];
+}
+
+__global__ __launch_bounds__(256) void k_merge_topk(const uint64_t* __restrict__ partial, uint32_t nseg, uint32_t k,
+                                                    float* __restrict__ out_s, int64_t* __restrict__ out_i) {
+    __shared__ TopKShared tk;
+    const uint64_t q = blockIdx.x;
+    const uint64_t total = (uint64_t)nseg * k;
+    const uint64_t* in = partial + q * total;
+    tk_init(tk);
+    for (uint64_t r0 = 0; r0 < total; r0 += blockDim.x) {
+        tk_reserve(tk, k, blockDim.x);
+        uint64_t j = r0 + threadIdx.x;
+        if (j < total) {
+            uint64_t key = in[j];
+            if (key != kKeyMax) tk_offer(tk, key);
+        }
+    }
+    __syncthreads();
+    tk_flush(tk, k);
+    for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) {
+        uint64_t key = tk.keys[j];
+        out_s[q * k + j] = key == kKeyMax ? -INFINITY : key_score(key);
+        out_i[q * k + j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
+    }
+}
+
+// FastScan b=2 query LUT (src/fastscan.rs:130-173): one workgroup per query.
+__global__ __launch_bounds__(256) void k_fastscan_prepare(const float* __restrict__ qhat, uint32_t d,
+                                                          uint8_t* __restrict__ lut, float* __restrict__ bias_invq) {
+    __shared__ float smin[256], smax[256];
+    const float cen[4] = {-1.5f, -0.5f, 0.5f, 1.5f};
+    const uint64_t q = blockIdx.x;
+    const float* x = qhat + q * d;
+    const uint32_t pairs = d / 2;
+    float mn = INFINITY, mx = -INFINITY;
+    for (uint32_t t = threadIdx.x; t < pairs * 16; t += blockDim.x) {
+        uint32_t p = t >> 4, a = (t >> 2) & 3, c = t & 3;
+        float v = __fadd_rn(__fmul_rn(x[2 * p], cen[a]), __fmul_rn(x[2 * p + 1], cen[c]));
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    const float gmin = smin[0], gmax = smax[0];
+    float span = __fadd_rn(gmax, -gmin);
+    if (!(span >= 1e-12f)) span = 1e-12f;
+    const float scale = __fdiv_rn(255.0f, span);
+    if (threadIdx.x == 0) {
+        bias_invq[q * 2 + 0] = __fmul_rn((float)pairs, gmin);
+        bias_invq[q * 2 + 1] = __fdiv_rn(1.0f, scale);
+    }
+    uint8_t* lq = lut + q * (uint64_t)pairs * 16;
+    for (uint32_t t = threadIdx.x; t < pairs * 16; t += blockDim.x) {
+        uint32_t p = t >> 4, a = (t >> 2) & 3, c = t & 3;
+        float v = __fadd_rn(__fmul_rn(x[2 * p], cen[a]), __fmul_rn(x[2 * p + 1], cen[c]));
+        float r = roundf(__fmul_rn(__fadd_rn(v, -gmin), scale));
+        r = fminf(fmaxf(r, 0.0f), 255.0f);
+        lq[t] = (uint8_t)r;
+    }
+}
+
+// FastScan scan (src/fastscan.rs:184-347): exact u32 accumulation of u8 LUT
+// entries, score = (bias_sum + acc*inv_q) * inv_norm.  One thread per doc.
+__global__ __launch_bounds__(256) void k_fullscan_fs(const uint8_t* __restrict__ packed, uint64_t n, uint32_t d,
+                                                     const uint8_t* __restrict__ lut, const float* __restrict__ bias_invq,
+                                                     float inv_norm, uint64_t blocks_per_seg, uint32_t k,
+                                                     uint64_t* __restrict__ partial) {
+    __shared__ TopKShared tk;
+    extern __shared__ uint8_t slut[];
+    const uint64_t q = blockIdx.y, seg = blockIdx.x, nseg = gridDim.x;
+    const uint32_t pairs = d / 2;
+    const uint8_t* lq = lut + q * (uint64_t)pairs * 16;
+    for (uint32_t i = threadIdx.x; i < pairs * 16; i += blockDim.x) slut[i] = lq[i];
+    const float bias = bias_invq[q * 2 + 0], invq = bias_invq[q * 2 + 1];
+    tk_init(tk);
+    const uint64_t nblk = (n + 31) / 32;
+    const uint64_t b0 = seg * blocks_per_seg, b1 = min(nblk, b0 + blocks_per_seg);
+    for (uint64_t blk = b0; blk < b1; blk += blockDim.x / 32) {
+        tk_reserve(tk, k, blockDim.x);
+        uint64_t mb = blk + threadIdx.x / 32;
+        uint32_t lane = threadIdx.x & 31;
+        uint64_t doc = mb * 32 + lane;
+        if (mb < b1 && doc < n) {
+            const uint8_t* bp = packed + mb * (uint64_t)pairs * 32 + lane;
+            uint32_t acc = 0;
+            for (uint32_t p = 0; p < pairs; p++) acc += slut[p * 16 + bp[(uint64_t)p * 32]];
+            float raw = __fadd_rn(bias, __fmul_rn((float)acc, invq));
+            float s = __fadd_rn(__fmul_rn(raw, inv_norm), 0.0f);
+            tk_offer(tk, score_key(s, (uint32_t)doc));
+        }
+    }
+    __syncthreads();
+    tk_flush(tk, k);
+    uint64_t* out = partial + (q * nseg + seg) * k;
+    for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) out[j] = tk.keys[j];
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+void check_launch() { OVX_HIP(hipGetLastError()); }
+
+}  // namespace
+
+int select_tier(uint32_t d, uint32_t bits) {
+    if (bits != 2 && bits != 4) return 0;
+    if (d % 64 == 0) return 2;
+    if ((bits == 2 && d % 16 == 0) || (bits == 4 && d % 8 == 0)) return 1;
+    return 0;
+}
+
+float rankquant_inv_norm(uint32_t d, uint32_t bits) {
+    double nb = (double)(1u << bits);
+    double var = (nb * nb - 1.0) / 12.0;
+    float norm = (float)std::sqrt((double)d * var);
+    return 1.0f / norm;
+}
+
+void launch_check_finite(const float* x, uint64_t count, uint32_t* flag, hipStream_t s) {
+    if (count == 0) return;
+    uint32_t grid = std::min<uint64_t>(4096, (count + 255) / 256);
+    hipLaunchKernelGGL(k_check_finite, dim3(grid), dim3(256), 0, s, x, count, flag);
+    check_launch();
+}
+
+void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bits, int tier, float inv_norm,
+                            float* qhat, float* offset, hipStream_t s) {
+    if (nq == 0) return;
+    float centre = ((float)(1u << bits) - 1.0f) / 2.0f;
+    hipLaunchKernelGGL(k_prepare, dim3((uint32_t)nq), dim3(64), 0, s, q, nq, d, tier, centre, inv_norm, qhat, offset);
+    check_launch();
+}
+
+void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, uint64_t first_doc, bool interleaved,
+                        hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_sign_encode, dim3(cdiv(n, 4)), dim3(256), 0, s, x, n, d, out, first_doc, interleaved ? 1 : 0);
+    check_launch();
+}
+
+void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, hipStream_t s) {
+    if (n == 0) return;
+    uint32_t P = 1;
+    while (P < d) P <<= 1;
+    size_t lds = (size_t)P * 8 + d;
+    uint32_t grid = (uint32_t)std::min<uint64_t>(n, 1u << 20);
+    hipLaunchKernelGGL(k_rank_encode, dim3(grid), dim3(256), lds, s, x, n, d, bits, codes);
+    check_launch();
+}
+
+void launch_fastscan_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint8_t* packed, hipStream_t s) {
+    if (n == 0) return;
+    uint64_t total = n * (d / 2);
+    uint32_t grid = (uint32_t)std::min<uint64_t>(65536, (total + 255) / 256);
+    hipLaunchKernelGGL(k_fastscan_pack, dim3(grid), dim3(256), 0, s, codes, n, d, packed);
+    check_launch();
+}
+
+Stage1Plan plan_stage1(uint64_t n, uint32_t d, uint64_t nq) {
+    Stage1Plan p{};
+    p.nbins = d + 1;
+    p.hwords = (p.nbins + 1) / 2;
+    uint32_t qt = nq <= 1 ? 1 : (nq <= 8 ? 4 : 16);
+    while (qt > 1 && (size_t)qt * p.hwords * 4 > 65536) qt = qt == 16 ? 4 : 1;
+    p.qt = qt;
+    p.lds_bytes = qt * p.hwords * 4;
+    uint64_t nblk = (n + 63) / 64;
+    p.n_pad = nblk * 64;
+    uint64_t tiles = (nq + qt - 1) / qt;
+    uint64_t want = std::max<uint64_t>(1, (2048 + tiles - 1) / tiles);
+    uint64_t nseg = std::min<uint64_t>(std::max<uint64_t>(1, nblk), want);
+    uint64_t bps = (nblk + nseg - 1) / nseg;
+    if (bps > 1023) bps = 1023;  // packed 16-bit LDS bins: <= 65472 docs per segment
+    if (bps == 0) bps = 1;
+    p.blocks_per_seg = (uint32_t)bps;
+    p.nseg = (uint32_t)std::max<uint64_t>(1, (nblk + bps - 1) / bps);
+    uint64_t want3 = std::max<uint64_t>(1, (2048 + nq - 1) / nq);
+    uint64_t max3 = std::max<uint64_t>(1, (n + 4095) / 4096);
+    p.nseg3 = (uint32_t)std::min(want3, max3);
+    return p;
+}
+
+size_t stage1_work_bytes(const Stage1Plan& p, uint64_t nq, uint64_t m_eff) {
+    size_t b = 0;
+    b += align256(nq * p.n_pad * 2);
+    b += align256(nq * (uint64_t)p.nseg * p.nbins * 2);
+    b += align256(nq * 16);
+    b += align256(nq * 8);
+    b += align256(nq * m_eff * 8);
+    b += align256(nq * (uint64_t)kTieCap * 4);
+    return b;
+}
+
+void carve_stage1(const Stage1Plan& p, uint64_t nq, uint64_t m_eff, void* base, Stage1Work* w) {
+    char* c = static_cast<char*>(base);
+    w->H = reinterpret_cast<uint16_t*>(c);
+    c += align256(nq * p.n_pad * 2);
+    w->hist_part = reinterpret_cast<uint16_t*>(c);
+    c += align256(nq * (uint64_t)p.nseg * p.nbins * 2);
+    w->thr = reinterpret_cast<uint32_t*>(c);
+    c += align256(nq * 16);
+    w->counters = reinterpret_cast<uint32_t*>(c);
+    c += align256(nq * 8);
+    w->below = reinterpret_cast<uint64_t*>(c);
+    c += align256(nq * m_eff * 8);
+    w->ties = reinterpret_cast<uint32_t*>(c);
+}
+
+template <int QT>
+static void launch_scan_qt(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
+                           const Stage1Plan& p, const Stage1Work& w, hipStream_t s) {
+    uint32_t nblk = (uint32_t)(p.n_pad / 64);
+    dim3 grid((uint32_t)((nq + QT - 1) / QT), p.nseg);
+    if (qpv % 4 == 0)
+        hipLaunchKernelGGL((k_sign_scan<QT, 4>), grid, dim3(256), p.lds_bytes, s, words, nblk, qpv, n, qbits,
+                           (uint32_t)nq, p.blocks_per_seg, p.n_pad, w.H, w.hist_part, p.nseg, p.nbins, p.hwords);
+    else if (qpv % 2 == 0)
+        hipLaunchKernelGGL((k_sign_scan<QT, 2>), grid, dim3(256), p.lds_bytes, s, words, nblk, qpv, n, qbits,
+                           (uint32_t)nq, p.blocks_per_seg, p.n_pad, w.H, w.hist_part, p.nseg, p.nbins, p.hwords);
+    else
+        hipLaunchKernelGGL((k_sign_scan<QT, 1>), grid, dim3(256), p.lds_bytes, s, words, nblk, qpv, n, qbits,
+                           (uint32_t)nq, p.blocks_per_seg, p.n_pad, w.H, w.hist_part, p.nseg, p.nbins, p.hwords);
+    check_launch();
+}
+
+void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t* qbits, uint64_t nq, uint64_t m_eff,
+                   const Stage1Plan& p, const Stage1Work& w, uint32_t* cand, uint32_t* cand_h, hipStream_t s) {
+    if (nq == 0 || m_eff == 0) return;
+    const uint32_t qpv = d / 64;
+    if (p.qt == 16) launch_scan_qt<16>(words, n, qpv, qbits, nq, p, w, s);
+    else if (p.qt == 4) launch_scan_qt<4>(words, n, qpv, qbits, nq, p, w, s);
+    else launch_scan_qt<1>(words, n, qpv, qbits, nq, p, w, s);
+    hipLaunchKernelGGL(k_stage1_threshold, dim3((uint32_t)nq), dim3(256), (size_t)p.nbins * 4, s, w.hist_part, p.nseg,
+                       p.nbins, m_eff, w.thr, w.counters);
+    check_launch();
+    uint64_t per3 = (n + p.nseg3 - 1) / p.nseg3;
+    per3 = (per3 + 7) & ~7ull;
+    hipLaunchKernelGGL(k_stage1_compact, dim3(p.nseg3, (uint32_t)nq), dim3(256), 0, s, w.H, n, p.n_pad, per3, w.thr,
+                       w.counters, m_eff, w.below, w.ties);
+    check_launch();
+    hipLaunchKernelGGL(k_stage1_finalize, dim3((uint32_t)nq), dim3(256), 0, s, w.H, n, p.n_pad, w.thr, w.below,
+                       w.ties, m_eff, cand, cand_h);
+    check_launch();
+}
+
+void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat,
+                   const float* offset, uint64_t nq, const uint64_t* offsets, const uint32_t* cands, uint64_t row_m,
+                   uint64_t n_docs, uint32_t out_k, float* out_s, int64_t* out_i, hipStream_t s) {
+    if (nq == 0 || out_k == 0) return;
+    dim3 grid((uint32_t)nq);
+    if (tier == 2)
+        hipLaunchKernelGGL(k_rerank<2>, grid, dim3(256), 0, s, codes, d, bits, inv_norm, qhat, offset, offsets, cands,
+                           row_m, n_docs, out_k, out_s, out_i);
+    else if (tier == 1)
+        hipLaunchKernelGGL(k_rerank<1>, grid, dim3(256), 0, s, codes, d, bits, inv_norm, qhat, offset, offsets, cands,
+                           row_m, n_docs, out_k, out_s, out_i);
+    else
+        hipLaunchKernelGGL(k_rerank<0>, grid, dim3(256), 0, s, codes, d, bits, inv_norm, qhat, offset, offsets, cands,
+                           row_m, n_docs, out_k, out_s, out_i);
+    check_launch();
+}
+
+static uint32_t fullscan_nseg(uint64_t n, uint64_t nq) {
+    uint64_t want = std::max<uint64_t>(1, (1024 + nq - 1) / nq);
+    uint64_t maxseg = std::max<uint64_t>(1, n / 1024);
+    return (uint32_t)std::min(want, maxseg);
+}
+
+size_t fullscan_work_bytes(uint64_t n, uint64_t nq, uint32_t k) {
+    return align256((size_t)nq * fullscan_nseg(n, nq) * k * 8);
+}
+
+void launch_fullscan_rq(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t bits, int tier, float inv_norm,
+                        const float* qhat, const float* offset, uint64_t nq, uint32_t k, void* work, float* out_s,
+                        int64_t* out_i, hipStream_t s) {
+    if (nq == 0 || k == 0) return;
+    uint32_t nseg = fullscan_nseg(n, nq);
+    uint64_t per = (n + nseg - 1) / nseg;
+    uint64_t* partial = static_cast<uint64_t*>(work);
+    dim3 grid(nseg, (uint32_t)nq);
+    if (tier == 2)
+        hipLaunchKernelGGL(k_fullscan_rq<2>, grid, dim3(256), 0, s, codes, n, d, bits, inv_norm, qhat, offset, per, k,
+                           partial);
+    else if (tier == 1)
+        hipLaunchKernelGGL(k_fullscan_rq<1>, grid, dim3(256), 0, s, codes, n, d, bits, inv_norm, qhat, offset, per, k,
+                           partial);
+    else
+        hipLaunchKernelGGL(k_fullscan_rq<0>, grid, dim3(256), 0, s, codes, n, d, bits, inv_norm, qhat, offset, per, k,
+                           partial);
+    check_launch();
+    hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, out_s, out_i);
+    check_launch();
+}
+
+void launch_fastscan_prepare(const float* qhat, uint64_t nq, uint32_t d, uint8_t* lut, float* bias_invq,
+                             hipStream_t s) {
+    if (nq == 0) return;
+    hipLaunchKernelGGL(k_fastscan_prepare, dim3((uint32_t)nq), dim3(256), 0, s, qhat, d, lut, bias_invq);
+    check_launch();
+}
+
+void launch_fullscan_fs(const uint8_t* packed, uint64_t n, uint32_t d, const uint8_t* lut, const float* bias_invq,
+                        float inv_norm, uint64_t nq, uint32_t k, void* work, float* out_s, int64_t* out_i,
+                        hipStream_t s) {
+    if (nq == 0 || k == 0) return;
+    uint32_t nseg = fullscan_nseg(n, nq);
+    uint64_t nblk = (n + 31) / 32;
+    uint64_t bps = (nblk + nseg - 1) / nseg;
+    uint64_t* partial = static_cast<uint64_t*>(work);
+    size_t lds = (size_t)(d / 2) * 16;
+    hipLaunchKernelGGL(k_fullscan_fs, dim3(nseg, (uint32_t)nq), dim3(256), lds, s, packed, n, d, lut, bias_invq,
+                       inv_norm, bps, k, partial);
+    check_launch();
+    hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, out_s, out_i);
+    check_launch();
+}
+
+}  // namespace ovx

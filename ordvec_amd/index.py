@@ -1,0 +1,430 @@
+"""Python mirror of ordvec's index API (ordvec-python/src/lib.rs:584-1480,
+``_ordvec.pyi:42-188``) over the MI355X engine.
+
+Every method calls the C ABI in ``libordvec_amd.so``; indexes live in HBM.
+Inputs may be NumPy arrays (any float dtype, coerced to float32 like the
+reference binding) or CUDA ``torch`` tensors.  NumPy in -> NumPy out; a CUDA
+tensor in -> CUDA tensors out, enqueued on torch's current stream without a
+host round trip.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Tuple
+
+import numpy as np
+
+from . import _native as N
+
+try:  # torch is optional for the host-array API
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+MAX_DIM = 65535
+MAX_SIGN_BITMAP_DIM = 1 << 24
+MAX_VECTORS = 64 * 1024 * 1024
+
+
+def _is_cuda_tensor(x) -> bool:
+    return torch is not None and isinstance(x, torch.Tensor) and x.is_cuda
+
+
+def _bind_stream():
+    N.check(N.lib().ordvec_set_stream(C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
+def _host_stream():
+    N.check(N.lib().ordvec_set_stream(None))
+
+
+def _ptr(a) -> C.c_void_p:
+    if _is_cuda_tensor(a):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _float_rows(x, dim: int, name: str):
+    """2-D float32 contiguous view; returns (array_or_tensor, rows, device_flag)."""
+    if _is_cuda_tensor(x):
+        if x.dim() != 2 or x.shape[1] != dim:
+            raise ValueError(f"{name} must be 2-D with shape (n, {dim}); got {tuple(x.shape)}")
+        if not x.is_floating_point():
+            raise ValueError(f"{name} must be a floating-point tensor")
+        x = x.to(torch.float32).contiguous()
+        return x, int(x.shape[0]), N.FLAG_DEVICE_PTRS
+    if torch is not None and isinstance(x, torch.Tensor):
+        x = x.detach().cpu().numpy()
+    a = np.asarray(x)
+    if a.ndim != 2 or a.shape[1] != dim:
+        raise ValueError(f"{name} must be 2-D with shape (n, {dim}); got {a.shape}")
+    if not np.issubdtype(a.dtype, np.floating):
+        raise ValueError(f"{name} must be a floating-point array")
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, int(a.shape[0]), 0
+
+
+def _float_vec(x, dim: int, name: str):
+    if _is_cuda_tensor(x):
+        if x.dim() != 1 or x.shape[0] != dim:
+            raise ValueError(f"{name} must be 1-D of length {dim}")
+        return x.to(torch.float32).contiguous(), N.FLAG_DEVICE_PTRS
+    a = np.asarray(x.detach().cpu().numpy() if torch is not None and isinstance(x, torch.Tensor) else x)
+    if a.ndim != 1 or a.shape[0] != dim:
+        raise ValueError(f"{name} must be 1-D of length {dim}; got {a.shape}")
+    if not np.issubdtype(a.dtype, np.floating):
+        raise ValueError(f"{name} must be a floating-point array")
+    return np.ascontiguousarray(a, dtype=np.float32), 0
+
+
+def _ids(x, name: str):
+    if _is_cuda_tensor(x):
+        return x.to(torch.int32).contiguous(), N.FLAG_DEVICE_PTRS
+    a = np.asarray(x.detach().cpu().numpy() if torch is not None and isinstance(x, torch.Tensor) else x)
+    if a.ndim != 1:
+        raise ValueError(f"{name} must be 1-D")
+    if a.size and not np.issubdtype(a.dtype, np.integer):
+        raise ValueError(f"{name} must be an integer array")
+    if a.size and (a.min() < 0 or a.max() > 0xFFFFFFFF):
+        raise ValueError(f"{name} ids must fit in u32")
+    return np.ascontiguousarray(a, dtype=np.uint32), 0
+
+
+def _out(shape, dtype, like_device):
+    if like_device:
+        tdt = {np.float32: torch.float32, np.int64: torch.int64, np.uint32: torch.int32}[dtype]
+        return torch.empty(shape, dtype=tdt, device="cuda")
+    return np.empty(shape, dtype=dtype)
+
+
+class RankQuant:
+    """B-bit bucketed-rank index resident in HBM (src/quant.rs:210-1017)."""
+
+    def __init__(self, dim: int, bits: int):
+        h = C.c_void_p()
+        N.check(N.lib().ordvec_rankquant_new(int(dim), int(bits), C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def _wrap(cls, h):
+        obj = cls.__new__(cls)
+        obj._h = h
+        return obj
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and N._lib is not None:
+            N.lib().ordvec_rankquant_free(h)
+            self._h = None
+
+    def _info(self):
+        d, b, n = C.c_uint64(), C.c_uint32(), C.c_uint64()
+        N.check(N.lib().ordvec_rankquant_info(self._h, C.byref(d), C.byref(b), C.byref(n)))
+        return d.value, b.value, n.value
+
+    @property
+    def dim(self) -> int:
+        return self._info()[0]
+
+    @property
+    def bits(self) -> int:
+        return self._info()[1]
+
+    def __len__(self) -> int:
+        return self._info()[2]
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    @property
+    def bytes_per_vec(self) -> int:
+        d, b, _ = self._info()
+        return d * b // 8
+
+    @property
+    def byte_size(self) -> int:
+        d, b, n = self._info()
+        return n * d * b // 8
+
+    def __repr__(self) -> str:
+        d, b, n = self._info()
+        return f"RankQuant(dim={d}, bits={b}, n_vectors={n})"
+
+    def add(self, vectors) -> None:
+        """RankQuant::add (src/quant.rs:587-614); encode runs on the GPU."""
+        x, n, flag = _float_rows(vectors, self.dim, "vectors")
+        if flag:
+            _bind_stream()
+        else:
+            _host_stream()
+        N.check(N.lib().ordvec_rankquant_add(self._h, _ptr(x), n, flag))
+
+    def search_asymmetric(self, queries, k: int):
+        """search_asymmetric (src/quant.rs:713-847): (scores f32[nq,k'], ids i64[nq,k'])."""
+        d, _, n = self._info()
+        q, nq, flag = _float_rows(queries, d, "queries")
+        if k < 0:
+            raise ValueError("k must be non-negative")
+        kk = min(int(k), n)
+        s = _out((nq, kk), np.float32, flag)
+        i = _out((nq, kk), np.int64, flag)
+        if nq and kk:
+            _bind_stream() if flag else _host_stream()
+            N.check(N.lib().ordvec_rankquant_search_asymmetric(self._h, _ptr(q), nq, kk, _ptr(s), _ptr(i), flag))
+        elif not flag:
+            s[:] = 0.0
+            i[:] = -1
+        return s, i
+
+    def search_asymmetric_subset(self, query, candidates, k: int):
+        """search_asymmetric_subset (src/quant.rs:1048-1083): 1-D arrays of
+        length min(k, len(candidates)), ordered (score desc, global id asc)."""
+        d, _, n = self._info()
+        q, qflag = _float_vec(query, d, "query")
+        c, cflag = _ids(candidates, "candidates")
+        if qflag != cflag:
+            raise ValueError("query and candidates must both be host arrays or both CUDA tensors")
+        m = int(c.shape[0])
+        out_k = min(int(k), m)
+        s = _out((out_k,), np.float32, qflag)
+        i = _out((out_k,), np.int64, qflag)
+        ret = C.c_uint64()
+        if qflag:
+            _bind_stream()
+        else:
+            _host_stream()
+        N.check(N.lib().ordvec_rankquant_search_subset(self._h, _ptr(q), _ptr(c) if m else None, m, int(k),
+                                                       _ptr(s) if out_k else None, _ptr(i) if out_k else None,
+                                                       C.byref(ret), qflag))
+        return s, i
+
+    def search_asymmetric_subset_batched(self, queries, offsets, candidates, k: int):
+        """search_asymmetric_subset_batched_serial (src/quant.rs:1417-1451):
+        CSR candidates, rectangular nq x min(k, len) output, padded -inf/-1."""
+        d, _, n = self._info()
+        q, nq, flag = _float_rows(queries, d, "queries")
+        if flag:
+            off = offsets.to(torch.int64).contiguous()
+            c = candidates.to(torch.int32).contiguous()
+        else:
+            off = np.ascontiguousarray(np.asarray(offsets), dtype=np.uint64)
+            c, _ = _ids(candidates, "candidates")
+            if off.shape[0] != nq + 1:
+                raise ValueError(f"candidate_offsets length {off.shape[0]} must be nq+1 ({nq + 1})")
+        nc = int(c.shape[0])
+        out_k = min(int(k), n)
+        s = _out((nq, out_k), np.float32, flag)
+        i = _out((nq, out_k), np.int64, flag)
+        if nq == 0 or out_k == 0:
+            return s, i
+        _bind_stream() if flag else _host_stream()
+        N.check(N.lib().ordvec_rankquant_search_subset_batched(
+            self._h, _ptr(q), nq, _ptr(off), _ptr(c) if nc else None, nc, int(k), _ptr(s), _ptr(i), flag))
+        return s, i
+
+    def codes(self) -> np.ndarray:
+        """Packed codes, n x bytes_per_vec (the .ovrq payload)."""
+        d, b, n = self._info()
+        bpv = d * b // 8
+        out = np.empty((n, bpv), dtype=np.uint8)
+        _host_stream()
+        N.check(N.lib().ordvec_rankquant_codes(self._h, out.ctypes.data_as(C.c_void_p), out.size, 0))
+        return out
+
+    def write(self, path: str) -> None:
+        _host_stream()
+        N.check(N.lib().ordvec_rankquant_write(self._h, str(path).encode()))
+
+    @classmethod
+    def load(cls, path: str) -> "RankQuant":
+        h = C.c_void_p()
+        _host_stream()
+        N.check(N.lib().ordvec_rankquant_load(str(path).encode(), C.byref(h)))
+        return cls._wrap(h)
+
+
+class SignBitmap:
+    """1-bit sign code index resident in HBM (src/sign_bitmap.rs:94-660)."""
+
+    def __init__(self, dim: int):
+        h = C.c_void_p()
+        N.check(N.lib().ordvec_sign_bitmap_new(int(dim), C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def _wrap(cls, h):
+        obj = cls.__new__(cls)
+        obj._h = h
+        return obj
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and N._lib is not None:
+            N.lib().ordvec_sign_bitmap_free(h)
+            self._h = None
+
+    def _info(self):
+        d, n = C.c_uint64(), C.c_uint64()
+        N.check(N.lib().ordvec_sign_bitmap_info(self._h, C.byref(d), C.byref(n)))
+        return d.value, n.value
+
+    @property
+    def dim(self) -> int:
+        return self._info()[0]
+
+    def __len__(self) -> int:
+        return self._info()[1]
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    @property
+    def bytes_per_vec(self) -> int:
+        return self.dim // 8
+
+    @property
+    def byte_size(self) -> int:
+        d, n = self._info()
+        return n * d // 8
+
+    def __repr__(self) -> str:
+        d, n = self._info()
+        return f"SignBitmap(dim={d}, n_vectors={n}, bytes_per_vector={d // 8})"
+
+    def add(self, vectors) -> None:
+        """SignBitmap::add (src/sign_bitmap.rs:180-200)."""
+        x, n, flag = _float_rows(vectors, self.dim, "vectors")
+        _bind_stream() if flag else _host_stream()
+        N.check(N.lib().ordvec_sign_bitmap_add(self._h, _ptr(x), n, flag))
+
+    def top_m_candidates_batched(self, queries, m: int, return_hamming: bool = False):
+        """top_m_candidates_batched (src/sign_bitmap.rs:369-415): nq x min(m, n)
+        candidate ids ordered (hamming asc, id asc)."""
+        d, n = self._info()
+        q, nq, flag = _float_rows(queries, d, "queries")
+        m_eff = min(int(m), n)
+        c = _out((nq, m_eff), np.uint32, flag)
+        h = _out((nq, m_eff), np.uint32, flag) if return_hamming else None
+        if nq and m_eff:
+            _bind_stream() if flag else _host_stream()
+            N.check(N.lib().ordvec_sign_bitmap_top_m(self._h, _ptr(q), nq, int(m), _ptr(c),
+                                                     _ptr(h) if h is not None else None, flag))
+        return (c, h) if return_hamming else c
+
+    def top_m_candidates(self, query, m: int):
+        """top_m_candidates (src/sign_bitmap.rs:330-362): 1-D, length min(m, n)."""
+        if _is_cuda_tensor(query):
+            return self.top_m_candidates_batched(query.reshape(1, -1), m)[0]
+        q = np.asarray(query)
+        if q.ndim != 1:
+            raise ValueError("query must be 1-D")
+        return self.top_m_candidates_batched(q.reshape(1, -1), m)[0]
+
+    def top_m_candidates_batched_serial_csr(self, queries, m: int):
+        """top_m_candidates_batched_serial_csr (src/sign_bitmap.rs:453-483):
+        returns (candidates u32[nq*m_eff], offsets u64[nq+1])."""
+        c = self.top_m_candidates_batched(queries, m)
+        nq, m_eff = c.shape
+        if _is_cuda_tensor(c):
+            offs = torch.arange(nq + 1, dtype=torch.int64, device=c.device) * m_eff
+            return c.reshape(-1), offs
+        return c.reshape(-1), np.arange(nq + 1, dtype=np.uint64) * np.uint64(m_eff)
+
+    def words(self) -> np.ndarray:
+        """Bitmaps row-major, n x dim/64 u64 (the .ovsb payload)."""
+        d, n = self._info()
+        out = np.empty((n, d // 64), dtype=np.uint64)
+        _host_stream()
+        N.check(N.lib().ordvec_sign_bitmap_words(self._h, out.ctypes.data_as(C.c_void_p), out.size, 0))
+        return out
+
+    def write(self, path: str) -> None:
+        _host_stream()
+        N.check(N.lib().ordvec_sign_bitmap_write(self._h, str(path).encode()))
+
+    @classmethod
+    def load(cls, path: str) -> "SignBitmap":
+        h = C.c_void_p()
+        _host_stream()
+        N.check(N.lib().ordvec_sign_bitmap_load(str(path).encode(), C.byref(h)))
+        return cls._wrap(h)
+
+
+class RankQuantFastscan:
+    """FastScan b=2 block-32 nibble-LUT index in HBM (src/fastscan.rs:532-705)."""
+
+    def __init__(self, dim: int):
+        h = C.c_void_p()
+        N.check(N.lib().ordvec_fastscan_new(int(dim), C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def _wrap(cls, h):
+        obj = cls.__new__(cls)
+        obj._h = h
+        return obj
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and N._lib is not None:
+            N.lib().ordvec_fastscan_free(h)
+            self._h = None
+
+    def _info(self):
+        d, n = C.c_uint64(), C.c_uint64()
+        N.check(N.lib().ordvec_fastscan_info(self._h, C.byref(d), C.byref(n)))
+        return d.value, n.value
+
+    @property
+    def dim(self) -> int:
+        return self._info()[0]
+
+    def __len__(self) -> int:
+        return self._info()[1]
+
+    def add(self, vectors) -> None:
+        x, n, flag = _float_rows(vectors, self.dim, "vectors")
+        _bind_stream() if flag else _host_stream()
+        N.check(N.lib().ordvec_fastscan_add(self._h, _ptr(x), n, flag))
+
+    def search(self, queries, k: int):
+        d, n = self._info()
+        q, nq, flag = _float_rows(queries, d, "queries")
+        kk = min(int(k), n)
+        s = _out((nq, kk), np.float32, flag)
+        i = _out((nq, kk), np.int64, flag)
+        if nq and kk:
+            _bind_stream() if flag else _host_stream()
+            N.check(N.lib().ordvec_fastscan_search(self._h, _ptr(q), nq, kk, _ptr(s), _ptr(i), flag))
+        return s, i
+
+    def write(self, path: str) -> None:
+        _host_stream()
+        N.check(N.lib().ordvec_fastscan_write(self._h, str(path).encode()))
+
+    @classmethod
+    def load(cls, path: str) -> "RankQuantFastscan":
+        h = C.c_void_p()
+        _host_stream()
+        N.check(N.lib().ordvec_fastscan_load(str(path).encode(), C.byref(h)))
+        return cls._wrap(h)
+
+
+def two_stage_search(sign: SignBitmap, rq: RankQuant, queries, m: int, k: int,
+                     assume_finite: bool = False):
+    """Sign-bitmap top-m candidates then RankQuant asymmetric rerank to k
+    (src/sign_bitmap.rs:453-483 + src/quant.rs:1336-1394) in one GPU call."""
+    d, _, n = rq._info()
+    q, nq, flag = _float_rows(queries, d, "queries")
+    out_k = min(int(k), n)
+    s = _out((nq, out_k), np.float32, flag)
+    i = _out((nq, out_k), np.int64, flag)
+    if nq and out_k:
+        _bind_stream() if flag else _host_stream()
+        f = flag | (N.FLAG_ASSUME_FINITE if assume_finite else 0)
+        N.check(N.lib().ordvec_two_stage_search(sign._h, rq._h, _ptr(q), nq, int(m), int(k), _ptr(s), _ptr(i), f))
+    return s, i
+
+
+def synchronize() -> None:
+    N.check(N.lib().ordvec_synchronize())

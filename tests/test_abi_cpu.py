@@ -1,0 +1,123 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, struct layouts match ordvec-ffi, and argument validation
+that happens before any device work returns the reference's status codes
+(ordvec-ffi/src/lib.rs:1061-1066, 1274-1472)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from ordvec_amd import _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "ordvec.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ordvec_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = N.lib()
+    names = header_functions()
+    assert len(names) >= 40
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(N.SIGNATURES), set(names) ^ set(N.SIGNATURES)
+
+
+def test_struct_sizes_match_abi_v1():
+    assert C.sizeof(N.IndexInfo) == 128
+    assert C.sizeof(N.SearchParams) == 128
+    assert C.sizeof(N.Hit) == 24
+    assert C.sizeof(N.SearchStats) == 184
+
+
+def test_versions_and_status_names():
+    lib = N.lib()
+    assert lib.ordvec_abi_version() == 1
+    assert lib.ordvec_version_string() == b"0.6.0"
+    names = [lib.ordvec_status_name(i).decode() for i in range(15)]
+    assert names == ["OK", "NULL_POINTER", "BAD_ARGUMENT", "BAD_STRUCT_SIZE", "UNSUPPORTED_FORMAT",
+                     "CORRUPT_INDEX", "IO", "DIM_MISMATCH", "NONFINITE_QUERY", "ROW_ID_OUT_OF_RANGE",
+                     "BUFFER_TOO_SMALL", "UNSUPPORTED_OPERATION", "PANIC", "INTERNAL", "UNKNOWN"]
+
+
+def test_init_defaults():
+    lib = N.lib()
+    info, params, stats = N.IndexInfo(), N.SearchParams(), N.SearchStats()
+    lib.ordvec_index_info_init(C.byref(info))
+    lib.ordvec_search_params_init(C.byref(params))
+    lib.ordvec_search_stats_init(C.byref(stats))
+    assert info.struct_size == 128 and params.struct_size == 128 and stats.struct_size == 184
+    assert stats.abi_version == 1 and params.flags == 0 and params.k == 0
+    lib.ordvec_index_info_init(None)  # NULL is a no-op
+
+
+def test_load_and_probe_status_paths(tmp_path):
+    lib = N.lib()
+    out = C.c_void_p()
+    assert lib.ordvec_index_load(None, 0, C.byref(out)) == N.NULL_POINTER
+    assert "path" in N.last_error()
+    p = tmp_path / "x.ovrq"
+    p.write_bytes(b"OVRQ\x01\x02" + (8).to_bytes(4, "little") + (1).to_bytes(4, "little") + b"\x05\xaf")
+    assert lib.ordvec_index_load(str(p).encode(), 1, C.byref(out)) == N.BAD_ARGUMENT
+    assert lib.ordvec_index_load(str(tmp_path / "missing").encode(), 0, C.byref(out)) == N.IO
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"ZZZZ\x01")
+    assert lib.ordvec_index_load(str(bad).encode(), 0, C.byref(out)) == N.CORRUPT_INDEX
+    sb = tmp_path / "a.ovsb"
+    sb.write_bytes(b"OVSB\x01" + (64).to_bytes(4, "little") + (1).to_bytes(4, "little") + bytes(8))
+    assert lib.ordvec_index_load(str(sb).encode(), 0, C.byref(out)) == N.UNSUPPORTED_FORMAT
+    # probe: metadata only, no device work
+    info = N.IndexInfo()
+    lib.ordvec_index_info_init(C.byref(info))
+    assert lib.ordvec_index_probe(str(p).encode(), 0, C.byref(info)) == N.OK
+    assert (info.kind, info.dim, info.bit_width, info.vector_count, info.bytes_per_vec) == (1, 8, 2, 1, 2)
+    assert info.source_file_size_bytes == 16 and info.capabilities == 15
+    info.struct_size = 127
+    assert lib.ordvec_index_probe(str(p).encode(), 0, C.byref(info)) == N.BAD_STRUCT_SIZE
+    info.struct_size = 128
+    trailing = tmp_path / "t.ovrq"
+    trailing.write_bytes(p.read_bytes() + b"\x00")
+    assert lib.ordvec_index_probe(str(trailing).encode(), 0, C.byref(info)) == N.CORRUPT_INDEX
+    assert "trailing" in N.last_error()
+    assert lib.ordvec_index_probe(str(sb).encode(), 0, C.byref(info)) == N.UNSUPPORTED_FORMAT
+    bm = tmp_path / "a.ovbm"
+    bm.write_bytes(b"OVBM\x01" + (64).to_bytes(4, "little") + (2).to_bytes(4, "little") +
+                   (1).to_bytes(4, "little") + bytes(7) + b"\xc0")
+    assert lib.ordvec_index_probe(str(bm).encode(), 0, C.byref(info)) == N.OK
+    assert (info.kind, info.n_top, info.bytes_per_vec) == (2, 2, 8)
+    v2 = tmp_path / "v2.ovrq"
+    v2.write_bytes(b"OVRQ\x02" + p.read_bytes()[5:])
+    assert lib.ordvec_index_probe(str(v2).encode(), 0, C.byref(info)) == N.CORRUPT_INDEX
+    assert "version" in N.last_error()
+
+
+def test_parameter_validation_needs_no_device():
+    lib = N.lib()
+    h = C.c_void_p()
+    assert lib.ordvec_rankquant_new(3, 2, C.byref(h)) == N.BAD_ARGUMENT
+    assert lib.ordvec_rankquant_new(64, 3, C.byref(h)) == N.BAD_ARGUMENT
+    assert lib.ordvec_rankquant_new(1, 1, C.byref(h)) == N.BAD_ARGUMENT
+    assert lib.ordvec_rankquant_new(70000, 1, C.byref(h)) == N.BAD_ARGUMENT
+    assert lib.ordvec_sign_bitmap_new(100, C.byref(h)) == N.BAD_ARGUMENT
+    assert lib.ordvec_fastscan_new(6, C.byref(h)) == N.BAD_ARGUMENT
+    assert lib.ordvec_rankquant_new(64, 2, None) == N.NULL_POINTER
+    assert lib.ordvec_index_search(None, None, None, 0, None, None) == N.NULL_POINTER
+    lib.ordvec_index_free(None)
+    lib.ordvec_rankquant_free(None)
+
+
+def test_python_api_validation():
+    import ordvec_amd
+    from ordvec_amd.index import _float_rows
+
+    with pytest.raises(ValueError):
+        ordvec_amd.RankQuant(3, 2)
+    with pytest.raises(ValueError):
+        _float_rows(__import__("numpy").zeros((2, 5), "float32"), 4, "vectors")
+    with pytest.raises(ValueError):
+        _float_rows(__import__("numpy").zeros((2, 4), "int32"), 4, "vectors")

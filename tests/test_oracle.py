@@ -1,0 +1,240 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each check comes from the reference's own tests (KATs / fixtures), a closed
+form, or an independent formulation (brute force, float64, numpy), chosen so
+that a plausible slip in the restatement (dropped term, wrong index, wrong
+tie rule, wrong summation order) fails one of them.
+"""
+import numpy as np
+import pytest
+
+from ordvec_amd import synthetic as S
+
+
+# --- src/rank.rs KATs (src/rank.rs:591-643,777-820) -------------------------
+
+def test_rank_transform_kats(oracle):
+    assert oracle.rank_transform([3, 1, 4, 1.5, 5, 9, 2, 6]).tolist() == [3, 0, 4, 1, 5, 7, 2, 6]
+    assert oracle.rank_transform([1, 1, 1, 1]).tolist() == [0, 1, 2, 3]
+    assert oracle.rank_transform([3, 1, 3, 2, 1]).tolist() == [3, 0, 4, 2, 1]
+    assert oracle.rank_transform(np.array([0.0, -0.0, 1.0, -0.0, 0.0], np.float32)).tolist() == [0, 1, 4, 2, 3]
+
+
+def test_rank_transform_matches_numpy_stable_argsort(oracle):
+    rng = np.random.default_rng(0)
+    for d in (2, 7, 64, 257, 1024):
+        v = rng.integers(-5, 5, size=d).astype(np.float32)  # heavy ties
+        want = np.argsort(np.argsort(v, kind="stable"), kind="stable")
+        assert np.array_equal(oracle.rank_transform(v), want)
+        assert np.array_equal(oracle.rank_transform_qsort(v), want)
+
+
+def test_bucket_pack_centre_norm(oracle):
+    assert [oracle.rank_to_bucket(r, 1024, 2) for r in (0, 255, 256, 511, 512, 1023)] == [0, 0, 1, 1, 2, 3]
+    b = np.array([0, 0, 1, 1, 2, 2, 3, 3], np.uint8)
+    assert oracle.pack_buckets(b, 2).tolist() == [0x05, 0xAF]
+    assert np.array_equal(oracle.unpack_buckets(oracle.pack_buckets(b, 2), 8, 2), b)
+    assert [oracle.bucket_centre(i, 2) for i in range(4)] == [-1.5, -0.5, 0.5, 1.5]
+    assert [oracle.bucket_centre(i, 1) for i in range(2)] == [-0.5, 0.5]
+    for d, bits in ((1024, 2), (768, 4), (128, 1)):
+        nb = 1 << bits
+        closed = np.float32(np.sqrt(d * (nb * nb - 1) / 12.0))
+        assert oracle.rankquant_norm(d, bits) == closed
+        # equals the empirical norm of any permutation's centred buckets
+        c = np.array([oracle.bucket_centre(oracle.rank_to_bucket(r, d, bits), bits) for r in range(d)], np.float64)
+        assert abs(np.sqrt((c * c).sum()) - float(closed)) < 1e-4
+
+
+def test_ovrq_payload_fixture(oracle):
+    """tests/persistence_compat.rs:130-157: RankQuant(8,2).add([0..7]) -> 05 AF."""
+    assert oracle.rq_encode(np.arange(8, dtype=np.float32)[None], 2).tolist() == [[0x05, 0xAF]]
+
+
+def test_ovsb_payload_fixture(oracle):
+    """tests/persistence_compat.rs:190-222: bits 0 and 63 set."""
+    v = -np.ones((1, 64), np.float32)
+    v[0, 0] = v[0, 63] = 1.0
+    assert int(oracle.sign_encode(v)[0, 0]) == 0x8000000000000001
+    z = np.zeros((1, 64), np.float32)
+    assert int(oracle.sign_encode(z)[0, 0]) == 0  # threshold: 0.0 is unset
+
+
+def test_encode_bucket_composition(oracle):
+    x = S.iid_unit(50, 1024, 3)
+    for bits in (1, 2, 4):
+        codes = oracle.rq_encode(x, bits)
+        for r in codes[:5]:
+            u = oracle.unpack_buckets(r, 1024, bits)
+            assert np.all(np.bincount(u, minlength=1 << bits) == 1024 >> bits)
+
+
+# --- scoring (src/quant_kernels.rs) -------------------------------------------
+
+def test_quickstart_kat(oracle):
+    """ordvec-python/tests/test_rank_quant.py:120-137: id 0, score 0.396."""
+    docs = np.array([[8, 7, 6, 5, 4, 3, 2, 1], [1, 2, 3, 4, 5, 6, 7, 8], [8, 1, 7, 2, 6, 3, 5, 4]], np.float32)
+    s, i = oracle.search_asymmetric(oracle.rq_encode(docs, 1), 8, 1, docs[:1], 1)
+    assert int(i[0, 0]) == 0 and round(float(s[0, 0]), 3) == 0.396
+    assert abs(float(s[0, 0]) - 8 / np.sqrt(204) / np.sqrt(2)) < 1e-6  # closed form
+
+
+@pytest.mark.parametrize("dim,bits", [(1024, 2), (1024, 4), (48, 2), (80, 4), (20, 2), (128, 1)])
+def test_scores_match_float64_closed_form(oracle, dim, bits):
+    """Every tier computes sum(q_hat * centre(code)) / R; a float64 evaluation
+    of that closed form bounds each tier's rounding."""
+    gen = S.MixtureNP(dim, seed=1)
+    docs = gen.docs(200)
+    q = gen.queries(docs, 1)[0][0]
+    codes = oracle.rq_encode(docs, bits)
+    qu = oracle.l2_normalise(q)
+    got = oracle.rq_scores(codes, bits, qu)
+    cent = np.array([oracle.bucket_centre(b, bits) for b in range(1 << bits)], np.float64)
+    want = np.array([(qu.astype(np.float64) * cent[oracle.unpack_buckets(r, dim, bits)]).sum() for r in codes])
+    want /= float(oracle.rankquant_norm(dim, bits))
+    assert np.max(np.abs(got - want)) < 2e-6
+
+
+def test_tiers_agree_within_contract(oracle):
+    """Scalar, AVX2 and AVX-512 orders differ only in rounding (1e-4 abs,
+    docs/determinism.md:37-47)."""
+    gen = S.MixtureNP(1024, seed=4)
+    docs = gen.docs(300)
+    qu = oracle.l2_normalise(gen.queries(docs, 1)[0][0])
+    for bits in (2, 4):
+        codes = oracle.rq_encode(docs, bits)
+        t = [oracle.rq_scores(codes, bits, qu, tier=tr) for tr in (0, 1, 2)]
+        assert np.max(np.abs(t[0] - t[1])) < 1e-5 and np.max(np.abs(t[0] - t[2])) < 1e-5
+        assert not np.array_equal(t[0], t[2])  # the orders really differ
+
+
+def test_tier_dispatch(oracle):
+    """src/quant.rs:300-329 on an AVX-512 host."""
+    assert oracle.select_tier(1024, 2) == oracle.TIER_AVX512
+    assert oracle.select_tier(768, 4) == oracle.TIER_AVX512
+    assert oracle.select_tier(48, 2) == oracle.TIER_AVX2
+    assert oracle.select_tier(80, 4) == oracle.TIER_AVX2
+    assert oracle.select_tier(20, 2) == oracle.TIER_SCALAR
+    assert oracle.select_tier(1024, 1) == oracle.TIER_SCALAR
+
+
+def test_l2_normalise(oracle):
+    v = np.array([3.0, 4.0], np.float32)
+    assert oracle.l2_normalise(v).tolist() == [np.float32(3.0) * (np.float32(1) / np.float32(5)),
+                                               np.float32(4.0) * (np.float32(1) / np.float32(5))]
+    assert np.all(oracle.l2_normalise(np.full(8, 1e-20, np.float32)) == 0.0)
+
+
+# --- TopK and tie contracts (src/util.rs:386-600, tests/determinism_contract.rs)
+
+def test_topk_matches_full_sort(oracle):
+    rng = np.random.default_rng(1)
+    s = rng.integers(0, 20, size=500).astype(np.float32) / 4
+    for k in (1, 7, 100, 500):
+        os_, oi = oracle.topk(s, k)
+        order = np.lexsort((np.arange(500), -s))[:k]
+        assert np.array_equal(oi, order) and np.array_equal(os_, s[order])
+
+
+def test_topk_signed_zero_total_order(oracle):
+    s = np.array([0.0, -0.0, 0.0, -0.0], np.float32)
+    _, oi = oracle.topk(s, 4)
+    assert oi.tolist() == [0, 2, 1, 3]  # +0 before -0 under total_cmp
+
+
+def test_subset_tie_kats(oracle):
+    docs = np.ones((12, 64), np.float32)
+    codes = oracle.rq_encode(docs, 2)
+    z = np.zeros(64, np.float32)
+    s, i = oracle.search_subset(codes, 64, 2, z, [9, 3, 7, 1], 2)
+    assert i.tolist() == [1, 3] and s.tolist() == [0.0, 0.0]
+    _, i = oracle.search_subset(codes, 64, 2, z, [7, 8, 7], 2)
+    assert i.tolist() == [7, 7]
+    s, i = oracle.subset_batched(codes, 64, 2, z[None], [0, 4], [7, 7, 3, 7], 4)
+    assert i[0].tolist() == [3, 7, 7, 7]
+    s, i = oracle.search_asymmetric(codes, 64, 2, z[None], 4)
+    assert i[0].tolist() == [0, 1, 2, 3] and np.all(s == 0.0)
+
+
+def test_subset_full_equals_full_search(oracle):
+    gen = S.MixtureNP(256, seed=5)
+    docs = gen.docs(400)
+    qs = gen.queries(docs, 3)[0]
+    codes = oracle.rq_encode(docs, 2)
+    fs, fi = oracle.search_asymmetric(codes, 256, 2, qs, 10)
+    perm = np.random.default_rng(0).permutation(400)
+    for q in range(3):
+        s, i = oracle.search_subset(codes, 256, 2, qs[q], perm, 10)
+        assert np.array_equal(i, fi[q]) and np.array_equal(s, fs[q])
+
+
+def test_subset_batched_empty_rows_padded(oracle):
+    docs = S.iid_unit(50, 64, 0)
+    codes = oracle.rq_encode(docs, 2)
+    s, i = oracle.subset_batched(codes, 64, 2, docs[:2], [0, 0, 3], [1, 2, 3], 5)
+    assert i[0].tolist() == [-1] * 5 and np.all(np.isneginf(s[0]))
+    assert i[1, 3:].tolist() == [-1, -1] and sorted(i[1, :3].tolist()) == [1, 2, 3]
+
+
+# --- stage 1 (src/sign_bitmap.rs) ---------------------------------------------
+
+def test_hamming_matches_numpy(oracle):
+    rng = np.random.default_rng(2)
+    bm = rng.integers(0, 2**63, size=(100, 4), dtype=np.uint64)
+    q = rng.integers(0, 2**63, size=4, dtype=np.uint64)
+    want = np.array([sum(bin(int(a) ^ int(b)).count("1") for a, b in zip(r, q)) for r in bm])
+    assert np.array_equal(oracle.hamming_all(bm, q), want)
+
+
+@pytest.mark.parametrize("threads", [1, 3])
+def test_streamed_top_m_equals_bruteforce(oracle, threads):
+    """tests/tiled_candgen.rs:41-175: random, tie-heavy and duplicate-run corpora."""
+    rng = np.random.default_rng(3)
+    for dim, n, nq, m in ((256, 3000, 5, 100), (768, 2500, 33, 2500), (64, 9000, 4, 700)):
+        x = rng.standard_normal((n, dim)).astype(np.float32)
+        if dim == 64:
+            x = x[rng.integers(0, 5, size=n)]  # duplicate runs
+        bm = oracle.sign_encode(x)
+        qb = oracle.sign_encode(rng.standard_normal((nq, dim)).astype(np.float32))
+        assert np.array_equal(oracle.sign_top_m(bm, qb, m, threads), oracle.sign_top_m_bruteforce(bm, qb, m))
+
+
+def test_identical_docs_give_lowest_ids(oracle):
+    """tests/determinism_contract.rs:127-150."""
+    bm = oracle.sign_encode(np.ones((10, 64), np.float32))
+    qb = oracle.sign_encode(np.ones((2, 64), np.float32))
+    assert oracle.sign_top_m(bm, qb, 5).tolist() == [[0, 1, 2, 3, 4]] * 2
+
+
+# --- FastScan (src/fastscan.rs) -----------------------------------------------
+
+def test_fastscan_layout_and_lut(oracle):
+    buckets = np.array([[0, 1, 2, 3, 3, 2, 1, 0]], np.uint8)
+    packed = oracle.fs_pack(buckets, 8)
+    assert packed.shape == (4 * 32,)
+    assert [packed[p * 32] for p in range(4)] == [0b0001, 0b1011, 0b1110, 0b0100]
+    assert np.all(packed.reshape(4, 32)[:, 1:] == 0)  # tail lanes zero
+    q = oracle.l2_normalise(np.array([1, -2, 3, 0.5], np.float32))
+    lut, bias, invq = oracle.fs_query(q)
+    cen = np.array([-1.5, -0.5, 0.5, 1.5], np.float32)
+    vals = np.array([[q[2 * p] * cen[a] + q[2 * p + 1] * cen[c] for a in range(4) for c in range(4)]
+                     for p in range(2)], np.float32)
+    gmin, gmax = vals.min(), vals.max()
+    assert bias == np.float32(2) * gmin
+    deq = bias + lut.reshape(2, 16).astype(np.float32).sum(0) * 0  # shape check only
+    assert deq.shape == (16,)
+    # each dequantised entry within half a quantisation step of the float LUT
+    step = (gmax - gmin) / 255
+    assert np.max(np.abs(gmin + lut.reshape(2, 16) * invq - vals)) <= 0.5 * step * (1 + 1e-5) + 1e-7
+
+
+def test_fastscan_top10_close_to_exact(oracle):
+    """tests/index/fastscan.rs:24-70: FastScan top-10 overlaps exact b=2."""
+    gen = S.MixtureNP(256, seed=6)
+    docs = gen.docs(2000)
+    qs = gen.queries(docs, 8)[0]
+    codes = oracle.rq_encode(docs, 2)
+    buckets = np.stack([oracle.unpack_buckets(r, 256, 2) for r in codes])
+    _, fi = oracle.fs_search(oracle.fs_pack(buckets, 256), 2000, 256, qs, 10)
+    _, ei = oracle.search_asymmetric(codes, 256, 2, qs, 10)
+    overlap = np.mean([len(set(a) & set(b)) for a, b in zip(fi, ei)])
+    assert overlap >= 9
