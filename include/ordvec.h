@@ -180,6 +180,20 @@ ordvec_status_t ordvec_set_stream(void *hip_stream);
 /* Wait for all work the library enqueued. */
 ordvec_status_t ordvec_synchronize(void);
 
+/* Event timing of the engine's kernel families (no reference counterpart;
+ * the reference's stats carry wall-clock only).  enable != 0 resets the
+ * counters and records a hipEvent pair around every launch of a family;
+ * read synchronises and returns launches and summed GPU milliseconds. */
+#define ORDVEC_KERNEL_SIGN_SCAN 0
+#define ORDVEC_KERNEL_STAGE1_SELECT 1
+#define ORDVEC_KERNEL_RERANK 2
+#define ORDVEC_KERNEL_FULL_SCAN 3
+#define ORDVEC_KERNEL_FASTSCAN 4
+#define ORDVEC_KERNEL_ENCODE 5
+#define ORDVEC_KERNEL_PREPARE 6
+ordvec_status_t ordvec_kernel_timing_enable(int enable);
+ordvec_status_t ordvec_kernel_timing_read(uint32_t family, uint64_t *launches, double *total_ms);
+
 /* RankQuant: src/quant.rs:438-491 new, 587-614 add, 713-847 search_asymmetric. */
 ordvec_status_t ordvec_rankquant_new(uint64_t dim, uint32_t bits, ordvec_rankquant_t **out);
 ordvec_status_t ordvec_rankquant_add(ordvec_rankquant_t *index, const float *vectors, uint64_t n, uint64_t flags);

@@ -1,7 +1,8 @@
 """Build step for the native library (hipcc, gfx950).
 
 Produces ``ordvec_amd/lib/libordvec_amd.so`` in-tree from ``ordvec_amd/csrc``.
-Run ``python -m ordvec_amd._build`` or call :func:`build`.
+Run ``python ordvec_amd/_build.py`` (by path: importing the package loads the
+library this builds) or call :func:`build`.
 """
 from __future__ import annotations
 

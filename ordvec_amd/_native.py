@@ -110,6 +110,8 @@ SIGNATURES = {
     "ordvec_index_search": (ST, [P, P, P, U64, P, P]),
     "ordvec_set_stream": (ST, [P]),
     "ordvec_synchronize": (ST, []),
+    "ordvec_kernel_timing_enable": (ST, [C.c_int]),
+    "ordvec_kernel_timing_read": (ST, [U32, P, P]),
     "ordvec_rankquant_new": (ST, [U64, U32, C.POINTER(P)]),
     "ordvec_rankquant_add": (ST, [P, P, U64, U64]),
     "ordvec_rankquant_info": (ST, [P, P, P, P]),
@@ -146,10 +148,17 @@ def load_library(path: str = LIB_PATH):
     global _lib
     if _lib is not None:
         return _lib
+    # torch ships its own HIP runtime (same soname as ROCm's).  Loading torch
+    # first makes this library bind to that runtime, so the process holds
+    # exactly one HIP runtime; two runtimes in one process cannot share the GPU.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     if not os.path.exists(path):
         raise ImportError(
             f"ordvec_amd native library not found at {path}; build it with "
-            "`python -m ordvec_amd._build` (hipcc, gfx950)")
+            "`python ordvec_amd/_build.py` (hipcc, gfx950)")
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
@@ -182,3 +191,19 @@ def check(status: int) -> None:
     if status == UNSUPPORTED_OPERATION:
         raise OrdvecUnsupported(status, text)
     raise OrdvecError(status, text)
+
+
+KERNEL_FAMILIES = {"sign_scan": 0, "stage1_select": 1, "rerank": 2, "full_scan": 3, "fastscan": 4,
+                   "encode": 5, "prepare": 6}
+
+
+def kernel_timing_enable(on: bool = True) -> None:
+    check(lib().ordvec_kernel_timing_enable(1 if on else 0))
+
+
+def kernel_timing_read(family: str):
+    """(launches, total GPU ms) of a kernel family since enable."""
+    n = C.c_uint64()
+    ms = C.c_double()
+    check(lib().ordvec_kernel_timing_read(KERNEL_FAMILIES[family], C.byref(n), C.byref(ms)))
+    return n.value, ms.value

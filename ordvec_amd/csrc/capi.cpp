@@ -66,7 +66,10 @@ struct Engine {
     void init() {
         if (inited) return;
         int n = 0;
-        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) fail(kInternal, "no HIP device is visible");
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n <= 0)
+            fail(kInternal, std::string("no HIP device is visible (hipGetDeviceCount: ") + hipGetErrorString(e) +
+                                ", count " + std::to_string(n) + ")");
         OVX_HIP(hipMalloc(&d_flag, 256));
         inited = true;
     }
@@ -719,7 +722,71 @@ uint64_t now_ns() {
         .count();
 }
 
+// ---------------------------------------------------------------------------
+// Optional event timing of kernel families (engine mutex held by callers)
+// ---------------------------------------------------------------------------
+struct KTimes {
+    bool enabled = false;
+    struct Pending {
+        hipEvent_t a, b;
+        int fam;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t open[kFamCount] = {};
+    uint64_t launches[kFamCount] = {};
+    double total_ms[kFamCount] = {};
+
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        OVX_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void resolve() {
+        if (pending.empty()) return;
+        OVX_HIP(hipEventSynchronize(pending.back().b));
+        for (auto& p : pending) {
+            float ms = 0.f;
+            OVX_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+            launches[p.fam]++;
+            total_ms[p.fam] += ms;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+};
+
+KTimes& kt() {
+    static KTimes* t = new KTimes();
+    return *t;
+}
+
 }  // namespace
+
+void ktime_begin(int family, hipStream_t s) {
+    KTimes& t = kt();
+    if (!t.enabled) return;
+    hipEvent_t e = t.get();
+    OVX_HIP(hipEventRecord(e, s));
+    t.open[family] = e;
+}
+
+void ktime_end(int family, hipStream_t s) {
+    KTimes& t = kt();
+    if (!t.enabled || !t.open[family]) return;
+    hipEvent_t e = t.get();
+    OVX_HIP(hipEventRecord(e, s));
+    t.pending.push_back({t.open[family], e, family});
+    t.open[family] = nullptr;
+    if (t.pending.size() > 4096) t.resolve();
+}
+
 }  // namespace ovx
 
 using namespace ovx;
@@ -1009,6 +1076,32 @@ ordvec_status_t ordvec_synchronize(void) {
         LOCK;
         eng().init();
         sync();
+    });
+}
+
+ordvec_status_t ordvec_kernel_timing_enable(int enable) {
+    return boundary([&] {
+        LOCK;
+        eng().init();
+        KTimes& t = kt();
+        t.resolve();
+        for (int f = 0; f < kFamCount; f++) {
+            t.launches[f] = 0;
+            t.total_ms[f] = 0.0;
+            t.open[f] = nullptr;
+        }
+        t.enabled = enable != 0;
+    });
+}
+
+ordvec_status_t ordvec_kernel_timing_read(uint32_t family, uint64_t* launches, double* total_ms) {
+    return boundary([&] {
+        if (family >= (uint32_t)kFamCount) fail(kBadArgument, "unknown kernel family " + std::to_string(family));
+        LOCK;
+        KTimes& t = kt();
+        t.resolve();
+        if (launches) *launches = t.launches[family];
+        if (total_ms) *total_ms = t.total_ms[family];
     });
 }
 

@@ -57,6 +57,21 @@ constexpr uint32_t kTieCap = 4096;       // boundary-tier ids buffered per query
 constexpr uint32_t kEncodeMaxDim = 4096; // LDS rank sort capacity per row
 constexpr uint32_t kSignMaxDim = 8192;   // LDS Hamming histogram capacity
 
+// ---- optional per-kernel event timing (capi.cpp) ----------------------------
+// Families of launches whose GPU time can be read back through the C ABI.
+enum KernelFamily : int {
+    kFamSignScan = 0,    // stage-1 XOR-popcount scan (k_sign_scan)
+    kFamStage1Select,    // stage-1 threshold + compaction + finalize
+    kFamRerank,          // stage-2 / subset rerank
+    kFamFullScan,        // full asymmetric scans and their merge
+    kFamFastScan,        // FastScan LUT build, scan and merge
+    kFamEncode,          // rank / sign / FastScan encode
+    kFamPrepare,         // query normalisation
+    kFamCount
+};
+void ktime_begin(int family, hipStream_t s);
+void ktime_end(int family, hipStream_t s);
+
 // ---- kernel launchers (kernels.hip) ----------------------------------------
 
 struct QueryPrep {

@@ -778,15 +778,19 @@ void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bi
                             float* qhat, float* offset, hipStream_t s) {
     if (nq == 0) return;
     float centre = ((float)(1u << bits) - 1.0f) / 2.0f;
+    ktime_begin(kFamPrepare, s);
     hipLaunchKernelGGL(k_prepare, dim3((uint32_t)nq), dim3(64), 0, s, q, nq, d, tier, centre, inv_norm, qhat, offset);
     check_launch();
+    ktime_end(kFamPrepare, s);
 }
 
 void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, uint64_t first_doc, bool interleaved,
                         hipStream_t s) {
     if (n == 0) return;
+    ktime_begin(kFamEncode, s);
     hipLaunchKernelGGL(k_sign_encode, dim3(cdiv(n, 4)), dim3(256), 0, s, x, n, d, out, first_doc, interleaved ? 1 : 0);
     check_launch();
+    ktime_end(kFamEncode, s);
 }
 
 void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, hipStream_t s) {
@@ -795,16 +799,20 @@ void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, u
     while (P < d) P <<= 1;
     size_t lds = (size_t)P * 8 + d;
     uint32_t grid = (uint32_t)std::min<uint64_t>(n, 1u << 20);
+    ktime_begin(kFamEncode, s);
     hipLaunchKernelGGL(k_rank_encode, dim3(grid), dim3(256), lds, s, x, n, d, bits, codes);
     check_launch();
+    ktime_end(kFamEncode, s);
 }
 
 void launch_fastscan_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint8_t* packed, hipStream_t s) {
     if (n == 0) return;
     uint64_t total = n * (d / 2);
     uint32_t grid = (uint32_t)std::min<uint64_t>(65536, (total + 255) / 256);
+    ktime_begin(kFamEncode, s);
     hipLaunchKernelGGL(k_fastscan_pack, dim3(grid), dim3(256), 0, s, codes, n, d, packed);
     check_launch();
+    ktime_end(kFamEncode, s);
 }
 
 Stage1Plan plan_stage1(uint64_t n, uint32_t d, uint64_t nq) {
@@ -878,9 +886,12 @@ void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t
                    const Stage1Plan& p, const Stage1Work& w, uint32_t* cand, uint32_t* cand_h, hipStream_t s) {
     if (nq == 0 || m_eff == 0) return;
     const uint32_t qpv = d / 64;
+    ktime_begin(kFamSignScan, s);
     if (p.qt == 16) launch_scan_qt<16>(words, n, qpv, qbits, nq, p, w, s);
     else if (p.qt == 4) launch_scan_qt<4>(words, n, qpv, qbits, nq, p, w, s);
     else launch_scan_qt<1>(words, n, qpv, qbits, nq, p, w, s);
+    ktime_end(kFamSignScan, s);
+    ktime_begin(kFamStage1Select, s);
     hipLaunchKernelGGL(k_stage1_threshold, dim3((uint32_t)nq), dim3(256), (size_t)p.nbins * 4, s, w.hist_part, p.nseg,
                        p.nbins, m_eff, w.thr, w.counters);
     check_launch();
@@ -892,6 +903,7 @@ void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t
     hipLaunchKernelGGL(k_stage1_finalize, dim3((uint32_t)nq), dim3(256), 0, s, w.H, n, p.n_pad, w.thr, w.below,
                        w.ties, m_eff, cand, cand_h);
     check_launch();
+    ktime_end(kFamStage1Select, s);
 }
 
 void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat,
@@ -899,6 +911,7 @@ void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, fl
                    uint64_t n_docs, uint32_t out_k, float* out_s, int64_t* out_i, hipStream_t s) {
     if (nq == 0 || out_k == 0) return;
     dim3 grid((uint32_t)nq);
+    ktime_begin(kFamRerank, s);
     if (tier == 2)
         hipLaunchKernelGGL(k_rerank<2>, grid, dim3(256), 0, s, codes, d, bits, inv_norm, qhat, offset, offsets, cands,
                            row_m, n_docs, out_k, out_s, out_i);
@@ -909,6 +922,7 @@ void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, fl
         hipLaunchKernelGGL(k_rerank<0>, grid, dim3(256), 0, s, codes, d, bits, inv_norm, qhat, offset, offsets, cands,
                            row_m, n_docs, out_k, out_s, out_i);
     check_launch();
+    ktime_end(kFamRerank, s);
 }
 
 static uint32_t fullscan_nseg(uint64_t n, uint64_t nq) {
@@ -929,6 +943,7 @@ void launch_fullscan_rq(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t b
     uint64_t per = (n + nseg - 1) / nseg;
     uint64_t* partial = static_cast<uint64_t*>(work);
     dim3 grid(nseg, (uint32_t)nq);
+    ktime_begin(kFamFullScan, s);
     if (tier == 2)
         hipLaunchKernelGGL(k_fullscan_rq<2>, grid, dim3(256), 0, s, codes, n, d, bits, inv_norm, qhat, offset, per, k,
                            partial);
@@ -941,13 +956,16 @@ void launch_fullscan_rq(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t b
     check_launch();
     hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, out_s, out_i);
     check_launch();
+    ktime_end(kFamFullScan, s);
 }
 
 void launch_fastscan_prepare(const float* qhat, uint64_t nq, uint32_t d, uint8_t* lut, float* bias_invq,
                              hipStream_t s) {
     if (nq == 0) return;
+    ktime_begin(kFamFastScan, s);
     hipLaunchKernelGGL(k_fastscan_prepare, dim3((uint32_t)nq), dim3(256), 0, s, qhat, d, lut, bias_invq);
     check_launch();
+    ktime_end(kFamFastScan, s);
 }
 
 void launch_fullscan_fs(const uint8_t* packed, uint64_t n, uint32_t d, const uint8_t* lut, const float* bias_invq,
@@ -959,11 +977,13 @@ void launch_fullscan_fs(const uint8_t* packed, uint64_t n, uint32_t d, const uin
     uint64_t bps = (nblk + nseg - 1) / nseg;
     uint64_t* partial = static_cast<uint64_t*>(work);
     size_t lds = (size_t)(d / 2) * 16;
+    ktime_begin(kFamFastScan, s);
     hipLaunchKernelGGL(k_fullscan_fs, dim3(nseg, (uint32_t)nq), dim3(256), lds, s, packed, n, d, lut, bias_invq,
                        inv_norm, bps, k, partial);
     check_launch();
     hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, out_s, out_i);
     check_launch();
+    ktime_end(kFamFastScan, s);
 }
 
 }  // namespace ovx

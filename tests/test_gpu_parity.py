@@ -57,10 +57,14 @@ def test_encode_ties_and_signed_zeros(engine, oracle):
         [0.0, -0.0, 1.0, -0.0, 0.0, 2.0, -1.0, 0.0],
         [3.0, 1.0, 3.0, 2.0, 1.0, 5.0, 5.0, -7.0],
     ], dtype=np.float32)
-    for bits in (1, 2, 4):
+    for bits in (1, 2):
         rq = engine.RankQuant(8, bits)
         rq.add(rows)
         assert np.array_equal(rq.codes(), oracle.rq_encode(rows, bits)), bits
+    rows16 = np.concatenate([rows, rows[:, ::-1]], axis=1)  # b=4 needs dim % 16 == 0
+    rq = engine.RankQuant(16, 4)
+    rq.add(rows16)
+    assert np.array_equal(rq.codes(), oracle.rq_encode(rows16, 4))
     rq = engine.RankQuant(8, 2)
     rq.add(rows[:1])
     assert rq.codes()[0].tolist() == [0x05, 0xAF]  # tests/persistence_compat.rs:130-157
