@@ -191,6 +191,7 @@ ordvec_status_t ordvec_synchronize(void);
 #define ORDVEC_KERNEL_FASTSCAN 4
 #define ORDVEC_KERNEL_ENCODE 5
 #define ORDVEC_KERNEL_PREPARE 6
+#define ORDVEC_KERNEL_STAGE1_SAMPLE 7
 ordvec_status_t ordvec_kernel_timing_enable(int enable);
 ordvec_status_t ordvec_kernel_timing_read(uint32_t family, uint64_t *launches, double *total_ms);
 

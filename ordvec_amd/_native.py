@@ -194,7 +194,7 @@ def check(status: int) -> None:
 
 
 KERNEL_FAMILIES = {"sign_scan": 0, "stage1_select": 1, "rerank": 2, "full_scan": 3, "fastscan": 4,
-                   "encode": 5, "prepare": 6}
+                   "encode": 5, "prepare": 6, "stage1_sample": 7}
 
 
 def kernel_timing_enable(on: bool = True) -> None:

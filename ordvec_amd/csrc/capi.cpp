@@ -55,13 +55,40 @@ constexpr uint64_t kMaxPayload = 128ull * 1024 * 1024 * 1024;
 // ---------------------------------------------------------------------------
 // Engine: one stream, a grow-only workspace and staging area, one mutex.
 // ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t size = 0;
+    void* get(size_t bytes, hipStream_t stream) {
+        if (bytes <= size && p) return p;
+        if (p) {
+            OVX_HIP(hipStreamSynchronize(stream));
+            OVX_HIP(hipFree(p));
+            p = nullptr;
+        }
+        size_t want = std::max(bytes, size + size / 2);
+        if (hipMalloc(&p, std::max<size_t>(want, 256)) != hipSuccess) {
+            (void)hipGetLastError();
+            size = 0;
+            if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+                (void)hipGetLastError();
+                p = nullptr;
+                fail(kInternal, "device workspace allocation of " + std::to_string(bytes) + " bytes failed");
+            }
+            want = bytes;
+        }
+        size = std::max<size_t>(want, 256);
+        return p;
+    }
+};
+
 struct Engine {
     std::recursive_mutex mu;
     hipStream_t stream = nullptr;
     bool inited = false;
-    void* ws = nullptr;
-    size_t ws_size = 0;
+    DevBuf ws, ws2;  // per-call workspace; ws2 serves the exact stage-1 recomputation
     uint32_t* d_flag = nullptr;
+    uint32_t* pinned = nullptr;  // host-visible status readback
+    size_t pinned_count = 0;
 
     void init() {
         if (inited) return;
@@ -76,26 +103,24 @@ struct Engine {
 
     void* workspace(size_t bytes) {
         init();
-        if (bytes > ws_size) {
-            if (ws) {
+        return ws.get(bytes, stream);
+    }
+    void* workspace2(size_t bytes) {
+        init();
+        return ws2.get(bytes, stream);
+    }
+    uint32_t* pinned_u32(size_t count) {
+        if (count > pinned_count) {
+            if (pinned) {
                 OVX_HIP(hipStreamSynchronize(stream));
-                OVX_HIP(hipFree(ws));
-                ws = nullptr;
+                OVX_HIP(hipHostFree(pinned));
+                pinned = nullptr;
             }
-            size_t want = std::max(bytes, ws_size + ws_size / 2);
-            if (hipMalloc(&ws, want) != hipSuccess) {
-                (void)hipGetLastError();
-                ws_size = 0;
-                if (hipMalloc(&ws, bytes) != hipSuccess) {
-                    (void)hipGetLastError();
-                    ws = nullptr;
-                    fail(kInternal, "device workspace allocation of " + std::to_string(bytes) + " bytes failed");
-                }
-                want = bytes;
-            }
-            ws_size = want;
+            size_t want = std::max<size_t>(count, 1024);
+            OVX_HIP(hipHostMalloc((void**)&pinned, want * 4, hipHostMallocDefault));
+            pinned_count = want;
         }
-        return ws;
+        return pinned;
     }
 };
 
@@ -299,10 +324,13 @@ void sb_add(SbIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
 }
 
 // ---------------------------------------------------------------------------
-// Search pipelines
+// Search pipelines.  *_dev functions enqueue work only (device pointers, no
+// synchronisation).  Each query carries a status word: bit 0 = non-finite
+// input, bit 1 = stage-1 survivor buffer could not prove exactness (the query
+// is then recomputed by the exact histogram pipeline).
 // ---------------------------------------------------------------------------
+constexpr uint32_t kStatNonfinite = 1, kStatFallback = 2;
 
-// Stage queries to the device if needed; returns a device pointer.
 const float* stage_queries(Carver& cv, const float* q, uint64_t count, uint64_t flags) {
     if (flags & ORDVEC_FLAG_DEVICE_PTRS) return q;
     float* d = cv.take<float>(count);
@@ -310,32 +338,137 @@ const float* stage_queries(Carver& cv, const float* q, uint64_t count, uint64_t 
     return d;
 }
 
+// Waits for the stream and returns the per-query status words.
+std::vector<uint32_t> read_qstat(const uint32_t* dstat, uint64_t nq) {
+    uint32_t* h = eng().pinned_u32(nq);
+    d2h(h, dstat, nq * 4);
+    sync();
+    return std::vector<uint32_t>(h, h + nq);
+}
+
+bool any_bit(const std::vector<uint32_t>& st, uint32_t bit) {
+    for (uint32_t v : st)
+        if (v & bit) return true;
+    return false;
+}
+
+void check_k(uint64_t k) {
+    if (k > kTopKMax) fail(kUnsupportedOperation, "k above " + std::to_string(kTopKMax) + " is not supported by the GPU engine");
+}
+
+void check_search_dim(uint32_t d) {
+    if (d > 8192) fail(kUnsupportedOperation, "GPU search stages q_hat in LDS and supports dim <= 8192");
+}
+
+void check_m(uint64_t m_eff) {
+    if (m_eff > kStage1MaxM)
+        fail(kUnsupportedOperation, "stage-1 candidate count above " + std::to_string(kStage1MaxM) +
+                                        " is not supported by the GPU engine");
+}
+
+size_t full_search_bytes(const RqIndex* ix, uint64_t nq, uint64_t kk) {
+    return al(nq * ix->dim * 4) + al(nq * 4) * 2 + fullscan_work_bytes(ix->n, nq, (uint32_t)kk);
+}
+
+void full_search_dev(const RqIndex* ix, const float* dq, uint64_t nq, uint64_t kk, float* ds, int64_t* di,
+                     uint32_t* qstat, Carver& cv) {
+    const uint32_t d = ix->dim;
+    float* qhat = cv.take<float>(nq * d);
+    float* off = cv.take<float>(nq);
+    void* work = cv.take<char>(fullscan_work_bytes(ix->n, nq, (uint32_t)kk));
+    const int tier = select_tier(d, ix->bits);
+    const float inv_norm = rankquant_inv_norm(d, ix->bits);
+    hipStream_t s = eng().stream;
+    launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, qstat, s);
+    launch_fullscan_rq(ix->codes, ix->n, d, ix->bits, tier, inv_norm, qhat, off, nq, (uint32_t)kk, work, ds, di, s);
+}
+
+size_t subset_search_bytes(const RqIndex* ix, uint64_t nq) { return al(nq * ix->dim * 4) + al(nq * 4) * 2; }
+
+void subset_search_dev(const RqIndex* ix, const float* dq, uint64_t nq, const uint64_t* doff, const uint32_t* dc,
+                       uint64_t row_m, uint64_t out_k, float* ds, int64_t* di, uint32_t* qstat, Carver& cv) {
+    const uint32_t d = ix->dim;
+    float* qhat = cv.take<float>(nq * d);
+    float* off = cv.take<float>(nq);
+    const int tier = select_tier(d, ix->bits);
+    const float inv_norm = rankquant_inv_norm(d, ix->bits);
+    hipStream_t s = eng().stream;
+    launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, qstat, s);
+    launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, doff, dc, row_m, ix->n, nullptr,
+                  (uint32_t)out_k, ds, di, s);
+}
+
+struct Stage1Bufs {
+    uint64_t* qbits;
+    FastPlan plan;
+    FastWork work;
+};
+
+size_t stage1_bytes(const SbIndex* ix, uint64_t nq, uint64_t m_eff) {
+    FastPlan p = plan_stage1_fast(ix->n, ix->dim, nq, m_eff);
+    return al(nq * ix->qpv * 8) + al(stage1_fast_bytes(p, nq));
+}
+
+Stage1Bufs stage1_dev(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t m_eff, uint32_t* cand,
+                      uint32_t* cand_h, uint32_t* qstat, Carver& cv) {
+    Stage1Bufs b{};
+    b.qbits = cv.take<uint64_t>(nq * ix->qpv);
+    b.plan = plan_stage1_fast(ix->n, ix->dim, nq, m_eff);
+    void* base = cv.take<char>(stage1_fast_bytes(b.plan, nq));
+    carve_stage1_fast(b.plan, nq, base, &b.work);
+    launch_stage1_fast(ix->words, ix->n, ix->dim, dq, nq, m_eff, b.plan, b.work, b.qbits, cand, cand_h, qstat,
+                       eng().stream);
+    return b;
+}
+
+std::vector<uint32_t> flagged(const std::vector<uint32_t>& st, uint32_t bit) {
+    std::vector<uint32_t> f;
+    for (uint32_t i = 0; i < st.size(); i++)
+        if (st[i] & bit) f.push_back(i);
+    return f;
+}
+
+void d2d(void* dst, const void* src, size_t bytes) {
+    if (bytes) OVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, eng().stream));
+}
+
+// Host-facing wrappers: host or device pointers per `flags`; host-pointer
+// calls return after the results are copied back.
+void host_finite_or_fail(const float* x, uint64_t n, uint64_t flags, uint32_t status, const char* what) {
+    if ((flags & (ORDVEC_FLAG_DEVICE_PTRS | ORDVEC_FLAG_ASSUME_FINITE)) == 0 && !host_all_finite(x, n))
+        fail(status, std::string(what) + " contains NaN or infinity");
+}
+
+// Host pointers: always wait (results).  Device pointers: wait only to read
+// the non-finite bit, unless the caller vouched for the inputs.
+void finish_status(const uint32_t* qstat, uint64_t nq, uint64_t flags, uint32_t status, const char* what) {
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    if (dev && (flags & ORDVEC_FLAG_ASSUME_FINITE)) return;
+    std::vector<uint32_t> st = read_qstat(qstat, nq);
+    if (dev && any_bit(st, kStatNonfinite)) fail(status, std::string(what) + " contains NaN or infinity");
+}
+
 void rq_full_search(const RqIndex* ix, const float* queries, uint64_t nq, uint64_t k, float* out_s, int64_t* out_i,
                     uint64_t flags) {
     const uint64_t kk = std::min<uint64_t>(k, ix->n);
     if (nq == 0 || kk == 0) return;
-    if (kk > kTopKMax) fail(kUnsupportedOperation, "k above " + std::to_string(kTopKMax) + " is not supported by the GPU engine");
-    const uint32_t d = ix->dim;
+    check_k(kk);
+    check_search_dim(ix->dim);
+    host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-    size_t bytes = al(nq * d * 4) * 2 + al(nq * 4) + fullscan_work_bytes(ix->n, nq, (uint32_t)kk) +
-                   (dev ? 0 : al(nq * kk * 4) + al(nq * kk * 8));
+    size_t bytes = full_search_bytes(ix, nq, kk) + al(nq * 4) +
+                   (dev ? 0 : al(nq * ix->dim * 4) + al(nq * kk * 4) + al(nq * kk * 8));
     Carver cv{static_cast<char*>(eng().workspace(bytes))};
-    const float* dq = stage_queries(cv, queries, nq * d, flags);
-    float* qhat = cv.take<float>(nq * d);
-    float* off = cv.take<float>(nq);
-    void* work = cv.take<uint64_t>(fullscan_work_bytes(ix->n, nq, (uint32_t)kk) / 8);
+    const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
+    uint32_t* qstat = cv.take<uint32_t>(nq);
     float* ds = dev ? out_s : cv.take<float>(nq * kk);
     int64_t* di = dev ? out_i : cv.take<int64_t>(nq * kk);
-    const int tier = select_tier(d, ix->bits);
-    const float inv_norm = rankquant_inv_norm(d, ix->bits);
-    hipStream_t s = eng().stream;
-    launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, s);
-    launch_fullscan_rq(ix->codes, ix->n, d, ix->bits, tier, inv_norm, qhat, off, nq, (uint32_t)kk, work, ds, di, s);
+    full_search_dev(ix, dq, nq, kk, ds, di, qstat, cv);
     if (!dev) {
         d2h(out_s, ds, nq * kk * 4);
         d2h(out_i, di, nq * kk * 8);
-        sync();
     }
+    finish_status(qstat, nq, flags, kNonfiniteQuery, "queries");
 }
 
 // CSR (offsets != null) or uniform rows; outputs nq x out_k.
@@ -343,15 +476,16 @@ void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, cons
                       const uint32_t* cands, uint64_t n_cands, uint64_t row_m, uint64_t out_k, float* out_s,
                       int64_t* out_i, uint64_t flags) {
     if (nq == 0 || out_k == 0) return;
-    if (out_k > kTopKMax) fail(kUnsupportedOperation, "k above " + std::to_string(kTopKMax) + " is not supported by the GPU engine");
-    const uint32_t d = ix->dim;
+    check_k(out_k);
+    check_search_dim(ix->dim);
+    host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-    size_t bytes = al(nq * d * 4) * 2 + al(nq * 4) +
-                   (dev ? 0 : al((nq + 1) * 8) + al(n_cands * 4) + al(nq * out_k * 4) + al(nq * out_k * 8));
+    size_t bytes = subset_search_bytes(ix, nq) + al(nq * 4) +
+                   (dev ? 0 : al(nq * ix->dim * 4) + al((nq + 1) * 8) + al(n_cands * 4) + al(nq * out_k * 4) +
+                                  al(nq * out_k * 8));
     Carver cv{static_cast<char*>(eng().workspace(bytes))};
-    const float* dq = stage_queries(cv, queries, nq * d, flags);
-    float* qhat = cv.take<float>(nq * d);
-    float* off = cv.take<float>(nq);
+    const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
+    uint32_t* qstat = cv.take<uint32_t>(nq);
     const uint64_t* doff = offsets;
     const uint32_t* dc = cands;
     float* ds = out_s;
@@ -368,41 +502,33 @@ void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, cons
         ds = cv.take<float>(nq * out_k);
         di = cv.take<int64_t>(nq * out_k);
     }
-    const int tier = select_tier(d, ix->bits);
-    const float inv_norm = rankquant_inv_norm(d, ix->bits);
-    hipStream_t s = eng().stream;
-    launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, s);
-    launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, doff, dc, row_m, ix->n, (uint32_t)out_k, ds, di, s);
+    subset_search_dev(ix, dq, nq, doff, dc, row_m, out_k, ds, di, qstat, cv);
     if (!dev) {
         d2h(out_s, ds, nq * out_k * 4);
         d2h(out_i, di, nq * out_k * 8);
-        sync();
     }
+    finish_status(qstat, nq, flags, kNonfiniteQuery, "queries");
 }
 
-// Stage 1 into device buffers cand (nq x m_eff) and optional hamming.
-void sb_top_m_device(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t m_eff, uint32_t* cand,
-                     uint32_t* cand_h, Carver& cv) {
-    const uint32_t d = ix->dim;
-    uint64_t* qbits = cv.take<uint64_t>(nq * ix->qpv);
-    Stage1Plan plan = plan_stage1(ix->n, d, nq);
+// Exact recomputation of flagged queries (histogram pipeline, workspace 2);
+// rewrites their rows of cand / cand_h in place.
+void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint32_t>& fl, uint64_t m_eff,
+                uint32_t* cand, uint32_t* cand_h) {
+    const uint64_t nf = fl.size(), qpv = ix->qpv;
+    Stage1Plan plan = plan_stage1(ix->n, ix->dim, nf);
+    size_t bytes = al(nf * qpv * 8) + al(stage1_work_bytes(plan, nf, m_eff)) + 2 * al(nf * m_eff * 4);
+    Carver cv{static_cast<char*>(eng().workspace2(bytes))};
+    uint64_t* qb = cv.take<uint64_t>(nf * qpv);
+    for (uint64_t i = 0; i < nf; i++) d2d(qb + i * qpv, qbits + (uint64_t)fl[i] * qpv, qpv * 8);
     Stage1Work w{};
-    void* base = cv.take<char>(stage1_work_bytes(plan, nq, m_eff));
-    carve_stage1(plan, nq, m_eff, base, &w);
-    hipStream_t s = eng().stream;
-    launch_sign_encode(dq, nq, d, qbits, 0, false, s);
-    launch_stage1(ix->words, ix->n, d, qbits, nq, m_eff, plan, w, cand, cand_h, s);
-}
-
-size_t sb_top_m_bytes(const SbIndex* ix, uint64_t nq, uint64_t m_eff) {
-    Stage1Plan plan = plan_stage1(ix->n, ix->dim, nq);
-    return al(nq * ix->qpv * 8) + al(stage1_work_bytes(plan, nq, m_eff));
-}
-
-void check_m(uint64_t m_eff) {
-    if (m_eff > kStage1MaxM)
-        fail(kUnsupportedOperation, "stage-1 candidate count above " + std::to_string(kStage1MaxM) +
-                                        " is not supported by the GPU engine");
+    carve_stage1(plan, nf, m_eff, cv.take<char>(stage1_work_bytes(plan, nf, m_eff)), &w);
+    uint32_t* cc = cv.take<uint32_t>(nf * m_eff);
+    uint32_t* ch = cv.take<uint32_t>(nf * m_eff);
+    launch_stage1(ix->words, ix->n, ix->dim, qb, nf, m_eff, plan, w, cc, cand_h ? ch : nullptr, eng().stream);
+    for (uint64_t i = 0; i < nf; i++) {
+        d2d(cand + (uint64_t)fl[i] * m_eff, cc + i * m_eff, m_eff * 4);
+        if (cand_h) d2d(cand_h + (uint64_t)fl[i] * m_eff, ch + i * m_eff, m_eff * 4);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1149,7 +1275,7 @@ ordvec_status_t ordvec_rankquant_search_asymmetric(const ordvec_rankquant_t* ind
         NONNULL(out_scores, "out_scores");
         NONNULL(out_ids, "out_ids");
         LOCK;
-        check_finite(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
+
         rq_full_search(ix, queries, nq, k, out_scores, out_ids, flags);
     });
 }
@@ -1171,7 +1297,7 @@ ordvec_status_t ordvec_rankquant_search_subset(const ordvec_rankquant_t* index, 
         NONNULL(out_scores, "out_scores");
         NONNULL(out_ids, "out_ids");
         LOCK;
-        check_finite(query, ix->dim, flags, kNonfiniteQuery, "query");
+
         if (!(flags & ORDVEC_FLAG_DEVICE_PTRS))
             for (uint64_t i = 0; i < n_rows; i++)
                 if (rows[i] >= ix->n) fail(kRowIdOutOfRange, "search_asymmetric_subset: candidate id out of range");
@@ -1213,7 +1339,7 @@ ordvec_status_t ordvec_rankquant_search_subset_batched(const ordvec_rankquant_t*
         NONNULL(out_ids, "out_ids");
         if (n_candidates) NONNULL(candidates, "candidates");
         LOCK;
-        check_finite(queries, nq * ix->dim, flags, kBadArgument, "queries");
+
         rq_subset_search(ix, queries, nq, offsets, candidates, n_candidates, 0, out_k, out_scores, out_ids, flags);
     });
 }
@@ -1305,20 +1431,27 @@ ordvec_status_t ordvec_sign_bitmap_top_m(const ordvec_sign_bitmap_t* index, cons
         NONNULL(out_candidates, "out_candidates");
         check_m(m_eff);
         if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
+        host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
         LOCK;
-        check_finite(queries, nq * ix->dim, flags, kBadArgument, "queries");
         const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-        size_t bytes = al(nq * ix->dim * 4) + sb_top_m_bytes(ix, nq, m_eff) + (dev ? 0 : 2 * al(nq * m_eff * 4));
+        size_t bytes = stage1_bytes(ix, nq, m_eff) + al(nq * 4) +
+                       (dev ? 0 : al(nq * ix->dim * 4) + 2 * al(nq * m_eff * 4));
         Carver cv{static_cast<char*>(eng().workspace(bytes))};
         const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
+        uint32_t* qstat = cv.take<uint32_t>(nq);
         uint32_t* dc = dev ? out_candidates : cv.take<uint32_t>(nq * m_eff);
         uint32_t* dh = dev ? out_hamming : (out_hamming ? cv.take<uint32_t>(nq * m_eff) : nullptr);
-        sb_top_m_device(ix, dq, nq, m_eff, dc, dh, cv);
+        Stage1Bufs b = stage1_dev(ix, dq, nq, m_eff, dc, dh, qstat, cv);
+        std::vector<uint32_t> st = read_qstat(qstat, nq);
+        if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
+            fail(kNonfiniteQuery, "queries contain NaN or infinity");
+        std::vector<uint32_t> fl = flagged(st, kStatFallback);
+        if (!fl.empty()) stage1_fix(ix, b.qbits, fl, m_eff, dc, dh);
         if (!dev) {
             d2h(out_candidates, dc, nq * m_eff * 4);
             if (out_hamming) d2h(out_hamming, dh, nq * m_eff * 4);
-            sync();
         }
+        sync();
     });
 }
 
@@ -1381,30 +1514,49 @@ ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const 
         NONNULL(out_scores, "out_scores");
         NONNULL(out_ids, "out_ids");
         check_m(m_eff);
-        if (out_k > kTopKMax) fail(kUnsupportedOperation, "k above 1024 is not supported by the GPU engine");
+        check_k(out_k);
+        check_search_dim(ix->dim);
         if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
+        host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
         LOCK;
-        check_finite(queries, nq * ix->dim, flags, kBadArgument, "queries");
         const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
         const uint32_t d = ix->dim;
-        size_t bytes = al(nq * d * 4) * 2 + al(nq * 4) + al(nq * m_eff * 4) + sb_top_m_bytes(sb, nq, m_eff) +
-                       (dev ? 0 : al(nq * out_k * 4) + al(nq * out_k * 8));
+        size_t bytes = al(nq * d * 4) + 2 * al(nq * 4) + al(nq * m_eff * 4) + stage1_bytes(sb, nq, m_eff) +
+                       (dev ? 0 : al(nq * d * 4) + al(nq * out_k * 4) + al(nq * out_k * 8)) + al(nq * 4);
         Carver cv{static_cast<char*>(eng().workspace(bytes))};
         const float* dq = stage_queries(cv, queries, nq * d, flags);
         float* qhat = cv.take<float>(nq * d);
         float* off = cv.take<float>(nq);
+        uint32_t* qstat = cv.take<uint32_t>(nq);
+        uint32_t* qmap = cv.take<uint32_t>(nq);
         uint32_t* cand = cv.take<uint32_t>(nq * m_eff);
         float* ds = dev ? out_scores : cv.take<float>(nq * out_k);
         int64_t* di = dev ? out_ids : cv.take<int64_t>(nq * out_k);
         const int tier = select_tier(d, ix->bits);
         const float inv_norm = rankquant_inv_norm(d, ix->bits);
         hipStream_t s = eng().stream;
-        launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, s);
-        if (m_eff) sb_top_m_device(sb, dq, nq, m_eff, cand, nullptr, cv);
-        launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, (uint32_t)out_k, ds, di, s);
+        launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, nullptr, s);
+        Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv);
+        launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, nullptr,
+                      (uint32_t)out_k, ds, di, s);
         if (!dev) {
             d2h(out_scores, ds, nq * out_k * 4);
             d2h(out_ids, di, nq * out_k * 8);
+        }
+        std::vector<uint32_t> st = read_qstat(qstat, nq);
+        if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
+            fail(kNonfiniteQuery, "queries contain NaN or infinity");
+        std::vector<uint32_t> fl = flagged(st, kStatFallback);
+        if (!fl.empty()) {
+            stage1_fix(sb, b.qbits, fl, m_eff, cand, nullptr);
+            h2d(qmap, fl.data(), fl.size() * 4);
+            launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, fl.size(), nullptr, cand, m_eff, ix->n,
+                          qmap, (uint32_t)out_k, ds, di, s);
+            if (!dev)
+                for (uint32_t q : fl) {
+                    d2h(out_scores + (uint64_t)q * out_k, ds + (uint64_t)q * out_k, out_k * 4);
+                    d2h(out_ids + (uint64_t)q * out_k, di + (uint64_t)q * out_k, out_k * 8);
+                }
             sync();
         }
     });
@@ -1466,19 +1618,20 @@ ordvec_status_t ordvec_fastscan_search(const ordvec_fastscan_t* index, const flo
         NONNULL(queries, "queries");
         NONNULL(out_scores, "out_scores");
         NONNULL(out_ids, "out_ids");
-        if (kk > kTopKMax) fail(kUnsupportedOperation, "k above 1024 is not supported by the GPU engine");
+        check_k(kk);
         if (ix->dim > 4096) fail(kUnsupportedOperation, "FastScan LUT staging supports dim <= 4096");
+        host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
         LOCK;
-        check_finite(queries, nq * ix->dim, flags, kBadArgument, "queries");
         const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
         const uint32_t d = ix->dim;
         size_t wk = fullscan_work_bytes(ix->n, nq, (uint32_t)kk);
-        size_t bytes = al(nq * d * 4) * 2 + al(nq * 4) + al(nq * (d / 2) * 16) + al(nq * 8) + wk +
+        size_t bytes = al(nq * d * 4) * 2 + 2 * al(nq * 4) + al(nq * (d / 2) * 16) + al(nq * 8) + wk +
                        (dev ? 0 : al(nq * kk * 4) + al(nq * kk * 8));
         Carver cv{static_cast<char*>(eng().workspace(bytes))};
         const float* dq = stage_queries(cv, queries, nq * d, flags);
         float* qhat = cv.take<float>(nq * d);
         float* off = cv.take<float>(nq);
+        uint32_t* qstat = cv.take<uint32_t>(nq);
         uint8_t* lut = cv.take<uint8_t>(nq * (d / 2) * 16);
         float* bq = cv.take<float>(nq * 2);
         void* work = cv.take<char>(wk);
@@ -1486,14 +1639,14 @@ ordvec_status_t ordvec_fastscan_search(const ordvec_fastscan_t* index, const flo
         int64_t* di = dev ? out_ids : cv.take<int64_t>(nq * kk);
         const float inv_norm = rankquant_inv_norm(d, 2);
         hipStream_t s = eng().stream;
-        launch_prepare_queries(dq, nq, d, 2, 0, inv_norm, qhat, off, s);
+        launch_prepare_queries(dq, nq, d, 2, 0, inv_norm, qhat, off, qstat, s);
         launch_fastscan_prepare(qhat, nq, d, lut, bq, s);
         launch_fullscan_fs(ix->packed, ix->n, d, lut, bq, inv_norm, nq, (uint32_t)kk, work, ds, di, s);
         if (!dev) {
             d2h(out_scores, ds, nq * kk * 4);
             d2h(out_ids, di, nq * kk * 8);
-            sync();
         }
+        finish_status(qstat, nq, flags, kNonfiniteQuery, "queries");
     });
 }
 

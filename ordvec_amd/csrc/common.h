@@ -67,6 +67,7 @@ enum KernelFamily : int {
     kFamFastScan,        // FastScan LUT build, scan and merge
     kFamEncode,          // rank / sign / FastScan encode
     kFamPrepare,         // query normalisation
+    kFamSample,          // stage-1 sampled threshold pass
     kFamCount
 };
 void ktime_begin(int family, hipStream_t s);
@@ -85,7 +86,7 @@ float rankquant_inv_norm(uint32_t d, uint32_t bits);
 
 void launch_check_finite(const float* x, uint64_t count, uint32_t* flag, hipStream_t s);
 void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bits, int tier, float inv_norm,
-                            float* qhat, float* offset, hipStream_t s);
+                            float* qhat, float* offset, uint32_t* qstat, hipStream_t s);
 void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, uint64_t first_doc, bool interleaved,
                         hipStream_t s);
 void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, hipStream_t s);
@@ -112,11 +113,36 @@ void carve_stage1(const Stage1Plan& p, uint64_t nq, uint64_t m_eff, void* base, 
 void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t* qbits, uint64_t nq, uint64_t m_eff,
                    const Stage1Plan& p, const Stage1Work& w, uint32_t* cand, uint32_t* cand_h, hipStream_t s);
 
+// Stage 1 fast path (sampled threshold + survivor compaction); queries whose
+// survivor buffer cannot prove exactness get qstat bit 1 and are recomputed
+// by launch_stage1 (the exact histogram pipeline).
+constexpr uint32_t kFastCap = 8192;  // survivor keys per query (64 KB LDS sort)
+struct FastPlan {
+    uint32_t qt, nbins, hwords, lds_bytes;
+    uint32_t stride, nsample, sample_bps, nsseg;
+    uint32_t scan_bps, nseg;
+    uint32_t s_target;  // ~0u: admit every document
+    uint32_t cap;
+};
+struct FastWork {
+    uint32_t* hist;  // [nq][nbins]
+    uint32_t* thr;   // [nq]
+    uint32_t* cnt;   // [nq]
+    uint64_t* keys;  // [nq][cap]
+};
+FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff);
+size_t stage1_fast_bytes(const FastPlan& p, uint64_t nq);
+void carve_stage1_fast(const FastPlan& p, uint64_t nq, void* base, FastWork* w);
+void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
+                        uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
+                        uint32_t* cand_h, uint32_t* qstat, hipStream_t s);
+
 // Stage 2 / subset rerank over candidate rows.  Rows are CSR (offsets != null)
 // or uniform (row q = cands[q*row_m .. (q+1)*row_m)).  Output nq x out_k.
 void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat,
                    const float* offset, uint64_t nq, const uint64_t* offsets, const uint32_t* cands, uint64_t row_m,
-                   uint64_t n_docs, uint32_t out_k, float* out_s, int64_t* out_i, hipStream_t s);
+                   uint64_t n_docs, const uint32_t* qmap, uint32_t out_k, float* out_s, int64_t* out_i,
+                   hipStream_t s);
 
 // Full scans: per (query, segment) block top-k into partial keys, then merge.
 size_t fullscan_work_bytes(uint64_t n, uint64_t nq, uint32_t k);

@@ -121,36 +121,82 @@ __global__ void k_check_finite(const float* __restrict__ x, uint64_t count, uint
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
 }
 
-// One wave per query.  The reference sums sequentially (Rust f32 Sum folds
-// from -0.0), so lane 0 carries both running sums; all lanes write q_hat.
-__global__ __launch_bounds__(64) void k_prepare(const float* __restrict__ q, uint64_t nq, uint32_t d, int tier,
-                                                float centre, float inv_norm, float* __restrict__ qhat,
-                                                float* __restrict__ offset) {
+// Sequential f32 sum of squares / of values over an LDS chunk by one thread
+// (the reference folds left to right starting at -0.0).
+__device__ __forceinline__ float seq_sum_sq(const float* xs, uint32_t len, float s) {
+    uint32_t i = 0;
+    for (; i + 4 <= len; i += 4) {
+        float4 v = *reinterpret_cast<const float4*>(xs + i);
+        s = __fadd_rn(s, __fmul_rn(v.x, v.x));
+        s = __fadd_rn(s, __fmul_rn(v.y, v.y));
+        s = __fadd_rn(s, __fmul_rn(v.z, v.z));
+        s = __fadd_rn(s, __fmul_rn(v.w, v.w));
+    }
+    for (; i < len; i++) s = __fadd_rn(s, __fmul_rn(xs[i], xs[i]));
+    return s;
+}
+
+__device__ __forceinline__ float seq_sum(const float* xs, uint32_t len, float s) {
+    uint32_t i = 0;
+    for (; i + 4 <= len; i += 4) {
+        float4 v = *reinterpret_cast<const float4*>(xs + i);
+        s = __fadd_rn(__fadd_rn(__fadd_rn(__fadd_rn(s, v.x), v.y), v.z), v.w);
+    }
+    for (; i < len; i++) s = __fadd_rn(s, xs[i]);
+    return s;
+}
+
+// One workgroup per query: q_hat = x * (1/||x||) (zeros when ||x|| <= 1e-12)
+// and the SIMD tiers' TopK offset -centre * sum(q_hat) / R.  The two sums are
+// sequential like the reference's; chunks are staged in LDS so the single
+// summing thread reads on-chip memory.  qstat bit 0 records non-finite input.
+constexpr uint32_t kPrepChunk = 2048;
+
+__global__ __launch_bounds__(256) void k_prepare(const float* __restrict__ q, uint64_t nq, uint32_t d, int tier,
+                                                 float centre, float inv_norm, float* __restrict__ qhat,
+                                                 float* __restrict__ offset, uint32_t* __restrict__ qstat) {
+    __shared__ __align__(16) float xs[kPrepChunk];
+    __shared__ float s_inv;
+    __shared__ int s_zero;
     const uint64_t qi = blockIdx.x;
     const float* x = q + qi * d;
     float* out = qhat + qi * d;
-    __shared__ float s_inv;
-    __shared__ int s_zero;
+    float sq = -0.0f;
+    bool bad = false;
+    for (uint32_t c0 = 0; c0 < d; c0 += kPrepChunk) {
+        const uint32_t len = min(kPrepChunk, d - c0);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+            float v = x[c0 + i];
+            bad |= !isfinite(v);
+            xs[i] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) sq = seq_sum_sq(xs, len, sq);
+    }
+    const int any_bad = __syncthreads_or(bad);
     if (threadIdx.x == 0) {
-        float s = -0.0f;
-        for (uint32_t i = 0; i < d; i++) s = __fadd_rn(s, __fmul_rn(x[i], x[i]));
-        float norm = __fsqrt_rn(s);
+        float norm = __fsqrt_rn(sq);
         s_zero = norm <= 1e-12f;
         s_inv = __fdiv_rn(1.0f, norm);
+        if (qstat) qstat[qi] = any_bad ? 1u : 0u;
     }
     __syncthreads();
     const float inv = s_inv;
     const int zero = s_zero;
-    for (uint32_t i = threadIdx.x; i < d; i += 64) out[i] = zero ? 0.0f : __fmul_rn(x[i], inv);
-    if (threadIdx.x == 0) {
-        float off = 0.0f;
-        if (tier != 0) {
-            float qs = -0.0f;
-            for (uint32_t i = 0; i < d; i++) qs = __fadd_rn(qs, zero ? 0.0f : __fmul_rn(x[i], inv));
-            off = __fmul_rn(__fmul_rn(-centre, qs), inv_norm);
+    float qsum = -0.0f;
+    for (uint32_t c0 = 0; c0 < d; c0 += kPrepChunk) {
+        const uint32_t len = min(kPrepChunk, d - c0);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+            float v = zero ? 0.0f : __fmul_rn(x[c0 + i], inv);
+            out[c0 + i] = v;
+            xs[i] = v;
         }
-        offset[qi] = off;
+        __syncthreads();
+        if (threadIdx.x == 0 && tier != 0) qsum = seq_sum(xs, len, qsum);
     }
+    if (threadIdx.x == 0) offset[qi] = tier != 0 ? __fmul_rn(__fmul_rn(-centre, qsum), inv_norm) : 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -442,93 +488,337 @@ __global__ __launch_bounds__(256) void k_stage1_finalize(const uint16_t* __restr
 }
 
 // ---------------------------------------------------------------------------
-// Asymmetric scoring (src/quant_kernels.rs), one helper per reference tier.
+// Stage 1, fast path: sampled threshold + survivor compaction.
+//
+// A strided sample of the corpus gives each query a Hamming threshold T
+// expected to admit ~2.5m documents.  The full scan then appends every
+// document with hamming <= T as a key (hamming << 32 | id).  If at least m
+// keys arrived and none were dropped, the buffer holds *every* document with
+// hamming <= T, hence every member of the exact top-m by (hamming, id), and a
+// sort of the buffer yields the reference's result.  Otherwise the query is
+// flagged and the exact histogram pipeline above recomputes it.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t code_at(const uint8_t* row, uint32_t p, uint32_t bits) {
-    const uint32_t cpb = 8 / bits;
-    const uint32_t shift = (cpb - 1 - p % cpb) * bits;
-    return (row[p / cpb] >> shift) & ((1u << bits) - 1);
+// Per query: sign bits (wave ballot), zero the sample histogram and the
+// survivor counter, record non-finite input in qstat bit 0.
+__global__ __launch_bounds__(64) void k_s1_prep(const float* __restrict__ q, uint32_t d, uint64_t* __restrict__ qbits,
+                                                uint32_t* __restrict__ hist, uint32_t nbins, uint32_t* __restrict__ cnt,
+                                                uint32_t* __restrict__ qstat) {
+    const uint64_t qi = blockIdx.x;
+    const uint32_t lane = threadIdx.x, qpv = d / 64;
+    const float* x = q + qi * d;
+    bool bad = false;
+    for (uint32_t w = 0; w < qpv; w++) {
+        float v = x[w * 64 + lane];
+        bad |= !isfinite(v);
+        uint64_t mask = __ballot(v > 0.0f);
+        if (lane == 0) qbits[qi * qpv + w] = mask;
+    }
+    for (uint32_t i = lane; i < nbins; i += 64) hist[qi * nbins + i] = 0;
+    bool any_bad = __any(bad);
+    if (lane == 0) {
+        cnt[qi] = 0;
+        if (qstat) qstat[qi] = any_bad ? 1u : 0u;
+    }
 }
 
-// Scalar LUT tier (src/quant_kernels.rs:89-160): acc += q[p]*centre(code) in
-// coordinate order, one thread per document.
-__device__ __forceinline__ float score_scalar(const uint8_t* row, uint32_t d, uint32_t bits, const float* qv,
-                                              float half_span) {
+// Hamming histogram of the sample blocks (every `stride`-th 64-doc block),
+// LDS packed 16-bit bins per query, merged into hist with global atomics.
+template <int QT, int WCH>
+__global__ __launch_bounds__(256) void k_s1_sample(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
+                                                   uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
+                                                   uint32_t stride, uint32_t nsample, uint32_t blocks_per_seg,
+                                                   uint32_t* __restrict__ hist, uint32_t nbins, uint32_t hwords) {
+    extern __shared__ uint32_t lds_hist[];
+    const uint32_t q0 = blockIdx.x * QT;
+    const uint32_t tq = min((uint32_t)QT, nq - q0);
+    for (uint32_t i = threadIdx.x; i < QT * hwords; i += blockDim.x) lds_hist[i] = 0;
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t j0 = blockIdx.y * blocks_per_seg, j1 = min(nsample, j0 + blocks_per_seg);
+    const uint64_t* qb = qbits + (uint64_t)q0 * qpv;
+    for (uint32_t j = j0 + wave; j < j1; j += 4) {
+        const uint32_t b = j * stride;
+        if (b >= nblk) break;
+        uint32_t acc[QT];
+#pragma unroll
+        for (int i = 0; i < QT; i++) acc[i] = 0;
+        const uint64_t* base = words + (uint64_t)b * qpv * 64 + lane;
+        for (uint32_t w0 = 0; w0 < qpv; w0 += WCH) {
+            uint64_t dw[WCH];
+#pragma unroll
+            for (int jj = 0; jj < WCH; jj++) dw[jj] = base[(uint64_t)(w0 + jj) * 64];
+#pragma unroll
+            for (int i = 0; i < QT; i++) {
+                if (i < (int)tq) {
+                    const uint64_t* qp = qb + (uint64_t)i * qpv + w0;
+#pragma unroll
+                    for (int jj = 0; jj < WCH; jj++) acc[i] += (uint32_t)__popcll(dw[jj] ^ qp[jj]);
+                }
+            }
+        }
+        if ((uint64_t)b * 64 + lane < n) {
+#pragma unroll
+            for (int i = 0; i < QT; i++)
+                if (i < (int)tq) atomicAdd(&lds_hist[i * hwords + (acc[i] >> 1)], 1u << ((acc[i] & 1) * 16));
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tq * nbins; i += blockDim.x) {
+        uint32_t qi = i / nbins, bin = i - qi * nbins;
+        uint32_t w = lds_hist[qi * hwords + (bin >> 1)];
+        uint32_t c = (bin & 1) ? (w >> 16) : (w & 0xffffu);
+        if (c) atomicAdd(&hist[(uint64_t)(q0 + qi) * nbins + bin], c);
+    }
+}
+
+// T = smallest h with sample count(hamming <= h) >= s_target, or the largest
+// Hamming value (admit everything) when s_target is ~0u or unreachable.
+__global__ __launch_bounds__(64) void k_s1_threshold(const uint32_t* __restrict__ hist, uint32_t nbins,
+                                                     uint32_t s_target, uint32_t* __restrict__ thr) {
+    const uint64_t q = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    if (s_target == 0xffffffffu) {
+        if (lane == 0) thr[q] = nbins - 1;
+        return;
+    }
+    const uint32_t* h = hist + q * nbins;
+    const uint32_t per = (nbins + 63) / 64;
+    const uint32_t lo = min(nbins, lane * per), hi = min(nbins, lo + per);
+    uint32_t local = 0;
+    for (uint32_t b = lo; b < hi; b++) local += h[b];
+    uint32_t incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t v = __shfl_up(incl, off);
+        if ((int)lane >= off) incl += v;
+    }
+    const uint32_t excl = incl - local;
+    const bool here = excl < s_target && incl >= s_target;
+    if (here) {
+        uint32_t cum = excl;
+        for (uint32_t b = lo; b < hi; b++) {
+            cum += h[b];
+            if (cum >= s_target) {
+                thr[q] = b;
+                break;
+            }
+        }
+    }
+    if (__ballot(here) == 0 && lane == 0) thr[q] = nbins - 1;
+}
+
+// Full scan: every document with hamming <= T[q] is appended (wave-aggregated
+// atomics) to the query's key buffer; counts past `cap` are counted, not stored.
+template <int QT, int WCH>
+__global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
+                                                 uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
+                                                 uint32_t blocks_per_seg, const uint32_t* __restrict__ thr, uint32_t cap,
+                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ cnt) {
+    const uint32_t q0 = blockIdx.x * QT;
+    const uint32_t tq = min((uint32_t)QT, nq - q0);
+    uint32_t T[QT];
+#pragma unroll
+    for (int i = 0; i < QT; i++) T[i] = i < (int)tq ? thr[q0 + i] : 0u;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t b_begin = blockIdx.y * blocks_per_seg;
+    const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
+    const uint64_t* qb = qbits + (uint64_t)q0 * qpv;
+    for (uint32_t b = b_begin + wave; b < b_end; b += 4) {
+        uint32_t acc[QT];
+#pragma unroll
+        for (int i = 0; i < QT; i++) acc[i] = 0;
+        const uint64_t* base = words + (uint64_t)b * qpv * 64 + lane;
+        for (uint32_t w0 = 0; w0 < qpv; w0 += WCH) {
+            uint64_t dw[WCH];
+#pragma unroll
+            for (int jj = 0; jj < WCH; jj++) dw[jj] = base[(uint64_t)(w0 + jj) * 64];
+#pragma unroll
+            for (int i = 0; i < QT; i++) {
+                if (i < (int)tq) {
+                    const uint64_t* qp = qb + (uint64_t)i * qpv + w0;
+#pragma unroll
+                    for (int jj = 0; jj < WCH; jj++) acc[i] += (uint32_t)__popcll(dw[jj] ^ qp[jj]);
+                }
+            }
+        }
+        const uint64_t doc = (uint64_t)b * 64 + lane;
+        const bool valid = doc < n;
+#pragma unroll
+        for (int i = 0; i < QT; i++) {
+            if (i < (int)tq) {
+                const bool pass = valid && acc[i] <= T[i];
+                const uint64_t mask = __ballot(pass);
+                if (mask) {
+                    const int leader = __ffsll((long long)mask) - 1;
+                    uint32_t basepos = 0;
+                    if ((int)lane == leader) basepos = atomicAdd(&cnt[q0 + i], (uint32_t)__popcll(mask));
+                    basepos = __shfl(basepos, leader);
+                    const uint32_t pos = basepos + (uint32_t)__popcll(mask & lt_mask);
+                    if (pass && pos < cap) keys[(uint64_t)(q0 + i) * cap + pos] = ((uint64_t)acc[i] << 32) | doc;
+                }
+            }
+        }
+    }
+}
+
+// Sort each query's survivors by (hamming, id) and keep the first m; flag the
+// query (qstat bit 1) when the buffer cannot prove exactness.
+__global__ __launch_bounds__(256) void k_s1_select(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ cnt,
+                                                   uint32_t cap, uint64_t m_eff, uint32_t* __restrict__ cand,
+                                                   uint32_t* __restrict__ cand_h, uint32_t* __restrict__ qstat) {
+    extern __shared__ uint64_t sk[];
+    const uint64_t q = blockIdx.x;
+    const uint32_t count = cnt[q];
+    if (count < m_eff || count > cap) {
+        if (threadIdx.x == 0) qstat[q] |= 2u;
+        return;
+    }
+    const uint32_t P = next_pow2(count);
+    const uint64_t* kq = keys + q * cap;
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) sk[i] = i < count ? kq[i] : kKeyMax;
+    __syncthreads();
+    bitonic_sort(sk, P);
+    for (uint32_t j = threadIdx.x; j < m_eff; j += blockDim.x) {
+        cand[q * m_eff + j] = (uint32_t)(sk[j] & 0xffffffffu);
+        if (cand_h) cand_h[q * m_eff + j] = (uint32_t)(sk[j] >> 32);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Asymmetric scoring (src/quant_kernels.rs): one thread per document, with the
+// reference tier's accumulator layout held in registers.  A SIMD lane l of
+// accumulator i in the reference becomes register acc[16*i + l]; since chunk c
+// of 16 codes feeds accumulator c % 4, coordinate p always lands in
+// acc[p % 64] (AVX-512), acc[p % 16] (AVX2 b=2) or acc[p % 8] (AVX2 b=4).
+// q_hat sits in LDS and every lane reads the same coordinate at the same
+// time (broadcast reads).
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kQInLds = 8192;  // largest dim whose q_hat is staged in LDS
+
+template <int BITS>
+__device__ __forceinline__ uint32_t code_of(const uint32_t* w, int j) {
+    constexpr int CPB = 8 / BITS;
+    const int byte_idx = j / CPB;
+    const int shift = (byte_idx & 3) * 8 + (CPB - 1 - j % CPB) * BITS;
+    return (w[byte_idx >> 2] >> shift) & ((1u << BITS) - 1);
+}
+
+// Scalar LUT tier (src/quant_kernels.rs:89-160): acc += q[p] * centre(code)
+// in coordinate order; returns acc.
+template <int BITS>
+__device__ __forceinline__ float raw_scalar(const uint8_t* __restrict__ row, uint32_t d, const float* qs) {
+    constexpr uint32_t CPB = 8 / BITS, MASK = (1u << BITS) - 1;
+    const float half = ((float)(1u << BITS) - 1.0f) / 2.0f;
+    const uint32_t bpv = d / CPB;
     float acc = 0.0f;
-    const uint32_t cpb = 8 / bits, bpv = d / cpb, mask = (1u << bits) - 1;
-    for (uint32_t g = 0; g < bpv; g++) {
-        uint32_t byte = row[g];
-        for (uint32_t j = 0; j < cpb; j++) {
-            uint32_t code = (byte >> ((cpb - 1 - j) * bits)) & mask;
-            float c = __fadd_rn((float)code, -half_span);
-            acc = __fadd_rn(acc, __fmul_rn(qv[g * cpb + j], c));
+    uint32_t g = 0;
+    if ((bpv & 3) == 0) {
+        for (; g < bpv; g += 4) {
+            uint32_t w = *reinterpret_cast<const uint32_t*>(row + g);
+#pragma unroll
+            for (int j = 0; j < (int)(4 * CPB); j++) {
+                float c = __fadd_rn((float)code_of<BITS>(&w, j), -half);
+                acc = __fadd_rn(acc, __fmul_rn(qs[g * CPB + j], c));
+            }
+        }
+    } else {
+        for (; g < bpv; g++) {
+            uint32_t byte = row[g];
+#pragma unroll
+            for (int j = 0; j < (int)CPB; j++) {
+                uint32_t code = (byte >> ((CPB - 1 - j) * BITS)) & MASK;
+                acc = __fadd_rn(acc, __fmul_rn(qs[g * CPB + j], __fadd_rn((float)code, -half)));
+            }
         }
     }
     return acc;
 }
 
-// AVX-512 tier (src/quant_kernels.rs:406-581), one wave per document: lane
-// L = 16*i + l is accumulator i (chunks c = i mod 4), SIMD lane l; then the
-// (acc0+acc1)+(acc2+acc3) combine and the 8/4/2/1 halving tree.  Result is
-// valid in lane 0.
-__device__ __forceinline__ float score_avx512_wave(const uint8_t* row, uint32_t d, uint32_t bits, const float* qv) {
-    const uint32_t L = threadIdx.x & 63, i = L >> 4, l = L & 15;
-    float acc = 0.0f;
-    const uint32_t outer = d / 64;
-    for (uint32_t t = 0; t < outer; t++) {
-        uint32_t p = (i + 4 * t) * 16 + l;
-        acc = __builtin_fmaf((float)code_at(row, p, bits), qv[p], acc);
+// AVX-512 tier (src/quant_kernels.rs:406-581), d % 64 == 0.
+template <int BITS>
+__device__ __forceinline__ float raw_avx512(const uint8_t* __restrict__ row, uint32_t d, const float* qs) {
+    constexpr int NW = 2 * BITS;  // dwords per 64 codes
+    float a[64];
+#pragma unroll
+    for (int i = 0; i < 64; i++) a[i] = 0.0f;
+    const uint4* r4 = reinterpret_cast<const uint4*>(row);
+    for (uint32_t c = 0; c < d / 64; c++) {
+        uint32_t w[NW];
+#pragma unroll
+        for (int v = 0; v < NW / 4; v++) {
+            uint4 x = r4[c * (NW / 4) + v];
+            w[4 * v] = x.x;
+            w[4 * v + 1] = x.y;
+            w[4 * v + 2] = x.z;
+            w[4 * v + 3] = x.w;
+        }
+        const float* qc = qs + c * 64;
+#pragma unroll
+        for (int j = 0; j < 64; j++) a[j] = __builtin_fmaf((float)code_of<BITS>(w, j), qc[j], a[j]);
     }
-    float a1 = __shfl(acc, (int)(L + 16) & 63);
-    float a2 = __shfl(acc, (int)(L + 32) & 63);
-    float a3 = __shfl(acc, (int)(L + 48) & 63);
-    float t16 = __fadd_rn(__fadd_rn(acc, a1), __fadd_rn(a2, a3));
-    float u = __fadd_rn(t16, __shfl(t16, (int)(L + 8) & 63));
-    float w = __fadd_rn(u, __shfl(u, (int)(L + 4) & 63));
-    float x = __fadd_rn(w, __shfl(w, (int)(L + 2) & 63));
-    return __fadd_rn(x, __shfl(x, (int)(L + 1) & 63));
+    float t[16];
+#pragma unroll
+    for (int l = 0; l < 16; l++) t[l] = __fadd_rn(__fadd_rn(a[l], a[16 + l]), __fadd_rn(a[32 + l], a[48 + l]));
+#pragma unroll
+    for (int j = 0; j < 8; j++) t[j] = __fadd_rn(t[j], t[j + 8]);
+#pragma unroll
+    for (int j = 0; j < 4; j++) t[j] = __fadd_rn(t[j], t[j + 4]);
+#pragma unroll
+    for (int j = 0; j < 2; j++) t[j] = __fadd_rn(t[j], t[j + 2]);
+    return __fadd_rn(t[0], t[1]);
 }
 
-// AVX2 tier (src/quant_kernels.rs:226-389), one wave per document using the
-// low 16 (b=2) or 8 (b=4) lanes, then horizontal_sum_avx2's order.
-__device__ __forceinline__ float score_avx2_wave(const uint8_t* row, uint32_t d, uint32_t bits, const float* qv) {
-    const uint32_t L = threadIdx.x & 63;
-    float acc = 0.0f;
-    if (bits == 2) {
-        if (L < 16)
-            for (uint32_t c = 0; c < d / 16; c++) {
-                uint32_t p = c * 16 + L;
-                acc = __builtin_fmaf((float)code_at(row, p, bits), qv[p], acc);
-            }
-        float t8 = __fadd_rn(acc, __shfl(acc, (int)(L + 8) & 63));
-        acc = t8;
-    } else {
-        if (L < 8)
-            for (uint32_t c = 0; c < d / 8; c++) {
-                uint32_t p = c * 8 + L;
-                acc = __builtin_fmaf((float)code_at(row, p, bits), qv[p], acc);
-            }
+// AVX2 tier (src/quant_kernels.rs:226-389): b=2 keeps 16 lanes (acc_hi codes
+// 0-7, acc_lo codes 8-15 of each 4-byte chunk), b=4 keeps 8 lanes; then
+// horizontal_sum_avx2's order.
+template <int BITS>
+__device__ __forceinline__ float raw_avx2(const uint8_t* __restrict__ row, uint32_t d, const float* qs) {
+    constexpr int W = BITS == 2 ? 16 : 8;  // codes per 4-byte chunk
+    float a[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) a[i] = 0.0f;
+    const uint32_t* r32 = reinterpret_cast<const uint32_t*>(row);
+    for (uint32_t c = 0; c < d / W; c++) {
+        uint32_t w = r32[c];
+        const float* qc = qs + c * W;
+#pragma unroll
+        for (int j = 0; j < W; j++) a[j] = __builtin_fmaf((float)code_of<BITS>(&w, j), qc[j], a[j]);
     }
-    float s = __fadd_rn(acc, __shfl(acc, (int)(L + 4) & 63));
-    float s1 = __shfl(s, (int)(L + 1) & 63);
-    float s2 = __shfl(s, (int)(L + 2) & 63);
-    float s3 = __shfl(s, (int)(L + 3) & 63);
-    return __fadd_rn(__fadd_rn(s, s1), __fadd_rn(s2, s3));
+    float t[8];
+#pragma unroll
+    for (int l = 0; l < 8; l++) t[l] = W == 16 ? __fadd_rn(a[l], a[l + 8]) : a[l];
+    float s0 = __fadd_rn(t[0], t[4]), s1 = __fadd_rn(t[1], t[5]), s2 = __fadd_rn(t[2], t[6]), s3 = __fadd_rn(t[3], t[7]);
+    return __fadd_rn(__fadd_rn(s0, s1), __fadd_rn(s2, s3));
 }
 
-constexpr uint32_t kQInLds = 8192;  // q_hat staged in LDS up to this dim
+// Score exactly as the reference's TopK sees it: raw * (1/R) + offset.
+template <int TIER, int BITS>
+__device__ __forceinline__ float doc_score(const uint8_t* row, uint32_t d, const float* qs, float inv_norm, float off) {
+    float raw;
+    if (TIER == 2) raw = raw_avx512<BITS>(row, d, qs);
+    else if (TIER == 1) raw = raw_avx2<BITS>(row, d, qs);
+    else raw = raw_scalar<BITS>(row, d, qs);
+    return __fadd_rn(__fmul_rn(raw, inv_norm), off);
+}
 
-// Stage 2 / subset rerank: one workgroup per query row.
-template <int TIER>
-__global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ codes, uint32_t d, uint32_t bits,
-                                                float inv_norm, const float* __restrict__ qhat,
-                                                const float* __restrict__ offset, const uint64_t* __restrict__ offsets,
-                                                const uint32_t* __restrict__ cands, uint64_t row_m, uint64_t n_docs,
-                                                uint32_t out_k, float* __restrict__ out_s, int64_t* __restrict__ out_i) {
+__device__ __forceinline__ void stage_q(float* qs, const float* qg, uint32_t d) {
+    for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) qs[i] = qg[i];
+}
+
+// Stage 2 / subset rerank (src/quant.rs:1126-1281): one 1024-thread workgroup
+// per query row, one candidate per thread per round; tie key = global id.
+template <int TIER, int BITS>
+__global__ __launch_bounds__(1024) void k_rerank(const uint8_t* __restrict__ codes, uint32_t d, float inv_norm,
+                                                 const float* __restrict__ qhat, const float* __restrict__ offset,
+                                                 const uint64_t* __restrict__ offsets,
+                                                 const uint32_t* __restrict__ cands, uint64_t row_m, uint64_t n_docs,
+                                                 const uint32_t* __restrict__ qmap, uint32_t out_k,
+                                                 float* __restrict__ out_s, int64_t* __restrict__ out_i) {
     __shared__ TopKShared tk;
     __shared__ float qs[kQInLds];
-    const uint64_t q = blockIdx.x;
+    const uint64_t q = qmap ? qmap[blockIdx.x] : blockIdx.x;
     const uint64_t b = offsets ? offsets[q] : q * row_m;
     const uint64_t e = offsets ? offsets[q + 1] : (q + 1) * row_m;
     const uint64_t m = e - b;
@@ -541,101 +831,48 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
         }
         return;
     }
-    const float* qg = qhat + q * d;
-    const float* qv = qg;
-    if (d <= kQInLds) {
-        for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) qs[i] = qg[i];
-        qv = qs;
-    }
+    stage_q(qs, qhat + q * d, d);
     const float off = offset[q];
-    const uint32_t bpv = d * bits / 8;
-    const float half_span = ((float)(1u << bits) - 1.0f) / 2.0f;
+    const uint32_t bpv = d * BITS / 8;
     tk_init(tk);
-    const uint32_t kk = out_k;
-    if (TIER == 0) {
-        for (uint64_t r0 = 0; r0 < m; r0 += blockDim.x) {
-            tk_reserve(tk, kk, blockDim.x);
-            uint64_t j = r0 + threadIdx.x;
-            uint32_t id = j < m ? cands[b + j] : 0u;
-            if (j < m && id < n_docs) {
-                float s = __fadd_rn(__fmul_rn(score_scalar(codes + (uint64_t)id * bpv, d, bits, qv, half_span), inv_norm), off);
-                tk_offer(tk, score_key(s, id));
-            }
-        }
-    } else {
-        const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        constexpr uint32_t PER_WAVE = 8;
-        for (uint64_t r0 = 0; r0 < m; r0 += (uint64_t)nw * PER_WAVE) {
-            tk_reserve(tk, kk, nw * PER_WAVE);
-            for (uint32_t u = 0; u < PER_WAVE; u++) {
-                uint64_t j = r0 + (uint64_t)wave * PER_WAVE + u;
-                uint32_t id = j < m ? cands[b + j] : 0u;
-                if (j < m && id < n_docs) {
-                    const uint8_t* row = codes + (uint64_t)id * bpv;
-                    float raw = TIER == 2 ? score_avx512_wave(row, d, bits, qv) : score_avx2_wave(row, d, bits, qv);
-                    if ((threadIdx.x & 63) == 0) tk_offer(tk, score_key(__fadd_rn(__fmul_rn(raw, inv_norm), off), id));
-                }
-            }
+    for (uint64_t r0 = 0; r0 < m; r0 += blockDim.x) {
+        tk_reserve(tk, out_k, blockDim.x);
+        const uint64_t j = r0 + threadIdx.x;
+        const uint32_t id = j < m ? cands[b + j] : 0u;
+        if (j < m && id < n_docs) {
+            float s = doc_score<TIER, BITS>(codes + (uint64_t)id * bpv, d, qs, inv_norm, off);
+            tk_offer(tk, score_key(s, id));
         }
     }
     __syncthreads();
-    tk_flush(tk, kk);
+    tk_flush(tk, out_k);
     for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
         uint64_t key = tk.keys[j];
-        if (key == kKeyMax) {
-            os[j] = -INFINITY;
-            oi[j] = -1;
-        } else {
-            os[j] = key_score(key);
-            oi[j] = (int64_t)(key & 0xffffffffu);
-        }
+        os[j] = key == kKeyMax ? -INFINITY : key_score(key);
+        oi[j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
     }
 }
 
-// Full asymmetric scan (src/quant.rs:713-847): grid (segments, queries), each
-// workgroup keeps its segment's top-k; k_merge_topk combines segments.
-template <int TIER>
+// Full asymmetric scan (src/quant.rs:713-847): grid (segments, queries), one
+// document per thread per round; each workgroup keeps its segment's top-k and
+// k_merge_topk combines the segments.
+template <int TIER, int BITS>
 __global__ __launch_bounds__(256) void k_fullscan_rq(const uint8_t* __restrict__ codes, uint64_t n, uint32_t d,
-                                                     uint32_t bits, float inv_norm, const float* __restrict__ qhat,
+                                                     float inv_norm, const float* __restrict__ qhat,
                                                      const float* __restrict__ offset, uint64_t per_seg, uint32_t k,
                                                      uint64_t* __restrict__ partial) {
     __shared__ TopKShared tk;
     __shared__ float qs[kQInLds];
     const uint64_t q = blockIdx.y, seg = blockIdx.x, nseg = gridDim.x;
     const uint64_t begin = seg * per_seg, end = min(n, begin + per_seg);
-    const float* qg = qhat + q * d;
-    const float* qv = qg;
-    if (d <= kQInLds) {
-        for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) qs[i] = qg[i];
-        qv = qs;
-    }
+    stage_q(qs, qhat + q * d, d);
     const float off = offset[q];
-    const uint32_t bpv = d * bits / 8;
-    const float half_span = ((float)(1u << bits) - 1.0f) / 2.0f;
+    const uint32_t bpv = d * BITS / 8;
     tk_init(tk);
-    if (TIER == 0) {
-        for (uint64_t r0 = begin; r0 < end; r0 += blockDim.x) {
-            tk_reserve(tk, k, blockDim.x);
-            uint64_t di = r0 + threadIdx.x;
-            if (di < end) {
-                float s = __fadd_rn(__fmul_rn(score_scalar(codes + di * bpv, d, bits, qv, half_span), inv_norm), off);
-                tk_offer(tk, score_key(s, (uint32_t)di));
-            }
-        }
-    } else {
-        const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        constexpr uint32_t PER_WAVE = 8;
-        for (uint64_t r0 = begin; r0 < end; r0 += (uint64_t)nw * PER_WAVE) {
-            tk_reserve(tk, k, nw * PER_WAVE);
-            for (uint32_t u = 0; u < PER_WAVE; u++) {
-                uint64_t di = r0 + (uint64_t)wave * PER_WAVE + u;
-                if (di < end) {
-                    const uint8_t* row = codes + di * bpv;
-                    float raw = TIER == 2 ? score_avx512_wave(row, d, bits, qv) : score_avx2_wave(row, d, bits, qv);
-                    if ((threadIdx.x & 63) == 0) tk_offer(tk, score_key(__fadd_rn(__fmul_rn(raw, inv_norm), off), (uint32_t)di));
-                }
-            }
-        }
+    for (uint64_t r0 = begin; r0 < end; r0 += blockDim.x) {
+        tk_reserve(tk, k, blockDim.x);
+        const uint64_t di = r0 + threadIdx.x;
+        if (di < end) tk_offer(tk, score_key(doc_score<TIER, BITS>(codes + di * bpv, d, qs, inv_norm, off), (uint32_t)di));
     }
     __syncthreads();
     tk_flush(tk, k);
@@ -775,11 +1012,11 @@ void launch_check_finite(const float* x, uint64_t count, uint32_t* flag, hipStre
 }
 
 void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bits, int tier, float inv_norm,
-                            float* qhat, float* offset, hipStream_t s) {
+                            float* qhat, float* offset, uint32_t* qstat, hipStream_t s) {
     if (nq == 0) return;
     float centre = ((float)(1u << bits) - 1.0f) / 2.0f;
     ktime_begin(kFamPrepare, s);
-    hipLaunchKernelGGL(k_prepare, dim3((uint32_t)nq), dim3(64), 0, s, q, nq, d, tier, centre, inv_norm, qhat, offset);
+    hipLaunchKernelGGL(k_prepare, dim3((uint32_t)nq), dim3(256), 0, s, q, nq, d, tier, centre, inv_norm, qhat, offset, qstat);
     check_launch();
     ktime_end(kFamPrepare, s);
 }
@@ -906,21 +1143,137 @@ void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t
     ktime_end(kFamStage1Select, s);
 }
 
+FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
+    FastPlan p{};
+    p.nbins = d + 1;
+    p.hwords = (p.nbins + 1) / 2;
+    uint32_t qt = nq <= 1 ? 1 : (nq <= 8 ? 4 : 16);
+    while (qt > 1 && (size_t)qt * p.hwords * 4 > 65536) qt = qt == 16 ? 4 : 1;
+    p.qt = qt;
+    p.lds_bytes = qt * p.hwords * 4;
+    p.cap = kFastCap;
+    const uint64_t nblk = (n + 63) / 64;
+    const uint64_t tiles = (nq + qt - 1) / qt;
+    if (n <= p.cap) {
+        p.s_target = 0xffffffffu;
+        p.stride = 1;
+        p.nsample = 0;
+    } else {
+        uint64_t stride = std::min<uint64_t>(32, std::max<uint64_t>(1, nblk / 512));
+        p.stride = (uint32_t)stride;
+        p.nsample = (uint32_t)((nblk + stride - 1) / stride);
+        if (stride == 1) {
+            p.s_target = (uint32_t)m_eff;  // the sample is the corpus: exact threshold
+        } else {
+            double sample_docs = std::min<double>((double)n, (double)p.nsample * 64.0);
+            double target_full = std::min(2.5 * (double)m_eff, 0.5 * (double)(m_eff + p.cap));
+            p.s_target = (uint32_t)std::ceil(target_full * sample_docs / (double)n) + 8;
+        }
+        uint64_t want = std::max<uint64_t>(1, (1024 + tiles - 1) / tiles);
+        uint64_t nsseg = std::min<uint64_t>(std::max<uint64_t>(1, p.nsample), want);
+        uint64_t sbps = (p.nsample + nsseg - 1) / nsseg;
+        if (sbps > 1023) sbps = 1023;
+        if (sbps == 0) sbps = 1;
+        p.sample_bps = (uint32_t)sbps;
+        p.nsseg = (uint32_t)std::max<uint64_t>(1, (p.nsample + sbps - 1) / sbps);
+    }
+    uint64_t want = std::max<uint64_t>(1, (2048 + tiles - 1) / tiles);
+    uint64_t nseg = std::min<uint64_t>(std::max<uint64_t>(1, nblk), want);
+    uint64_t bps = std::max<uint64_t>(1, (nblk + nseg - 1) / nseg);
+    p.scan_bps = (uint32_t)bps;
+    p.nseg = (uint32_t)std::max<uint64_t>(1, (nblk + bps - 1) / bps);
+    return p;
+}
+
+size_t stage1_fast_bytes(const FastPlan& p, uint64_t nq) {
+    return align256(nq * (uint64_t)p.nbins * 4) + 2 * align256(nq * 4) + align256(nq * (uint64_t)p.cap * 8);
+}
+
+void carve_stage1_fast(const FastPlan& p, uint64_t nq, void* base, FastWork* w) {
+    char* c = static_cast<char*>(base);
+    w->hist = reinterpret_cast<uint32_t*>(c);
+    c += align256(nq * (uint64_t)p.nbins * 4);
+    w->thr = reinterpret_cast<uint32_t*>(c);
+    c += align256(nq * 4);
+    w->cnt = reinterpret_cast<uint32_t*>(c);
+    c += align256(nq * 4);
+    w->keys = reinterpret_cast<uint64_t*>(c);
+}
+
+template <int QT, int WCH>
+static void launch_fast_qt(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
+                           const FastPlan& p, const FastWork& w, hipStream_t s) {
+    const uint32_t nblk = (uint32_t)((n + 63) / 64);
+    const uint32_t tiles = (uint32_t)((nq + QT - 1) / QT);
+    if (p.s_target != 0xffffffffu) {
+        ktime_begin(kFamSample, s);
+        hipLaunchKernelGGL((k_s1_sample<QT, WCH>), dim3(tiles, p.nsseg), dim3(256), p.lds_bytes, s, words, nblk, qpv,
+                           n, qbits, (uint32_t)nq, p.stride, p.nsample, p.sample_bps, w.hist, p.nbins, p.hwords);
+        check_launch();
+        ktime_end(kFamSample, s);
+    }
+    hipLaunchKernelGGL(k_s1_threshold, dim3((uint32_t)nq), dim3(64), 0, s, w.hist, p.nbins, p.s_target, w.thr);
+    check_launch();
+    ktime_begin(kFamSignScan, s);
+    hipLaunchKernelGGL((k_s1_scan<QT, WCH>), dim3(tiles, p.nseg), dim3(256), 0, s, words, nblk, qpv, n, qbits,
+                       (uint32_t)nq, p.scan_bps, w.thr, p.cap, w.keys, w.cnt);
+    check_launch();
+    ktime_end(kFamSignScan, s);
+}
+
+template <int QT>
+static void launch_fast_wch(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
+                            const FastPlan& p, const FastWork& w, hipStream_t s) {
+    if (qpv % 4 == 0) launch_fast_qt<QT, 4>(words, n, qpv, qbits, nq, p, w, s);
+    else if (qpv % 2 == 0) launch_fast_qt<QT, 2>(words, n, qpv, qbits, nq, p, w, s);
+    else launch_fast_qt<QT, 1>(words, n, qpv, qbits, nq, p, w, s);
+}
+
+void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
+                        uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
+                        uint32_t* cand_h, uint32_t* qstat, hipStream_t s) {
+    if (nq == 0) return;
+    const uint32_t qpv = d / 64;
+    ktime_begin(kFamStage1Select, s);
+    hipLaunchKernelGGL(k_s1_prep, dim3((uint32_t)nq), dim3(64), 0, s, queries, d, qbits, w.hist, p.nbins, w.cnt, qstat);
+    check_launch();
+    ktime_end(kFamStage1Select, s);
+    if (m_eff == 0) return;
+    if (p.qt == 16) launch_fast_wch<16>(words, n, qpv, qbits, nq, p, w, s);
+    else if (p.qt == 4) launch_fast_wch<4>(words, n, qpv, qbits, nq, p, w, s);
+    else launch_fast_wch<1>(words, n, qpv, qbits, nq, p, w, s);
+    ktime_begin(kFamStage1Select, s);
+    uint32_t P = 1;
+    while (P < p.cap) P <<= 1;
+    hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(256), (size_t)P * 8, s, w.keys, w.cnt, p.cap, m_eff, cand,
+                       cand_h, qstat);
+    check_launch();
+    ktime_end(kFamStage1Select, s);
+}
+
+#define OVX_TIER_BITS_DISPATCH(KERNEL, ...)                                      \
+    do {                                                                        \
+        if (tier == 2 && bits == 2) KERNEL(2, 2, __VA_ARGS__);                  \
+        else if (tier == 2 && bits == 4) KERNEL(2, 4, __VA_ARGS__);             \
+        else if (tier == 1 && bits == 2) KERNEL(1, 2, __VA_ARGS__);             \
+        else if (tier == 1 && bits == 4) KERNEL(1, 4, __VA_ARGS__);             \
+        else if (bits == 1) KERNEL(0, 1, __VA_ARGS__);                          \
+        else if (bits == 2) KERNEL(0, 2, __VA_ARGS__);                          \
+        else KERNEL(0, 4, __VA_ARGS__);                                         \
+    } while (0)
+
 void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat,
                    const float* offset, uint64_t nq, const uint64_t* offsets, const uint32_t* cands, uint64_t row_m,
-                   uint64_t n_docs, uint32_t out_k, float* out_s, int64_t* out_i, hipStream_t s) {
+                   uint64_t n_docs, const uint32_t* qmap, uint32_t out_k, float* out_s, int64_t* out_i,
+                   hipStream_t s) {
     if (nq == 0 || out_k == 0) return;
     dim3 grid((uint32_t)nq);
     ktime_begin(kFamRerank, s);
-    if (tier == 2)
-        hipLaunchKernelGGL(k_rerank<2>, grid, dim3(256), 0, s, codes, d, bits, inv_norm, qhat, offset, offsets, cands,
-                           row_m, n_docs, out_k, out_s, out_i);
-    else if (tier == 1)
-        hipLaunchKernelGGL(k_rerank<1>, grid, dim3(256), 0, s, codes, d, bits, inv_norm, qhat, offset, offsets, cands,
-                           row_m, n_docs, out_k, out_s, out_i);
-    else
-        hipLaunchKernelGGL(k_rerank<0>, grid, dim3(256), 0, s, codes, d, bits, inv_norm, qhat, offset, offsets, cands,
-                           row_m, n_docs, out_k, out_s, out_i);
+#define OVX_RERANK(T, B, _)                                                                                    \
+    hipLaunchKernelGGL((k_rerank<T, B>), grid, dim3(1024), 0, s, codes, d, inv_norm, qhat, offset, offsets, cands, \
+                       row_m, n_docs, qmap, out_k, out_s, out_i)
+    OVX_TIER_BITS_DISPATCH(OVX_RERANK, 0);
+#undef OVX_RERANK
     check_launch();
     ktime_end(kFamRerank, s);
 }
@@ -944,15 +1297,11 @@ void launch_fullscan_rq(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t b
     uint64_t* partial = static_cast<uint64_t*>(work);
     dim3 grid(nseg, (uint32_t)nq);
     ktime_begin(kFamFullScan, s);
-    if (tier == 2)
-        hipLaunchKernelGGL(k_fullscan_rq<2>, grid, dim3(256), 0, s, codes, n, d, bits, inv_norm, qhat, offset, per, k,
-                           partial);
-    else if (tier == 1)
-        hipLaunchKernelGGL(k_fullscan_rq<1>, grid, dim3(256), 0, s, codes, n, d, bits, inv_norm, qhat, offset, per, k,
-                           partial);
-    else
-        hipLaunchKernelGGL(k_fullscan_rq<0>, grid, dim3(256), 0, s, codes, n, d, bits, inv_norm, qhat, offset, per, k,
-                           partial);
+#define OVX_FULLSCAN(T, B, _)                                                                                   \
+    hipLaunchKernelGGL((k_fullscan_rq<T, B>), grid, dim3(256), 0, s, codes, n, d, inv_norm, qhat, offset, per, k, \
+                       partial)
+    OVX_TIER_BITS_DISPATCH(OVX_FULLSCAN, 0);
+#undef OVX_FULLSCAN
     check_launch();
     hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, out_s, out_i);
     check_launch();
