@@ -383,19 +383,25 @@ void full_search_dev(const RqIndex* ix, const float* dq, uint64_t nq, uint64_t k
     launch_fullscan_rq(ix->codes, ix->n, d, ix->bits, tier, inv_norm, qhat, off, nq, (uint32_t)kk, work, ds, di, s);
 }
 
-size_t subset_search_bytes(const RqIndex* ix, uint64_t nq) { return al(nq * ix->dim * 4) + al(nq * 4) * 2; }
+size_t subset_search_bytes(const RqIndex* ix, uint64_t nq, uint64_t out_k, uint64_t avg_m) {
+    return al(nq * ix->dim * 4) + al(nq * 4) * 2 +
+           rerank_work_bytes(nq, (uint32_t)out_k, rerank_nsplit(nq, avg_m));
+}
 
 void subset_search_dev(const RqIndex* ix, const float* dq, uint64_t nq, const uint64_t* doff, const uint32_t* dc,
-                       uint64_t row_m, uint64_t out_k, float* ds, int64_t* di, uint32_t* qstat, Carver& cv) {
+                       uint64_t row_m, uint64_t avg_m, uint64_t out_k, float* ds, int64_t* di, uint32_t* qstat,
+                       Carver& cv) {
     const uint32_t d = ix->dim;
     float* qhat = cv.take<float>(nq * d);
     float* off = cv.take<float>(nq);
+    const uint32_t nsplit = rerank_nsplit(nq, avg_m);
+    void* work = cv.take<char>(rerank_work_bytes(nq, (uint32_t)out_k, nsplit));
     const int tier = select_tier(d, ix->bits);
     const float inv_norm = rankquant_inv_norm(d, ix->bits);
     hipStream_t s = eng().stream;
     launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, qstat, s);
     launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, doff, dc, row_m, ix->n, nullptr,
-                  (uint32_t)out_k, ds, di, s);
+                  (uint32_t)out_k, ds, di, nsplit, work, s);
 }
 
 struct Stage1Bufs {
@@ -480,7 +486,8 @@ void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, cons
     check_search_dim(ix->dim);
     host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-    size_t bytes = subset_search_bytes(ix, nq) + al(nq * 4) +
+    const uint64_t avg_m = offsets ? (n_cands + nq - 1) / nq : row_m;
+    size_t bytes = subset_search_bytes(ix, nq, out_k, avg_m) + al(nq * 4) +
                    (dev ? 0 : al(nq * ix->dim * 4) + al((nq + 1) * 8) + al(n_cands * 4) + al(nq * out_k * 4) +
                                   al(nq * out_k * 8));
     Carver cv{static_cast<char*>(eng().workspace(bytes))};
@@ -502,7 +509,7 @@ void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, cons
         ds = cv.take<float>(nq * out_k);
         di = cv.take<int64_t>(nq * out_k);
     }
-    subset_search_dev(ix, dq, nq, doff, dc, row_m, out_k, ds, di, qstat, cv);
+    subset_search_dev(ix, dq, nq, doff, dc, row_m, avg_m, out_k, ds, di, qstat, cv);
     if (!dev) {
         d2h(out_s, ds, nq * out_k * 4);
         d2h(out_i, di, nq * out_k * 8);
@@ -1521,8 +1528,10 @@ ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const 
         LOCK;
         const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
         const uint32_t d = ix->dim;
+        const uint32_t nsplit = rerank_nsplit(nq, m_eff);
         size_t bytes = al(nq * d * 4) + 2 * al(nq * 4) + al(nq * m_eff * 4) + stage1_bytes(sb, nq, m_eff) +
-                       (dev ? 0 : al(nq * d * 4) + al(nq * out_k * 4) + al(nq * out_k * 8)) + al(nq * 4);
+                       (dev ? 0 : al(nq * d * 4) + al(nq * out_k * 4) + al(nq * out_k * 8)) + al(nq * 4) +
+                       rerank_work_bytes(nq, (uint32_t)out_k, nsplit);
         Carver cv{static_cast<char*>(eng().workspace(bytes))};
         const float* dq = stage_queries(cv, queries, nq * d, flags);
         float* qhat = cv.take<float>(nq * d);
@@ -1530,6 +1539,7 @@ ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const 
         uint32_t* qstat = cv.take<uint32_t>(nq);
         uint32_t* qmap = cv.take<uint32_t>(nq);
         uint32_t* cand = cv.take<uint32_t>(nq * m_eff);
+        void* rwork = cv.take<char>(rerank_work_bytes(nq, (uint32_t)out_k, nsplit));
         float* ds = dev ? out_scores : cv.take<float>(nq * out_k);
         int64_t* di = dev ? out_ids : cv.take<int64_t>(nq * out_k);
         const int tier = select_tier(d, ix->bits);
@@ -1538,7 +1548,7 @@ ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const 
         launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, nullptr, s);
         Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv);
         launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, nullptr,
-                      (uint32_t)out_k, ds, di, s);
+                      (uint32_t)out_k, ds, di, nsplit, rwork, s);
         if (!dev) {
             d2h(out_scores, ds, nq * out_k * 4);
             d2h(out_ids, di, nq * out_k * 8);
@@ -1551,7 +1561,7 @@ ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const 
             stage1_fix(sb, b.qbits, fl, m_eff, cand, nullptr);
             h2d(qmap, fl.data(), fl.size() * 4);
             launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, fl.size(), nullptr, cand, m_eff, ix->n,
-                          qmap, (uint32_t)out_k, ds, di, s);
+                          qmap, (uint32_t)out_k, ds, di, 1, nullptr, s);
             if (!dev)
                 for (uint32_t q : fl) {
                     d2h(out_scores + (uint64_t)q * out_k, ds + (uint64_t)q * out_k, out_k * 4);

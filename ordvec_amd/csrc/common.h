@@ -120,7 +120,7 @@ constexpr uint32_t kFastCap = 8192;  // survivor keys per query (64 KB LDS sort)
 struct FastPlan {
     uint32_t qt, nbins, hwords, lds_bytes;
     uint32_t stride, nsample, sample_bps, nsseg;
-    uint32_t scan_bps, nseg;
+    uint32_t scan_qt, scan_bps, nseg;
     uint32_t s_target;  // ~0u: admit every document
     uint32_t cap;
 };
@@ -139,10 +139,14 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
 
 // Stage 2 / subset rerank over candidate rows.  Rows are CSR (offsets != null)
 // or uniform (row q = cands[q*row_m .. (q+1)*row_m)).  Output nq x out_k.
+// Small batches split each row across `nsplit` workgroups (partial top-k in
+// `work`, rerank_work_bytes) and merge; rerank_nsplit picks the split.
+uint32_t rerank_nsplit(uint64_t nq, uint64_t avg_m);
+size_t rerank_work_bytes(uint64_t nq, uint32_t out_k, uint32_t nsplit);
 void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat,
                    const float* offset, uint64_t nq, const uint64_t* offsets, const uint32_t* cands, uint64_t row_m,
                    uint64_t n_docs, const uint32_t* qmap, uint32_t out_k, float* out_s, int64_t* out_i,
-                   hipStream_t s);
+                   uint32_t nsplit, void* work, hipStream_t s);
 
 // Full scans: per (query, segment) block top-k into partial keys, then merge.
 size_t fullscan_work_bytes(uint64_t n, uint64_t nq, uint32_t k);

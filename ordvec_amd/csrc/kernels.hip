@@ -125,12 +125,17 @@ __global__ void k_check_finite(const float* __restrict__ x, uint64_t count, uint
 // (the reference folds left to right starting at -0.0).
 __device__ __forceinline__ float seq_sum_sq(const float* xs, uint32_t len, float s) {
     uint32_t i = 0;
-    for (; i + 4 <= len; i += 4) {
-        float4 v = *reinterpret_cast<const float4*>(xs + i);
-        s = __fadd_rn(s, __fmul_rn(v.x, v.x));
-        s = __fadd_rn(s, __fmul_rn(v.y, v.y));
-        s = __fadd_rn(s, __fmul_rn(v.z, v.z));
-        s = __fadd_rn(s, __fmul_rn(v.w, v.w));
+    for (; i + 32 <= len; i += 32) {
+        float4 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = *reinterpret_cast<const float4*>(xs + i + 4 * j);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            s = __fadd_rn(s, __fmul_rn(v[j].x, v[j].x));
+            s = __fadd_rn(s, __fmul_rn(v[j].y, v[j].y));
+            s = __fadd_rn(s, __fmul_rn(v[j].z, v[j].z));
+            s = __fadd_rn(s, __fmul_rn(v[j].w, v[j].w));
+        }
     }
     for (; i < len; i++) s = __fadd_rn(s, __fmul_rn(xs[i], xs[i]));
     return s;
@@ -138,9 +143,12 @@ __device__ __forceinline__ float seq_sum_sq(const float* xs, uint32_t len, float
 
 __device__ __forceinline__ float seq_sum(const float* xs, uint32_t len, float s) {
     uint32_t i = 0;
-    for (; i + 4 <= len; i += 4) {
-        float4 v = *reinterpret_cast<const float4*>(xs + i);
-        s = __fadd_rn(__fadd_rn(__fadd_rn(__fadd_rn(s, v.x), v.y), v.z), v.w);
+    for (; i + 32 <= len; i += 32) {
+        float4 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = *reinterpret_cast<const float4*>(xs + i + 4 * j);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s = __fadd_rn(__fadd_rn(__fadd_rn(__fadd_rn(s, v[j].x), v[j].y), v[j].z), v[j].w);
     }
     for (; i < len; i++) s = __fadd_rn(s, xs[i]);
     return s;
@@ -609,74 +617,135 @@ __global__ __launch_bounds__(64) void k_s1_threshold(const uint32_t* __restrict_
 
 // Full scan: every document with hamming <= T[q] is appended (wave-aggregated
 // atomics) to the query's key buffer; counts past `cap` are counted, not stored.
-template <int QT, int WCH>
+// Each wave holds one 64-doc block's sign words in VGPRs (WCH = words per
+// doc, 512-byte coalesced loads) and loops over the tile's queries, whose
+// words are wave-uniform scalar loads: 2*WCH VALU XOR + 2*WCH popcount per
+// query and doc.  WCH == 0 is the generic path for word counts without a
+// specialisation (doc words re-read per query from L1).
+template <int WCH>
+__device__ __forceinline__ uint32_t hamming_regs(const uint64_t* dw, const uint64_t* __restrict__ qp) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < WCH; j++) acc += (uint32_t)__popcll(dw[j] ^ qp[j]);
+    return acc;
+}
+
+__device__ __forceinline__ void s1_append(bool pass, uint32_t acc, uint64_t doc, uint32_t lane, uint64_t lt_mask,
+                                          uint32_t* cnt_q, uint64_t* keys_q, uint32_t cap) {
+    const uint64_t mask = __ballot(pass);
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        uint32_t basepos = 0;
+        if ((int)lane == leader) basepos = atomicAdd(cnt_q, (uint32_t)__popcll(mask));
+        basepos = __shfl(basepos, leader);
+        const uint32_t pos = basepos + (uint32_t)__popcll(mask & lt_mask);
+        if (pass && pos < cap) keys_q[pos] = ((uint64_t)acc << 32) | doc;
+    }
+}
+
+template <int WCH>
 __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
                                                  uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
-                                                 uint32_t blocks_per_seg, const uint32_t* __restrict__ thr, uint32_t cap,
-                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ cnt) {
-    const uint32_t q0 = blockIdx.x * QT;
-    const uint32_t tq = min((uint32_t)QT, nq - q0);
-    uint32_t T[QT];
-#pragma unroll
-    for (int i = 0; i < QT; i++) T[i] = i < (int)tq ? thr[q0 + i] : 0u;
+                                                 uint32_t qt, uint32_t blocks_per_seg, const uint32_t* __restrict__ thr,
+                                                 uint32_t cap, uint64_t* __restrict__ keys, uint32_t* __restrict__ cnt) {
+    const uint32_t q0 = blockIdx.x * qt;
+    const uint32_t tq = min(qt, nq - q0);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t b_begin = blockIdx.y * blocks_per_seg;
     const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
     const uint64_t* qb = qbits + (uint64_t)q0 * qpv;
+    const uint32_t* tb = thr + q0;
+    uint32_t* cb = cnt + q0;
+    uint64_t* kb = keys + (uint64_t)q0 * cap;
     for (uint32_t b = b_begin + wave; b < b_end; b += 4) {
-        uint32_t acc[QT];
-#pragma unroll
-        for (int i = 0; i < QT; i++) acc[i] = 0;
         const uint64_t* base = words + (uint64_t)b * qpv * 64 + lane;
-        for (uint32_t w0 = 0; w0 < qpv; w0 += WCH) {
-            uint64_t dw[WCH];
-#pragma unroll
-            for (int jj = 0; jj < WCH; jj++) dw[jj] = base[(uint64_t)(w0 + jj) * 64];
-#pragma unroll
-            for (int i = 0; i < QT; i++) {
-                if (i < (int)tq) {
-                    const uint64_t* qp = qb + (uint64_t)i * qpv + w0;
-#pragma unroll
-                    for (int jj = 0; jj < WCH; jj++) acc[i] += (uint32_t)__popcll(dw[jj] ^ qp[jj]);
-                }
-            }
-        }
         const uint64_t doc = (uint64_t)b * 64 + lane;
         const bool valid = doc < n;
+        if (WCH > 0) {
+            uint64_t dw[WCH > 0 ? WCH : 1];
 #pragma unroll
-        for (int i = 0; i < QT; i++) {
-            if (i < (int)tq) {
-                const bool pass = valid && acc[i] <= T[i];
-                const uint64_t mask = __ballot(pass);
-                if (mask) {
-                    const int leader = __ffsll((long long)mask) - 1;
-                    uint32_t basepos = 0;
-                    if ((int)lane == leader) basepos = atomicAdd(&cnt[q0 + i], (uint32_t)__popcll(mask));
-                    basepos = __shfl(basepos, leader);
-                    const uint32_t pos = basepos + (uint32_t)__popcll(mask & lt_mask);
-                    if (pass && pos < cap) keys[(uint64_t)(q0 + i) * cap + pos] = ((uint64_t)acc[i] << 32) | doc;
-                }
+            for (int j = 0; j < WCH; j++) dw[j] = base[(uint64_t)j * 64];
+            uint32_t i = 0;
+            // Two queries per step: their scalar loads share one wait.
+            for (; i + 1 < tq; i += 2) {
+                const uint32_t a0 = hamming_regs<WCH>(dw, qb + (uint64_t)i * WCH);
+                const uint32_t a1 = hamming_regs<WCH>(dw, qb + (uint64_t)(i + 1) * WCH);
+                const uint32_t t0 = tb[i], t1 = tb[i + 1];
+                s1_append(valid && a0 <= t0, a0, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
+                s1_append(valid && a1 <= t1, a1, doc, lane, lt_mask, cb + i + 1, kb + (uint64_t)(i + 1) * cap, cap);
+            }
+            if (i < tq) {
+                const uint32_t a0 = hamming_regs<WCH>(dw, qb + (uint64_t)i * WCH);
+                s1_append(valid && a0 <= tb[i], a0, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
+            }
+        } else {
+            for (uint32_t i = 0; i < tq; i++) {
+                const uint64_t* qp = qb + (uint64_t)i * qpv;
+                uint32_t acc = 0;
+                for (uint32_t j = 0; j < qpv; j++) acc += (uint32_t)__popcll(base[(uint64_t)j * 64] ^ qp[j]);
+                s1_append(valid && acc <= tb[i], acc, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
             }
         }
     }
 }
 
 // Sort each query's survivors by (hamming, id) and keep the first m; flag the
-// query (qstat bit 1) when the buffer cannot prove exactness.
-__global__ __launch_bounds__(256) void k_s1_select(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ cnt,
-                                                   uint32_t cap, uint64_t m_eff, uint32_t* __restrict__ cand,
-                                                   uint32_t* __restrict__ cand_h, uint32_t* __restrict__ qstat) {
-    extern __shared__ uint64_t sk[];
+// query (qstat bit 1) when the buffer cannot prove exactness.  A histogram of
+// min(hamming, kSelBins-1) first narrows the sort to the keys at or below the
+// bin that reaches m (a superset of the answer that keeps every tie).
+constexpr uint32_t kSelBins = 2048, kSelBuf = 4096;
+
+__global__ __launch_bounds__(1024) void k_s1_select(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ cnt,
+                                                    uint32_t cap, uint64_t m_eff, uint32_t* __restrict__ cand,
+                                                    uint32_t* __restrict__ cand_h, uint32_t* __restrict__ qstat) {
+    __shared__ uint32_t hist[kSelBins];
+    __shared__ uint64_t sk[kSelBuf];
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t s_cut, s_n;
     const uint64_t q = blockIdx.x;
     const uint32_t count = cnt[q];
     if (count < m_eff || count > cap) {
         if (threadIdx.x == 0) qstat[q] |= 2u;
         return;
     }
-    const uint32_t P = next_pow2(count);
     const uint64_t* kq = keys + q * cap;
-    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) sk[i] = i < count ? kq[i] : kKeyMax;
+    for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x)
+        atomicAdd(&hist[min((uint32_t)(kq[i] >> 32), kSelBins - 1)], 1u);
+    __syncthreads();
+    // cut = smallest bin with cumulative count >= m (2 bins per thread).
+    const uint32_t lo = threadIdx.x * 2;
+    const uint32_t local = hist[lo] + hist[lo + 1];
+    part[threadIdx.x] = local;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t incl = part[threadIdx.x], excl = incl - local;
+    if (excl < m_eff && incl >= m_eff) s_cut = (excl + hist[lo] >= m_eff) ? lo : lo + 1;
+    __syncthreads();
+    const uint32_t cut = s_cut;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+        uint64_t k = kq[i];
+        if (min((uint32_t)(k >> 32), kSelBins - 1) <= cut) {
+            uint32_t pos = atomicAdd(&s_n, 1u);
+            if (pos < kSelBuf) sk[pos] = k;
+        }
+    }
+    __syncthreads();
+    const uint32_t nsel = s_n;
+    if (nsel > kSelBuf) {
+        if (threadIdx.x == 0) qstat[q] |= 2u;
+        return;
+    }
+    const uint32_t P = next_pow2(nsel);
+    for (uint32_t i = nsel + threadIdx.x; i < P; i += blockDim.x) sk[i] = kKeyMax;
     __syncthreads();
     bitonic_sort(sk, P);
     for (uint32_t j = threadIdx.x; j < m_eff; j += blockDim.x) {
@@ -807,49 +876,55 @@ __device__ __forceinline__ void stage_q(float* qs, const float* qg, uint32_t d) 
     for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) qs[i] = qg[i];
 }
 
-// Stage 2 / subset rerank (src/quant.rs:1126-1281): one 1024-thread workgroup
-// per query row, one candidate per thread per round; tie key = global id.
+// Stage 2 / subset rerank (src/quant.rs:1126-1281): grid (queries, splits);
+// each workgroup scores one slice of a query's candidate row, one candidate
+// per thread per round, tie key = global id.  With one split the workgroup
+// writes the final row; otherwise it writes its slice's top-k keys to
+// `partial` and k_merge_topk finishes the row.  `qmap` (optional) maps grid
+// rows to query rows.  q_hat is staged in dynamic LDS (d floats).
 template <int TIER, int BITS>
-__global__ __launch_bounds__(1024) void k_rerank(const uint8_t* __restrict__ codes, uint32_t d, float inv_norm,
-                                                 const float* __restrict__ qhat, const float* __restrict__ offset,
-                                                 const uint64_t* __restrict__ offsets,
-                                                 const uint32_t* __restrict__ cands, uint64_t row_m, uint64_t n_docs,
-                                                 const uint32_t* __restrict__ qmap, uint32_t out_k,
-                                                 float* __restrict__ out_s, int64_t* __restrict__ out_i) {
+__global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ codes, uint32_t d, float inv_norm,
+                                                const float* __restrict__ qhat, const float* __restrict__ offset,
+                                                const uint64_t* __restrict__ offsets,
+                                                const uint32_t* __restrict__ cands, uint64_t row_m, uint64_t n_docs,
+                                                const uint32_t* __restrict__ qmap, uint32_t out_k,
+                                                float* __restrict__ out_s, int64_t* __restrict__ out_i,
+                                                uint64_t* __restrict__ partial) {
     __shared__ TopKShared tk;
-    __shared__ float qs[kQInLds];
-    const uint64_t q = qmap ? qmap[blockIdx.x] : blockIdx.x;
+    extern __shared__ float qs[];
+    const uint64_t qidx = blockIdx.x;
+    const uint64_t q = qmap ? qmap[qidx] : qidx;
+    const uint32_t split = blockIdx.y, nsplit = gridDim.y;
     const uint64_t b = offsets ? offsets[q] : q * row_m;
     const uint64_t e = offsets ? offsets[q + 1] : (q + 1) * row_m;
-    const uint64_t m = e - b;
-    float* os = out_s + q * out_k;
-    int64_t* oi = out_i + q * out_k;
-    if (m == 0) {
-        for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
-            os[j] = -INFINITY;
-            oi[j] = -1;
-        }
-        return;
-    }
+    const uint64_t per = (e - b + nsplit - 1) / nsplit;
+    const uint64_t sb = min(e, b + split * per), se = min(e, sb + per);
     stage_q(qs, qhat + q * d, d);
     const float off = offset[q];
     const uint32_t bpv = d * BITS / 8;
     tk_init(tk);
-    for (uint64_t r0 = 0; r0 < m; r0 += blockDim.x) {
+    for (uint64_t r0 = sb; r0 < se; r0 += blockDim.x) {
         tk_reserve(tk, out_k, blockDim.x);
         const uint64_t j = r0 + threadIdx.x;
-        const uint32_t id = j < m ? cands[b + j] : 0u;
-        if (j < m && id < n_docs) {
+        const uint32_t id = j < se ? cands[j] : 0u;
+        if (j < se && id < n_docs) {
             float s = doc_score<TIER, BITS>(codes + (uint64_t)id * bpv, d, qs, inv_norm, off);
             tk_offer(tk, score_key(s, id));
         }
     }
     __syncthreads();
     tk_flush(tk, out_k);
-    for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
-        uint64_t key = tk.keys[j];
-        os[j] = key == kKeyMax ? -INFINITY : key_score(key);
-        oi[j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
+    if (nsplit == 1) {
+        float* os = out_s + q * out_k;
+        int64_t* oi = out_i + q * out_k;
+        for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
+            uint64_t key = tk.keys[j];
+            os[j] = key == kKeyMax ? -INFINITY : key_score(key);
+            oi[j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
+        }
+    } else {
+        uint64_t* pp = partial + (qidx * nsplit + split) * out_k;
+        for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) pp[j] = tk.keys[j];
     }
 }
 
@@ -881,11 +956,13 @@ __global__ __launch_bounds__(256) void k_fullscan_rq(const uint8_t* __restrict__
 }
 
 __global__ __launch_bounds__(256) void k_merge_topk(const uint64_t* __restrict__ partial, uint32_t nseg, uint32_t k,
-                                                    float* __restrict__ out_s, int64_t* __restrict__ out_i) {
+                                                    const uint32_t* __restrict__ qmap, float* __restrict__ out_s,
+                                                    int64_t* __restrict__ out_i) {
     __shared__ TopKShared tk;
-    const uint64_t q = blockIdx.x;
+    const uint64_t qidx = blockIdx.x;
+    const uint64_t q = qmap ? qmap[qidx] : qidx;
     const uint64_t total = (uint64_t)nseg * k;
-    const uint64_t* in = partial + q * total;
+    const uint64_t* in = partial + qidx * total;
     tk_init(tk);
     for (uint64_t r0 = 0; r0 < total; r0 += blockDim.x) {
         tk_reserve(tk, k, blockDim.x);
@@ -1177,7 +1254,9 @@ FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
         p.sample_bps = (uint32_t)sbps;
         p.nsseg = (uint32_t)std::max<uint64_t>(1, (p.nsample + sbps - 1) / sbps);
     }
-    uint64_t want = std::max<uint64_t>(1, (2048 + tiles - 1) / tiles);
+    p.scan_qt = nq >= 256 ? 64 : (nq >= 32 ? 16 : (nq >= 8 ? 4 : 1));
+    const uint64_t scan_tiles = (nq + p.scan_qt - 1) / p.scan_qt;
+    uint64_t want = std::max<uint64_t>(1, (2048 + scan_tiles - 1) / scan_tiles);
     uint64_t nseg = std::min<uint64_t>(std::max<uint64_t>(1, nblk), want);
     uint64_t bps = std::max<uint64_t>(1, (nblk + nseg - 1) / nseg);
     p.scan_bps = (uint32_t)bps;
@@ -1201,32 +1280,42 @@ void carve_stage1_fast(const FastPlan& p, uint64_t nq, void* base, FastWork* w) 
 }
 
 template <int QT, int WCH>
-static void launch_fast_qt(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
-                           const FastPlan& p, const FastWork& w, hipStream_t s) {
+static void launch_sample_qt(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
+                             const FastPlan& p, const FastWork& w, hipStream_t s) {
     const uint32_t nblk = (uint32_t)((n + 63) / 64);
     const uint32_t tiles = (uint32_t)((nq + QT - 1) / QT);
-    if (p.s_target != 0xffffffffu) {
-        ktime_begin(kFamSample, s);
-        hipLaunchKernelGGL((k_s1_sample<QT, WCH>), dim3(tiles, p.nsseg), dim3(256), p.lds_bytes, s, words, nblk, qpv,
-                           n, qbits, (uint32_t)nq, p.stride, p.nsample, p.sample_bps, w.hist, p.nbins, p.hwords);
-        check_launch();
-        ktime_end(kFamSample, s);
-    }
-    hipLaunchKernelGGL(k_s1_threshold, dim3((uint32_t)nq), dim3(64), 0, s, w.hist, p.nbins, p.s_target, w.thr);
+    hipLaunchKernelGGL((k_s1_sample<QT, WCH>), dim3(tiles, p.nsseg), dim3(256), p.lds_bytes, s, words, nblk, qpv, n,
+                       qbits, (uint32_t)nq, p.stride, p.nsample, p.sample_bps, w.hist, p.nbins, p.hwords);
     check_launch();
-    ktime_begin(kFamSignScan, s);
-    hipLaunchKernelGGL((k_s1_scan<QT, WCH>), dim3(tiles, p.nseg), dim3(256), 0, s, words, nblk, qpv, n, qbits,
-                       (uint32_t)nq, p.scan_bps, w.thr, p.cap, w.keys, w.cnt);
-    check_launch();
-    ktime_end(kFamSignScan, s);
 }
 
 template <int QT>
-static void launch_fast_wch(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
-                            const FastPlan& p, const FastWork& w, hipStream_t s) {
-    if (qpv % 4 == 0) launch_fast_qt<QT, 4>(words, n, qpv, qbits, nq, p, w, s);
-    else if (qpv % 2 == 0) launch_fast_qt<QT, 2>(words, n, qpv, qbits, nq, p, w, s);
-    else launch_fast_qt<QT, 1>(words, n, qpv, qbits, nq, p, w, s);
+static void launch_sample_wch(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
+                              const FastPlan& p, const FastWork& w, hipStream_t s) {
+    if (qpv % 4 == 0) launch_sample_qt<QT, 4>(words, n, qpv, qbits, nq, p, w, s);
+    else if (qpv % 2 == 0) launch_sample_qt<QT, 2>(words, n, qpv, qbits, nq, p, w, s);
+    else launch_sample_qt<QT, 1>(words, n, qpv, qbits, nq, p, w, s);
+}
+
+static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
+                        const FastPlan& p, const FastWork& w, hipStream_t s) {
+    const uint32_t nblk = (uint32_t)((n + 63) / 64);
+    const uint32_t tiles = (uint32_t)((nq + p.scan_qt - 1) / p.scan_qt);
+    dim3 grid(tiles, p.nseg);
+#define OVX_SCAN(W)                                                                                            \
+    hipLaunchKernelGGL((k_s1_scan<W>), grid, dim3(256), 0, s, words, nblk, qpv, n, qbits, (uint32_t)nq, p.scan_qt, \
+                       p.scan_bps, w.thr, p.cap, w.keys, w.cnt)
+    switch (qpv) {
+        case 16: OVX_SCAN(16); break;
+        case 12: OVX_SCAN(12); break;
+        case 8: OVX_SCAN(8); break;
+        case 4: OVX_SCAN(4); break;
+        case 2: OVX_SCAN(2); break;
+        case 1: OVX_SCAN(1); break;
+        default: OVX_SCAN(0); break;
+    }
+#undef OVX_SCAN
+    check_launch();
 }
 
 void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
@@ -1239,14 +1328,21 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
     check_launch();
     ktime_end(kFamStage1Select, s);
     if (m_eff == 0) return;
-    if (p.qt == 16) launch_fast_wch<16>(words, n, qpv, qbits, nq, p, w, s);
-    else if (p.qt == 4) launch_fast_wch<4>(words, n, qpv, qbits, nq, p, w, s);
-    else launch_fast_wch<1>(words, n, qpv, qbits, nq, p, w, s);
+    if (p.s_target != 0xffffffffu) {
+        ktime_begin(kFamSample, s);
+        if (p.qt == 16) launch_sample_wch<16>(words, n, qpv, qbits, nq, p, w, s);
+        else if (p.qt == 4) launch_sample_wch<4>(words, n, qpv, qbits, nq, p, w, s);
+        else launch_sample_wch<1>(words, n, qpv, qbits, nq, p, w, s);
+        ktime_end(kFamSample, s);
+    }
+    hipLaunchKernelGGL(k_s1_threshold, dim3((uint32_t)nq), dim3(64), 0, s, w.hist, p.nbins, p.s_target, w.thr);
+    check_launch();
+    ktime_begin(kFamSignScan, s);
+    launch_scan(words, n, qpv, qbits, nq, p, w, s);
+    ktime_end(kFamSignScan, s);
     ktime_begin(kFamStage1Select, s);
-    uint32_t P = 1;
-    while (P < p.cap) P <<= 1;
-    hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(256), (size_t)P * 8, s, w.keys, w.cnt, p.cap, m_eff, cand,
-                       cand_h, qstat);
+    hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(1024), 0, s, w.keys, w.cnt, p.cap, m_eff, cand, cand_h,
+                       qstat);
     check_launch();
     ktime_end(kFamStage1Select, s);
 }
@@ -1262,19 +1358,38 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
         else KERNEL(0, 4, __VA_ARGS__);                                         \
     } while (0)
 
+uint32_t rerank_nsplit(uint64_t nq, uint64_t avg_m) {
+    if (nq >= 128) return 1;
+    uint64_t want = (avg_m + 255) / 256;
+    uint64_t cap = std::max<uint64_t>(1, 512 / std::max<uint64_t>(1, nq));
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, cap, 32}));
+}
+
+size_t rerank_work_bytes(uint64_t nq, uint32_t out_k, uint32_t nsplit) {
+    return nsplit > 1 ? align256(nq * (uint64_t)nsplit * out_k * 8) : 0;
+}
+
 void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat,
                    const float* offset, uint64_t nq, const uint64_t* offsets, const uint32_t* cands, uint64_t row_m,
                    uint64_t n_docs, const uint32_t* qmap, uint32_t out_k, float* out_s, int64_t* out_i,
-                   hipStream_t s) {
+                   uint32_t nsplit, void* work, hipStream_t s) {
     if (nq == 0 || out_k == 0) return;
-    dim3 grid((uint32_t)nq);
+    if (nsplit < 1) nsplit = 1;
+    dim3 grid((uint32_t)nq, nsplit);
+    uint64_t* partial = static_cast<uint64_t*>(work);
+    const size_t lds = (size_t)d * 4;
     ktime_begin(kFamRerank, s);
-#define OVX_RERANK(T, B, _)                                                                                    \
-    hipLaunchKernelGGL((k_rerank<T, B>), grid, dim3(1024), 0, s, codes, d, inv_norm, qhat, offset, offsets, cands, \
-                       row_m, n_docs, qmap, out_k, out_s, out_i)
+#define OVX_RERANK(T, B, _)                                                                                       \
+    hipLaunchKernelGGL((k_rerank<T, B>), grid, dim3(256), lds, s, codes, d, inv_norm, qhat, offset, offsets, cands, \
+                       row_m, n_docs, qmap, out_k, out_s, out_i, partial)
     OVX_TIER_BITS_DISPATCH(OVX_RERANK, 0);
 #undef OVX_RERANK
     check_launch();
+    if (nsplit > 1) {
+        hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nsplit, out_k, qmap, out_s,
+                           out_i);
+        check_launch();
+    }
     ktime_end(kFamRerank, s);
 }
 
@@ -1303,7 +1418,7 @@ void launch_fullscan_rq(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t b
     OVX_TIER_BITS_DISPATCH(OVX_FULLSCAN, 0);
 #undef OVX_FULLSCAN
     check_launch();
-    hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, out_s, out_i);
+    hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, nullptr, out_s, out_i);
     check_launch();
     ktime_end(kFamFullScan, s);
 }
@@ -1330,7 +1445,7 @@ void launch_fullscan_fs(const uint8_t* packed, uint64_t n, uint32_t d, const uin
     hipLaunchKernelGGL(k_fullscan_fs, dim3(nseg, (uint32_t)nq), dim3(256), lds, s, packed, n, d, lut, bias_invq,
                        inv_norm, bps, k, partial);
     check_launch();
-    hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, out_s, out_i);
+    hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, nullptr, out_s, out_i);
     check_launch();
     ktime_end(kFamFastScan, s);
 }

@@ -256,3 +256,46 @@ def test_device_tensor_path(engine, oracle):
     assert np.array_equal(i.cpu().numpy(), hi)
     assert np.array_equal(s.cpu().numpy(), hs)
     assert np.array_equal(rq.codes(), oracle.rq_encode(docs, 2))
+
+
+def test_two_stage_large_batch_and_mixed_fallback(engine, oracle):
+    """>=128 queries (single-split rerank) and a batch where only some
+    queries overflow the survivor buffer (exact recomputation of those rows)."""
+    rng = np.random.default_rng(17)
+    dim = 256
+    gen = S.MixtureNP(dim, seed=19)
+    docs = gen.docs(40_000)
+    dup = docs[123].copy()
+    docs[1000:21000] = dup  # 20k identical docs: queries near them tie massively
+    qs, _ = gen.queries(docs, 200)
+    qs[::7] = dup + 0.001 * rng.standard_normal((len(qs[::7]), dim)).astype(np.float32)
+    sb = engine.SignBitmap(dim)
+    sb.add(docs)
+    rq = engine.RankQuant(dim, 2)
+    rq.add(docs)
+    s, i = engine.two_stage_search(sb, rq, qs, 700, 10)
+    bm, qb = oracle.sign_encode(docs), oracle.sign_encode(qs)
+    cand = oracle.sign_top_m(bm, qb, 700)
+    assert np.array_equal(sb.top_m_candidates_batched(qs, 700), cand)
+    offs = np.arange(len(qs) + 1, dtype=np.uint64) * 700
+    ws, wi = oracle.subset_batched(oracle.rq_encode(docs, 2), dim, 2, qs, offs, cand.reshape(-1), 10)
+    assert np.array_equal(i, wi)
+    _assert_scores(s, ws)
+
+
+def test_nonfinite_device_query_is_rejected(engine):
+    import torch
+
+    sb = engine.SignBitmap(64)
+    rq = engine.RankQuant(64, 2)
+    x = S.iid_unit(500, 64, 0)
+    sb.add(x)
+    rq.add(x)
+    q = torch.from_numpy(x[:3].copy()).cuda()
+    q[1, 5] = float("nan")
+    with pytest.raises(ValueError):
+        engine.two_stage_search(sb, rq, q, 50, 5)
+    with pytest.raises(ValueError):
+        rq.search_asymmetric(q, 5)
+    with pytest.raises(ValueError):
+        engine.two_stage_search(sb, rq, x[:3] * np.array([1, np.inf, 1], np.float32)[:, None], 50, 5)
