@@ -66,10 +66,11 @@ __device__ void bitonic_sort(T* a, uint32_t P) {
 
 // ---------------------------------------------------------------------------
 // Block-level streaming top-k on 64-bit keys (ascending = better).
-// keys[0, KMAX) hold the best seen so far (sorted after each flush); offers
-// that beat the current k-th key are appended to keys[KMAX, CAP).
+// keys[0, k) hold the best seen so far (sorted after each flush, padded with
+// kKeyMax); offers that beat the current k-th key are appended at
+// keys[k + pos].  A flush sorts only the next power of two >= k + buffered.
 // ---------------------------------------------------------------------------
-constexpr uint32_t TK_CAP = 2048, TK_KMAX = kTopKMax, TK_BUF = TK_CAP - TK_KMAX;
+constexpr uint32_t TK_CAP = 2048;
 
 struct TopKShared {
     uint64_t keys[TK_CAP];
@@ -77,8 +78,8 @@ struct TopKShared {
     uint32_t nbuf;
 };
 
-__device__ void tk_init(TopKShared& s) {
-    for (uint32_t i = threadIdx.x; i < TK_CAP; i += blockDim.x) s.keys[i] = kKeyMax;
+__device__ void tk_init(TopKShared& s, uint32_t k) {
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) s.keys[i] = kKeyMax;
     if (threadIdx.x == 0) {
         s.thresh = kKeyMax;
         s.nbuf = 0;
@@ -87,8 +88,11 @@ __device__ void tk_init(TopKShared& s) {
 }
 
 __device__ void tk_flush(TopKShared& s, uint32_t k) {
-    bitonic_sort(s.keys, TK_CAP);
-    for (uint32_t i = TK_KMAX + threadIdx.x; i < TK_CAP; i += blockDim.x) s.keys[i] = kKeyMax;
+    const uint32_t total = k + s.nbuf;
+    const uint32_t P = next_pow2(total);
+    for (uint32_t i = total + threadIdx.x; i < P; i += blockDim.x) s.keys[i] = kKeyMax;
+    __syncthreads();
+    bitonic_sort(s.keys, P);
     if (threadIdx.x == 0) {
         s.nbuf = 0;
         s.thresh = s.keys[k - 1];
@@ -99,13 +103,13 @@ __device__ void tk_flush(TopKShared& s, uint32_t k) {
 // Called by all threads between rounds: make room for up to `r` offers.
 __device__ void tk_reserve(TopKShared& s, uint32_t k, uint32_t r) {
     __syncthreads();
-    if (s.nbuf + r > TK_BUF) tk_flush(s, k);
+    if (k + s.nbuf + r > TK_CAP) tk_flush(s, k);
 }
 
-__device__ __forceinline__ void tk_offer(TopKShared& s, uint64_t key) {
+__device__ __forceinline__ void tk_offer(TopKShared& s, uint32_t k, uint64_t key) {
     if (key < s.thresh) {
         uint32_t pos = atomicAdd(&s.nbuf, 1u);
-        s.keys[TK_KMAX + pos] = key;
+        s.keys[k + pos] = key;
     }
 }
 
@@ -839,6 +843,44 @@ __device__ __forceinline__ float raw_avx512(const uint8_t* __restrict__ row, uin
     return __fadd_rn(t[0], t[1]);
 }
 
+// AVX-512 tier on a quad of lanes: lane j of the quad owns the reference's
+// accumulator j (chunks c = j mod 4, 16 SIMD lanes each in registers), so
+// (acc0+acc1)+(acc2+acc3) is two quad shuffles and every lane of the quad
+// ends with the same raw score.
+template <int BITS>
+__device__ __forceinline__ float raw_avx512_quad(const uint8_t* __restrict__ row, uint32_t d, const float* qs,
+                                                 uint32_t j) {
+    float a[16];
+#pragma unroll
+    for (int l = 0; l < 16; l++) a[l] = 0.0f;
+    const uint32_t nchunk = d / 16;
+    for (uint32_t c = j; c < nchunk; c += 4) {
+        uint32_t w[BITS == 2 ? 1 : 2];
+        if (BITS == 2) {
+            w[0] = *reinterpret_cast<const uint32_t*>(row + c * 4);
+        } else {
+            uint2 x = *reinterpret_cast<const uint2*>(row + c * 8);
+            w[0] = x.x;
+            w[BITS == 2 ? 0 : 1] = x.y;
+        }
+        const float* qc = qs + c * 16;
+#pragma unroll
+        for (int l = 0; l < 16; l++) a[l] = __builtin_fmaf((float)code_of<BITS>(w, l), qc[l], a[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < 16; l++) {
+        float s01 = __fadd_rn(a[l], __shfl_xor(a[l], 1));
+        a[l] = __fadd_rn(s01, __shfl_xor(s01, 2));
+    }
+#pragma unroll
+    for (int l = 0; l < 8; l++) a[l] = __fadd_rn(a[l], a[l + 8]);
+#pragma unroll
+    for (int l = 0; l < 4; l++) a[l] = __fadd_rn(a[l], a[l + 4]);
+#pragma unroll
+    for (int l = 0; l < 2; l++) a[l] = __fadd_rn(a[l], a[l + 2]);
+    return __fadd_rn(a[0], a[1]);
+}
+
 // AVX2 tier (src/quant_kernels.rs:226-389): b=2 keeps 16 lanes (acc_hi codes
 // 0-7, acc_lo codes 8-15 of each 4-byte chunk), b=4 keeps 8 lanes; then
 // horizontal_sum_avx2's order.
@@ -902,14 +944,26 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
     stage_q(qs, qhat + q * d, d);
     const float off = offset[q];
     const uint32_t bpv = d * BITS / 8;
-    tk_init(tk);
-    for (uint64_t r0 = sb; r0 < se; r0 += blockDim.x) {
-        tk_reserve(tk, out_k, blockDim.x);
-        const uint64_t j = r0 + threadIdx.x;
-        const uint32_t id = j < se ? cands[j] : 0u;
-        if (j < se && id < n_docs) {
-            float s = doc_score<TIER, BITS>(codes + (uint64_t)id * bpv, d, qs, inv_norm, off);
-            tk_offer(tk, score_key(s, id));
+    tk_init(tk, out_k);
+    if (TIER == 2) {
+        const uint32_t per_round = blockDim.x / 4, quad = threadIdx.x >> 2, jl = threadIdx.x & 3;
+        for (uint64_t r0 = sb; r0 < se; r0 += per_round) {
+            tk_reserve(tk, out_k, per_round);
+            const uint64_t j = r0 + quad;
+            const uint32_t id = j < se ? cands[j] : 0u;
+            const bool ok = j < se && id < n_docs;
+            const float raw = raw_avx512_quad<BITS>(codes + (uint64_t)(ok ? id : 0u) * bpv, d, qs, jl);
+            if (ok && jl == 0) tk_offer(tk, out_k, score_key(__fadd_rn(__fmul_rn(raw, inv_norm), off), id));
+        }
+    } else {
+        for (uint64_t r0 = sb; r0 < se; r0 += blockDim.x) {
+            tk_reserve(tk, out_k, blockDim.x);
+            const uint64_t j = r0 + threadIdx.x;
+            const uint32_t id = j < se ? cands[j] : 0u;
+            if (j < se && id < n_docs) {
+                float s = doc_score<TIER, BITS>(codes + (uint64_t)id * bpv, d, qs, inv_norm, off);
+                tk_offer(tk, out_k, score_key(s, id));
+            }
         }
     }
     __syncthreads();
@@ -943,11 +997,11 @@ __global__ __launch_bounds__(256) void k_fullscan_rq(const uint8_t* __restrict__
     stage_q(qs, qhat + q * d, d);
     const float off = offset[q];
     const uint32_t bpv = d * BITS / 8;
-    tk_init(tk);
+    tk_init(tk, k);
     for (uint64_t r0 = begin; r0 < end; r0 += blockDim.x) {
         tk_reserve(tk, k, blockDim.x);
         const uint64_t di = r0 + threadIdx.x;
-        if (di < end) tk_offer(tk, score_key(doc_score<TIER, BITS>(codes + di * bpv, d, qs, inv_norm, off), (uint32_t)di));
+        if (di < end) tk_offer(tk, k, score_key(doc_score<TIER, BITS>(codes + di * bpv, d, qs, inv_norm, off), (uint32_t)di));
     }
     __syncthreads();
     tk_flush(tk, k);
@@ -963,13 +1017,13 @@ __global__ __launch_bounds__(256) void k_merge_topk(const uint64_t* __restrict__
     const uint64_t q = qmap ? qmap[qidx] : qidx;
     const uint64_t total = (uint64_t)nseg * k;
     const uint64_t* in = partial + qidx * total;
-    tk_init(tk);
+    tk_init(tk, k);
     for (uint64_t r0 = 0; r0 < total; r0 += blockDim.x) {
         tk_reserve(tk, k, blockDim.x);
         uint64_t j = r0 + threadIdx.x;
         if (j < total) {
             uint64_t key = in[j];
-            if (key != kKeyMax) tk_offer(tk, key);
+            if (key != kKeyMax) tk_offer(tk, k, key);
         }
     }
     __syncthreads();
@@ -1037,7 +1091,7 @@ __global__ __launch_bounds__(256) void k_fullscan_fs(const uint8_t* __restrict__
     const uint8_t* lq = lut + q * (uint64_t)pairs * 16;
     for (uint32_t i = threadIdx.x; i < pairs * 16; i += blockDim.x) slut[i] = lq[i];
     const float bias = bias_invq[q * 2 + 0], invq = bias_invq[q * 2 + 1];
-    tk_init(tk);
+    tk_init(tk, k);
     const uint64_t nblk = (n + 31) / 32;
     const uint64_t b0 = seg * blocks_per_seg, b1 = min(nblk, b0 + blocks_per_seg);
     for (uint64_t blk = b0; blk < b1; blk += blockDim.x / 32) {
@@ -1051,7 +1105,7 @@ __global__ __launch_bounds__(256) void k_fullscan_fs(const uint8_t* __restrict__
             for (uint32_t p = 0; p < pairs; p++) acc += slut[p * 16 + bp[(uint64_t)p * 32]];
             float raw = __fadd_rn(bias, __fmul_rn((float)acc, invq));
             float s = __fadd_rn(__fmul_rn(raw, inv_norm), 0.0f);
-            tk_offer(tk, score_key(s, (uint32_t)doc));
+            tk_offer(tk, k, score_key(s, (uint32_t)doc));
         }
     }
     __syncthreads();
@@ -1360,7 +1414,7 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
 
 uint32_t rerank_nsplit(uint64_t nq, uint64_t avg_m) {
     if (nq >= 128) return 1;
-    uint64_t want = (avg_m + 255) / 256;
+    uint64_t want = (avg_m + 63) / 64;
     uint64_t cap = std::max<uint64_t>(1, 512 / std::max<uint64_t>(1, nq));
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, cap, 32}));
 }
