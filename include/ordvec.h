@@ -260,6 +260,56 @@ ordvec_status_t ordvec_fastscan_write(const ordvec_fastscan_t *index, const char
 ordvec_status_t ordvec_fastscan_load(const char *path, ordvec_fastscan_t **out);
 void ordvec_fastscan_free(ordvec_fastscan_t *index);
 
+/* ---- Serving: continuous batching over resident indexes -------------------
+ * No reference counterpart (ordvec serves in-process, one call at a time);
+ * built on ordvec_two_stage_search's semantics.  `slots` worker contexts (each
+ * a HIP stream with its own workspace) take pending requests in arrival order
+ * and coalesce them up to the batch capacity that fits each slot's share of
+ * `memory_budget_bytes` (and `max_batch_queries`); batches of different slots
+ * overlap on the GPU.  A request's result is independent of the mix: rows are
+ * the first min(k, n) entries of the batch's (score desc, id asc) order. */
+typedef struct ordvec_server_t ordvec_server_t;
+
+typedef struct {
+    uint64_t struct_size;
+    uint64_t requests;
+    uint64_t queries;
+    uint64_t errors;
+    double duration_s;
+    double offered_qps;   /* queries per second offered (open loop) */
+    double achieved_qps;  /* queries completed per second */
+    double achieved_rps;  /* requests completed per second */
+    double p50_ms, p90_ms, p99_ms, p999_ms, max_ms, mean_ms;  /* request latency, arrival -> result */
+    double batches;
+    double mean_batch_queries;
+    uint64_t reserved[8];
+} ordvec_serve_report_t;
+
+ordvec_status_t ordvec_server_start(const ordvec_sign_bitmap_t *sign, const ordvec_rankquant_t *rq, uint64_t m,
+                                    uint64_t max_k, uint32_t slots, uint64_t max_batch_queries,
+                                    uint64_t memory_budget_bytes, ordvec_server_t **out);
+/* Batch capacity chosen at start and the number of batches/queries run so far. */
+ordvec_status_t ordvec_server_info(ordvec_server_t *server, uint64_t *max_batch_queries, uint64_t *batches,
+                                   uint64_t *batched_queries);
+/* Host buffers: queries [nq][dim]; outputs [nq][min(k, n)] must stay valid
+ * until ordvec_server_wait returns for the ticket. */
+ordvec_status_t ordvec_server_submit(ordvec_server_t *server, const float *queries, uint64_t nq, uint64_t k,
+                                     float *out_scores, int64_t *out_ids, uint64_t *ticket);
+/* Blocks until the request completes; reports submit -> completion latency
+ * and the size of the batch it ran in. */
+ordvec_status_t ordvec_server_wait(ordvec_server_t *server, uint64_t ticket, uint64_t *latency_ns,
+                                   uint64_t *batch_queries);
+/* Open-loop load: Poisson request arrivals at offered_qps / mean request size
+ * per second for duration_s, request sizes uniform in [min_nq, max_nq]
+ * (queries taken from query_pool [pool_n][dim]), k uniform in [min_k, max_k];
+ * latency counts from the scheduled arrival. */
+ordvec_status_t ordvec_server_open_loop(ordvec_server_t *server, const float *query_pool, uint64_t pool_n,
+                                        double offered_qps, double duration_s, uint32_t min_nq, uint32_t max_nq,
+                                        uint32_t min_k, uint32_t max_k, uint64_t seed,
+                                        ordvec_serve_report_t *report);
+/* Drains pending requests, joins the slots and frees their resources. */
+void ordvec_server_stop(ordvec_server_t *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,12 +18,14 @@ from .index import (  # noqa: E402,F401
     synchronize,
     two_stage_search,
 )
+from .serving import Server  # noqa: E402,F401
 
 __all__ = [
     "RankQuant",
     "SignBitmap",
     "RankQuantFastscan",
     "two_stage_search",
+    "Server",
     "synchronize",
     "OrdvecError",
     "MAX_DIM",

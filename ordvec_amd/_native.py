@@ -84,6 +84,28 @@ class SearchStats(C.Structure):
     ]
 
 
+class ServeReport(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint64),
+        ("requests", C.c_uint64),
+        ("queries", C.c_uint64),
+        ("errors", C.c_uint64),
+        ("duration_s", C.c_double),
+        ("offered_qps", C.c_double),
+        ("achieved_qps", C.c_double),
+        ("achieved_rps", C.c_double),
+        ("p50_ms", C.c_double),
+        ("p90_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("p999_ms", C.c_double),
+        ("max_ms", C.c_double),
+        ("mean_ms", C.c_double),
+        ("batches", C.c_double),
+        ("mean_batch_queries", C.c_double),
+        ("reserved", C.c_uint64 * 8),
+    ]
+
+
 class Hit(C.Structure):
     _fields_ = [("row_id", C.c_uint64), ("id", C.c_uint64), ("score", C.c_float), ("reserved", C.c_uint32)]
 
@@ -138,6 +160,12 @@ SIGNATURES = {
     "ordvec_fastscan_write": (ST, [P, C.c_char_p]),
     "ordvec_fastscan_load": (ST, [C.c_char_p, C.POINTER(P)]),
     "ordvec_fastscan_free": (None, [P]),
+    "ordvec_server_start": (ST, [P, P, U64, U64, U32, U64, U64, C.POINTER(P)]),
+    "ordvec_server_info": (ST, [P, P, P, P]),
+    "ordvec_server_submit": (ST, [P, P, U64, U64, P, P, P]),
+    "ordvec_server_wait": (ST, [P, U64, P, P]),
+    "ordvec_server_open_loop": (ST, [P, P, U64, C.c_double, C.c_double, U32, U32, U32, U32, U64, P]),
+    "ordvec_server_stop": (None, [P]),
 }
 
 _lib = None

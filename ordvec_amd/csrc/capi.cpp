@@ -12,7 +12,13 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <condition_variable>
+#include <deque>
+#include <memory>
 #include <mutex>
+#include <random>
+#include <thread>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -45,6 +51,9 @@ ordvec_status_t boundary(F&& f) {
 }
 
 [[noreturn]] void fail(uint32_t status, const std::string& msg) { throw Error(status, msg); }
+
+#define NONNULL(p, name) \
+    if (!(p)) fail(kNullPointer, name " pointer is NULL")
 
 // Limits shared with the reference loaders (src/rank_io.rs:124-138,268-286).
 constexpr uint64_t kMaxDim = 65535;
@@ -81,34 +90,17 @@ struct DevBuf {
     }
 };
 
-struct Engine {
-    std::recursive_mutex mu;
+// An execution context: one HIP stream with its own workspaces and
+// host-visible status buffer.  API calls use the engine's default context
+// (serialised by the engine mutex); serving slots each own one.
+struct Ctx {
     hipStream_t stream = nullptr;
-    bool inited = false;
     DevBuf ws, ws2;  // per-call workspace; ws2 serves the exact stage-1 recomputation
-    uint32_t* d_flag = nullptr;
-    uint32_t* pinned = nullptr;  // host-visible status readback
+    uint32_t* pinned = nullptr;
     size_t pinned_count = 0;
 
-    void init() {
-        if (inited) return;
-        int n = 0;
-        hipError_t e = hipGetDeviceCount(&n);
-        if (e != hipSuccess || n <= 0)
-            fail(kInternal, std::string("no HIP device is visible (hipGetDeviceCount: ") + hipGetErrorString(e) +
-                                ", count " + std::to_string(n) + ")");
-        OVX_HIP(hipMalloc(&d_flag, 256));
-        inited = true;
-    }
-
-    void* workspace(size_t bytes) {
-        init();
-        return ws.get(bytes, stream);
-    }
-    void* workspace2(size_t bytes) {
-        init();
-        return ws2.get(bytes, stream);
-    }
+    void* workspace(size_t bytes) { return ws.get(bytes, stream); }
+    void* workspace2(size_t bytes) { return ws2.get(bytes, stream); }
     uint32_t* pinned_u32(size_t count) {
         if (count > pinned_count) {
             if (pinned) {
@@ -122,12 +114,50 @@ struct Engine {
         }
         return pinned;
     }
+    void release() {
+        if (ws.p) (void)hipFree(ws.p);
+        if (ws2.p) (void)hipFree(ws2.p);
+        if (pinned) (void)hipHostFree(pinned);
+        ws.p = ws2.p = nullptr;
+        ws.size = ws2.size = 0;
+        pinned = nullptr;
+        pinned_count = 0;
+    }
+};
+
+struct Engine {
+    std::recursive_mutex mu;
+    bool inited = false;
+    Ctx def;
+    uint32_t* d_flag = nullptr;
+
+    void init() {
+        if (inited) return;
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n <= 0)
+            fail(kInternal, std::string("no HIP device is visible (hipGetDeviceCount: ") + hipGetErrorString(e) +
+                                ", count " + std::to_string(n) + ")");
+        OVX_HIP(hipMalloc(&d_flag, 256));
+        inited = true;
+    }
 };
 
 Engine& eng() {
     static Engine* e = new Engine();
     return *e;
 }
+
+thread_local Ctx* t_ctx = nullptr;
+
+// The context the calling thread runs on (a serving slot, or the default).
+Ctx& cur() { return t_ctx ? *t_ctx : eng().def; }
+
+struct CtxScope {
+    Ctx* prev;
+    explicit CtxScope(Ctx* c) : prev(t_ctx) { t_ctx = c; }
+    ~CtxScope() { t_ctx = prev; }
+};
 
 struct Carver {
     char* base;
@@ -151,11 +181,12 @@ bool host_all_finite(const float* x, uint64_t n) {
 // Device finite check (synchronises); returns true when every value is finite.
 bool device_all_finite(const float* x, uint64_t n) {
     Engine& E = eng();
-    OVX_HIP(hipMemsetAsync(E.d_flag, 0, 4, E.stream));
-    launch_check_finite(x, n, E.d_flag, E.stream);
+    hipStream_t st = cur().stream;
+    OVX_HIP(hipMemsetAsync(E.d_flag, 0, 4, st));
+    launch_check_finite(x, n, E.d_flag, st);
     uint32_t flag = 0;
-    OVX_HIP(hipMemcpyAsync(&flag, E.d_flag, 4, hipMemcpyDeviceToHost, E.stream));
-    OVX_HIP(hipStreamSynchronize(E.stream));
+    OVX_HIP(hipMemcpyAsync(&flag, E.d_flag, 4, hipMemcpyDeviceToHost, st));
+    OVX_HIP(hipStreamSynchronize(st));
     return flag == 0;
 }
 
@@ -166,12 +197,12 @@ void check_finite(const float* x, uint64_t n, uint64_t flags, uint32_t status, c
 }
 
 void h2d(void* dst, const void* src, size_t bytes) {
-    if (bytes) OVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, eng().stream));
+    if (bytes) OVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, cur().stream));
 }
 void d2h(void* dst, const void* src, size_t bytes) {
-    if (bytes) OVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, eng().stream));
+    if (bytes) OVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, cur().stream));
 }
-void sync() { OVX_HIP(hipStreamSynchronize(eng().stream)); }
+void sync() { OVX_HIP(hipStreamSynchronize(cur().stream)); }
 
 // ---------------------------------------------------------------------------
 // Index handles
@@ -192,7 +223,7 @@ struct RqIndex {
             fail(kInternal, "HBM allocation for RankQuant codes failed");
         }
         if (codes) {
-            OVX_HIP(hipMemcpyAsync(p, codes, n * bpv, hipMemcpyDeviceToDevice, eng().stream));
+            OVX_HIP(hipMemcpyAsync(p, codes, n * bpv, hipMemcpyDeviceToDevice, cur().stream));
             sync();
             OVX_HIP(hipFree(codes));
         }
@@ -218,9 +249,9 @@ struct SbIndex {
             (void)hipGetLastError();
             fail(kInternal, "HBM allocation for SignBitmap words failed");
         }
-        OVX_HIP(hipMemsetAsync(p, 0, nc * block_words() * 8, eng().stream));
+        OVX_HIP(hipMemsetAsync(p, 0, nc * block_words() * 8, cur().stream));
         if (words) {
-            OVX_HIP(hipMemcpyAsync(p, words, cap_blocks * block_words() * 8, hipMemcpyDeviceToDevice, eng().stream));
+            OVX_HIP(hipMemcpyAsync(p, words, cap_blocks * block_words() * 8, hipMemcpyDeviceToDevice, cur().stream));
             sync();
             OVX_HIP(hipFree(words));
         }
@@ -287,7 +318,7 @@ void for_each_chunk(const float* vectors, uint64_t n, uint32_t dim, uint64_t fla
         return;
     }
     uint64_t rows = std::max<uint64_t>(1, kChunkBytes / ((uint64_t)dim * 4));
-    float* stage = static_cast<float*>(eng().workspace(rows * dim * 4));
+    float* stage = static_cast<float*>(cur().workspace(rows * dim * 4));
     for (uint64_t r0 = 0; r0 < n; r0 += rows) {
         uint64_t cnt = std::min(rows, n - r0);
         h2d(stage, vectors + r0 * dim, cnt * dim * 4);
@@ -304,7 +335,7 @@ void rq_add(RqIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
     ix->reserve(ix->n + n);
     uint64_t base = ix->n;
     for_each_chunk(vectors, n, ix->dim, flags, [&](const float* dv, uint64_t r0, uint64_t cnt) {
-        launch_rank_encode(dv, cnt, ix->dim, ix->bits, ix->codes + (base + r0) * ix->bpv, eng().stream);
+        launch_rank_encode(dv, cnt, ix->dim, ix->bits, ix->codes + (base + r0) * ix->bpv, cur().stream);
     });
     ix->n += n;
     if (!(flags & ORDVEC_FLAG_DEVICE_PTRS)) sync();
@@ -317,7 +348,7 @@ void sb_add(SbIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
     ix->reserve(ix->n + n);
     uint64_t base = ix->n;
     for_each_chunk(vectors, n, ix->dim, flags, [&](const float* dv, uint64_t r0, uint64_t cnt) {
-        launch_sign_encode(dv, cnt, ix->dim, ix->words, base + r0, true, eng().stream);
+        launch_sign_encode(dv, cnt, ix->dim, ix->words, base + r0, true, cur().stream);
     });
     ix->n += n;
     if (!(flags & ORDVEC_FLAG_DEVICE_PTRS)) sync();
@@ -340,7 +371,7 @@ const float* stage_queries(Carver& cv, const float* q, uint64_t count, uint64_t 
 
 // Waits for the stream and returns the per-query status words.
 std::vector<uint32_t> read_qstat(const uint32_t* dstat, uint64_t nq) {
-    uint32_t* h = eng().pinned_u32(nq);
+    uint32_t* h = cur().pinned_u32(nq);
     d2h(h, dstat, nq * 4);
     sync();
     return std::vector<uint32_t>(h, h + nq);
@@ -378,7 +409,7 @@ void full_search_dev(const RqIndex* ix, const float* dq, uint64_t nq, uint64_t k
     void* work = cv.take<char>(fullscan_work_bytes(ix->n, nq, (uint32_t)kk));
     const int tier = select_tier(d, ix->bits);
     const float inv_norm = rankquant_inv_norm(d, ix->bits);
-    hipStream_t s = eng().stream;
+    hipStream_t s = cur().stream;
     launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, qstat, s);
     launch_fullscan_rq(ix->codes, ix->n, d, ix->bits, tier, inv_norm, qhat, off, nq, (uint32_t)kk, work, ds, di, s);
 }
@@ -398,7 +429,7 @@ void subset_search_dev(const RqIndex* ix, const float* dq, uint64_t nq, const ui
     void* work = cv.take<char>(rerank_work_bytes(nq, (uint32_t)out_k, nsplit));
     const int tier = select_tier(d, ix->bits);
     const float inv_norm = rankquant_inv_norm(d, ix->bits);
-    hipStream_t s = eng().stream;
+    hipStream_t s = cur().stream;
     launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, qstat, s);
     launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, doff, dc, row_m, ix->n, nullptr,
                   (uint32_t)out_k, ds, di, nsplit, work, s);
@@ -423,7 +454,7 @@ Stage1Bufs stage1_dev(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t 
     void* base = cv.take<char>(stage1_fast_bytes(b.plan, nq));
     carve_stage1_fast(b.plan, nq, base, &b.work);
     launch_stage1_fast(ix->words, ix->n, ix->dim, dq, nq, m_eff, b.plan, b.work, b.qbits, cand, cand_h, qstat,
-                       eng().stream);
+                       cur().stream);
     return b;
 }
 
@@ -435,7 +466,7 @@ std::vector<uint32_t> flagged(const std::vector<uint32_t>& st, uint32_t bit) {
 }
 
 void d2d(void* dst, const void* src, size_t bytes) {
-    if (bytes) OVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, eng().stream));
+    if (bytes) OVX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, cur().stream));
 }
 
 // Host-facing wrappers: host or device pointers per `flags`; host-pointer
@@ -464,7 +495,7 @@ void rq_full_search(const RqIndex* ix, const float* queries, uint64_t nq, uint64
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
     size_t bytes = full_search_bytes(ix, nq, kk) + al(nq * 4) +
                    (dev ? 0 : al(nq * ix->dim * 4) + al(nq * kk * 4) + al(nq * kk * 8));
-    Carver cv{static_cast<char*>(eng().workspace(bytes))};
+    Carver cv{static_cast<char*>(cur().workspace(bytes))};
     const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
     uint32_t* qstat = cv.take<uint32_t>(nq);
     float* ds = dev ? out_s : cv.take<float>(nq * kk);
@@ -490,7 +521,7 @@ void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, cons
     size_t bytes = subset_search_bytes(ix, nq, out_k, avg_m) + al(nq * 4) +
                    (dev ? 0 : al(nq * ix->dim * 4) + al((nq + 1) * 8) + al(n_cands * 4) + al(nq * out_k * 4) +
                                   al(nq * out_k * 8));
-    Carver cv{static_cast<char*>(eng().workspace(bytes))};
+    Carver cv{static_cast<char*>(cur().workspace(bytes))};
     const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
     uint32_t* qstat = cv.take<uint32_t>(nq);
     const uint64_t* doff = offsets;
@@ -524,18 +555,245 @@ void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint
     const uint64_t nf = fl.size(), qpv = ix->qpv;
     Stage1Plan plan = plan_stage1(ix->n, ix->dim, nf);
     size_t bytes = al(nf * qpv * 8) + al(stage1_work_bytes(plan, nf, m_eff)) + 2 * al(nf * m_eff * 4);
-    Carver cv{static_cast<char*>(eng().workspace2(bytes))};
+    Carver cv{static_cast<char*>(cur().workspace2(bytes))};
     uint64_t* qb = cv.take<uint64_t>(nf * qpv);
     for (uint64_t i = 0; i < nf; i++) d2d(qb + i * qpv, qbits + (uint64_t)fl[i] * qpv, qpv * 8);
     Stage1Work w{};
     carve_stage1(plan, nf, m_eff, cv.take<char>(stage1_work_bytes(plan, nf, m_eff)), &w);
     uint32_t* cc = cv.take<uint32_t>(nf * m_eff);
     uint32_t* ch = cv.take<uint32_t>(nf * m_eff);
-    launch_stage1(ix->words, ix->n, ix->dim, qb, nf, m_eff, plan, w, cc, cand_h ? ch : nullptr, eng().stream);
+    launch_stage1(ix->words, ix->n, ix->dim, qb, nf, m_eff, plan, w, cc, cand_h ? ch : nullptr, cur().stream);
     for (uint64_t i = 0; i < nf; i++) {
         d2d(cand + (uint64_t)fl[i] * m_eff, cc + i * m_eff, m_eff * 4);
         if (cand_h) d2d(cand_h + (uint64_t)fl[i] * m_eff, ch + i * m_eff, m_eff * 4);
     }
+}
+
+// Two-stage search on the calling thread's context (src/sign_bitmap.rs:453-483
+// then src/quant.rs:1336-1394).  Outputs nq x min(k, n).
+void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, uint64_t nq, uint64_t m, uint64_t k,
+                   float* out_scores, int64_t* out_ids, uint64_t flags) {
+    if (sb->dim != ix->dim) fail(kBadArgument, "sign_probe.dim must match RankQuant dim " + std::to_string(ix->dim));
+    if (sb->n != ix->n) fail(kBadArgument, "sign_probe.len must match RankQuant len " + std::to_string(ix->n));
+    const uint64_t out_k = std::min<uint64_t>(k, ix->n);
+    const uint64_t m_eff = std::min<uint64_t>(m, ix->n);
+    if (nq == 0 || out_k == 0) return;
+    NONNULL(queries, "queries");
+    NONNULL(out_scores, "out_scores");
+    NONNULL(out_ids, "out_ids");
+    check_m(m_eff);
+    check_k(out_k);
+    check_search_dim(ix->dim);
+    if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
+    host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
+    eng().init();
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    const uint32_t d = ix->dim;
+    const uint32_t nsplit = rerank_nsplit(nq, m_eff);
+    size_t bytes = al(nq * d * 4) + 2 * al(nq * 4) + al(nq * m_eff * 4) + stage1_bytes(sb, nq, m_eff) +
+                   (dev ? 0 : al(nq * d * 4) + al(nq * out_k * 4) + al(nq * out_k * 8)) + al(nq * 4) +
+                   rerank_work_bytes(nq, (uint32_t)out_k, nsplit);
+    Carver cv{static_cast<char*>(cur().workspace(bytes))};
+    const float* dq = stage_queries(cv, queries, nq * d, flags);
+    float* qhat = cv.take<float>(nq * d);
+    float* off = cv.take<float>(nq);
+    uint32_t* qstat = cv.take<uint32_t>(nq);
+    uint32_t* qmap = cv.take<uint32_t>(nq);
+    uint32_t* cand = cv.take<uint32_t>(nq * m_eff);
+    void* rwork = cv.take<char>(rerank_work_bytes(nq, (uint32_t)out_k, nsplit));
+    float* ds = dev ? out_scores : cv.take<float>(nq * out_k);
+    int64_t* di = dev ? out_ids : cv.take<int64_t>(nq * out_k);
+    const int tier = select_tier(d, ix->bits);
+    const float inv_norm = rankquant_inv_norm(d, ix->bits);
+    hipStream_t s = cur().stream;
+    launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, nullptr, s);
+    Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv);
+    launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, nullptr,
+                  (uint32_t)out_k, ds, di, nsplit, rwork, s);
+    if (!dev) {
+        d2h(out_scores, ds, nq * out_k * 4);
+        d2h(out_ids, di, nq * out_k * 8);
+    }
+    std::vector<uint32_t> st = read_qstat(qstat, nq);
+    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
+        fail(kNonfiniteQuery, "queries contain NaN or infinity");
+    std::vector<uint32_t> fl = flagged(st, kStatFallback);
+    if (!fl.empty()) {
+        stage1_fix(sb, b.qbits, fl, m_eff, cand, nullptr);
+        h2d(qmap, fl.data(), fl.size() * 4);
+        launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, fl.size(), nullptr, cand, m_eff, ix->n,
+                      qmap, (uint32_t)out_k, ds, di, 1, nullptr, s);
+        if (!dev)
+            for (uint32_t q : fl) {
+                d2h(out_scores + (uint64_t)q * out_k, ds + (uint64_t)q * out_k, out_k * 4);
+                d2h(out_ids + (uint64_t)q * out_k, di + (uint64_t)q * out_k, out_k * 8);
+            }
+        sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching over resident indexes.
+//
+// Each slot is a worker thread with its own execution context (stream,
+// workspaces, pinned staging).  A free slot takes pending requests in arrival
+// order until the next one would exceed the slot's batch capacity (derived
+// from its share of the memory budget), runs one two-stage search for the
+// coalesced batch with the largest requested k, and hands each request the
+// first min(k, n) entries of its rows -- the (score desc, id asc) order makes a
+// smaller k a prefix of a larger one, so results do not depend on the mix.
+// Slots run concurrently, so batches overlap on the GPU.
+// ---------------------------------------------------------------------------
+uint64_t now_ns() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct ServeRequest {
+    const float* queries = nullptr;  // host, [nq][d]
+    uint64_t nq = 0, k = 0;
+    float* out_s = nullptr;  // host, [nq][min(k, n)]
+    int64_t* out_i = nullptr;
+    std::vector<float> own_s;  // buffers owned by the open-loop driver
+    std::vector<int64_t> own_i;
+    uint64_t t_arrival_ns = 0, t_done_ns = 0;
+    uint32_t status = 0;
+    std::string error;
+    bool done = false;
+    uint64_t batch_queries = 0;
+};
+
+struct ServeSlot {
+    Ctx ctx;
+    std::thread th;
+    float* q_stage = nullptr;  // pinned host staging
+    float* s_stage = nullptr;
+    int64_t* i_stage = nullptr;
+};
+
+struct Server {
+    const SbIndex* sb = nullptr;
+    const RqIndex* rq = nullptr;
+    uint64_t m = 0, max_batch = 0, kcap = 0;
+    std::vector<std::unique_ptr<ServeSlot>> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServeRequest*> pending;
+    std::unordered_map<uint64_t, ServeRequest*> tickets;
+    uint64_t next_ticket = 1;
+    bool stopping = false;
+    uint64_t batches = 0, batched_queries = 0;
+
+    void worker(ServeSlot* slot) {
+        CtxScope scope(&slot->ctx);
+        const uint64_t d = rq->dim;
+        for (;;) {
+            std::vector<ServeRequest*> batch;
+            uint64_t q = 0, kmax = 0;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stopping || !pending.empty(); });
+                if (pending.empty()) return;  // stopping and drained
+                while (!pending.empty()) {
+                    ServeRequest* r = pending.front();
+                    if (!batch.empty() && q + r->nq > max_batch) break;
+                    batch.push_back(r);
+                    pending.pop_front();
+                    q += r->nq;
+                    kmax = std::max(kmax, r->k);
+                }
+                batches++;
+                batched_queries += q;
+            }
+            uint32_t status = 0;
+            std::string err;
+            const uint64_t out_k = std::min<uint64_t>(kmax, rq->n);
+            try {
+                uint64_t o = 0;
+                for (ServeRequest* r : batch) {
+                    std::memcpy(slot->q_stage + o * d, r->queries, r->nq * d * 4);
+                    o += r->nq;
+                }
+                // Queries were checked for NaN/inf when they were submitted.
+                two_stage_run(sb, rq, slot->q_stage, q, m, kmax, slot->s_stage, slot->i_stage,
+                              ORDVEC_FLAG_ASSUME_FINITE);
+                o = 0;
+                for (ServeRequest* r : batch) {
+                    const uint64_t rk = std::min<uint64_t>(r->k, rq->n);
+                    for (uint64_t i = 0; i < r->nq; i++) {
+                        std::memcpy(r->out_s + i * rk, slot->s_stage + (o + i) * out_k, rk * 4);
+                        std::memcpy(r->out_i + i * rk, slot->i_stage + (o + i) * out_k, rk * 8);
+                    }
+                    o += r->nq;
+                }
+            } catch (const Error& e) {
+                status = e.status;
+                err = e.what();
+            } catch (const std::exception& e) {
+                status = kPanic;
+                err = e.what();
+            }
+            const uint64_t t = now_ns();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (ServeRequest* r : batch) {
+                    r->status = status;
+                    r->error = err;
+                    r->t_done_ns = t;
+                    r->batch_queries = q;
+                    r->done = true;
+                }
+            }
+            cv_done.notify_all();
+        }
+    }
+
+    void enqueue(ServeRequest* r) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            pending.push_back(r);
+        }
+        cv_work.notify_one();
+    }
+
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stopping = true;
+        }
+        cv_work.notify_all();
+        for (auto& sl : slots)
+            if (sl->th.joinable()) sl->th.join();
+        for (auto& sl : slots) {
+            {
+                CtxScope scope(&sl->ctx);
+                (void)hipStreamSynchronize(sl->ctx.stream);
+            }
+            if (sl->q_stage) (void)hipHostFree(sl->q_stage);
+            if (sl->s_stage) (void)hipHostFree(sl->s_stage);
+            if (sl->i_stage) (void)hipHostFree(sl->i_stage);
+            sl->ctx.release();
+            if (sl->ctx.stream) (void)hipStreamDestroy(sl->ctx.stream);
+        }
+        slots.clear();
+    }
+};
+
+// Device bytes one two-stage call of nq queries needs (both workspaces).
+size_t two_stage_bytes(const SbIndex* sb, const RqIndex* rq, uint64_t nq, uint64_t m, uint64_t k) {
+    const uint64_t d = rq->dim, m_eff = std::min<uint64_t>(m, rq->n), out_k = std::min<uint64_t>(k, rq->n);
+    const uint32_t nsplit = rerank_nsplit(nq, m_eff);
+    return al(nq * d * 4) * 2 + 3 * al(nq * 4) + al(nq * m_eff * 4) + stage1_bytes(sb, nq, m_eff) +
+           al(nq * out_k * 4) + al(nq * out_k * 8) + rerank_work_bytes(nq, (uint32_t)out_k, nsplit);
+}
+
+// Workspace that covers every batch size up to cap (the split rerank of small
+// batches can need more than a larger batch).
+size_t serve_ws_bytes(const SbIndex* sb, const RqIndex* rq, uint64_t cap, uint64_t m, uint64_t k) {
+    size_t b = two_stage_bytes(sb, rq, cap, m, k);
+    for (uint64_t nq = 1; nq <= std::min<uint64_t>(cap, 512); nq++) b = std::max(b, two_stage_bytes(sb, rq, nq, m, k));
+    return b;
 }
 
 // ---------------------------------------------------------------------------
@@ -849,12 +1107,6 @@ void check_exact_size(uint64_t got, size_t expected, const char* what) {
 constexpr uint64_t kAllCaps =
     ORDVEC_CAP_FULL_SEARCH | ORDVEC_CAP_SUBSET_SEARCH | ORDVEC_CAP_STATS | ORDVEC_CAP_ID_EQUALS_ROW_ID;
 
-uint64_t now_ns() {
-    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
-               std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
-
 // ---------------------------------------------------------------------------
 // Optional event timing of kernel families (engine mutex held by callers)
 // ---------------------------------------------------------------------------
@@ -904,7 +1156,7 @@ KTimes& kt() {
 
 void ktime_begin(int family, hipStream_t s) {
     KTimes& t = kt();
-    if (!t.enabled) return;
+    if (!t.enabled || t_ctx) return;  // serving slots are not event-timed
     hipEvent_t e = t.get();
     OVX_HIP(hipEventRecord(e, s));
     t.open[family] = e;
@@ -912,7 +1164,7 @@ void ktime_begin(int family, hipStream_t s) {
 
 void ktime_end(int family, hipStream_t s) {
     KTimes& t = kt();
-    if (!t.enabled || !t.open[family]) return;
+    if (!t.enabled || t_ctx || !t.open[family]) return;
     hipEvent_t e = t.get();
     OVX_HIP(hipEventRecord(e, s));
     t.pending.push_back({t.open[family], e, family});
@@ -928,6 +1180,7 @@ struct ordvec_index_t {};
 struct ordvec_rankquant_t {};
 struct ordvec_sign_bitmap_t {};
 struct ordvec_fastscan_t {};
+struct ordvec_server_t {};
 
 static IndexHandle* H(const ordvec_index_t* p) { return reinterpret_cast<IndexHandle*>(const_cast<ordvec_index_t*>(p)); }
 static RqIndex* RQ(const ordvec_rankquant_t* p) { return reinterpret_cast<RqIndex*>(const_cast<ordvec_rankquant_t*>(p)); }
@@ -935,8 +1188,6 @@ static SbIndex* SB(const ordvec_sign_bitmap_t* p) { return reinterpret_cast<SbIn
 static FsIndex* FS(const ordvec_fastscan_t* p) { return reinterpret_cast<FsIndex*>(const_cast<ordvec_fastscan_t*>(p)); }
 
 #define LOCK std::lock_guard<std::recursive_mutex> _lk(eng().mu)
-#define NONNULL(p, name) \
-    if (!(p)) fail(kNullPointer, name " pointer is NULL")
 
 extern "C" {
 
@@ -1198,9 +1449,9 @@ ordvec_status_t ordvec_set_stream(void* hip_stream) {
     return boundary([&] {
         LOCK;
         eng().init();
-        if (eng().stream == static_cast<hipStream_t>(hip_stream)) return;
-        OVX_HIP(hipStreamSynchronize(eng().stream));
-        eng().stream = static_cast<hipStream_t>(hip_stream);
+        if (cur().stream == static_cast<hipStream_t>(hip_stream)) return;
+        OVX_HIP(hipStreamSynchronize(cur().stream));
+        cur().stream = static_cast<hipStream_t>(hip_stream);
     });
 }
 
@@ -1361,7 +1612,7 @@ ordvec_status_t ordvec_rankquant_codes(const ordvec_rankquant_t* index, uint8_t*
         NONNULL(out, "out");
         LOCK;
         if (flags & ORDVEC_FLAG_DEVICE_PTRS) {
-            OVX_HIP(hipMemcpyAsync(out, ix->codes, bytes, hipMemcpyDeviceToDevice, eng().stream));
+            OVX_HIP(hipMemcpyAsync(out, ix->codes, bytes, hipMemcpyDeviceToDevice, cur().stream));
         } else {
             d2h(out, ix->codes, bytes);
             sync();
@@ -1443,7 +1694,7 @@ ordvec_status_t ordvec_sign_bitmap_top_m(const ordvec_sign_bitmap_t* index, cons
         const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
         size_t bytes = stage1_bytes(ix, nq, m_eff) + al(nq * 4) +
                        (dev ? 0 : al(nq * ix->dim * 4) + 2 * al(nq * m_eff * 4));
-        Carver cv{static_cast<char*>(eng().workspace(bytes))};
+        Carver cv{static_cast<char*>(cur().workspace(bytes))};
         const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
         uint32_t* qstat = cv.take<uint32_t>(nq);
         uint32_t* dc = dev ? out_candidates : cv.take<uint32_t>(nq * m_eff);
@@ -1510,65 +1761,8 @@ ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const 
     return boundary([&] {
         NONNULL(sign, "sign");
         NONNULL(rq, "rq");
-        const SbIndex* sb = SB(sign);
-        const RqIndex* ix = RQ(rq);
-        if (sb->dim != ix->dim) fail(kBadArgument, "sign_probe.dim must match RankQuant dim " + std::to_string(ix->dim));
-        if (sb->n != ix->n) fail(kBadArgument, "sign_probe.len must match RankQuant len " + std::to_string(ix->n));
-        const uint64_t out_k = std::min<uint64_t>(k, ix->n);
-        const uint64_t m_eff = std::min<uint64_t>(m, ix->n);
-        if (nq == 0 || out_k == 0) return;
-        NONNULL(queries, "queries");
-        NONNULL(out_scores, "out_scores");
-        NONNULL(out_ids, "out_ids");
-        check_m(m_eff);
-        check_k(out_k);
-        check_search_dim(ix->dim);
-        if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
-        host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
         LOCK;
-        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-        const uint32_t d = ix->dim;
-        const uint32_t nsplit = rerank_nsplit(nq, m_eff);
-        size_t bytes = al(nq * d * 4) + 2 * al(nq * 4) + al(nq * m_eff * 4) + stage1_bytes(sb, nq, m_eff) +
-                       (dev ? 0 : al(nq * d * 4) + al(nq * out_k * 4) + al(nq * out_k * 8)) + al(nq * 4) +
-                       rerank_work_bytes(nq, (uint32_t)out_k, nsplit);
-        Carver cv{static_cast<char*>(eng().workspace(bytes))};
-        const float* dq = stage_queries(cv, queries, nq * d, flags);
-        float* qhat = cv.take<float>(nq * d);
-        float* off = cv.take<float>(nq);
-        uint32_t* qstat = cv.take<uint32_t>(nq);
-        uint32_t* qmap = cv.take<uint32_t>(nq);
-        uint32_t* cand = cv.take<uint32_t>(nq * m_eff);
-        void* rwork = cv.take<char>(rerank_work_bytes(nq, (uint32_t)out_k, nsplit));
-        float* ds = dev ? out_scores : cv.take<float>(nq * out_k);
-        int64_t* di = dev ? out_ids : cv.take<int64_t>(nq * out_k);
-        const int tier = select_tier(d, ix->bits);
-        const float inv_norm = rankquant_inv_norm(d, ix->bits);
-        hipStream_t s = eng().stream;
-        launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, nullptr, s);
-        Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv);
-        launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, nullptr,
-                      (uint32_t)out_k, ds, di, nsplit, rwork, s);
-        if (!dev) {
-            d2h(out_scores, ds, nq * out_k * 4);
-            d2h(out_ids, di, nq * out_k * 8);
-        }
-        std::vector<uint32_t> st = read_qstat(qstat, nq);
-        if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
-            fail(kNonfiniteQuery, "queries contain NaN or infinity");
-        std::vector<uint32_t> fl = flagged(st, kStatFallback);
-        if (!fl.empty()) {
-            stage1_fix(sb, b.qbits, fl, m_eff, cand, nullptr);
-            h2d(qmap, fl.data(), fl.size() * 4);
-            launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, fl.size(), nullptr, cand, m_eff, ix->n,
-                          qmap, (uint32_t)out_k, ds, di, 1, nullptr, s);
-            if (!dev)
-                for (uint32_t q : fl) {
-                    d2h(out_scores + (uint64_t)q * out_k, ds + (uint64_t)q * out_k, out_k * 4);
-                    d2h(out_ids + (uint64_t)q * out_k, di + (uint64_t)q * out_k, out_k * 8);
-                }
-            sync();
-        }
+        two_stage_run(SB(sign), RQ(rq), queries, nq, m, k, out_scores, out_ids, flags);
     });
 }
 
@@ -1612,8 +1806,8 @@ ordvec_status_t ordvec_fastscan_add(ordvec_fastscan_t* index, const float* vecto
             ix->n = 0;
             fail(kInternal, "HBM allocation for FastScan payload failed");
         }
-        OVX_HIP(hipMemsetAsync(ix->packed, 0, bytes, eng().stream));
-        launch_fastscan_pack(tmp.codes, n, ix->dim, ix->packed, eng().stream);
+        OVX_HIP(hipMemsetAsync(ix->packed, 0, bytes, cur().stream));
+        launch_fastscan_pack(tmp.codes, n, ix->dim, ix->packed, cur().stream);
         sync();
     });
 }
@@ -1637,7 +1831,7 @@ ordvec_status_t ordvec_fastscan_search(const ordvec_fastscan_t* index, const flo
         size_t wk = fullscan_work_bytes(ix->n, nq, (uint32_t)kk);
         size_t bytes = al(nq * d * 4) * 2 + 2 * al(nq * 4) + al(nq * (d / 2) * 16) + al(nq * 8) + wk +
                        (dev ? 0 : al(nq * kk * 4) + al(nq * kk * 8));
-        Carver cv{static_cast<char*>(eng().workspace(bytes))};
+        Carver cv{static_cast<char*>(cur().workspace(bytes))};
         const float* dq = stage_queries(cv, queries, nq * d, flags);
         float* qhat = cv.take<float>(nq * d);
         float* off = cv.take<float>(nq);
@@ -1648,7 +1842,7 @@ ordvec_status_t ordvec_fastscan_search(const ordvec_fastscan_t* index, const flo
         float* ds = dev ? out_scores : cv.take<float>(nq * kk);
         int64_t* di = dev ? out_ids : cv.take<int64_t>(nq * kk);
         const float inv_norm = rankquant_inv_norm(d, 2);
-        hipStream_t s = eng().stream;
+        hipStream_t s = cur().stream;
         launch_prepare_queries(dq, nq, d, 2, 0, inv_norm, qhat, off, qstat, s);
         launch_fastscan_prepare(qhat, nq, d, lut, bq, s);
         launch_fullscan_fs(ix->packed, ix->n, d, lut, bq, inv_norm, nq, (uint32_t)kk, work, ds, di, s);
@@ -1692,6 +1886,225 @@ void ordvec_fastscan_free(ordvec_fastscan_t* index) {
     if (!index) return;
     LOCK;
     delete FS(index);
+}
+
+
+// ---- serving -------------------------------------------------------------------
+
+static Server* SV(ordvec_server_t* p) { return reinterpret_cast<Server*>(p); }
+
+ordvec_status_t ordvec_server_start(const ordvec_sign_bitmap_t* sign, const ordvec_rankquant_t* rq, uint64_t m,
+                                    uint64_t max_k, uint32_t slots, uint64_t max_batch_queries,
+                                    uint64_t memory_budget_bytes, ordvec_server_t** out) {
+    return boundary([&] {
+        NONNULL(sign, "sign");
+        NONNULL(rq, "rq");
+        NONNULL(out, "out");
+        *out = nullptr;
+        const SbIndex* sb = SB(sign);
+        const RqIndex* ix = RQ(rq);
+        if (sb->dim != ix->dim || sb->n != ix->n) fail(kBadArgument, "sign and RankQuant indexes must hold the same documents");
+        if (slots == 0 || slots > 16) fail(kBadArgument, "slots must be in [1, 16]");
+        if (max_k == 0 || max_k > kTopKMax) fail(kBadArgument, "max_k must be in [1, 1024]");
+        check_m(std::min<uint64_t>(m, ix->n));
+        LOCK;
+        eng().init();
+        // Admission by device memory: the largest batch whose workspace fits
+        // each slot's share of the budget.
+        const uint64_t per_slot = memory_budget_bytes / slots;
+        uint64_t cap = std::max<uint64_t>(1, max_batch_queries);
+        while (cap > 1 && serve_ws_bytes(sb, ix, cap, m, max_k) > per_slot) cap = cap * 3 / 4;
+        if (serve_ws_bytes(sb, ix, cap, m, max_k) > per_slot)
+            fail(kBadArgument, "memory budget is too small for a single query per slot");
+        std::unique_ptr<Server> sv(new Server());
+        sv->sb = sb;
+        sv->rq = ix;
+        sv->m = m;
+        sv->max_batch = cap;
+        sv->kcap = max_k;
+        for (uint32_t i = 0; i < slots; i++) {
+            std::unique_ptr<ServeSlot> sl(new ServeSlot());
+            OVX_HIP(hipStreamCreateWithFlags(&sl->ctx.stream, hipStreamNonBlocking));
+            OVX_HIP(hipHostMalloc((void**)&sl->q_stage, cap * ix->dim * 4, hipHostMallocDefault));
+            OVX_HIP(hipHostMalloc((void**)&sl->s_stage, cap * max_k * 4, hipHostMallocDefault));
+            OVX_HIP(hipHostMalloc((void**)&sl->i_stage, cap * max_k * 8, hipHostMallocDefault));
+            {
+                CtxScope scope(&sl->ctx);
+                (void)cur().workspace(serve_ws_bytes(sb, ix, cap, m, max_k));
+                (void)cur().pinned_u32(cap);
+            }
+            sv->slots.push_back(std::move(sl));
+        }
+        Server* raw = sv.get();
+        for (auto& sl : raw->slots) sl->th = std::thread(&Server::worker, raw, sl.get());
+        *out = reinterpret_cast<ordvec_server_t*>(sv.release());
+    });
+}
+
+ordvec_status_t ordvec_server_info(ordvec_server_t* server, uint64_t* max_batch_queries, uint64_t* batches,
+                                   uint64_t* batched_queries) {
+    return boundary([&] {
+        NONNULL(server, "server");
+        Server* sv = SV(server);
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (max_batch_queries) *max_batch_queries = sv->max_batch;
+        if (batches) *batches = sv->batches;
+        if (batched_queries) *batched_queries = sv->batched_queries;
+    });
+}
+
+ordvec_status_t ordvec_server_submit(ordvec_server_t* server, const float* queries, uint64_t nq, uint64_t k,
+                                     float* out_scores, int64_t* out_ids, uint64_t* ticket) {
+    return boundary([&] {
+        NONNULL(server, "server");
+        NONNULL(ticket, "ticket");
+        Server* sv = SV(server);
+        if (nq == 0 || nq > sv->max_batch)
+            fail(kBadArgument, "request size must be in [1, " + std::to_string(sv->max_batch) + "] queries");
+        if (k == 0 || k > sv->kcap) fail(kBadArgument, "k must be in [1, max_k]");
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        host_finite_or_fail(queries, nq * sv->rq->dim, 0, kNonfiniteQuery, "queries");
+        ServeRequest* r = new ServeRequest();
+        r->queries = queries;
+        r->nq = nq;
+        r->k = k;
+        r->out_s = out_scores;
+        r->out_i = out_ids;
+        r->t_arrival_ns = now_ns();
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            *ticket = sv->next_ticket++;
+            sv->tickets[*ticket] = r;
+        }
+        sv->enqueue(r);
+    });
+}
+
+ordvec_status_t ordvec_server_wait(ordvec_server_t* server, uint64_t ticket, uint64_t* latency_ns,
+                                   uint64_t* batch_queries) {
+    ServeRequest* r = nullptr;
+    ordvec_status_t st = boundary([&] {
+        NONNULL(server, "server");
+        Server* sv = SV(server);
+        std::unique_lock<std::mutex> lk(sv->mu);
+        auto it = sv->tickets.find(ticket);
+        if (it == sv->tickets.end()) fail(kBadArgument, "unknown ticket " + std::to_string(ticket));
+        r = it->second;
+        sv->cv_done.wait(lk, [&] { return r->done; });
+        sv->tickets.erase(it);
+        if (latency_ns) *latency_ns = r->t_done_ns - r->t_arrival_ns;
+        if (batch_queries) *batch_queries = r->batch_queries;
+        if (r->status) {
+            std::string msg = r->error;
+            uint32_t code = r->status;
+            delete r;
+            r = nullptr;
+            throw Error(code, msg);
+        }
+    });
+    delete r;
+    return st;
+}
+
+ordvec_status_t ordvec_server_open_loop(ordvec_server_t* server, const float* query_pool, uint64_t pool_n,
+                                        double offered_qps, double duration_s, uint32_t min_nq, uint32_t max_nq,
+                                        uint32_t min_k, uint32_t max_k, uint64_t seed, ordvec_serve_report_t* report) {
+    return boundary([&] {
+        NONNULL(server, "server");
+        NONNULL(query_pool, "query_pool");
+        NONNULL(report, "report");
+        check_exact_size(report->struct_size, sizeof(ordvec_serve_report_t), "ordvec_serve_report_t");
+        Server* sv = SV(server);
+        if (min_nq == 0 || max_nq < min_nq || max_nq > sv->max_batch || max_nq > pool_n)
+            fail(kBadArgument, "request sizes must satisfy 1 <= min_nq <= max_nq <= min(max batch, pool)");
+        if (min_k == 0 || max_k < min_k || max_k > sv->kcap) fail(kBadArgument, "k range must lie in [1, max_k]");
+        if (!(offered_qps > 0) || !(duration_s > 0)) fail(kBadArgument, "rate and duration must be positive");
+        const uint64_t d = sv->rq->dim;
+        host_finite_or_fail(query_pool, pool_n * d, 0, kNonfiniteQuery, "query_pool");
+        std::mt19937_64 rng(seed);
+        std::uniform_int_distribution<uint32_t> unq(min_nq, max_nq), uk(min_k, max_k);
+        const double mean_nq = 0.5 * (min_nq + max_nq);
+        std::exponential_distribution<double> gap(offered_qps / mean_nq);  // request arrivals per second
+        std::vector<std::unique_ptr<ServeRequest>> reqs;
+        const uint64_t b0 = sv->batches;
+        const uint64_t t0 = now_ns() + 2000000;  // start 2 ms from now
+        double t = 0.0;
+        for (;;) {
+            t += gap(rng);
+            if (t >= duration_s) break;
+            std::unique_ptr<ServeRequest> r(new ServeRequest());
+            r->nq = unq(rng);
+            r->k = uk(rng);
+            const uint64_t start = std::uniform_int_distribution<uint64_t>(0, pool_n - r->nq)(rng);
+            r->queries = query_pool + start * d;
+            r->t_arrival_ns = t0 + (uint64_t)(t * 1e9);
+            reqs.push_back(std::move(r));
+        }
+        for (auto& r : reqs) {  // allocate outputs before the clock starts
+            const uint64_t rk = std::min<uint64_t>(r->k, sv->rq->n);
+            r->own_s.resize(r->nq * rk);
+            r->own_i.resize(r->nq * rk);
+            r->out_s = r->own_s.data();
+            r->out_i = r->own_i.data();
+        }
+        // Open loop: submit at the scheduled instant regardless of completions.
+        for (auto& r : reqs) {
+            while (now_ns() < r->t_arrival_ns) {
+                if (r->t_arrival_ns - now_ns() > 200000) std::this_thread::sleep_for(std::chrono::microseconds(50));
+            }
+            sv->enqueue(r.get());
+        }
+        {
+            std::unique_lock<std::mutex> lk(sv->mu);
+            for (auto& r : reqs) sv->cv_done.wait(lk, [&] { return r->done; });
+        }
+        ordvec_serve_report_t rep;
+        std::memset(&rep, 0, sizeof(rep));
+        rep.struct_size = sizeof(rep);
+        std::vector<double> lat;
+        uint64_t last_done = t0, queries = 0;
+        for (auto& r : reqs) {
+            if (r->status) {
+                rep.errors++;
+                continue;
+            }
+            lat.push_back((double)(r->t_done_ns - r->t_arrival_ns) * 1e-6);
+            last_done = std::max(last_done, r->t_done_ns);
+            queries += r->nq;
+        }
+        rep.requests = reqs.size();
+        rep.queries = queries;
+        rep.duration_s = duration_s;
+        rep.offered_qps = offered_qps;
+        const double span = (double)(last_done - t0) * 1e-9;
+        rep.achieved_qps = span > 0 ? queries / span : 0;
+        rep.achieved_rps = span > 0 ? (double)lat.size() / span : 0;
+        if (!lat.empty()) {
+            std::sort(lat.begin(), lat.end());
+            auto pct = [&](double p) { return lat[std::min(lat.size() - 1, (size_t)(p * (lat.size() - 1) + 0.5))]; };
+            rep.p50_ms = pct(0.50);
+            rep.p90_ms = pct(0.90);
+            rep.p99_ms = pct(0.99);
+            rep.p999_ms = pct(0.999);
+            rep.max_ms = lat.back();
+            double sum = 0;
+            for (double v : lat) sum += v;
+            rep.mean_ms = sum / lat.size();
+        }
+        std::lock_guard<std::mutex> lk(sv->mu);
+        rep.batches = (double)(sv->batches - b0);
+        rep.mean_batch_queries = rep.batches > 0 ? (double)queries / rep.batches : 0;
+        *report = rep;
+    });
+}
+
+void ordvec_server_stop(ordvec_server_t* server) {
+    if (!server) return;
+    Server* sv = SV(server);
+    sv->stop();
+    delete sv;
 }
 
 }  // extern "C"

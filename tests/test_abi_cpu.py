@@ -33,6 +33,7 @@ def test_struct_sizes_match_abi_v1():
     assert C.sizeof(N.SearchParams) == 128
     assert C.sizeof(N.Hit) == 24
     assert C.sizeof(N.SearchStats) == 184
+    assert C.sizeof(N.ServeReport) == 192
 
 
 def test_versions_and_status_names():
@@ -109,6 +110,10 @@ def test_parameter_validation_needs_no_device():
     assert lib.ordvec_index_search(None, None, None, 0, None, None) == N.NULL_POINTER
     lib.ordvec_index_free(None)
     lib.ordvec_rankquant_free(None)
+    assert lib.ordvec_server_start(None, None, 1000, 10, 1, 64, 1 << 30, C.byref(h)) == N.NULL_POINTER
+    assert lib.ordvec_server_submit(None, None, 1, 10, None, None, None) == N.NULL_POINTER
+    assert lib.ordvec_server_wait(None, 1, None, None) == N.NULL_POINTER
+    lib.ordvec_server_stop(None)
 
 
 def test_python_api_validation():
