@@ -192,6 +192,7 @@ ordvec_status_t ordvec_synchronize(void);
 #define ORDVEC_KERNEL_ENCODE 5
 #define ORDVEC_KERNEL_PREPARE 6
 #define ORDVEC_KERNEL_STAGE1_SAMPLE 7
+#define ORDVEC_KERNEL_BACKWARD 8
 ordvec_status_t ordvec_kernel_timing_enable(int enable);
 ordvec_status_t ordvec_kernel_timing_read(uint32_t family, uint64_t *launches, double *total_ms);
 
@@ -259,6 +260,23 @@ ordvec_status_t ordvec_fastscan_info(const ordvec_fastscan_t *index, uint64_t *d
 ordvec_status_t ordvec_fastscan_write(const ordvec_fastscan_t *index, const char *path);
 ordvec_status_t ordvec_fastscan_load(const char *path, ordvec_fastscan_t **out);
 void ordvec_fastscan_free(ordvec_fastscan_t *index);
+
+/* ---- Differentiable rerank -------------------------------------------------
+ * No reference counterpart: the forward is the reference's asymmetric score
+ * (src/quant.rs:1126-1281 scoring, src/quant_kernels.rs) evaluated for given
+ * ids instead of a top-k, and the backward gives its gradient, so a caller
+ * can train query-side parameters through a fixed candidate selection.
+ * ids: [nq][k] int64 in [-1, n) (-1 = no row: score -inf, no gradient).
+ * score_ids: out_scores[nq][k], bit-identical to the search kernels' scores.
+ * score_ids_backward: grad_queries[nq][dim] = dL/dqueries (raw, unnormalised
+ * queries) given grad_scores[nq][k] = dL/dscores, re-decoding the rows' codes
+ * (no score matrix is kept).  k <= 4096.  Out-of-range ids fail with
+ * ROW_ID_OUT_OF_RANGE (device pointers: after the kernels ran). */
+ordvec_status_t ordvec_rankquant_score_ids(const ordvec_rankquant_t *index, const float *queries, uint64_t nq,
+                                           const int64_t *ids, uint64_t k, float *out_scores, uint64_t flags);
+ordvec_status_t ordvec_rankquant_score_ids_backward(const ordvec_rankquant_t *index, const float *queries,
+                                                    uint64_t nq, const int64_t *ids, uint64_t k,
+                                                    const float *grad_scores, float *grad_queries, uint64_t flags);
 
 /* ---- Serving: continuous batching over resident indexes -------------------
  * No reference counterpart (ordvec serves in-process, one call at a time);

@@ -64,6 +64,7 @@ def _declare(L):
         "orc_l2_normalise": (None, [P, U32, P]),
         "orc_select_tier": (C.c_int, [U32, U32]),
         "orc_rq_scores": (None, [P, P, U64, U32, U32, P, C.c_int, P]),
+        "orc_score_ids_grad": (None, [P, U32, U32, P, U64, P, U64, P, P]),
         "orc_topk": (None, [P, P, U64, U64, P, P]),
         "orc_search_asymmetric": (None, [P, U64, U32, U32, P, U64, U64, P, P]),
         "orc_subset_batched": (None, [P, U64, U32, U32, P, U64, P, P, U64, P, P, C.c_int]),
@@ -178,6 +179,41 @@ def rq_scores(codes, bits: int, q_unit, ids=None, tier: int = -1) -> np.ndarray:
     out = np.empty(n, dtype=np.float32)
     lib().orc_rq_scores(_p(codes), idp, n, d, bits, _p(q_unit), tier, _p(out))
     return out
+
+
+def score_ids(codes, bits: int, queries, ids) -> np.ndarray:
+    """Scores of rows ``ids`` [nq, k] (-1 = none -> -inf) for raw queries,
+    normalised first (src/util.rs:84-115), as the reference's TopK sees them."""
+    q = _f32(np.atleast_2d(queries))
+    ids = np.asarray(ids, dtype=np.int64).reshape(q.shape[0], -1)
+    out = np.full(ids.shape, -np.inf, dtype=np.float32)
+    for b in range(q.shape[0]):
+        ok = ids[b] >= 0
+        out[b, ok] = rq_scores(codes, bits, l2_normalise(q[b]), ids[b, ok])
+    return out
+
+
+def score_ids_grad(codes, bits: int, queries, ids, grad_scores) -> np.ndarray:
+    """f64 dL/dqueries of score_ids given dL/dscores (analytic; see the C
+    function's comment)."""
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    q = _f32(np.atleast_2d(queries))
+    nq, d = q.shape
+    ids = np.ascontiguousarray(np.asarray(ids, dtype=np.int64).reshape(nq, -1))
+    g = np.ascontiguousarray(np.asarray(grad_scores, dtype=np.float64).reshape(ids.shape))
+    out = np.empty((nq, d), dtype=np.float64)
+    lib().orc_score_ids_grad(_p(codes), d, bits, _p(q), nq, _p(ids), ids.shape[1], _p(g), _p(out))
+    return out
+
+
+def decode_centres(codes, d: int, bits: int) -> np.ndarray:
+    """Rows of packed codes -> centre(code) as f64 [n, d] (MSB-first packing,
+    src/rank.rs:89-156; centre, src/rank.rs:220-228)."""
+    c = np.asarray(codes, dtype=np.uint8)
+    cpb = 8 // bits
+    shifts = np.array([(cpb - 1 - j) * bits for j in range(cpb)], dtype=np.uint8)
+    v = (c[..., :, None] >> shifts) & ((1 << bits) - 1)
+    return v.reshape(*c.shape[:-1], d).astype(np.float64) - ((1 << bits) - 1) / 2.0
 
 
 def topk(scores, k: int, tie_keys=None):

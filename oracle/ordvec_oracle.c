@@ -833,6 +833,50 @@ void orc_fs_search(const uint8_t *packed, uint64_t n, uint32_t d, const float *q
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Score gradient (the reference has no backward pass)                  */
+/* ------------------------------------------------------------------ */
+
+/* Analytic gradient, in f64, of the asymmetric score of fixed rows with
+ * respect to the raw query x.  The score is the closed form of the reference's
+ * kernels, s_i = sum_j qhat_j * centre(code_ij) / R (src/quant_kernels.rs:23-160;
+ * the SIMD tiers' centre-drop plus offset, src/quant.rs:788-798, is the same
+ * function), with qhat = x / ||x|| (src/util.rs:84-115) and R from
+ * src/rank.rs:275-280.  Given G_i = dL/ds_i:
+ *   g = sum_i G_i centre(code_i) / R,  dx = (g - qhat (qhat . g)) / ||x||,
+ * and dx = 0 when ||x|| <= 1e-12 (the normalised query is then constant).
+ * ids[q*k + i] == -1 contributes nothing. */
+void orc_score_ids_grad(const uint8_t *codes, uint32_t d, uint32_t bits, const float *x, uint64_t nq,
+                        const int64_t *ids, uint64_t k, const double *G, double *dx) {
+    uint32_t bpv = d * bits / 8;
+    double R = sqrt((double)d * ((double)(1u << bits) * (double)(1u << bits) - 1.0) / 12.0);
+    double half = ((double)(1u << bits) - 1.0) / 2.0;
+    double *g = (double *)malloc(sizeof(double) * d);
+    for (uint64_t q = 0; q < nq; q++) {
+        const float *xq = x + q * d;
+        double *o = dx + q * d;
+        double ss = 0.0;
+        for (uint32_t j = 0; j < d; j++) ss += (double)xq[j] * (double)xq[j];
+        double norm = sqrt(ss);
+        if (norm <= 1e-12) {
+            for (uint32_t j = 0; j < d; j++) o[j] = 0.0;
+            continue;
+        }
+        for (uint32_t j = 0; j < d; j++) g[j] = 0.0;
+        for (uint64_t i = 0; i < k; i++) {
+            int64_t id = ids[q * k + i];
+            if (id < 0) continue;
+            const uint8_t *row = codes + (uint64_t)id * bpv;
+            double gi = G[q * k + i] / R;
+            for (uint32_t j = 0; j < d; j++) g[j] += gi * ((double)code_at(row, j, bits) - half);
+        }
+        double dot = 0.0;
+        for (uint32_t j = 0; j < d; j++) dot += (double)xq[j] / norm * g[j];
+        for (uint32_t j = 0; j < d; j++) o[j] = (g[j] - (double)xq[j] / norm * dot) / norm;
+    }
+    free(g);
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -20,12 +20,18 @@ from .index import (  # noqa: E402,F401
 )
 from .serving import Server  # noqa: E402,F401
 
+try:  # the autograd binding needs torch
+    from .autograd import rerank_scores  # noqa: E402,F401
+except ImportError:  # pragma: no cover
+    pass
+
 __all__ = [
     "RankQuant",
     "SignBitmap",
     "RankQuantFastscan",
     "two_stage_search",
     "Server",
+    "rerank_scores",
     "synchronize",
     "OrdvecError",
     "MAX_DIM",

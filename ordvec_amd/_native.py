@@ -160,6 +160,8 @@ SIGNATURES = {
     "ordvec_fastscan_write": (ST, [P, C.c_char_p]),
     "ordvec_fastscan_load": (ST, [C.c_char_p, C.POINTER(P)]),
     "ordvec_fastscan_free": (None, [P]),
+    "ordvec_rankquant_score_ids": (ST, [P, P, U64, P, U64, P, U64]),
+    "ordvec_rankquant_score_ids_backward": (ST, [P, P, U64, P, U64, P, P, U64]),
     "ordvec_server_start": (ST, [P, P, U64, U64, U32, U64, U64, C.POINTER(P)]),
     "ordvec_server_info": (ST, [P, P, P, P]),
     "ordvec_server_submit": (ST, [P, P, U64, U64, P, P, P]),
@@ -222,7 +224,7 @@ def check(status: int) -> None:
 
 
 KERNEL_FAMILIES = {"sign_scan": 0, "stage1_select": 1, "rerank": 2, "full_scan": 3, "fastscan": 4,
-                   "encode": 5, "prepare": 6, "stage1_sample": 7}
+                   "encode": 5, "prepare": 6, "stage1_sample": 7, "backward": 8}
 
 
 def kernel_timing_enable(on: bool = True) -> None:

@@ -1602,6 +1602,98 @@ ordvec_status_t ordvec_rankquant_search_subset_batched(const ordvec_rankquant_t*
     });
 }
 
+// Shared checks of the score-by-id pair; ids in [-1, n) (-1 = no row).
+void check_score_ids(const RqIndex* ix, const int64_t* ids, uint64_t nq, uint64_t k, uint64_t flags) {
+    check_search_dim(ix->dim);
+    if (ix->n == 0) fail(kBadArgument, "score_ids needs a non-empty index");
+    if (k > kStage1MaxM) fail(kUnsupportedOperation, "score_ids supports k <= " + std::to_string(kStage1MaxM));
+    if (!(flags & ORDVEC_FLAG_DEVICE_PTRS))
+        for (uint64_t i = 0; i < nq * k; i++)
+            if (ids[i] < -1 || ids[i] >= (int64_t)ix->n)
+                fail(kRowIdOutOfRange, "id out of range (n_vectors " + std::to_string(ix->n) + ", -1 = no row)");
+}
+
+ordvec_status_t ordvec_rankquant_score_ids(const ordvec_rankquant_t* index, const float* queries, uint64_t nq,
+                                           const int64_t* ids, uint64_t k, float* out_scores, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const RqIndex* ix = RQ(index);
+        if (nq == 0 || k == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(ids, "ids");
+        NONNULL(out_scores, "out_scores");
+        check_score_ids(ix, ids, nq, k, flags);
+        LOCK;
+        eng().init();
+        check_finite(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        const uint32_t d = ix->dim;
+        size_t bytes = (dev ? 0 : al(nq * d * 4) + al(nq * k * 8) + al(nq * k * 4)) + al(nq * d * 4) + al(nq * 4) + 256;
+        Carver cv{static_cast<char*>(cur().workspace(bytes))};
+        const float* dq = stage_queries(cv, queries, nq * d, flags);
+        const int64_t* did = ids;
+        if (!dev) {
+            int64_t* t = cv.take<int64_t>(nq * k);
+            h2d(t, ids, nq * k * 8);
+            did = t;
+        }
+        float* ds = dev ? out_scores : cv.take<float>(nq * k);
+        float* qhat = cv.take<float>(nq * d);
+        float* off = cv.take<float>(nq);
+        uint32_t* bad = cv.take<uint32_t>(1);
+        hipStream_t s = cur().stream;
+        OVX_HIP(hipMemsetAsync(bad, 0, 4, s));
+        const int tier = select_tier(d, ix->bits);
+        const float inv_norm = rankquant_inv_norm(d, ix->bits);
+        launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, nullptr, s);
+        launch_score_ids(ix->codes, ix->n, d, ix->bits, tier, inv_norm, qhat, off, nq, did, (uint32_t)k, ds, bad, s);
+        if (!dev) d2h(out_scores, ds, nq * k * 4);
+        if (read_qstat(bad, 1)[0]) fail(kRowIdOutOfRange, "id out of range (n_vectors " + std::to_string(ix->n) + ")");
+    });
+}
+
+ordvec_status_t ordvec_rankquant_score_ids_backward(const ordvec_rankquant_t* index, const float* queries, uint64_t nq,
+                                                    const int64_t* ids, uint64_t k, const float* grad_scores,
+                                                    float* grad_queries, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const RqIndex* ix = RQ(index);
+        if (nq == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(grad_queries, "grad_queries");
+        if (k) {
+            NONNULL(ids, "ids");
+            NONNULL(grad_scores, "grad_scores");
+        }
+        check_score_ids(ix, ids, nq, k, flags);
+        LOCK;
+        eng().init();
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        const uint32_t d = ix->dim;
+        size_t bytes = (dev ? 0 : 2 * al(nq * d * 4) + al(nq * k * 8) + al(nq * k * 4)) + 256;
+        Carver cv{static_cast<char*>(cur().workspace(bytes))};
+        const float* dq = stage_queries(cv, queries, nq * d, flags);
+        const int64_t* did = ids;
+        const float* dg = grad_scores;
+        if (!dev && k) {
+            int64_t* t = cv.take<int64_t>(nq * k);
+            h2d(t, ids, nq * k * 8);
+            did = t;
+            float* g = cv.take<float>(nq * k);
+            h2d(g, grad_scores, nq * k * 4);
+            dg = g;
+        }
+        float* dx = dev ? grad_queries : cv.take<float>(nq * d);
+        uint32_t* bad = cv.take<uint32_t>(1);
+        hipStream_t s = cur().stream;
+        OVX_HIP(hipMemsetAsync(bad, 0, 4, s));
+        launch_score_ids_backward(ix->codes, ix->n, d, ix->bits, rankquant_inv_norm(d, ix->bits), dq, nq, did,
+                                  (uint32_t)k, dg, dx, bad, s);
+        if (!dev) d2h(grad_queries, dx, nq * d * 4);
+        if (read_qstat(bad, 1)[0]) fail(kRowIdOutOfRange, "id out of range (n_vectors " + std::to_string(ix->n) + ")");
+    });
+}
+
 ordvec_status_t ordvec_rankquant_codes(const ordvec_rankquant_t* index, uint8_t* out, uint64_t capacity, uint64_t flags) {
     return boundary([&] {
         NONNULL(index, "index");

@@ -68,6 +68,7 @@ enum KernelFamily : int {
     kFamEncode,          // rank / sign / FastScan encode
     kFamPrepare,         // query normalisation
     kFamSample,          // stage-1 sampled threshold pass
+    kFamBackward,        // score-by-id backward (differentiable rerank)
     kFamCount
 };
 void ktime_begin(int family, hipStream_t s);
@@ -85,6 +86,15 @@ int select_tier(uint32_t d, uint32_t bits);
 float rankquant_inv_norm(uint32_t d, uint32_t bits);
 
 void launch_check_finite(const float* x, uint64_t count, uint32_t* flag, hipStream_t s);
+// Scores of given ids (n_docs bounds the ids; -1 gives -inf; others outside
+// [0, n_docs) give -inf and set *bad).
+void launch_score_ids(const uint8_t* codes, uint64_t n_docs, uint32_t d, uint32_t bits, int tier, float inv_norm,
+                      const float* qhat, const float* offset, uint64_t nq, const int64_t* ids, uint32_t k, float* out,
+                      uint32_t* bad, hipStream_t s);
+// dL/dx of the scores above given dL/dscores, through x -> x / ||x||.
+void launch_score_ids_backward(const uint8_t* codes, uint64_t n_docs, uint32_t d, uint32_t bits, float inv_norm,
+                               const float* x, uint64_t nq, const int64_t* ids, uint32_t k, const float* grad,
+                               float* dx, uint32_t* bad, hipStream_t s);
 void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bits, int tier, float inv_norm,
                             float* qhat, float* offset, uint32_t* qstat, hipStream_t s);
 void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, uint64_t first_doc, bool interleaved,

@@ -1035,6 +1035,123 @@ __global__ __launch_bounds__(256) void k_merge_topk(const uint64_t* __restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Differentiable rerank: scores of given ids, and their gradient with respect
+// to the raw query by re-decoding the selected rows' codes.
+// ---------------------------------------------------------------------------
+
+// Sum over the workgroup (256 threads); every thread gets the result.  Fixed
+// reduction order, so the result is deterministic.
+__device__ __forceinline__ float block_sum256(float v, float* red) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = __fadd_rn(v, __shfl_xor(v, o));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return __fadd_rn(__fadd_rn(red[0], red[1]), __fadd_rn(red[2], red[3]));
+}
+
+// out[q][i] = score of row ids[q][i] for query q, computed exactly as the
+// search kernels compute it; id -1 gives -inf, any other id outside [0, n)
+// also gives -inf and sets *bad.
+template <int TIER, int BITS>
+__global__ __launch_bounds__(256) void k_score_ids(const uint8_t* __restrict__ codes, uint64_t n_docs, uint32_t d,
+                                                   float inv_norm, const float* __restrict__ qhat,
+                                                   const float* __restrict__ offset, const int64_t* __restrict__ ids,
+                                                   uint32_t k, float* __restrict__ out, uint32_t* __restrict__ bad) {
+    extern __shared__ float qs[];
+    const uint64_t q = blockIdx.x;
+    stage_q(qs, qhat + q * d, d);
+    __syncthreads();
+    const float off = offset[q];
+    const uint32_t bpv = d * BITS / 8;
+    const int64_t* row = ids + q * k;
+    float* o = out + q * k;
+    if (TIER == 2) {
+        const uint32_t quad = threadIdx.x >> 2, jl = threadIdx.x & 3;
+        for (uint32_t r0 = 0; r0 < k; r0 += 64) {
+            const uint32_t j = r0 + quad;
+            const int64_t id = j < k ? row[j] : -1;
+            const bool ok = id >= 0 && (uint64_t)id < n_docs;
+            const float raw = raw_avx512_quad<BITS>(codes + (uint64_t)(ok ? id : 0) * bpv, d, qs, jl);
+            if (j < k && jl == 0) {
+                o[j] = ok ? __fadd_rn(__fmul_rn(raw, inv_norm), off) : -INFINITY;
+                if (!ok && id != -1) bad[0] = 1u;
+            }
+        }
+    } else {
+        for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) {
+            const int64_t id = row[j];
+            const bool ok = id >= 0 && (uint64_t)id < n_docs;
+            o[j] = ok ? doc_score<TIER, BITS>(codes + (uint64_t)id * bpv, d, qs, inv_norm, off) : -INFINITY;
+            if (!ok && id != -1) bad[0] = 1u;
+        }
+    }
+}
+
+// Backward of k_score_ids through the query normalisation.  With
+// s_i = sum_j qhat_j * c(code_ij) / R and qhat = x / ||x||:
+//   g_j  = sum_i G_i * c(code_ij) / R          (re-decoded from the codes)
+//   dx_j = (g_j - qhat_j * (qhat . g)) / ||x||  (zero when ||x|| <= 1e-12)
+// One workgroup per query; rows (ids) and G staged in LDS, g in LDS (d floats).
+template <int BITS>
+__global__ __launch_bounds__(256) void k_score_ids_bwd(const uint8_t* __restrict__ codes, uint64_t n_docs,
+                                                       uint32_t d, float inv_norm, const float* __restrict__ x,
+                                                       const int64_t* __restrict__ ids, uint32_t k,
+                                                       const float* __restrict__ grad, float* __restrict__ dx,
+                                                       uint32_t* __restrict__ bad) {
+    constexpr uint32_t CPB = 8 / BITS, MASK = (1u << BITS) - 1;
+    const float half = ((float)(1u << BITS) - 1.0f) / 2.0f;
+    extern __shared__ uint64_t smem[];
+    uint32_t* sid = reinterpret_cast<uint32_t*>(smem);
+    float* sg = reinterpret_cast<float*>(sid + k);
+    float* gs = sg + k;  // d floats
+    __shared__ float red[4];
+    const uint64_t q = blockIdx.x;
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+        const int64_t id = ids[q * k + i];
+        const bool ok = id >= 0 && (uint64_t)id < n_docs;
+        if (!ok && id != -1) bad[0] = 1u;
+        sid[i] = ok ? (uint32_t)id : 0u;
+        sg[i] = ok ? grad[q * k + i] : 0.0f;
+    }
+    const float* xq = x + q * d;
+    float ss = 0.0f;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) ss = __builtin_fmaf(xq[j], xq[j], ss);
+    const float norm = __fsqrt_rn(block_sum256(ss, red));  // also orders the LDS staging above
+    const bool zero = !(norm > 1e-12f);
+    const uint32_t bpv = d * BITS / 8;
+    for (uint32_t byte = threadIdx.x; byte < bpv; byte += blockDim.x) {
+        float acc[CPB];
+#pragma unroll
+        for (uint32_t c = 0; c < CPB; c++) acc[c] = 0.0f;
+        const uint8_t* col = codes + byte;
+#pragma unroll 8
+        for (uint32_t i = 0; i < k; i++) {
+            const uint32_t v = col[(uint64_t)sid[i] * bpv];
+            const float g = sg[i];
+#pragma unroll
+            for (uint32_t c = 0; c < CPB; c++) {
+                const float cen = __fadd_rn((float)((v >> ((CPB - 1 - c) * BITS)) & MASK), -half);
+                acc[c] = __builtin_fmaf(g, cen, acc[c]);
+            }
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < CPB; c++) gs[byte * CPB + c] = __fmul_rn(acc[c], inv_norm);
+    }
+    __syncthreads();
+    float xg = 0.0f;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) xg = __builtin_fmaf(xq[j], gs[j], xg);
+    const float inv = zero ? 0.0f : __fdiv_rn(1.0f, norm);
+    const float dot = __fmul_rn(block_sum256(xg, red), inv);  // qhat . g
+    float* o = dx + q * d;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) {
+        const float qh = __fmul_rn(xq[j], inv);
+        o[j] = zero ? 0.0f : __fmul_rn(__fsub_rn(gs[j], __fmul_rn(qh, dot)), inv);
+    }
+}
+
+
 // FastScan b=2 query LUT (src/fastscan.rs:130-173): one workgroup per query.
 __global__ __launch_bounds__(256) void k_fastscan_prepare(const float* __restrict__ qhat, uint32_t d,
                                                           uint8_t* __restrict__ lut, float* __restrict__ bias_invq) {
@@ -1445,6 +1562,39 @@ void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, fl
         check_launch();
     }
     ktime_end(kFamRerank, s);
+}
+
+void launch_score_ids(const uint8_t* codes, uint64_t n_docs, uint32_t d, uint32_t bits, int tier, float inv_norm,
+                      const float* qhat, const float* offset, uint64_t nq, const int64_t* ids, uint32_t k, float* out,
+                      uint32_t* bad, hipStream_t s) {
+    if (nq == 0 || k == 0) return;
+    ktime_begin(kFamRerank, s);
+#define OVX_SCORE_IDS(T, B, _)                                                                                    \
+    hipLaunchKernelGGL((k_score_ids<T, B>), dim3((uint32_t)nq), dim3(256), (size_t)d * 4, s, codes, n_docs, d, \
+                       inv_norm, qhat, offset, ids, k, out, bad)
+    OVX_TIER_BITS_DISPATCH(OVX_SCORE_IDS, 0);
+#undef OVX_SCORE_IDS
+    check_launch();
+    ktime_end(kFamRerank, s);
+}
+
+void launch_score_ids_backward(const uint8_t* codes, uint64_t n_docs, uint32_t d, uint32_t bits, float inv_norm,
+                               const float* x, uint64_t nq, const int64_t* ids, uint32_t k, const float* grad,
+                               float* dx, uint32_t* bad, hipStream_t s) {
+    if (nq == 0) return;
+    const size_t lds = (size_t)k * 8 + (size_t)d * 4;
+    ktime_begin(kFamBackward, s);
+    if (bits == 1)
+        hipLaunchKernelGGL(k_score_ids_bwd<1>, dim3((uint32_t)nq), dim3(256), lds, s, codes, n_docs, d, inv_norm, x,
+                           ids, k, grad, dx, bad);
+    else if (bits == 2)
+        hipLaunchKernelGGL(k_score_ids_bwd<2>, dim3((uint32_t)nq), dim3(256), lds, s, codes, n_docs, d, inv_norm, x,
+                           ids, k, grad, dx, bad);
+    else
+        hipLaunchKernelGGL(k_score_ids_bwd<4>, dim3((uint32_t)nq), dim3(256), lds, s, codes, n_docs, d, inv_norm, x,
+                           ids, k, grad, dx, bad);
+    check_launch();
+    ktime_end(kFamBackward, s);
 }
 
 static uint32_t fullscan_nseg(uint64_t n, uint64_t nq) {

@@ -222,6 +222,50 @@ class RankQuant:
             self._h, _ptr(q), nq, _ptr(off), _ptr(c) if nc else None, nc, int(k), _ptr(s), _ptr(i), flag))
         return s, i
 
+    def _id_matrix(self, ids, nq: int, flag: int):
+        if flag:
+            if not _is_cuda_tensor(ids):
+                raise ValueError("ids must be a CUDA tensor when queries are")
+            t = ids.to(torch.int64).contiguous()
+        else:
+            t = np.ascontiguousarray(np.asarray(ids.detach().cpu().numpy() if torch is not None and
+                                                isinstance(ids, torch.Tensor) else ids), dtype=np.int64)
+        if t.ndim != 2 or t.shape[0] != nq:
+            raise ValueError(f"ids must have shape ({nq}, k); got {tuple(t.shape)}")
+        return t, int(t.shape[1])
+
+    def score_ids(self, queries, ids, assume_finite: bool = False):
+        """Scores of rows ``ids`` (nq, k) for each query, bit-identical to the
+        scores the search methods return; id -1 gives -inf."""
+        d, _, _ = self._info()
+        q, nq, flag = _float_rows(queries, d, "queries")
+        t, k = self._id_matrix(ids, nq, flag)
+        out = _out((nq, k), np.float32, flag)
+        if nq and k:
+            _bind_stream() if flag else _host_stream()
+            f = flag | (N.FLAG_ASSUME_FINITE if assume_finite else 0)
+            N.check(N.lib().ordvec_rankquant_score_ids(self._h, _ptr(q), nq, _ptr(t), k, _ptr(out), f))
+        return out
+
+    def score_ids_backward(self, queries, ids, grad_scores):
+        """dL/dqueries (nq, dim) of ``score_ids`` given dL/dscores (nq, k),
+        from the rows' re-decoded codes."""
+        d, _, _ = self._info()
+        q, nq, flag = _float_rows(queries, d, "queries")
+        t, k = self._id_matrix(ids, nq, flag)
+        if flag:
+            g = grad_scores.to(torch.float32).contiguous()
+        else:
+            g = np.ascontiguousarray(np.asarray(grad_scores), dtype=np.float32)
+        if tuple(g.shape) != (nq, k):
+            raise ValueError(f"grad_scores must have shape ({nq}, {k})")
+        out = _out((nq, d), np.float32, flag)
+        if nq:
+            _bind_stream() if flag else _host_stream()
+            N.check(N.lib().ordvec_rankquant_score_ids_backward(self._h, _ptr(q), nq, _ptr(t), k, _ptr(g),
+                                                                _ptr(out), flag))
+        return out
+
     def codes(self) -> np.ndarray:
         """Packed codes, n x bytes_per_vec (the .ovrq payload)."""
         d, b, n = self._info()

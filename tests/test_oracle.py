@@ -238,3 +238,54 @@ def test_fastscan_top10_close_to_exact(oracle):
     _, ei = oracle.search_asymmetric(codes, 256, 2, qs, 10)
     overlap = np.mean([len(set(a) & set(b)) for a, b in zip(fi, ei)])
     assert overlap >= 9
+
+
+# ---- score gradient (differentiable rerank checker) ----------------------------
+
+def _loss_f64(centres, x, G, R):
+    """L(x) = sum_i G_i * (centres_i . x/||x||) / R in float64."""
+    qh = x / np.linalg.norm(x)
+    return float(G @ (centres @ qh) / R)
+
+
+@pytest.mark.parametrize("dim,bits", [(64, 2), (48, 4), (32, 1)])
+def test_score_ids_closed_form_and_grad_vs_finite_differences(oracle, dim, bits):
+    rng = np.random.default_rng(dim + bits)
+    docs = S.iid_unit(300, dim, seed=dim)
+    codes = oracle.rq_encode(docs, bits)
+    x = rng.standard_normal((3, dim)).astype(np.float32) * 2.5
+    ids = rng.integers(0, 300, size=(3, 12)).astype(np.int64)
+    ids[1, 4] = -1
+    G = rng.standard_normal((3, 12))
+    R = float(np.sqrt(dim * (4 ** bits - 1) / 12.0))
+    # score_ids equals the f64 closed form (within f32 rounding)
+    s = oracle.score_ids(codes, bits, x, ids)
+    for b in range(3):
+        ok = ids[b] >= 0
+        want = oracle.decode_centres(codes[ids[b, ok]], dim, bits) @ (x[b] / np.linalg.norm(x[b])) / R
+        np.testing.assert_allclose(s[b, ok], want, rtol=1e-5, atol=1e-6)
+        assert np.all(np.isneginf(s[b, ~ok]))
+    grad = oracle.score_ids_grad(codes, bits, x, ids, G)
+    h = 1e-3
+    for b in range(3):
+        ok = ids[b] >= 0
+        cen = oracle.decode_centres(codes[ids[b, ok]], dim, bits)
+        xb = x[b].astype(np.float64)
+        fd = np.empty(dim)
+        for j in range(dim):
+            e = np.zeros(dim)
+            e[j] = h
+            fd[j] = (_loss_f64(cen, xb + e, G[b, ok], R) - _loss_f64(cen, xb - e, G[b, ok], R)) / (2 * h)
+        np.testing.assert_allclose(grad[b], fd, rtol=1e-5, atol=1e-8)
+        # the projection term matters: leaving it out is caught by the same check
+        g = (G[b, ok] @ cen) / R
+        assert np.max(np.abs(g / np.linalg.norm(xb) - fd)) > 1e-4
+
+
+def test_score_grad_zero_query_is_zero(oracle):
+    docs = S.iid_unit(10, 16, seed=1)
+    codes = oracle.rq_encode(docs, 2)
+    x = np.zeros((1, 16), np.float32)
+    ids = np.arange(5, dtype=np.int64)[None]
+    assert np.all(oracle.score_ids_grad(codes, 2, x, ids, np.ones((1, 5))) == 0.0)
+    assert np.all(oracle.score_ids(codes, 2, x, ids) == 0.0)
