@@ -428,7 +428,7 @@ def grad_main(args, dev):
                             "gbs": round(bw_bytes / bw_us / 1e3, 1),
                             "frac_of_hbm": round(bw_bytes / bw_us / 1e3 / HBM_PEAK_GBS, 4)},
         "rerank_family_avg_us": round(fam["rerank"][1] / max(1, fam["rerank"][0]) * 1e3, 2),
-        "loss": float(loss),
+        "loss": float(loss.detach()),
         "forward_equals_search_scores": fwd_equal,
         "grad_max_rel_err_vs_cpu_f64": grad_err,
         "dW_max_rel_err_vs_cpu_f64": dW_err,
