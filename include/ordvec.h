@@ -139,8 +139,9 @@ void ordvec_index_info_init(ordvec_index_info_t *info);
 void ordvec_search_params_init(ordvec_search_params_t *params);
 /* ordvec-ffi/src/lib.rs:670-683 */
 void ordvec_search_stats_init(ordvec_search_stats_t *stats);
-/* ordvec-ffi/src/lib.rs:685-776: .ovrq loads into HBM; .ovbm, .ovr, .ovsb and
- * .ovfs return UNSUPPORTED_FORMAT here (use the Part 2 loaders for .ovsb/.ovfs). */
+/* ordvec-ffi/src/lib.rs:685-776: .ovrq (RankQuant) and .ovbm (Bitmap) load into
+ * HBM; .ovr, .ovsb and .ovfs return UNSUPPORTED_FORMAT as in the reference (use
+ * the Part 2 loaders for .ovsb/.ovfs). */
 ordvec_status_t ordvec_index_load(const char *path, uint64_t flags, ordvec_index_t **out);
 /* ordvec-ffi/src/lib.rs:778-840 */
 ordvec_status_t ordvec_index_probe(const char *path, uint64_t flags, ordvec_index_info_t *info_out);
@@ -242,6 +243,45 @@ ordvec_status_t ordvec_sign_bitmap_words(const ordvec_sign_bitmap_t *index, uint
 ordvec_status_t ordvec_sign_bitmap_write(const ordvec_sign_bitmap_t *index, const char *path);
 ordvec_status_t ordvec_sign_bitmap_load(const char *path, ordvec_sign_bitmap_t **out);
 void ordvec_sign_bitmap_free(ordvec_sign_bitmap_t *index);
+
+/* Bitmap (top-bucket, constant-weight): src/bitmap.rs.  Bit j of a document
+ * is set iff coordinate j is among its n_top highest ranks; a query's bits are
+ * its n_top largest values; score = popcount(Q AND D).  The same indexes load
+ * through ABI v1 (ordvec_index_load on .ovbm) where ordvec_index_search runs
+ * Bitmap::search / Bitmap::search_subset.  GPU encode and search: dim <= 4096. */
+typedef struct ordvec_bitmap_t ordvec_bitmap_t;
+/* src/bitmap.rs:71-136 validate_params / new. */
+ordvec_status_t ordvec_bitmap_new(uint64_t dim, uint64_t n_top, ordvec_bitmap_t **out);
+/* src/bitmap.rs:140-162. */
+ordvec_status_t ordvec_bitmap_add(ordvec_bitmap_t *index, const float *vectors, uint64_t n, uint64_t flags);
+ordvec_status_t ordvec_bitmap_info(const ordvec_bitmap_t *index, uint64_t *dim, uint64_t *n_top, uint64_t *n_vectors);
+/* src/bitmap.rs:202-250 search: nq x min(k, n) overlaps (as f32) and ids by
+ * (overlap desc, id asc); k <= 4096. */
+ordvec_status_t ordvec_bitmap_search(const ordvec_bitmap_t *index, const float *queries, uint64_t nq, uint64_t k,
+                                     float *out_scores, int64_t *out_ids, uint64_t flags);
+/* src/bitmap.rs:264-362 top_m_candidates(_batched): nq x min(m, n) ids. */
+ordvec_status_t ordvec_bitmap_top_m(const ordvec_bitmap_t *index, const float *queries, uint64_t nq, uint64_t m,
+                                    uint32_t *out_candidates, uint64_t flags);
+/* src/bitmap.rs:477-505 search_subset per query over CSR rows (host arrays):
+ * rows nq x k, the first min(k, row count) filled (count in out_counts,
+ * optional), the rest -inf / -1; k <= 1024. */
+ordvec_status_t ordvec_bitmap_search_subset_batched(const ordvec_bitmap_t *index, const float *queries, uint64_t nq,
+                                                    const uint64_t *offsets, const uint32_t *rows, uint64_t n_rows,
+                                                    uint64_t k, float *out_scores, int64_t *out_ids,
+                                                    uint64_t *out_counts, uint64_t flags);
+/* src/bitmap.rs:164-187 build_query_bitmap_fp32: nq x dim/64 u64. */
+ordvec_status_t ordvec_bitmap_query_bits(const ordvec_bitmap_t *index, const float *queries, uint64_t nq,
+                                         uint64_t *out_words, uint64_t flags);
+/* src/bitmap.rs:425-460 body_overlap_scores_subset: out[i] = popcount(doc
+ * rows[i] AND q_words) (host arrays; q_words = dim/64 u64). */
+ordvec_status_t ordvec_bitmap_overlap_subset(const ordvec_bitmap_t *index, const uint64_t *q_words,
+                                             const uint32_t *rows, uint64_t n_rows, uint32_t *out);
+/* Row-major copy of the document bitmaps (.ovbm payload order). */
+ordvec_status_t ordvec_bitmap_words(const ordvec_bitmap_t *index, uint64_t *out, uint64_t capacity);
+/* src/rank_io.rs:826-957: .ovbm persistence (load checks every row has n_top bits). */
+ordvec_status_t ordvec_bitmap_write(const ordvec_bitmap_t *index, const char *path);
+ordvec_status_t ordvec_bitmap_load(const char *path, ordvec_bitmap_t **out);
+void ordvec_bitmap_free(ordvec_bitmap_t *index);
 
 /* Two-stage search in one call: SignBitmap::top_m_candidates_batched_serial_csr
  * followed by RankQuant::search_asymmetric_subset_batched_serial_into

@@ -65,6 +65,9 @@ def _declare(L):
         "orc_select_tier": (C.c_int, [U32, U32]),
         "orc_rq_scores": (None, [P, P, U64, U32, U32, P, C.c_int, P]),
         "orc_score_ids_grad": (None, [P, U32, U32, P, U64, P, U64, P, P]),
+        "orc_bitmap_encode": (None, [P, U64, U32, U32, P]),
+        "orc_bitmap_query": (None, [P, U64, U32, U32, P]),
+        "orc_and_popcount_all": (None, [P, U64, U32, P, U64, P]),
         "orc_topk": (None, [P, P, U64, U64, P, P]),
         "orc_search_asymmetric": (None, [P, U64, U32, U32, P, U64, U64, P, P]),
         "orc_subset_batched": (None, [P, U64, U32, U32, P, U64, P, P, U64, P, P, C.c_int]),
@@ -204,6 +207,64 @@ def score_ids_grad(codes, bits: int, queries, ids, grad_scores) -> np.ndarray:
     out = np.empty((nq, d), dtype=np.float64)
     lib().orc_score_ids_grad(_p(codes), d, bits, _p(q), nq, _p(ids), ids.shape[1], _p(g), _p(out))
     return out
+
+
+def bitmap_encode(x, n_top: int) -> np.ndarray:
+    """Bitmap::add (src/bitmap.rs:140-162): n x d/64 u64, row-major."""
+    x = _f32(np.atleast_2d(x))
+    n, d = x.shape
+    out = np.empty((n, d // 64), dtype=np.uint64)
+    lib().orc_bitmap_encode(_p(x), n, d, n_top, _p(out))
+    return out
+
+
+def bitmap_query(q, n_top: int) -> np.ndarray:
+    """build_query_bitmap_fp32 (src/bitmap.rs:164-187): nq x d/64 u64."""
+    q = _f32(np.atleast_2d(q))
+    nq, d = q.shape
+    out = np.empty((nq, d // 64), dtype=np.uint64)
+    lib().orc_bitmap_query(_p(q), nq, d, n_top, _p(out))
+    return out
+
+
+def and_popcount_all(bm, qb) -> np.ndarray:
+    """overlap[q, i] = popcount(doc_i AND query_q) (src/util.rs and_popcount)."""
+    bm = np.ascontiguousarray(bm, dtype=np.uint64)
+    qb = np.ascontiguousarray(np.atleast_2d(qb), dtype=np.uint64)
+    out = np.empty((qb.shape[0], bm.shape[0]), dtype=np.uint32)
+    lib().orc_and_popcount_all(_p(bm), bm.shape[0], bm.shape[1], _p(qb), qb.shape[0], _p(out))
+    return out
+
+
+def bitmap_search(bm, qb, k: int):
+    """Bitmap::search (src/bitmap.rs:202-250): TopK over overlaps as f32 with
+    the document id as tie key."""
+    ov = and_popcount_all(bm, qb)
+    kk = min(k, bm.shape[0])
+    s = np.empty((qb.shape[0], kk), np.float32)
+    i = np.empty((qb.shape[0], kk), np.int64)
+    for q in range(qb.shape[0]):
+        s[q], i[q] = topk(ov[q].astype(np.float32), kk)
+    return s, i
+
+
+def bitmap_top_m(bm, qb, m: int) -> np.ndarray:
+    """top_m_candidates(_batched) (src/bitmap.rs:264-362): ids sorted by
+    (overlap desc, id asc)."""
+    ov = and_popcount_all(bm, qb)
+    n = bm.shape[0]
+    m_eff = min(m, n)
+    ids = np.arange(n)
+    return np.stack([np.lexsort((ids, -ov[q].astype(np.int64)))[:m_eff] for q in range(ov.shape[0])]).astype(np.uint32)
+
+
+def bitmap_search_subset(bm, qb_row, rows, k: int):
+    """Bitmap::search_subset (src/bitmap.rs:477-505): score every listed row,
+    order (overlap desc, row asc), keep min(k, len)."""
+    rows = np.asarray(rows, dtype=np.uint32)
+    ov = and_popcount_all(bm[rows] if len(rows) else bm[:0], qb_row)[0].astype(np.int64)
+    order = np.lexsort((rows.astype(np.int64), -ov))[:min(k, len(rows))]
+    return ov[order].astype(np.float32), rows[order].astype(np.int64)
 
 
 def decode_centres(codes, d: int, bits: int) -> np.ndarray:

@@ -834,6 +834,76 @@ void orc_fs_search(const uint8_t *packed, uint64_t n, uint32_t d, const float *q
 }
 
 /* ------------------------------------------------------------------ */
+/* Bitmap (top-bucket, AND-popcount): src/bitmap.rs                     */
+/* ------------------------------------------------------------------ */
+
+/* src/bitmap.rs:140-162 Bitmap::add: bit j set iff rank_transform(v)[j] >=
+ * d - n_top.  Row-major u64 words, bit j of word j/64. */
+void orc_bitmap_encode(const float *x, uint64_t n, uint32_t d, uint32_t ntop, uint64_t *out) {
+    uint32_t qpv = d / 64;
+#pragma omp parallel
+    {
+        uint16_t *rk = (uint16_t *)malloc(d * sizeof(uint16_t));
+#pragma omp for schedule(static)
+        for (int64_t r = 0; r < (int64_t)n; r++) {
+            uint64_t *o = out + (uint64_t)r * qpv;
+            memset(o, 0, qpv * 8);
+            orc_rank_transform(x + (uint64_t)r * d, d, rk);
+            for (uint32_t j = 0; j < d; j++)
+                if (rk[j] >= d - ntop) o[j / 64] |= 1ull << (j % 64);
+        }
+        free(rk);
+    }
+}
+
+/* src/bitmap.rs:164-187 build_query_bitmap_fp32: a stable sort of the
+ * coordinates by value descending (partial_cmp, so -0 == +0 and ties keep
+ * index order); bits for the first n_top.  Insertion into a stable merge. */
+static void msort_desc(uint16_t *a, uint16_t *tmp, uint32_t n, const float *v) {
+    if (n <= 1) return;
+    uint32_t h = n / 2;
+    msort_desc(a, tmp, h, v);
+    msort_desc(a + h, tmp, n - h, v);
+    uint32_t i = 0, j = h, o = 0;
+    while (i < h && j < n) {
+        /* take the right element only if strictly greater (stability) */
+        if (v[a[j]] > v[a[i]]) tmp[o++] = a[j++];
+        else tmp[o++] = a[i++];
+    }
+    while (i < h) tmp[o++] = a[i++];
+    while (j < n) tmp[o++] = a[j++];
+    memcpy(a, tmp, n * sizeof(uint16_t));
+}
+
+void orc_bitmap_query(const float *q, uint64_t nq, uint32_t d, uint32_t ntop, uint64_t *out) {
+    uint32_t qpv = d / 64;
+    uint16_t *idx = (uint16_t *)malloc(d * sizeof(uint16_t));
+    uint16_t *tmp = (uint16_t *)malloc(d * sizeof(uint16_t));
+    for (uint64_t r = 0; r < nq; r++) {
+        const float *v = q + r * d;
+        uint64_t *o = out + r * qpv;
+        for (uint32_t i = 0; i < d; i++) idx[i] = (uint16_t)i;
+        msort_desc(idx, tmp, d, v);
+        memset(o, 0, qpv * 8);
+        for (uint32_t t = 0; t < ntop; t++) o[idx[t] / 64] |= 1ull << (idx[t] % 64);
+    }
+    free(idx);
+    free(tmp);
+}
+
+/* src/util.rs and_popcount over rows: out[q*n + i] = popcount(doc_i AND q). */
+void orc_and_popcount_all(const uint64_t *bm, uint64_t n, uint32_t qpv, const uint64_t *qb, uint64_t nq,
+                          uint32_t *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < (int64_t)nq; q++)
+        for (uint64_t i = 0; i < n; i++) {
+            uint32_t c = 0;
+            for (uint32_t w = 0; w < qpv; w++) c += (uint32_t)__builtin_popcountll(bm[i * qpv + w] & qb[q * qpv + w]);
+            out[q * n + i] = c;
+        }
+}
+
+/* ------------------------------------------------------------------ */
 /* Score gradient (the reference has no backward pass)                  */
 /* ------------------------------------------------------------------ */
 

@@ -12,6 +12,7 @@ from .index import (  # noqa: E402,F401
     MAX_DIM,
     MAX_SIGN_BITMAP_DIM,
     MAX_VECTORS,
+    Bitmap,
     RankQuant,
     RankQuantFastscan,
     SignBitmap,
@@ -27,6 +28,7 @@ except ImportError:  # pragma: no cover
 
 __all__ = [
     "RankQuant",
+    "Bitmap",
     "SignBitmap",
     "RankQuantFastscan",
     "two_stage_search",

@@ -153,6 +153,18 @@ SIGNATURES = {
     "ordvec_sign_bitmap_load": (ST, [C.c_char_p, C.POINTER(P)]),
     "ordvec_sign_bitmap_free": (None, [P]),
     "ordvec_two_stage_search": (ST, [P, P, P, U64, U64, U64, P, P, U64]),
+    "ordvec_bitmap_new": (ST, [U64, U64, C.POINTER(P)]),
+    "ordvec_bitmap_add": (ST, [P, P, U64, U64]),
+    "ordvec_bitmap_info": (ST, [P, P, P, P]),
+    "ordvec_bitmap_search": (ST, [P, P, U64, U64, P, P, U64]),
+    "ordvec_bitmap_top_m": (ST, [P, P, U64, U64, P, U64]),
+    "ordvec_bitmap_search_subset_batched": (ST, [P, P, U64, P, P, U64, U64, P, P, P, U64]),
+    "ordvec_bitmap_query_bits": (ST, [P, P, U64, P, U64]),
+    "ordvec_bitmap_overlap_subset": (ST, [P, P, P, U64, P]),
+    "ordvec_bitmap_words": (ST, [P, P, U64]),
+    "ordvec_bitmap_write": (ST, [P, C.c_char_p]),
+    "ordvec_bitmap_load": (ST, [C.c_char_p, C.POINTER(P)]),
+    "ordvec_bitmap_free": (None, [P]),
     "ordvec_fastscan_new": (ST, [U64, C.POINTER(P)]),
     "ordvec_fastscan_add": (ST, [P, P, U64, U64]),
     "ordvec_fastscan_search": (ST, [P, P, U64, U64, P, P, U64]),

@@ -232,8 +232,10 @@ struct RqIndex {
     }
 };
 
+// Sign bitmap (ntop == 0) or top-bucket Bitmap (ntop > 0): same interleaved
+// storage, different per-word statistic.
 struct SbIndex {
-    uint32_t dim = 0, qpv = 0;
+    uint32_t dim = 0, qpv = 0, ntop = 0;
     uint64_t n = 0, cap_blocks = 0;
     uint64_t* words = nullptr;  // interleaved [block][word][64]
     ~SbIndex() {
@@ -270,10 +272,15 @@ struct FsIndex {
     uint64_t payload() const { return ((n + 31) / 32) * (uint64_t)(dim / 2) * 32; }
 };
 
+// ABI v1 handle: a RankQuant (.ovrq) or a top-bucket Bitmap (.ovbm).
 struct IndexHandle {
     RqIndex* rq = nullptr;
+    SbIndex* bm = nullptr;
     uint64_t source_file_size_bytes = 0;
-    ~IndexHandle() { delete rq; }
+    ~IndexHandle() {
+        delete rq;
+        delete bm;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -454,7 +461,7 @@ Stage1Bufs stage1_dev(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t 
     void* base = cv.take<char>(stage1_fast_bytes(b.plan, nq));
     carve_stage1_fast(b.plan, nq, base, &b.work);
     launch_stage1_fast(ix->words, ix->n, ix->dim, dq, nq, m_eff, b.plan, b.work, b.qbits, cand, cand_h, qstat,
-                       cur().stream);
+                       ix->ntop, cur().stream);
     return b;
 }
 
@@ -548,6 +555,125 @@ void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, cons
     finish_status(qstat, nq, flags, kNonfiniteQuery, "queries");
 }
 
+// Top-bucket Bitmap add (src/bitmap.rs:140-162): rank each row, set the bits
+// of its n_top highest-ranked coordinates.
+void bm_add(SbIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
+    if (ix->dim > kEncodeMaxDim) fail(kUnsupportedOperation, "GPU Bitmap encode supports dim <= " + std::to_string(kEncodeMaxDim));
+    check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
+    check_new_count(ix->n, n);
+    ix->reserve(ix->n + n);
+    uint64_t base = ix->n;
+    for_each_chunk(vectors, n, ix->dim, flags, [&](const float* dv, uint64_t r0, uint64_t cnt) {
+        launch_topbits_docs(dv, cnt, ix->dim, ix->ntop, ix->words, base + r0, cur().stream);
+    });
+    ix->n += n;
+    if (!(flags & ORDVEC_FLAG_DEVICE_PTRS)) sync();
+}
+
+void check_bitmap_query_dim(const SbIndex* ix) {
+    if (ix->dim > kEncodeMaxDim) fail(kUnsupportedOperation, "GPU Bitmap search supports dim <= " + std::to_string(kEncodeMaxDim));
+}
+
+void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint32_t>& fl, uint64_t m_eff,
+                uint32_t* cand, uint32_t* cand_h);
+
+// Exact top-m by (distance, id) for a sign bitmap (Hamming) or a Bitmap
+// (n_top - overlap): src/sign_bitmap.rs:231-483, src/bitmap.rs:264-362.
+// Outputs nq x m_eff ids and, when requested, distances.
+void sb_top_m(const SbIndex* ix, const float* queries, uint64_t nq, uint64_t m_eff, uint32_t* out_c, uint32_t* out_h,
+              uint64_t flags) {
+    check_m(m_eff);
+    if (ix->ntop) check_bitmap_query_dim(ix);
+    else if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
+    host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    size_t bytes = stage1_bytes(ix, nq, m_eff) + al(nq * 4) + (dev ? 0 : al(nq * ix->dim * 4) + 2 * al(nq * m_eff * 4));
+    Carver cv{static_cast<char*>(cur().workspace(bytes))};
+    const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
+    uint32_t* qstat = cv.take<uint32_t>(nq);
+    uint32_t* dc = dev ? out_c : cv.take<uint32_t>(nq * m_eff);
+    uint32_t* dh = dev ? out_h : (out_h ? cv.take<uint32_t>(nq * m_eff) : nullptr);
+    Stage1Bufs b = stage1_dev(ix, dq, nq, m_eff, dc, dh, qstat, cv);
+    std::vector<uint32_t> st = read_qstat(qstat, nq);
+    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
+        fail(kNonfiniteQuery, "queries contain NaN or infinity");
+    std::vector<uint32_t> fl = flagged(st, kStatFallback);
+    if (!fl.empty()) stage1_fix(ix, b.qbits, fl, m_eff, dc, dh);
+    if (!dev) {
+        d2h(out_c, dc, nq * m_eff * 4);
+        if (out_h) d2h(out_h, dh, nq * m_eff * 4);
+    }
+    sync();
+}
+
+// Bitmap::search (src/bitmap.rs:202-250): top-k_eff by (overlap desc, id asc),
+// scores = overlap as f32.  k_eff = min(k, n) > 0.
+void bm_search(const SbIndex* ix, const float* queries, uint64_t nq, uint64_t k_eff, float* out_s, int64_t* out_i,
+               uint64_t flags) {
+    if (k_eff > kStage1MaxM) fail(kUnsupportedOperation, "GPU Bitmap search supports k <= " + std::to_string(kStage1MaxM));
+    check_bitmap_query_dim(ix);
+    host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    size_t bytes = stage1_bytes(ix, nq, k_eff) + al(nq * 4) + 2 * al(nq * k_eff * 4) +
+                   (dev ? 0 : al(nq * ix->dim * 4) + al(nq * k_eff * 4) + al(nq * k_eff * 8));
+    Carver cv{static_cast<char*>(cur().workspace(bytes))};
+    const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
+    uint32_t* qstat = cv.take<uint32_t>(nq);
+    uint32_t* c = cv.take<uint32_t>(nq * k_eff);
+    uint32_t* h = cv.take<uint32_t>(nq * k_eff);
+    float* ds = dev ? out_s : cv.take<float>(nq * k_eff);
+    int64_t* di = dev ? out_i : cv.take<int64_t>(nq * k_eff);
+    Stage1Bufs b = stage1_dev(ix, dq, nq, k_eff, c, h, qstat, cv);
+    std::vector<uint32_t> st = read_qstat(qstat, nq);
+    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
+        fail(kNonfiniteQuery, "queries contain NaN or infinity");
+    std::vector<uint32_t> fl = flagged(st, kStatFallback);
+    if (!fl.empty()) stage1_fix(ix, b.qbits, fl, k_eff, c, h);
+    launch_bm_finish(c, h, nq * k_eff, ix->ntop, ds, di, cur().stream);
+    if (!dev) {
+        d2h(out_s, ds, nq * k_eff * 4);
+        d2h(out_i, di, nq * k_eff * 8);
+    }
+    sync();
+}
+
+// Bitmap::search_subset per query (src/bitmap.rs:477-505) over CSR rows.
+void bm_subset(const SbIndex* ix, const float* queries, uint64_t nq, const uint64_t* offsets, const uint32_t* rows,
+               uint64_t n_rows, uint64_t out_k, float* out_s, int64_t* out_i, uint64_t flags) {
+    if (nq == 0 || out_k == 0) return;
+    check_k(out_k);
+    check_bitmap_query_dim(ix);
+    host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    size_t bytes = al(nq * ix->qpv * 8) + al(nq * 4) +
+                   (dev ? 0 : al(nq * ix->dim * 4) + al((nq + 1) * 8) + al(n_rows * 4) + al(nq * out_k * 12));
+    Carver cv{static_cast<char*>(cur().workspace(bytes))};
+    const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
+    uint64_t* qb = cv.take<uint64_t>(nq * ix->qpv);
+    uint32_t* qstat = cv.take<uint32_t>(nq);
+    const uint64_t* doff = offsets;
+    const uint32_t* dr = rows;
+    float* ds = out_s;
+    int64_t* di = out_i;
+    if (!dev) {
+        uint64_t* t = cv.take<uint64_t>(nq + 1);
+        h2d(t, offsets, (nq + 1) * 8);
+        doff = t;
+        uint32_t* tr = cv.take<uint32_t>(n_rows);
+        h2d(tr, rows, n_rows * 4);
+        dr = tr;
+        ds = cv.take<float>(nq * out_k);
+        di = cv.take<int64_t>(nq * out_k);
+    }
+    launch_bitmap_query_bits(dq, nq, ix->dim, ix->ntop, qb, qstat, cur().stream);
+    launch_bm_subset(ix->words, ix->dim, ix->n, qb, nq, doff, dr, ix->ntop, (uint32_t)out_k, ds, di, cur().stream);
+    if (!dev) {
+        d2h(out_s, ds, nq * out_k * 4);
+        d2h(out_i, di, nq * out_k * 8);
+    }
+    finish_status(qstat, nq, flags, kNonfiniteQuery, "queries");
+}
+
 // Exact recomputation of flagged queries (histogram pipeline, workspace 2);
 // rewrites their rows of cand / cand_h in place.
 void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint32_t>& fl, uint64_t m_eff,
@@ -562,7 +688,8 @@ void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint
     carve_stage1(plan, nf, m_eff, cv.take<char>(stage1_work_bytes(plan, nf, m_eff)), &w);
     uint32_t* cc = cv.take<uint32_t>(nf * m_eff);
     uint32_t* ch = cv.take<uint32_t>(nf * m_eff);
-    launch_stage1(ix->words, ix->n, ix->dim, qb, nf, m_eff, plan, w, cc, cand_h ? ch : nullptr, cur().stream);
+    launch_stage1(ix->words, ix->n, ix->dim, qb, nf, m_eff, plan, w, cc, cand_h ? ch : nullptr, ix->ntop,
+                  cur().stream);
     for (uint64_t i = 0; i < nf; i++) {
         d2d(cand + (uint64_t)fl[i] * m_eff, cc + i * m_eff, m_eff * 4);
         if (cand_h) d2d(cand_h + (uint64_t)fl[i] * m_eff, ch + i * m_eff, m_eff * 4);
@@ -1011,6 +1138,57 @@ void write_ovsb(const SbIndex* ix, const char* path) {
     write_file(path, h, rows.data(), rows.size() * 8);
 }
 
+// src/rank_io.rs:826-957 (.ovbm): "OVBM"/"TVBM" | 1 | dim u32 | n_top u32 |
+// n u32 | u64 LE words row-major; every row must have exactly n_top bits set.
+SbIndex* load_ovbm(const char* path) {
+    Reader r;
+    r.buf = read_file(path);
+    if (!magic_is(r.buf, "OVBM", "TVBM")) fail(kCorruptIndex, "not an OVBM/TVBM (Bitmap) file: wrong magic");
+    r.label = "OVBM";
+    r.pos = 4;
+    r.version();
+    uint32_t dim = r.u32("dim");
+    if (dim < 2 || dim > kMaxDim) fail(kCorruptIndex, "dim out of range [2, 65535]");
+    if (dim % 64) fail(kCorruptIndex, "OVBM dim " + std::to_string(dim) + " is not a multiple of 64");
+    uint32_t n_top = r.u32("n_top");
+    if (n_top == 0 || n_top >= dim)
+        fail(kCorruptIndex, "OVBM n_top " + std::to_string(n_top) + " must satisfy 0 < n_top < dim (" + std::to_string(dim) + ")");
+    uint32_t n = r.u32("n_vectors");
+    if (n > kMaxVectors) fail(kCorruptIndex, "n_vectors exceeds MAX_VECTORS");
+    const uint32_t qpv = dim / 64;
+    r.payload_exact((uint64_t)n * qpv * 8);
+    std::vector<uint64_t> rows((uint64_t)n * qpv);
+    std::memcpy(rows.data(), r.buf.data() + r.pos, rows.size() * 8);  // little-endian host
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t pop = 0;
+        for (uint32_t w = 0; w < qpv; w++) pop += (uint32_t)__builtin_popcountll(rows[i * qpv + w]);
+        if (pop != n_top)
+            fail(kCorruptIndex, "OVBM row " + std::to_string(i) + " has " + std::to_string(pop) +
+                                    " bits set, expected n_top = " + std::to_string(n_top));
+    }
+    eng().init();
+    SbIndex* ix = new SbIndex();
+    ix->dim = dim;
+    ix->qpv = qpv;
+    ix->ntop = n_top;
+    try {
+        sb_upload_rowmajor(ix, rows.data(), n);
+    } catch (...) {
+        delete ix;
+        throw;
+    }
+    return ix;
+}
+
+void write_ovbm(const SbIndex* ix, const char* path) {
+    std::vector<uint64_t> rows = sb_download_rowmajor(ix);
+    std::vector<uint8_t> h = {'O', 'V', 'B', 'M', 1};
+    put_u32(h, ix->dim);
+    put_u32(h, ix->ntop);
+    put_u32(h, (uint32_t)ix->n);
+    write_file(path, h, rows.data(), rows.size() * 8);
+}
+
 // src/rank_io.rs:1077-1250 (.ovfs): header, nibble domain, b=2 composition,
 // zero tail padding.
 FsIndex* load_ovfs(const char* path) {
@@ -1181,6 +1359,7 @@ struct ordvec_rankquant_t {};
 struct ordvec_sign_bitmap_t {};
 struct ordvec_fastscan_t {};
 struct ordvec_server_t {};
+struct ordvec_bitmap_t {};
 
 static IndexHandle* H(const ordvec_index_t* p) { return reinterpret_cast<IndexHandle*>(const_cast<ordvec_index_t*>(p)); }
 static RqIndex* RQ(const ordvec_rankquant_t* p) { return reinterpret_cast<RqIndex*>(const_cast<ordvec_rankquant_t*>(p)); }
@@ -1256,7 +1435,16 @@ ordvec_status_t ordvec_index_load(const char* path, uint64_t flags, ordvec_index
             h->source_file_size_bytes = fsz;
             *out = reinterpret_cast<ordvec_index_t*>(h);
         } else if (magic_is(head, "OVBM", "TVBM")) {
-            fail(kUnsupportedFormat, "Bitmap (.ovbm) indexes are not served by the GPU engine in this version");
+            LOCK;
+            IndexHandle* h = new IndexHandle();
+            try {
+                h->bm = load_ovbm(path);
+            } catch (...) {
+                delete h;
+                throw;
+            }
+            h->source_file_size_bytes = fsz;
+            *out = reinterpret_cast<ordvec_index_t*>(h);
         } else if (magic_is(head, "OVR1", "TVR1") || magic_is(head, "OVSB", "TVSB")) {
             fail(kUnsupportedFormat, "ABI v1 supports only RankQuant and Bitmap indexes");
         } else if (magic_is(head, "OVFS", nullptr)) {
@@ -1335,12 +1523,20 @@ ordvec_status_t ordvec_index_info(const ordvec_index_t* index, ordvec_index_info
         check_exact_size(info_out->struct_size, sizeof(ordvec_index_info_t), "ordvec_index_info_t");
         const IndexHandle* h = H(index);
         ordvec_index_info_t info = default_info();
-        info.kind = ORDVEC_INDEX_KIND_RANK_QUANT;
         info.format_version = 1;
-        info.dim = h->rq->dim;
-        info.bit_width = h->rq->bits;
-        info.vector_count = h->rq->n;
-        info.bytes_per_vec = h->rq->bpv;
+        if (h->bm) {
+            info.kind = ORDVEC_INDEX_KIND_BITMAP;
+            info.dim = h->bm->dim;
+            info.n_top = h->bm->ntop;
+            info.vector_count = h->bm->n;
+            info.bytes_per_vec = h->bm->dim / 8;
+        } else {
+            info.kind = ORDVEC_INDEX_KIND_RANK_QUANT;
+            info.dim = h->rq->dim;
+            info.bit_width = h->rq->bits;
+            info.vector_count = h->rq->n;
+            info.bytes_per_vec = h->rq->bpv;
+        }
         info.source_file_size_bytes = h->source_file_size_bytes;
         info.capabilities = kAllCaps;
         *info_out = info;
@@ -1362,6 +1558,8 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t* index, const ordvec_se
         NONNULL(params, "params");
         const IndexHandle* h = H(index);
         const RqIndex* ix = h->rq;
+        const SbIndex* bm = h->bm;
+        const uint64_t idx_dim = bm ? bm->dim : ix->dim, idx_n = bm ? bm->n : ix->n;
         // Validation order: ordvec-ffi/src/lib.rs:444-568.
         check_exact_size(params->struct_size, sizeof(ordvec_search_params_t), "ordvec_search_params_t");
         if (params->flags != ORDVEC_SEARCH_FLAG_NONE) fail(kBadArgument, "unknown search flags: " + std::to_string(params->flags));
@@ -1370,9 +1568,9 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t* index, const ordvec_se
         NONNULL(returned_out, "returned_out");
         if (stats_out) check_exact_size(stats_out->struct_size, sizeof(ordvec_search_stats_t), "ordvec_search_stats_t");
         NONNULL(params->query, "query");
-        if (params->dim != ix->dim)
-            fail(kDimMismatch, "query dim " + std::to_string(params->dim) + " does not match index dim " + std::to_string(ix->dim));
-        if (!host_all_finite(params->query, ix->dim)) fail(kNonfiniteQuery, "query contains NaN or infinity");
+        if (params->dim != idx_dim)
+            fail(kDimMismatch, "query dim " + std::to_string(params->dim) + " does not match index dim " + std::to_string(idx_dim));
+        if (!host_all_finite(params->query, idx_dim)) fail(kNonfiniteQuery, "query contains NaN or infinity");
         const uint32_t* rows = nullptr;
         uint64_t count = params->candidate_count;
         if (count == 0 && params->candidate_rows) fail(kBadArgument, "candidate_count is zero but candidate_rows is non-NULL");
@@ -1380,11 +1578,11 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t* index, const ordvec_se
         if (count) {
             rows = params->candidate_rows;
             for (uint64_t i = 0; i < count; i++)
-                if (rows[i] >= ix->n)
+                if (rows[i] >= idx_n)
                     fail(kRowIdOutOfRange, "candidate row " + std::to_string(rows[i]) + " is out of range for " +
-                                               std::to_string(ix->n) + " vectors");
+                                               std::to_string(idx_n) + " vectors");
         }
-        const uint64_t space = rows ? count : ix->n;
+        const uint64_t space = rows ? count : idx_n;
         const uint64_t required = std::min<uint64_t>(params->k, space);
         if (required > 0) {
             NONNULL(hits_out, "hits_out");
@@ -1395,7 +1593,7 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t* index, const ordvec_se
         uint64_t t_prep = now_ns(), t_score = t_prep, t_sel = t_prep;
         if (required > 0) {
             // search_asymmetric_subset asserts rows <= n_vectors (src/quant.rs:1057-1063).
-            if (rows && count > ix->n)
+            if (!bm && rows && count > ix->n)
                 throw std::runtime_error("search_asymmetric_subset: candidate list length " + std::to_string(count) +
                                          " exceeds n_vectors " + std::to_string(ix->n) + "; deduplicate repeated ids before calling");
             std::vector<float> sc(required);
@@ -1404,8 +1602,14 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t* index, const ordvec_se
                 LOCK;
                 eng().init();
                 t_prep = now_ns();
-                if (rows) {
-                    uint64_t off[2] = {0, count};
+                uint64_t off[2] = {0, count};
+                if (bm && rows) {
+                    // Bitmap::search_subset (ordvec-ffi/src/lib.rs:978-980).
+                    bm_subset(bm, params->query, 1, off, rows, count, required, sc.data(), id.data(), 0);
+                } else if (bm) {
+                    // Bitmap::search (ordvec-ffi/src/lib.rs:974-977).
+                    bm_search(bm, params->query, 1, required, sc.data(), id.data(), 0);
+                } else if (rows) {
                     rq_subset_search(ix, params->query, 1, off, rows, count, 0, required, sc.data(), id.data(), 0);
                 } else {
                     rq_full_search(ix, params->query, 1, required, sc.data(), id.data(), 0);
@@ -1424,12 +1628,13 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t* index, const ordvec_se
         *returned_out = required;
         if (stats_out) {
             ordvec_search_stats_t st = default_stats();
-            st.kind = ORDVEC_INDEX_KIND_RANK_QUANT;
-            st.dim = ix->dim;
-            st.bit_width = ix->bits;
+            st.kind = bm ? ORDVEC_INDEX_KIND_BITMAP : ORDVEC_INDEX_KIND_RANK_QUANT;
+            st.dim = idx_dim;
+            st.bit_width = bm ? 0 : ix->bits;
+            st.n_top = bm ? bm->ntop : 0;
             st.k = params->k;
             st.user_tag = params->user_tag;
-            st.vector_count = ix->n;
+            st.vector_count = idx_n;
             st.candidate_count = space;
             st.returned_count = required;
             st.total_ns = now_ns() - t0;
@@ -1437,7 +1642,7 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t* index, const ordvec_se
             st.score_ns = t_score - t_prep;
             st.select_ns = t_sel - t_score;
             st.vectors_scored = space;
-            st.bytes_read = required ? space * ix->bpv + (uint64_t)ix->dim * 4 : 0;
+            st.bytes_read = required ? space * (bm ? bm->dim / 8 : ix->bpv) + idx_dim * 4 : 0;
             *stats_out = st;
         }
     });
@@ -1779,29 +1984,8 @@ ordvec_status_t ordvec_sign_bitmap_top_m(const ordvec_sign_bitmap_t* index, cons
         if (nq == 0 || m_eff == 0) return;
         NONNULL(queries, "queries");
         NONNULL(out_candidates, "out_candidates");
-        check_m(m_eff);
-        if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
-        host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
         LOCK;
-        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-        size_t bytes = stage1_bytes(ix, nq, m_eff) + al(nq * 4) +
-                       (dev ? 0 : al(nq * ix->dim * 4) + 2 * al(nq * m_eff * 4));
-        Carver cv{static_cast<char*>(cur().workspace(bytes))};
-        const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
-        uint32_t* qstat = cv.take<uint32_t>(nq);
-        uint32_t* dc = dev ? out_candidates : cv.take<uint32_t>(nq * m_eff);
-        uint32_t* dh = dev ? out_hamming : (out_hamming ? cv.take<uint32_t>(nq * m_eff) : nullptr);
-        Stage1Bufs b = stage1_dev(ix, dq, nq, m_eff, dc, dh, qstat, cv);
-        std::vector<uint32_t> st = read_qstat(qstat, nq);
-        if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
-            fail(kNonfiniteQuery, "queries contain NaN or infinity");
-        std::vector<uint32_t> fl = flagged(st, kStatFallback);
-        if (!fl.empty()) stage1_fix(ix, b.qbits, fl, m_eff, dc, dh);
-        if (!dev) {
-            d2h(out_candidates, dc, nq * m_eff * 4);
-            if (out_hamming) d2h(out_hamming, dh, nq * m_eff * 4);
-        }
-        sync();
+        sb_top_m(ix, queries, nq, m_eff, out_candidates, out_hamming, flags);
     });
 }
 
@@ -1845,6 +2029,195 @@ void ordvec_sign_bitmap_free(ordvec_sign_bitmap_t* index) {
     if (!index) return;
     LOCK;
     delete SB(index);
+}
+
+// ---- Bitmap (top-bucket, AND-popcount) ------------------------------------------
+
+static SbIndex* BM(const ordvec_bitmap_t* p) { return reinterpret_cast<SbIndex*>(const_cast<ordvec_bitmap_t*>(p)); }
+
+ordvec_status_t ordvec_bitmap_new(uint64_t dim, uint64_t n_top, ordvec_bitmap_t** out) {
+    return boundary([&] {
+        NONNULL(out, "out");
+        *out = nullptr;
+        // Bitmap::validate_params (src/bitmap.rs:71-103)
+        if (dim == 0) fail(kBadArgument, "dim must be > 0");
+        if (dim % 64) fail(kBadArgument, "dim must be a multiple of 64");
+        if (dim > kMaxDim) fail(kBadArgument, "dim must be <= 65535 (u16 rank invariant)");
+        if (!(n_top > 0 && n_top < dim)) fail(kBadArgument, "n_top must satisfy 0 < n_top < dim");
+        LOCK;
+        eng().init();
+        SbIndex* ix = new SbIndex();
+        ix->dim = (uint32_t)dim;
+        ix->qpv = (uint32_t)(dim / 64);
+        ix->ntop = (uint32_t)n_top;
+        *out = reinterpret_cast<ordvec_bitmap_t*>(ix);
+    });
+}
+
+ordvec_status_t ordvec_bitmap_add(ordvec_bitmap_t* index, const float* vectors, uint64_t n, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        if (n == 0) return;
+        NONNULL(vectors, "vectors");
+        LOCK;
+        bm_add(BM(index), vectors, n, flags);
+    });
+}
+
+ordvec_status_t ordvec_bitmap_info(const ordvec_bitmap_t* index, uint64_t* dim, uint64_t* n_top, uint64_t* n) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        if (dim) *dim = BM(index)->dim;
+        if (n_top) *n_top = BM(index)->ntop;
+        if (n) *n = BM(index)->n;
+    });
+}
+
+ordvec_status_t ordvec_bitmap_search(const ordvec_bitmap_t* index, const float* queries, uint64_t nq, uint64_t k,
+                                     float* out_scores, int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = BM(index);
+        const uint64_t k_eff = std::min<uint64_t>(k, ix->n);
+        if (nq == 0 || k_eff == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        LOCK;
+        bm_search(ix, queries, nq, k_eff, out_scores, out_ids, flags);
+    });
+}
+
+ordvec_status_t ordvec_bitmap_top_m(const ordvec_bitmap_t* index, const float* queries, uint64_t nq, uint64_t m,
+                                    uint32_t* out_candidates, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = BM(index);
+        const uint64_t m_eff = std::min<uint64_t>(m, ix->n);
+        if (nq == 0 || m_eff == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_candidates, "out_candidates");
+        LOCK;
+        sb_top_m(ix, queries, nq, m_eff, out_candidates, nullptr, flags);
+    });
+}
+
+ordvec_status_t ordvec_bitmap_search_subset_batched(const ordvec_bitmap_t* index, const float* queries, uint64_t nq,
+                                                    const uint64_t* offsets, const uint32_t* rows, uint64_t n_rows,
+                                                    uint64_t k, float* out_scores, int64_t* out_ids,
+                                                    uint64_t* out_counts, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = BM(index);
+        NONNULL(offsets, "offsets");
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        if (dev) fail(kUnsupportedOperation, "Bitmap subset search takes host CSR arrays");
+        if (offsets[0] != 0) fail(kBadArgument, "offsets[0] must be 0");
+        if (offsets[nq] != n_rows) fail(kBadArgument, "offsets[nq] must equal the number of rows");
+        for (uint64_t q = 0; q < nq; q++)
+            if (offsets[q + 1] < offsets[q]) fail(kBadArgument, "offsets must be monotonic non-decreasing");
+        for (uint64_t i = 0; i < n_rows; i++)
+            if (rows[i] >= ix->n) fail(kRowIdOutOfRange, "doc_id out of range [0, " + std::to_string(ix->n) + ")");
+        // Per query k_eff = min(k, row count); rows are padded to k with -inf / -1.
+        if (out_counts)
+            for (uint64_t q = 0; q < nq; q++) out_counts[q] = std::min<uint64_t>(k, offsets[q + 1] - offsets[q]);
+        if (nq == 0 || k == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        if (n_rows) NONNULL(rows, "rows");
+        LOCK;
+        bm_subset(ix, queries, nq, offsets, rows, n_rows, k, out_scores, out_ids, flags);
+    });
+}
+
+ordvec_status_t ordvec_bitmap_query_bits(const ordvec_bitmap_t* index, const float* queries, uint64_t nq,
+                                         uint64_t* out_words, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = BM(index);
+        if (nq == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_words, "out_words");
+        check_bitmap_query_dim(ix);
+        host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
+        LOCK;
+        eng().init();
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        size_t bytes = al(nq * 4) + (dev ? 0 : al(nq * ix->dim * 4) + al(nq * ix->qpv * 8));
+        Carver cv{static_cast<char*>(cur().workspace(bytes))};
+        const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
+        uint32_t* qstat = cv.take<uint32_t>(nq);
+        uint64_t* qb = dev ? out_words : cv.take<uint64_t>(nq * ix->qpv);
+        launch_bitmap_query_bits(dq, nq, ix->dim, ix->ntop, qb, qstat, cur().stream);
+        if (!dev) d2h(out_words, qb, nq * ix->qpv * 8);
+        finish_status(qstat, nq, flags, kNonfiniteQuery, "queries");
+    });
+}
+
+ordvec_status_t ordvec_bitmap_overlap_subset(const ordvec_bitmap_t* index, const uint64_t* q_words,
+                                             const uint32_t* rows, uint64_t n_rows, uint32_t* out) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = BM(index);
+        if (n_rows == 0) return;
+        NONNULL(q_words, "q_words");
+        NONNULL(rows, "rows");
+        NONNULL(out, "out");
+        for (uint64_t i = 0; i < n_rows; i++)
+            if (rows[i] >= ix->n)
+                fail(kRowIdOutOfRange, "body_overlap_scores_subset: doc_id out of range [0, " + std::to_string(ix->n) + ")");
+        LOCK;
+        eng().init();
+        Carver cv{static_cast<char*>(cur().workspace(al(ix->qpv * 8) + 2 * al(n_rows * 4)))};
+        uint64_t* dq = cv.take<uint64_t>(ix->qpv);
+        uint32_t* dr = cv.take<uint32_t>(n_rows);
+        uint32_t* dout = cv.take<uint32_t>(n_rows);
+        h2d(dq, q_words, ix->qpv * 8);
+        h2d(dr, rows, n_rows * 4);
+        launch_bm_overlap_rows(ix->words, ix->dim, dq, dr, n_rows, dout, cur().stream);
+        d2h(out, dout, n_rows * 4);
+        sync();
+    });
+}
+
+ordvec_status_t ordvec_bitmap_words(const ordvec_bitmap_t* index, uint64_t* out, uint64_t capacity) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = BM(index);
+        uint64_t cnt = ix->n * ix->qpv;
+        if (capacity < cnt) fail(kBufferTooSmall, "words buffer too small");
+        if (!cnt) return;
+        NONNULL(out, "out");
+        LOCK;
+        std::vector<uint64_t> rows = sb_download_rowmajor(ix);
+        std::memcpy(out, rows.data(), cnt * 8);
+    });
+}
+
+ordvec_status_t ordvec_bitmap_write(const ordvec_bitmap_t* index, const char* path) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        NONNULL(path, "path");
+        LOCK;
+        write_ovbm(BM(index), path);
+    });
+}
+
+ordvec_status_t ordvec_bitmap_load(const char* path, ordvec_bitmap_t** out) {
+    return boundary([&] {
+        NONNULL(path, "path");
+        NONNULL(out, "out");
+        *out = nullptr;
+        LOCK;
+        *out = reinterpret_cast<ordvec_bitmap_t*>(load_ovbm(path));
+    });
+}
+
+void ordvec_bitmap_free(ordvec_bitmap_t* index) {
+    if (!index) return;
+    LOCK;
+    delete BM(index);
 }
 
 ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const ordvec_rankquant_t* rq,

@@ -101,6 +101,22 @@ void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, u
                         hipStream_t s);
 void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, hipStream_t s);
 void launch_fastscan_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint8_t* packed, hipStream_t s);
+// Top-bucket bitmaps of documents (interleaved layout), d <= kEncodeMaxDim.
+void launch_topbits_docs(const float* x, uint64_t n, uint32_t d, uint32_t ntop, uint64_t* out, uint64_t first_doc,
+                         hipStream_t s);
+// build_query_bitmap_fp32 for each query (qstat bit 0: non-finite input).
+void launch_bitmap_query_bits(const float* queries, uint64_t nq, uint32_t d, uint32_t ntop, uint64_t* qbits,
+                              uint32_t* qstat, hipStream_t s);
+// Bitmap subset search: CSR rows per query -> top-k by (overlap desc, row asc).
+void launch_bm_subset(const uint64_t* words, uint32_t d, uint64_t n_docs, const uint64_t* qbits, uint64_t nq,
+                      const uint64_t* offsets, const uint32_t* rows, uint32_t ntop, uint32_t out_k, float* out_s,
+                      int64_t* out_i, hipStream_t s);
+// Overlap of listed rows (rows < n) with one query bitmap.
+void launch_bm_overlap_rows(const uint64_t* words, uint32_t d, const uint64_t* q, const uint32_t* rows, uint64_t count,
+                            uint32_t* out, hipStream_t s);
+// (id, distance) rows -> (overlap as f32, i64 id).
+void launch_bm_finish(const uint32_t* cand, const uint32_t* cand_h, uint64_t count, uint32_t ntop, float* out_s,
+                      int64_t* out_i, hipStream_t s);
 
 // Stage 1: exact top-m by (hamming asc, id asc) for nq queries over the
 // interleaved sign corpus.  Writes cand[nq][m_eff] (sorted) and, when
@@ -121,7 +137,11 @@ Stage1Plan plan_stage1(uint64_t n, uint32_t d, uint64_t nq);
 size_t stage1_work_bytes(const Stage1Plan& p, uint64_t nq, uint64_t m_eff);
 void carve_stage1(const Stage1Plan& p, uint64_t nq, uint64_t m_eff, void* base, Stage1Work* w);
 void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t* qbits, uint64_t nq, uint64_t m_eff,
-                   const Stage1Plan& p, const Stage1Work& w, uint32_t* cand, uint32_t* cand_h, hipStream_t s);
+                   const Stage1Plan& p, const Stage1Work& w, uint32_t* cand, uint32_t* cand_h, uint32_t ntop,
+                   hipStream_t s);
+// ntop == 0: sign bitmaps, distance = popcount(d XOR q); ntop > 0: top-bucket
+// bitmaps, distance = ntop - popcount(d AND q) (queries then get their top-ntop
+// bits in the fast path's prep).
 
 // Stage 1 fast path (sampled threshold + survivor compaction); queries whose
 // survivor buffer cannot prove exactness get qstat bit 1 and are recomputed
@@ -145,7 +165,7 @@ size_t stage1_fast_bytes(const FastPlan& p, uint64_t nq);
 void carve_stage1_fast(const FastPlan& p, uint64_t nq, void* base, FastWork* w);
 void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
                         uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
-                        uint32_t* cand_h, uint32_t* qstat, hipStream_t s);
+                        uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s);
 
 // Stage 2 / subset rerank over candidate rows.  Rows are CSR (offsets != null)
 // or uniform (row q = cands[q*row_m .. (q+1)*row_m)).  Output nq x out_k.

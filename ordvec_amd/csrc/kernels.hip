@@ -291,16 +291,29 @@ __global__ void k_fastscan_pack(const uint8_t* __restrict__ codes, uint64_t n, u
 // Stage 1: XOR-popcount scan + exact top-m (src/sign_bitmap.rs:231-483)
 // ---------------------------------------------------------------------------
 
+// Per-word statistic of the candidate scans.  OP 0: popcount(d XOR q), the
+// sign-bitmap Hamming distance (src/util.rs:192-238).  OP 1: popcount(d AND q),
+// the top-bucket bitmap overlap (src/bitmap.rs:607-1000), reported as the
+// distance n_top - overlap so both kinds select ascending with ties by id.
+template <int OP>
+__device__ __forceinline__ uint32_t word_stat(uint64_t d, uint64_t q) {
+    return (uint32_t)__popcll(OP ? (d & q) : (d ^ q));
+}
+template <int OP>
+__device__ __forceinline__ uint32_t to_dist(uint32_t acc, uint32_t ntop) {
+    return OP ? ntop - acc : acc;
+}
+
 // K1: grid (query tiles, doc segments).  Each wave scores one 64-doc block per
 // step against QT queries: doc words are 512-byte coalesced loads, query
 // words are wave-uniform (scalar loads).  Writes H[q][doc] (u16) and a
 // per-(query, segment) Hamming histogram kept in LDS as packed 16-bit bins.
-template <int QT, int WCH>
+template <int QT, int WCH, int OP>
 __global__ __launch_bounds__(256) void k_sign_scan(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
                                                    uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
                                                    uint32_t blocks_per_seg, uint64_t n_pad, uint16_t* __restrict__ H,
                                                    uint16_t* __restrict__ hist_part, uint32_t nseg, uint32_t nbins,
-                                                   uint32_t hwords) {
+                                                   uint32_t hwords, uint32_t ntop) {
     extern __shared__ uint32_t lds_hist[];
     const uint32_t q0 = blockIdx.x * QT;
     const uint32_t tq = min((uint32_t)QT, nq - q0);
@@ -325,7 +338,7 @@ __global__ __launch_bounds__(256) void k_sign_scan(const uint64_t* __restrict__ 
                 if (i < (int)tq) {
                     const uint64_t* qp = qb + (uint64_t)i * qpv + w0;
 #pragma unroll
-                    for (int j = 0; j < WCH; j++) acc[i] += (uint32_t)__popcll(dw[j] ^ qp[j]);
+                    for (int j = 0; j < WCH; j++) acc[i] += word_stat<OP>(dw[j], qp[j]);
                 }
             }
         }
@@ -333,6 +346,7 @@ __global__ __launch_bounds__(256) void k_sign_scan(const uint64_t* __restrict__ 
         const bool valid = doc < n;
 #pragma unroll
         for (int i = 0; i < QT; i++) {
+            acc[i] = to_dist<OP>(acc[i], ntop);
             if (i < (int)tq) {
                 H[(uint64_t)(q0 + i) * n_pad + doc] = (uint16_t)acc[i];
                 if (valid) atomicAdd(&lds_hist[i * hwords + (acc[i] >> 1)], 1u << ((acc[i] & 1) * 16));
@@ -534,13 +548,130 @@ __global__ __launch_bounds__(64) void k_s1_prep(const float* __restrict__ q, uin
     }
 }
 
+// Top-bucket bitmaps (src/bitmap.rs:140-187): one workgroup per row, LDS
+// bitonic sort of 64-bit keys, bit j set for the chosen n_top coordinates.
+//  * document (QUERY == false): rank_transform's order (value ascending, -0 ==
+//    +0, ties by index ascending); bits for ranks >= d - n_top.
+//  * query (QUERY == true): build_query_bitmap_fp32's stable sort by value
+//    descending (ties keep index order); bits for the first n_top.
+// Documents go to the interleaved block layout, queries to qbits[q][w]; the
+// query form also zeroes the fast path's histogram and counter and records
+// non-finite input in qstat bit 0.
+template <bool QUERY>
+__global__ __launch_bounds__(256) void k_topbits(const float* __restrict__ x, uint64_t n, uint32_t d, uint32_t ntop,
+                                                 uint64_t* __restrict__ out, uint64_t first_doc,
+                                                 uint32_t* __restrict__ hist, uint32_t nbins,
+                                                 uint32_t* __restrict__ cnt, uint32_t* __restrict__ qstat) {
+    extern __shared__ uint64_t smem[];
+    const uint32_t P = next_pow2(d), qpv = d / 64;
+    uint64_t* keys = smem;
+    uint32_t* bits = reinterpret_cast<uint32_t*>(smem + P);
+    for (uint64_t r = blockIdx.x; r < n; r += gridDim.x) {
+        const float* v = x + r * d;
+        bool bad = false;
+        for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+            if (i < d) {
+                float f = v[i];
+                bad |= !isfinite(f);
+                if (f == 0.0f) f = 0.0f;
+                const uint32_t o = QUERY ? ~f32_ord(f) : f32_ord(f);
+                keys[i] = ((uint64_t)o << 16) | i;
+            } else {
+                keys[i] = kKeyMax;
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < 2 * qpv; i += blockDim.x) bits[i] = 0;
+        const int any_bad = __syncthreads_or(bad);
+        bitonic_sort(keys, P);
+        const uint32_t lo = QUERY ? 0 : d - ntop, hi = QUERY ? ntop : d;
+        for (uint32_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
+            const uint32_t idx = (uint32_t)(keys[p] & 0xffffu);
+            atomicOr(&bits[idx >> 5], 1u << (idx & 31));
+        }
+        __syncthreads();
+        for (uint32_t w = threadIdx.x; w < qpv; w += blockDim.x) {
+            const uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+            if (QUERY) {
+                out[r * qpv + w] = word;
+            } else {
+                const uint64_t doc = first_doc + r;
+                out[(((doc >> 6) * qpv + w) << 6) + (doc & 63)] = word;
+            }
+        }
+        if (QUERY) {
+            for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) hist[r * nbins + i] = 0;
+            if (threadIdx.x == 0) {
+                if (cnt) cnt[r] = 0;
+                if (qstat) qstat[r] = any_bad ? 1u : 0u;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Bitmap subset search (src/bitmap.rs:425-505): per query, overlap of each
+// listed row (duplicates scored independently), top-k by (overlap desc, row
+// asc); scores are the overlaps as f32.
+__global__ __launch_bounds__(256) void k_bm_subset(const uint64_t* __restrict__ words, uint32_t qpv, uint64_t n_docs,
+                                                   const uint64_t* __restrict__ qbits, const uint64_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ rows, uint32_t ntop, uint32_t out_k,
+                                                   float* __restrict__ out_s, int64_t* __restrict__ out_i) {
+    __shared__ TopKShared tk;
+    const uint64_t q = blockIdx.x;
+    const uint64_t b = offsets[q], e = offsets[q + 1];
+    const uint64_t* qb = qbits + q * qpv;
+    tk_init(tk, out_k);
+    for (uint64_t r0 = b; r0 < e; r0 += blockDim.x) {
+        tk_reserve(tk, out_k, blockDim.x);
+        const uint64_t j = r0 + threadIdx.x;
+        const uint32_t row = j < e ? rows[j] : 0u;
+        if (j < e && row < n_docs) {
+            const uint64_t* base = words + ((uint64_t)(row >> 6) * qpv << 6) + (row & 63);
+            uint32_t ov = 0;
+            for (uint32_t w = 0; w < qpv; w++) ov += word_stat<1>(base[(uint64_t)w << 6], qb[w]);
+            tk_offer(tk, out_k, ((uint64_t)(ntop - ov) << 32) | row);
+        }
+    }
+    __syncthreads();
+    tk_flush(tk, out_k);
+    for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
+        const uint64_t key = tk.keys[j];
+        out_s[q * out_k + j] = key == kKeyMax ? -INFINITY : (float)(ntop - (uint32_t)(key >> 32));
+        out_i[q * out_k + j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
+    }
+}
+
+// body_overlap_scores_subset (src/bitmap.rs:425-460): out[i] = popcount(doc
+// rows[i] AND q) for a caller-supplied query bitmap.
+__global__ void k_bm_overlap_rows(const uint64_t* __restrict__ words, uint32_t qpv, const uint64_t* __restrict__ q,
+                                  const uint32_t* __restrict__ rows, uint64_t count, uint32_t* __restrict__ out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t row = rows[i];
+        const uint64_t* base = words + ((uint64_t)(row >> 6) * qpv << 6) + (row & 63);
+        uint32_t ov = 0;
+        for (uint32_t w = 0; w < qpv; w++) ov += word_stat<1>(base[(uint64_t)w << 6], q[w]);
+        out[i] = ov;
+    }
+}
+
+// Full Bitmap search output (src/bitmap.rs:202-250): stage-1 rows of (distance,
+// id) -> overlap scores as f32 and i64 ids.
+__global__ void k_bm_finish(const uint32_t* __restrict__ cand, const uint32_t* __restrict__ cand_h, uint64_t count,
+                            uint32_t ntop, float* __restrict__ out_s, int64_t* __restrict__ out_i) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        out_s[i] = (float)(ntop - cand_h[i]);
+        out_i[i] = (int64_t)cand[i];
+    }
+}
+
 // Hamming histogram of the sample blocks (every `stride`-th 64-doc block),
 // LDS packed 16-bit bins per query, merged into hist with global atomics.
-template <int QT, int WCH>
+template <int QT, int WCH, int OP>
 __global__ __launch_bounds__(256) void k_s1_sample(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
                                                    uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
                                                    uint32_t stride, uint32_t nsample, uint32_t blocks_per_seg,
-                                                   uint32_t* __restrict__ hist, uint32_t nbins, uint32_t hwords) {
+                                                   uint32_t* __restrict__ hist, uint32_t nbins, uint32_t hwords,
+                                                   uint32_t ntop) {
     extern __shared__ uint32_t lds_hist[];
     const uint32_t q0 = blockIdx.x * QT;
     const uint32_t tq = min((uint32_t)QT, nq - q0);
@@ -565,14 +696,16 @@ __global__ __launch_bounds__(256) void k_s1_sample(const uint64_t* __restrict__ 
                 if (i < (int)tq) {
                     const uint64_t* qp = qb + (uint64_t)i * qpv + w0;
 #pragma unroll
-                    for (int jj = 0; jj < WCH; jj++) acc[i] += (uint32_t)__popcll(dw[jj] ^ qp[jj]);
+                    for (int jj = 0; jj < WCH; jj++) acc[i] += word_stat<OP>(dw[jj], qp[jj]);
                 }
             }
         }
         if ((uint64_t)b * 64 + lane < n) {
 #pragma unroll
-            for (int i = 0; i < QT; i++)
-                if (i < (int)tq) atomicAdd(&lds_hist[i * hwords + (acc[i] >> 1)], 1u << ((acc[i] & 1) * 16));
+            for (int i = 0; i < QT; i++) {
+                const uint32_t h = to_dist<OP>(acc[i], ntop);
+                if (i < (int)tq) atomicAdd(&lds_hist[i * hwords + (h >> 1)], 1u << ((h & 1) * 16));
+            }
         }
     }
     __syncthreads();
@@ -626,12 +759,12 @@ __global__ __launch_bounds__(64) void k_s1_threshold(const uint32_t* __restrict_
 // words are wave-uniform scalar loads: 2*WCH VALU XOR + 2*WCH popcount per
 // query and doc.  WCH == 0 is the generic path for word counts without a
 // specialisation (doc words re-read per query from L1).
-template <int WCH>
-__device__ __forceinline__ uint32_t hamming_regs(const uint64_t* dw, const uint64_t* __restrict__ qp) {
+template <int WCH, int OP>
+__device__ __forceinline__ uint32_t hamming_regs(const uint64_t* dw, const uint64_t* __restrict__ qp, uint32_t ntop) {
     uint32_t acc = 0;
 #pragma unroll
-    for (int j = 0; j < WCH; j++) acc += (uint32_t)__popcll(dw[j] ^ qp[j]);
-    return acc;
+    for (int j = 0; j < WCH; j++) acc += word_stat<OP>(dw[j], qp[j]);
+    return to_dist<OP>(acc, ntop);
 }
 
 __device__ __forceinline__ void s1_append(bool pass, uint32_t acc, uint64_t doc, uint32_t lane, uint64_t lt_mask,
@@ -647,11 +780,12 @@ __device__ __forceinline__ void s1_append(bool pass, uint32_t acc, uint64_t doc,
     }
 }
 
-template <int WCH>
+template <int WCH, int OP>
 __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
                                                  uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
                                                  uint32_t qt, uint32_t blocks_per_seg, const uint32_t* __restrict__ thr,
-                                                 uint32_t cap, uint64_t* __restrict__ keys, uint32_t* __restrict__ cnt) {
+                                                 uint32_t cap, uint64_t* __restrict__ keys, uint32_t* __restrict__ cnt,
+                                                 uint32_t ntop) {
     const uint32_t q0 = blockIdx.x * qt;
     const uint32_t tq = min(qt, nq - q0);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -673,21 +807,22 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
             uint32_t i = 0;
             // Two queries per step: their scalar loads share one wait.
             for (; i + 1 < tq; i += 2) {
-                const uint32_t a0 = hamming_regs<WCH>(dw, qb + (uint64_t)i * WCH);
-                const uint32_t a1 = hamming_regs<WCH>(dw, qb + (uint64_t)(i + 1) * WCH);
+                const uint32_t a0 = hamming_regs<WCH, OP>(dw, qb + (uint64_t)i * WCH, ntop);
+                const uint32_t a1 = hamming_regs<WCH, OP>(dw, qb + (uint64_t)(i + 1) * WCH, ntop);
                 const uint32_t t0 = tb[i], t1 = tb[i + 1];
                 s1_append(valid && a0 <= t0, a0, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
                 s1_append(valid && a1 <= t1, a1, doc, lane, lt_mask, cb + i + 1, kb + (uint64_t)(i + 1) * cap, cap);
             }
             if (i < tq) {
-                const uint32_t a0 = hamming_regs<WCH>(dw, qb + (uint64_t)i * WCH);
+                const uint32_t a0 = hamming_regs<WCH, OP>(dw, qb + (uint64_t)i * WCH, ntop);
                 s1_append(valid && a0 <= tb[i], a0, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
             }
         } else {
             for (uint32_t i = 0; i < tq; i++) {
                 const uint64_t* qp = qb + (uint64_t)i * qpv;
                 uint32_t acc = 0;
-                for (uint32_t j = 0; j < qpv; j++) acc += (uint32_t)__popcll(base[(uint64_t)j * 64] ^ qp[j]);
+                for (uint32_t j = 0; j < qpv; j++) acc += word_stat<OP>(base[(uint64_t)j * 64], qp[j]);
+                acc = to_dist<OP>(acc, ntop);
                 s1_append(valid && acc <= tb[i], acc, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
             }
         }
@@ -1300,6 +1435,63 @@ void launch_fastscan_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint8_t*
     ktime_end(kFamEncode, s);
 }
 
+static uint32_t next_pow2_host(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+void launch_topbits_docs(const float* x, uint64_t n, uint32_t d, uint32_t ntop, uint64_t* out, uint64_t first_doc,
+                         hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t P = next_pow2_host(d);
+    ktime_begin(kFamEncode, s);
+    hipLaunchKernelGGL(k_topbits<false>, dim3((uint32_t)std::min<uint64_t>(n, 65535)), dim3(256),
+                       (size_t)P * 8 + (size_t)(d / 64) * 8, s, x, n, d, ntop, out, first_doc, nullptr, 0, nullptr,
+                       nullptr);
+    check_launch();
+    ktime_end(kFamEncode, s);
+}
+
+void launch_bitmap_query_bits(const float* queries, uint64_t nq, uint32_t d, uint32_t ntop, uint64_t* qbits,
+                              uint32_t* qstat, hipStream_t s) {
+    if (nq == 0) return;
+    const uint32_t P = next_pow2_host(d);
+    ktime_begin(kFamPrepare, s);
+    hipLaunchKernelGGL(k_topbits<true>, dim3((uint32_t)std::min<uint64_t>(nq, 65535)), dim3(256),
+                       (size_t)P * 8 + (size_t)(d / 64) * 8, s, queries, nq, d, ntop, qbits, 0, nullptr, 0, nullptr,
+                       qstat);
+    check_launch();
+    ktime_end(kFamPrepare, s);
+}
+
+void launch_bm_subset(const uint64_t* words, uint32_t d, uint64_t n_docs, const uint64_t* qbits, uint64_t nq,
+                      const uint64_t* offsets, const uint32_t* rows, uint32_t ntop, uint32_t out_k, float* out_s,
+                      int64_t* out_i, hipStream_t s) {
+    if (nq == 0 || out_k == 0) return;
+    ktime_begin(kFamRerank, s);
+    hipLaunchKernelGGL(k_bm_subset, dim3((uint32_t)nq), dim3(256), 0, s, words, d / 64, n_docs, qbits, offsets, rows,
+                       ntop, out_k, out_s, out_i);
+    check_launch();
+    ktime_end(kFamRerank, s);
+}
+
+void launch_bm_overlap_rows(const uint64_t* words, uint32_t d, const uint64_t* q, const uint32_t* rows, uint64_t count,
+                            uint32_t* out, hipStream_t s) {
+    if (count == 0) return;
+    uint32_t grid = (uint32_t)std::min<uint64_t>(4096, (count + 255) / 256);
+    hipLaunchKernelGGL(k_bm_overlap_rows, dim3(grid), dim3(256), 0, s, words, d / 64, q, rows, count, out);
+    check_launch();
+}
+
+void launch_bm_finish(const uint32_t* cand, const uint32_t* cand_h, uint64_t count, uint32_t ntop, float* out_s,
+                      int64_t* out_i, hipStream_t s) {
+    if (count == 0) return;
+    uint32_t grid = (uint32_t)std::min<uint64_t>(4096, (count + 255) / 256);
+    hipLaunchKernelGGL(k_bm_finish, dim3(grid), dim3(256), 0, s, cand, cand_h, count, ntop, out_s, out_i);
+    check_launch();
+}
+
 Stage1Plan plan_stage1(uint64_t n, uint32_t d, uint64_t nq) {
     Stage1Plan p{};
     p.nbins = d + 1;
@@ -1350,31 +1542,37 @@ void carve_stage1(const Stage1Plan& p, uint64_t nq, uint64_t m_eff, void* base, 
     w->ties = reinterpret_cast<uint32_t*>(c);
 }
 
-template <int QT>
+template <int QT, int OP>
 static void launch_scan_qt(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
-                           const Stage1Plan& p, const Stage1Work& w, hipStream_t s) {
+                           const Stage1Plan& p, const Stage1Work& w, uint32_t ntop, hipStream_t s) {
     uint32_t nblk = (uint32_t)(p.n_pad / 64);
     dim3 grid((uint32_t)((nq + QT - 1) / QT), p.nseg);
-    if (qpv % 4 == 0)
-        hipLaunchKernelGGL((k_sign_scan<QT, 4>), grid, dim3(256), p.lds_bytes, s, words, nblk, qpv, n, qbits,
-                           (uint32_t)nq, p.blocks_per_seg, p.n_pad, w.H, w.hist_part, p.nseg, p.nbins, p.hwords);
-    else if (qpv % 2 == 0)
-        hipLaunchKernelGGL((k_sign_scan<QT, 2>), grid, dim3(256), p.lds_bytes, s, words, nblk, qpv, n, qbits,
-                           (uint32_t)nq, p.blocks_per_seg, p.n_pad, w.H, w.hist_part, p.nseg, p.nbins, p.hwords);
-    else
-        hipLaunchKernelGGL((k_sign_scan<QT, 1>), grid, dim3(256), p.lds_bytes, s, words, nblk, qpv, n, qbits,
-                           (uint32_t)nq, p.blocks_per_seg, p.n_pad, w.H, w.hist_part, p.nseg, p.nbins, p.hwords);
+#define OVX_SIGN_SCAN(W)                                                                                           \
+    hipLaunchKernelGGL((k_sign_scan<QT, W, OP>), grid, dim3(256), p.lds_bytes, s, words, nblk, qpv, n, qbits,      \
+                       (uint32_t)nq, p.blocks_per_seg, p.n_pad, w.H, w.hist_part, p.nseg, p.nbins, p.hwords, ntop)
+    if (qpv % 4 == 0) OVX_SIGN_SCAN(4);
+    else if (qpv % 2 == 0) OVX_SIGN_SCAN(2);
+    else OVX_SIGN_SCAN(1);
+#undef OVX_SIGN_SCAN
     check_launch();
 }
 
+template <int OP>
+static void launch_scan_qts(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
+                            const Stage1Plan& p, const Stage1Work& w, uint32_t ntop, hipStream_t s) {
+    if (p.qt == 16) launch_scan_qt<16, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
+    else if (p.qt == 4) launch_scan_qt<4, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
+    else launch_scan_qt<1, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
+}
+
 void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t* qbits, uint64_t nq, uint64_t m_eff,
-                   const Stage1Plan& p, const Stage1Work& w, uint32_t* cand, uint32_t* cand_h, hipStream_t s) {
+                   const Stage1Plan& p, const Stage1Work& w, uint32_t* cand, uint32_t* cand_h, uint32_t ntop,
+                   hipStream_t s) {
     if (nq == 0 || m_eff == 0) return;
     const uint32_t qpv = d / 64;
     ktime_begin(kFamSignScan, s);
-    if (p.qt == 16) launch_scan_qt<16>(words, n, qpv, qbits, nq, p, w, s);
-    else if (p.qt == 4) launch_scan_qt<4>(words, n, qpv, qbits, nq, p, w, s);
-    else launch_scan_qt<1>(words, n, qpv, qbits, nq, p, w, s);
+    if (ntop) launch_scan_qts<1>(words, n, qpv, qbits, nq, p, w, ntop, s);
+    else launch_scan_qts<0>(words, n, qpv, qbits, nq, p, w, 0, s);
     ktime_end(kFamSignScan, s);
     ktime_begin(kFamStage1Select, s);
     hipLaunchKernelGGL(k_stage1_threshold, dim3((uint32_t)nq), dim3(256), (size_t)p.nbins * 4, s, w.hist_part, p.nseg,
@@ -1450,32 +1648,41 @@ void carve_stage1_fast(const FastPlan& p, uint64_t nq, void* base, FastWork* w) 
     w->keys = reinterpret_cast<uint64_t*>(c);
 }
 
-template <int QT, int WCH>
+template <int QT, int WCH, int OP>
 static void launch_sample_qt(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
-                             const FastPlan& p, const FastWork& w, hipStream_t s) {
+                             const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
     const uint32_t nblk = (uint32_t)((n + 63) / 64);
     const uint32_t tiles = (uint32_t)((nq + QT - 1) / QT);
-    hipLaunchKernelGGL((k_s1_sample<QT, WCH>), dim3(tiles, p.nsseg), dim3(256), p.lds_bytes, s, words, nblk, qpv, n,
-                       qbits, (uint32_t)nq, p.stride, p.nsample, p.sample_bps, w.hist, p.nbins, p.hwords);
+    hipLaunchKernelGGL((k_s1_sample<QT, WCH, OP>), dim3(tiles, p.nsseg), dim3(256), p.lds_bytes, s, words, nblk, qpv,
+                       n, qbits, (uint32_t)nq, p.stride, p.nsample, p.sample_bps, w.hist, p.nbins, p.hwords, ntop);
     check_launch();
 }
 
-template <int QT>
+template <int QT, int OP>
 static void launch_sample_wch(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
-                              const FastPlan& p, const FastWork& w, hipStream_t s) {
-    if (qpv % 4 == 0) launch_sample_qt<QT, 4>(words, n, qpv, qbits, nq, p, w, s);
-    else if (qpv % 2 == 0) launch_sample_qt<QT, 2>(words, n, qpv, qbits, nq, p, w, s);
-    else launch_sample_qt<QT, 1>(words, n, qpv, qbits, nq, p, w, s);
+                              const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
+    if (qpv % 4 == 0) launch_sample_qt<QT, 4, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
+    else if (qpv % 2 == 0) launch_sample_qt<QT, 2, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
+    else launch_sample_qt<QT, 1, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
 }
 
+template <int OP>
+static void launch_sample(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
+                          const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
+    if (p.qt == 16) launch_sample_wch<16, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
+    else if (p.qt == 4) launch_sample_wch<4, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
+    else launch_sample_wch<1, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
+}
+
+template <int OP>
 static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
-                        const FastPlan& p, const FastWork& w, hipStream_t s) {
+                        const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
     const uint32_t nblk = (uint32_t)((n + 63) / 64);
     const uint32_t tiles = (uint32_t)((nq + p.scan_qt - 1) / p.scan_qt);
     dim3 grid(tiles, p.nseg);
-#define OVX_SCAN(W)                                                                                            \
-    hipLaunchKernelGGL((k_s1_scan<W>), grid, dim3(256), 0, s, words, nblk, qpv, n, qbits, (uint32_t)nq, p.scan_qt, \
-                       p.scan_bps, w.thr, p.cap, w.keys, w.cnt)
+#define OVX_SCAN(W)                                                                                               \
+    hipLaunchKernelGGL((k_s1_scan<W, OP>), grid, dim3(256), 0, s, words, nblk, qpv, n, qbits, (uint32_t)nq,        \
+                       p.scan_qt, p.scan_bps, w.thr, p.cap, w.keys, w.cnt, ntop)
     switch (qpv) {
         case 16: OVX_SCAN(16); break;
         case 12: OVX_SCAN(12); break;
@@ -1491,25 +1698,33 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
 
 void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
                         uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
-                        uint32_t* cand_h, uint32_t* qstat, hipStream_t s) {
+                        uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s) {
     if (nq == 0) return;
     const uint32_t qpv = d / 64;
     ktime_begin(kFamStage1Select, s);
-    hipLaunchKernelGGL(k_s1_prep, dim3((uint32_t)nq), dim3(64), 0, s, queries, d, qbits, w.hist, p.nbins, w.cnt, qstat);
+    if (ntop) {
+        const uint32_t P = next_pow2_host(d);
+        hipLaunchKernelGGL(k_topbits<true>, dim3((uint32_t)std::min<uint64_t>(nq, 65535)), dim3(256),
+                           (size_t)P * 8 + (size_t)qpv * 8, s, queries, nq, d, ntop, qbits, 0, w.hist, p.nbins, w.cnt,
+                           qstat);
+    } else {
+        hipLaunchKernelGGL(k_s1_prep, dim3((uint32_t)nq), dim3(64), 0, s, queries, d, qbits, w.hist, p.nbins, w.cnt,
+                           qstat);
+    }
     check_launch();
     ktime_end(kFamStage1Select, s);
     if (m_eff == 0) return;
     if (p.s_target != 0xffffffffu) {
         ktime_begin(kFamSample, s);
-        if (p.qt == 16) launch_sample_wch<16>(words, n, qpv, qbits, nq, p, w, s);
-        else if (p.qt == 4) launch_sample_wch<4>(words, n, qpv, qbits, nq, p, w, s);
-        else launch_sample_wch<1>(words, n, qpv, qbits, nq, p, w, s);
+        if (ntop) launch_sample<1>(words, n, qpv, qbits, nq, p, w, ntop, s);
+        else launch_sample<0>(words, n, qpv, qbits, nq, p, w, 0, s);
         ktime_end(kFamSample, s);
     }
     hipLaunchKernelGGL(k_s1_threshold, dim3((uint32_t)nq), dim3(64), 0, s, w.hist, p.nbins, p.s_target, w.thr);
     check_launch();
     ktime_begin(kFamSignScan, s);
-    launch_scan(words, n, qpv, qbits, nq, p, w, s);
+    if (ntop) launch_scan<1>(words, n, qpv, qbits, nq, p, w, ntop, s);
+    else launch_scan<0>(words, n, qpv, qbits, nq, p, w, 0, s);
     ktime_end(kFamSignScan, s);
     ktime_begin(kFamStage1Select, s);
     hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(1024), 0, s, w.keys, w.cnt, p.cap, m_eff, cand, cand_h,

@@ -394,6 +394,179 @@ class SignBitmap:
         return cls._wrap(h)
 
 
+class Bitmap:
+    """Top-bucket (constant-weight) bitmap index in HBM (src/bitmap.rs;
+    ``_ordvec.pyi:109-144``): bit j of a document is set iff coordinate j is
+    among its ``n_top`` highest ranks; score = popcount(Q AND D)."""
+
+    def __init__(self, dim: int, n_top: int):
+        h = C.c_void_p()
+        N.check(N.lib().ordvec_bitmap_new(int(dim), int(n_top), C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def _wrap(cls, h):
+        obj = cls.__new__(cls)
+        obj._h = h
+        return obj
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and N._lib is not None:
+            N.lib().ordvec_bitmap_free(h)
+            self._h = None
+
+    def _info(self):
+        d, t, n = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        N.check(N.lib().ordvec_bitmap_info(self._h, C.byref(d), C.byref(t), C.byref(n)))
+        return d.value, t.value, n.value
+
+    @property
+    def dim(self) -> int:
+        return self._info()[0]
+
+    @property
+    def n_top(self) -> int:
+        return self._info()[1]
+
+    def __len__(self) -> int:
+        return self._info()[2]
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    @property
+    def bytes_per_vec(self) -> int:
+        return self.dim // 8
+
+    @property
+    def byte_size(self) -> int:
+        d, _, n = self._info()
+        return n * d // 8
+
+    def __repr__(self) -> str:
+        d, t, n = self._info()
+        return f"Bitmap(dim={d}, n_top={t}, n_vectors={n}, bytes_per_vector={d // 8})"
+
+    def add(self, vectors) -> None:
+        """Bitmap::add (src/bitmap.rs:140-162)."""
+        x, n, flag = _float_rows(vectors, self.dim, "vectors")
+        _bind_stream() if flag else _host_stream()
+        N.check(N.lib().ordvec_bitmap_add(self._h, _ptr(x), n, flag))
+
+    def search(self, queries, k: int):
+        """Bitmap::search (src/bitmap.rs:202-250): (nq, min(k, n)) overlaps as
+        f32 and i64 ids, ordered (overlap desc, id asc)."""
+        d, _, n = self._info()
+        q, nq, flag = _float_rows(queries, d, "queries")
+        kk = min(int(k), n)
+        s = _out((nq, kk), np.float32, flag)
+        i = _out((nq, kk), np.int64, flag)
+        if nq and kk:
+            _bind_stream() if flag else _host_stream()
+            N.check(N.lib().ordvec_bitmap_search(self._h, _ptr(q), nq, kk, _ptr(s), _ptr(i), flag))
+        return s, i
+
+    def top_m_candidates_batched(self, queries, m: int):
+        """top_m_candidates_batched (src/bitmap.rs:295-362): nq x min(m, n) ids
+        ordered (overlap desc, id asc)."""
+        d, _, n = self._info()
+        q, nq, flag = _float_rows(queries, d, "queries")
+        m_eff = min(int(m), n)
+        c = _out((nq, m_eff), np.uint32, flag)
+        if nq and m_eff:
+            _bind_stream() if flag else _host_stream()
+            N.check(N.lib().ordvec_bitmap_top_m(self._h, _ptr(q), nq, int(m), _ptr(c), flag))
+        return c
+
+    def top_m_candidates(self, query, m: int):
+        """top_m_candidates (src/bitmap.rs:264-293): 1-D, length min(m, n)."""
+        if _is_cuda_tensor(query):
+            return self.top_m_candidates_batched(query.reshape(1, -1), m)[0]
+        q = np.asarray(query)
+        if q.ndim != 1:
+            raise ValueError("query must be 1-D")
+        return self.top_m_candidates_batched(q.reshape(1, -1), m)[0]
+
+    def top_m_candidates_batched_chunked(self, queries, m: int, batch_size: int):
+        """top_m_candidates_batched_chunked (src/bitmap.rs:364-396): same rows as
+        the batched form, computed batch_size queries at a time."""
+        if int(batch_size) <= 0:
+            raise ValueError("batch_size must be > 0")
+        q, nq, _ = _float_rows(queries, self.dim, "queries")
+        parts = [self.top_m_candidates_batched(q[i:i + int(batch_size)], m) for i in range(0, nq, int(batch_size))]
+        if not parts:
+            return self.top_m_candidates_batched(q, m)
+        return torch.cat(parts) if _is_cuda_tensor(parts[0]) else np.concatenate(parts)
+
+    def search_subset(self, query, doc_ids, k: int):
+        """Bitmap::search_subset (src/bitmap.rs:477-505): 1-D arrays of length
+        min(k, len(doc_ids)); duplicates are scored independently."""
+        d, _, n = self._info()
+        q, _ = _float_vec(query, d, "query")
+        if _is_cuda_tensor(q):
+            q = q.cpu().numpy()
+        ids, _ = _ids(doc_ids.cpu() if _is_cuda_tensor(doc_ids) else doc_ids, "doc_ids")
+        kk = min(int(k), ids.shape[0])
+        s = np.empty(kk, np.float32)
+        i = np.empty(kk, np.int64)
+        if kk:
+            offs = np.array([0, ids.shape[0]], dtype=np.uint64)
+            _host_stream()
+            N.check(N.lib().ordvec_bitmap_search_subset_batched(
+                self._h, q.ctypes.data_as(C.c_void_p), 1, offs.ctypes.data_as(C.c_void_p),
+                ids.ctypes.data_as(C.c_void_p), ids.shape[0], kk, s.ctypes.data_as(C.c_void_p),
+                i.ctypes.data_as(C.c_void_p), None, 0))
+        return s, i
+
+    def build_query_bitmap_fp32(self, query) -> np.ndarray:
+        """build_query_bitmap_fp32 (src/bitmap.rs:164-187): dim/64 u64 words."""
+        d, _, _ = self._info()
+        q, _ = _float_vec(query, d, "query")
+        if _is_cuda_tensor(q):
+            q = q.cpu().numpy()
+        out = np.empty(d // 64, np.uint64)
+        _host_stream()
+        N.check(N.lib().ordvec_bitmap_query_bits(self._h, q.ctypes.data_as(C.c_void_p), 1,
+                                                 out.ctypes.data_as(C.c_void_p), 0))
+        return out
+
+    def body_overlap_scores_subset(self, q_bitmap, doc_ids) -> np.ndarray:
+        """body_overlap_scores_subset (src/bitmap.rs:425-460): u32 overlaps in
+        doc_ids order."""
+        d, _, _ = self._info()
+        qb = np.ascontiguousarray(np.asarray(q_bitmap), dtype=np.uint64)
+        if qb.shape != (d // 64,):
+            raise ValueError(f"q_bitmap must have {d // 64} u64 words")
+        ids, _ = _ids(doc_ids.cpu() if _is_cuda_tensor(doc_ids) else doc_ids, "doc_ids")
+        out = np.empty(ids.shape[0], np.uint32)
+        if ids.shape[0]:
+            _host_stream()
+            N.check(N.lib().ordvec_bitmap_overlap_subset(self._h, qb.ctypes.data_as(C.c_void_p),
+                                                         ids.ctypes.data_as(C.c_void_p), ids.shape[0],
+                                                         out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def words(self) -> np.ndarray:
+        """Bitmaps row-major, n x dim/64 u64 (the .ovbm payload)."""
+        d, _, n = self._info()
+        out = np.empty((n, d // 64), dtype=np.uint64)
+        _host_stream()
+        N.check(N.lib().ordvec_bitmap_words(self._h, out.ctypes.data_as(C.c_void_p), out.size))
+        return out
+
+    def write(self, path: str) -> None:
+        _host_stream()
+        N.check(N.lib().ordvec_bitmap_write(self._h, str(path).encode()))
+
+    @classmethod
+    def load(cls, path: str) -> "Bitmap":
+        h = C.c_void_p()
+        _host_stream()
+        N.check(N.lib().ordvec_bitmap_load(str(path).encode(), C.byref(h)))
+        return cls._wrap(h)
+
+
 class RankQuantFastscan:
     """FastScan b=2 block-32 nibble-LUT index in HBM (src/fastscan.rs:532-705)."""
 

@@ -91,6 +91,17 @@ def test_load_and_probe_status_paths(tmp_path):
                    (1).to_bytes(4, "little") + bytes(7) + b"\xc0")
     assert lib.ordvec_index_probe(str(bm).encode(), 0, C.byref(info)) == N.OK
     assert (info.kind, info.n_top, info.bytes_per_vec) == (2, 2, 8)
+    # .ovbm loads through ABI v1; a row without exactly n_top bits is corrupt
+    # (src/rank_io.rs:938-951), found before any device work.
+    forged = tmp_path / "forged.ovbm"
+    forged.write_bytes(bm.read_bytes()[:-1] + b"\xe0")
+    assert lib.ordvec_index_load(str(forged).encode(), 0, C.byref(out)) == N.CORRUPT_INDEX
+    assert "bits set" in N.last_error()
+    assert lib.ordvec_bitmap_load(str(forged).encode(), C.byref(out)) == N.CORRUPT_INDEX
+    zero_top = tmp_path / "z.ovbm"
+    zero_top.write_bytes(b"OVBM\x01" + (64).to_bytes(4, "little") + (0).to_bytes(4, "little") +
+                         (0).to_bytes(4, "little"))
+    assert lib.ordvec_index_load(str(zero_top).encode(), 0, C.byref(out)) == N.CORRUPT_INDEX
     v2 = tmp_path / "v2.ovrq"
     v2.write_bytes(b"OVRQ\x02" + p.read_bytes()[5:])
     assert lib.ordvec_index_probe(str(v2).encode(), 0, C.byref(info)) == N.CORRUPT_INDEX
@@ -106,6 +117,9 @@ def test_parameter_validation_needs_no_device():
     assert lib.ordvec_rankquant_new(70000, 1, C.byref(h)) == N.BAD_ARGUMENT
     assert lib.ordvec_sign_bitmap_new(100, C.byref(h)) == N.BAD_ARGUMENT
     assert lib.ordvec_fastscan_new(6, C.byref(h)) == N.BAD_ARGUMENT
+    assert lib.ordvec_bitmap_new(100, 4, C.byref(h)) == N.BAD_ARGUMENT  # dim % 64
+    assert lib.ordvec_bitmap_new(64, 0, C.byref(h)) == N.BAD_ARGUMENT   # 0 < n_top < dim
+    assert lib.ordvec_bitmap_new(64, 64, C.byref(h)) == N.BAD_ARGUMENT
     assert lib.ordvec_rankquant_new(64, 2, None) == N.NULL_POINTER
     assert lib.ordvec_index_search(None, None, None, 0, None, None) == N.NULL_POINTER
     lib.ordvec_index_free(None)

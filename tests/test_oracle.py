@@ -289,3 +289,68 @@ def test_score_grad_zero_query_is_zero(oracle):
     ids = np.arange(5, dtype=np.int64)[None]
     assert np.all(oracle.score_ids_grad(codes, 2, x, ids, np.ones((1, 5))) == 0.0)
     assert np.all(oracle.score_ids(codes, 2, x, ids) == 0.0)
+
+
+# ---- Bitmap (top-bucket, AND-popcount) ------------------------------------------
+
+def _stable_top_bits(order, n_top, d):
+    w = np.zeros(d // 64, dtype=np.uint64)
+    for j in order:
+        w[j // 64] |= np.uint64(1) << np.uint64(j % 64)
+    return w
+
+
+def test_bitmap_fixture_kats(oracle):
+    # tests/persistence_compat.rs:160-166: Bitmap(64, 2) of 0..63 -> bits 62, 63
+    w = oracle.bitmap_encode(np.arange(64, dtype=np.float32)[None], 2)
+    assert int(w[0, 0]) == 0xC000000000000000
+    # ordvec-ffi/src/lib.rs:1035-1049, 1227-1270: Bitmap(64, 4), docs with
+    # 10..13 in coordinates 0..3; subset [3, 1, 1, 2], k = 3 -> [1, 1, 2]
+    doc = np.zeros(64, np.float32)
+    doc[:4] = 10 + np.arange(4)
+    bm = oracle.bitmap_encode(np.repeat(doc[None], 4, axis=0), 4)
+    assert np.all(bm[:, 0] == 0xF)
+    qb = oracle.bitmap_query(doc, 4)
+    assert int(qb[0, 0]) == 0xF
+    s, i = oracle.bitmap_search_subset(bm, qb, [3, 1, 1, 2], 3)
+    assert list(i) == [1, 1, 2] and list(s) == [4.0, 4.0, 4.0]
+    s, i = oracle.bitmap_search(bm, qb, 2)
+    assert list(i[0]) == [0, 1]
+
+
+def test_bitmap_encode_and_query_bruteforce(oracle):
+    rng = np.random.default_rng(41)
+    for d, n_top in [(64, 16), (128, 1), (192, 100), (1024, 256)]:
+        x = np.round(rng.standard_normal((20, d)) * 2).astype(np.float32)  # heavy ties
+        x[0, :10] = -0.0
+        x[1, :10] = 0.0
+        bm = oracle.bitmap_encode(x, n_top)
+        qb = oracle.bitmap_query(x, n_top)
+        for r in range(20):
+            canon = np.where(x[r] == 0, np.float32(0), x[r])
+            asc = np.argsort(canon, kind="stable")  # rank order: ties by index
+            assert np.array_equal(bm[r], _stable_top_bits(asc[d - n_top:], n_top, d)), (d, r)
+            desc = np.argsort(-canon, kind="stable")  # value desc, ties by index
+            assert np.array_equal(qb[r], _stable_top_bits(desc[:n_top], n_top, d)), (d, r)
+            assert sum(bin(int(v)).count("1") for v in bm[r]) == n_top
+
+
+def test_bitmap_search_and_top_m_orders(oracle):
+    rng = np.random.default_rng(42)
+    d, n_top = 128, 32
+    docs = np.round(rng.standard_normal((500, d))).astype(np.float32)
+    docs[100:140] = docs[7]  # duplicates tie
+    bm = oracle.bitmap_encode(docs, n_top)
+    qb = oracle.bitmap_query(docs[:6], n_top)
+    ov = oracle.and_popcount_all(bm, qb)
+    s, i = oracle.bitmap_search(bm, qb, 50)
+    tm = oracle.bitmap_top_m(bm, qb, 50)
+    for q in range(6):
+        want = sorted(range(500), key=lambda j: (-int(ov[q, j]), j))[:50]
+        assert list(i[q]) == want and list(tm[q]) == want
+        assert np.array_equal(s[q], ov[q, want].astype(np.float32))
+    # identical docs: ids 0..k-1 with equal scores (tests/determinism_contract.rs:40-49, 135-150)
+    same = oracle.bitmap_encode(np.repeat(docs[:1], 10, axis=0), n_top)
+    s, i = oracle.bitmap_search(same, oracle.bitmap_query(docs[3], n_top), 4)
+    assert list(i[0]) == [0, 1, 2, 3] and len(set(s[0])) == 1
+    assert list(oracle.bitmap_top_m(same, oracle.bitmap_query(docs[3], n_top), 5)[0]) == [0, 1, 2, 3, 4]
