@@ -164,6 +164,9 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t *index, const ordvec_se
 #define ORDVEC_FLAG_DEVICE_PTRS 1
 /* Caller guarantees inputs are finite; skips the NaN/Inf scan. */
 #define ORDVEC_FLAG_ASSUME_FINITE 2
+/* ordvec_fastscan_batch_run only: replay the launch sequence eagerly instead
+ * of through its captured HIP graph (same results). */
+#define ORDVEC_FLAG_NO_GRAPH 4
 
 /* Engine limits of this build. */
 #define ORDVEC_MAX_TOPK 1024
@@ -300,6 +303,43 @@ ordvec_status_t ordvec_fastscan_info(const ordvec_fastscan_t *index, uint64_t *d
 ordvec_status_t ordvec_fastscan_write(const ordvec_fastscan_t *index, const char *path);
 ordvec_status_t ordvec_fastscan_load(const char *path, ordvec_fastscan_t **out);
 void ordvec_fastscan_free(ordvec_fastscan_t *index);
+
+/* FastScan b=4 (no reference counterpart: the reference's FastScan is b=2).
+ * The b=2 recipe carried to one 4-bit rank code per coordinate: per query a
+ * u8 LUT of 16 entries per coordinate, entry c = q_unit[j] * (c - 7.5) under
+ * one global affine scale (round half away from zero), exact u32 sums, score
+ * = (d * g_min + acc / scale) / rankquant_norm(d, 4), top-k by (score desc,
+ * id asc).  bits = 2 is RankQuantFastscan; bits = 4 needs dim % 8 == 0 and
+ * dim <= 2048, k <= 256, one add() as for b=2, and has no .ovfs form. */
+ordvec_status_t ordvec_fastscan_new_bits(uint64_t dim, uint32_t bits, ordvec_fastscan_t **out);
+ordvec_status_t ordvec_fastscan_bits(const ordvec_fastscan_t *index, uint32_t *bits);
+
+/* Batched multi-index FastScan b=4: many indexes of one dim, each with its
+ * own query group (queries_per_index[i] rows, in index order, of the query
+ * array) and k.  The whole batch is one launch sequence (normalise, LUTs, one
+ * persistent scan kernel pulling (index, query group, document range) tiles,
+ * per-query merge) captured once in a HIP graph and replayed by run().
+ * Results equal ordvec_fastscan_search on each index.  Outputs are
+ * total_queries x kout (kout = max over indexes of min(k_i, n_i)); row q holds
+ * min(k_i, n_i) entries, the rest -inf / -1.  autotune() times tile shapes
+ * (queries per tile, 64-document blocks per tile, persistent grid) on the
+ * given queries, keeps the fastest, re-captures the graph and remembers the
+ * choice for batches of the same shape class. */
+typedef struct ordvec_fastscan_batch_t ordvec_fastscan_batch_t;
+ordvec_status_t ordvec_fastscan_batch_new(const ordvec_fastscan_t *const *indexes, uint64_t n_indexes,
+                                          const uint32_t *queries_per_index, const uint32_t *k_per_index,
+                                          ordvec_fastscan_batch_t **out);
+ordvec_status_t ordvec_fastscan_batch_autotune(ordvec_fastscan_batch_t *batch, const float *queries, uint32_t reps,
+                                               uint64_t flags);
+ordvec_status_t ordvec_fastscan_batch_run(ordvec_fastscan_batch_t *batch, const float *queries, float *out_scores,
+                                          int64_t *out_ids, uint64_t flags);
+ordvec_status_t ordvec_fastscan_batch_config(ordvec_fastscan_batch_t *batch, uint64_t *total_queries, uint32_t *kout,
+                                             uint32_t *queries_per_tile, uint32_t *blocks_per_tile, uint32_t *grid,
+                                             uint64_t *n_tiles, uint64_t *n_tuned);
+/* Row i of the last autotune: the configuration and its best time (ms). */
+ordvec_status_t ordvec_fastscan_batch_tuning(ordvec_fastscan_batch_t *batch, uint64_t i, uint32_t *queries_per_tile,
+                                             uint32_t *blocks_per_tile, uint32_t *grid, double *ms);
+void ordvec_fastscan_batch_free(ordvec_fastscan_batch_t *batch);
 
 /* ---- Differentiable rerank -------------------------------------------------
  * No reference counterpart: the forward is the reference's asymmetric score

@@ -66,6 +66,8 @@ def _declare(L):
         "orc_rq_scores": (None, [P, P, U64, U32, U32, P, C.c_int, P]),
         "orc_score_ids_grad": (None, [P, U32, U32, P, U64, P, U64, P, P]),
         "orc_bitmap_encode": (None, [P, U64, U32, U32, P]),
+        "orc_fs4_query": (None, [P, U32, P, P, P]),
+        "orc_fs4_search": (None, [P, U64, U32, P, U64, U64, P, P]),
         "orc_bitmap_query": (None, [P, U64, U32, U32, P]),
         "orc_and_popcount_all": (None, [P, U64, U32, P, U64, P]),
         "orc_topk": (None, [P, P, U64, U64, P, P]),
@@ -393,6 +395,31 @@ def fs_search(packed, n: int, d: int, queries, k: int):
     oi = np.full((nq, kk), -1, dtype=np.int64)
     if kk:
         lib().orc_fs_search(_p(p), n, d, _p(q), nq, kk, _p(os_), _p(oi))
+    return os_, oi
+
+
+def fs4_query(q_unit):
+    """b=4 FastScan query LUT (d x 16 u8), bias_sum, inv_q."""
+    q = _f32(q_unit)
+    d = q.shape[0]
+    lut = np.empty(d * 16, dtype=np.uint8)
+    bias = C.c_float()
+    invq = C.c_float()
+    lib().orc_fs4_query(_p(q), d, _p(lut), C.byref(bias), C.byref(invq))
+    return lut, np.float32(bias.value), np.float32(invq.value)
+
+
+def fs4_search(codes, d: int, queries, k: int):
+    """b=4 FastScan over RankQuant b=4 codes (n x d/2 bytes)."""
+    c = np.ascontiguousarray(codes, dtype=np.uint8).reshape(-1, d // 2)
+    n = c.shape[0]
+    q = _f32(queries).reshape(-1, d)
+    nq = q.shape[0]
+    kk = min(k, n)
+    os_ = np.zeros((nq, kk), dtype=np.float32)
+    oi = np.full((nq, kk), -1, dtype=np.int64)
+    if kk:
+        lib().orc_fs4_search(_p(c), n, d, _p(q), nq, kk, _p(os_), _p(oi))
     return os_, oi
 
 

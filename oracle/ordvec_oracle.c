@@ -834,6 +834,81 @@ void orc_fs_search(const uint8_t *packed, uint64_t n, uint32_t d, const float *q
 }
 
 /* ------------------------------------------------------------------ */
+/* FastScan b=4: the b=2 recipe (src/fastscan.rs:130-173, 316-476) carried
+ * to one 4-bit rank code per coordinate.  The reference ships b=2 only; this
+ * is the extension the b=4 GPU path implements, stated once here:
+ *   lut_f[j*16 + c] = q_unit[j] * centre(c, 4)      (centre = c - 7.5)
+ *   g_min/g_max over all entries, span = max(g_max - g_min, 1e-12),
+ *   scale = 255/span, inv_q = 1/scale, bias_sum = d * g_min,
+ *   lut_u8 = clamp(round_half_away((v - g_min) * scale), 0, 255),
+ *   acc = sum_j lut_u8[j*16 + code_j] (exact u32),
+ *   score = (bias_sum + acc * inv_q) * (1 / rankquant_norm(d, 4)) + 0.0,
+ * top-k by TopK (score desc, id asc).  Codes are RankQuant b=4 rows (two
+ * codes per byte, first code in the high nibble). */
+/* ------------------------------------------------------------------ */
+
+void orc_fs4_query(const float *q, uint32_t d, uint8_t *lut_u8, float *bias_sum, float *inv_q) {
+    float *lut = (float *)malloc((size_t)d * 16 * sizeof(float));
+    float gmin = INFINITY, gmax = -INFINITY;
+    for (uint32_t j = 0; j < d; j++)
+        for (uint32_t c = 0; c < 16; c++) {
+            float v = q[j] * ((float)c - 7.5f);
+            lut[j * 16 + c] = v;
+            if (v < gmin) gmin = v;
+            if (v > gmax) gmax = v;
+        }
+    float span = gmax - gmin;
+    if (!(span >= 1e-12f)) span = 1e-12f;
+    float scale = 255.0f / span;
+    *inv_q = 1.0f / scale;
+    *bias_sum = (float)d * gmin;
+    for (uint32_t i = 0; i < d * 16; i++) {
+        float v = round_half_away((lut[i] - gmin) * scale);
+        if (v < 0.0f) v = 0.0f;
+        if (v > 255.0f) v = 255.0f;
+        lut_u8[i] = (uint8_t)v;
+    }
+    free(lut);
+}
+
+void orc_fs4_search(const uint8_t *codes, uint64_t n, uint32_t d, const float *queries, uint64_t nq, uint64_t k,
+                    float *os, int64_t *oi) {
+    if (k > n) k = n;
+    if (k == 0) return;
+    uint32_t bpv = d / 2;
+    float inv_norm = 1.0f / orc_rankquant_norm(d, 4);
+#pragma omp parallel
+    {
+        float *qu = (float *)malloc(d * sizeof(float));
+        uint8_t *lut = (uint8_t *)malloc((size_t)d * 16);
+        topk_t t;
+        topk_init(&t, k);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t qi = 0; qi < (int64_t)nq; qi++) {
+            float bias, invq;
+            orc_l2_normalise(queries + (uint64_t)qi * d, d, qu);
+            orc_fs4_query(qu, d, lut, &bias, &invq);
+            t.filled = 0;
+            t.worst_pos = 0;
+            t.worst_val = INFINITY;
+            t.worst_tie = INT64_MAX;
+            for (uint64_t i = 0; i < n; i++) {
+                const uint8_t *row = codes + i * bpv;
+                uint32_t acc = 0;
+                for (uint32_t b = 0; b < bpv; b++)
+                    acc += lut[(2 * b) * 16 + (row[b] >> 4)] + lut[(2 * b + 1) * 16 + (row[b] & 15)];
+                float raw = bias + (float)acc * invq;
+                topk_insert(&t, raw * inv_norm + 0.0f, (int64_t)i, (int64_t)i);
+            }
+            topk_finalize(&t, os + (uint64_t)qi * k, oi + (uint64_t)qi * k, k);
+        }
+        topk_free(&t);
+        free(qu);
+        free(lut);
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Bitmap (top-bucket, AND-popcount): src/bitmap.rs                     */
 /* ------------------------------------------------------------------ */
 

@@ -13,6 +13,7 @@ from .index import (  # noqa: E402,F401
     MAX_SIGN_BITMAP_DIM,
     MAX_VECTORS,
     Bitmap,
+    FastscanBatch,
     RankQuant,
     RankQuantFastscan,
     SignBitmap,
@@ -29,6 +30,7 @@ except ImportError:  # pragma: no cover
 __all__ = [
     "RankQuant",
     "Bitmap",
+    "FastscanBatch",
     "SignBitmap",
     "RankQuantFastscan",
     "two_stage_search",

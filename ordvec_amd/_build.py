@@ -16,7 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libordvec_amd.so")
-SOURCES = ["kernels.hip", "capi.cpp"]
+SOURCES = ["kernels.hip", "fastscan4.hip", "capi.cpp"]
 
 FLAGS = [
     "-O3",

@@ -28,6 +28,7 @@ INTERNAL = 13
 
 FLAG_DEVICE_PTRS = 1
 FLAG_ASSUME_FINITE = 2
+FLAG_NO_GRAPH = 4
 
 MAX_TOPK = 1024
 MAX_STAGE1_M = 4096
@@ -172,6 +173,14 @@ SIGNATURES = {
     "ordvec_fastscan_write": (ST, [P, C.c_char_p]),
     "ordvec_fastscan_load": (ST, [C.c_char_p, C.POINTER(P)]),
     "ordvec_fastscan_free": (None, [P]),
+    "ordvec_fastscan_new_bits": (ST, [U64, U32, C.POINTER(P)]),
+    "ordvec_fastscan_bits": (ST, [P, P]),
+    "ordvec_fastscan_batch_new": (ST, [P, U64, P, P, C.POINTER(P)]),
+    "ordvec_fastscan_batch_autotune": (ST, [P, P, U32, U64]),
+    "ordvec_fastscan_batch_run": (ST, [P, P, P, P, U64]),
+    "ordvec_fastscan_batch_config": (ST, [P, P, P, P, P, P, P, P]),
+    "ordvec_fastscan_batch_tuning": (ST, [P, U64, P, P, P, P]),
+    "ordvec_fastscan_batch_free": (None, [P]),
     "ordvec_rankquant_score_ids": (ST, [P, P, U64, P, U64, P, U64]),
     "ordvec_rankquant_score_ids_backward": (ST, [P, P, U64, P, U64, P, P, U64]),
     "ordvec_server_start": (ST, [P, P, U64, U64, U32, U64, U64, C.POINTER(P)]),

@@ -14,10 +14,12 @@
 #include <fstream>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <random>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <string>
 #include <vector>
@@ -262,14 +264,21 @@ struct SbIndex {
     }
 };
 
+// RankQuantFastscan: b=2 in the reference's block-32 nibble layout, or b=4
+// (the extension) as RankQuant b=4 rows regrouped into 64-document blocks of
+// interleaved dwords.
 struct FsIndex {
-    uint32_t dim = 0;
+    uint32_t dim = 0, bits = 2;
     uint64_t n = 0;
-    uint8_t* packed = nullptr;
+    uint8_t* packed = nullptr;    // b = 2
+    uint32_t* packed4 = nullptr;  // b = 4
     ~FsIndex() {
         if (packed) (void)hipFree(packed);
+        if (packed4) (void)hipFree(packed4);
     }
     uint64_t payload() const { return ((n + 31) / 32) * (uint64_t)(dim / 2) * 32; }
+    uint64_t payload4() const { return ((n + 63) / 64) * (uint64_t)(dim / 8) * 64 * 4; }
+    uint32_t blocks64() const { return (uint32_t)((n + 63) / 64); }
 };
 
 // ABI v1 handle: a RankQuant (.ovrq) or a top-bucket Bitmap (.ovbm).
@@ -922,6 +931,239 @@ size_t serve_ws_bytes(const SbIndex* sb, const RqIndex* rq, uint64_t cap, uint64
     for (uint64_t nq = 1; nq <= std::min<uint64_t>(cap, 512); nq++) b = std::max(b, two_stage_bytes(sb, rq, nq, m, k));
     return b;
 }
+
+// ---------------------------------------------------------------------------
+// FastScan b=4: tile tables, the plain per-index search and the batched
+// multi-index form (persistent scan captured in a HIP graph, autotuned tiles).
+// ---------------------------------------------------------------------------
+struct Fs4Config {
+    uint32_t qg = 1;          // LUT groups of four queries per tile (1 or 2)
+    uint32_t seg_blocks = 64; // 64-document blocks per tile
+    uint32_t grid = 512;      // persistent workgroups
+};
+
+struct Fs4Tables {
+    std::vector<Fs4IndexDesc> idx;
+    std::vector<Fs4Group> groups;
+    std::vector<Fs4Tile> tiles;
+    std::vector<Fs4Merge> merges;
+    uint32_t nq = 0, kcap = 1;
+};
+
+// Queries of index i are rows [q_base_i, q_base_i + nq_i) of the batch.
+Fs4Tables fs4_tables(const std::vector<const FsIndex*>& ixs, const std::vector<uint32_t>& nq_per,
+                     const std::vector<uint32_t>& k_per, const Fs4Config& c) {
+    Fs4Tables t;
+    uint32_t q_base = 0;
+    for (size_t i = 0; i < ixs.size(); i++) {
+        const FsIndex* ix = ixs[i];
+        t.idx.push_back(Fs4IndexDesc{ix->packed4, ix->n});
+        const uint32_t nq = nq_per[i];
+        const uint32_t k = (uint32_t)std::min<uint64_t>(k_per[i], ix->n);
+        t.kcap = std::max(t.kcap, k);
+        const uint32_t g_first = (uint32_t)t.groups.size();
+        for (uint32_t q = 0; q < nq; q += 4) t.groups.push_back(Fs4Group{q_base + q, std::min(4u, nq - q)});
+        const uint32_t ng = (uint32_t)t.groups.size() - g_first;
+        const uint32_t nblk = ix->blocks64();
+        const uint32_t nseg = nblk ? (nblk + c.seg_blocks - 1) / c.seg_blocks : 0;
+        for (uint32_t gs = 0; gs < ng; gs += c.qg) {
+            const uint32_t t0 = (uint32_t)t.tiles.size();
+            const uint32_t gn = std::min(c.qg, ng - gs);
+            for (uint32_t seg = 0; seg < nseg; seg++)
+                t.tiles.push_back(Fs4Tile{(uint32_t)i, g_first + gs, gn, seg * c.seg_blocks,
+                                          std::min(nblk, (seg + 1) * c.seg_blocks), 0});
+            for (uint32_t g = 0; g < gn; g++)
+                for (uint32_t s = 0; s < t.groups[g_first + gs + g].count; s++)
+                    t.merges.push_back(Fs4Merge{t0, nseg, g * 4 + s, nseg ? k : 0});
+        }
+        q_base += nq;
+    }
+    t.nq = q_base;
+    return t;
+}
+
+// Device copy of the tables plus every per-batch buffer, one allocation.
+struct Fs4Dev {
+    DevBuf buf;
+    Fs4IndexDesc* idx = nullptr;
+    Fs4Group* groups = nullptr;
+    Fs4Tile* tiles = nullptr;
+    Fs4Merge* merges = nullptr;
+    float *dq = nullptr, *qhat = nullptr, *off = nullptr, *bq = nullptr, *out_s = nullptr;
+    uint32_t *qstat = nullptr, *counter = nullptr;
+    uint8_t* lut = nullptr;
+    uint64_t* partial = nullptr;
+    int64_t* out_i = nullptr;
+    size_t partial_bytes = 0;
+    uint32_t ntiles = 0, ngroups = 0, nq = 0, kcap = 1;
+};
+
+size_t fs4_dev_bytes(const Fs4Tables& t, uint32_t d, uint32_t qg, size_t partial_floor) {
+    const uint64_t nq = t.nq;
+    const size_t part = std::max<size_t>(partial_floor, (size_t)t.tiles.size() * 4 * qg * t.kcap * 8);
+    return al(t.idx.size() * sizeof(Fs4IndexDesc)) + al(t.groups.size() * sizeof(Fs4Group)) +
+           al(t.tiles.size() * sizeof(Fs4Tile)) + al(t.merges.size() * sizeof(Fs4Merge)) + 3 * al(nq * d * 4) +
+           al(nq * 4) * 2 + al(nq * 8) + al(t.groups.size() * (uint64_t)d * 64) + al(part) + al(256) +
+           al(nq * t.kcap * 4) + al(nq * t.kcap * 8);
+}
+
+// Carves the batch buffers out of `base` (fs4_dev_bytes long) and uploads the
+// tables.
+void fs4_upload(Fs4Dev& dv, const Fs4Tables& t, uint32_t d, uint32_t qg, size_t partial_floor, void* base) {
+    Carver cv{static_cast<char*>(base)};
+    const uint64_t nq = t.nq;
+    dv.idx = cv.take<Fs4IndexDesc>(t.idx.size());
+    dv.groups = cv.take<Fs4Group>(t.groups.size());
+    dv.tiles = cv.take<Fs4Tile>(t.tiles.size());
+    dv.merges = cv.take<Fs4Merge>(t.merges.size());
+    dv.dq = cv.take<float>(nq * d);
+    dv.qhat = cv.take<float>(nq * d);
+    cv.take<float>(nq * d);
+    dv.off = cv.take<float>(nq);
+    dv.qstat = cv.take<uint32_t>(nq);
+    dv.bq = cv.take<float>(nq * 2);
+    dv.lut = cv.take<uint8_t>(t.groups.size() * (uint64_t)d * 64);
+    dv.partial_bytes = std::max<size_t>(partial_floor, (size_t)t.tiles.size() * 4 * qg * t.kcap * 8);
+    dv.partial = cv.take<uint64_t>(dv.partial_bytes / 8);
+    dv.counter = cv.take<uint32_t>(64);
+    dv.out_s = cv.take<float>(nq * t.kcap);
+    dv.out_i = cv.take<int64_t>(nq * t.kcap);
+    h2d(dv.idx, t.idx.data(), t.idx.size() * sizeof(Fs4IndexDesc));
+    h2d(dv.groups, t.groups.data(), t.groups.size() * sizeof(Fs4Group));
+    h2d(dv.tiles, t.tiles.data(), t.tiles.size() * sizeof(Fs4Tile));
+    h2d(dv.merges, t.merges.data(), t.merges.size() * sizeof(Fs4Merge));
+    dv.ntiles = (uint32_t)t.tiles.size();
+    dv.ngroups = (uint32_t)t.groups.size();
+    dv.nq = t.nq;
+    dv.kcap = t.kcap;
+}
+
+// The launch sequence of one batch: normalise, LUTs, scan, merge.
+// persistent: one fixed-grid launch pulling tiles; else one workgroup per tile.
+void fs4_enqueue(const Fs4Dev& dv, uint32_t d, const Fs4Config& c, bool persistent, bool prepare = true) {
+    hipStream_t s = cur().stream;
+    const float inv_norm = rankquant_inv_norm(d, 4);
+    if (prepare) {
+        launch_prepare_queries(dv.dq, dv.nq, d, 4, 0, inv_norm, dv.qhat, dv.off, dv.qstat, s);
+        launch_fs4_lut(dv.qhat, d, dv.groups, dv.ngroups, dv.lut, dv.bq, s);
+    }
+    launch_fs4_scan(c.qg, persistent ? c.grid : dv.ntiles, dv.idx, dv.tiles, dv.ntiles,
+                    persistent ? dv.counter : nullptr, dv.groups, dv.lut, dv.bq, d, inv_norm, dv.kcap, dv.partial, s);
+    launch_fs4_merge(dv.partial, dv.merges, dv.nq, 4 * c.qg, dv.kcap, dv.kcap, dv.out_s, dv.out_i, s);
+}
+
+bool fs4_config_fits(uint32_t qg, uint32_t d) { return fs4_scan_lds_bytes(qg, d) + 1024 <= 160 * 1024; }
+
+// Plain search of one b=4 index (nq queries, top-k).
+void fs4_plain_search(const FsIndex* ix, const float* queries, uint64_t nq, uint64_t kk, float* out_s, int64_t* out_i,
+                      uint64_t flags) {
+    const uint32_t d = ix->dim;
+    Fs4Config c;
+    c.qg = nq > 4 && fs4_config_fits(2, d) ? 2 : 1;
+    const uint64_t gsets = (nq + 4 * c.qg - 1) / (4 * c.qg);
+    const uint64_t want = std::max<uint64_t>(1, (2048 + gsets - 1) / gsets);
+    c.seg_blocks = (uint32_t)std::max<uint64_t>(1, (ix->blocks64() + want - 1) / want);
+    Fs4Tables t = fs4_tables({ix}, {(uint32_t)nq}, {(uint32_t)kk}, c);
+    Fs4Dev dv;
+    fs4_upload(dv, t, d, c.qg, 0, cur().workspace(fs4_dev_bytes(t, d, c.qg, 0)));
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    if (dev) d2d(dv.dq, queries, nq * d * 4);
+    else h2d(dv.dq, queries, nq * d * 4);
+    fs4_enqueue(dv, d, c, false);
+    if (dev) {
+        OVX_HIP(hipMemcpy2DAsync(out_s, kk * 4, dv.out_s, dv.kcap * 4, kk * 4, nq, hipMemcpyDeviceToDevice, cur().stream));
+        OVX_HIP(hipMemcpy2DAsync(out_i, kk * 8, dv.out_i, dv.kcap * 8, kk * 8, nq, hipMemcpyDeviceToDevice, cur().stream));
+    } else {
+        OVX_HIP(hipMemcpy2DAsync(out_s, kk * 4, dv.out_s, dv.kcap * 4, kk * 4, nq, hipMemcpyDeviceToHost, cur().stream));
+        OVX_HIP(hipMemcpy2DAsync(out_i, kk * 8, dv.out_i, dv.kcap * 8, kk * 8, nq, hipMemcpyDeviceToHost, cur().stream));
+    }
+    std::vector<uint32_t> st = read_qstat(dv.qstat, nq);
+    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
+        fail(kNonfiniteQuery, "queries contain NaN or infinity");
+}
+
+// Autotuned configurations, keyed by the batch's shape class.
+struct Fs4TuneKey {
+    uint32_t d, kcap, log2_docs, log2_queries;
+    bool operator<(const Fs4TuneKey& o) const {
+        return std::tie(d, kcap, log2_docs, log2_queries) < std::tie(o.d, o.kcap, o.log2_docs, o.log2_queries);
+    }
+};
+std::mutex g_tune_mu;
+std::map<Fs4TuneKey, Fs4Config>& tune_cache() {
+    static std::map<Fs4TuneKey, Fs4Config>* m = new std::map<Fs4TuneKey, Fs4Config>();
+    return *m;
+}
+uint32_t ilog2(uint64_t x) {
+    uint32_t r = 0;
+    while (x > 1) {
+        x >>= 1;
+        r++;
+    }
+    return r;
+}
+
+struct Fs4TuneRow {
+    Fs4Config c;
+    double ms;
+};
+
+struct Fs4Batch {
+    Ctx ctx;
+    std::vector<const FsIndex*> ixs;
+    std::vector<uint32_t> nq_per, k_per;
+    uint32_t d = 0;
+    Fs4Config cfg;
+    Fs4Tables tables;
+    Fs4Dev dv;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<Fs4TuneRow> tuning;
+    Fs4TuneKey key{};
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    // Tables for `c`, and the graph of the whole launch sequence.
+    void build(const Fs4Config& c) {
+        drop_graph();
+        cfg = c;
+        tables = fs4_tables(ixs, nq_per, k_per, c);
+        const size_t bytes = fs4_dev_bytes(tables, d, c.qg, 0);
+        if (bytes > dv.buf.size && dv.buf.p) {
+            sync();
+            (void)hipFree(dv.buf.p);
+            dv.buf.p = nullptr;
+            dv.buf.size = 0;
+        }
+        fs4_upload(dv, tables, d, c.qg, 0, dv.buf.get(bytes, cur().stream));
+        sync();
+        OVX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            fs4_enqueue(dv, d, cfg, true);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(ctx.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        OVX_HIP(hipStreamEndCapture(ctx.stream, &graph));
+        OVX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    ~Fs4Batch() {
+        {
+            CtxScope scope(&ctx);
+            (void)hipStreamSynchronize(ctx.stream);
+            drop_graph();
+            if (dv.buf.p) (void)hipFree(dv.buf.p);
+            ctx.release();
+        }
+        if (ctx.stream) (void)hipStreamDestroy(ctx.stream);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Persisted formats (src/rank_io.rs; docs/PERSISTED_FORMAT.md)
@@ -2246,6 +2488,31 @@ ordvec_status_t ordvec_fastscan_new(uint64_t dim, ordvec_fastscan_t** out) {
     });
 }
 
+ordvec_status_t ordvec_fastscan_new_bits(uint64_t dim, uint32_t bits, ordvec_fastscan_t** out) {
+    if (bits == 2) return ordvec_fastscan_new(dim, out);
+    return boundary([&] {
+        NONNULL(out, "out");
+        *out = nullptr;
+        if (bits != 4) fail(kBadArgument, "FastScan bits must be 2 or 4");
+        if (dim < 8 || dim % 8) fail(kBadArgument, "FastScan b=4: dim " + std::to_string(dim) + " must be a positive multiple of 8");
+        if (dim > kFs4MaxDim) fail(kUnsupportedOperation, "FastScan b=4 supports dim <= " + std::to_string(kFs4MaxDim));
+        LOCK;
+        eng().init();
+        FsIndex* ix = new FsIndex();
+        ix->dim = (uint32_t)dim;
+        ix->bits = 4;
+        *out = reinterpret_cast<ordvec_fastscan_t*>(ix);
+    });
+}
+
+ordvec_status_t ordvec_fastscan_bits(const ordvec_fastscan_t* index, uint32_t* bits) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        NONNULL(bits, "bits");
+        *bits = FS(index)->bits;
+    });
+}
+
 ordvec_status_t ordvec_fastscan_add(ordvec_fastscan_t* index, const float* vectors, uint64_t n, uint64_t flags) {
     return boundary([&] {
         NONNULL(index, "index");
@@ -2261,10 +2528,22 @@ ordvec_status_t ordvec_fastscan_add(ordvec_fastscan_t* index, const float* vecto
         check_new_count(0, n);
         RqIndex tmp;
         tmp.dim = ix->dim;
-        tmp.bits = 2;
-        tmp.bpv = ix->dim / 4;
+        tmp.bits = ix->bits;
+        tmp.bpv = ix->dim * ix->bits / 8;
         rq_add(&tmp, vectors, n, flags | ORDVEC_FLAG_ASSUME_FINITE);
         ix->n = n;
+        if (ix->bits == 4) {
+            const uint64_t bytes4 = ix->payload4();
+            if (hipMalloc(&ix->packed4, bytes4) != hipSuccess) {
+                (void)hipGetLastError();
+                ix->n = 0;
+                fail(kInternal, "HBM allocation for FastScan payload failed");
+            }
+            OVX_HIP(hipMemsetAsync(ix->packed4, 0, bytes4, cur().stream));
+            launch_fs4_pack(tmp.codes, n, ix->dim, ix->packed4, cur().stream);
+            sync();
+            return;
+        }
         uint64_t bytes = ix->payload();
         if (hipMalloc(&ix->packed, bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -2291,6 +2570,12 @@ ordvec_status_t ordvec_fastscan_search(const ordvec_fastscan_t* index, const flo
         if (ix->dim > 4096) fail(kUnsupportedOperation, "FastScan LUT staging supports dim <= 4096");
         host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
         LOCK;
+        if (ix->bits == 4) {
+            if (kk > kFs4MaxK) fail(kUnsupportedOperation, "FastScan b=4 supports k <= " + std::to_string(kFs4MaxK));
+            eng().init();
+            fs4_plain_search(ix, queries, nq, kk, out_scores, out_ids, flags);
+            return;
+        }
         const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
         const uint32_t d = ix->dim;
         size_t wk = fullscan_work_bytes(ix->n, nq, (uint32_t)kk);
@@ -2331,6 +2616,8 @@ ordvec_status_t ordvec_fastscan_write(const ordvec_fastscan_t* index, const char
     return boundary([&] {
         NONNULL(index, "index");
         NONNULL(path, "path");
+        if (FS(index)->bits != 2)
+            fail(kUnsupportedOperation, "the .ovfs format holds FastScan b=2 indexes only");
         LOCK;
         write_ovfs(FS(index), path);
     });
@@ -2345,6 +2632,210 @@ ordvec_status_t ordvec_fastscan_load(const char* path, ordvec_fastscan_t** out) 
         eng().init();
         *out = reinterpret_cast<ordvec_fastscan_t*>(load_ovfs(path));
     });
+}
+
+// ---- batched multi-index FastScan b=4 -------------------------------------------
+
+struct ordvec_fastscan_batch_t {};
+static Fs4Batch* FB(ordvec_fastscan_batch_t* p) { return reinterpret_cast<Fs4Batch*>(p); }
+
+static uint32_t device_cus() {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    return (uint32_t)cus;
+}
+
+ordvec_status_t ordvec_fastscan_batch_new(const ordvec_fastscan_t* const* indexes, uint64_t n_indexes,
+                                          const uint32_t* queries_per_index, const uint32_t* k_per_index,
+                                          ordvec_fastscan_batch_t** out) {
+    return boundary([&] {
+        NONNULL(out, "out");
+        *out = nullptr;
+        if (n_indexes == 0) fail(kBadArgument, "a batch needs at least one index");
+        NONNULL(indexes, "indexes");
+        NONNULL(queries_per_index, "queries_per_index");
+        NONNULL(k_per_index, "k_per_index");
+        std::unique_ptr<Fs4Batch> b(new Fs4Batch());
+        uint64_t docs = 0, nq = 0;
+        for (uint64_t i = 0; i < n_indexes; i++) {
+            NONNULL(indexes[i], "indexes[i]");
+            const FsIndex* ix = FS(indexes[i]);
+            if (ix->bits != 4) fail(kBadArgument, "batched FastScan takes b=4 indexes");
+            if (i == 0) b->d = ix->dim;
+            if (ix->dim != b->d) fail(kDimMismatch, "every index of a batch must have the same dim");
+            if (k_per_index[i] == 0 || k_per_index[i] > kFs4MaxK)
+                fail(kBadArgument, "k must be in [1, " + std::to_string(kFs4MaxK) + "]");
+            b->ixs.push_back(ix);
+            b->nq_per.push_back(queries_per_index[i]);
+            b->k_per.push_back(k_per_index[i]);
+            docs += ix->n;
+            nq += queries_per_index[i];
+        }
+        if (nq == 0) fail(kBadArgument, "a batch needs at least one query");
+        if (nq > (1u << 26)) fail(kBadArgument, "too many queries in one batch");
+        LOCK;
+        eng().init();
+        OVX_HIP(hipStreamCreateWithFlags(&b->ctx.stream, hipStreamNonBlocking));
+        CtxScope scope(&b->ctx);
+        uint32_t kcap = 1;
+        for (size_t i = 0; i < b->ixs.size(); i++) kcap = std::max<uint32_t>(kcap, (uint32_t)std::min<uint64_t>(b->k_per[i], b->ixs[i]->n));
+        b->key = Fs4TuneKey{b->d, kcap, ilog2(std::max<uint64_t>(docs, 1)), ilog2(nq)};
+        Fs4Config c;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = tune_cache().find(b->key);
+            if (it != tune_cache().end()) {
+                c = it->second;
+            } else {
+                const bool two = fs4_config_fits(2, b->d) && nq >= 8 * n_indexes;
+                c.qg = two ? 2 : 1;
+                c.seg_blocks = 64;
+                c.grid = device_cus() * (two ? 1 : 2);
+            }
+        }
+        b->build(c);
+        *out = reinterpret_cast<ordvec_fastscan_batch_t*>(b.release());
+    });
+}
+
+// Queries of the batch (rows ordered by index, queries_per_index[i] each) into
+// the batch's buffer on its stream.
+static void fs4_stage(Fs4Batch* b, const float* queries, uint64_t flags) {
+    const uint64_t count = (uint64_t)b->dv.nq * b->d;
+    if (flags & ORDVEC_FLAG_DEVICE_PTRS) d2d(b->dv.dq, queries, count * 4);
+    else h2d(b->dv.dq, queries, count * 4);
+}
+
+ordvec_status_t ordvec_fastscan_batch_autotune(ordvec_fastscan_batch_t* batch, const float* queries, uint32_t reps,
+                                               uint64_t flags) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        NONNULL(queries, "queries");
+        Fs4Batch* b = FB(batch);
+        CtxScope scope(&b->ctx);
+        host_finite_or_fail(queries, (uint64_t)b->dv.nq * b->d, flags, kNonfiniteQuery, "queries");
+        if (reps == 0) reps = 2;
+        const uint32_t cus = device_cus();
+        std::vector<Fs4Config> cands;
+        for (uint32_t qg : {1u, 2u}) {
+            if (!fs4_config_fits(qg, b->d)) continue;
+            for (uint32_t sb : {16u, 32u, 64u, 128u, 256u})
+                for (uint32_t gm : {1u, 2u, 4u}) cands.push_back(Fs4Config{qg, sb, cus * gm});
+        }
+        size_t bytes = 0;
+        for (const Fs4Config& c : cands) bytes = std::max(bytes, fs4_dev_bytes(fs4_tables(b->ixs, b->nq_per, b->k_per, c), b->d, c.qg, 0));
+        b->drop_graph();
+        if (bytes > b->dv.buf.size && b->dv.buf.p) {
+            sync();
+            (void)hipFree(b->dv.buf.p);
+            b->dv.buf.p = nullptr;
+            b->dv.buf.size = 0;
+        }
+        void* base = b->dv.buf.get(bytes, cur().stream);
+        hipEvent_t e0, e1;
+        OVX_HIP(hipEventCreate(&e0));
+        OVX_HIP(hipEventCreate(&e1));
+        b->tuning.clear();
+        try {
+            for (const Fs4Config& c : cands) {
+                Fs4Tables t = fs4_tables(b->ixs, b->nq_per, b->k_per, c);
+                fs4_upload(b->dv, t, b->d, c.qg, 0, base);
+                fs4_stage(b, queries, flags);
+                fs4_enqueue(b->dv, b->d, c, true);  // warm
+                double best = 1e30;
+                for (uint32_t r = 0; r < reps; r++) {
+                    OVX_HIP(hipEventRecord(e0, cur().stream));
+                    fs4_enqueue(b->dv, b->d, c, true);
+                    OVX_HIP(hipEventRecord(e1, cur().stream));
+                    OVX_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    OVX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    best = std::min(best, (double)ms);
+                }
+                b->tuning.push_back(Fs4TuneRow{c, best});
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        size_t bi = 0;
+        for (size_t i = 1; i < b->tuning.size(); i++)
+            if (b->tuning[i].ms < b->tuning[bi].ms) bi = i;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            tune_cache()[b->key] = b->tuning[bi].c;
+        }
+        b->build(b->tuning[bi].c);
+    });
+}
+
+ordvec_status_t ordvec_fastscan_batch_run(ordvec_fastscan_batch_t* batch, const float* queries, float* out_scores,
+                                          int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        Fs4Batch* b = FB(batch);
+        CtxScope scope(&b->ctx);
+        const uint64_t nq = b->dv.nq, kout = b->dv.kcap;
+        host_finite_or_fail(queries, nq * b->d, flags, kNonfiniteQuery, "queries");
+        fs4_stage(b, queries, flags);
+        if (flags & ORDVEC_FLAG_NO_GRAPH) fs4_enqueue(b->dv, b->d, b->cfg, true);
+        else OVX_HIP(hipGraphLaunch(b->exec, cur().stream));
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        if (dev) {
+            d2d(out_scores, b->dv.out_s, nq * kout * 4);
+            d2d(out_ids, b->dv.out_i, nq * kout * 8);
+        } else {
+            d2h(out_scores, b->dv.out_s, nq * kout * 4);
+            d2h(out_ids, b->dv.out_i, nq * kout * 8);
+        }
+        std::vector<uint32_t> st = read_qstat(b->dv.qstat, nq);
+        if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
+            fail(kNonfiniteQuery, "queries contain NaN or infinity");
+    });
+}
+
+ordvec_status_t ordvec_fastscan_batch_config(ordvec_fastscan_batch_t* batch, uint64_t* total_queries, uint32_t* kout,
+                                             uint32_t* queries_per_tile, uint32_t* blocks_per_tile,
+                                             uint32_t* grid, uint64_t* n_tiles, uint64_t* n_tuned) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        Fs4Batch* b = FB(batch);
+        if (total_queries) *total_queries = b->dv.nq;
+        if (kout) *kout = b->dv.kcap;
+        if (queries_per_tile) *queries_per_tile = 4 * b->cfg.qg;
+        if (blocks_per_tile) *blocks_per_tile = b->cfg.seg_blocks;
+        if (grid) *grid = b->cfg.grid;
+        if (n_tiles) *n_tiles = b->dv.ntiles;
+        if (n_tuned) *n_tuned = b->tuning.size();
+    });
+}
+
+ordvec_status_t ordvec_fastscan_batch_tuning(ordvec_fastscan_batch_t* batch, uint64_t i, uint32_t* queries_per_tile,
+                                             uint32_t* blocks_per_tile, uint32_t* grid, double* ms) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        Fs4Batch* b = FB(batch);
+        if (i >= b->tuning.size()) fail(kBadArgument, "tuning row out of range");
+        const Fs4TuneRow& r = b->tuning[i];
+        if (queries_per_tile) *queries_per_tile = 4 * r.c.qg;
+        if (blocks_per_tile) *blocks_per_tile = r.c.seg_blocks;
+        if (grid) *grid = r.c.grid;
+        if (ms) *ms = r.ms;
+    });
+}
+
+void ordvec_fastscan_batch_free(ordvec_fastscan_batch_t* batch) {
+    if (!batch) return;
+    delete FB(batch);
 }
 
 void ordvec_fastscan_free(ordvec_fastscan_t* index) {

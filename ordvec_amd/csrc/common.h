@@ -189,4 +189,37 @@ void launch_fullscan_fs(const uint8_t* packed, uint64_t n, uint32_t d, const uin
                         float inv_norm, uint64_t nq, uint32_t k, void* work, float* out_s, int64_t* out_i,
                         hipStream_t s);
 
+// ---- FastScan b=4 and the batched multi-index form ----------------------------
+// One index of a batch: codes in 64-document blocks of interleaved dwords.
+struct Fs4IndexDesc {
+    const uint32_t* packed;
+    uint64_t n;
+};
+// Up to four queries (consecutive rows q0 .. q0+count-1) sharing one LUT group.
+struct Fs4Group {
+    uint32_t q0, count;
+};
+// Work unit: groups g0 .. g0+ng-1 (ng <= QG) of one index x 64-doc blocks [blk0, blk1).
+struct Fs4Tile {
+    uint32_t index, g0, ng, blk0, blk1, pad;
+};
+// Per query: its partial top-k rows are ((t0 + seg) * 4*QG + slot) for seg < nseg.
+struct Fs4Merge {
+    uint32_t t0, nseg, slot, k;
+};
+constexpr uint32_t kFs4MaxK = 256;
+constexpr uint32_t kFs4MaxDim = 2048;
+
+size_t fs4_scan_lds_bytes(uint32_t qg, uint32_t d);
+void launch_fs4_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t* packed, hipStream_t s);
+void launch_fs4_lut(const float* qhat, uint32_t d, const Fs4Group* groups, uint32_t ngroups, uint8_t* lut,
+                    float* bias_invq, hipStream_t s);
+// counter != nullptr: persistent launch of `grid` workgroups pulling tiles
+// (the counter is reset inside the launch sequence, graph-safe).
+void launch_fs4_scan(uint32_t qg, uint32_t grid, const Fs4IndexDesc* indexes, const Fs4Tile* tiles, uint32_t ntiles,
+                     uint32_t* counter, const Fs4Group* groups, const uint8_t* lut, const float* bias_invq, uint32_t d,
+                     float inv_norm, uint32_t kcap, uint64_t* partial, hipStream_t s);
+void launch_fs4_merge(const uint64_t* partial, const Fs4Merge* merges, uint32_t nq, uint32_t qt, uint32_t kcap,
+                      uint32_t kout, float* out_s, int64_t* out_i, hipStream_t s);
+
 }  // namespace ovx

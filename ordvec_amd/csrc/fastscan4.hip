@@ -1,0 +1,300 @@
+// FastScan b=4 on CDNA4 and its batched multi-index form.
+//
+// Scoring is the b=2 FastScan recipe (src/fastscan.rs:130-173, 316-476)
+// carried to one 4-bit rank code per coordinate: a per-query u8 LUT of 16
+// entries per coordinate under one global affine scale, exact u32
+// accumulation, score = (bias_sum + acc * inv_q) * (1/R) + 0.0, top-k by
+// (score desc, id asc).
+//
+// Layout choices for the GPU:
+//  * codes: the RankQuant b=4 rows (first code in the high nibble), regrouped
+//    into 64-document blocks of interleaved dwords, packed[(blk*wpd + w)*64 +
+//    lane] = dword w of document blk*64 + lane (wpd = d/8).  A wave reads one
+//    dword of 64 documents per load (256 contiguous bytes).
+//  * LUTs: four queries interleaved per entry, lut[((g*d + j)*16 + c)*4 + s] =
+//    entry c of coordinate j for query s of group g.  One 32-bit LDS read
+//    fetches the entry for four queries; v_dot4_u32_u8 against a one-hot byte
+//    selector adds each query's byte to its own exact u32 accumulator.
+//
+// The scan kernel walks a table of tiles (index, query groups, 64-document
+// block range).  Launched with one workgroup per tile it is the plain
+// per-index path; launched persistent (a fixed grid pulling tiles from an
+// atomic counter) it runs every index of a batch in one launch.
+#include "common.h"
+#include "device_common.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+namespace ovx {
+
+namespace {
+
+constexpr uint32_t kFs4TkCap = 512;  // k (<= 256) + one round of 256 offers
+using Fs4TopK = TopKS<kFs4TkCap>;
+
+// Per query group of up to four queries: min/max of the f32 LUT under the
+// global affine quantisation, then the u8 entries in the interleaved layout.
+__global__ __launch_bounds__(256) void k_fs4_lut(const float* __restrict__ qhat, uint32_t d,
+                                                 const Fs4Group* __restrict__ groups, uint8_t* __restrict__ lut,
+                                                 float* __restrict__ bias_invq) {
+    __shared__ float smin[256], smax[256];
+    const uint64_t g = blockIdx.x;
+    const Fs4Group grp = groups[g];
+    uint8_t* lg = lut + g * (uint64_t)d * 64;
+    for (uint32_t s = 0; s < 4; s++) {
+        if (s >= grp.count) {
+            for (uint32_t t = threadIdx.x; t < d * 16; t += blockDim.x) lg[t * 4 + s] = 0;
+            continue;
+        }
+        const uint64_t q = grp.q0 + s;
+        const float* x = qhat + q * d;
+        float mn = INFINITY, mx = -INFINITY;
+        for (uint32_t t = threadIdx.x; t < d * 16; t += blockDim.x) {
+            const float v = __fmul_rn(x[t >> 4], (float)(t & 15) - 7.5f);
+            mn = fminf(mn, v);
+            mx = fmaxf(mx, v);
+        }
+        smin[threadIdx.x] = mn;
+        smax[threadIdx.x] = mx;
+        __syncthreads();
+        for (uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h) {
+                smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + h]);
+                smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + h]);
+            }
+            __syncthreads();
+        }
+        const float gmin = smin[0], gmax = smax[0];
+        __syncthreads();
+        float span = __fadd_rn(gmax, -gmin);
+        if (!(span >= 1e-12f)) span = 1e-12f;
+        const float scale = __fdiv_rn(255.0f, span);
+        if (threadIdx.x == 0) {
+            bias_invq[q * 2 + 0] = __fmul_rn((float)d, gmin);
+            bias_invq[q * 2 + 1] = __fdiv_rn(1.0f, scale);
+        }
+        for (uint32_t t = threadIdx.x; t < d * 16; t += blockDim.x) {
+            const float v = __fmul_rn(x[t >> 4], (float)(t & 15) - 7.5f);
+            float r = roundf(__fmul_rn(__fadd_rn(v, -gmin), scale));
+            r = fminf(fmaxf(r, 0.0f), 255.0f);
+            lg[t * 4 + s] = (uint8_t)r;
+        }
+    }
+}
+
+// Tiles: QG groups (4*QG query slots) x a 64-document block range of one
+// index.  counter == nullptr: workgroup b runs tile b once; otherwise
+// workgroups pull tiles until the counter passes ntiles.  Each slot's top-kcap
+// keys (score_key, index-local id) go to partial[(tile*4*QG + slot)*kcap ...].
+template <int QG>
+__global__ __launch_bounds__(256) void k_fs4_scan(const Fs4IndexDesc* __restrict__ indexes,
+                                                  const Fs4Tile* __restrict__ tiles, uint32_t ntiles,
+                                                  uint32_t* __restrict__ counter, const Fs4Group* __restrict__ groups,
+                                                  const uint8_t* __restrict__ lut, const float* __restrict__ bias_invq,
+                                                  uint32_t d, float inv_norm, uint32_t kcap,
+                                                  uint64_t* __restrict__ partial) {
+    constexpr int QT = 4 * QG;
+    extern __shared__ uint64_t smem[];
+    Fs4TopK* tk = reinterpret_cast<Fs4TopK*>(smem);
+    uint32_t* slut = reinterpret_cast<uint32_t*>(tk + QT);
+    __shared__ uint32_t s_tile;
+    __shared__ float s_bias[QT], s_invq[QT];
+    __shared__ uint32_t s_ok[QT];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t wpd = d / 8;
+    for (uint32_t iter = 0;; iter++) {
+        if (counter) {
+            if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+            __syncthreads();
+        } else if (threadIdx.x == 0) {
+            s_tile = iter == 0 ? blockIdx.x : ntiles;
+        }
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= ntiles) break;
+        const Fs4Tile tile = tiles[t];
+        // Stage the tile's LUTs (missing groups read as zeros) and slot constants.
+        const uint4* src = reinterpret_cast<const uint4*>(lut + (uint64_t)tile.g0 * d * 64);
+        uint4* dst = reinterpret_cast<uint4*>(slut);
+        const uint32_t have = tile.ng * d * 4, total = QG * d * 4;
+        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) dst[i] = i < have ? src[i] : make_uint4(0, 0, 0, 0);
+        if (threadIdx.x < QT) {
+            const uint32_t g = threadIdx.x >> 2, s = threadIdx.x & 3;
+            const bool ok = g < tile.ng && s < groups[tile.g0 + g].count;
+            const uint64_t q = ok ? (uint64_t)groups[tile.g0 + g].q0 + s : 0;
+            s_ok[threadIdx.x] = ok;
+            s_bias[threadIdx.x] = ok ? bias_invq[q * 2 + 0] : 0.0f;
+            s_invq[threadIdx.x] = ok ? bias_invq[q * 2 + 1] : 0.0f;
+        }
+        for (int s = 0; s < QT; s++) tk_init(tk[s], kcap);  // synchronises
+        const Fs4IndexDesc ix = indexes[tile.index];
+        for (uint32_t b = tile.blk0; b < tile.blk1; b += 4) {
+            const uint32_t blk = b + wave;
+            const uint64_t doc = (uint64_t)blk * 64 + lane;
+            const bool valid = blk < tile.blk1 && doc < ix.n;
+            uint32_t acc[QT];
+#pragma unroll
+            for (int s = 0; s < QT; s++) acc[s] = 0;
+            if (blk < tile.blk1) {
+                const uint32_t* base = ix.packed + (uint64_t)blk * wpd * 64 + lane;
+                for (uint32_t w = 0; w < wpd; w++) {
+                    const uint32_t v = base[(uint64_t)w * 64];
+#pragma unroll
+                    for (int c = 0; c < 8; c++) {
+                        // coordinate 8w + c: byte c/2, high nibble first
+                        const uint32_t nib = (v >> ((c >> 1) * 8 + ((c & 1) ? 0 : 4))) & 15u;
+                        const uint32_t* row = slut + (8 * w + c) * 16 + nib;
+#pragma unroll
+                        for (int g = 0; g < QG; g++) {
+                            const uint32_t e = row[g * d * 16];
+                            acc[4 * g + 0] = __builtin_amdgcn_udot4(e, 0x00000001u, acc[4 * g + 0], false);
+                            acc[4 * g + 1] = __builtin_amdgcn_udot4(e, 0x00000100u, acc[4 * g + 1], false);
+                            acc[4 * g + 2] = __builtin_amdgcn_udot4(e, 0x00010000u, acc[4 * g + 2], false);
+                            acc[4 * g + 3] = __builtin_amdgcn_udot4(e, 0x01000000u, acc[4 * g + 3], false);
+                        }
+                    }
+                }
+            }
+            for (int s = 0; s < QT; s++) tk_reserve(tk[s], kcap, blockDim.x);
+#pragma unroll
+            for (int s = 0; s < QT; s++) {
+                if (valid && s_ok[s]) {
+                    const float raw = __fadd_rn(s_bias[s], __fmul_rn((float)acc[s], s_invq[s]));
+                    tk_offer(tk[s], kcap, score_key(__fadd_rn(__fmul_rn(raw, inv_norm), 0.0f), (uint32_t)doc));
+                }
+            }
+        }
+        __syncthreads();
+        for (int s = 0; s < QT; s++) {
+            tk_flush(tk[s], kcap);
+            uint64_t* out = partial + ((uint64_t)t * QT + s) * kcap;
+            for (uint32_t j = threadIdx.x; j < kcap; j += blockDim.x) out[j] = tk[s].keys[j];
+        }
+        __syncthreads();
+    }
+}
+
+// Per query: merge its segments' top-kcap lists into the final top-k row
+// (row stride kout; entries past k padded with -inf / -1).
+__global__ __launch_bounds__(256) void k_fs4_merge(const uint64_t* __restrict__ partial, const Fs4Merge* __restrict__ m,
+                                                   uint32_t qt, uint32_t kcap, uint32_t kout, float* __restrict__ out_s,
+                                                   int64_t* __restrict__ out_i) {
+    __shared__ TopKShared tk;
+    const uint64_t q = blockIdx.x;
+    const Fs4Merge mq = m[q];
+    const uint32_t k = mq.k;
+    if (k == 0 || mq.nseg == 0) {
+        for (uint32_t j = threadIdx.x; j < kout; j += blockDim.x) {
+            out_s[q * kout + j] = -INFINITY;
+            out_i[q * kout + j] = -1;
+        }
+        return;
+    }
+    tk_init(tk, k);
+    const uint64_t total = (uint64_t)mq.nseg * kcap;
+    for (uint64_t r0 = 0; r0 < total; r0 += blockDim.x) {
+        tk_reserve(tk, k, blockDim.x);
+        const uint64_t i = r0 + threadIdx.x;
+        if (i < total) {
+            const uint64_t seg = i / kcap, j = i - seg * kcap;
+            const uint64_t key = partial[((mq.t0 + seg) * qt + mq.slot) * kcap + j];
+            if (key != kKeyMax) tk_offer(tk, k, key);
+        }
+    }
+    __syncthreads();
+    tk_flush(tk, k);
+    for (uint32_t j = threadIdx.x; j < kout; j += blockDim.x) {
+        const uint64_t key = j < k ? tk.keys[j] : kKeyMax;
+        out_s[q * kout + j] = key == kKeyMax ? -INFINITY : key_score(key);
+        out_i[q * kout + j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
+    }
+}
+
+// RankQuant b=4 rows -> interleaved dword blocks (tail lanes of the last
+// block are left as the caller cleared them).
+__global__ void k_fs4_pack(const uint8_t* __restrict__ codes, uint64_t n, uint32_t d, uint32_t* __restrict__ out) {
+    const uint32_t wpd = d / 8;
+    const uint64_t total = n * wpd;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t doc = t / wpd;
+        const uint32_t w = (uint32_t)(t - doc * wpd);
+        out[((doc >> 6) * wpd + w) * 64 + (doc & 63)] = reinterpret_cast<const uint32_t*>(codes + doc * (d / 2))[w];
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t count, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+void check() { OVX_HIP(hipGetLastError()); }
+
+}  // namespace
+
+size_t fs4_scan_lds_bytes(uint32_t qg, uint32_t d) {
+    return (size_t)4 * qg * sizeof(Fs4TopK) + (size_t)qg * d * 64;
+}
+
+void fs4_configure_lds() {
+    static bool done = false;
+    if (done) return;
+    // Larger dynamic LDS than the default 64 KiB (gfx950 has 160 KiB per CU).
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fs4_scan<1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fs4_scan<2>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipGetLastError();
+    done = true;
+}
+
+void launch_fs4_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t* packed, hipStream_t s) {
+    if (n == 0) return;
+    const uint64_t total = n * (d / 8);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(65536, (total + 255) / 256);
+    ktime_begin(kFamEncode, s);
+    hipLaunchKernelGGL(k_fs4_pack, dim3(grid), dim3(256), 0, s, codes, n, d, packed);
+    check();
+    ktime_end(kFamEncode, s);
+}
+
+void launch_fs4_lut(const float* qhat, uint32_t d, const Fs4Group* groups, uint32_t ngroups, uint8_t* lut,
+                    float* bias_invq, hipStream_t s) {
+    if (ngroups == 0) return;
+    ktime_begin(kFamFastScan, s);
+    hipLaunchKernelGGL(k_fs4_lut, dim3(ngroups), dim3(256), 0, s, qhat, d, groups, lut, bias_invq);
+    check();
+    ktime_end(kFamFastScan, s);
+}
+
+void launch_fs4_scan(uint32_t qg, uint32_t grid, const Fs4IndexDesc* indexes, const Fs4Tile* tiles, uint32_t ntiles,
+                     uint32_t* counter, const Fs4Group* groups, const uint8_t* lut, const float* bias_invq, uint32_t d,
+                     float inv_norm, uint32_t kcap, uint64_t* partial, hipStream_t s) {
+    if (ntiles == 0) return;
+    fs4_configure_lds();
+    const size_t lds = fs4_scan_lds_bytes(qg, d);
+    ktime_begin(kFamFastScan, s);
+    if (counter) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counter, 1u, 0u);
+        check();
+    }
+    if (qg == 2)
+        hipLaunchKernelGGL(k_fs4_scan<2>, dim3(grid), dim3(256), lds, s, indexes, tiles, ntiles, counter, groups, lut,
+                           bias_invq, d, inv_norm, kcap, partial);
+    else
+        hipLaunchKernelGGL(k_fs4_scan<1>, dim3(grid), dim3(256), lds, s, indexes, tiles, ntiles, counter, groups, lut,
+                           bias_invq, d, inv_norm, kcap, partial);
+    check();
+    ktime_end(kFamFastScan, s);
+}
+
+void launch_fs4_merge(const uint64_t* partial, const Fs4Merge* merges, uint32_t nq, uint32_t qt, uint32_t kcap,
+                      uint32_t kout, float* out_s, int64_t* out_i, hipStream_t s) {
+    if (nq == 0) return;
+    ktime_begin(kFamFastScan, s);
+    hipLaunchKernelGGL(k_fs4_merge, dim3(nq), dim3(256), 0, s, partial, merges, qt, kcap, kout, out_s, out_i);
+    check();
+    ktime_end(kFamFastScan, s);
+}
+
+}  // namespace ovx

@@ -568,12 +568,20 @@ class Bitmap:
 
 
 class RankQuantFastscan:
-    """FastScan b=2 block-32 nibble-LUT index in HBM (src/fastscan.rs:532-705)."""
+    """FastScan nibble-LUT index in HBM: b=2 is the reference's
+    RankQuantFastscan (src/fastscan.rs:532-705); ``bits=4`` is the b=4
+    extension (one 4-bit rank code per coordinate, same LUT recipe)."""
 
-    def __init__(self, dim: int):
+    def __init__(self, dim: int, bits: int = 2):
         h = C.c_void_p()
-        N.check(N.lib().ordvec_fastscan_new(int(dim), C.byref(h)))
+        N.check(N.lib().ordvec_fastscan_new_bits(int(dim), int(bits), C.byref(h)))
         self._h = h
+
+    @property
+    def bits(self) -> int:
+        b = C.c_uint32()
+        N.check(N.lib().ordvec_fastscan_bits(self._h, C.byref(b)))
+        return b.value
 
     @classmethod
     def _wrap(cls, h):
@@ -625,6 +633,87 @@ class RankQuantFastscan:
         _host_stream()
         N.check(N.lib().ordvec_fastscan_load(str(path).encode(), C.byref(h)))
         return cls._wrap(h)
+
+
+class FastscanBatch:
+    """Many b=4 FastScan indexes searched as one batch: query group i (rows
+    ``queries_per_index[i]``, concatenated in index order) against index i with
+    its own k, in one persistent launch sequence captured as a HIP graph.
+
+    ``run`` returns (scores, ids) of shape (total_queries, kout), row q valid
+    up to its index's min(k, n) (padding -inf / -1); ``split`` cuts them into
+    per-index arrays.  ``autotune`` times tile shapes on real queries and keeps
+    the fastest."""
+
+    def __init__(self, indexes, queries_per_index, k_per_index):
+        self._keep = list(indexes)
+        n = len(self._keep)
+        arr = (C.c_void_p * n)(*[ix._h.value for ix in self._keep])
+        nq = np.ascontiguousarray(queries_per_index, dtype=np.uint32)
+        ks = np.ascontiguousarray(k_per_index, dtype=np.uint32)
+        if nq.shape != (n,) or ks.shape != (n,):
+            raise ValueError("queries_per_index and k_per_index need one entry per index")
+        h = C.c_void_p()
+        N.check(N.lib().ordvec_fastscan_batch_new(C.cast(arr, C.c_void_p), n, nq.ctypes.data_as(C.c_void_p),
+                                                  ks.ctypes.data_as(C.c_void_p), C.byref(h)))
+        self._h = h
+        self.queries_per_index = nq
+        self.k_per_index = np.array([min(int(k), len(ix)) for k, ix in zip(ks, self._keep)], dtype=np.int64)
+        self.dim = self._keep[0].dim
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and N._lib is not None:
+            N.lib().ordvec_fastscan_batch_free(h)
+            self._h = None
+
+    def config(self) -> dict:
+        tq, nt, ntu = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        ko, qpt, bpt, grid = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        N.check(N.lib().ordvec_fastscan_batch_config(self._h, C.byref(tq), C.byref(ko), C.byref(qpt), C.byref(bpt),
+                                                     C.byref(grid), C.byref(nt), C.byref(ntu)))
+        return {"total_queries": tq.value, "kout": ko.value, "queries_per_tile": qpt.value,
+                "blocks64_per_tile": bpt.value, "grid": grid.value, "tiles": nt.value, "tuned_configs": ntu.value}
+
+    def tuning(self) -> list:
+        rows = []
+        for i in range(self.config()["tuned_configs"]):
+            qpt, bpt, grid, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
+            N.check(N.lib().ordvec_fastscan_batch_tuning(self._h, i, C.byref(qpt), C.byref(bpt), C.byref(grid),
+                                                         C.byref(ms)))
+            rows.append({"queries_per_tile": qpt.value, "blocks64_per_tile": bpt.value, "grid": grid.value,
+                         "ms": ms.value})
+        return rows
+
+    def _queries(self, queries):
+        q, nq, flag = _float_rows(queries, self.dim, "queries")
+        if nq != int(self.queries_per_index.sum()):
+            raise ValueError(f"expected {int(self.queries_per_index.sum())} query rows, got {nq}")
+        return q, nq, flag
+
+    def autotune(self, queries, reps: int = 2) -> list:
+        q, _, flag = self._queries(queries)
+        _bind_stream() if flag else _host_stream()
+        N.check(N.lib().ordvec_fastscan_batch_autotune(self._h, _ptr(q), int(reps), flag))
+        return self.tuning()
+
+    def run(self, queries, use_graph: bool = True):
+        q, nq, flag = self._queries(queries)
+        kout = self.config()["kout"]
+        s = _out((nq, kout), np.float32, flag)
+        i = _out((nq, kout), np.int64, flag)
+        _bind_stream() if flag else _host_stream()
+        f = flag | (0 if use_graph else N.FLAG_NO_GRAPH)
+        N.check(N.lib().ordvec_fastscan_batch_run(self._h, _ptr(q), _ptr(s), _ptr(i), f))
+        return s, i
+
+    def split(self, scores, ids):
+        """Per-index (scores, ids) arrays of shape (queries_i, min(k_i, n_i))."""
+        out, r = [], 0
+        for nq, k in zip(self.queries_per_index, self.k_per_index):
+            out.append((scores[r:r + nq, :k], ids[r:r + nq, :k]))
+            r += int(nq)
+        return out
 
 
 def two_stage_search(sign: SignBitmap, rq: RankQuant, queries, m: int, k: int,

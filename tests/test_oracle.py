@@ -354,3 +354,53 @@ def test_bitmap_search_and_top_m_orders(oracle):
     s, i = oracle.bitmap_search(same, oracle.bitmap_query(docs[3], n_top), 4)
     assert list(i[0]) == [0, 1, 2, 3] and len(set(s[0])) == 1
     assert list(oracle.bitmap_top_m(same, oracle.bitmap_query(docs[3], n_top), 5)[0]) == [0, 1, 2, 3, 4]
+
+
+# ---- FastScan b=4 (extension of the b=2 recipe) -----------------------------------
+
+def test_fs4_lut_structure(oracle):
+    rng = np.random.default_rng(51)
+    q = oracle.l2_normalise(rng.standard_normal(64).astype(np.float32))
+    lut, bias, invq = oracle.fs4_query(q)
+    t = lut.reshape(64, 16).astype(int)
+    assert t.min() == 0 and t.max() == 255
+    for j in range(64):
+        d = np.diff(t[j])
+        assert np.all(d >= 0) if q[j] > 0 else np.all(d <= 0)
+    gmin = np.float32(np.min(q[:, None] * (np.arange(16, dtype=np.float32) - np.float32(7.5))))
+    assert bias == np.float32(64) * gmin
+    # zero query: every entry equal, every score 0, ids ascending
+    lut0, b0, _ = oracle.fs4_query(np.zeros(64, np.float32))
+    assert np.all(lut0 == 0) and b0 == 0
+    codes = oracle.rq_encode(S.iid_unit(20, 64, seed=2), 4)
+    s, i = oracle.fs4_search(codes, 64, np.zeros((1, 64), np.float32), 5)
+    assert list(i[0]) == [0, 1, 2, 3, 4] and np.all(s == 0)
+
+
+@pytest.mark.parametrize("dim", [64, 768])
+def test_fs4_scores_within_quantisation_bound(oracle, dim):
+    gen = S.MixtureNP(dim, seed=52)
+    docs = gen.docs(400)
+    qs, _ = gen.queries(docs, 6)
+    codes = oracle.rq_encode(docs, 4)
+    R = float(np.sqrt(dim * 255 / 12.0))
+    cen = oracle.decode_centres(codes, dim, 4)
+    for q in qs:
+        qu = oracle.l2_normalise(q)
+        _, _, invq = oracle.fs4_query(qu)
+        s, i = oracle.fs4_search(codes, dim, q[None], 400)
+        exact = cen[i[0]] @ qu.astype(np.float64) / R
+        # each of the dim entries is off by at most half a quantisation step
+        bound = (dim * 0.5 * float(invq)) / R * (1 + 1e-5) + 1e-6
+        assert np.max(np.abs(s[0] - exact)) <= bound
+
+
+def test_fs4_top10_overlaps_exact_b4(oracle):
+    gen = S.MixtureNP(256, seed=53)
+    docs = gen.docs(3000)
+    qs, _ = gen.queries(docs, 20)
+    codes = oracle.rq_encode(docs, 4)
+    fs, fi = oracle.fs4_search(codes, 256, qs, 10)
+    es, ei = oracle.search_asymmetric(codes, 256, 4, qs, 10)
+    overlap = np.mean([len(set(a) & set(b)) for a, b in zip(fi, ei)])
+    assert overlap >= 9.0, overlap
