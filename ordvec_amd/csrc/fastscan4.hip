@@ -84,11 +84,33 @@ __global__ __launch_bounds__(256) void k_fs4_lut(const float* __restrict__ qhat,
     }
 }
 
+typedef const __attribute__((address_space(1))) uint32_t gu32_t;
+
+// The 8 coordinates of one code dword (w): coordinate 8w + c sits in byte c/2,
+// high nibble first.  One 32-bit LDS read per coordinate and group fetches
+// the entries of four queries; each dot4 adds one query's byte.
+template <int QG>
+__device__ __forceinline__ void fs4_dword(uint32_t v, uint32_t w, const uint32_t* slut, uint32_t d, uint32_t* acc) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const uint32_t nib = __builtin_amdgcn_ubfe(v, (c >> 1) * 8 + ((c & 1) ? 0 : 4), 4);
+        const uint32_t* row = slut + (8 * w + c) * 16 + nib;
+#pragma unroll
+        for (int g = 0; g < QG; g++) {
+            const uint32_t e = row[g * d * 16];
+            acc[4 * g + 0] = __builtin_amdgcn_udot4(e, 0x00000001u, acc[4 * g + 0], false);
+            acc[4 * g + 1] = __builtin_amdgcn_udot4(e, 0x00000100u, acc[4 * g + 1], false);
+            acc[4 * g + 2] = __builtin_amdgcn_udot4(e, 0x00010000u, acc[4 * g + 2], false);
+            acc[4 * g + 3] = __builtin_amdgcn_udot4(e, 0x01000000u, acc[4 * g + 3], false);
+        }
+    }
+}
+
 // Tiles: QG groups (4*QG query slots) x a 64-document block range of one
 // index.  counter == nullptr: workgroup b runs tile b once; otherwise
 // workgroups pull tiles until the counter passes ntiles.  Each slot's top-kcap
 // keys (score_key, index-local id) go to partial[(tile*4*QG + slot)*kcap ...].
-template <int QG>
+template <int QG, int CH>
 __global__ __launch_bounds__(256) void k_fs4_scan(const Fs4IndexDesc* __restrict__ indexes,
                                                   const Fs4Tile* __restrict__ tiles, uint32_t ntiles,
                                                   uint32_t* __restrict__ counter, const Fs4Group* __restrict__ groups,
@@ -138,23 +160,21 @@ __global__ __launch_bounds__(256) void k_fs4_scan(const Fs4IndexDesc* __restrict
 #pragma unroll
             for (int s = 0; s < QT; s++) acc[s] = 0;
             if (blk < tile.blk1) {
-                const uint32_t* base = ix.packed + (uint64_t)blk * wpd * 64 + lane;
-                for (uint32_t w = 0; w < wpd; w++) {
-                    const uint32_t v = base[(uint64_t)w * 64];
+                // Global (not flat) loads, CH dwords in flight ahead of the chunk
+                // being scored (CH divides wpd).
+                gu32_t* base = (gu32_t*)(ix.packed + (uint64_t)blk * wpd * 64 + lane);
+                uint32_t cur[CH], nxt[CH];
 #pragma unroll
-                    for (int c = 0; c < 8; c++) {
-                        // coordinate 8w + c: byte c/2, high nibble first
-                        const uint32_t nib = (v >> ((c >> 1) * 8 + ((c & 1) ? 0 : 4))) & 15u;
-                        const uint32_t* row = slut + (8 * w + c) * 16 + nib;
+                for (int i = 0; i < CH; i++) cur[i] = base[(uint64_t)i * 64];
+                for (uint32_t w0 = 0; w0 < wpd; w0 += CH) {
+                    // next chunk (the last one again at the end: loads stay unconditional)
+                    const uint32_t w1 = min(w0 + CH, wpd - CH);
 #pragma unroll
-                        for (int g = 0; g < QG; g++) {
-                            const uint32_t e = row[g * d * 16];
-                            acc[4 * g + 0] = __builtin_amdgcn_udot4(e, 0x00000001u, acc[4 * g + 0], false);
-                            acc[4 * g + 1] = __builtin_amdgcn_udot4(e, 0x00000100u, acc[4 * g + 1], false);
-                            acc[4 * g + 2] = __builtin_amdgcn_udot4(e, 0x00010000u, acc[4 * g + 2], false);
-                            acc[4 * g + 3] = __builtin_amdgcn_udot4(e, 0x01000000u, acc[4 * g + 3], false);
-                        }
-                    }
+                    for (int i = 0; i < CH; i++) nxt[i] = base[(uint64_t)(w1 + i) * 64];
+#pragma unroll
+                    for (int i = 0; i < CH; i++) fs4_dword<QG>(cur[i], w0 + i, slut, d, acc);
+#pragma unroll
+                    for (int i = 0; i < CH; i++) cur[i] = nxt[i];
                 }
             }
             for (int s = 0; s < QT; s++) tk_reserve(tk[s], kcap, blockDim.x);
@@ -240,10 +260,18 @@ void fs4_configure_lds() {
     static bool done = false;
     if (done) return;
     // Larger dynamic LDS than the default 64 KiB (gfx950 has 160 KiB per CU).
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fs4_scan<1>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fs4_scan<2>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+#define OVX_FS4_LDS(QG, CH)                                                                       \
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fs4_scan<QG, CH>),                \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)
+    OVX_FS4_LDS(1, 8);
+    OVX_FS4_LDS(1, 4);
+    OVX_FS4_LDS(1, 2);
+    OVX_FS4_LDS(1, 1);
+    OVX_FS4_LDS(2, 8);
+    OVX_FS4_LDS(2, 4);
+    OVX_FS4_LDS(2, 2);
+    OVX_FS4_LDS(2, 1);
+#undef OVX_FS4_LDS
     (void)hipGetLastError();
     done = true;
 }
@@ -278,12 +306,22 @@ void launch_fs4_scan(uint32_t qg, uint32_t grid, const Fs4IndexDesc* indexes, co
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counter, 1u, 0u);
         check();
     }
-    if (qg == 2)
-        hipLaunchKernelGGL(k_fs4_scan<2>, dim3(grid), dim3(256), lds, s, indexes, tiles, ntiles, counter, groups, lut,
-                           bias_invq, d, inv_norm, kcap, partial);
-    else
-        hipLaunchKernelGGL(k_fs4_scan<1>, dim3(grid), dim3(256), lds, s, indexes, tiles, ntiles, counter, groups, lut,
-                           bias_invq, d, inv_norm, kcap, partial);
+#define OVX_FS4_SCAN(QG, CH)                                                                                    \
+    hipLaunchKernelGGL((k_fs4_scan<QG, CH>), dim3(grid), dim3(256), lds, s, indexes, tiles, ntiles, counter, groups, \
+                       lut, bias_invq, d, inv_norm, kcap, partial)
+    const uint32_t wpd = d / 8;
+    if (qg == 2) {
+        if (wpd % 8 == 0) OVX_FS4_SCAN(2, 8);
+        else if (wpd % 4 == 0) OVX_FS4_SCAN(2, 4);
+        else if (wpd % 2 == 0) OVX_FS4_SCAN(2, 2);
+        else OVX_FS4_SCAN(2, 1);
+    } else {
+        if (wpd % 8 == 0) OVX_FS4_SCAN(1, 8);
+        else if (wpd % 4 == 0) OVX_FS4_SCAN(1, 4);
+        else if (wpd % 2 == 0) OVX_FS4_SCAN(1, 2);
+        else OVX_FS4_SCAN(1, 1);
+    }
+#undef OVX_FS4_SCAN
     check();
     ktime_end(kFamFastScan, s);
 }

@@ -32,9 +32,9 @@ def test_b4_validation(engine):
     with pytest.raises(ValueError):
         engine.RankQuantFastscan(64, bits=3)
     ix = engine.RankQuantFastscan(64, bits=4)
-    ix.add(S.iid_unit(100, 64, seed=1))
+    ix.add(S.iid_unit(400, 64, seed=1))
     with pytest.raises(NotImplementedError):
-        ix.search(S.iid_unit(2, 64, seed=2), 300)
+        ix.search(S.iid_unit(2, 64, seed=2), 300)  # k > 256
     with pytest.raises(NotImplementedError):
         ix.write("/tmp/unused.ovfs")
 
