@@ -737,6 +737,172 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
     }
 }
 
+// Full scan on the matrix cores.  Documents expand to {0,1} bytes (d_k) and
+// queries to s_k = +1 / -1 for bit 1 / 0 (sign bitmaps) or to {0,1} (top-bucket
+// bitmaps); a 32 x 32 tile of (document, query) dot products is then
+// v_mfma_i32_32x32x32_i8 over 32-coordinate chunks, exact in i32:
+//   sign:   sum_k d_k s_k = 2|d & q| - |d|,  so  popcount(d ^ q) = |q| - dot
+//   bitmap: sum_k d_k q_k = |d & q|,         so  n_top - overlap = n_top - dot
+// A (documents): lane l holds row l%32 and coordinates 16*(l/32) .. +15 of the
+// chunk as bytes 0..15; B (queries) uses the same coordinate assignment, so
+// the k order inside the instruction never matters.  D: lane l holds column
+// (query) l%32 and rows 8*(v/4) + 4*(l/32) + v%4, v = 0..15.
+// A workgroup (8 waves) holds NQT query tiles of 32 whose bytes are expanded
+// once into LDS (one 16-byte fragment per lane per chunk); each wave scores
+// 64-document blocks (two row groups), expanding each document's bits once
+// per chunk and reusing them for all NQT tiles.  Code words stream through a
+// ring of RING words loaded ahead.  Survivors (distance <= T[q], i.e.
+// dot >= base - T[q]) are appended as in k_s1_scan.
+typedef int ovx_v4i __attribute__((ext_vector_type(4)));
+typedef int ovx_v16i __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint32_t spread4(uint32_t nib) { return (nib * 0x00204081u) & 0x01010101u; }
+
+// 16 bits -> 16 bytes in {0, 1}, bit p -> byte p.
+__device__ __forceinline__ ovx_v4i expand16(uint32_t b) {
+    ovx_v4i r;
+    r[0] = (int)spread4(b & 15u);
+    r[1] = (int)spread4((b >> 4) & 15u);
+    r[2] = (int)spread4((b >> 8) & 15u);
+    r[3] = (int)spread4((b >> 12) & 15u);
+    return r;
+}
+
+// 16 bits -> 16 bytes: +1 for a set bit, -1 (0xff) otherwise.  With y the
+// {0,1} bytes, 0xff - 0xfe*y per byte has no cross-byte carry or borrow.
+__device__ __forceinline__ ovx_v4i expand16_pm(uint32_t b) {
+    const ovx_v4i y = expand16(b);
+    ovx_v4i o;
+#pragma unroll
+    for (int i = 0; i < 4; i++) o[i] = (int)~((uint32_t)y[i] * 0xfeu);
+    return o;
+}
+
+constexpr uint32_t kMfmaThreads = 512;
+
+template <int NQT, int OP, int QPV>
+__global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t* __restrict__ words, uint32_t nblk,
+                                                                  uint64_t n, const uint64_t* __restrict__ qbits,
+                                                                  uint32_t nq, uint32_t blocks_per_seg,
+                                                                  const uint32_t* __restrict__ thr, uint32_t cap,
+                                                                  uint64_t* __restrict__ keys,
+                                                                  uint32_t* __restrict__ cnt, uint32_t ntop) {
+    constexpr uint32_t QW = 32 * NQT, NCHUNK = 2 * QPV, NWAVE = kMfmaThreads / 64;
+    constexpr int RING = QPV < 4 ? QPV : 4;
+    extern __shared__ ovx_v4i sB[];  // [NQT][NCHUNK][64]
+    __shared__ int s_base[QW];
+    __shared__ uint32_t s_thr[QW];
+    const uint32_t q0 = blockIdx.x * QW;
+    for (uint32_t e = threadIdx.x; e < NQT * NCHUNK * 64; e += blockDim.x) {
+        const uint32_t ln = e & 63, c = (e >> 6) % NCHUNK, t = (e >> 6) / NCHUNK;
+        const uint32_t q = q0 + t * 32 + (ln & 31);
+        if (q < nq) {
+            const uint32_t bits =
+                (uint32_t)(qbits[(uint64_t)q * QPV + c / 2] >> ((c & 1) * 32 + 16 * (ln >> 5))) & 0xffffu;
+            sB[e] = OP ? expand16(bits) : expand16_pm(bits);
+        } else {
+            sB[e] = ovx_v4i{0, 0, 0, 0};
+        }
+    }
+    for (uint32_t j = threadIdx.x; j < QW; j += blockDim.x) {
+        const uint32_t q = q0 + j;
+        uint32_t pc = 0;
+        if (q < nq && !OP)
+            for (uint32_t w = 0; w < QPV; w++) pc += (uint32_t)__popcll(qbits[(uint64_t)q * QPV + w]);
+        s_base[j] = (int)(OP ? ntop : pc);
+        s_thr[j] = q < nq ? thr[q] : 0u;
+    }
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t col = lane & 31, g = lane >> 5;
+    // pass <=> dot >= base - T; columns past nq never pass.
+    int need[NQT], basev[NQT];
+#pragma unroll
+    for (int t = 0; t < NQT; t++) {
+        const uint32_t j = t * 32 + col;
+        basev[t] = s_base[j];
+        need[t] = q0 + j < nq ? basev[t] - (int)s_thr[j] : 0x7fffffff;
+    }
+    const uint32_t b_begin = blockIdx.y * blocks_per_seg;
+    const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
+    for (uint32_t b = b_begin + wave; b < b_end; b += NWAVE) {
+        ovx_v16i acc[2][NQT];
+#pragma unroll
+        for (int G = 0; G < 2; G++)
+#pragma unroll
+            for (int t = 0; t < NQT; t++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) acc[G][t][v] = 0;
+        const uint64_t* base = words + (uint64_t)b * QPV * 64 + col;  // documents b*64 + col (+32)
+        uint64_t r0[RING], r1[RING];
+#pragma unroll
+        for (int i = 0; i < RING; i++) {
+            r0[i] = base[i * 64];
+            r1[i] = base[i * 64 + 32];
+        }
+        for (int w0 = 0; w0 < QPV; w0 += RING) {
+#pragma unroll
+            for (int i = 0; i < RING; i++) {
+                const uint64_t d0 = r0[i], d1 = r1[i];
+                const int wn = w0 + i + RING;
+                if (wn < QPV) {
+                    r0[i] = base[wn * 64];
+                    r1[i] = base[wn * 64 + 32];
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int c = 2 * (w0 + i) + h;
+                    const uint32_t sh = h * 32 + 16 * g;
+                    const ovx_v4i a0 = expand16((uint32_t)(d0 >> sh) & 0xffffu);
+                    const ovx_v4i a1 = expand16((uint32_t)(d1 >> sh) & 0xffffu);
+#pragma unroll
+                    for (int t = 0; t < NQT; t++) {
+                        const ovx_v4i bf = sB[(t * NCHUNK + c) * 64 + lane];
+                        acc[0][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bf, acc[0][t], 0, 0, 0);
+                        acc[1][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bf, acc[1][t], 0, 0, 0);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        bool any = false;
+#pragma unroll
+        for (int G = 0; G < 2; G++)
+#pragma unroll
+            for (int t = 0; t < NQT; t++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) any |= acc[G][t][v] >= need[t];
+        if (__any(any)) {
+#pragma unroll
+            for (int t = 0; t < NQT; t++) {
+                const uint32_t q = q0 + t * 32 + col;
+                uint32_t npass = 0;
+#pragma unroll
+                for (int G = 0; G < 2; G++)
+#pragma unroll
+                    for (int v = 0; v < 16; v++) {
+                        const uint64_t doc = (uint64_t)b * 64 + G * 32 + 8 * (v / 4) + 4 * g + (v % 4);
+                        npass += (acc[G][t][v] >= need[t] && doc < n) ? 1u : 0u;
+                    }
+                if (npass) {
+                    uint32_t pos = atomicAdd(&cnt[q], npass);
+#pragma unroll
+                    for (int G = 0; G < 2; G++)
+#pragma unroll
+                        for (int v = 0; v < 16; v++) {
+                            const uint64_t doc = (uint64_t)b * 64 + G * 32 + 8 * (v / 4) + 4 * g + (v % 4);
+                            if (acc[G][t][v] >= need[t] && doc < n) {
+                                const uint32_t h = (uint32_t)(basev[t] - acc[G][t][v]);
+                                if (pos < cap) keys[(uint64_t)q * cap + pos] = ((uint64_t)h << 32) | doc;
+                                pos++;
+                            }
+                        }
+                }
+            }
+        }
+    }
+}
+
 // Sort each query's survivors by (hamming, id) and keep the first m; flag the
 // query (qstat bit 1) when the buffer cannot prove exactness.  A histogram of
 // min(hamming, kSelBins-1) first narrows the sort to the keys at or below the
@@ -1582,10 +1748,74 @@ static void launch_sample(const uint64_t* words, uint64_t n, uint32_t qpv, const
     else launch_sample_wch<1, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
 }
 
+// The matrix-core scan pays off once a workgroup has a full 32-query tile;
+// its query fragments live in LDS (nqt x d bytes).
+static bool g_mfma_scan = true;
+template <int NQT, int OP>
+static void set_mfma_lds() {
+#define OVX_SET(QPV)                                                                          \
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_s1_scan_mfma<NQT, OP, QPV>), \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)
+    OVX_SET(16);
+    OVX_SET(12);
+    OVX_SET(8);
+    OVX_SET(4);
+    OVX_SET(2);
+    OVX_SET(1);
+#undef OVX_SET
+}
+// The matrix-core scan needs at least one full 32-query tile; its query
+// fragments live in LDS (32 * d bytes per 32-query tile, <= 128 KiB).
+static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
+    static bool configured = false;
+    if (!configured) {
+        set_mfma_lds<4, 0>();
+        set_mfma_lds<4, 1>();
+        set_mfma_lds<2, 0>();
+        set_mfma_lds<2, 1>();
+        (void)hipGetLastError();
+        configured = true;
+    }
+    return g_mfma_scan && nq >= 32 && (qpv == 16 || qpv == 12 || qpv == 8 || qpv == 4 || qpv == 2 || qpv == 1);
+}
+
 template <int OP>
 static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
                         const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
     const uint32_t nblk = (uint32_t)((n + 63) / 64);
+    if (use_mfma_scan(nq, qpv)) {
+        // Matrix-core scan: 32, 64 or 128 queries per workgroup of 8 waves.
+        const uint32_t nqt = nq >= 128 && qpv <= 16 ? 4 : (nq >= 64 ? 2 : 1);
+        const uint32_t tiles_m = (uint32_t)((nq + 32 * nqt - 1) / (32 * nqt));
+        const uint64_t want = std::max<uint64_t>(1, (1024 + tiles_m - 1) / tiles_m);
+        const uint64_t nseg = std::min<uint64_t>(std::max<uint64_t>(1, (nblk + 7) / 8), want);
+        const uint32_t bps = (uint32_t)((nblk + nseg - 1) / nseg);
+        const uint32_t segs = (nblk + bps - 1) / bps;
+        const size_t lds = (size_t)nqt * qpv * 2 * 64 * 16;
+#define OVX_MFMA(NQT, QPV)                                                                                         \
+    hipLaunchKernelGGL((k_s1_scan_mfma<NQT, OP, QPV>), dim3(tiles_m, segs), dim3(kMfmaThreads), lds, s, words, nblk, \
+                       n, qbits, (uint32_t)nq, bps, w.thr, p.cap, w.keys, w.cnt, ntop)
+#define OVX_MFMA_Q(NQT)                        \
+    switch (qpv) {                             \
+        case 16: OVX_MFMA(NQT, 16); break;     \
+        case 12: OVX_MFMA(NQT, 12); break;     \
+        case 8: OVX_MFMA(NQT, 8); break;       \
+        case 4: OVX_MFMA(NQT, 4); break;       \
+        case 2: OVX_MFMA(NQT, 2); break;       \
+        default: OVX_MFMA(NQT, 1); break;      \
+    }
+        if (nqt == 4) {
+            OVX_MFMA_Q(4);
+        } else if (nqt == 2) {
+            OVX_MFMA_Q(2);
+        } else {
+            OVX_MFMA_Q(1);
+        }
+#undef OVX_MFMA_Q
+#undef OVX_MFMA
+        check_launch();
+        return;
+    }
     const uint32_t tiles = (uint32_t)((nq + p.scan_qt - 1) / p.scan_qt);
     dim3 grid(tiles, p.nseg);
 #define OVX_SCAN(W)                                                                                               \

@@ -111,7 +111,11 @@ def test_empty_and_zero_k_shapes(engine):
 
 
 @pytest.mark.parametrize("dim,n,nq,m", [(1024, 50_000, 16, 1000), (768, 20_011, 40, 300), (64, 5000, 3, 64),
-                                        (128, 777, 9, 1000), (1024, 4096, 1, 4096), (192, 3001, 20, 1)])
+                                        (128, 777, 9, 1000), (1024, 4096, 1, 4096), (192, 3001, 20, 1),
+                                        # matrix-core scan shapes (>= 32 queries): full and partial query
+                                        # tiles, a ragged last document block, tiny corpora
+                                        (1024, 30_001, 100, 1000), (64, 5000, 70, 64), (192, 7001, 33, 300),
+                                        (1024, 65, 64, 10), (512, 12_345, 32, 500)])
 def test_sign_top_m_exact(engine, oracle, dim, n, nq, m):
     gen = S.MixtureNP(dim, seed=11)
     docs = gen.docs(n)
