@@ -780,6 +780,38 @@ __device__ __forceinline__ ovx_v4i expand16_pm(uint32_t b) {
 
 constexpr uint32_t kMfmaThreads = 512;
 
+// One code word (64 coordinates = two 32-coordinate chunks) of documents col
+// and col + 32 against every query tile.  Document bits expand to {0,1} bytes
+// through a 256-entry LDS table (byte -> 8 bytes), two reads per 16 bits.
+// FIRST: the accumulators start from zero (inline constant), no clearing pass.
+template <int NQT, int NCHUNK, bool FIRST>
+__device__ __forceinline__ void mfma_word(uint64_t d0, uint64_t d1, int w, uint32_t g, uint32_t lane,
+                                          const uint2* __restrict__ s_exp, const ovx_v4i* __restrict__ sB,
+                                          ovx_v16i (&acc)[2][NQT]) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int c = 2 * w + h;
+        const uint32_t sh = h * 32 + 16 * g;
+        const uint32_t x0 = (uint32_t)(d0 >> sh), x1 = (uint32_t)(d1 >> sh);
+        const uint2 l0 = s_exp[x0 & 255u], u0 = s_exp[(x0 >> 8) & 255u];
+        const uint2 l1 = s_exp[x1 & 255u], u1 = s_exp[(x1 >> 8) & 255u];
+        const ovx_v4i a0 = {(int)l0.x, (int)l0.y, (int)u0.x, (int)u0.y};
+        const ovx_v4i a1 = {(int)l1.x, (int)l1.y, (int)u1.x, (int)u1.y};
+#pragma unroll
+        for (int t = 0; t < NQT; t++) {
+            const ovx_v4i bf = sB[(t * NCHUNK + c) * 64 + lane];
+            if (FIRST && h == 0) {
+                const ovx_v16i z = {};
+                acc[0][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bf, z, 0, 0, 0);
+                acc[1][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bf, z, 0, 0, 0);
+            } else {
+                acc[0][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bf, acc[0][t], 0, 0, 0);
+                acc[1][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bf, acc[1][t], 0, 0, 0);
+            }
+        }
+    }
+}
+
 template <int NQT, int OP, int QPV>
 __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t* __restrict__ words, uint32_t nblk,
                                                                   uint64_t n, const uint64_t* __restrict__ qbits,
@@ -792,6 +824,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     extern __shared__ ovx_v4i sB[];  // [NQT][NCHUNK][64]
     __shared__ int s_base[QW];
     __shared__ uint32_t s_thr[QW];
+    __shared__ uint2 s_exp[256];
     const uint32_t q0 = blockIdx.x * QW;
     for (uint32_t e = threadIdx.x; e < NQT * NCHUNK * 64; e += blockDim.x) {
         const uint32_t ln = e & 63, c = (e >> 6) % NCHUNK, t = (e >> 6) / NCHUNK;
@@ -804,6 +837,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
             sB[e] = ovx_v4i{0, 0, 0, 0};
         }
     }
+    for (uint32_t v = threadIdx.x; v < 256; v += blockDim.x) s_exp[v] = make_uint2(spread4(v & 15u), spread4(v >> 4));
     for (uint32_t j = threadIdx.x; j < QW; j += blockDim.x) {
         const uint32_t q = q0 + j;
         uint32_t pc = 0;
@@ -827,12 +861,6 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
     for (uint32_t b = b_begin + wave; b < b_end; b += NWAVE) {
         ovx_v16i acc[2][NQT];
-#pragma unroll
-        for (int G = 0; G < 2; G++)
-#pragma unroll
-            for (int t = 0; t < NQT; t++)
-#pragma unroll
-                for (int v = 0; v < 16; v++) acc[G][t][v] = 0;
         const uint64_t* base = words + (uint64_t)b * QPV * 64 + col;  // documents b*64 + col (+32)
         uint64_t r0[RING], r1[RING];
 #pragma unroll
@@ -849,29 +877,21 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                     r0[i] = base[wn * 64];
                     r1[i] = base[wn * 64 + 32];
                 }
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int c = 2 * (w0 + i) + h;
-                    const uint32_t sh = h * 32 + 16 * g;
-                    const ovx_v4i a0 = expand16((uint32_t)(d0 >> sh) & 0xffffu);
-                    const ovx_v4i a1 = expand16((uint32_t)(d1 >> sh) & 0xffffu);
-#pragma unroll
-                    for (int t = 0; t < NQT; t++) {
-                        const ovx_v4i bf = sB[(t * NCHUNK + c) * 64 + lane];
-                        acc[0][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bf, acc[0][t], 0, 0, 0);
-                        acc[1][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bf, acc[1][t], 0, 0, 0);
-                    }
-                }
+                if (i == 0 && w0 == 0) mfma_word<NQT, NCHUNK, true>(d0, d1, 0, g, lane, s_exp, sB, acc);
+                else mfma_word<NQT, NCHUNK, false>(d0, d1, w0 + i, g, lane, s_exp, sB, acc);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
         bool any = false;
 #pragma unroll
-        for (int G = 0; G < 2; G++)
+        for (int t = 0; t < NQT; t++) {
+            int mx = acc[0][t][0];
 #pragma unroll
-            for (int t = 0; t < NQT; t++)
+            for (int G = 0; G < 2; G++)
 #pragma unroll
-                for (int v = 0; v < 16; v++) any |= acc[G][t][v] >= need[t];
+                for (int v = 0; v < 16; v++) mx = max(mx, acc[G][t][v]);
+            any |= mx >= need[t];
+        }
         if (__any(any)) {
 #pragma unroll
             for (int t = 0; t < NQT; t++) {
