@@ -818,14 +818,30 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                                                                   uint32_t nq, uint32_t blocks_per_seg,
                                                                   const uint32_t* __restrict__ thr, uint32_t cap,
                                                                   uint64_t* __restrict__ keys,
-                                                                  uint32_t* __restrict__ cnt, uint32_t ntop) {
+                                                                  uint32_t* __restrict__ cnt, uint32_t ntop,
+                                                                  uint32_t ntiles, uint32_t nsegs) {
     constexpr uint32_t QW = 32 * NQT, NCHUNK = 2 * QPV, NWAVE = kMfmaThreads / 64;
     constexpr int RING = QPV < 4 ? QPV : 4;
     extern __shared__ ovx_v4i sB[];  // [NQT][NCHUNK][64]
     __shared__ int s_base[QW];
     __shared__ uint32_t s_thr[QW];
     __shared__ uint2 s_exp[256];
-    const uint32_t q0 = blockIdx.x * QW;
+    // Workgroups are dealt to the 8 XCDs round-robin by id; map ids so that
+    // every query tile of a document segment lands on the same XCD and the
+    // segment's codes are read from HBM once per XCD L2 rather than per tile.
+    uint32_t tile, seg;
+    {
+        const uint32_t f = blockIdx.x;
+        if (nsegs % 8 == 0) {
+            const uint32_t xcd = f % 8, k = f / 8;
+            seg = xcd + 8 * (k / ntiles);
+            tile = k % ntiles;
+        } else {
+            tile = f % ntiles;
+            seg = f / ntiles;
+        }
+    }
+    const uint32_t q0 = tile * QW;
     for (uint32_t e = threadIdx.x; e < NQT * NCHUNK * 64; e += blockDim.x) {
         const uint32_t ln = e & 63, c = (e >> 6) % NCHUNK, t = (e >> 6) / NCHUNK;
         const uint32_t q = q0 + t * 32 + (ln & 31);
@@ -857,7 +873,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
         basev[t] = s_base[j];
         need[t] = q0 + j < nq ? basev[t] - (int)s_thr[j] : 0x7fffffff;
     }
-    const uint32_t b_begin = blockIdx.y * blocks_per_seg;
+    const uint32_t b_begin = seg * blocks_per_seg;
     const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
     for (uint32_t b = b_begin + wave; b < b_end; b += NWAVE) {
         ovx_v16i acc[2][NQT];
@@ -1808,13 +1824,14 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
         const uint32_t nqt = nq >= 128 && qpv <= 16 ? 4 : (nq >= 64 ? 2 : 1);
         const uint32_t tiles_m = (uint32_t)((nq + 32 * nqt - 1) / (32 * nqt));
         const uint64_t want = std::max<uint64_t>(1, (1024 + tiles_m - 1) / tiles_m);
-        const uint64_t nseg = std::min<uint64_t>(std::max<uint64_t>(1, (nblk + 7) / 8), want);
+        uint64_t nseg = std::min<uint64_t>(std::max<uint64_t>(1, (nblk + 7) / 8), want);
+        if (nseg >= 8) nseg &= ~7ull;  // XCD-grouped segments (see the kernel)
         const uint32_t bps = (uint32_t)((nblk + nseg - 1) / nseg);
         const uint32_t segs = (nblk + bps - 1) / bps;
         const size_t lds = (size_t)nqt * qpv * 2 * 64 * 16;
 #define OVX_MFMA(NQT, QPV)                                                                                         \
-    hipLaunchKernelGGL((k_s1_scan_mfma<NQT, OP, QPV>), dim3(tiles_m, segs), dim3(kMfmaThreads), lds, s, words, nblk, \
-                       n, qbits, (uint32_t)nq, bps, w.thr, p.cap, w.keys, w.cnt, ntop)
+    hipLaunchKernelGGL((k_s1_scan_mfma<NQT, OP, QPV>), dim3(tiles_m * segs), dim3(kMfmaThreads), lds, s, words,   \
+                       nblk, n, qbits, (uint32_t)nq, bps, w.thr, p.cap, w.keys, w.cnt, ntop, tiles_m, segs)
 #define OVX_MFMA_Q(NQT)                        \
     switch (qpv) {                             \
         case 16: OVX_MFMA(NQT, 16); break;     \
