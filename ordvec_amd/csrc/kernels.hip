@@ -1715,7 +1715,7 @@ FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
         p.stride = 1;
         p.nsample = 0;
     } else {
-        uint64_t stride = std::min<uint64_t>(32, std::max<uint64_t>(1, nblk / 512));
+        uint64_t stride = std::min<uint64_t>(64, std::max<uint64_t>(1, nblk / 256));
         p.stride = (uint32_t)stride;
         p.nsample = (uint32_t)((nblk + stride - 1) / stride);
         if (stride == 1) {
