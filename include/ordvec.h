@@ -228,6 +228,38 @@ ordvec_status_t ordvec_rankquant_codes(const ordvec_rankquant_t *index, uint8_t 
 ordvec_status_t ordvec_rankquant_write(const ordvec_rankquant_t *index, const char *path);
 ordvec_status_t ordvec_rankquant_load(const char *path, ordvec_rankquant_t **out);
 void ordvec_rankquant_free(ordvec_rankquant_t *index);
+/* src/quant.rs:629-695 search (symmetric): the query is rank-bucketed and
+ * scored through the scalar LUT scan, scale 1 / (R * R); outputs nq x
+ * min(k, n); dim <= 4096. */
+ordvec_status_t ordvec_rankquant_search(const ordvec_rankquant_t *index, const float *queries, uint64_t nq, uint64_t k,
+                                        float *out_scores, int64_t *out_ids, uint64_t flags);
+/* src/quant.rs:869-881 swap_remove: the last row moves into idx; *moved_from
+ * (optional) receives its old index (n - 1).  idx >= n is BAD_ARGUMENT. */
+ordvec_status_t ordvec_rankquant_swap_remove(ordvec_rankquant_t *index, uint64_t idx, uint64_t *moved_from);
+
+/* Rank (full-precision u16 rank vectors): src/rank.rs:291-584.  GPU add and
+ * symmetric search: dim <= 4096; asymmetric search: dim <= 8192. */
+typedef struct ordvec_rank_t ordvec_rank_t;
+/* src/rank.rs:317-324 new: 2 <= dim <= 65535. */
+ordvec_status_t ordvec_rank_new(uint64_t dim, ordvec_rank_t **out);
+/* src/rank.rs:337-356 add (rank transform per row). */
+ordvec_status_t ordvec_rank_add(ordvec_rank_t *index, const float *vectors, uint64_t n, uint64_t flags);
+ordvec_status_t ordvec_rank_info(const ordvec_rank_t *index, uint64_t *dim, uint64_t *n_vectors);
+/* src/rank.rs:364-412 search (Spearman): nq x min(k, n). */
+ordvec_status_t ordvec_rank_search(const ordvec_rank_t *index, const float *queries, uint64_t nq, uint64_t k,
+                                   float *out_scores, int64_t *out_ids, uint64_t flags);
+/* src/rank.rs:429-480 search_asymmetric: nq x min(k, n). */
+ordvec_status_t ordvec_rank_search_asymmetric(const ordvec_rank_t *index, const float *queries, uint64_t nq,
+                                              uint64_t k, float *out_scores, int64_t *out_ids, uint64_t flags);
+/* Copy the rank vectors (n x dim u16, .ovr payload order). */
+ordvec_status_t ordvec_rank_ranks(const ordvec_rank_t *index, uint16_t *out, uint64_t capacity, uint64_t flags);
+/* src/rank.rs:506-517 swap_remove (as ordvec_rankquant_swap_remove). */
+ordvec_status_t ordvec_rank_swap_remove(ordvec_rank_t *index, uint64_t idx, uint64_t *moved_from);
+/* src/rank_io.rs:514-640: .ovr persistence (load checks every row is a
+ * permutation of [0, dim)). */
+ordvec_status_t ordvec_rank_write(const ordvec_rank_t *index, const char *path);
+ordvec_status_t ordvec_rank_load(const char *path, ordvec_rank_t **out);
+void ordvec_rank_free(ordvec_rank_t *index);
 
 /* SignBitmap: src/sign_bitmap.rs:143-200 new/add, 330-483 top-m candidates. */
 ordvec_status_t ordvec_sign_bitmap_new(uint64_t dim, ordvec_sign_bitmap_t **out);
@@ -246,6 +278,8 @@ ordvec_status_t ordvec_sign_bitmap_words(const ordvec_sign_bitmap_t *index, uint
 ordvec_status_t ordvec_sign_bitmap_write(const ordvec_sign_bitmap_t *index, const char *path);
 ordvec_status_t ordvec_sign_bitmap_load(const char *path, ordvec_sign_bitmap_t **out);
 void ordvec_sign_bitmap_free(ordvec_sign_bitmap_t *index);
+/* src/sign_bitmap.rs:578-590 swap_remove (as ordvec_rankquant_swap_remove). */
+ordvec_status_t ordvec_sign_bitmap_swap_remove(ordvec_sign_bitmap_t *index, uint64_t idx, uint64_t *moved_from);
 
 /* Bitmap (top-bucket, constant-weight): src/bitmap.rs.  Bit j of a document
  * is set iff coordinate j is among its n_top highest ranks; a query's bits are
@@ -285,6 +319,8 @@ ordvec_status_t ordvec_bitmap_words(const ordvec_bitmap_t *index, uint64_t *out,
 ordvec_status_t ordvec_bitmap_write(const ordvec_bitmap_t *index, const char *path);
 ordvec_status_t ordvec_bitmap_load(const char *path, ordvec_bitmap_t **out);
 void ordvec_bitmap_free(ordvec_bitmap_t *index);
+/* src/bitmap.rs:515-527 swap_remove (as ordvec_rankquant_swap_remove). */
+ordvec_status_t ordvec_bitmap_swap_remove(ordvec_bitmap_t *index, uint64_t idx, uint64_t *moved_from);
 
 /* Two-stage search in one call: SignBitmap::top_m_candidates_batched_serial_csr
  * followed by RankQuant::search_asymmetric_subset_batched_serial_into

@@ -67,6 +67,10 @@ def _declare(L):
         "orc_score_ids_grad": (None, [P, U32, U32, P, U64, P, U64, P, P]),
         "orc_bitmap_encode": (None, [P, U64, U32, U32, P]),
         "orc_fs4_query": (None, [P, U32, P, P, P]),
+        "orc_rq_search_symmetric": (None, [P, U64, U32, U32, P, U64, U64, P, P]),
+        "orc_rank_norm": (C.c_float, [U32]),
+        "orc_rank_search": (None, [P, U64, U32, P, U64, U64, P, P]),
+        "orc_rank_search_asymmetric": (None, [P, U64, U32, P, U64, U64, P, P]),
         "orc_fs4_search": (None, [P, U64, U32, P, U64, U64, P, P]),
         "orc_bitmap_query": (None, [P, U64, U32, U32, P]),
         "orc_and_popcount_all": (None, [P, U64, U32, P, U64, P]),
@@ -420,6 +424,56 @@ def fs4_search(codes, d: int, queries, k: int):
     oi = np.full((nq, kk), -1, dtype=np.int64)
     if kk:
         lib().orc_fs4_search(_p(c), n, d, _p(q), nq, kk, _p(os_), _p(oi))
+    return os_, oi
+
+
+def _search_out(nq, k):
+    return np.zeros((nq, k), dtype=np.float32), np.full((nq, k), -1, dtype=np.int64)
+
+
+def rq_search_symmetric(codes, d: int, bits: int, queries, k: int):
+    """RankQuant::search, symmetric (src/quant.rs:629-695)."""
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    q = _f32(np.atleast_2d(queries))
+    n = c.shape[0]
+    kk = min(k, n)
+    os_, oi = _search_out(q.shape[0], kk)
+    if kk:
+        lib().orc_rq_search_symmetric(_p(c), n, d, bits, _p(q), q.shape[0], kk, _p(os_), _p(oi))
+    return os_, oi
+
+
+def rank_encode(x) -> np.ndarray:
+    """Rank::add (src/rank.rs:329-350): u16 ranks, row-major."""
+    x = _f32(np.atleast_2d(x))
+    return np.stack([rank_transform(r) for r in x]) if len(x) else np.zeros((0, x.shape[1]), np.uint16)
+
+
+def rank_norm(d: int) -> np.float32:
+    return np.float32(lib().orc_rank_norm(d))
+
+
+def rank_search(ranks, queries, k: int):
+    """Rank::search, symmetric (src/rank.rs:351-405)."""
+    r = np.ascontiguousarray(ranks, dtype=np.uint16)
+    q = _f32(np.atleast_2d(queries))
+    n, d = r.shape
+    kk = min(k, n)
+    os_, oi = _search_out(q.shape[0], kk)
+    if kk:
+        lib().orc_rank_search(_p(r), n, d, _p(q), q.shape[0], kk, _p(os_), _p(oi))
+    return os_, oi
+
+
+def rank_search_asymmetric(ranks, queries, k: int):
+    """Rank::search_asymmetric (src/rank.rs:420-471)."""
+    r = np.ascontiguousarray(ranks, dtype=np.uint16)
+    q = _f32(np.atleast_2d(queries))
+    n, d = r.shape
+    kk = min(k, n)
+    os_, oi = _search_out(q.shape[0], kk)
+    if kk:
+        lib().orc_rank_search_asymmetric(_p(r), n, d, _p(q), q.shape[0], kk, _p(os_), _p(oi))
     return os_, oi
 
 

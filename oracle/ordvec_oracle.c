@@ -834,6 +834,129 @@ void orc_fs_search(const uint8_t *packed, uint64_t n, uint32_t d, const float *q
 }
 
 /* ------------------------------------------------------------------ */
+/* Symmetric RankQuant search and the Rank index                         */
+/* ------------------------------------------------------------------ */
+
+static void topk_reset(topk_t *t) {
+    t->filled = 0;
+    t->worst_pos = 0;
+    t->worst_val = INFINITY;
+    t->worst_tie = INT64_MAX;
+}
+
+/* src/quant.rs:629-695 RankQuant::search (symmetric): the query is rank
+ * transformed and bucketed; lut[p*nb + b] = centre(qbucket_p) * centre(b);
+ * the scalar LUT scan (src/quant_kernels.rs:89-160) sums lut entries in
+ * coordinate order and scales by 1 / (R * R) (f32). */
+void orc_rq_search_symmetric(const uint8_t *codes, uint64_t n, uint32_t d, uint32_t bits, const float *queries,
+                             uint64_t nq, uint64_t k, float *os, int64_t *oi) {
+    if (k > n) k = n;
+    if (k == 0) return;
+    uint32_t bpv = d * bits / 8, nb = 1u << bits;
+    float norm = orc_rankquant_norm(d, bits);
+    float inv_norm_sq = 1.0f / (norm * norm);
+#pragma omp parallel
+    {
+        uint16_t *rk = (uint16_t *)malloc(d * sizeof(uint16_t));
+        float *lut = (float *)malloc((size_t)d * nb * sizeof(float));
+        topk_t t;
+        topk_init(&t, k);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t qi = 0; qi < (int64_t)nq; qi++) {
+            orc_rank_transform(queries + (uint64_t)qi * d, d, rk);
+            for (uint32_t p = 0; p < d; p++) {
+                float qc = orc_bucket_centre(orc_rank_to_bucket(rk[p], d, bits), bits);
+                for (uint32_t b = 0; b < nb; b++) lut[p * nb + b] = qc * orc_bucket_centre(b, bits);
+            }
+            topk_reset(&t);
+            for (uint64_t di = 0; di < n; di++) {
+                const uint8_t *row = codes + di * bpv;
+                float acc = 0.0f;
+                for (uint32_t p = 0; p < d; p++) acc += lut[p * nb + code_at(row, p, bits)];
+                topk_insert(&t, acc * inv_norm_sq + 0.0f, (int64_t)di, (int64_t)di);
+            }
+            topk_finalize(&t, os + (uint64_t)qi * k, oi + (uint64_t)qi * k, k);
+        }
+        topk_free(&t);
+        free(rk);
+        free(lut);
+    }
+}
+
+/* src/rank.rs:249-252 rank_norm = sqrt(d (d^2 - 1) / 12), f64 then f32. */
+float orc_rank_norm(uint32_t d) {
+    double x = (double)d;
+    return (float)sqrt(x * (x * x - 1.0) / 12.0);
+}
+
+/* src/rank.rs:340-405 Rank::search (symmetric): exact i64 sum of centred
+ * doubled ranks, score = (f32)acc * 0.25 * (1 / (norm * norm)). */
+void orc_rank_search(const uint16_t *ranks, uint64_t n, uint32_t d, const float *queries, uint64_t nq, uint64_t k,
+                     float *os, int64_t *oi) {
+    if (k > n) k = n;
+    if (k == 0) return;
+    int32_t mean_2x = (int32_t)d - 1;
+    float norm = orc_rank_norm(d);
+    float inv_norm_sq = 1.0f / (norm * norm);
+#pragma omp parallel
+    {
+        uint16_t *rk = (uint16_t *)malloc(d * sizeof(uint16_t));
+        topk_t t;
+        topk_init(&t, k);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t qi = 0; qi < (int64_t)nq; qi++) {
+            orc_rank_transform(queries + (uint64_t)qi * d, d, rk);
+            topk_reset(&t);
+            for (uint64_t di = 0; di < n; di++) {
+                const uint16_t *doc = ranks + di * d;
+                int64_t acc = 0;
+                for (uint32_t p = 0; p < d; p++)
+                    acc += (int64_t)(2 * (int32_t)rk[p] - mean_2x) * (int64_t)(2 * (int32_t)doc[p] - mean_2x);
+                float s = (float)acc * 0.25f * inv_norm_sq;
+                topk_insert(&t, s + 0.0f, (int64_t)di, (int64_t)di);
+            }
+            topk_finalize(&t, os + (uint64_t)qi * k, oi + (uint64_t)qi * k, k);
+        }
+        topk_free(&t);
+        free(rk);
+    }
+}
+
+/* src/rank.rs:420-471 Rank::search_asymmetric: q_unit = l2_normalise(q),
+ * q_sum = sum q_unit (from -0.0), acc = sum q_unit[p] * rank[p] in order,
+ * score = (acc - mean * q_sum) * (1 / norm), mean = (d - 1) / 2. */
+void orc_rank_search_asymmetric(const uint16_t *ranks, uint64_t n, uint32_t d, const float *queries, uint64_t nq,
+                                uint64_t k, float *os, int64_t *oi) {
+    if (k > n) k = n;
+    if (k == 0) return;
+    float inv_norm = 1.0f / orc_rank_norm(d);
+    float mean = ((float)d - 1.0f) / 2.0f;
+#pragma omp parallel
+    {
+        float *qu = (float *)malloc(d * sizeof(float));
+        topk_t t;
+        topk_init(&t, k);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t qi = 0; qi < (int64_t)nq; qi++) {
+            orc_l2_normalise(queries + (uint64_t)qi * d, d, qu);
+            float q_sum = -0.0f;
+            for (uint32_t p = 0; p < d; p++) q_sum += qu[p];
+            topk_reset(&t);
+            for (uint64_t di = 0; di < n; di++) {
+                const uint16_t *doc = ranks + di * d;
+                float acc = 0.0f;
+                for (uint32_t p = 0; p < d; p++) acc += qu[p] * (float)doc[p];
+                float s = (acc - mean * q_sum) * inv_norm;
+                topk_insert(&t, s + 0.0f, (int64_t)di, (int64_t)di);
+            }
+            topk_finalize(&t, os + (uint64_t)qi * k, oi + (uint64_t)qi * k, k);
+        }
+        topk_free(&t);
+        free(qu);
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* FastScan b=4: the b=2 recipe (src/fastscan.rs:130-173, 316-476) carried
  * to one 4-bit rank code per coordinate.  The reference ships b=2 only; this
  * is the extension the b=4 GPU path implements, stated once here:

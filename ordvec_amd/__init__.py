@@ -1,5 +1,5 @@
 """MI355X-native ordinal retrieval (ordvec's RankQuant / SignBitmap /
-FastScan hot path) behind ordvec's C ABI.
+FastScan hot path, with the Rank and Bitmap indexes) behind ordvec's C ABI.
 
 Importing the package loads ``ordvec_amd/lib/libordvec_amd.so``; there is no
 other compute path.
@@ -14,6 +14,7 @@ from .index import (  # noqa: E402,F401
     MAX_VECTORS,
     Bitmap,
     FastscanBatch,
+    Rank,
     RankQuant,
     RankQuantFastscan,
     SignBitmap,
@@ -28,6 +29,7 @@ except ImportError:  # pragma: no cover
     pass
 
 __all__ = [
+    "Rank",
     "RankQuant",
     "Bitmap",
     "FastscanBatch",

@@ -189,6 +189,20 @@ SIGNATURES = {
     "ordvec_server_wait": (ST, [P, U64, P, P]),
     "ordvec_server_open_loop": (ST, [P, P, U64, C.c_double, C.c_double, U32, U32, U32, U32, U64, P]),
     "ordvec_server_stop": (None, [P]),
+    "ordvec_rankquant_search": (ST, [P, P, U64, U64, P, P, U64]),
+    "ordvec_rankquant_swap_remove": (ST, [P, U64, P]),
+    "ordvec_sign_bitmap_swap_remove": (ST, [P, U64, P]),
+    "ordvec_bitmap_swap_remove": (ST, [P, U64, P]),
+    "ordvec_rank_new": (ST, [U64, C.POINTER(P)]),
+    "ordvec_rank_add": (ST, [P, P, U64, U64]),
+    "ordvec_rank_info": (ST, [P, P, P]),
+    "ordvec_rank_search": (ST, [P, P, U64, U64, P, P, U64]),
+    "ordvec_rank_search_asymmetric": (ST, [P, P, U64, U64, P, P, U64]),
+    "ordvec_rank_ranks": (ST, [P, P, U64, U64]),
+    "ordvec_rank_swap_remove": (ST, [P, U64, P]),
+    "ordvec_rank_write": (ST, [P, C.c_char_p]),
+    "ordvec_rank_load": (ST, [C.c_char_p, C.POINTER(P)]),
+    "ordvec_rank_free": (None, [P]),
 }
 
 _lib = None

@@ -209,6 +209,25 @@ void sync() { OVX_HIP(hipStreamSynchronize(cur().stream)); }
 // ---------------------------------------------------------------------------
 // Index handles
 // ---------------------------------------------------------------------------
+// Grows a row-major device buffer of `n` rows to hold `want` rows.
+template <typename T>
+void grow_rows(T*& buf, uint64_t& cap, uint64_t n, uint64_t row_bytes, uint64_t want, const char* what) {
+    if (want <= cap) return;
+    uint64_t nc = std::max<uint64_t>(want, cap + cap / 2);
+    T* p = nullptr;
+    if (hipMalloc(&p, std::max<uint64_t>(16, nc * row_bytes)) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(kInternal, std::string("HBM allocation for ") + what + " failed");
+    }
+    if (buf) {
+        OVX_HIP(hipMemcpyAsync(p, buf, n * row_bytes, hipMemcpyDeviceToDevice, cur().stream));
+        sync();
+        OVX_HIP(hipFree(buf));
+    }
+    buf = p;
+    cap = nc;
+}
+
 struct RqIndex {
     uint32_t dim = 0, bits = 0, bpv = 0;
     uint64_t n = 0, cap = 0;
@@ -216,22 +235,19 @@ struct RqIndex {
     ~RqIndex() {
         if (codes) (void)hipFree(codes);
     }
-    void reserve(uint64_t want) {
-        if (want <= cap) return;
-        uint64_t nc = std::max<uint64_t>(want, cap + cap / 2);
-        uint8_t* p = nullptr;
-        if (hipMalloc(&p, std::max<uint64_t>(1, nc * bpv)) != hipSuccess) {
-            (void)hipGetLastError();
-            fail(kInternal, "HBM allocation for RankQuant codes failed");
-        }
-        if (codes) {
-            OVX_HIP(hipMemcpyAsync(p, codes, n * bpv, hipMemcpyDeviceToDevice, cur().stream));
-            sync();
-            OVX_HIP(hipFree(codes));
-        }
-        codes = p;
-        cap = nc;
+    void reserve(uint64_t want) { grow_rows(codes, cap, n, bpv, want, "RankQuant codes"); }
+};
+
+// Rank (src/rank.rs:291-305): u16 rank vectors, row-major [n][dim].
+struct RkIndex {
+    uint32_t dim = 0;
+    uint64_t n = 0, cap = 0;
+    uint16_t* ranks = nullptr;
+    ~RkIndex() {
+        if (ranks) (void)hipFree(ranks);
     }
+    uint64_t row_bytes() const { return (uint64_t)dim * 2; }
+    void reserve(uint64_t want) { grow_rows(ranks, cap, n, row_bytes(), want, "Rank vectors"); }
 };
 
 // Sign bitmap (ntop == 0) or top-bucket Bitmap (ntop > 0): same interleaved
@@ -566,6 +582,112 @@ void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, cons
 
 // Top-bucket Bitmap add (src/bitmap.rs:140-162): rank each row, set the bits
 // of its n_top highest-ranked coordinates.
+// ---------------------------------------------------------------------------
+// Rank index and symmetric RankQuant search (src/rank.rs:329-471,
+// src/quant.rs:629-695).  Queries are processed in chunks so the grid's
+// query dimension and the workspace stay bounded.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kQueryChunk = 16384;
+
+void rank_add(RkIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
+    if (ix->dim > kEncodeMaxDim)
+        fail(kUnsupportedOperation, "GPU rank transform supports dim <= " + std::to_string(kEncodeMaxDim));
+    check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
+    check_new_count(ix->n, n);
+    ix->reserve(ix->n + n);
+    const uint64_t base = ix->n;
+    for_each_chunk(vectors, n, ix->dim, flags, [&](const float* dv, uint64_t r0, uint64_t cnt) {
+        launch_rank_u16(dv, cnt, ix->dim, ix->ranks + (base + r0) * ix->dim, cur().stream);
+    });
+    ix->n += n;
+    if (!(flags & ORDVEC_FLAG_DEVICE_PTRS)) sync();
+}
+
+// Shared driver: `enqueue(dq, cn, kk, ds, di, qstat, cv)` scores one chunk of
+// queries; `per_query_status` says whether it fills qstat (asymmetric paths)
+// or the queries were checked up front (rank-transformed paths).
+template <typename Enqueue>
+void chunked_full_search(uint32_t d, uint64_t n, const float* queries, uint64_t nq, uint64_t k, float* out_s,
+                         int64_t* out_i, uint64_t flags, bool per_query_status, size_t extra_bytes_per_query,
+                         Enqueue&& enqueue) {
+    const uint64_t kk = std::min<uint64_t>(k, n);
+    if (nq == 0 || kk == 0) return;
+    check_k(kk);
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    if (per_query_status) host_finite_or_fail(queries, nq * d, flags, kNonfiniteQuery, "queries");
+    else check_finite(queries, nq * d, flags, kNonfiniteQuery, "queries");
+    for (uint64_t q0 = 0; q0 < nq; q0 += kQueryChunk) {
+        const uint64_t cn = std::min<uint64_t>(kQueryChunk, nq - q0);
+        // + 1024: alignment slack of the enqueue's up to four carved buffers
+        size_t bytes = al(cn * 4) + 1024 + al(cn * extra_bytes_per_query) + fullscan_work_bytes(n, cn, (uint32_t)kk) +
+                       (dev ? 0 : al(cn * d * 4) + al(cn * kk * 4) + al(cn * kk * 8));
+        Carver cv{static_cast<char*>(cur().workspace(bytes))};
+        const float* dq = stage_queries(cv, queries + q0 * d, cn * d, flags);
+        uint32_t* qstat = cv.take<uint32_t>(cn);
+        float* ds = dev ? out_s + q0 * kk : cv.take<float>(cn * kk);
+        int64_t* di = dev ? out_i + q0 * kk : cv.take<int64_t>(cn * kk);
+        enqueue(dq, cn, (uint32_t)kk, ds, di, qstat, cv);
+        if (!dev) {
+            d2h(out_s + q0 * kk, ds, cn * kk * 4);
+            d2h(out_i + q0 * kk, di, cn * kk * 8);
+        }
+        if (per_query_status) finish_status(qstat, cn, flags, kNonfiniteQuery, "queries");
+        else if (!dev) sync();
+    }
+}
+
+void rank_search(const RkIndex* ix, const float* queries, uint64_t nq, uint64_t k, bool symmetric, float* out_s,
+                 int64_t* out_i, uint64_t flags) {
+    const uint32_t d = ix->dim;
+    if (symmetric && d > kEncodeMaxDim)
+        fail(kUnsupportedOperation, "GPU query rank transform supports dim <= " + std::to_string(kEncodeMaxDim));
+    check_search_dim(d);
+    const size_t extra = symmetric ? (size_t)d * 2 : (size_t)d * 4 + 4;
+    chunked_full_search(d, ix->n, queries, nq, k, out_s, out_i, flags, !symmetric, extra,
+                        [&](const float* dq, uint64_t cn, uint32_t kk, float* ds, int64_t* di, uint32_t* qstat,
+                            Carver& cv) {
+                            hipStream_t s = cur().stream;
+                            void* work = cv.take<char>(fullscan_work_bytes(ix->n, cn, kk));
+                            if (symmetric) {
+                                uint16_t* qr = cv.take<uint16_t>(cn * d);
+                                launch_rank_u16(dq, cn, d, qr, s);
+                                launch_rank_search(ix->ranks, ix->n, d, true, qr, nullptr, nullptr, cn, kk, work, ds,
+                                                   di, s);
+                            } else {
+                                float* qhat = cv.take<float>(cn * d);
+                                float* qsum = cv.take<float>(cn);
+                                launch_prepare_sum(dq, cn, d, qhat, qsum, qstat, s);
+                                launch_rank_search(ix->ranks, ix->n, d, false, nullptr, qhat, qsum, cn, kk, work, ds,
+                                                   di, s);
+                            }
+                        });
+}
+
+// RankQuant::search: rank-bucket the query, q[p] = centre(bucket_p), then the
+// scalar LUT scan with scale 1 / (R * R) and a zero offset.
+void rq_symmetric_search(const RqIndex* ix, const float* queries, uint64_t nq, uint64_t k, float* out_s,
+                         int64_t* out_i, uint64_t flags) {
+    const uint32_t d = ix->dim;
+    if (d > kEncodeMaxDim)
+        fail(kUnsupportedOperation, "GPU query rank transform supports dim <= " + std::to_string(kEncodeMaxDim));
+    const double nb = (double)(1u << ix->bits);  // src/rank.rs:275-280 rankquant_norm
+    const float R = (float)std::sqrt((double)d * ((nb * nb - 1.0) / 12.0));
+    const float inv_norm_sq = 1.0f / (R * R);
+    chunked_full_search(d, ix->n, queries, nq, k, out_s, out_i, flags, false, (size_t)d * 6 + 4,
+                        [&](const float* dq, uint64_t cn, uint32_t kk, float* ds, int64_t* di, uint32_t*, Carver& cv) {
+                            hipStream_t s = cur().stream;
+                            void* work = cv.take<char>(fullscan_work_bytes(ix->n, cn, kk));
+                            uint16_t* qr = cv.take<uint16_t>(cn * d);
+                            float* qc = cv.take<float>(cn * d);
+                            float* zero = cv.take<float>(cn);
+                            launch_rank_u16(dq, cn, d, qr, s);
+                            launch_rank_centres(qr, cn * d, d, ix->bits, qc, s);
+                            OVX_HIP(hipMemsetAsync(zero, 0, cn * 4, s));
+                            launch_fullscan_rq(ix->codes, ix->n, d, ix->bits, 0, inv_norm_sq, qc, zero, cn, kk, work,
+                                               ds, di, s);
+                        });
+}
+
 void bm_add(SbIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
     if (ix->dim > kEncodeMaxDim) fail(kUnsupportedOperation, "GPU Bitmap encode supports dim <= " + std::to_string(kEncodeMaxDim));
     check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
@@ -1321,6 +1443,81 @@ void write_ovrq(const RqIndex* ix, const char* path) {
     write_file(path, h, host.data(), host.size());
 }
 
+// src/rank_io.rs:574-640: header, exact payload, every row a permutation of
+// [0, dim) (bound check plus a per-row stamp for repeats).
+RkIndex* load_ovr(const char* path) {
+    Reader r;
+    r.buf = read_file(path);
+    if (!magic_is(r.buf, "OVR1", "TVR1")) fail(kCorruptIndex, "not an OVR1/TVR1 (Rank) file: wrong magic");
+    r.label = "OVR1";
+    r.pos = 4;
+    r.version();
+    const uint32_t dim = r.u32("dim");
+    if (dim < 2 || dim > kMaxDim) fail(kCorruptIndex, "dim " + std::to_string(dim) + " out of range [2, 65535]");
+    const uint32_t n = r.u32("n_vectors");
+    if (n > kMaxVectors) fail(kCorruptIndex, "n_vectors " + std::to_string(n) + " exceeds MAX_VECTORS");
+    const uint64_t payload = (uint64_t)n * dim * 2;
+    r.payload_exact(payload);
+    const uint8_t* pay = r.buf.data() + r.pos;
+    std::vector<uint32_t> seen(dim, 0);
+    for (uint64_t row = 0; row < n; row++) {
+        const uint32_t stamp = (uint32_t)row + 1;
+        for (uint32_t p = 0; p < dim; p++) {
+            const uint64_t o = (row * dim + p) * 2;
+            const uint32_t v = (uint32_t)pay[o] | ((uint32_t)pay[o + 1] << 8);
+            if (v >= dim)
+                fail(kCorruptIndex, "OVR1 rank value " + std::to_string(v) + " >= dim (" + std::to_string(dim) +
+                                        "); ranks must be a permutation of [0, dim)");
+            if (seen[v] == stamp)
+                fail(kCorruptIndex, "OVR1 row " + std::to_string(row) + " is not a permutation of [0, dim): value " +
+                                        std::to_string(v) + " repeats");
+            seen[v] = stamp;
+        }
+    }
+    RkIndex* ix = new RkIndex();
+    ix->dim = dim;
+    try {
+        eng().init();
+        ix->reserve(n);
+        h2d(ix->ranks, pay, payload);  // u16 little-endian on both sides
+        sync();
+    } catch (...) {
+        delete ix;
+        throw;
+    }
+    ix->n = n;
+    return ix;
+}
+
+// src/rank_io.rs:544-560: "OVR1" | version 1 | dim u32 | n u32 | u16 LE ranks.
+void write_ovr(const RkIndex* ix, const char* path) {
+    const uint64_t bytes = ix->n * ix->row_bytes();
+    if (bytes > kMaxPayload) fail(kBadArgument, "payload exceeds MAX_PAYLOAD");
+    std::vector<uint8_t> host(bytes);
+    d2h(host.data(), ix->ranks, bytes);
+    sync();
+    std::vector<uint8_t> h = {'O', 'V', 'R', '1', 1};
+    put_u32(h, ix->dim);
+    put_u32(h, (uint32_t)ix->n);
+    write_file(path, h, host.data(), bytes);
+}
+
+// swap_remove on row-major storage (src/rank.rs:506-517, src/quant.rs:869-881):
+// the last row moves into `idx`; returns the moved row's old index.
+uint64_t swap_remove_rows(void* buf, uint64_t& n, uint64_t row_bytes, uint64_t idx) {
+    if (idx >= n)
+        fail(kBadArgument, "index " + std::to_string(idx) + " out of range (index holds " + std::to_string(n) +
+                               " vectors)");
+    const uint64_t last = n - 1;
+    if (idx != last) {
+        char* b = static_cast<char*>(buf);
+        d2d(b + idx * row_bytes, b + last * row_bytes, row_bytes);
+        sync();
+    }
+    n = last;
+    return last;
+}
+
 // Row-major host words <-> interleaved device layout.
 void sb_upload_rowmajor(SbIndex* ix, const uint64_t* rows, uint64_t n) {
     const uint64_t nblk = (n + 63) / 64, bw = ix->block_words();
@@ -1602,10 +1799,12 @@ struct ordvec_sign_bitmap_t {};
 struct ordvec_fastscan_t {};
 struct ordvec_server_t {};
 struct ordvec_bitmap_t {};
+struct ordvec_rank_t {};
 
 static IndexHandle* H(const ordvec_index_t* p) { return reinterpret_cast<IndexHandle*>(const_cast<ordvec_index_t*>(p)); }
 static RqIndex* RQ(const ordvec_rankquant_t* p) { return reinterpret_cast<RqIndex*>(const_cast<ordvec_rankquant_t*>(p)); }
 static SbIndex* SB(const ordvec_sign_bitmap_t* p) { return reinterpret_cast<SbIndex*>(const_cast<ordvec_sign_bitmap_t*>(p)); }
+static RkIndex* RK(const ordvec_rank_t* p) { return reinterpret_cast<RkIndex*>(const_cast<ordvec_rank_t*>(p)); }
 static FsIndex* FS(const ordvec_fastscan_t* p) { return reinterpret_cast<FsIndex*>(const_cast<ordvec_fastscan_t*>(p)); }
 
 #define LOCK std::lock_guard<std::recursive_mutex> _lk(eng().mu)
@@ -2185,6 +2384,142 @@ void ordvec_rankquant_free(ordvec_rankquant_t* index) {
     delete RQ(index);
 }
 
+ordvec_status_t ordvec_rankquant_search(const ordvec_rankquant_t* index, const float* queries, uint64_t nq, uint64_t k,
+                                       float* out_scores, int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const RqIndex* ix = RQ(index);
+        if (nq == 0 || std::min<uint64_t>(k, ix->n) == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        LOCK;
+        rq_symmetric_search(ix, queries, nq, k, out_scores, out_ids, flags);
+    });
+}
+
+ordvec_status_t ordvec_rankquant_swap_remove(ordvec_rankquant_t* index, uint64_t idx, uint64_t* moved_from) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        LOCK;
+        RqIndex* ix = RQ(index);
+        uint64_t last = swap_remove_rows(ix->codes, ix->n, ix->bpv, idx);
+        if (moved_from) *moved_from = last;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Rank (src/rank.rs:291-584)
+// ---------------------------------------------------------------------------
+ordvec_status_t ordvec_rank_new(uint64_t dim, ordvec_rank_t** out) {
+    return boundary([&] {
+        NONNULL(out, "out");
+        *out = nullptr;
+        if (dim < 2) fail(kBadArgument, "dim must be >= 2");
+        if (dim > kMaxDim) fail(kBadArgument, "dim must fit in u16");
+        LOCK;
+        eng().init();
+        RkIndex* ix = new RkIndex();
+        ix->dim = (uint32_t)dim;
+        *out = reinterpret_cast<ordvec_rank_t*>(ix);
+    });
+}
+
+ordvec_status_t ordvec_rank_add(ordvec_rank_t* index, const float* vectors, uint64_t n, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        if (n == 0) return;
+        NONNULL(vectors, "vectors");
+        LOCK;
+        rank_add(RK(index), vectors, n, flags);
+    });
+}
+
+ordvec_status_t ordvec_rank_info(const ordvec_rank_t* index, uint64_t* dim, uint64_t* n) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        if (dim) *dim = RK(index)->dim;
+        if (n) *n = RK(index)->n;
+    });
+}
+
+static ordvec_status_t rank_search_api(const ordvec_rank_t* index, const float* queries, uint64_t nq, uint64_t k,
+                                       float* out_scores, int64_t* out_ids, uint64_t flags, bool symmetric) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const RkIndex* ix = RK(index);
+        if (nq == 0 || std::min<uint64_t>(k, ix->n) == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        LOCK;
+        rank_search(ix, queries, nq, k, symmetric, out_scores, out_ids, flags);
+    });
+}
+
+ordvec_status_t ordvec_rank_search(const ordvec_rank_t* index, const float* queries, uint64_t nq, uint64_t k,
+                                   float* out_scores, int64_t* out_ids, uint64_t flags) {
+    return rank_search_api(index, queries, nq, k, out_scores, out_ids, flags, true);
+}
+
+ordvec_status_t ordvec_rank_search_asymmetric(const ordvec_rank_t* index, const float* queries, uint64_t nq,
+                                              uint64_t k, float* out_scores, int64_t* out_ids, uint64_t flags) {
+    return rank_search_api(index, queries, nq, k, out_scores, out_ids, flags, false);
+}
+
+ordvec_status_t ordvec_rank_ranks(const ordvec_rank_t* index, uint16_t* out, uint64_t capacity, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const RkIndex* ix = RK(index);
+        const uint64_t count = ix->n * ix->dim;
+        if (capacity < count) fail(kBufferTooSmall, "ranks buffer too small");
+        if (!count) return;
+        NONNULL(out, "out");
+        LOCK;
+        if (flags & ORDVEC_FLAG_DEVICE_PTRS) {
+            d2d(out, ix->ranks, count * 2);
+        } else {
+            d2h(out, ix->ranks, count * 2);
+            sync();
+        }
+    });
+}
+
+ordvec_status_t ordvec_rank_swap_remove(ordvec_rank_t* index, uint64_t idx, uint64_t* moved_from) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        LOCK;
+        RkIndex* ix = RK(index);
+        uint64_t last = swap_remove_rows(ix->ranks, ix->n, ix->row_bytes(), idx);
+        if (moved_from) *moved_from = last;
+    });
+}
+
+ordvec_status_t ordvec_rank_write(const ordvec_rank_t* index, const char* path) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        NONNULL(path, "path");
+        LOCK;
+        write_ovr(RK(index), path);
+    });
+}
+
+ordvec_status_t ordvec_rank_load(const char* path, ordvec_rank_t** out) {
+    return boundary([&] {
+        NONNULL(path, "path");
+        NONNULL(out, "out");
+        *out = nullptr;
+        LOCK;
+        *out = reinterpret_cast<ordvec_rank_t*>(load_ovr(path));
+    });
+}
+
+void ordvec_rank_free(ordvec_rank_t* index) {
+    if (!index) return;
+    LOCK;
+    delete RK(index);
+}
+
 ordvec_status_t ordvec_sign_bitmap_new(uint64_t dim, ordvec_sign_bitmap_t** out) {
     return boundary([&] {
         NONNULL(out, "out");
@@ -2264,6 +2599,28 @@ ordvec_status_t ordvec_sign_bitmap_load(const char* path, ordvec_sign_bitmap_t**
         LOCK;
         eng().init();
         *out = reinterpret_cast<ordvec_sign_bitmap_t*>(load_ovsb(path));
+    });
+}
+
+// src/sign_bitmap.rs:578-590 swap_remove on the interleaved layout; the
+// vacated slot is cleared so padding stays zero.
+static uint64_t sb_swap_remove(SbIndex* ix, uint64_t idx) {
+    if (idx >= ix->n)
+        fail(kBadArgument, "index " + std::to_string(idx) + " out of range (index holds " + std::to_string(ix->n) +
+                               " vectors)");
+    const uint64_t last = ix->n - 1;
+    launch_move_doc_interleaved(ix->words, ix->qpv, last, idx, cur().stream);
+    sync();
+    ix->n = last;
+    return last;
+}
+
+ordvec_status_t ordvec_sign_bitmap_swap_remove(ordvec_sign_bitmap_t* index, uint64_t idx, uint64_t* moved_from) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        LOCK;
+        uint64_t last = sb_swap_remove(SB(index), idx);
+        if (moved_from) *moved_from = last;
     });
 }
 
@@ -2453,6 +2810,15 @@ ordvec_status_t ordvec_bitmap_load(const char* path, ordvec_bitmap_t** out) {
         *out = nullptr;
         LOCK;
         *out = reinterpret_cast<ordvec_bitmap_t*>(load_ovbm(path));
+    });
+}
+
+ordvec_status_t ordvec_bitmap_swap_remove(ordvec_bitmap_t* index, uint64_t idx, uint64_t* moved_from) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        LOCK;
+        uint64_t last = sb_swap_remove(BM(index), idx);  // src/bitmap.rs:515-527
+        if (moved_from) *moved_from = last;
     });
 }
 

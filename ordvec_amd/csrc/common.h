@@ -101,6 +101,20 @@ void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, u
                         hipStream_t s);
 void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, hipStream_t s);
 void launch_fastscan_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint8_t* packed, hipStream_t s);
+// Rank vectors (u16 [n][d]) of rows; centre(bucket(rank)) of rank vectors.
+void launch_rank_u16(const float* x, uint64_t n, uint32_t d, uint16_t* ranks, hipStream_t s);
+void launch_rank_centres(const uint16_t* ranks, uint64_t count, uint32_t d, uint32_t bits, float* out,
+                         hipStream_t s);
+// Rank index full search: symmetric (query ranks) or asymmetric (q_hat and
+// its sum); work = fullscan_work_bytes(n, nq, k).
+void launch_rank_search(const uint16_t* ranks, uint64_t n, uint32_t d, bool symmetric, const uint16_t* qranks,
+                        const float* qhat, const float* qsum, uint64_t nq, uint32_t k, void* work, float* out_s,
+                        int64_t* out_i, hipStream_t s);
+// q_hat = l2_normalise(q) and q_sum = sum q_hat (sequential, from -0.0).
+void launch_prepare_sum(const float* q, uint64_t nq, uint32_t d, float* qhat, float* qsum, uint32_t* qstat,
+                        hipStream_t s);
+// swap_remove on the interleaved bitmap layout: document `from` -> `to`.
+void launch_move_doc_interleaved(uint64_t* words, uint32_t qpv, uint64_t from, uint64_t to, hipStream_t s);
 // Top-bucket bitmaps of documents (interleaved layout), d <= kEncodeMaxDim.
 void launch_topbits_docs(const float* x, uint64_t n, uint32_t d, uint32_t ntop, uint64_t* out, uint64_t first_doc,
                          hipStream_t s);

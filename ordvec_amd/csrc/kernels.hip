@@ -146,13 +146,16 @@ __global__ __launch_bounds__(256) void k_sign_encode(const float* __restrict__ x
 // scatter rank -> bucket = floor(rank * 2^bits / d), pack MSB-first.
 // -0.0 is canonicalised to +0.0 first: the reference compares with `==`
 // before total_cmp, so the two zeros tie and fall back to index order.
+// RANKS: write the u16 rank vector (Rank::add, src/rank.rs:329-350) instead.
+template <bool RANKS>
 __global__ __launch_bounds__(256) void k_rank_encode(const float* __restrict__ x, uint64_t n, uint32_t d,
-                                                     uint32_t bits, uint8_t* __restrict__ codes) {
+                                                     uint32_t bits, uint8_t* __restrict__ codes,
+                                                     uint16_t* __restrict__ ranks) {
     extern __shared__ uint64_t smem[];
     const uint32_t P = next_pow2(d);
     uint64_t* keys = smem;
     uint8_t* buck = reinterpret_cast<uint8_t*>(smem + P);
-    const uint32_t cpb = 8 / bits, bpv = d / cpb, nb = 1u << bits;
+    const uint32_t cpb = RANKS ? 1 : 8 / bits, bpv = d / cpb, nb = 1u << bits;
     for (uint64_t r = blockIdx.x; r < n; r += gridDim.x) {
         const float* v = x + r * d;
         for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
@@ -166,6 +169,11 @@ __global__ __launch_bounds__(256) void k_rank_encode(const float* __restrict__ x
         }
         __syncthreads();
         bitonic_sort(keys, P);
+        if (RANKS) {
+            for (uint32_t p = threadIdx.x; p < d; p += blockDim.x) ranks[r * d + (keys[p] & 0xffffu)] = (uint16_t)p;
+            __syncthreads();
+            continue;
+        }
         for (uint32_t p = threadIdx.x; p < d; p += blockDim.x) {
             uint32_t idx = (uint32_t)(keys[p] & 0xffffu);
             buck[idx] = (uint8_t)(((uint64_t)p * nb) / d);
@@ -1254,6 +1262,131 @@ __global__ __launch_bounds__(256) void k_fullscan_rq(const uint8_t* __restrict__
     for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) out[j] = tk.keys[j];
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric scoring and the Rank index (src/quant.rs:629-695, src/rank.rs:291-471)
+// ---------------------------------------------------------------------------
+
+// centre(bucket(rank)) for the symmetric RankQuant search's query side.
+__global__ void k_rank_centres(const uint16_t* __restrict__ ranks, uint64_t count, uint32_t d, uint32_t bits,
+                               float* __restrict__ out) {
+    const float half = ((float)(1u << bits) - 1.0f) / 2.0f;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)(((uint64_t)ranks[i] << bits) / d);
+        out[i] = __fadd_rn((float)b, -half);
+    }
+}
+
+// Rank rows are u16 [n][d]; VEC loads 8 ranks at a time (d % 8 == 0).
+template <bool VEC>
+__device__ __forceinline__ void rank_row8(const uint16_t* __restrict__ row, uint32_t p, uint32_t* r) {
+    if (VEC) {
+        const uint4 v = *reinterpret_cast<const uint4*>(row + p);
+        r[0] = v.x & 0xffffu; r[1] = v.x >> 16; r[2] = v.y & 0xffffu; r[3] = v.y >> 16;
+        r[4] = v.z & 0xffffu; r[5] = v.z >> 16; r[6] = v.w & 0xffffu; r[7] = v.w >> 16;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; j++) r[j] = row[p + j];
+    }
+}
+
+// Rank::search: acc = sum (2 q_p - (d-1)) (2 r_p - (d-1)) exactly (i32 while
+// d^3 fits, else i64), score = (f32)acc * 0.25 * inv_norm_sq.  Grid
+// (segments, queries), one document per thread per round, segment top-k.
+template <bool VEC, bool WIDE>
+__global__ __launch_bounds__(256) void k_rank_sym(const uint16_t* __restrict__ ranks, uint64_t n, uint32_t d,
+                                                  const uint16_t* __restrict__ qranks, float inv_norm_sq,
+                                                  uint64_t per_seg, uint32_t k, uint64_t* __restrict__ partial) {
+    __shared__ TopKShared tk;
+    extern __shared__ int32_t qc[];
+    const uint64_t q = blockIdx.y, seg = blockIdx.x, nseg = gridDim.x;
+    const int32_t m2 = (int32_t)d - 1;
+    for (uint32_t p = threadIdx.x; p < d; p += blockDim.x) qc[p] = 2 * (int32_t)qranks[q * d + p] - m2;
+    const uint64_t begin = seg * per_seg, end = min(n, begin + per_seg);
+    tk_init(tk, k);
+    for (uint64_t r0 = begin; r0 < end; r0 += blockDim.x) {
+        tk_reserve(tk, k, blockDim.x);
+        const uint64_t di = r0 + threadIdx.x;
+        if (di < end) {
+            const uint16_t* row = ranks + di * d;
+            int64_t acc64 = 0;
+            int32_t acc32 = 0;
+            uint32_t p = 0;
+            for (; p + 8 <= d; p += 8) {
+                uint32_t r[8];
+                rank_row8<VEC>(row, p, r);
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int32_t dc = 2 * (int32_t)r[j] - m2;
+                    if (WIDE) acc64 += (int64_t)qc[p + j] * dc;
+                    else acc32 += qc[p + j] * dc;
+                }
+            }
+            for (; p < d; p++) {
+                const int32_t dc = 2 * (int32_t)row[p] - m2;
+                if (WIDE) acc64 += (int64_t)qc[p] * dc;
+                else acc32 += qc[p] * dc;
+            }
+            const float a = WIDE ? (float)acc64 : (float)acc32;
+            const float sc = __fadd_rn(__fmul_rn(__fmul_rn(a, 0.25f), inv_norm_sq), 0.0f);
+            tk_offer(tk, k, score_key(sc, (uint32_t)di));
+        }
+    }
+    __syncthreads();
+    tk_flush(tk, k);
+    uint64_t* out = partial + (q * nseg + seg) * k;
+    for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) out[j] = tk.keys[j];
+}
+
+// Rank::search_asymmetric: acc = sum q_unit[p] * (f32)r_p in coordinate order,
+// score = (acc - mean * q_sum) * inv_norm (+ 0.0, the TopK offset).
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_rank_asym(const uint16_t* __restrict__ ranks, uint64_t n, uint32_t d,
+                                                   const float* __restrict__ qhat, const float* __restrict__ qsum,
+                                                   float inv_norm, float mean, uint64_t per_seg, uint32_t k,
+                                                   uint64_t* __restrict__ partial) {
+    __shared__ TopKShared tk;
+    extern __shared__ float qsh[];
+    const uint64_t q = blockIdx.y, seg = blockIdx.x, nseg = gridDim.x;
+    for (uint32_t p = threadIdx.x; p < d; p += blockDim.x) qsh[p] = qhat[q * d + p];
+    const float shift = __fmul_rn(mean, qsum[q]);
+    const uint64_t begin = seg * per_seg, end = min(n, begin + per_seg);
+    tk_init(tk, k);
+    for (uint64_t r0 = begin; r0 < end; r0 += blockDim.x) {
+        tk_reserve(tk, k, blockDim.x);
+        const uint64_t di = r0 + threadIdx.x;
+        if (di < end) {
+            const uint16_t* row = ranks + di * d;
+            float acc = 0.0f;
+            uint32_t p = 0;
+            for (; p + 8 <= d; p += 8) {
+                uint32_t r[8];
+                rank_row8<VEC>(row, p, r);
+#pragma unroll
+                for (int j = 0; j < 8; j++) acc = __fadd_rn(acc, __fmul_rn(qsh[p + j], (float)r[j]));
+            }
+            for (; p < d; p++) acc = __fadd_rn(acc, __fmul_rn(qsh[p], (float)row[p]));
+            const float sc = __fadd_rn(__fmul_rn(__fadd_rn(acc, -shift), inv_norm), 0.0f);
+            tk_offer(tk, k, score_key(sc, (uint32_t)di));
+        }
+    }
+    __syncthreads();
+    tk_flush(tk, k);
+    uint64_t* out = partial + (q * nseg + seg) * k;
+    for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) out[j] = tk.keys[j];
+}
+
+// Moves document `from` onto `to` in the interleaved bitmap layout
+// (swap_remove; one thread per word).
+__global__ void k_move_doc_interleaved(uint64_t* __restrict__ words, uint32_t qpv, uint64_t from, uint64_t to) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < qpv) {
+        const uint64_t src = (((from >> 6) * qpv + w) << 6) + (from & 63);
+        const uint64_t dst = (((to >> 6) * qpv + w) << 6) + (to & 63);
+        words[dst] = words[src];
+        words[src] = 0;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_merge_topk(const uint64_t* __restrict__ partial, uint32_t nseg, uint32_t k,
                                                     const uint32_t* __restrict__ qmap, float* __restrict__ out_s,
                                                     int64_t* __restrict__ out_i) {
@@ -1530,7 +1663,19 @@ void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, u
     size_t lds = (size_t)P * 8 + d;
     uint32_t grid = (uint32_t)std::min<uint64_t>(n, 1u << 20);
     ktime_begin(kFamEncode, s);
-    hipLaunchKernelGGL(k_rank_encode, dim3(grid), dim3(256), lds, s, x, n, d, bits, codes);
+    hipLaunchKernelGGL(k_rank_encode<false>, dim3(grid), dim3(256), lds, s, x, n, d, bits, codes, nullptr);
+    check_launch();
+    ktime_end(kFamEncode, s);
+}
+
+void launch_rank_u16(const float* x, uint64_t n, uint32_t d, uint16_t* ranks, hipStream_t s) {
+    if (n == 0) return;
+    uint32_t P = 1;
+    while (P < d) P <<= 1;
+    size_t lds = (size_t)P * 8 + d;
+    uint32_t grid = (uint32_t)std::min<uint64_t>(n, 1u << 20);
+    ktime_begin(kFamEncode, s);
+    hipLaunchKernelGGL(k_rank_encode<true>, dim3(grid), dim3(256), lds, s, x, n, d, 0u, nullptr, ranks);
     check_launch();
     ktime_end(kFamEncode, s);
 }
@@ -2015,6 +2160,70 @@ void launch_fullscan_rq(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t b
     hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, nullptr, out_s, out_i);
     check_launch();
     ktime_end(kFamFullScan, s);
+}
+
+void launch_rank_centres(const uint16_t* ranks, uint64_t count, uint32_t d, uint32_t bits, float* out,
+                         hipStream_t s) {
+    if (count == 0) return;
+    uint32_t grid = (uint32_t)std::min<uint64_t>(4096, (count + 255) / 256);
+    hipLaunchKernelGGL(k_rank_centres, dim3(grid), dim3(256), 0, s, ranks, count, d, bits, out);
+    check_launch();
+}
+
+void launch_rank_search(const uint16_t* ranks, uint64_t n, uint32_t d, bool symmetric, const uint16_t* qranks,
+                        const float* qhat, const float* qsum, uint64_t nq, uint32_t k, void* work, float* out_s,
+                        int64_t* out_i, hipStream_t s) {
+    if (nq == 0 || k == 0) return;
+    const uint32_t nseg = fullscan_nseg(n, nq);
+    const uint64_t per = (n + nseg - 1) / nseg;
+    uint64_t* partial = static_cast<uint64_t*>(work);
+    dim3 grid(nseg, (uint32_t)nq);
+    const bool vec = d % 8 == 0;
+    const double nd = (double)d;
+    const float norm = (float)std::sqrt(nd * (nd * nd - 1.0) / 12.0);  // src/rank.rs:249-252
+    ktime_begin(kFamFullScan, s);
+    if (symmetric) {
+        const float inv_norm_sq = 1.0f / (norm * norm);
+        const bool wide = (double)d * (double)(d - 1) * (double)(d - 1) >= 2147483647.0;
+        const size_t lds = (size_t)d * 4;
+#define OVX_RSYM(V, W) \
+    hipLaunchKernelGGL((k_rank_sym<V, W>), grid, dim3(256), lds, s, ranks, n, d, qranks, inv_norm_sq, per, k, partial)
+        if (vec && !wide) OVX_RSYM(true, false);
+        else if (vec) OVX_RSYM(true, true);
+        else if (!wide) OVX_RSYM(false, false);
+        else OVX_RSYM(false, true);
+#undef OVX_RSYM
+    } else {
+        const float inv_norm = 1.0f / norm;
+        const float mean = ((float)d - 1.0f) / 2.0f;
+        const size_t lds = (size_t)d * 4;
+        if (vec)
+            hipLaunchKernelGGL(k_rank_asym<true>, grid, dim3(256), lds, s, ranks, n, d, qhat, qsum, inv_norm, mean, per,
+                               k, partial);
+        else
+            hipLaunchKernelGGL(k_rank_asym<false>, grid, dim3(256), lds, s, ranks, n, d, qhat, qsum, inv_norm, mean,
+                               per, k, partial);
+    }
+    check_launch();
+    hipLaunchKernelGGL(k_merge_topk, dim3((uint32_t)nq), dim3(256), 0, s, partial, nseg, k, nullptr, out_s, out_i);
+    check_launch();
+    ktime_end(kFamFullScan, s);
+}
+
+void launch_prepare_sum(const float* q, uint64_t nq, uint32_t d, float* qhat, float* qsum, uint32_t* qstat,
+                        hipStream_t s) {
+    if (nq == 0) return;
+    // k_prepare's SIMD-tier offset with centre -1 and scale 1 is the plain
+    // sequential sum of q_hat (from -0.0).
+    ktime_begin(kFamPrepare, s);
+    hipLaunchKernelGGL(k_prepare, dim3((uint32_t)nq), dim3(256), 0, s, q, nq, d, 1, -1.0f, 1.0f, qhat, qsum, qstat);
+    check_launch();
+    ktime_end(kFamPrepare, s);
+}
+
+void launch_move_doc_interleaved(uint64_t* words, uint32_t qpv, uint64_t from, uint64_t to, hipStream_t s) {
+    hipLaunchKernelGGL(k_move_doc_interleaved, dim3((qpv + 63) / 64), dim3(64), 0, s, words, qpv, from, to);
+    check_launch();
 }
 
 void launch_fastscan_prepare(const float* qhat, uint64_t nq, uint32_t d, uint8_t* lut, float* bias_invq,

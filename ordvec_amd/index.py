@@ -97,6 +97,31 @@ def _out(shape, dtype, like_device):
     return np.empty(shape, dtype=dtype)
 
 
+def _swap_remove(fn, h, idx: int, n: int) -> int:
+    """swap_remove: the last row moves into ``idx``; returns its old index."""
+    idx = int(idx)
+    if idx < 0 or idx >= n:
+        raise IndexError(f"index {idx} out of range (index holds {n} vectors)")
+    moved = C.c_uint64()
+    _host_stream()
+    N.check(fn(h, idx, C.byref(moved)))
+    return moved.value
+
+
+def _full_search(fn, h, queries, d: int, n: int, k: int):
+    """Shared nq x min(k, n) search wrapper (host arrays or CUDA tensors)."""
+    q, nq, flag = _float_rows(queries, d, "queries")
+    if k < 0:
+        raise ValueError("k must be non-negative")
+    kk = min(int(k), n)
+    s = _out((nq, kk), np.float32, flag)
+    i = _out((nq, kk), np.int64, flag)
+    if nq and kk:
+        _bind_stream() if flag else _host_stream()
+        N.check(fn(h, _ptr(q), nq, kk, _ptr(s), _ptr(i), flag))
+    return s, i
+
+
 class RankQuant:
     """B-bit bucketed-rank index resident in HBM (src/quant.rs:210-1017)."""
 
@@ -158,6 +183,16 @@ class RankQuant:
         else:
             _host_stream()
         N.check(N.lib().ordvec_rankquant_add(self._h, _ptr(x), n, flag))
+
+    def search(self, queries, k: int):
+        """search, symmetric (src/quant.rs:629-695): the query is rank-bucketed
+        and scored against the codes; (scores f32[nq,k'], ids i64[nq,k'])."""
+        d, _, n = self._info()
+        return _full_search(N.lib().ordvec_rankquant_search, self._h, queries, d, n, k)
+
+    def swap_remove(self, idx: int) -> int:
+        """swap_remove (src/quant.rs:869-881); returns the moved row's old index."""
+        return _swap_remove(N.lib().ordvec_rankquant_swap_remove, self._h, idx, len(self))
 
     def search_asymmetric(self, queries, k: int):
         """search_asymmetric (src/quant.rs:713-847): (scores f32[nq,k'], ids i64[nq,k'])."""
@@ -287,6 +322,96 @@ class RankQuant:
         return cls._wrap(h)
 
 
+class Rank:
+    """Full-precision u16 rank vectors resident in HBM (src/rank.rs:291-584)."""
+
+    def __init__(self, dim: int):
+        h = C.c_void_p()
+        N.check(N.lib().ordvec_rank_new(int(dim), C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def _wrap(cls, h):
+        obj = cls.__new__(cls)
+        obj._h = h
+        return obj
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and N._lib is not None:
+            N.lib().ordvec_rank_free(h)
+            self._h = None
+
+    def _info(self):
+        d, n = C.c_uint64(), C.c_uint64()
+        N.check(N.lib().ordvec_rank_info(self._h, C.byref(d), C.byref(n)))
+        return d.value, n.value
+
+    @property
+    def dim(self) -> int:
+        return self._info()[0]
+
+    def __len__(self) -> int:
+        return self._info()[1]
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    @property
+    def bytes_per_vec(self) -> int:
+        return self.dim * 2
+
+    @property
+    def byte_size(self) -> int:
+        d, n = self._info()
+        return n * d * 2
+
+    def __repr__(self) -> str:
+        d, n = self._info()
+        return f"Rank(dim={d}, n={n})"
+
+    def add(self, vectors) -> None:
+        """add (src/rank.rs:337-356): rank transform on the GPU."""
+        x, n, flag = _float_rows(vectors, self.dim, "vectors")
+        _bind_stream() if flag else _host_stream()
+        N.check(N.lib().ordvec_rank_add(self._h, _ptr(x), n, flag))
+
+    def search(self, queries, k: int):
+        """search (src/rank.rs:364-412): Spearman correlation of the query's
+        ranks with every stored rank vector."""
+        d, n = self._info()
+        return _full_search(N.lib().ordvec_rank_search, self._h, queries, d, n, k)
+
+    def search_asymmetric(self, queries, k: int):
+        """search_asymmetric (src/rank.rs:429-480): L2-normalised float query
+        against the stored ranks."""
+        d, n = self._info()
+        return _full_search(N.lib().ordvec_rank_search_asymmetric, self._h, queries, d, n, k)
+
+    def swap_remove(self, idx: int) -> int:
+        """swap_remove (src/rank.rs:506-517); returns the moved row's old index."""
+        return _swap_remove(N.lib().ordvec_rank_swap_remove, self._h, idx, len(self))
+
+    def ranks(self) -> np.ndarray:
+        """The rank vectors, n x dim u16 (the .ovr payload)."""
+        d, n = self._info()
+        out = np.empty((n, d), dtype=np.uint16)
+        _host_stream()
+        N.check(N.lib().ordvec_rank_ranks(self._h, out.ctypes.data_as(C.c_void_p), out.size, 0))
+        return out
+
+    def write(self, path: str) -> None:
+        _host_stream()
+        N.check(N.lib().ordvec_rank_write(self._h, str(path).encode()))
+
+    @classmethod
+    def load(cls, path: str) -> "Rank":
+        h = C.c_void_p()
+        _host_stream()
+        N.check(N.lib().ordvec_rank_load(str(path).encode(), C.byref(h)))
+        return cls._wrap(h)
+
+
 class SignBitmap:
     """1-bit sign code index resident in HBM (src/sign_bitmap.rs:94-660)."""
 
@@ -373,6 +498,10 @@ class SignBitmap:
             offs = torch.arange(nq + 1, dtype=torch.int64, device=c.device) * m_eff
             return c.reshape(-1), offs
         return c.reshape(-1), np.arange(nq + 1, dtype=np.uint64) * np.uint64(m_eff)
+
+    def swap_remove(self, idx: int) -> int:
+        """swap_remove (src/sign_bitmap.rs:578-590); returns the moved row's old index."""
+        return _swap_remove(N.lib().ordvec_sign_bitmap_swap_remove, self._h, idx, len(self))
 
     def words(self) -> np.ndarray:
         """Bitmaps row-major, n x dim/64 u64 (the .ovsb payload)."""
@@ -546,6 +675,10 @@ class Bitmap:
                                                          ids.ctypes.data_as(C.c_void_p), ids.shape[0],
                                                          out.ctypes.data_as(C.c_void_p)))
         return out
+
+    def swap_remove(self, idx: int) -> int:
+        """swap_remove (src/bitmap.rs:515-527); returns the moved row's old index."""
+        return _swap_remove(N.lib().ordvec_bitmap_swap_remove, self._h, idx, len(self))
 
     def words(self) -> np.ndarray:
         """Bitmaps row-major, n x dim/64 u64 (the .ovbm payload)."""

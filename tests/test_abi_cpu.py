@@ -140,3 +140,40 @@ def test_python_api_validation():
         _float_rows(__import__("numpy").zeros((2, 5), "float32"), 4, "vectors")
     with pytest.raises(ValueError):
         _float_rows(__import__("numpy").zeros((2, 4), "int32"), 4, "vectors")
+
+
+def test_rank_validation_and_forged_ovr_need_no_device(tmp_path):
+    """Rank::new bounds (src/rank.rs:317-324) and the .ovr loader's checks
+    (src/rank_io.rs:580-640) run before any device work."""
+    lib = N.lib()
+    h = C.c_void_p()
+    assert lib.ordvec_rank_new(1, C.byref(h)) == N.BAD_ARGUMENT
+    assert lib.ordvec_rank_new(65536, C.byref(h)) == N.BAD_ARGUMENT
+    assert lib.ordvec_rank_new(64, None) == N.NULL_POINTER
+    assert lib.ordvec_rank_swap_remove(None, 0, None) == N.NULL_POINTER
+    assert lib.ordvec_rankquant_swap_remove(None, 0, None) == N.NULL_POINTER
+    assert lib.ordvec_rank_search(None, None, 1, 1, None, None, 0) == N.NULL_POINTER
+    lib.ordvec_rank_free(None)
+
+    def hdr(dim, n, magic=b"OVR1", version=1):
+        return magic + bytes([version]) + dim.to_bytes(4, "little") + n.to_bytes(4, "little")
+
+    def rows(*rs):
+        return b"".join(int(v).to_bytes(2, "little") for r in rs for v in r)
+
+    cases = [
+        (hdr(4, 1) + rows([0, 1, 2, 4]), "4 >= dim"),
+        (hdr(4, 2) + rows([0, 1, 2, 3], [3, 1, 1, 0]), "row 1 is not a permutation"),
+        (hdr(4, 1) + rows([0, 1, 2, 3])[:-1], "truncated"),
+        (hdr(4, 1) + rows([0, 1, 2, 3]) + b"\x00", "trailing"),
+        (hdr(1, 0), "out of range"),
+        (hdr(4, 1, version=2) + rows([0, 1, 2, 3]), "version"),
+        (hdr(4, 1, magic=b"OVRQ") + rows([0, 1, 2, 3]), "wrong magic"),
+        (b"OVR1\x01\x04\x00", "truncated"),
+    ]
+    for j, (body, msg) in enumerate(cases):
+        p = tmp_path / f"f{j}.ovr"
+        p.write_bytes(body)
+        assert lib.ordvec_rank_load(str(p).encode(), C.byref(h)) == N.CORRUPT_INDEX, j
+        assert msg in N.last_error(), (j, N.last_error())
+    assert lib.ordvec_rank_load(str(tmp_path / "missing.ovr").encode(), C.byref(h)) == N.IO

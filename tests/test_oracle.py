@@ -404,3 +404,110 @@ def test_fs4_top10_overlaps_exact_b4(oracle):
     es, ei = oracle.search_asymmetric(codes, 256, 4, qs, 10)
     overlap = np.mean([len(set(a) & set(b)) for a, b in zip(fi, ei)])
     assert overlap >= 9.0, overlap
+
+
+# --- Rank and symmetric RankQuant search (src/rank.rs:249-252, 364-480,
+# src/quant.rs:629-695; tests/index/rank.rs) --------------------------------
+
+def _uniform(n, d, seed):
+    return np.random.default_rng(seed).uniform(-1.0, 1.0, size=(n, d)).astype(np.float32)
+
+
+def _ranks64(x):
+    """Ranks by stable argsort of argsort (ties by index), float64."""
+    return np.argsort(np.argsort(x, axis=1, kind="stable"), axis=1, kind="stable").astype(np.float64)
+
+
+def test_rank_norm_is_the_centred_permutation_norm(oracle):
+    for d in (2, 3, 64, 100, 768, 1024, 4096, 65535):
+        centred = np.arange(d, dtype=np.float64) - (d - 1) / 2.0
+        assert oracle.rank_norm(d) == np.float32(np.sqrt(np.sum(centred * centred))), d
+
+
+def test_rank_search_is_spearman(oracle):
+    """tests/index/rank.rs:10-48: top-10 of brute-force rank cosine, scores
+    within 1e-4; here the rank cosine is scipy's Spearman correlation."""
+    from scipy.stats import spearmanr
+
+    d, n = 128, 256
+    docs = _uniform(n, d, 1)
+    qs = _uniform(5, d, 99)
+    ranks = oracle.rank_encode(docs)
+    s, i = oracle.rank_search(ranks, qs, 10)
+    for qi in range(len(qs)):
+        rho = np.array([spearmanr(qs[qi], docs[j]).statistic for j in range(n)])
+        order = np.lexsort((np.arange(n), -rho))[:10]
+        assert np.array_equal(i[qi], order)
+        assert np.allclose(s[qi], rho[order], rtol=0, atol=1e-5)
+
+
+def test_rank_search_asymmetric_closed_form(oracle):
+    """tests/index/rank.rs:51-86 (ref_asymmetric): <q/|q|, r - mean> / norm."""
+    d, n = 128, 256
+    docs = _uniform(n, d, 2)
+    qs = _uniform(5, d, 100)
+    ranks = oracle.rank_encode(docs)
+    s, i = oracle.rank_search_asymmetric(ranks, qs, 10)
+    centred = _ranks64(docs) - (d - 1) / 2.0
+    norm = np.sqrt(d * (d * d - 1) / 12.0)
+    for qi in range(len(qs)):
+        q = qs[qi].astype(np.float64)
+        sc = centred @ (q / np.linalg.norm(q)) / norm
+        order = np.lexsort((np.arange(n), -sc))[:10]
+        assert np.array_equal(i[qi], order)
+        assert np.allclose(s[qi], sc[order], rtol=0, atol=1e-5)
+
+
+def test_rank_search_recall_against_fp32_cosine(oracle):
+    """tests/index/rank.rs:89-138: top-10 overlap with fp32 cosine >= 0.7."""
+    d, n = 128, 256
+    docs = _uniform(n, d, 7)
+    qs = _uniform(20, d, 8)
+    _, i = oracle.rank_search(oracle.rank_encode(docs), qs, 10)
+    dn = docs / np.linalg.norm(docs, axis=1, keepdims=True)
+    cos = (qs / np.linalg.norm(qs, axis=1, keepdims=True)) @ dn.T
+    overlap = sum(len(set(i[r]) & set(np.argsort(-cos[r])[:10])) for r in range(20))
+    assert overlap / 200 >= 0.7
+
+
+def test_rank_search_self_reverse_ties_and_zero_query(oracle):
+    d = 96
+    docs = _uniform(40, d, 5)
+    docs[7] = docs[3]                      # identical documents: lower id first
+    ranks = oracle.rank_encode(docs)
+    s, i = oracle.rank_search(ranks, docs[3:4], 40)
+    assert i[0, 0] == 3 and i[0, 1] == 7 and s[0, 0] == s[0, 1]
+    assert abs(float(s[0, 0]) - 1.0) < 1e-6
+    s, i = oracle.rank_search(ranks, -docs[11:12], 40)
+    assert i[0, -1] == 11 and abs(float(s[0, -1]) + 1.0) < 1e-6
+    # a zero query normalises to zeros: every asymmetric score is +0.0
+    s, i = oracle.rank_search_asymmetric(ranks, np.zeros((1, d), np.float32), 5)
+    assert i[0].tolist() == [0, 1, 2, 3, 4] and np.all(s.view(np.uint32) == 0)
+    s, i = oracle.rank_search(ranks, docs[:2], 100)  # k clamps to n
+    assert s.shape == (2, 40)
+
+
+@pytest.mark.parametrize("bits", [1, 2, 4])
+def test_rq_search_symmetric_closed_form(oracle, bits):
+    """Bucket-centre correlation: <centre(bq), centre(bd)> / R^2, R^2 =
+    d (4^b - 1) / 12 (constant composition makes every row's norm R)."""
+    d, n = 128, 300
+    docs = _uniform(n, d, 10 + bits)
+    qs = _uniform(4, d, 20 + bits)
+    nb = 1 << bits
+    codes = oracle.rq_encode(docs, bits)
+    s, i = oracle.rq_search_symmetric(codes, d, bits, qs, 10)
+    cd = np.floor(_ranks64(docs) * nb / d) - (nb - 1) / 2.0
+    cq = np.floor(_ranks64(qs) * nb / d) - (nb - 1) / 2.0
+    r2 = d * (nb * nb - 1) / 12.0
+    assert np.allclose(np.sum(cd * cd, axis=1), r2)
+    sc = cq @ cd.T / r2
+    for qi in range(len(qs)):
+        order = np.lexsort((np.arange(n), -sc[qi]))[:10]
+        assert np.allclose(s[qi], sc[qi][order], rtol=0, atol=1e-5)
+        # ties are common at b=1: the id sets may differ only within equal scores
+        assert np.allclose(sc[qi][i[qi]], sc[qi][order], rtol=0, atol=1e-9)
+        for j in range(9):  # equal scores: ascending ids
+            assert s[qi, j] > s[qi, j + 1] or (s[qi, j] == s[qi, j + 1] and i[qi, j] < i[qi, j + 1])
+    s, i = oracle.rq_search_symmetric(codes, d, bits, docs[5:6], 1)
+    assert i[0, 0] == 5 and abs(float(s[0, 0]) - 1.0) < 1e-6
