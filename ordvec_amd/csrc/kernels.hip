@@ -745,79 +745,101 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
     }
 }
 
-// Full scan on the matrix cores.  Documents expand to {0,1} bytes (d_k) and
-// queries to s_k = +1 / -1 for bit 1 / 0 (sign bitmaps) or to {0,1} (top-bucket
-// bitmaps); a 32 x 32 tile of (document, query) dot products is then
-// v_mfma_i32_32x32x32_i8 over 32-coordinate chunks, exact in i32:
+// Full scan on the matrix cores.  Documents expand to d_k in {0, 1} and
+// queries to s_k = +1 / -1 for bit 1 / 0 (sign bitmaps) or to {0, 1}
+// (top-bucket bitmaps), both as FP4 (e2m1: 0x2 = 1.0, 0xA = -1.0, 0x0 = 0);
+// a 32 x 32 tile of (document, query) dot products is then
+// v_mfma_scale_f32_32x32x64_f8f6f4 (FP4 x FP4, unit scales) over one 64-bit
+// code word per instruction.  Every product is 0 or +-1 and every partial sum
+// an integer below 2^24, so the f32 accumulation is exact:
 //   sign:   sum_k d_k s_k = 2|d & q| - |d|,  so  popcount(d ^ q) = |q| - dot
 //   bitmap: sum_k d_k q_k = |d & q|,         so  n_top - overlap = n_top - dot
-// A (documents): lane l holds row l%32 and coordinates 16*(l/32) .. +15 of the
-// chunk as bytes 0..15; B (queries) uses the same coordinate assignment, so
-// the k order inside the instruction never matters.  D: lane l holds column
-// (query) l%32 and rows 8*(v/4) + 4*(l/32) + v%4, v = 0..15.
-// A workgroup (8 waves) holds NQT query tiles of 32 whose bytes are expanded
-// once into LDS (one 16-byte fragment per lane per chunk); each wave scores
-// 64-document blocks (two row groups), expanding each document's bits once
-// per chunk and reusing them for all NQT tiles.  Code words stream through a
+// A (documents): lane l holds row l%32 and the 32 coordinates of half l/32 of
+// the word as 4 dwords of 8 nibbles, coordinate 4i + j in nibble i of dword j
+// (four shift-and-mask ops, no table); B (queries) uses the same assignment,
+// so the k order inside the instruction never matters.  D: lane l holds
+// column (query) l%32 and rows 8*(v/4) + 4*(l/32) + v%4, v = 0..15.
+// A workgroup (8 waves) holds NQT query tiles of 32 whose nibbles are
+// expanded once into LDS (one 16-byte fragment per lane per word); each wave
+// scores 64-document blocks (two row groups), expanding each document half
+// word once and reusing it for all NQT tiles.  Code dwords stream through a
 // ring of RING words loaded ahead.  Survivors (distance <= T[q], i.e.
 // dot >= base - T[q]) are appended as in k_s1_scan.
-typedef int ovx_v4i __attribute__((ext_vector_type(4)));
-typedef int ovx_v16i __attribute__((ext_vector_type(16)));
+typedef int ovx_v8i __attribute__((ext_vector_type(8)));
+typedef float ovx_v16f __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ uint32_t spread4(uint32_t nib) { return (nib * 0x00204081u) & 0x01010101u; }
-
-// 16 bits -> 16 bytes in {0, 1}, bit p -> byte p.
-__device__ __forceinline__ ovx_v4i expand16(uint32_t b) {
-    ovx_v4i r;
-    r[0] = (int)spread4(b & 15u);
-    r[1] = (int)spread4((b >> 4) & 15u);
-    r[2] = (int)spread4((b >> 8) & 15u);
-    r[3] = (int)spread4((b >> 12) & 15u);
+// 32 bits -> 32 FP4 values {0, 1.0}: nibble i of dword j is bit 4i + j.
+__device__ __forceinline__ ovx_v8i fp4_01(uint32_t x) {
+    ovx_v8i r;
+    r[0] = (int)((x << 1) & 0x22222222u);
+    r[1] = (int)(x & 0x22222222u);
+    r[2] = (int)((x >> 1) & 0x22222222u);
+    r[3] = (int)((x >> 2) & 0x22222222u);
+    r[4] = r[5] = r[6] = r[7] = 0;  // FP4 operands use the first four dwords
     return r;
 }
 
-// 16 bits -> 16 bytes: +1 for a set bit, -1 (0xff) otherwise.  With y the
-// {0,1} bytes, 0xff - 0xfe*y per byte has no cross-byte carry or borrow.
-__device__ __forceinline__ ovx_v4i expand16_pm(uint32_t b) {
-    const ovx_v4i y = expand16(b);
-    ovx_v4i o;
-#pragma unroll
-    for (int i = 0; i < 4; i++) o[i] = (int)~((uint32_t)y[i] * 0xfeu);
+// 32 bits -> 32 FP4 values: +1.0 (0x2) for a set bit, -1.0 (0xA) otherwise.
+__device__ __forceinline__ uint4 fp4_pm(uint32_t x) {
+    const ovx_v8i y = fp4_01(x);
+    uint4 o;
+    o.x = 0x22222222u | ((~(uint32_t)y[0] << 2) & 0x88888888u);
+    o.y = 0x22222222u | ((~(uint32_t)y[1] << 2) & 0x88888888u);
+    o.z = 0x22222222u | ((~(uint32_t)y[2] << 2) & 0x88888888u);
+    o.w = 0x22222222u | ((~(uint32_t)y[3] << 2) & 0x88888888u);
     return o;
 }
 
 constexpr uint32_t kMfmaThreads = 512;
+constexpr int kFp4Fmt = 4;                  // e2m1 operand format
+constexpr int kUnitScale = 0x7f7f7f7f;      // E8M0 127 = 2^0 in every byte
 
-// One code word (64 coordinates = two 32-coordinate chunks) of documents col
-// and col + 32 against every query tile.  Document bits expand to {0,1} bytes
-// through a 256-entry LDS table (byte -> 8 bytes), two reads per 16 bits.
+// One code word (64 coordinates) of documents col and col + 32 (this lane's
+// 32-bit halves x0, x1) against every query tile.
 // FIRST: the accumulators start from zero (inline constant), no clearing pass.
-template <int NQT, int NCHUNK, bool FIRST>
-__device__ __forceinline__ void mfma_word(uint64_t d0, uint64_t d1, int w, uint32_t g, uint32_t lane,
-                                          const uint2* __restrict__ s_exp, const ovx_v4i* __restrict__ sB,
-                                          ovx_v16i (&acc)[2][NQT]) {
+template <int NQT, int QPV, bool FIRST>
+__device__ __forceinline__ void mfma_word(uint32_t x0, uint32_t x1, int w, uint32_t lane,
+                                          const uint4* __restrict__ sB, ovx_v16f (&acc)[2][NQT]) {
+    // All of the word's query fragments are requested before the document
+    // expansion, so the VALU work covers part of the LDS latency.
+    uint4 bq[NQT];
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const int c = 2 * w + h;
-        const uint32_t sh = h * 32 + 16 * g;
-        const uint32_t x0 = (uint32_t)(d0 >> sh), x1 = (uint32_t)(d1 >> sh);
-        const uint2 l0 = s_exp[x0 & 255u], u0 = s_exp[(x0 >> 8) & 255u];
-        const uint2 l1 = s_exp[x1 & 255u], u1 = s_exp[(x1 >> 8) & 255u];
-        const ovx_v4i a0 = {(int)l0.x, (int)l0.y, (int)u0.x, (int)u0.y};
-        const ovx_v4i a1 = {(int)l1.x, (int)l1.y, (int)u1.x, (int)u1.y};
+    for (int t = 0; t < NQT; t++) bq[t] = sB[(t * QPV + w) * 64 + lane];
+    const ovx_v8i a0 = fp4_01(x0), a1 = fp4_01(x1);
 #pragma unroll
-        for (int t = 0; t < NQT; t++) {
-            const ovx_v4i bf = sB[(t * NCHUNK + c) * 64 + lane];
-            if (FIRST && h == 0) {
-                const ovx_v16i z = {};
-                acc[0][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bf, z, 0, 0, 0);
-                acc[1][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bf, z, 0, 0, 0);
-            } else {
-                acc[0][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bf, acc[0][t], 0, 0, 0);
-                acc[1][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bf, acc[1][t], 0, 0, 0);
-            }
+    for (int t = 0; t < NQT; t++) {
+        const ovx_v8i bf = {(int)bq[t].x, (int)bq[t].y, (int)bq[t].z, (int)bq[t].w, 0, 0, 0, 0};
+        if (FIRST) {
+            const ovx_v16f z = {};
+            acc[0][t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a0, bf, z, kFp4Fmt, kFp4Fmt, 0, kUnitScale,
+                                                                        0, kUnitScale);
+            acc[1][t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1, bf, z, kFp4Fmt, kFp4Fmt, 0, kUnitScale,
+                                                                        0, kUnitScale);
+        } else {
+            acc[0][t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a0, bf, acc[0][t], kFp4Fmt, kFp4Fmt, 0,
+                                                                        kUnitScale, 0, kUnitScale);
+            acc[1][t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1, bf, acc[1][t], kFp4Fmt, kFp4Fmt, 0,
+                                                                        kUnitScale, 0, kUnitScale);
         }
     }
+}
+
+// Survivor lists: kListCap entries of (query slot << 56 | hamming << 32 | doc)
+// per wave, drained to the per-query buffers when half full and at the end.
+constexpr uint32_t kListCap = 1024;
+
+__device__ __forceinline__ void s1_flush_list(const uint64_t* list, uint32_t count, uint32_t lane, uint32_t q0,
+                                              uint64_t n, uint32_t cap, uint64_t* __restrict__ keys,
+                                              uint32_t* __restrict__ cnt) {
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = lane; i < count; i += 64) {
+        const uint64_t e = list[i];
+        if ((e & 0xffffffffull) >= n) continue;  // padding row of the last block
+        const uint32_t q = q0 + (uint32_t)(e >> 56);
+        const uint32_t pos = atomicAdd(&cnt[q], 1u);
+        if (pos < cap) keys[(uint64_t)q * cap + pos] = e & ((1ull << 56) - 1);
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 template <int NQT, int OP, int QPV>
@@ -828,12 +850,12 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                                                                   uint64_t* __restrict__ keys,
                                                                   uint32_t* __restrict__ cnt, uint32_t ntop,
                                                                   uint32_t ntiles, uint32_t nsegs) {
-    constexpr uint32_t QW = 32 * NQT, NCHUNK = 2 * QPV, NWAVE = kMfmaThreads / 64;
-    constexpr int RING = QPV < 4 ? QPV : 4;
-    extern __shared__ ovx_v4i sB[];  // [NQT][NCHUNK][64]
+    constexpr uint32_t QW = 32 * NQT, NWAVE = kMfmaThreads / 64;
+    constexpr int RING = QPV % 8 == 0 ? 8 : (QPV % 4 == 0 ? 4 : QPV);  // words in flight
+    extern __shared__ uint4 sB[];  // [NQT][QPV][64]
     __shared__ int s_base[QW];
     __shared__ uint32_t s_thr[QW];
-    __shared__ uint2 s_exp[256];
+    __shared__ uint64_t s_list[NWAVE * (kListCap + 1)];  // per-wave survivor lists (+ a spare slot)
     // Workgroups are dealt to the 8 XCDs round-robin by id; map ids so that
     // every query tile of a document segment lands on the same XCD and the
     // segment's codes are read from HBM once per XCD L2 rather than per tile.
@@ -850,18 +872,22 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
         }
     }
     const uint32_t q0 = tile * QW;
-    for (uint32_t e = threadIdx.x; e < NQT * NCHUNK * 64; e += blockDim.x) {
-        const uint32_t ln = e & 63, c = (e >> 6) % NCHUNK, t = (e >> 6) / NCHUNK;
+    uint32_t wbase = 0;  // entries in this wave's list (wave-uniform)
+    for (uint32_t e = threadIdx.x; e < NQT * QPV * 64; e += blockDim.x) {
+        const uint32_t ln = e & 63, w = (e >> 6) % QPV, t = (e >> 6) / QPV;
         const uint32_t q = q0 + t * 32 + (ln & 31);
         if (q < nq) {
-            const uint32_t bits =
-                (uint32_t)(qbits[(uint64_t)q * QPV + c / 2] >> ((c & 1) * 32 + 16 * (ln >> 5))) & 0xffffu;
-            sB[e] = OP ? expand16(bits) : expand16_pm(bits);
+            const uint32_t bits = (uint32_t)(qbits[(uint64_t)q * QPV + w] >> (32 * (ln >> 5)));
+            if (OP) {
+                const ovx_v8i y = fp4_01(bits);
+                sB[e] = make_uint4((uint32_t)y[0], (uint32_t)y[1], (uint32_t)y[2], (uint32_t)y[3]);
+            } else {
+                sB[e] = fp4_pm(bits);
+            }
         } else {
-            sB[e] = ovx_v4i{0, 0, 0, 0};
+            sB[e] = make_uint4(0, 0, 0, 0);
         }
     }
-    for (uint32_t v = threadIdx.x; v < 256; v += blockDim.x) s_exp[v] = make_uint2(spread4(v & 15u), spread4(v >> 4));
     for (uint32_t j = threadIdx.x; j < QW; j += blockDim.x) {
         const uint32_t q = q0 + j;
         uint32_t pc = 0;
@@ -874,77 +900,110 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t col = lane & 31, g = lane >> 5;
     // pass <=> dot >= base - T; columns past nq never pass.
-    int need[NQT], basev[NQT];
+    float need[NQT];
+    int basev[NQT];
 #pragma unroll
     for (int t = 0; t < NQT; t++) {
         const uint32_t j = t * 32 + col;
         basev[t] = s_base[j];
-        need[t] = q0 + j < nq ? basev[t] - (int)s_thr[j] : 0x7fffffff;
+        need[t] = q0 + j < nq ? (float)(basev[t] - (int)s_thr[j]) : 3.0e38f;
     }
+    uint32_t* const cnt_col = cnt + q0 + col;
     const uint32_t b_begin = seg * blocks_per_seg;
     const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
-    for (uint32_t b = b_begin + wave; b < b_end; b += NWAVE) {
-        ovx_v16i acc[2][NQT];
-        const uint64_t* base = words + (uint64_t)b * QPV * 64 + col;  // documents b*64 + col (+32)
-        uint64_t r0[RING], r1[RING];
+    const uint32_t* words32 = reinterpret_cast<const uint32_t*>(words);
+    // dword g of documents b*64 + col (+32): word w at [w * 128] (+64).  The
+    // ring runs across blocks: the last RING words of a block prefetch the
+    // first RING words of the wave's next block.
+    uint32_t r0[RING], r1[RING];
+    if (b_begin + wave < b_end) {
+        const uint32_t* first = words32 + ((uint64_t)(b_begin + wave) * QPV * 64 + col) * 2 + g;
 #pragma unroll
         for (int i = 0; i < RING; i++) {
-            r0[i] = base[i * 64];
-            r1[i] = base[i * 64 + 32];
+            r0[i] = first[i * 128];
+            r1[i] = first[i * 128 + 64];
         }
+    }
+    for (uint32_t b = b_begin + wave; b < b_end; b += NWAVE) {
+        ovx_v16f acc[2][NQT];
+        // The query fragments do not depend on b; an opaque lane index keeps
+        // the compiler from hoisting all of them out of the loop (they would
+        // not fit in registers).
+        uint32_t slane = lane;
+        asm volatile("" : "+v"(slane));
+        const uint32_t* base = words32 + ((uint64_t)b * QPV * 64 + col) * 2 + g;
+        // the wave's next block, or (after its last block) this block again:
+        // an in-bounds load whose value is never used, which keeps the
+        // unrolled word loop free of branches
+        const uint32_t* next = b + NWAVE < b_end ? base + (uint64_t)NWAVE * QPV * 128 : base;
+#pragma unroll
         for (int w0 = 0; w0 < QPV; w0 += RING) {
 #pragma unroll
             for (int i = 0; i < RING; i++) {
-                const uint64_t d0 = r0[i], d1 = r1[i];
-                const int wn = w0 + i + RING;
+                const int w = w0 + i;
+                const uint32_t x0 = r0[i], x1 = r1[i];
+                const int wn = w + RING;
                 if (wn < QPV) {
-                    r0[i] = base[wn * 64];
-                    r1[i] = base[wn * 64 + 32];
+                    r0[i] = base[wn * 128];
+                    r1[i] = base[wn * 128 + 64];
+                } else {
+                    r0[i] = next[(wn - QPV) * 128];
+                    r1[i] = next[(wn - QPV) * 128 + 64];
                 }
-                if (i == 0 && w0 == 0) mfma_word<NQT, NCHUNK, true>(d0, d1, 0, g, lane, s_exp, sB, acc);
-                else mfma_word<NQT, NCHUNK, false>(d0, d1, w0 + i, g, lane, s_exp, sB, acc);
+                if (w == 0) mfma_word<NQT, QPV, true>(x0, x1, 0, slane, sB, acc);
+                else mfma_word<NQT, QPV, false>(x0, x1, w, slane, sB, acc);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        bool any = false;
+        // Every accumulator is consumed here (an empty asm reading one element
+        // of each), so the compiler cannot sink MFMA chains into the
+        // conditional code below.
+#pragma unroll
+        for (int t = 0; t < NQT; t++) asm volatile("" ::"v"(acc[0][t][0]), "v"(acc[1][t][0]));
+        // Survivors: for each 16-slot row group, 16 compares give the wave
+        // masks (scalar registers); each non-empty mask appends (query, key)
+        // entries to this wave's LDS list at ballot offsets: no atomics and no
+        // vector-to-branch dependency per slot.  Rows past n (last block only)
+        // are dropped when the list is drained; entries beyond the list's
+        // capacity go to a spare slot and force their query onto the exact
+        // path.
+        const uint32_t dbase = (uint32_t)b * 64 + 4 * g;
+        bool lost[NQT];
 #pragma unroll
         for (int t = 0; t < NQT; t++) {
-            int mx = acc[0][t][0];
+            lost[t] = false;
 #pragma unroll
-            for (int G = 0; G < 2; G++)
+            for (int G = 0; G < 2; G++) {
+                uint64_t m[16];
 #pragma unroll
-                for (int v = 0; v < 16; v++) mx = max(mx, acc[G][t][v]);
-            any |= mx >= need[t];
-        }
-        if (__any(any)) {
+                for (int v = 0; v < 16; v++) m[v] = __ballot(acc[G][t][v] >= need[t]);
 #pragma unroll
-            for (int t = 0; t < NQT; t++) {
-                const uint32_t q = q0 + t * 32 + col;
-                uint32_t npass = 0;
-#pragma unroll
-                for (int G = 0; G < 2; G++)
-#pragma unroll
-                    for (int v = 0; v < 16; v++) {
-                        const uint64_t doc = (uint64_t)b * 64 + G * 32 + 8 * (v / 4) + 4 * g + (v % 4);
-                        npass += (acc[G][t][v] >= need[t] && doc < n) ? 1u : 0u;
-                    }
-                if (npass) {
-                    uint32_t pos = atomicAdd(&cnt[q], npass);
-#pragma unroll
-                    for (int G = 0; G < 2; G++)
-#pragma unroll
-                        for (int v = 0; v < 16; v++) {
-                            const uint64_t doc = (uint64_t)b * 64 + G * 32 + 8 * (v / 4) + 4 * g + (v % 4);
-                            if (acc[G][t][v] >= need[t] && doc < n) {
-                                const uint32_t h = (uint32_t)(basev[t] - acc[G][t][v]);
-                                if (pos < cap) keys[(uint64_t)q * cap + pos] = ((uint64_t)h << 32) | doc;
-                                pos++;
-                            }
+                for (int v = 0; v < 16; v++) {
+                    if (__builtin_expect(m[v] != 0, 0)) {  // out of line: the common path falls through
+                        if (acc[G][t][v] >= need[t]) {
+                            const uint32_t idx = wbase + __builtin_amdgcn_mbcnt_hi(
+                                                             (uint32_t)(m[v] >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m[v], 0u));
+                            lost[t] |= idx >= kListCap;
+                            const uint32_t h = (uint32_t)(basev[t] - (int)acc[G][t][v]);
+                            const uint32_t doc = dbase + (G * 32 + 8 * (v / 4) + (v % 4));
+                            s_list[wave * (kListCap + 1) + min(idx, kListCap)] =
+                                ((uint64_t)(t * 32 + col) << 56) | ((uint64_t)h << 32) | doc;
                         }
+                        wbase = min(kListCap, wbase + (uint32_t)__popcll(m[v]));
+                    }
                 }
             }
         }
+#pragma unroll
+        for (int t = 0; t < NQT; t++)
+            if (lost[t]) atomicAdd(cnt_col + t * 32, cap + 1);
+        if (wbase >= kListCap / 2) {
+            s1_flush_list(s_list + wave * (kListCap + 1), wbase, lane, q0, n, cap, keys, cnt);
+            wbase = 0;
+        }
     }
+    s1_flush_list(s_list + wave * (kListCap + 1), wbase, lane, q0, n, cap, keys, cnt);
 }
 
 // Sort each query's survivors by (hamming, id) and keep the first m; flag the
@@ -1936,7 +1995,7 @@ template <int NQT, int OP>
 static void set_mfma_lds() {
 #define OVX_SET(QPV)                                                                          \
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_s1_scan_mfma<NQT, OP, QPV>), \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024)
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024)
     OVX_SET(16);
     OVX_SET(12);
     OVX_SET(8);
@@ -1946,7 +2005,7 @@ static void set_mfma_lds() {
 #undef OVX_SET
 }
 // The matrix-core scan needs at least one full 32-query tile; its query
-// fragments live in LDS (32 * d bytes per 32-query tile, <= 128 KiB).
+// fragments live in LDS (16 * d bytes per 32-query tile, <= 64 KiB).
 static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
     static bool configured = false;
     if (!configured) {
@@ -1973,7 +2032,7 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
         if (nseg >= 8) nseg &= ~7ull;  // XCD-grouped segments (see the kernel)
         const uint32_t bps = (uint32_t)((nblk + nseg - 1) / nseg);
         const uint32_t segs = (nblk + bps - 1) / bps;
-        const size_t lds = (size_t)nqt * qpv * 2 * 64 * 16;
+        const size_t lds = (size_t)nqt * qpv * 64 * 16;
 #define OVX_MFMA(NQT, QPV)                                                                                         \
     hipLaunchKernelGGL((k_s1_scan_mfma<NQT, OP, QPV>), dim3(tiles_m * segs), dim3(kMfmaThreads), lds, s, words,   \
                        nblk, n, qbits, (uint32_t)nq, bps, w.thr, p.cap, w.keys, w.cnt, ntop, tiles_m, segs)
