@@ -1165,15 +1165,51 @@ __device__ __forceinline__ float raw_avx512_quad(const uint8_t* __restrict__ row
     float a[16];
 #pragma unroll
     for (int l = 0; l < 16; l++) a[l] = 0.0f;
-    const uint32_t nchunk = d / 16;
-    for (uint32_t c = j; c < nchunk; c += 4) {
-        uint32_t w[BITS == 2 ? 1 : 2];
+    // Lane j takes chunks j, j + 4, ... (d % 64 == 0, so every lane has d / 64
+    // of them), U at a time: their code words are requested together and q is
+    // read from LDS as float4, so one wait covers U chunks.
+    constexpr int NW = BITS == 2 ? 1 : 2;
+    constexpr uint32_t U = 8;
+    const uint32_t per_lane = d / 64;
+    uint32_t i = 0;
+    for (; i + U <= per_lane; i += U) {
+        uint32_t w[U][NW];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t c = j + 4 * (i + u);
+            if (BITS == 2) {
+                w[u][0] = *reinterpret_cast<const uint32_t*>(row + c * 4);
+            } else {
+                const uint2 x = *reinterpret_cast<const uint2*>(row + c * 8);
+                w[u][0] = x.x;
+                w[u][NW - 1] = x.y;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const float4* q4 = reinterpret_cast<const float4*>(qs + (j + 4 * (i + u)) * 16);
+            float qv[16];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float4 x = q4[v];
+                qv[4 * v] = x.x;
+                qv[4 * v + 1] = x.y;
+                qv[4 * v + 2] = x.z;
+                qv[4 * v + 3] = x.w;
+            }
+#pragma unroll
+            for (int l = 0; l < 16; l++) a[l] = __builtin_fmaf((float)code_of<BITS>(w[u], l), qv[l], a[l]);
+        }
+    }
+    for (; i < per_lane; i++) {
+        const uint32_t c = j + 4 * i;
+        uint32_t w[NW];
         if (BITS == 2) {
             w[0] = *reinterpret_cast<const uint32_t*>(row + c * 4);
         } else {
-            uint2 x = *reinterpret_cast<const uint2*>(row + c * 8);
+            const uint2 x = *reinterpret_cast<const uint2*>(row + c * 8);
             w[0] = x.x;
-            w[BITS == 2 ? 0 : 1] = x.y;
+            w[NW - 1] = x.y;
         }
         const float* qc = qs + c * 16;
 #pragma unroll
