@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--serve-docs", type=int, default=20_000_000)
     ap.add_argument("--rates", type=str, default="1000,10000,50000", help="offered queries/s, comma separated")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=1)
     ap.add_argument("--serve-max-batch", type=int, default=1024)
     ap.add_argument("--multi", action="store_true",
                     help="batched multi-index FastScan b=4 run instead: 64 indexes of 768-d, Poisson(16) queries")
@@ -307,7 +307,9 @@ def serve_main(args, dev):
             rep = srv.open_loop(pool, rate, args.serve_seconds, nq_range=(1, 64), k_range=(10, 100), seed=int(rate))
             rep = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.items()}
             log(f"offered {rate:,.0f} q/s: achieved {rep['achieved_qps']:,.0f} q/s ({rep['achieved_rps']:,.0f} req/s), "
-                f"p50 {rep['p50_ms']:.2f} ms, p99 {rep['p99_ms']:.2f} ms, mean batch {rep['mean_batch_queries']:.1f}")
+                f"p50 {rep['p50_ms']:.2f} ms, p99 {rep['p99_ms']:.2f} ms (queue {rep['p99_queue_ms']:.2f}, service "
+                f"{rep['p99_service_ms']:.2f}), mean batch {rep['mean_batch_queries']:.1f}, exact-path queries "
+                f"{rep['exact_path_queries']}")
             runs.append(rep)
     ten_k = next((r for r in runs if abs(r["offered_qps"] - 10000) < 1), None)
     out = {

@@ -416,7 +416,10 @@ typedef struct {
     double p50_ms, p90_ms, p99_ms, p999_ms, max_ms, mean_ms;  /* request latency, arrival -> result */
     double batches;
     double mean_batch_queries;
-    uint64_t reserved[8];
+    double p99_queue_ms;           /* arrival -> its batch starts */
+    double p99_service_ms;         /* batch start -> results copied back */
+    uint64_t exact_path_queries;   /* queries whose stage 1 was recomputed exactly */
+    uint64_t reserved[5];
 } ordvec_serve_report_t;
 
 ordvec_status_t ordvec_server_start(const ordvec_sign_bitmap_t *sign, const ordvec_rankquant_t *rq, uint64_t m,

@@ -103,7 +103,10 @@ class ServeReport(C.Structure):
         ("mean_ms", C.c_double),
         ("batches", C.c_double),
         ("mean_batch_queries", C.c_double),
-        ("reserved", C.c_uint64 * 8),
+        ("p99_queue_ms", C.c_double),
+        ("p99_service_ms", C.c_double),
+        ("exact_path_queries", C.c_uint64),
+        ("reserved", C.c_uint64 * 5),
     ]
 
 

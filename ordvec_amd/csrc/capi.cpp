@@ -97,6 +97,7 @@ struct DevBuf {
 // (serialised by the engine mutex); serving slots each own one.
 struct Ctx {
     hipStream_t stream = nullptr;
+    uint64_t exact_queries = 0;  // queries whose stage 1 went through the exact recomputation
     DevBuf ws, ws2;  // per-call workspace; ws2 serves the exact stage-1 recomputation
     uint32_t* pinned = nullptr;
     size_t pinned_count = 0;
@@ -877,6 +878,7 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
         fail(kNonfiniteQuery, "queries contain NaN or infinity");
     std::vector<uint32_t> fl = flagged(st, kStatFallback);
     if (!fl.empty()) {
+        cur().exact_queries += fl.size();
         stage1_fix(sb, b.qbits, fl, m_eff, cand, nullptr);
         h2d(qmap, fl.data(), fl.size() * 4);
         launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, fl.size(), nullptr, cand, m_eff, ix->n,
@@ -915,7 +917,7 @@ struct ServeRequest {
     int64_t* out_i = nullptr;
     std::vector<float> own_s;  // buffers owned by the open-loop driver
     std::vector<int64_t> own_i;
-    uint64_t t_arrival_ns = 0, t_done_ns = 0;
+    uint64_t t_arrival_ns = 0, t_start_ns = 0, t_done_ns = 0;
     uint32_t status = 0;
     std::string error;
     bool done = false;
@@ -966,6 +968,7 @@ struct Server {
             }
             uint32_t status = 0;
             std::string err;
+            const uint64_t t_start = now_ns();
             const uint64_t out_k = std::min<uint64_t>(kmax, rq->n);
             try {
                 uint64_t o = 0;
@@ -998,6 +1001,7 @@ struct Server {
                 for (ServeRequest* r : batch) {
                     r->status = status;
                     r->error = err;
+                    r->t_start_ns = t_start;
                     r->t_done_ns = t;
                     r->batch_queries = q;
                     r->done = true;
@@ -3351,6 +3355,8 @@ ordvec_status_t ordvec_server_open_loop(ordvec_server_t* server, const float* qu
         std::exponential_distribution<double> gap(offered_qps / mean_nq);  // request arrivals per second
         std::vector<std::unique_ptr<ServeRequest>> reqs;
         const uint64_t b0 = sv->batches;
+        uint64_t exact0 = 0;
+        for (auto& sl : sv->slots) exact0 += sl->ctx.exact_queries;
         const uint64_t t0 = now_ns() + 2000000;  // start 2 ms from now
         double t = 0.0;
         for (;;) {
@@ -3385,7 +3391,7 @@ ordvec_status_t ordvec_server_open_loop(ordvec_server_t* server, const float* qu
         ordvec_serve_report_t rep;
         std::memset(&rep, 0, sizeof(rep));
         rep.struct_size = sizeof(rep);
-        std::vector<double> lat;
+        std::vector<double> lat, queue, service;
         uint64_t last_done = t0, queries = 0;
         for (auto& r : reqs) {
             if (r->status) {
@@ -3393,6 +3399,8 @@ ordvec_status_t ordvec_server_open_loop(ordvec_server_t* server, const float* qu
                 continue;
             }
             lat.push_back((double)(r->t_done_ns - r->t_arrival_ns) * 1e-6);
+            queue.push_back((double)(r->t_start_ns > r->t_arrival_ns ? r->t_start_ns - r->t_arrival_ns : 0) * 1e-6);
+            service.push_back((double)(r->t_done_ns - r->t_start_ns) * 1e-6);
             last_done = std::max(last_done, r->t_done_ns);
             queries += r->nq;
         }
@@ -3414,8 +3422,16 @@ ordvec_status_t ordvec_server_open_loop(ordvec_server_t* server, const float* qu
             double sum = 0;
             for (double v : lat) sum += v;
             rep.mean_ms = sum / lat.size();
+            std::sort(queue.begin(), queue.end());
+            std::sort(service.begin(), service.end());
+            const size_t i99 = std::min(lat.size() - 1, (size_t)(0.99 * (lat.size() - 1) + 0.5));
+            rep.p99_queue_ms = queue[i99];
+            rep.p99_service_ms = service[i99];
         }
         std::lock_guard<std::mutex> lk(sv->mu);
+        uint64_t exact1 = 0;
+        for (auto& sl : sv->slots) exact1 += sl->ctx.exact_queries;
+        rep.exact_path_queries = exact1 - exact0;
         rep.batches = (double)(sv->batches - b0);
         rep.mean_batch_queries = rep.batches > 0 ? (double)queries / rep.batches : 0;
         *report = rep;

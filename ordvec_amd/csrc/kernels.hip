@@ -2040,8 +2040,11 @@ static void set_mfma_lds() {
     OVX_SET(1);
 #undef OVX_SET
 }
-// The matrix-core scan needs at least one full 32-query tile; its query
-// fragments live in LDS (16 * d bytes per 32-query tile, <= 64 KiB).
+// The matrix-core scan pads a batch to 32-query tiles; from 2 queries up that
+// beats the VALU scan (whose work grows with every query; at 20M x 1024 the
+// padded tile streams the codes at the same speed as a 1-query VALU pass).
+// Query fragments live in LDS (16 * d bytes per 32-query tile, <= 64 KiB).
+constexpr uint64_t kMfmaMinQueries = 2;
 static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
     static bool configured = false;
     if (!configured) {
@@ -2052,7 +2055,7 @@ static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
         (void)hipGetLastError();
         configured = true;
     }
-    return g_mfma_scan && nq >= 32 && (qpv == 16 || qpv == 12 || qpv == 8 || qpv == 4 || qpv == 2 || qpv == 1);
+    return g_mfma_scan && nq >= kMfmaMinQueries && (qpv == 16 || qpv == 12 || qpv == 8 || qpv == 4 || qpv == 2 || qpv == 1);
 }
 
 template <int OP>
