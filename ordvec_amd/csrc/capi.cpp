@@ -1397,30 +1397,53 @@ OvrqParsed parse_ovrq_header(Reader& r, bool check_payload_len) {
     return {bits, dim, n, r.pos};
 }
 
+// Files whose payload reaches this size are validated on the GPU after the
+// upload (the reference's per-row checks, the same first-row message).
+constexpr uint64_t kDeviceCheckBytes = 64ull << 20;
+
+// Device slot for a first-invalid-row search, set to "none".
+unsigned long long* first_bad_slot() {
+    auto* p = static_cast<unsigned long long*>(cur().workspace2(256));
+    OVX_HIP(hipMemsetAsync(p, 0xff, 8, cur().stream));
+    return p;
+}
+
+uint64_t read_first_bad(const unsigned long long* p) {
+    uint64_t v = 0;
+    d2h(&v, p, 8);
+    sync();
+    return v;
+}
+
+// src/rank_io.rs:764-816: the reference's message for a row whose bucket
+// counts are not all dim / 2^bits (empty when the row is valid).
+std::string ovrq_row_error(const uint8_t* rp, uint64_t row, uint32_t dim, uint32_t bits) {
+    const uint32_t bpv = dim * bits / 8, cpb = 8 / bits, nb = 1u << bits, per = dim / nb;
+    uint32_t hist[16] = {0};
+    for (uint32_t g = 0; g < bpv; g++)
+        for (uint32_t slot = 0; slot < cpb; slot++) hist[(rp[g] >> ((cpb - 1 - slot) * bits)) & (nb - 1)]++;
+    for (uint32_t b = 0; b < nb; b++)
+        if (hist[b] != per)
+            return "OVRQ row " + std::to_string(row) + " violates constant composition: bucket " + std::to_string(b) +
+                   " has " + std::to_string(hist[b]) + " codes, expected " + std::to_string(per) + " (= dim / 2^bits)";
+    return "";
+}
+
 RqIndex* load_ovrq(const char* path) {
     Reader r;
     r.buf = read_file(path);
     if (!magic_is(r.buf, "OVRQ", "TVRQ")) fail(kCorruptIndex, "not an OVRQ/TVRQ (RankQuant) file: wrong magic");
     OvrqParsed h = parse_ovrq_header(r, true);
     const uint8_t* pay = r.buf.data() + h.payload_off;
-    const uint32_t bpv = h.dim * h.bits / 8, cpb = 8 / h.bits, nb = 1u << h.bits, per = h.dim / nb;
-    // Per-row constant composition (src/rank_io.rs:764-816), first bad row reported.
-    uint32_t lut[256][16];
-    std::memset(lut, 0, sizeof(lut));
-    for (uint32_t byte = 0; byte < 256; byte++)
-        for (uint32_t slot = 0; slot < cpb; slot++)
-            lut[byte][(byte >> ((cpb - 1 - slot) * h.bits)) & (nb - 1)]++;
-    for (uint64_t row = 0; row < h.n; row++) {
-        uint32_t hist[16] = {0};
-        const uint8_t* rp = pay + row * bpv;
-        for (uint32_t g = 0; g < bpv; g++)
-            for (uint32_t b = 0; b < nb; b++) hist[b] += lut[rp[g]][b];
-        for (uint32_t b = 0; b < nb; b++)
-            if (hist[b] != per)
-                fail(kCorruptIndex, "OVRQ row " + std::to_string(row) + " violates constant composition: bucket " +
-                                        std::to_string(b) + " has " + std::to_string(hist[b]) + " codes, expected " +
-                                        std::to_string(per) + " (= dim / 2^bits)");
-    }
+    const uint32_t bpv = h.dim * h.bits / 8;
+    const bool on_device = (uint64_t)h.n * bpv >= kDeviceCheckBytes;
+    // Per-row constant composition, first bad row reported.
+    if (!on_device)
+        for (uint64_t row = 0; row < h.n; row++) {
+            std::string e = ovrq_row_error(pay + row * bpv, row, h.dim, h.bits);
+            if (!e.empty()) fail(kCorruptIndex, e);
+        }
+    eng().init();
     RqIndex* ix = new RqIndex();
     ix->dim = h.dim;
     ix->bits = h.bits;
@@ -1428,6 +1451,12 @@ RqIndex* load_ovrq(const char* path) {
     try {
         ix->reserve(h.n);
         h2d(ix->codes, pay, h.n * bpv);
+        if (on_device) {
+            unsigned long long* fb = first_bad_slot();
+            launch_check_rq_rows(ix->codes, h.n, h.dim, h.bits, fb, cur().stream);
+            const uint64_t bad = read_first_bad(fb);
+            if (bad < h.n) fail(kCorruptIndex, ovrq_row_error(pay + bad * bpv, bad, h.dim, h.bits));
+        }
         sync();
     } catch (...) {
         delete ix;
@@ -1449,6 +1478,25 @@ void write_ovrq(const RqIndex* ix, const char* path) {
 
 // src/rank_io.rs:574-640: header, exact payload, every row a permutation of
 // [0, dim) (bound check plus a per-row stamp for repeats).
+// src/rank_io.rs:600-640: the reference's message for the first value of a
+// row that is >= dim or repeats (empty when the row is a permutation).
+std::string ovr_row_error(const uint8_t* rp, uint64_t row, uint32_t dim, std::vector<uint32_t>& seen,
+                          uint32_t stamp) {
+    for (uint32_t p = 0; p < dim; p++) {
+        const uint32_t v = (uint32_t)rp[2 * p] | ((uint32_t)rp[2 * p + 1] << 8);
+        if (v >= dim)
+            return "OVR1 rank value " + std::to_string(v) + " >= dim (" + std::to_string(dim) +
+                   "); ranks must be a permutation of [0, dim)";
+        if (seen[v] == stamp)
+            return "OVR1 row " + std::to_string(row) + " is not a permutation of [0, dim): value " + std::to_string(v) +
+                   " repeats";
+        seen[v] = stamp;
+    }
+    return "";
+}
+
+// src/rank_io.rs:574-640: header, exact payload, every row a permutation of
+// [0, dim) (bound check plus a per-row stamp for repeats).
 RkIndex* load_ovr(const char* path) {
     Reader r;
     r.buf = read_file(path);
@@ -1463,27 +1511,25 @@ RkIndex* load_ovr(const char* path) {
     const uint64_t payload = (uint64_t)n * dim * 2;
     r.payload_exact(payload);
     const uint8_t* pay = r.buf.data() + r.pos;
+    const bool on_device = payload >= kDeviceCheckBytes;
     std::vector<uint32_t> seen(dim, 0);
-    for (uint64_t row = 0; row < n; row++) {
-        const uint32_t stamp = (uint32_t)row + 1;
-        for (uint32_t p = 0; p < dim; p++) {
-            const uint64_t o = (row * dim + p) * 2;
-            const uint32_t v = (uint32_t)pay[o] | ((uint32_t)pay[o + 1] << 8);
-            if (v >= dim)
-                fail(kCorruptIndex, "OVR1 rank value " + std::to_string(v) + " >= dim (" + std::to_string(dim) +
-                                        "); ranks must be a permutation of [0, dim)");
-            if (seen[v] == stamp)
-                fail(kCorruptIndex, "OVR1 row " + std::to_string(row) + " is not a permutation of [0, dim): value " +
-                                        std::to_string(v) + " repeats");
-            seen[v] = stamp;
+    if (!on_device)
+        for (uint64_t row = 0; row < n; row++) {
+            std::string e = ovr_row_error(pay + row * dim * 2, row, dim, seen, (uint32_t)row + 1);
+            if (!e.empty()) fail(kCorruptIndex, e);
         }
-    }
     RkIndex* ix = new RkIndex();
     ix->dim = dim;
     try {
         eng().init();
         ix->reserve(n);
         h2d(ix->ranks, pay, payload);  // u16 little-endian on both sides
+        if (on_device) {
+            unsigned long long* fb = first_bad_slot();
+            launch_check_rank_rows(ix->ranks, n, dim, fb, cur().stream);
+            const uint64_t bad = read_first_bad(fb);
+            if (bad < n) fail(kCorruptIndex, ovr_row_error(pay + bad * dim * 2, bad, dim, seen, 1));
+        }
         sync();
     } catch (...) {
         delete ix;
@@ -1523,15 +1569,24 @@ uint64_t swap_remove_rows(void* buf, uint64_t& n, uint64_t row_bytes, uint64_t i
 }
 
 // Row-major host words <-> interleaved device layout.
-void sb_upload_rowmajor(SbIndex* ix, const uint64_t* rows, uint64_t n) {
-    const uint64_t nblk = (n + 63) / 64, bw = ix->block_words();
-    std::vector<uint64_t> inter(nblk * bw, 0);
-    for (uint64_t doc = 0; doc < n; doc++)
-        for (uint32_t w = 0; w < ix->qpv; w++) inter[(doc / 64) * bw + (uint64_t)w * 64 + (doc & 63)] = rows[doc * ix->qpv + w];
+// Row-major host bitmaps -> interleaved device layout: rows are staged in
+// chunks and scattered by a kernel.  ntop > 0 also checks every row's
+// popcount on the device; the lowest bad row is returned (n when none).
+uint64_t sb_upload_rowmajor(SbIndex* ix, const uint64_t* rows, uint64_t n, uint32_t ntop = 0) {
     ix->reserve(n);
-    h2d(ix->words, inter.data(), inter.size() * 8);
-    sync();
+    if (n == 0) return 0;
+    const uint64_t row_bytes = (uint64_t)ix->qpv * 8;
+    const uint64_t chunk = std::max<uint64_t>(1, kChunkBytes / row_bytes);
+    unsigned long long* fb = ntop ? first_bad_slot() : nullptr;
+    uint64_t* stage = static_cast<uint64_t*>(cur().workspace(std::min(n, chunk) * row_bytes));
+    for (uint64_t r0 = 0; r0 < n; r0 += chunk) {
+        const uint64_t cnt = std::min(chunk, n - r0);
+        h2d(stage, rows + r0 * ix->qpv, cnt * row_bytes);
+        launch_interleave_rows(stage, cnt, r0, ix->qpv, ix->words, ntop, fb, cur().stream);
+        sync();  // the staging buffer is reused by the next chunk
+    }
     ix->n = n;
+    return ntop ? std::min<uint64_t>(read_first_bad(fb), n) : n;
 }
 
 std::vector<uint64_t> sb_download_rowmajor(const SbIndex* ix) {
@@ -1602,20 +1657,28 @@ SbIndex* load_ovbm(const char* path) {
     r.payload_exact((uint64_t)n * qpv * 8);
     std::vector<uint64_t> rows((uint64_t)n * qpv);
     std::memcpy(rows.data(), r.buf.data() + r.pos, rows.size() * 8);  // little-endian host
-    for (uint64_t i = 0; i < n; i++) {
+    // src/rank_io.rs:938-951: every row holds exactly n_top bits.
+    auto row_error = [&](uint64_t i) {
         uint32_t pop = 0;
         for (uint32_t w = 0; w < qpv; w++) pop += (uint32_t)__builtin_popcountll(rows[i * qpv + w]);
-        if (pop != n_top)
-            fail(kCorruptIndex, "OVBM row " + std::to_string(i) + " has " + std::to_string(pop) +
-                                    " bits set, expected n_top = " + std::to_string(n_top));
-    }
+        return pop == n_top ? std::string()
+                            : "OVBM row " + std::to_string(i) + " has " + std::to_string(pop) +
+                                  " bits set, expected n_top = " + std::to_string(n_top);
+    };
+    const bool on_device = (uint64_t)n * qpv * 8 >= kDeviceCheckBytes;
+    if (!on_device)
+        for (uint64_t i = 0; i < n; i++) {
+            std::string e = row_error(i);
+            if (!e.empty()) fail(kCorruptIndex, e);
+        }
     eng().init();
     SbIndex* ix = new SbIndex();
     ix->dim = dim;
     ix->qpv = qpv;
     ix->ntop = n_top;
     try {
-        sb_upload_rowmajor(ix, rows.data(), n);
+        const uint64_t bad = sb_upload_rowmajor(ix, rows.data(), n, on_device ? n_top : 0);
+        if (bad < n) fail(kCorruptIndex, row_error(bad));
     } catch (...) {
         delete ix;
         throw;

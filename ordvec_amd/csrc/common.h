@@ -113,6 +113,16 @@ void launch_rank_search(const uint16_t* ranks, uint64_t n, uint32_t d, bool symm
 // q_hat = l2_normalise(q) and q_sum = sum q_hat (sequential, from -0.0).
 void launch_prepare_sum(const float* q, uint64_t nq, uint32_t d, float* qhat, float* qsum, uint32_t* qstat,
                         hipStream_t s);
+// Loaded-file validation on the device: lowest invalid row -> *first_bad
+// (atomicMin; the caller initialises it to ~0).
+void launch_check_rq_rows(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t bits, unsigned long long* first_bad,
+                          hipStream_t s);
+void launch_check_rank_rows(const uint16_t* ranks, uint64_t n, uint32_t d, unsigned long long* first_bad,
+                            hipStream_t s);
+// Row-major bitmaps -> interleaved layout (documents first_doc..); ntop > 0
+// also checks every row's popcount.
+void launch_interleave_rows(const uint64_t* rows, uint64_t cnt, uint64_t first_doc, uint32_t qpv, uint64_t* words,
+                            uint32_t ntop, unsigned long long* first_bad, hipStream_t s);
 // swap_remove on the interleaved bitmap layout: document `from` -> `to`.
 void launch_move_doc_interleaved(uint64_t* words, uint32_t qpv, uint64_t from, uint64_t to, hipStream_t s);
 // Top-bucket bitmaps of documents (interleaved layout), d <= kEncodeMaxDim.

@@ -1470,6 +1470,95 @@ __global__ __launch_bounds__(256) void k_rank_asym(const uint16_t* __restrict__ 
     for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) out[j] = tk.keys[j];
 }
 
+// ---------------------------------------------------------------------------
+// Device-side validation of loaded files (src/rank_io.rs:600-640, 764-816,
+// 938-951): each kernel records the lowest invalid row with atomicMin; the
+// host then rebuilds the reference's message from that one row.
+// ---------------------------------------------------------------------------
+
+// Zero nibbles of x, one bit (bit 3 of the nibble) each.
+__device__ __forceinline__ uint32_t zero_nibbles(uint32_t x) {
+    return ~(((x & 0x77777777u) + 0x77777777u) | x) & 0x88888888u;
+}
+
+// RankQuant rows: every bucket must hold exactly d / 2^bits codes.  One thread
+// per row, the row read as dwords (bpv % 4 == 0 for every valid (d, bits)).
+__global__ void k_check_rq_rows(const uint8_t* __restrict__ codes, uint64_t n, uint32_t d, uint32_t bits,
+                                unsigned long long* __restrict__ first_bad) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t bpv = d * bits / 8, per = d >> bits;
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(codes + r * bpv);
+    bool bad = false;
+    if (bits == 1) {
+        uint32_t ones = 0;
+        for (uint32_t i = 0; i < bpv / 4; i++) ones += __popc(row[i]);
+        bad = ones != per;
+    } else if (bits == 2) {
+        uint32_t c1 = 0, c2 = 0, c3 = 0;
+        for (uint32_t i = 0; i < bpv / 4; i++) {
+            const uint32_t w = row[i], hi = (w >> 1) & 0x55555555u, lo = w & 0x55555555u;
+            c3 += __popc(hi & lo);
+            c2 += __popc(hi & ~lo);
+            c1 += __popc(~hi & lo & 0x55555555u);
+        }
+        bad = c1 != per || c2 != per || c3 != per;  // then bucket 0 holds per as well
+    } else {
+        for (uint32_t c = 0; c < 16 && !bad; c++) {
+            const uint32_t rep = c * 0x11111111u;
+            uint32_t cnt = 0;
+            for (uint32_t i = 0; i < bpv / 4; i++) cnt += __popc(zero_nibbles(row[i] ^ rep));
+            bad = cnt != per;
+        }
+    }
+    if (bad) atomicMin(first_bad, (unsigned long long)r);
+}
+
+// Rank rows: a permutation of [0, d).  One workgroup per row (grid-stride),
+// a d-bit LDS bitmap; a value >= d or a bit already set marks the row.
+__global__ __launch_bounds__(256) void k_check_rank_rows(const uint16_t* __restrict__ ranks, uint64_t n, uint32_t d,
+                                                         unsigned long long* __restrict__ first_bad) {
+    extern __shared__ uint32_t seen[];
+    __shared__ int s_bad;
+    const uint32_t words = (d + 31) / 32;
+    for (uint64_t r = blockIdx.x; r < n; r += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) seen[i] = 0;
+        if (threadIdx.x == 0) s_bad = 0;
+        __syncthreads();
+        bool bad = false;
+        for (uint32_t p = threadIdx.x; p < d; p += blockDim.x) {
+            const uint32_t v = ranks[r * d + p];
+            if (v >= d) {
+                bad = true;
+            } else {
+                const uint32_t bit = 1u << (v & 31);
+                bad |= (atomicOr(&seen[v >> 5], bit) & bit) != 0;
+            }
+        }
+        if (bad) s_bad = 1;
+        __syncthreads();
+        if (threadIdx.x == 0 && s_bad) atomicMin(first_bad, (unsigned long long)r);
+        __syncthreads();
+    }
+}
+
+// Row-major bitmap rows [cnt][qpv] of documents first_doc.. -> interleaved
+// words; with ntop > 0 also records the first row whose popcount != ntop.
+__global__ void k_interleave_rows(const uint64_t* __restrict__ rows, uint64_t cnt, uint64_t first_doc, uint32_t qpv,
+                                  uint64_t* __restrict__ words, uint32_t ntop,
+                                  unsigned long long* __restrict__ first_bad) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const uint64_t doc = first_doc + i;
+    uint32_t pop = 0;
+    for (uint32_t w = 0; w < qpv; w++) {
+        const uint64_t v = rows[i * qpv + w];
+        pop += (uint32_t)__popcll(v);
+        words[(((doc >> 6) * qpv + w) << 6) + (doc & 63)] = v;
+    }
+    if (ntop && pop != ntop) atomicMin(first_bad, (unsigned long long)doc);
+}
+
 // Moves document `from` onto `to` in the interleaved bitmap layout
 // (swap_remove; one thread per word).
 __global__ void k_move_doc_interleaved(uint64_t* __restrict__ words, uint32_t qpv, uint64_t from, uint64_t to) {
@@ -2317,6 +2406,31 @@ void launch_prepare_sum(const float* q, uint64_t nq, uint32_t d, float* qhat, fl
     hipLaunchKernelGGL(k_prepare, dim3((uint32_t)nq), dim3(256), 0, s, q, nq, d, 1, -1.0f, 1.0f, qhat, qsum, qstat);
     check_launch();
     ktime_end(kFamPrepare, s);
+}
+
+void launch_check_rq_rows(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t bits, unsigned long long* first_bad,
+                          hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_check_rq_rows, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, codes, n, d, bits,
+                       first_bad);
+    check_launch();
+}
+
+void launch_check_rank_rows(const uint16_t* ranks, uint64_t n, uint32_t d, unsigned long long* first_bad,
+                            hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(n, 1u << 16);
+    hipLaunchKernelGGL(k_check_rank_rows, dim3(grid), dim3(256), (size_t)((d + 31) / 32) * 4, s, ranks, n, d,
+                       first_bad);
+    check_launch();
+}
+
+void launch_interleave_rows(const uint64_t* rows, uint64_t cnt, uint64_t first_doc, uint32_t qpv, uint64_t* words,
+                            uint32_t ntop, unsigned long long* first_bad, hipStream_t s) {
+    if (cnt == 0) return;
+    hipLaunchKernelGGL(k_interleave_rows, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, s, rows, cnt, first_doc,
+                       qpv, words, ntop, first_bad);
+    check_launch();
 }
 
 void launch_move_doc_interleaved(uint64_t* words, uint32_t qpv, uint64_t from, uint64_t to, hipStream_t s) {
