@@ -1006,6 +1006,124 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     s1_flush_list(s_list + wave * (kListCap + 1), wbase, lane, q0, n, cap, keys, cnt);
 }
 
+// Sampled Hamming histogram on the matrix cores (the k_s1_sample contract):
+// 32 queries per workgroup, their FP4 fragments in LDS as in k_s1_scan_mfma;
+// each wave scores sampled 64-document blocks and adds every distance to the
+// workgroup's packed 16-bit LDS histogram.  Only bins up to the workgroup's own
+// threshold (smallest h whose local count reaches s_target) are added to the
+// global histogram: the global threshold can only be lower, so the bins the
+// threshold pass reads are exact.
+template <int OP, int QPV>
+__global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
+    const uint64_t* __restrict__ words, uint32_t nblk, uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
+    uint32_t stride, uint32_t nsample, uint32_t blocks_per_seg, uint32_t* __restrict__ hist, uint32_t nbins,
+    uint32_t hwords, uint32_t s_target, uint32_t ntop) {
+    constexpr uint32_t NWAVE = kMfmaThreads / 64;
+    extern __shared__ uint4 sB[];  // [QPV][64] query fragments, then [32][hwords] packed bins
+    uint32_t* lds_hist = reinterpret_cast<uint32_t*>(sB + QPV * 64);
+    __shared__ int s_base[32];
+    const uint32_t q0 = blockIdx.x * 32;
+    for (uint32_t e = threadIdx.x; e < QPV * 64; e += blockDim.x) {
+        const uint32_t ln = e & 63, w = e >> 6, q = q0 + (ln & 31);
+        if (q < nq) {
+            const uint32_t bits = (uint32_t)(qbits[(uint64_t)q * QPV + w] >> (32 * (ln >> 5)));
+            if (OP) {
+                const ovx_v8i y = fp4_01(bits);
+                sB[e] = make_uint4((uint32_t)y[0], (uint32_t)y[1], (uint32_t)y[2], (uint32_t)y[3]);
+            } else {
+                sB[e] = fp4_pm(bits);
+            }
+        } else {
+            sB[e] = make_uint4(0, 0, 0, 0);
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < 32 * hwords; i += blockDim.x) lds_hist[i] = 0;
+    for (uint32_t j = threadIdx.x; j < 32; j += blockDim.x) {
+        const uint32_t q = q0 + j;
+        uint32_t pc = 0;
+        if (q < nq && !OP)
+            for (uint32_t w = 0; w < QPV; w++) pc += (uint32_t)__popcll(qbits[(uint64_t)q * QPV + w]);
+        s_base[j] = (int)(OP ? ntop : pc);
+    }
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t col = lane & 31, g = lane >> 5;
+    const int basev = s_base[col];
+    const bool qvalid = q0 + col < nq;
+    const uint32_t* words32 = reinterpret_cast<const uint32_t*>(words);
+    const uint32_t j0 = blockIdx.y * blocks_per_seg, j1 = min(nsample, j0 + blocks_per_seg);
+    for (uint32_t j = j0 + wave; j < j1; j += NWAVE) {
+        const uint32_t b = j * stride;
+        if (b >= nblk) break;
+        uint32_t slane = lane;
+        asm volatile("" : "+v"(slane));
+        ovx_v16f acc[2][1];
+        const uint32_t* base = words32 + ((uint64_t)b * QPV * 64 + col) * 2 + g;
+        uint32_t x0[QPV], x1[QPV];
+#pragma unroll
+        for (int w = 0; w < QPV; w++) {
+            x0[w] = base[w * 128];
+            x1[w] = base[w * 128 + 64];
+        }
+#pragma unroll
+        for (int w = 0; w < QPV; w++) {
+            if (w == 0) mfma_word<1, QPV, true>(x0[w], x1[w], 0, slane, sB, acc);
+            else mfma_word<1, QPV, false>(x0[w], x1[w], w, slane, sB, acc);
+        }
+        if (qvalid) {
+            const uint64_t doc0 = (uint64_t)b * 64 + 4 * g;
+#pragma unroll
+            for (int G = 0; G < 2; G++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const uint32_t row = G * 32 + 8 * (v / 4) + (v % 4);
+                    if (doc0 + row < n) {
+                        const uint32_t h = (uint32_t)(basev - (int)acc[G][0][v]);
+                        atomicAdd(&lds_hist[col * hwords + (h >> 1)], 1u << ((h & 1) * 16));
+                    }
+                }
+        }
+    }
+    __syncthreads();
+    // One wave per query: local threshold by a wave prefix over bin chunks,
+    // then the bins up to it go to the global histogram.
+    const uint32_t per = (nbins + 63) / 64;
+    for (uint32_t qi = wave; qi < 32 && q0 + qi < nq; qi += NWAVE) {
+        const uint32_t* hq = lds_hist + qi * hwords;
+        const uint32_t lo = min(nbins, lane * per), hi = min(nbins, lo + per);
+        uint32_t local = 0;
+        for (uint32_t bin = lo; bin < hi; bin++) local += (hq[bin >> 1] >> ((bin & 1) * 16)) & 0xffffu;
+        uint32_t incl = local;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off);
+            if ((int)lane >= off) incl += v;
+        }
+        // first lane whose inclusive count reaches s_target (64 = never)
+        const uint64_t reach = __ballot(incl >= s_target);
+        uint32_t last = nbins - 1;
+        if (reach) {
+            const uint32_t l0 = (uint32_t)__builtin_ctzll(reach);
+            uint32_t cut = 0;
+            if (lane == l0) {
+                uint32_t cum = incl - local;
+                cut = hi - 1;
+                for (uint32_t bin = lo; bin < hi; bin++) {
+                    cum += (hq[bin >> 1] >> ((bin & 1) * 16)) & 0xffffu;
+                    if (cum >= s_target) {
+                        cut = bin;
+                        break;
+                    }
+                }
+            }
+            last = __shfl(cut, (int)l0);
+        }
+        for (uint32_t bin = lane; bin <= last; bin += 64) {
+            const uint32_t c = (hq[bin >> 1] >> ((bin & 1) * 16)) & 0xffffu;
+            if (c) atomicAdd(&hist[(uint64_t)(q0 + qi) * nbins + bin], c);
+        }
+    }
+}
+
 // Sort each query's survivors by (hamming, id) and keep the first m; flag the
 // query (qstat bit 1) when the buffer cannot prove exactness.  A histogram of
 // min(hamming, kSelBins-1) first narrows the sort to the keys at or below the
@@ -2105,9 +2223,36 @@ static void launch_sample_wch(const uint64_t* words, uint64_t n, uint32_t qpv, c
     else launch_sample_qt<QT, 1, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
 }
 
+static bool use_mfma_scan(uint64_t nq, uint32_t qpv);
+
 template <int OP>
 static void launch_sample(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
                           const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
+    if (use_mfma_scan(nq, qpv) && qpv <= 16 && p.s_target != 0xffffffffu) {
+        // matrix-core sample: 32 queries per workgroup, >= 8 sampled blocks
+        // (one per wave) per workgroup, about 512 workgroups
+        const uint32_t nblk = (uint32_t)((n + 63) / 64);
+        const uint32_t tiles = (uint32_t)((nq + 31) / 32);
+        const uint64_t want = std::max<uint64_t>(1, (512 + tiles - 1) / tiles);
+        const uint64_t nseg = std::max<uint64_t>(1, std::min<uint64_t>(want, (p.nsample + 7) / 8));
+        const uint32_t bps = (uint32_t)std::min<uint64_t>(1023, (p.nsample + nseg - 1) / nseg);
+        const uint32_t segs = (p.nsample + bps - 1) / bps;
+        const size_t lds = (size_t)qpv * 64 * 16 + (size_t)32 * p.hwords * 4;
+#define OVX_SMPL(QPV)                                                                                            \
+    hipLaunchKernelGGL((k_s1_sample_mfma<OP, QPV>), dim3(tiles, segs), dim3(kMfmaThreads), lds, s, words, nblk, n, \
+                       qbits, (uint32_t)nq, p.stride, p.nsample, bps, w.hist, p.nbins, p.hwords, p.s_target, ntop)
+        switch (qpv) {
+            case 16: OVX_SMPL(16); break;
+            case 12: OVX_SMPL(12); break;
+            case 8: OVX_SMPL(8); break;
+            case 4: OVX_SMPL(4); break;
+            case 2: OVX_SMPL(2); break;
+            default: OVX_SMPL(1); break;
+        }
+#undef OVX_SMPL
+        check_launch();
+        return;
+    }
     if (p.qt == 16) launch_sample_wch<16, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
     else if (p.qt == 4) launch_sample_wch<4, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
     else launch_sample_wch<1, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
@@ -2129,6 +2274,20 @@ static void set_mfma_lds() {
     OVX_SET(1);
 #undef OVX_SET
 }
+template <int OP>
+static void set_sample_lds() {
+#define OVX_SET(QPV)                                                                             \
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_s1_sample_mfma<OP, QPV>),        \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024)
+    OVX_SET(16);
+    OVX_SET(12);
+    OVX_SET(8);
+    OVX_SET(4);
+    OVX_SET(2);
+    OVX_SET(1);
+#undef OVX_SET
+}
+
 // The matrix-core scan pads a batch to 32-query tiles; from 2 queries up that
 // beats the VALU scan (whose work grows with every query; at 20M x 1024 the
 // padded tile streams the codes at the same speed as a 1-query VALU pass).
@@ -2139,6 +2298,8 @@ static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
     if (!configured) {
         set_mfma_lds<4, 0>();
         set_mfma_lds<4, 1>();
+        set_sample_lds<0>();
+        set_sample_lds<1>();
         set_mfma_lds<2, 0>();
         set_mfma_lds<2, 1>();
         (void)hipGetLastError();
