@@ -2224,16 +2224,19 @@ static void launch_sample_wch(const uint64_t* words, uint64_t n, uint32_t qpv, c
 }
 
 static bool use_mfma_scan(uint64_t nq, uint32_t qpv);
+// about one sample workgroup per CU: fewer leave CUs idle, more flush more
+// (each workgroup's local threshold sits higher) -- 256 measured best at 1M x 1024
+constexpr uint64_t kSampleWgs = 256;
 
 template <int OP>
 static void launch_sample(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
                           const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
     if (use_mfma_scan(nq, qpv) && qpv <= 16 && p.s_target != 0xffffffffu) {
         // matrix-core sample: 32 queries per workgroup, >= 8 sampled blocks
-        // (one per wave) per workgroup, about 512 workgroups
+        // (one per wave) per workgroup, about kSampleWgs workgroups
         const uint32_t nblk = (uint32_t)((n + 63) / 64);
         const uint32_t tiles = (uint32_t)((nq + 31) / 32);
-        const uint64_t want = std::max<uint64_t>(1, (512 + tiles - 1) / tiles);
+        const uint64_t want = std::max<uint64_t>(1, (kSampleWgs + tiles - 1) / tiles);
         const uint64_t nseg = std::max<uint64_t>(1, std::min<uint64_t>(want, (p.nsample + 7) / 8));
         const uint32_t bps = (uint32_t)std::min<uint64_t>(1023, (p.nsample + nseg - 1) / nseg);
         const uint32_t segs = (p.nsample + bps - 1) / bps;
