@@ -1133,10 +1133,13 @@ constexpr uint32_t kSelBins = 2048, kSelBuf = 4096;
 __global__ __launch_bounds__(1024) void k_s1_select(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ cnt,
                                                     uint32_t cap, uint64_t m_eff, uint32_t* __restrict__ cand,
                                                     uint32_t* __restrict__ cand_h, uint32_t* __restrict__ qstat) {
-    __shared__ uint32_t hist[kSelBins];
-    __shared__ uint64_t sk[kSelBuf];
-    __shared__ uint32_t part[1024];
-    __shared__ uint32_t s_cut, s_n;
+    constexpr uint32_t KPT = kFastCap / 1024;  // keys per thread (buffer capacity / threads)
+    __shared__ uint32_t hist[kSelBins];        // keys per bin
+    __shared__ uint32_t start[kSelBins];       // first slot of each bin in sk
+    __shared__ uint32_t fill[kSelBins];
+    __shared__ uint64_t sk[kSelBuf];           // selected keys grouped by bin
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t s_cut, s_big;
     const uint64_t q = blockIdx.x;
     const uint32_t count = cnt[q];
     if (count < m_eff || count > cap) {
@@ -1144,40 +1147,80 @@ __global__ __launch_bounds__(1024) void k_s1_select(const uint64_t* __restrict__
         return;
     }
     const uint64_t* kq = keys + q * cap;
-    for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x) hist[i] = 0;
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x)
-        atomicAdd(&hist[min((uint32_t)(kq[i] >> 32), kSelBins - 1)], 1u);
-    __syncthreads();
-    // cut = smallest bin with cumulative count >= m (2 bins per thread).
-    const uint32_t lo = threadIdx.x * 2;
-    const uint32_t local = hist[lo] + hist[lo + 1];
-    part[threadIdx.x] = local;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+    // One read of the survivors into registers.
+    uint64_t kr[KPT];
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++) {
+        const uint32_t i = threadIdx.x + j * 1024;
+        kr[j] = i < count ? kq[i] : kKeyMax;
     }
-    const uint32_t incl = part[threadIdx.x], excl = incl - local;
-    if (excl < m_eff && incl >= m_eff) s_cut = (excl + hist[lo] >= m_eff) ? lo : lo + 1;
+    for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x) {
+        hist[i] = 0;
+        fill[i] = 0;
+    }
+    if (threadIdx.x == 0) s_big = 0;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++)
+        if (kr[j] != kKeyMax) atomicAdd(&hist[min((uint32_t)(kr[j] >> 32), kSelBins - 1)], 1u);
+    __syncthreads();
+    // Exclusive bin starts (2 bins per thread): wave scans, then the 16 wave totals.
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lo = threadIdx.x * 2;
+    const uint32_t h0 = hist[lo], h1 = hist[lo + 1], local = h0 + h1;
+    uint32_t incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off);
+        if ((int)lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t w = threadIdx.x < 16 ? wsum[threadIdx.x] : 0u;
+        uint32_t wi = w;
+        for (int off = 1; off < 16; off <<= 1) {
+            const uint32_t v = __shfl_up(wi, off);
+            if ((int)threadIdx.x >= off) wi += v;
+        }
+        if (threadIdx.x < 16) wsum[threadIdx.x] = wi - w;  // exclusive wave offsets
+    }
+    __syncthreads();
+    const uint32_t excl = wsum[wave] + incl - local;
+    start[lo] = excl;
+    start[lo + 1] = excl + h0;
+    // cut = smallest bin whose cumulative count reaches m
+    if (excl < m_eff && excl + local >= m_eff) s_cut = (excl + h0 >= m_eff) ? lo : lo + 1;
     __syncthreads();
     const uint32_t cut = s_cut;
-    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
-        uint64_t k = kq[i];
-        if (min((uint32_t)(k >> 32), kSelBins - 1) <= cut) {
-            uint32_t pos = atomicAdd(&s_n, 1u);
-            if (pos < kSelBuf) sk[pos] = k;
-        }
-    }
-    __syncthreads();
-    const uint32_t nsel = s_n;
+    const uint32_t nsel = start[cut] + hist[cut];
     if (nsel > kSelBuf) {
         if (threadIdx.x == 0) qstat[q] |= 2u;
         return;
     }
+    if (lo <= cut) atomicMax(&s_big, max(h0, lo + 1 <= cut ? h1 : 0u));
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++) {
+        const uint32_t bin = min((uint32_t)(kr[j] >> 32), kSelBins - 1);
+        if (kr[j] != kKeyMax && bin <= cut) sk[start[bin] + atomicAdd(&fill[bin], 1u)] = kr[j];
+    }
+    __syncthreads();
+    if (s_big <= 512) {
+        // Position of a key = its bin's start + the keys of its bin that order
+        // before it ((hamming, id) order; one bin holds one hamming value
+        // below the clamp and ids are unique).
+        for (uint32_t i = threadIdx.x; i < nsel; i += blockDim.x) {
+            const uint64_t k = sk[i];
+            const uint32_t bin = min((uint32_t)(k >> 32), kSelBins - 1), b0 = start[bin], sz = hist[bin];
+            uint32_t rank = 0;
+            for (uint32_t t = 0; t < sz; t++) rank += sk[b0 + t] < k ? 1u : 0u;
+            const uint32_t pos = b0 + rank;
+            if (pos < m_eff) {
+                cand[q * m_eff + pos] = (uint32_t)(k & 0xffffffffu);
+                if (cand_h) cand_h[q * m_eff + pos] = (uint32_t)(k >> 32);
+            }
+        }
+        return;
+    }
+    // Large tie groups: sort the selection.
     const uint32_t P = next_pow2(nsel);
     for (uint32_t i = nsel + threadIdx.x; i < P; i += blockDim.x) sk[i] = kKeyMax;
     __syncthreads();
