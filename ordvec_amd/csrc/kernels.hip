@@ -825,7 +825,8 @@ __device__ __forceinline__ void mfma_word(uint32_t x0, uint32_t x1, int w, uint3
 }
 
 // Survivor lists: kListCap entries of (query slot << 56 | hamming << 32 | doc)
-// per wave, drained to the per-query buffers when half full and at the end.
+// per wave (plus one spare slot per lane), drained to the per-query buffers
+// when half full and at the end.
 constexpr uint32_t kListCap = 1024;
 
 __device__ __forceinline__ void s1_flush_list(const uint64_t* list, uint32_t count, uint32_t lane, uint32_t q0,
@@ -855,7 +856,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     extern __shared__ uint4 sB[];  // [NQT][QPV][64]
     __shared__ int s_base[QW];
     __shared__ uint32_t s_thr[QW];
-    __shared__ uint64_t s_list[NWAVE * (kListCap + 1)];  // per-wave survivor lists (+ a spare slot)
+    __shared__ uint64_t s_list[NWAVE * (kListCap + 64)];  // per-wave survivor lists (+ a spare slot per lane)
     // Workgroups are dealt to the 8 XCDs round-robin by id; map ids so that
     // every query tile of a document segment lands on the same XCD and the
     // segment's codes are read from HBM once per XCD L2 rather than per tile.
@@ -962,16 +963,15 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
         for (int t = 0; t < NQT; t++) asm volatile("" ::"v"(acc[0][t][0]), "v"(acc[1][t][0]));
         // Survivors: for each 16-slot row group, 16 compares give the wave
         // masks (scalar registers); each non-empty mask appends (query, key)
-        // entries to this wave's LDS list at ballot offsets: no atomics and no
-        // vector-to-branch dependency per slot.  Rows past n (last block only)
-        // are dropped when the list is drained; entries beyond the list's
-        // capacity go to a spare slot and force their query onto the exact
-        // path.
+        // entries to this wave's LDS list at ballot offsets.  The append has no
+        // branch or exec change inside: lanes that do not pass, and entries
+        // past the list's capacity, write a spare slot of their own.  Rows past
+        // n (last block only) are dropped when the list is drained.
         const uint32_t dbase = (uint32_t)b * 64 + 4 * g;
-        bool lost[NQT];
+        uint64_t* const wlist = s_list + wave * (kListCap + 64);
 #pragma unroll
         for (int t = 0; t < NQT; t++) {
-            lost[t] = false;
+            const uint32_t hbase = ((uint32_t)(t * 32 + col) << 24) | (uint32_t)basev[t];
 #pragma unroll
             for (int G = 0; G < 2; G++) {
                 uint64_t m[16];
@@ -980,30 +980,29 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
 #pragma unroll
                 for (int v = 0; v < 16; v++) {
                     if (__builtin_expect(m[v] != 0, 0)) {  // out of line: the common path falls through
-                        if (acc[G][t][v] >= need[t]) {
-                            const uint32_t idx = wbase + __builtin_amdgcn_mbcnt_hi(
-                                                             (uint32_t)(m[v] >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m[v], 0u));
-                            lost[t] |= idx >= kListCap;
-                            const uint32_t h = (uint32_t)(basev[t] - (int)acc[G][t][v]);
-                            const uint32_t doc = dbase + (G * 32 + 8 * (v / 4) + (v % 4));
-                            s_list[wave * (kListCap + 1) + min(idx, kListCap)] =
-                                ((uint64_t)(t * 32 + col) << 56) | ((uint64_t)h << 32) | doc;
-                        }
-                        wbase = min(kListCap, wbase + (uint32_t)__popcll(m[v]));
+                        const uint32_t off =
+                            __builtin_amdgcn_mbcnt_hi((uint32_t)(m[v] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[v], 0u));
+                        const bool p = acc[G][t][v] >= need[t];
+                        const uint32_t slot = min(p ? wbase + off : 0xffffffffu, kListCap + lane);
+                        // (query slot << 24 | base) - dot = query slot << 24 | hamming
+                        const uint32_t hi = hbase - (uint32_t)(int)acc[G][t][v];
+                        wlist[slot] = ((uint64_t)hi << 32) | (dbase + (G * 32 + 8 * (v / 4) + (v % 4)));
+                        wbase += (uint32_t)__popcll(m[v]);
                     }
                 }
             }
         }
+        if (wbase > kListCap) {  // entries were dropped: send every query of the tile to the exact path
 #pragma unroll
-        for (int t = 0; t < NQT; t++)
-            if (lost[t]) atomicAdd(cnt_col + t * 32, cap + 1);
+            for (int t = 0; t < NQT; t++) atomicAdd(cnt_col + t * 32, cap + 1);
+            wbase = kListCap;
+        }
         if (wbase >= kListCap / 2) {
-            s1_flush_list(s_list + wave * (kListCap + 1), wbase, lane, q0, n, cap, keys, cnt);
+            s1_flush_list(wlist, wbase, lane, q0, n, cap, keys, cnt);
             wbase = 0;
         }
     }
-    s1_flush_list(s_list + wave * (kListCap + 1), wbase, lane, q0, n, cap, keys, cnt);
+    s1_flush_list(s_list + wave * (kListCap + 64), wbase, lane, q0, n, cap, keys, cnt);
 }
 
 // Sampled Hamming histogram on the matrix cores (the k_s1_sample contract):
