@@ -1332,6 +1332,36 @@ __device__ __forceinline__ float raw_avx512_quad(const uint8_t* __restrict__ row
     constexpr uint32_t U = 8;
     const uint32_t per_lane = d / 64;
     uint32_t i = 0;
+    if (per_lane == 16) {  // d = 1024: the whole row share in one batch of loads
+        uint32_t w[16][NW];
+#pragma unroll
+        for (uint32_t u = 0; u < 16; u++) {
+            const uint32_t c = j + 4 * u;
+            if (BITS == 2) {
+                w[u][0] = *reinterpret_cast<const uint32_t*>(row + c * 4);
+            } else {
+                const uint2 x = *reinterpret_cast<const uint2*>(row + c * 8);
+                w[u][0] = x.x;
+                w[u][NW - 1] = x.y;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 16; u++) {
+            const float4* q4 = reinterpret_cast<const float4*>(qs + (j + 4 * u) * 16);
+            float qv[16];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float4 x = q4[v];
+                qv[4 * v] = x.x;
+                qv[4 * v + 1] = x.y;
+                qv[4 * v + 2] = x.z;
+                qv[4 * v + 3] = x.w;
+            }
+#pragma unroll
+            for (int l = 0; l < 16; l++) a[l] = __builtin_fmaf((float)code_of<BITS>(w[u], l), qv[l], a[l]);
+        }
+        i = per_lane;
+    }
     for (; i + U <= per_lane; i += U) {
         uint32_t w[U][NW];
 #pragma unroll
@@ -1455,10 +1485,12 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
     tk_init(tk, out_k);
     if (TIER == 2) {
         const uint32_t per_round = blockDim.x / 4, quad = threadIdx.x >> 2, jl = threadIdx.x & 3;
+        uint32_t id_next = sb + quad < se ? cands[sb + quad] : 0u;  // candidate ids run one round ahead
         for (uint64_t r0 = sb; r0 < se; r0 += per_round) {
             tk_reserve(tk, out_k, per_round);
             const uint64_t j = r0 + quad;
-            const uint32_t id = j < se ? cands[j] : 0u;
+            const uint32_t id = id_next;
+            id_next = j + per_round < se ? cands[j + per_round] : 0u;
             const bool ok = j < se && id < n_docs;
             const float raw = raw_avx512_quad<BITS>(codes + (uint64_t)(ok ? id : 0u) * bpv, d, qs, jl);
             if (ok && jl == 0) tk_offer(tk, out_k, score_key(__fadd_rn(__fmul_rn(raw, inv_norm), off), id));
