@@ -183,6 +183,11 @@ typedef struct ordvec_fastscan_t ordvec_fastscan_t;
 ordvec_status_t ordvec_set_stream(void *hip_stream);
 /* Wait for all work the library enqueued. */
 ordvec_status_t ordvec_synchronize(void);
+/* Queries whose stage-1 candidates were recomputed by the exact fallback
+ * (survivor buffer underfull or overflowed) since the library was loaded, on
+ * the engine's default context: a monitoring counter (no reference
+ * counterpart). */
+ordvec_status_t ordvec_stage1_exact_queries(uint64_t *count);
 
 /* Event timing of the engine's kernel families (no reference counterpart;
  * the reference's stats carry wall-clock only).  enable != 0 resets the

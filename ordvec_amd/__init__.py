@@ -18,6 +18,7 @@ from .index import (  # noqa: E402,F401
     RankQuant,
     RankQuantFastscan,
     SignBitmap,
+    stage1_exact_queries,
     synchronize,
     two_stage_search,
 )
@@ -38,6 +39,7 @@ __all__ = [
     "two_stage_search",
     "Server",
     "rerank_scores",
+    "stage1_exact_queries",
     "synchronize",
     "OrdvecError",
     "MAX_DIM",

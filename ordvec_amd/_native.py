@@ -136,6 +136,7 @@ SIGNATURES = {
     "ordvec_index_search": (ST, [P, P, P, U64, P, P]),
     "ordvec_set_stream": (ST, [P]),
     "ordvec_synchronize": (ST, []),
+    "ordvec_stage1_exact_queries": (ST, [P]),
     "ordvec_kernel_timing_enable": (ST, [C.c_int]),
     "ordvec_kernel_timing_read": (ST, [U32, P, P]),
     "ordvec_rankquant_new": (ST, [U64, U32, C.POINTER(P)]),

@@ -2176,6 +2176,14 @@ ordvec_status_t ordvec_synchronize(void) {
     });
 }
 
+ordvec_status_t ordvec_stage1_exact_queries(uint64_t* count) {
+    return boundary([&] {
+        NONNULL(count, "count");
+        LOCK;
+        *count = cur().exact_queries;
+    });
+}
+
 ordvec_status_t ordvec_kernel_timing_enable(int enable) {
     return boundary([&] {
         LOCK;

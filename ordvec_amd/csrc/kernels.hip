@@ -958,7 +958,8 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
         }
         // Every accumulator is consumed here (an empty asm reading one element
         // of each), so the compiler cannot sink MFMA chains into the
-        // conditional code below.
+        // conditional code below: MFMAs read every lane's operands and must
+        // run with the full exec mask.
 #pragma unroll
         for (int t = 0; t < NQT; t++) asm volatile("" ::"v"(acc[0][t][0]), "v"(acc[1][t][0]));
         // Survivors: for each 16-slot row group, 16 compares give the wave
@@ -994,7 +995,8 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
         }
         if (wbase > kListCap) {  // entries were dropped: send every query of the tile to the exact path
 #pragma unroll
-            for (int t = 0; t < NQT; t++) atomicAdd(cnt_col + t * 32, cap + 1);
+            for (int t = 0; t < NQT; t++)
+                if (q0 + t * 32 + col < nq) atomicAdd(cnt_col + t * 32, cap + 1);
             wbase = kListCap;
         }
         if (wbase >= kListCap / 2) {
@@ -1069,6 +1071,10 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
             if (w == 0) mfma_word<1, QPV, true>(x0[w], x1[w], 0, slane, sB, acc);
             else mfma_word<1, QPV, false>(x0[w], x1[w], w, slane, sB, acc);
         }
+        // The MFMAs read every lane's operands, so they must run with the full
+        // exec mask: an empty asm reading the results keeps the compiler from
+        // sinking them into the per-lane branch below.
+        asm volatile("" ::"v"(acc[0][0][0]), "v"(acc[1][0][0]));
         if (qvalid) {
             const uint64_t doc0 = (uint64_t)b * 64 + 4 * g;
 #pragma unroll

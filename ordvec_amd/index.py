@@ -865,5 +865,13 @@ def two_stage_search(sign: SignBitmap, rq: RankQuant, queries, m: int, k: int,
     return s, i
 
 
+def stage1_exact_queries() -> int:
+    """Queries whose stage-1 candidates went through the exact fallback so far
+    (default context): a monitoring counter."""
+    c = C.c_uint64()
+    N.check(N.lib().ordvec_stage1_exact_queries(C.byref(c)))
+    return c.value
+
+
 def synchronize() -> None:
     N.check(N.lib().ordvec_synchronize())

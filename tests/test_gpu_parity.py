@@ -303,3 +303,22 @@ def test_nonfinite_device_query_is_rejected(engine):
         rq.search_asymmetric(q, 5)
     with pytest.raises(ValueError):
         engine.two_stage_search(sb, rq, x[:3] * np.array([1, np.inf, 1], np.float32)[:, None], 50, 5)
+
+
+def test_fast_stage1_path_needs_no_exact_fallback_on_partial_tiles(engine, oracle):
+    """Batches of 2-31 queries fill one 32-query matrix-core tile partially;
+    on ordinary data every query must finish on the fast path (a wrong
+    sampled threshold or survivor list would still give exact results, but
+    only through the exact fallback, which the counter exposes)."""
+    d, n = 1024, 200_000
+    gen = S.MixtureNP(d, seed=21)
+    docs = gen.docs(n)
+    qs, _ = gen.queries(docs, 64)
+    sb = engine.SignBitmap(d)
+    sb.add(docs)
+    words = sb.words()
+    for nq in (2, 8, 31, 32, 64):
+        e0 = engine.stage1_exact_queries()
+        got = sb.top_m_candidates_batched(qs[:nq], 1000)
+        assert engine.stage1_exact_queries() == e0, nq
+        assert np.array_equal(got, oracle.sign_top_m(words, oracle.sign_encode(qs[:nq]), 1000, 8)), nq
