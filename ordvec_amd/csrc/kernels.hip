@@ -824,10 +824,10 @@ __device__ __forceinline__ void mfma_word(uint32_t x0, uint32_t x1, int w, uint3
     }
 }
 
-// Survivor lists: kListCap entries of (query slot << 56 | hamming << 32 | doc)
+// Survivor lists: 256 entries per 32-query tile (1024 at four tiles) of
+// (query slot << 56 | hamming << 32 | doc)
 // per wave (plus one spare slot per lane), drained to the per-query buffers
 // when half full and at the end.
-constexpr uint32_t kListCap = 1024;
 
 __device__ __forceinline__ void s1_flush_list(const uint64_t* list, uint32_t count, uint32_t lane, uint32_t q0,
                                               uint64_t n, uint32_t cap, uint64_t* __restrict__ keys,
@@ -856,7 +856,10 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     extern __shared__ uint4 sB[];  // [NQT][QPV][64]
     __shared__ int s_base[QW];
     __shared__ uint32_t s_thr[QW];
-    __shared__ uint64_t s_list[NWAVE * (kListCap + 64)];  // per-wave survivor lists (+ a spare slot per lane)
+    // Per-wave survivor lists (+ a spare slot per lane), sized by the tile
+    // count so one- and two-tile workgroups leave LDS for a second workgroup.
+    constexpr uint32_t kListCap = 256 * NQT;
+    __shared__ uint64_t s_list[NWAVE * (kListCap + 64)];
     // Workgroups are dealt to the 8 XCDs round-robin by id; map ids so that
     // every query tile of a document segment lands on the same XCD and the
     // segment's codes are read from HBM once per XCD L2 rather than per tile.
