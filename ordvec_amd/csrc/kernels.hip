@@ -827,18 +827,43 @@ __device__ __forceinline__ void mfma_word(uint32_t x0, uint32_t x1, int w, uint3
 // Survivor lists: 256 entries per 32-query tile (1024 at four tiles) of
 // (query slot << 56 | hamming << 32 | doc)
 // per wave (plus one spare slot per lane), drained to the per-query buffers
-// when half full and at the end.
-
+// when half full and at the end.  A drain takes one global atomic per query
+// slot rather than one per entry: the wave counts its entries per slot in
+// LDS (wcnt, QW counters), reserves that many positions with one atomicAdd
+// per slot, leaves the reserved base in wcnt, and then hands out positions
+// with LDS atomics.  Positions at or past cap are dropped; cnt[q] still
+// counts them, which sends query q to the exact path.
+template <uint32_t QW>
 __device__ __forceinline__ void s1_flush_list(const uint64_t* list, uint32_t count, uint32_t lane, uint32_t q0,
                                               uint64_t n, uint32_t cap, uint64_t* __restrict__ keys,
-                                              uint32_t* __restrict__ cnt) {
+                                              uint32_t* __restrict__ cnt, uint32_t* wcnt) {
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (uint32_t s = lane; s < QW; s += 64) wcnt[s] = 0;
     __builtin_amdgcn_wave_barrier();
     for (uint32_t i = lane; i < count; i += 64) {
         const uint64_t e = list[i];
-        if ((e & 0xffffffffull) >= n) continue;  // padding row of the last block
-        const uint32_t q = q0 + (uint32_t)(e >> 56);
-        const uint32_t pos = atomicAdd(&cnt[q], 1u);
-        if (pos < cap) keys[(uint64_t)q * cap + pos] = e & ((1ull << 56) - 1);
+        if ((e & 0xffffffffull) < n) atomicAdd(&wcnt[(uint32_t)(e >> 56)], 1u);  // padding rows of the last block skip
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t c[(QW + 63) / 64], r[(QW + 63) / 64];
+#pragma unroll
+    for (uint32_t j = 0; j < (QW + 63) / 64; j++) {
+        const uint32_t s = lane + 64 * j;
+        c[j] = s < QW ? wcnt[s] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < (QW + 63) / 64; j++) r[j] = c[j] ? atomicAdd(&cnt[q0 + lane + 64 * j], c[j]) : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < (QW + 63) / 64; j++)
+        if (lane + 64 * j < QW) wcnt[lane + 64 * j] = r[j];
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = lane; i < count; i += 64) {
+        const uint64_t e = list[i];
+        if ((e & 0xffffffffull) >= n) continue;
+        const uint32_t slot = (uint32_t)(e >> 56);
+        const uint32_t pos = atomicAdd(&wcnt[slot], 1u);
+        if (pos < cap) keys[(uint64_t)(q0 + slot) * cap + pos] = e & ((1ull << 56) - 1);
     }
     __builtin_amdgcn_wave_barrier();
 }
@@ -860,6 +885,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     // count so one- and two-tile workgroups leave LDS for a second workgroup.
     constexpr uint32_t kListCap = 256 * NQT;
     __shared__ uint64_t s_list[NWAVE * (kListCap + 64)];
+    __shared__ uint32_t s_wcnt[NWAVE * QW];  // per-wave slot counters of the list drain
     // Workgroups are dealt to the 8 XCDs round-robin by id; map ids so that
     // every query tile of a document segment lands on the same XCD and the
     // segment's codes are read from HBM once per XCD L2 rather than per tile.
@@ -1003,11 +1029,11 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
             wbase = kListCap;
         }
         if (wbase >= kListCap / 2) {
-            s1_flush_list(wlist, wbase, lane, q0, n, cap, keys, cnt);
+            s1_flush_list<QW>(wlist, wbase, lane, q0, n, cap, keys, cnt, s_wcnt + wave * QW);
             wbase = 0;
         }
     }
-    s1_flush_list(s_list + wave * (kListCap + 64), wbase, lane, q0, n, cap, keys, cnt);
+    s1_flush_list<QW>(s_list + wave * (kListCap + 64), wbase, lane, q0, n, cap, keys, cnt, s_wcnt + wave * QW);
 }
 
 // Sampled Hamming histogram on the matrix cores (the k_s1_sample contract):
