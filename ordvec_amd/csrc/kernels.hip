@@ -1350,85 +1350,66 @@ __device__ __forceinline__ float raw_avx512(const uint8_t* __restrict__ row, uin
     return __fadd_rn(t[0], t[1]);
 }
 
-// AVX-512 tier on a quad of lanes: lane j of the quad owns the reference's
-// accumulator j (chunks c = j mod 4, 16 SIMD lanes each in registers), so
-// (acc0+acc1)+(acc2+acc3) is two quad shuffles and every lane of the quad
-// ends with the same raw score.
+// The 16 codes of one chunk (BITS = 2: one dword; BITS = 4: two) as floats,
+// code l first-code-in-the-MSB as code_of.  Codes are masked a byte lane at a
+// time (w >> shift) & 0x0303..03 (0x0f0f..0f), so each code is one byte of a
+// masked dword and converts with one byte-to-float instruction.
 template <int BITS>
-__device__ __forceinline__ float raw_avx512_quad(const uint8_t* __restrict__ row, uint32_t d, const float* qs,
-                                                 uint32_t j) {
-    float a[16];
+__device__ __forceinline__ void codes16(const uint32_t* w, float (&c)[16]) {
+    if (BITS == 2) {
+        uint32_t m[4];  // m[p]: bit pair p of every byte (codes 4b + 3 - p)
 #pragma unroll
-    for (int l = 0; l < 16; l++) a[l] = 0.0f;
+        for (int p = 0; p < 4; p++) {
+            m[p] = (w[0] >> (2 * p)) & 0x03030303u;
+            // opaque to the optimiser, which would otherwise fold the masks
+            // into one bit-field extract per code
+            asm volatile("" : "+v"(m[p]));
+        }
+#pragma unroll
+        for (int l = 0; l < 16; l++) c[l] = (float)((m[3 - l % 4] >> (8 * (l / 4))) & 0xffu);
+    } else if (BITS == 4) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t hi = (w[h] >> 4) & 0x0f0f0f0fu, lo = w[h] & 0x0f0f0f0fu;
+            asm volatile("" : "+v"(hi), "+v"(lo));
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                c[8 * h + 2 * b] = (float)((hi >> (8 * b)) & 0xffu);
+                c[8 * h + 2 * b + 1] = (float)((lo >> (8 * b)) & 0xffu);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int l = 0; l < 16; l++) c[l] = (float)code_of<BITS>(w, l);
+    }
+}
+
+// x + (x of quad lane j ^ 1), then of lane j ^ 2, by DPP quad permutes (no LDS).
+__device__ __forceinline__ float quad_sum(float x) {
+    const float y = __fadd_rn(x, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, true)));
+    return __fadd_rn(y, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(y), 0x4E, 0xF, 0xF, true)));
+}
+
+// AVX-512 tier on a quad of lanes, NC candidates per quad: lane j of the quad
+// owns the reference's accumulator j (chunks c = j mod 4, 16 SIMD lanes each
+// in registers) of every candidate, so (acc0+acc1)+(acc2+acc3) is two quad
+// permutes and every lane of the quad ends with the same raw scores.  Each q
+// value read from LDS serves all NC candidates.
+template <int BITS, int NC>
+__device__ __forceinline__ void raw_avx512_quad_n(const uint8_t* const (&rows)[NC], uint32_t d, const float* qs,
+                                                  uint32_t j, float (&out)[NC]) {
+    float a[NC][16];
+#pragma unroll
+    for (int r = 0; r < NC; r++)
+#pragma unroll
+        for (int l = 0; l < 16; l++) a[r][l] = 0.0f;
     // Lane j takes chunks j, j + 4, ... (d % 64 == 0, so every lane has d / 64
     // of them), U at a time: their code words are requested together and q is
     // read from LDS as float4, so one wait covers U chunks.
     constexpr int NW = BITS == 2 ? 1 : 2;
-    constexpr uint32_t U = 8;
+    constexpr uint32_t U = NC == 1 ? 8 : 4;
     const uint32_t per_lane = d / 64;
-    uint32_t i = 0;
-    if (per_lane == 16) {  // d = 1024: the whole row share in one batch of loads
-        uint32_t w[16][NW];
-#pragma unroll
-        for (uint32_t u = 0; u < 16; u++) {
-            const uint32_t c = j + 4 * u;
-            if (BITS == 2) {
-                w[u][0] = *reinterpret_cast<const uint32_t*>(row + c * 4);
-            } else {
-                const uint2 x = *reinterpret_cast<const uint2*>(row + c * 8);
-                w[u][0] = x.x;
-                w[u][NW - 1] = x.y;
-            }
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < 16; u++) {
-            const float4* q4 = reinterpret_cast<const float4*>(qs + (j + 4 * u) * 16);
-            float qv[16];
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const float4 x = q4[v];
-                qv[4 * v] = x.x;
-                qv[4 * v + 1] = x.y;
-                qv[4 * v + 2] = x.z;
-                qv[4 * v + 3] = x.w;
-            }
-#pragma unroll
-            for (int l = 0; l < 16; l++) a[l] = __builtin_fmaf((float)code_of<BITS>(w[u], l), qv[l], a[l]);
-        }
-        i = per_lane;
-    }
-    for (; i + U <= per_lane; i += U) {
-        uint32_t w[U][NW];
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            const uint32_t c = j + 4 * (i + u);
-            if (BITS == 2) {
-                w[u][0] = *reinterpret_cast<const uint32_t*>(row + c * 4);
-            } else {
-                const uint2 x = *reinterpret_cast<const uint2*>(row + c * 8);
-                w[u][0] = x.x;
-                w[u][NW - 1] = x.y;
-            }
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            const float4* q4 = reinterpret_cast<const float4*>(qs + (j + 4 * (i + u)) * 16);
-            float qv[16];
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const float4 x = q4[v];
-                qv[4 * v] = x.x;
-                qv[4 * v + 1] = x.y;
-                qv[4 * v + 2] = x.z;
-                qv[4 * v + 3] = x.w;
-            }
-#pragma unroll
-            for (int l = 0; l < 16; l++) a[l] = __builtin_fmaf((float)code_of<BITS>(w[u], l), qv[l], a[l]);
-        }
-    }
-    for (; i < per_lane; i++) {
-        const uint32_t c = j + 4 * i;
-        uint32_t w[NW];
+    auto load = [&](uint32_t (&w)[NW], const uint8_t* row, uint32_t c) {
         if (BITS == 2) {
             w[0] = *reinterpret_cast<const uint32_t*>(row + c * 4);
         } else {
@@ -1436,22 +1417,73 @@ __device__ __forceinline__ float raw_avx512_quad(const uint8_t* __restrict__ row
             w[0] = x.x;
             w[NW - 1] = x.y;
         }
-        const float* qc = qs + c * 16;
+    };
+    auto chunk = [&](const uint32_t (&w)[NC][NW], uint32_t c) {
+        const float4* q4 = reinterpret_cast<const float4*>(qs + c * 16);
+        float qv[16];
 #pragma unroll
-        for (int l = 0; l < 16; l++) a[l] = __builtin_fmaf((float)code_of<BITS>(w, l), qc[l], a[l]);
+        for (int v = 0; v < 4; v++) {
+            const float4 x = q4[v];
+            qv[4 * v] = x.x;
+            qv[4 * v + 1] = x.y;
+            qv[4 * v + 2] = x.z;
+            qv[4 * v + 3] = x.w;
+        }
+#pragma unroll
+        for (int r = 0; r < NC; r++) {
+            float cf[16];
+            codes16<BITS>(w[r], cf);
+#pragma unroll
+            for (int l = 0; l < 16; l++) a[r][l] = __builtin_fmaf(cf[l], qv[l], a[r][l]);
+        }
+    };
+    uint32_t i = 0;
+    if (per_lane == 16) {  // d = 1024: the whole row share in one batch of loads
+        uint32_t w[16][NC][NW];
+#pragma unroll
+        for (uint32_t u = 0; u < 16; u++)
+#pragma unroll
+            for (int r = 0; r < NC; r++) load(w[u][r], rows[r], j + 4 * u);
+#pragma unroll
+        for (uint32_t u = 0; u < 16; u++) chunk(w[u], j + 4 * u);
+        i = per_lane;
+    }
+    for (; i + U <= per_lane; i += U) {
+        uint32_t w[U][NC][NW];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++)
+#pragma unroll
+            for (int r = 0; r < NC; r++) load(w[u][r], rows[r], j + 4 * (i + u));
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) chunk(w[u], j + 4 * (i + u));
+    }
+    for (; i < per_lane; i++) {
+        uint32_t w[NC][NW];
+#pragma unroll
+        for (int r = 0; r < NC; r++) load(w[r], rows[r], j + 4 * i);
+        chunk(w, j + 4 * i);
     }
 #pragma unroll
-    for (int l = 0; l < 16; l++) {
-        float s01 = __fadd_rn(a[l], __shfl_xor(a[l], 1));
-        a[l] = __fadd_rn(s01, __shfl_xor(s01, 2));
+    for (int r = 0; r < NC; r++) {
+#pragma unroll
+        for (int l = 0; l < 16; l++) a[r][l] = quad_sum(a[r][l]);
+#pragma unroll
+        for (int l = 0; l < 8; l++) a[r][l] = __fadd_rn(a[r][l], a[r][l + 8]);
+#pragma unroll
+        for (int l = 0; l < 4; l++) a[r][l] = __fadd_rn(a[r][l], a[r][l + 4]);
+#pragma unroll
+        for (int l = 0; l < 2; l++) a[r][l] = __fadd_rn(a[r][l], a[r][l + 2]);
+        out[r] = __fadd_rn(a[r][0], a[r][1]);
     }
-#pragma unroll
-    for (int l = 0; l < 8; l++) a[l] = __fadd_rn(a[l], a[l + 8]);
-#pragma unroll
-    for (int l = 0; l < 4; l++) a[l] = __fadd_rn(a[l], a[l + 4]);
-#pragma unroll
-    for (int l = 0; l < 2; l++) a[l] = __fadd_rn(a[l], a[l + 2]);
-    return __fadd_rn(a[0], a[1]);
+}
+
+template <int BITS>
+__device__ __forceinline__ float raw_avx512_quad(const uint8_t* __restrict__ row, uint32_t d, const float* qs,
+                                                 uint32_t j) {
+    const uint8_t* rows[1] = {row};
+    float out[1];
+    raw_avx512_quad_n<BITS, 1>(rows, d, qs, j, out);
+    return out[0];
 }
 
 // AVX2 tier (src/quant_kernels.rs:226-389): b=2 keeps 16 lanes (acc_hi codes
@@ -1519,16 +1551,31 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
     const uint32_t bpv = d * BITS / 8;
     tk_init(tk, out_k);
     if (TIER == 2) {
-        const uint32_t per_round = blockDim.x / 4, quad = threadIdx.x >> 2, jl = threadIdx.x & 3;
-        uint32_t id_next = sb + quad < se ? cands[sb + quad] : 0u;  // candidate ids run one round ahead
+        // Two candidates per quad per round (quad and quad + 64), so every q
+        // value read from LDS is used twice.
+        constexpr uint32_t NC = 2;
+        const uint32_t nquad = blockDim.x / 4, per_round = NC * nquad, quad = threadIdx.x >> 2, jl = threadIdx.x & 3;
+        uint32_t id_next[NC];  // candidate ids run one round ahead
+#pragma unroll
+        for (uint32_t r = 0; r < NC; r++) id_next[r] = sb + quad + r * nquad < se ? cands[sb + quad + r * nquad] : 0u;
         for (uint64_t r0 = sb; r0 < se; r0 += per_round) {
             tk_reserve(tk, out_k, per_round);
-            const uint64_t j = r0 + quad;
-            const uint32_t id = id_next;
-            id_next = j + per_round < se ? cands[j + per_round] : 0u;
-            const bool ok = j < se && id < n_docs;
-            const float raw = raw_avx512_quad<BITS>(codes + (uint64_t)(ok ? id : 0u) * bpv, d, qs, jl);
-            if (ok && jl == 0) tk_offer(tk, out_k, score_key(__fadd_rn(__fmul_rn(raw, inv_norm), off), id));
+            const uint8_t* rows[NC];
+            uint32_t id[NC];
+            bool ok[NC];
+#pragma unroll
+            for (uint32_t r = 0; r < NC; r++) {
+                const uint64_t j = r0 + quad + r * nquad;
+                id[r] = id_next[r];
+                id_next[r] = j + per_round < se ? cands[j + per_round] : 0u;
+                ok[r] = j < se && id[r] < n_docs;
+                rows[r] = codes + (uint64_t)(ok[r] ? id[r] : 0u) * bpv;
+            }
+            float raw[NC];
+            raw_avx512_quad_n<BITS, NC>(rows, d, qs, jl, raw);
+#pragma unroll
+            for (uint32_t r = 0; r < NC; r++)
+                if (ok[r] && jl == 0) tk_offer(tk, out_k, score_key(__fadd_rn(__fmul_rn(raw[r], inv_norm), off), id[r]));
         }
     } else {
         for (uint64_t r0 = sb; r0 < se; r0 += blockDim.x) {
