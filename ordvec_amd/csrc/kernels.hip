@@ -1350,38 +1350,77 @@ __device__ __forceinline__ float raw_avx512(const uint8_t* __restrict__ row, uin
     return __fadd_rn(t[0], t[1]);
 }
 
-// The 16 codes of one chunk (BITS = 2: one dword; BITS = 4: two) as floats,
-// code l first-code-in-the-MSB as code_of.  Codes are masked a byte lane at a
-// time (w >> shift) & 0x0303..03 (0x0f0f..0f), so each code is one byte of a
-// masked dword and converts with one byte-to-float instruction.
+// One chunk of 16 codes (BITS = 2: one dword; BITS = 4: two; code l as
+// code_of) multiplied into the 16 accumulators, a[l] = fma(c_l, q_l, a[l]),
+// with each code read as an FP16 subnormal.  The code dword is masked so that
+// each 16-bit half holds one code at bit t' <= 8 (BITS = 2) or <= 4 (BITS = 4);
+// that half is the FP16 value c * 2^(t' - 24), exact, and the mixed-precision
+// FMA (f32 x f16 + f32, one rounding) multiplies it by the staged
+// q_l * 2^(24 - t') (a power-of-two scale, exact for |q_l| <= 1).  The
+// product is c * q_l exactly, so every accumulator rounds as the plain f32
+// FMA does.  qsc holds q pre-scaled by stage_q_mix.
 template <int BITS>
-__device__ __forceinline__ void codes16(const uint32_t* w, float (&c)[16]) {
+__device__ __forceinline__ void chunk_fma_mix(const uint32_t* w, const float (&qv)[16], float (&a)[16]) {
     if (BITS == 2) {
-        uint32_t m[4];  // m[p]: bit pair p of every byte (codes 4b + 3 - p)
+        // code l sits at bit s = 8 (l / 4) + 2 (3 - l % 4) of the dword: half
+        // s / 16, bit t = s % 16 of the half; t >= 10 is read shifted down by 10
+        uint32_t m[8];
+        const uint32_t x = w[0] >> 10;
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
-            m[p] = (w[0] >> (2 * p)) & 0x03030303u;
-            // opaque to the optimiser, which would otherwise fold the masks
-            // into one bit-field extract per code
-            asm volatile("" : "+v"(m[p]));
+        for (int t = 0; t < 16; t += 2) {
+            m[t / 2] = t <= 8 ? (w[0] & (0x00030003u << t)) : (x & (0x00030003u << (t - 10)));
+            asm volatile("" : "+v"(m[t / 2]));  // keeps the masks (no per-code bit-field extracts)
         }
 #pragma unroll
-        for (int l = 0; l < 16; l++) c[l] = (float)((m[3 - l % 4] >> (8 * (l / 4))) & 0xffu);
-    } else if (BITS == 4) {
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            uint32_t hi = (w[h] >> 4) & 0x0f0f0f0fu, lo = w[h] & 0x0f0f0f0fu;
-            asm volatile("" : "+v"(hi), "+v"(lo));
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                c[8 * h + 2 * b] = (float)((hi >> (8 * b)) & 0xffu);
-                c[8 * h + 2 * b + 1] = (float)((lo >> (8 * b)) & 0xffu);
-            }
+        for (int l = 0; l < 16; l++) {
+            const int sb = 8 * (l / 4) + 2 * (3 - l % 4), h = sb / 16, t = sb % 16;
+            const uint32_t mm = m[t / 2];
+            const uint16_t half = (uint16_t)(h ? (mm >> 16) : (mm & 0xffffu));
+            a[l] = __builtin_fmaf(qv[l], (float)__builtin_bit_cast(_Float16, half), a[l]);
         }
     } else {
+        // BITS = 4: code l is nibble s = 8 ((l % 8) / 2) + 4 (1 - l % 2) of
+        // dword l / 8; t = s % 16 in {0, 4, 8, 12}, t >= 8 read shifted down by 8
 #pragma unroll
-        for (int l = 0; l < 16; l++) c[l] = (float)code_of<BITS>(w, l);
+        for (int dw = 0; dw < 2; dw++) {
+            uint32_t m[4];
+            const uint32_t x = w[dw] >> 8;
+#pragma unroll
+            for (int t = 0; t < 16; t += 4) {
+                m[t / 4] = t <= 4 ? (w[dw] & (0x000f000fu << t)) : (x & (0x000f000fu << (t - 8)));
+                asm volatile("" : "+v"(m[t / 4]));
+            }
+#pragma unroll
+            for (int l8 = 0; l8 < 8; l8++) {
+                const int sb = 8 * (l8 / 2) + 4 * (1 - l8 % 2), h = sb / 16, t = sb % 16;
+                const uint32_t mm = m[t / 4];
+                const uint16_t half = (uint16_t)(h ? (mm >> 16) : (mm & 0xffffu));
+                const int l = 8 * dw + l8;
+                a[l] = __builtin_fmaf(qv[l], (float)__builtin_bit_cast(_Float16, half), a[l]);
+            }
+        }
     }
+}
+
+// The scale chunk_fma_mix expects on coordinate i: 2^(24 - t'), t' the bit of
+// its code inside the masked half (a function of i % 16 only).
+template <int BITS>
+__device__ __forceinline__ float mix_scale(uint32_t i) {
+    const uint32_t l = i % 16;
+    int tp;
+    if (BITS == 2) {
+        const int t = (8 * (l / 4) + 2 * (3 - l % 4)) % 16;
+        tp = t <= 8 ? t : t - 10;
+    } else {
+        const int t = (8 * ((l % 8) / 2) + 4 * (1 - l % 2)) % 16;
+        tp = t <= 4 ? t : t - 8;
+    }
+    return __int_as_float((127 + 24 - tp) << 23);
+}
+
+template <int BITS>
+__device__ __forceinline__ void stage_q_mix(float* qs, const float* qg, uint32_t d) {
+    for (uint32_t i = threadIdx.x; i < d; i += blockDim.x) qs[i] = __fmul_rn(qg[i], mix_scale<BITS>(i));
 }
 
 // x + (x of quad lane j ^ 1), then of lane j ^ 2, by DPP quad permutes (no LDS).
@@ -1394,7 +1433,8 @@ __device__ __forceinline__ float quad_sum(float x) {
 // owns the reference's accumulator j (chunks c = j mod 4, 16 SIMD lanes each
 // in registers) of every candidate, so (acc0+acc1)+(acc2+acc3) is two quad
 // permutes and every lane of the quad ends with the same raw scores.  Each q
-// value read from LDS serves all NC candidates.
+// value read from LDS serves all NC candidates.  BITS is 2 or 4 (the SIMD
+// tiers), and qs is staged by stage_q_mix<BITS>.
 template <int BITS, int NC>
 __device__ __forceinline__ void raw_avx512_quad_n(const uint8_t* const (&rows)[NC], uint32_t d, const float* qs,
                                                   uint32_t j, float (&out)[NC]) {
@@ -1430,12 +1470,7 @@ __device__ __forceinline__ void raw_avx512_quad_n(const uint8_t* const (&rows)[N
             qv[4 * v + 3] = x.w;
         }
 #pragma unroll
-        for (int r = 0; r < NC; r++) {
-            float cf[16];
-            codes16<BITS>(w[r], cf);
-#pragma unroll
-            for (int l = 0; l < 16; l++) a[r][l] = __builtin_fmaf(cf[l], qv[l], a[r][l]);
-        }
+        for (int r = 0; r < NC; r++) chunk_fma_mix<BITS>(w[r], qv, a[r]);
     };
     uint32_t i = 0;
     if (per_lane == 16) {  // d = 1024: the whole row share in one batch of loads
@@ -1546,7 +1581,8 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
     const uint64_t e = offsets ? offsets[q + 1] : (q + 1) * row_m;
     const uint64_t per = (e - b + nsplit - 1) / nsplit;
     const uint64_t sb = min(e, b + split * per), se = min(e, sb + per);
-    stage_q(qs, qhat + q * d, d);
+    if (TIER == 2) stage_q_mix<BITS>(qs, qhat + q * d, d);
+    else stage_q(qs, qhat + q * d, d);
     const float off = offset[q];
     const uint32_t bpv = d * BITS / 8;
     tk_init(tk, out_k);
@@ -1897,7 +1933,8 @@ __global__ __launch_bounds__(256) void k_score_ids(const uint8_t* __restrict__ c
                                                    uint32_t k, float* __restrict__ out, uint32_t* __restrict__ bad) {
     extern __shared__ float qs[];
     const uint64_t q = blockIdx.x;
-    stage_q(qs, qhat + q * d, d);
+    if (TIER == 2) stage_q_mix<BITS>(qs, qhat + q * d, d);
+    else stage_q(qs, qhat + q * d, d);
     __syncthreads();
     const float off = offset[q];
     const uint32_t bpv = d * BITS / 8;
