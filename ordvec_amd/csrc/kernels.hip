@@ -2509,8 +2509,11 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
                         const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
     const uint32_t nblk = (uint32_t)((n + 63) / 64);
     if (use_mfma_scan(nq, qpv)) {
-        // Matrix-core scan: 32, 64 or 128 queries per workgroup of 8 waves.
-        const uint32_t nqt = nq >= 128 && qpv <= 16 ? 4 : (nq >= 64 ? 2 : 1);
+        // Matrix-core scan: 32, 64 or 128 queries per workgroup of 8 waves,
+        // the smallest tile that holds the batch: every extra query tile
+        // re-reads the codes, and a large corpus does not stay in L2 between
+        // the tiles of a segment.
+        const uint32_t nqt = nq > 64 ? 4 : (nq > 32 ? 2 : 1);
         const uint32_t tiles_m = (uint32_t)((nq + 32 * nqt - 1) / (32 * nqt));
         const uint64_t want = std::max<uint64_t>(1, (1024 + tiles_m - 1) / tiles_m);
         uint64_t nseg = std::min<uint64_t>(std::max<uint64_t>(1, (nblk + 7) / 8), want);

@@ -306,18 +306,18 @@ def test_nonfinite_device_query_is_rejected(engine):
 
 
 def test_fast_stage1_path_needs_no_exact_fallback_on_partial_tiles(engine, oracle):
-    """Batches of 2-31 queries fill one 32-query matrix-core tile partially;
-    on ordinary data every query must finish on the fast path (a wrong
-    sampled threshold or survivor list would still give exact results, but
-    only through the exact fallback, which the counter exposes)."""
+    """Batches that fill their matrix-core query tiles (32, 64 or 128 queries)
+    partially; on ordinary data every query must finish on the fast path (a
+    wrong sampled threshold or survivor list would still give exact results,
+    but only through the exact fallback, which the counter exposes)."""
     d, n = 1024, 200_000
     gen = S.MixtureNP(d, seed=21)
     docs = gen.docs(n)
-    qs, _ = gen.queries(docs, 64)
+    qs, _ = gen.queries(docs, 200)
     sb = engine.SignBitmap(d)
     sb.add(docs)
     words = sb.words()
-    for nq in (2, 8, 31, 32, 64):
+    for nq in (2, 8, 31, 32, 33, 48, 64, 65, 100, 128, 129, 200):
         e0 = engine.stage1_exact_queries()
         got = sb.top_m_candidates_batched(qs[:nq], 1000)
         assert engine.stage1_exact_queries() == e0, nq
