@@ -175,6 +175,7 @@ struct FastPlan {
     uint32_t qt, nbins, hwords, lds_bytes;
     uint32_t stride, nsample, sample_bps, nsseg;
     uint32_t scan_qt, scan_bps, nseg;
+    uint32_t scan_wgl;  // VALU scan: survivor list entries per workgroup (0: global appends)
     uint32_t s_target;  // ~0u: admit every document
     uint32_t cap;
 };

@@ -696,12 +696,31 @@ __device__ __forceinline__ void s1_append(bool pass, uint32_t acc, uint64_t doc,
     }
 }
 
+// Workgroup-list form of s1_append: the wave's passing lanes take consecutive
+// entries (query slot << 56 | hamming << 32 | doc) of the workgroup's LDS list
+// through one LDS atomic; the list is drained once at the end of the
+// workgroup, with one global atomic per query.  (One global atomic per
+// appending wave on a single query's counter serialises: at 1M documents and
+// one query, about 3000 of them doubled the scan time.)
+__device__ __forceinline__ void s1_append_lds(bool pass, uint32_t acc, uint64_t doc, uint32_t lane, uint64_t lt_mask,
+                                              uint32_t slot, uint64_t* list, uint32_t* count) {
+    const uint64_t mask = __ballot(pass);
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        uint32_t basepos = 0;
+        if ((int)lane == leader) basepos = atomicAdd(count, (uint32_t)__popcll(mask));
+        basepos = __shfl(basepos, leader);
+        const uint32_t pos = basepos + (uint32_t)__popcll(mask & lt_mask);
+        if (pass) list[pos] = ((uint64_t)slot << 56) | ((uint64_t)acc << 32) | doc;
+    }
+}
+
 template <int WCH, int OP>
 __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
                                                  uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
                                                  uint32_t qt, uint32_t blocks_per_seg, const uint32_t* __restrict__ thr,
                                                  uint32_t cap, uint64_t* __restrict__ keys, uint32_t* __restrict__ cnt,
-                                                 uint32_t ntop) {
+                                                 uint32_t ntop, uint32_t wgl) {
     const uint32_t q0 = blockIdx.x * qt;
     const uint32_t tq = min(qt, nq - q0);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -712,35 +731,83 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
     const uint32_t* tb = thr + q0;
     uint32_t* cb = cnt + q0;
     uint64_t* kb = keys + (uint64_t)q0 * cap;
-    for (uint32_t b = b_begin + wave; b < b_end; b += 4) {
-        const uint64_t* base = words + (uint64_t)b * qpv * 64 + lane;
-        const uint64_t doc = (uint64_t)b * 64 + lane;
-        const bool valid = doc < n;
-        if (WCH > 0) {
-            uint64_t dw[WCH > 0 ? WCH : 1];
+    // wgl != 0: survivors go to the workgroup's LDS list (wgl entries, at
+    // least qt x 64 x blocks_per_seg, so it cannot overflow) and are drained
+    // at the end; otherwise each appending wave takes global positions.
+    extern __shared__ uint64_t s_wl[];
+    __shared__ uint32_t s_wn, s_slot[4];
+    if (wgl) {
+        if (threadIdx.x == 0) s_wn = 0;
+        if (threadIdx.x < 4) s_slot[threadIdx.x] = 0;
+        __syncthreads();
+    }
+    auto append = [&](bool pass, uint32_t a, uint64_t doc, uint32_t i) {
+        if (wgl) s1_append_lds(pass, a, doc, lane, lt_mask, i, s_wl, &s_wn);
+        else s1_append(pass, a, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
+    };
+    if (WCH > 0) {
+        // Each wave's next block is requested before the current one is
+        // scored (the last block requests itself again: an in-bounds load
+        // whose value is not used keeps the loop free of branches).
+        uint64_t dw[WCH > 0 ? WCH : 1], nx[WCH > 0 ? WCH : 1];
+        if (b_begin + wave < b_end) {
+            const uint64_t* base = words + (uint64_t)(b_begin + wave) * qpv * 64 + lane;
 #pragma unroll
             for (int j = 0; j < WCH; j++) dw[j] = base[(uint64_t)j * 64];
+        }
+        for (uint32_t b = b_begin + wave; b < b_end; b += 4) {
+            const uint64_t doc = (uint64_t)b * 64 + lane;
+            const bool valid = doc < n;
+            const uint64_t* nb = words + (uint64_t)(b + 4 < b_end ? b + 4 : b) * qpv * 64 + lane;
+#pragma unroll
+            for (int j = 0; j < WCH; j++) nx[j] = nb[(uint64_t)j * 64];
             uint32_t i = 0;
             // Two queries per step: their scalar loads share one wait.
             for (; i + 1 < tq; i += 2) {
                 const uint32_t a0 = hamming_regs<WCH, OP>(dw, qb + (uint64_t)i * WCH, ntop);
                 const uint32_t a1 = hamming_regs<WCH, OP>(dw, qb + (uint64_t)(i + 1) * WCH, ntop);
                 const uint32_t t0 = tb[i], t1 = tb[i + 1];
-                s1_append(valid && a0 <= t0, a0, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
-                s1_append(valid && a1 <= t1, a1, doc, lane, lt_mask, cb + i + 1, kb + (uint64_t)(i + 1) * cap, cap);
+                append(valid && a0 <= t0, a0, doc, i);
+                append(valid && a1 <= t1, a1, doc, i + 1);
             }
             if (i < tq) {
                 const uint32_t a0 = hamming_regs<WCH, OP>(dw, qb + (uint64_t)i * WCH, ntop);
-                s1_append(valid && a0 <= tb[i], a0, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
+                append(valid && a0 <= tb[i], a0, doc, i);
             }
-        } else {
+#pragma unroll
+            for (int j = 0; j < WCH; j++) dw[j] = nx[j];
+        }
+    } else {
+        for (uint32_t b = b_begin + wave; b < b_end; b += 4) {
+            const uint64_t* base = words + (uint64_t)b * qpv * 64 + lane;
+            const uint64_t doc = (uint64_t)b * 64 + lane;
+            const bool valid = doc < n;
             for (uint32_t i = 0; i < tq; i++) {
                 const uint64_t* qp = qb + (uint64_t)i * qpv;
                 uint32_t acc = 0;
                 for (uint32_t j = 0; j < qpv; j++) acc += word_stat<OP>(base[(uint64_t)j * 64], qp[j]);
                 acc = to_dist<OP>(acc, ntop);
-                s1_append(valid && acc <= tb[i], acc, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
+                append(valid && acc <= tb[i], acc, doc, i);
             }
+        }
+    }
+    if (wgl) {
+        // Drain: count per query slot, reserve each slot's run with one
+        // global atomic, hand out positions inside the runs from LDS.
+        __syncthreads();
+        const uint32_t total = s_wn;
+        for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) atomicAdd(&s_slot[(uint32_t)(s_wl[e] >> 56)], 1u);
+        __syncthreads();
+        if (threadIdx.x < tq && threadIdx.x < 4) {
+            const uint32_t c = s_slot[threadIdx.x];
+            s_slot[threadIdx.x] = c ? atomicAdd(&cb[threadIdx.x], c) : 0u;
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+            const uint64_t x = s_wl[e];
+            const uint32_t slot = (uint32_t)(x >> 56);
+            const uint32_t pos = atomicAdd(&s_slot[slot], 1u);
+            if (pos < cap) kb[(uint64_t)slot * cap + pos] = x & ((1ull << 56) - 1);
         }
     }
 }
@@ -2378,6 +2445,14 @@ FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
     uint64_t want = std::max<uint64_t>(1, (2048 + scan_tiles - 1) / scan_tiles);
     uint64_t nseg = std::min<uint64_t>(std::max<uint64_t>(1, nblk), want);
     uint64_t bps = std::max<uint64_t>(1, (nblk + nseg - 1) / nseg);
+    p.scan_wgl = 0;
+    if (p.scan_qt <= 4) {
+        // Tiles of up to 4 queries collect survivors in a workgroup LDS list
+        // of qt x 64 x bps entries (16 KiB: 32 blocks per segment at one
+        // query), which no segment can overflow.
+        bps = std::min<uint64_t>(std::max<uint64_t>(1, nblk), 32 / p.scan_qt);
+        p.scan_wgl = (uint32_t)(p.scan_qt * 64 * bps);
+    }
     p.scan_bps = (uint32_t)bps;
     p.nseg = (uint32_t)std::max<uint64_t>(1, (nblk + bps - 1) / bps);
     return p;
@@ -2548,8 +2623,8 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
     const uint32_t tiles = (uint32_t)((nq + p.scan_qt - 1) / p.scan_qt);
     dim3 grid(tiles, p.nseg);
 #define OVX_SCAN(W)                                                                                               \
-    hipLaunchKernelGGL((k_s1_scan<W, OP>), grid, dim3(256), 0, s, words, nblk, qpv, n, qbits, (uint32_t)nq,        \
-                       p.scan_qt, p.scan_bps, w.thr, p.cap, w.keys, w.cnt, ntop)
+    hipLaunchKernelGGL((k_s1_scan<W, OP>), grid, dim3(256), (size_t)p.scan_wgl * 8, s, words, nblk, qpv, n, qbits, \
+                       (uint32_t)nq, p.scan_qt, p.scan_bps, w.thr, p.cap, w.keys, w.cnt, ntop, p.scan_wgl)
     switch (qpv) {
         case 16: OVX_SCAN(16); break;
         case 12: OVX_SCAN(12); break;

@@ -115,7 +115,9 @@ def test_empty_and_zero_k_shapes(engine):
                                         # matrix-core scan shapes (>= 32 queries): full and partial query
                                         # tiles, a ragged last document block, tiny corpora
                                         (1024, 30_001, 100, 1000), (64, 5000, 70, 64), (192, 7001, 33, 300),
-                                        (1024, 65, 64, 10), (512, 12_345, 32, 500)])
+                                        (1024, 65, 64, 10), (512, 12_345, 32, 500),
+                                        # VALU scan without a word-count specialisation
+                                        (2048, 9000, 5, 500), (320, 10_000, 1, 700)])
 def test_sign_top_m_exact(engine, oracle, dim, n, nq, m):
     gen = S.MixtureNP(dim, seed=11)
     docs = gen.docs(n)
