@@ -377,6 +377,12 @@ ordvec_status_t ordvec_fastscan_batch_run(ordvec_fastscan_batch_t *batch, const 
 ordvec_status_t ordvec_fastscan_batch_config(ordvec_fastscan_batch_t *batch, uint64_t *total_queries, uint32_t *kout,
                                              uint32_t *queries_per_tile, uint32_t *blocks_per_tile, uint32_t *grid,
                                              uint64_t *n_tiles, uint64_t *n_tuned);
+/* Pins the tile shape (no reference counterpart): queries_per_tile 4 or 8
+ * (LDS LUT lookups) or 16 (matrix-core tiles, dim % 64 == 0), 64-document
+ * blocks per tile >= 1, persistent grid >= 1; re-captures the graph.
+ * BAD_ARGUMENT for a shape this dim cannot run. */
+ordvec_status_t ordvec_fastscan_batch_set_config(ordvec_fastscan_batch_t *batch, uint32_t queries_per_tile,
+                                                 uint32_t blocks_per_tile, uint32_t grid);
 /* Row i of the last autotune: the configuration and its best time (ms). */
 ordvec_status_t ordvec_fastscan_batch_tuning(ordvec_fastscan_batch_t *batch, uint64_t i, uint32_t *queries_per_tile,
                                              uint32_t *blocks_per_tile, uint32_t *grid, double *ms);

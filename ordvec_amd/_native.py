@@ -184,6 +184,7 @@ SIGNATURES = {
     "ordvec_fastscan_batch_run": (ST, [P, P, P, P, U64]),
     "ordvec_fastscan_batch_config": (ST, [P, P, P, P, P, P, P, P]),
     "ordvec_fastscan_batch_tuning": (ST, [P, U64, P, P, P, P]),
+    "ordvec_fastscan_batch_set_config": (ST, [P, U32, U32, U32]),
     "ordvec_fastscan_batch_free": (None, [P]),
     "ordvec_rankquant_score_ids": (ST, [P, P, U64, P, U64, P, U64]),
     "ordvec_rankquant_score_ids_backward": (ST, [P, P, U64, P, U64, P, P, U64]),

@@ -1062,8 +1062,19 @@ size_t serve_ws_bytes(const SbIndex* sb, const RqIndex* rq, uint64_t cap, uint64
 // FastScan b=4: tile tables, the plain per-index search and the batched
 // multi-index form (persistent scan captured in a HIP graph, autotuned tiles).
 // ---------------------------------------------------------------------------
+static uint32_t device_cus() {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    return (uint32_t)cus;
+}
+
+constexpr uint64_t kFs4MfmaMinDocs = 16384;  // plain search: matrix-core tiles from this index size
+
 struct Fs4Config {
-    uint32_t qg = 1;          // LUT groups of four queries per tile (1 or 2)
+    uint32_t qg = 1;          // LUT groups of four queries per tile (1 or 2; 4: matrix-core tiles)
     uint32_t seg_blocks = 64; // 64-document blocks per tile
     uint32_t grid = 512;      // persistent workgroups
 };
@@ -1073,6 +1084,7 @@ struct Fs4Tables {
     std::vector<Fs4Group> groups;
     std::vector<Fs4Tile> tiles;
     std::vector<Fs4Merge> merges;
+    std::vector<Fs4Group> qsets;  // matrix-core tiles: {first group, groups} per query set
     uint32_t nq = 0, kcap = 1;
 };
 
@@ -1095,9 +1107,11 @@ Fs4Tables fs4_tables(const std::vector<const FsIndex*>& ixs, const std::vector<u
         for (uint32_t gs = 0; gs < ng; gs += c.qg) {
             const uint32_t t0 = (uint32_t)t.tiles.size();
             const uint32_t gn = std::min(c.qg, ng - gs);
+            const uint32_t qset = (uint32_t)t.qsets.size();
+            if (c.qg == 4) t.qsets.push_back(Fs4Group{g_first + gs, gn});
             for (uint32_t seg = 0; seg < nseg; seg++)
                 t.tiles.push_back(Fs4Tile{(uint32_t)i, g_first + gs, gn, seg * c.seg_blocks,
-                                          std::min(nblk, (seg + 1) * c.seg_blocks), 0});
+                                          std::min(nblk, (seg + 1) * c.seg_blocks), qset});
             for (uint32_t g = 0; g < gn; g++)
                 for (uint32_t s = 0; s < t.groups[g_first + gs + g].count; s++)
                     t.merges.push_back(Fs4Merge{t0, nseg, g * 4 + s, nseg ? k : 0});
@@ -1118,10 +1132,12 @@ struct Fs4Dev {
     float *dq = nullptr, *qhat = nullptr, *off = nullptr, *bq = nullptr, *out_s = nullptr;
     uint32_t *qstat = nullptr, *counter = nullptr;
     uint8_t* lut = nullptr;
+    Fs4Group* qsets = nullptr;
+    uint32_t* frag = nullptr;
     uint64_t* partial = nullptr;
     int64_t* out_i = nullptr;
     size_t partial_bytes = 0;
-    uint32_t ntiles = 0, ngroups = 0, nq = 0, kcap = 1;
+    uint32_t ntiles = 0, ngroups = 0, nqsets = 0, nq = 0, kcap = 1;
 };
 
 size_t fs4_dev_bytes(const Fs4Tables& t, uint32_t d, uint32_t qg, size_t partial_floor) {
@@ -1130,7 +1146,8 @@ size_t fs4_dev_bytes(const Fs4Tables& t, uint32_t d, uint32_t qg, size_t partial
     return al(t.idx.size() * sizeof(Fs4IndexDesc)) + al(t.groups.size() * sizeof(Fs4Group)) +
            al(t.tiles.size() * sizeof(Fs4Tile)) + al(t.merges.size() * sizeof(Fs4Merge)) + 3 * al(nq * d * 4) +
            al(nq * 4) * 2 + al(nq * 8) + al(t.groups.size() * (uint64_t)d * 64) + al(part) + al(256) +
-           al(nq * t.kcap * 4) + al(nq * t.kcap * 8);
+           al(nq * t.kcap * 4) + al(nq * t.kcap * 8) + al(t.qsets.size() * sizeof(Fs4Group)) +
+           al(t.qsets.size() * fs4_frag_bytes(d));
 }
 
 // Carves the batch buffers out of `base` (fs4_dev_bytes long) and uploads the
@@ -1154,6 +1171,10 @@ void fs4_upload(Fs4Dev& dv, const Fs4Tables& t, uint32_t d, uint32_t qg, size_t 
     dv.counter = cv.take<uint32_t>(64);
     dv.out_s = cv.take<float>(nq * t.kcap);
     dv.out_i = cv.take<int64_t>(nq * t.kcap);
+    dv.qsets = cv.take<Fs4Group>(t.qsets.size());
+    dv.frag = reinterpret_cast<uint32_t*>(cv.take<char>(t.qsets.size() * fs4_frag_bytes(d)));
+    dv.nqsets = (uint32_t)t.qsets.size();
+    if (dv.nqsets) h2d(dv.qsets, t.qsets.data(), t.qsets.size() * sizeof(Fs4Group));
     h2d(dv.idx, t.idx.data(), t.idx.size() * sizeof(Fs4IndexDesc));
     h2d(dv.groups, t.groups.data(), t.groups.size() * sizeof(Fs4Group));
     h2d(dv.tiles, t.tiles.data(), t.tiles.size() * sizeof(Fs4Tile));
@@ -1172,40 +1193,76 @@ void fs4_enqueue(const Fs4Dev& dv, uint32_t d, const Fs4Config& c, bool persiste
     if (prepare) {
         launch_prepare_queries(dv.dq, dv.nq, d, 4, 0, inv_norm, dv.qhat, dv.off, dv.qstat, s);
         launch_fs4_lut(dv.qhat, d, dv.groups, dv.ngroups, dv.lut, dv.bq, s);
+        if (c.qg == 4) launch_fs4_frag(dv.lut, dv.qsets, dv.nqsets, d, dv.frag, s);
     }
+    const uint8_t* tables = c.qg == 4 ? reinterpret_cast<const uint8_t*>(dv.frag) : dv.lut;
     launch_fs4_scan(c.qg, persistent ? c.grid : dv.ntiles, dv.idx, dv.tiles, dv.ntiles,
-                    persistent ? dv.counter : nullptr, dv.groups, dv.lut, dv.bq, d, inv_norm, dv.kcap, dv.partial, s);
+                    persistent ? dv.counter : nullptr, dv.groups, tables, dv.bq, d, inv_norm, dv.kcap, dv.partial, s);
     launch_fs4_merge(dv.partial, dv.merges, dv.nq, 4 * c.qg, dv.kcap, dv.kcap, dv.out_s, dv.out_i, s);
 }
 
-bool fs4_config_fits(uint32_t qg, uint32_t d) { return fs4_scan_lds_bytes(qg, d) + 1024 <= 160 * 1024; }
+bool fs4_config_fits(uint32_t qg, uint32_t d) {
+    if (qg == 4 && !fs4_mfma_ok(d)) return false;
+    return fs4_scan_lds_bytes(qg, d) + 1024 <= 160 * 1024;
+}
 
 // Plain search of one b=4 index (nq queries, top-k).
+// Plain search of one b=4 index (nq queries, top-k).  Whole sets of 16
+// queries run on matrix-core tiles (the autotuner's faster shape for full
+// sets), the rest on LDS-LUT tiles of 4 or 8 queries: a partly filled 16-query
+// tile costs as much as a full one.
 void fs4_plain_search(const FsIndex* ix, const float* queries, uint64_t nq, uint64_t kk, float* out_s, int64_t* out_i,
                       uint64_t flags) {
     const uint32_t d = ix->dim;
-    Fs4Config c;
-    c.qg = nq > 4 && fs4_config_fits(2, d) ? 2 : 1;
-    const uint64_t gsets = (nq + 4 * c.qg - 1) / (4 * c.qg);
-    const uint64_t want = std::max<uint64_t>(1, (2048 + gsets - 1) / gsets);
-    c.seg_blocks = (uint32_t)std::max<uint64_t>(1, (ix->blocks64() + want - 1) / want);
-    Fs4Tables t = fs4_tables({ix}, {(uint32_t)nq}, {(uint32_t)kk}, c);
-    Fs4Dev dv;
-    fs4_upload(dv, t, d, c.qg, 0, cur().workspace(fs4_dev_bytes(t, d, c.qg, 0)));
-    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-    if (dev) d2d(dv.dq, queries, nq * d * 4);
-    else h2d(dv.dq, queries, nq * d * 4);
-    fs4_enqueue(dv, d, c, false);
-    if (dev) {
-        OVX_HIP(hipMemcpy2DAsync(out_s, kk * 4, dv.out_s, dv.kcap * 4, kk * 4, nq, hipMemcpyDeviceToDevice, cur().stream));
-        OVX_HIP(hipMemcpy2DAsync(out_i, kk * 8, dv.out_i, dv.kcap * 8, kk * 8, nq, hipMemcpyDeviceToDevice, cur().stream));
-    } else {
-        OVX_HIP(hipMemcpy2DAsync(out_s, kk * 4, dv.out_s, dv.kcap * 4, kk * 4, nq, hipMemcpyDeviceToHost, cur().stream));
-        OVX_HIP(hipMemcpy2DAsync(out_i, kk * 8, dv.out_i, dv.kcap * 8, kk * 8, nq, hipMemcpyDeviceToHost, cur().stream));
+    // (small indexes stay on LUT tiles: a 16-query tile's fixed cost -- 16
+    // top-k lists to clear and sort -- outweighs its scan there)
+    const uint64_t n16 = fs4_config_fits(4, d) && ix->n >= kFs4MfmaMinDocs ? nq / 16 * 16 : 0;
+    struct Part {
+        Fs4Config c;
+        Fs4Tables t;
+        Fs4Dev dv;
+        uint64_t q0, nq;
+        size_t bytes;
+    };
+    std::vector<Part> parts;
+    for (int pi = 0; pi < 2; pi++) {
+        const uint64_t q0 = pi == 0 ? 0 : n16, cnt = pi == 0 ? n16 : nq - n16;
+        if (cnt == 0) continue;
+        Part p;
+        p.q0 = q0;
+        p.nq = cnt;
+        p.c.qg = pi == 0 ? 4 : (cnt > 4 && fs4_config_fits(2, d) ? 2 : 1);
+        const uint64_t gsets = (cnt + 4 * p.c.qg - 1) / (4 * p.c.qg);
+        if (p.c.qg == 4) {  // about one tile per CU, whole 16-block (1024-doc) steps
+            const uint64_t want = std::max<uint64_t>(1, (device_cus() + gsets - 1) / gsets);
+            const uint64_t sb = std::max<uint64_t>(16, (ix->blocks64() + want - 1) / want);
+            p.c.seg_blocks = (uint32_t)((sb + 15) / 16 * 16);
+        } else {
+            const uint64_t want = std::max<uint64_t>(1, (2048 + gsets - 1) / gsets);
+            p.c.seg_blocks = (uint32_t)std::max<uint64_t>(1, (ix->blocks64() + want - 1) / want);
+        }
+        p.t = fs4_tables({ix}, {(uint32_t)cnt}, {(uint32_t)kk}, p.c);
+        p.bytes = fs4_dev_bytes(p.t, d, p.c.qg, 0);
+        parts.push_back(std::move(p));
     }
-    std::vector<uint32_t> st = read_qstat(dv.qstat, nq);
-    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
-        fail(kNonfiniteQuery, "queries contain NaN or infinity");
+    size_t total = 0;
+    for (const Part& p : parts) total += p.bytes;
+    char* base = static_cast<char*>(cur().workspace(total));
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    for (Part& p : parts) {
+        fs4_upload(p.dv, p.t, d, p.c.qg, 0, base);
+        base += p.bytes;
+        const float* q = queries + p.q0 * d;
+        if (dev) d2d(p.dv.dq, q, p.nq * d * 4);
+        else h2d(p.dv.dq, q, p.nq * d * 4);
+        fs4_enqueue(p.dv, d, p.c, false);
+        const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        OVX_HIP(hipMemcpy2DAsync(out_s + p.q0 * kk, kk * 4, p.dv.out_s, p.dv.kcap * 4, kk * 4, p.nq, kind, cur().stream));
+        OVX_HIP(hipMemcpy2DAsync(out_i + p.q0 * kk, kk * 8, p.dv.out_i, p.dv.kcap * 8, kk * 8, p.nq, kind, cur().stream));
+    }
+    bool bad = false;
+    for (const Part& p : parts) bad |= any_bit(read_qstat(p.dv.qstat, p.nq), kStatNonfinite);
+    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && bad) fail(kNonfiniteQuery, "queries contain NaN or infinity");
 }
 
 // Autotuned configurations, keyed by the batch's shape class.
@@ -3080,15 +3137,6 @@ ordvec_status_t ordvec_fastscan_load(const char* path, ordvec_fastscan_t** out) 
 struct ordvec_fastscan_batch_t {};
 static Fs4Batch* FB(ordvec_fastscan_batch_t* p) { return reinterpret_cast<Fs4Batch*>(p); }
 
-static uint32_t device_cus() {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) {
-        (void)hipGetLastError();
-        cus = 256;
-    }
-    return (uint32_t)cus;
-}
-
 ordvec_status_t ordvec_fastscan_batch_new(const ordvec_fastscan_t* const* indexes, uint64_t n_indexes,
                                           const uint32_t* queries_per_index, const uint32_t* k_per_index,
                                           ordvec_fastscan_batch_t** out) {
@@ -3166,6 +3214,9 @@ ordvec_status_t ordvec_fastscan_batch_autotune(ordvec_fastscan_batch_t* batch, c
             for (uint32_t sb : {16u, 32u, 64u, 128u, 256u})
                 for (uint32_t gm : {1u, 2u, 4u}) cands.push_back(Fs4Config{qg, sb, cus * gm});
         }
+        if (fs4_config_fits(4, b->d))  // matrix-core tiles: one 512-thread workgroup per CU
+            for (uint32_t sb : {16u, 32u, 64u, 128u, 256u})
+                for (uint32_t gm : {1u, 2u}) cands.push_back(Fs4Config{4, sb, cus * gm});
         size_t bytes = 0;
         for (const Fs4Config& c : cands) bytes = std::max(bytes, fs4_dev_bytes(fs4_tables(b->ixs, b->nq_per, b->k_per, c), b->d, c.qg, 0));
         b->drop_graph();
@@ -3257,6 +3308,22 @@ ordvec_status_t ordvec_fastscan_batch_config(ordvec_fastscan_batch_t* batch, uin
         if (grid) *grid = b->cfg.grid;
         if (n_tiles) *n_tiles = b->dv.ntiles;
         if (n_tuned) *n_tuned = b->tuning.size();
+    });
+}
+
+ordvec_status_t ordvec_fastscan_batch_set_config(ordvec_fastscan_batch_t* batch, uint32_t queries_per_tile,
+                                                 uint32_t blocks_per_tile, uint32_t grid) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        Fs4Batch* b = FB(batch);
+        if (queries_per_tile != 4 && queries_per_tile != 8 && queries_per_tile != 16)
+            fail(kBadArgument, "queries_per_tile must be 4, 8 or 16");
+        if (blocks_per_tile == 0 || grid == 0) fail(kBadArgument, "blocks_per_tile and grid must be >= 1");
+        const uint32_t qg = queries_per_tile / 4;
+        if (!fs4_config_fits(qg, b->d))
+            fail(kBadArgument, std::to_string(queries_per_tile) + " queries per tile do not fit dim " + std::to_string(b->d));
+        CtxScope scope(&b->ctx);
+        b->build(Fs4Config{qg, blocks_per_tile, grid});
     });
 }
 

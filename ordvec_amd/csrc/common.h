@@ -236,6 +236,13 @@ constexpr uint32_t kFs4MaxK = 256;
 constexpr uint32_t kFs4MaxDim = 2048;
 
 size_t fs4_scan_lds_bytes(uint32_t qg, uint32_t d);
+// Matrix-core tiles (qg == 4: 16 queries): dims they support, the B-fragment
+// bytes per 16-query set, and the kernel building them from the u8 LUTs
+// (query sets: Fs4Group{first LUT group, number of groups}).
+bool fs4_mfma_ok(uint32_t d);
+size_t fs4_frag_bytes(uint32_t d);
+void launch_fs4_frag(const uint8_t* lut, const Fs4Group* qsets, uint32_t nqsets, uint32_t d, uint32_t* frag,
+                     hipStream_t s);
 void launch_fs4_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t* packed, hipStream_t s);
 void launch_fs4_lut(const float* qhat, uint32_t d, const Fs4Group* groups, uint32_t ngroups, uint8_t* lut,
                     float* bias_invq, hipStream_t s);

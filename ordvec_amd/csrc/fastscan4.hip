@@ -196,6 +196,204 @@ __global__ __launch_bounds__(256) void k_fs4_scan(const Fs4IndexDesc* __restrict
     }
 }
 
+// ---- Matrix-core form (tiles of 16 queries) ----------------------------------
+// The exact integer sum  acc = sum_j lut[j][code_j]  is a dot product over
+// K = (coordinate, code) pairs: documents give one-hot rows (1.0 at the code
+// of each coordinate), queries give their LUT rows.  A u8 LUT entry is split
+// into base-16 digits lo + 16 hi (each 0..15, exact in FP8 e4m3), so one
+// v_mfma_scale_f32_32x32x64_f8f6f4 with FP4 documents (A: 32 documents x 64 K)
+// and FP8 digits (B: 64 K x 32 columns = 16 queries x {lo, hi}) covers four
+// coordinates.  Every product is a digit and every partial sum an integer
+// below 768 * 15, so the f32 accumulation is exact and lo + 16 hi equals the
+// u32 sum of the VALU form bit for bit.
+// K layout of one instruction (lane l, K half h = l / 32): FP4 document slot
+// s (nibble s % 8 of dword s / 8) is coordinate 4 ks + 2 h + s / 16, code
+// s % 16.  The two formats split K differently over the lane halves (FP8 byte
+// b of half h is K = 32 h + b; FP4 slot s of half h is K = 16 (2 (s / 16) + h)
+// + s % 16, measured), so the FP8 byte b of half h carries coordinate
+// 4 ks + 2 (b / 16) + h, code b % 16.  B fragments ([query set][ks][lane][32 bytes]) are built once
+// per batch from the u8 LUTs (k_fs4_frag) and staged through LDS 16 K-steps
+// (32 KiB) at a time; each wave scores four 32-document groups (two 64-doc
+// blocks) against every staged K-step.
+constexpr uint32_t kFs4MfmaThreads = 512;
+constexpr uint32_t kFs4MfmaQT = 16;
+constexpr uint32_t kFs4ChunkKs = 16;                           // K-steps per staged chunk
+constexpr uint32_t kFs4ChunkU4 = kFs4ChunkKs * 64 * 2;         // uint4 per chunk (32 KiB)
+typedef int fs4_v8i __attribute__((ext_vector_type(8)));
+typedef float fs4_v16f __attribute__((ext_vector_type(16)));
+
+// e4m3 encodings of the integers 0..15 (all exact).
+__constant__ uint8_t kE4m3Int[16] = {0x00, 0x38, 0x40, 0x44, 0x48, 0x4A, 0x4C, 0x4E,
+                                     0x50, 0x51, 0x52, 0x53, 0x54, 0x55, 0x56, 0x57};
+
+// Fragments of query set qs = groups [g0, g0 + gn) (up to 16 queries): block
+// (qs, ks), thread t writes dwords t and t + 256 of the 512 (64 lanes x 8).
+__global__ __launch_bounds__(256) void k_fs4_frag(const uint8_t* __restrict__ lut, const Fs4Group* __restrict__ qsets,
+                                                  uint32_t d, uint32_t* __restrict__ frag) {
+    const uint32_t nks = d / 4;
+    const uint32_t qs = blockIdx.x / nks, ks = blockIdx.x % nks;
+    const Fs4Group set = qsets[qs];
+    uint32_t* out = frag + ((uint64_t)qs * nks + ks) * 512;
+    for (uint32_t e = threadIdx.x; e < 512; e += blockDim.x) {
+        const uint32_t lane = e >> 3, dw = e & 7;
+        const uint32_t col = lane & 31, h = lane >> 5;
+        const uint32_t slot = col & 15, hd = col >> 4, gi = slot >> 2, sq = slot & 3;
+        uint32_t word = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t b = dw * 4 + i;
+            // FP8 byte b of K half h is K = 32 h + b, the FP4 document slot
+            // that pairs with it (16-wide K blocks alternate between the
+            // halves for FP4): coordinate 4 ks + 2 (b / 16) + h, code b % 16
+            const uint32_t j = 4 * ks + 2 * (b >> 4) + h, c = b & 15;
+            uint32_t v = 0;
+            if (gi < set.count) v = lut[(((uint64_t)(set.q0 + gi) * d + j) * 16 + c) * 4 + sq];
+            word |= (uint32_t)kE4m3Int[hd ? (v >> 4) : (v & 15)] << (8 * i);
+        }
+        out[e] = word;
+    }
+}
+
+// Make room for r offers in each of the n top-k lists (one barrier unless a
+// list must flush).
+template <class T>
+__device__ void tk_reserve_all(T* tk, uint32_t n, uint32_t k, uint32_t r) {
+    __syncthreads();
+    for (uint32_t s = 0; s < n; s++)
+        if (k + tk[s].nbuf + r > T::kCap) tk_flush(tk[s], k);
+}
+
+// Tiles (index, query set of up to 16 queries = tile.pad, 64-doc blocks
+// [blk0, blk1)); counter as k_fs4_scan.  Outputs as k_fs4_scan with 16 slots.
+__global__ __launch_bounds__(kFs4MfmaThreads, 1) void k_fs4_scan_mfma(
+    const Fs4IndexDesc* __restrict__ indexes, const Fs4Tile* __restrict__ tiles, uint32_t ntiles,
+    uint32_t* __restrict__ counter, const Fs4Group* __restrict__ groups, const uint4* __restrict__ frag,
+    const float* __restrict__ bias_invq, uint32_t d, float inv_norm, uint32_t kcap, uint64_t* __restrict__ partial) {
+    extern __shared__ uint64_t smem[];
+    Fs4TopK* tk = reinterpret_cast<Fs4TopK*>(smem);
+    uint4* sB = reinterpret_cast<uint4*>(tk + kFs4MfmaQT);
+    __shared__ uint32_t s_tile;
+    __shared__ float s_bias[kFs4MfmaQT], s_invq[kFs4MfmaQT];
+    __shared__ uint32_t s_ok[kFs4MfmaQT];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31, h = lane >> 5;
+    const uint32_t wpd = d / 8, nks = d / 4, nchunk = nks / kFs4ChunkKs;
+    for (uint32_t iter = 0;; iter++) {
+        if (counter) {
+            if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+            __syncthreads();
+        } else if (threadIdx.x == 0) {
+            s_tile = iter == 0 ? blockIdx.x : ntiles;
+        }
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= ntiles) break;
+        const Fs4Tile tile = tiles[t];
+        if (threadIdx.x < kFs4MfmaQT) {
+            const uint32_t g = threadIdx.x >> 2, s = threadIdx.x & 3;
+            const bool ok = g < tile.ng && s < groups[tile.g0 + g].count;
+            const uint64_t q = ok ? (uint64_t)groups[tile.g0 + g].q0 + s : 0;
+            s_ok[threadIdx.x] = ok;
+            s_bias[threadIdx.x] = ok ? bias_invq[q * 2 + 0] : 0.0f;
+            s_invq[threadIdx.x] = ok ? bias_invq[q * 2 + 1] : 0.0f;
+        }
+        for (uint32_t s = 0; s < kFs4MfmaQT; s++) tk_init(tk[s], kcap);  // synchronises
+        const Fs4IndexDesc ix = indexes[tile.index];
+        const uint4* fq = frag + (uint64_t)tile.pad * nks * 128;  // 128 uint4 (2 KiB) per K-step
+        for (uint32_t b0 = tile.blk0; b0 < tile.blk1; b0 += 16) {
+            // this wave's blocks b0 + 2 wave + {0, 1}; an invalid block reads
+            // the tile's first block (in bounds) and offers nothing
+            uint32_t blk[2];
+            bool bval[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const uint32_t b = b0 + 2 * wave + i;
+                bval[i] = b < tile.blk1;
+                blk[i] = bval[i] ? b : tile.blk0;
+            }
+            // dword w of group g's document: dp[g][w * 64]
+            const uint32_t* dp[4];
+#pragma unroll
+            for (int g = 0; g < 4; g++) dp[g] = ix.packed + (uint64_t)blk[g >> 1] * wpd * 64 + 32 * (g & 1) + col;
+            fs4_v16f acc[4];
+#pragma unroll
+            for (int g = 0; g < 4; g++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) acc[g][v] = 0.0f;
+            uint4 stage[4];
+            uint32_t xn[4][8];
+#pragma unroll
+            for (int i = 0; i < 4; i++) stage[i] = fq[threadIdx.x + i * kFs4MfmaThreads];
+#pragma unroll
+            for (int g = 0; g < 4; g++)
+#pragma unroll
+                for (int i = 0; i < 8; i++) xn[g][i] = dp[g][(uint64_t)i * 64];
+            for (uint32_t c = 0; c < nchunk; c++) {
+                __syncthreads();  // every wave is done with the previous chunk
+#pragma unroll
+                for (int i = 0; i < 4; i++) sB[threadIdx.x + i * kFs4MfmaThreads] = stage[i];
+                uint32_t xw[4][8];
+#pragma unroll
+                for (int g = 0; g < 4; g++)
+#pragma unroll
+                    for (int i = 0; i < 8; i++) xw[g][i] = xn[g][i] >> (8 * h);
+                __syncthreads();
+                // the next chunk (this one again after the last: in bounds, unused)
+                const uint32_t cn = c + 1 < nchunk ? c + 1 : c;
+#pragma unroll
+                for (int i = 0; i < 4; i++) stage[i] = fq[(uint64_t)cn * kFs4ChunkU4 + threadIdx.x + i * kFs4MfmaThreads];
+#pragma unroll
+                for (int g = 0; g < 4; g++)
+#pragma unroll
+                    for (int i = 0; i < 8; i++) xn[g][i] = dp[g][(uint64_t)(cn * 8 + i) * 64];
+#pragma unroll
+                for (int ks = 0; ks < (int)kFs4ChunkKs; ks++) {
+                    const uint4 q0 = sB[(ks * 64 + lane) * 2], q1 = sB[(ks * 64 + lane) * 2 + 1];
+                    const fs4_v8i bq = {(int)q0.x, (int)q0.y, (int)q0.z, (int)q0.w,
+                                        (int)q1.x, (int)q1.y, (int)q1.z, (int)q1.w};
+#pragma unroll
+                    for (int g = 0; g < 4; g++) {
+                        // byte 2 (ks % 2) + h of dword ks / 2: codes of coordinates
+                        // 4 ks + 2 h (high nibble) and 4 ks + 2 h + 1 (low nibble)
+                        const uint32_t x = xw[g][ks >> 1] >> (16 * (ks & 1));
+                        const uint32_t s1 = (x >> 2) & 0x3cu, s2 = (x << 2) & 0x3cu;
+                        const uint64_t o1 = 2ull << s1, o2 = 2ull << s2;  // FP4 1.0 at nibble = code
+                        const fs4_v8i a = {(int)(uint32_t)o1, (int)(uint32_t)(o1 >> 32), (int)(uint32_t)o2,
+                                           (int)(uint32_t)(o2 >> 32), 0, 0, 0, 0};
+                        acc[g] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, bq, acc[g], 4, 0, 0, 0x7f7f7f7f, 0,
+                                                                                 0x7f7f7f7f);
+                    }
+                }
+            }
+            // lanes with column < 16 hold query col's low digit sums; column
+            // col + 16 (lane ^ 16) the high digit sums
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                tk_reserve_all(tk, kFs4MfmaQT, kcap, 256);
+                const uint32_t b = blk[g >> 1];
+                const bool gv = bval[g >> 1];
+                const bool mine = col < 16 && s_ok[col & 15];
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const float hi = __shfl_xor(acc[g][v], 16);
+                    const uint64_t doc = (uint64_t)b * 64 + 32 * (g & 1) + 8 * (v / 4) + 4 * h + (v % 4);
+                    if (mine && gv && doc < ix.n) {
+                        const float total = __fadd_rn(acc[g][v], __fmul_rn(16.0f, hi));  // exact integer
+                        const float raw = __fadd_rn(s_bias[col], __fmul_rn(total, s_invq[col]));
+                        tk_offer(tk[col], kcap, score_key(__fadd_rn(__fmul_rn(raw, inv_norm), 0.0f), (uint32_t)doc));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t s = 0; s < kFs4MfmaQT; s++) {
+            tk_flush(tk[s], kcap);
+            uint64_t* out = partial + ((uint64_t)t * kFs4MfmaQT + s) * kcap;
+            for (uint32_t j = threadIdx.x; j < kcap; j += blockDim.x) out[j] = tk[s].keys[j];
+        }
+        __syncthreads();
+    }
+}
+
 // Per query: merge its segments' top-kcap lists into the final top-k row
 // (row stride kout; entries past k padded with -inf / -1).
 __global__ __launch_bounds__(256) void k_fs4_merge(const uint64_t* __restrict__ partial, const Fs4Merge* __restrict__ m,
@@ -253,7 +451,21 @@ void check() { OVX_HIP(hipGetLastError()); }
 }  // namespace
 
 size_t fs4_scan_lds_bytes(uint32_t qg, uint32_t d) {
+    if (qg == 4) return (size_t)kFs4MfmaQT * sizeof(Fs4TopK) + (size_t)kFs4ChunkU4 * 16;
     return (size_t)4 * qg * sizeof(Fs4TopK) + (size_t)qg * d * 64;
+}
+
+bool fs4_mfma_ok(uint32_t d) { return d % (4 * kFs4ChunkKs) == 0 && d <= kFs4MaxDim; }
+
+size_t fs4_frag_bytes(uint32_t d) { return (size_t)d * 512; }
+
+void launch_fs4_frag(const uint8_t* lut, const Fs4Group* qsets, uint32_t nqsets, uint32_t d, uint32_t* frag,
+                     hipStream_t s) {
+    if (nqsets == 0) return;
+    ktime_begin(kFamFastScan, s);
+    hipLaunchKernelGGL(k_fs4_frag, dim3(nqsets * (d / 4)), dim3(256), 0, s, lut, qsets, d, frag);
+    check();
+    ktime_end(kFamFastScan, s);
 }
 
 void fs4_configure_lds() {
@@ -272,6 +484,8 @@ void fs4_configure_lds() {
     OVX_FS4_LDS(2, 2);
     OVX_FS4_LDS(2, 1);
 #undef OVX_FS4_LDS
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fs4_scan_mfma),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipGetLastError();
     done = true;
 }
@@ -305,6 +519,13 @@ void launch_fs4_scan(uint32_t qg, uint32_t grid, const Fs4IndexDesc* indexes, co
     if (counter) {
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counter, 1u, 0u);
         check();
+    }
+    if (qg == 4) {  // matrix-core tiles; `lut` holds the B fragments
+        hipLaunchKernelGGL(k_fs4_scan_mfma, dim3(grid), dim3(kFs4MfmaThreads), lds, s, indexes, tiles, ntiles, counter,
+                           groups, reinterpret_cast<const uint4*>(lut), bias_invq, d, inv_norm, kcap, partial);
+        check();
+        ktime_end(kFamFastScan, s);
+        return;
     }
 #define OVX_FS4_SCAN(QG, CH)                                                                                    \
     hipLaunchKernelGGL((k_fs4_scan<QG, CH>), dim3(grid), dim3(256), lds, s, indexes, tiles, ntiles, counter, groups, \

@@ -808,6 +808,14 @@ class FastscanBatch:
         return {"total_queries": tq.value, "kout": ko.value, "queries_per_tile": qpt.value,
                 "blocks64_per_tile": bpt.value, "grid": grid.value, "tiles": nt.value, "tuned_configs": ntu.value}
 
+    def set_config(self, queries_per_tile: int, blocks64_per_tile: int, grid: int) -> dict:
+        """Pin the tile shape (4 or 8 queries per tile: LDS LUT lookups; 16:
+        matrix-core tiles, dim % 64 == 0) and re-capture the graph."""
+        _host_stream()
+        N.check(N.lib().ordvec_fastscan_batch_set_config(self._h, int(queries_per_tile), int(blocks64_per_tile),
+                                                         int(grid)))
+        return self.config()
+
     def tuning(self) -> list:
         rows = []
         for i in range(self.config()["tuned_configs"]):

@@ -13,12 +13,13 @@ pytestmark = pytest.mark.gpu
 def test_plain_b4_matches_oracle(engine, oracle, dim, n):
     gen = S.MixtureNP(dim, seed=dim + n)
     docs = gen.docs(n)
-    qs, _ = gen.queries(docs, 17)
+    qs, _ = gen.queries(docs, 40)
     ix = engine.RankQuantFastscan(dim, bits=4)
     ix.add(docs)
     assert len(ix) == n and ix.bits == 4
     codes = oracle.rq_encode(docs, 4)
-    for nq in (1, 5, 17):
+    # 17 and 40: whole 16-query sets on matrix-core tiles, the rest on LUT tiles
+    for nq in (1, 5, 17, 40):
         for k in (1, 10, 100, 256):
             s, i = ix.search(qs[:nq], k)
             ws, wi = oracle.fs4_search(codes, dim, qs[:nq], k)
@@ -100,6 +101,14 @@ def test_batch_equals_plain_and_oracle(engine, oracle):
             ss, ii = b.split(s, i)[j]
             assert np.array_equal(ii, wi) and np.array_equal(ss.view(np.uint32), ws.view(np.uint32)), j
         r += nq
+    # every tile shape gives the same rows: LDS-LUT tiles of 4 and 8 queries,
+    # matrix-core tiles of 16 (ragged query sets, segments and last blocks)
+    for qpt, bpt, grid in ((16, 16, 256), (16, 5, 64), (16, 256, 512), (8, 32, 256), (4, 7, 100)):
+        cfg = b.set_config(qpt, bpt, grid)
+        assert (cfg["queries_per_tile"], cfg["blocks64_per_tile"], cfg["grid"]) == (qpt, bpt, grid)
+        for use_graph in (True, False):
+            s2, i2 = b.run(queries, use_graph=use_graph)
+            assert np.array_equal(i2, i) and np.array_equal(s2.view(np.uint32), s.view(np.uint32)), (qpt, bpt, grid)
 
 
 def test_graph_replays_eager_runs_and_autotune(engine):
@@ -123,7 +132,9 @@ def test_graph_replays_eager_runs_and_autotune(engine):
         (x @ x).sum().item()
         torch.cuda.synchronize()
     rows = b.autotune(queries, reps=1)
-    assert len(rows) == 30 and all(r["ms"] > 0 for r in rows)
+    # 30 LDS-LUT shapes + 10 matrix-core shapes (dim % 64 == 0)
+    assert len(rows) == 40 and all(r["ms"] > 0 for r in rows)
+    assert sum(r["queries_per_tile"] == 16 for r in rows) == 10
     best = min(rows, key=lambda r: r["ms"])
     cfg = b.config()
     assert (cfg["queries_per_tile"], cfg["blocks64_per_tile"], cfg["grid"]) == (
@@ -156,6 +167,13 @@ def test_batch_validation(engine):
     bt = engine.FastscanBatch([a], [2], [10])
     with pytest.raises(ValueError):
         bt.run(S.iid_unit(3, 64, seed=3))
+    for bad_cfg in ((12, 16, 64), (16, 0, 64), (16, 16, 0)):
+        with pytest.raises(ValueError):
+            bt.set_config(*bad_cfg)
+    d72 = engine.RankQuantFastscan(72, bits=4)  # matrix-core tiles need dim % 64 == 0
+    d72.add(S.iid_unit(100, 72, seed=1))
+    with pytest.raises(ValueError):
+        engine.FastscanBatch([d72], [2], [10]).set_config(16, 16, 64)
     bad = S.iid_unit(2, 64, seed=3)
     bad[0, 0] = np.nan
     with pytest.raises(ValueError):
