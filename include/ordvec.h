@@ -378,9 +378,12 @@ ordvec_status_t ordvec_fastscan_batch_config(ordvec_fastscan_batch_t *batch, uin
                                              uint32_t *queries_per_tile, uint32_t *blocks_per_tile, uint32_t *grid,
                                              uint64_t *n_tiles, uint64_t *n_tuned);
 /* Pins the tile shape (no reference counterpart): queries_per_tile 4 or 8
- * (LDS LUT lookups) or 16 (matrix-core tiles, dim % 64 == 0), 64-document
+ * (LDS LUT lookups), 16 (matrix-core tiles, dim % 64 == 0; each index's last
+ * query set padded to 16) or 20 / 24 (whole 16-query sets on the matrix
+ * cores, each index's remaining queries on LUT tiles of 4 / 8), 64-document
  * blocks per tile >= 1, persistent grid >= 1; re-captures the graph.
- * BAD_ARGUMENT for a shape this dim cannot run. */
+ * BAD_ARGUMENT for a shape this dim cannot run.  config() and tuning report
+ * queries_per_tile in the same encoding. */
 ordvec_status_t ordvec_fastscan_batch_set_config(ordvec_fastscan_batch_t *batch, uint32_t queries_per_tile,
                                                  uint32_t blocks_per_tile, uint32_t grid);
 /* Row i of the last autotune: the configuration and its best time (ms). */

@@ -1077,15 +1077,30 @@ struct Fs4Config {
     uint32_t qg = 1;          // LUT groups of four queries per tile (1 or 2; 4: matrix-core tiles)
     uint32_t seg_blocks = 64; // 64-document blocks per tile
     uint32_t grid = 512;      // persistent workgroups
+    uint32_t rem_qg = 0;      // qg == 4: 0 pads each index's last query set to 16; 1 or 2 runs
+                              // only whole 16-query sets on the matrix cores and the rest on
+                              // LUT tiles of 4 * rem_qg queries
+    // queries_per_tile as the C ABI reports it: 4, 8, 16, or 20 / 24 for the mixed form
+    uint32_t qpt() const { return 4 * qg + 4 * rem_qg; }
 };
+Fs4Config fs4_config_of(uint32_t qpt, uint32_t seg_blocks, uint32_t grid) {
+    Fs4Config c;
+    c.qg = qpt >= 16 ? 4 : qpt / 4;
+    c.rem_qg = qpt > 16 ? (qpt - 16) / 4 : 0;
+    c.seg_blocks = seg_blocks;
+    c.grid = grid;
+    return c;
+}
 
 struct Fs4Tables {
     std::vector<Fs4IndexDesc> idx;
     std::vector<Fs4Group> groups;
-    std::vector<Fs4Tile> tiles;
-    std::vector<Fs4Merge> merges;
+    std::vector<Fs4Tile> tiles;   // tiles of qg (matrix-core tiles when qg == 4)
+    std::vector<Fs4Tile> tiles2;  // mixed form: LUT tiles of rem_qg for each index's last queries
+    std::vector<Fs4Merge> merges; // one per query, in query order
     std::vector<Fs4Group> qsets;  // matrix-core tiles: {first group, groups} per query set
     uint32_t nq = 0, kcap = 1;
+    uint64_t rows1 = 0, rows2 = 0;  // partial rows of tiles / tiles2 (tiles2's follow tiles')
 };
 
 // Queries of index i are rows [q_base_i, q_base_i + nq_i) of the batch.
@@ -1093,6 +1108,7 @@ Fs4Tables fs4_tables(const std::vector<const FsIndex*>& ixs, const std::vector<u
                      const std::vector<uint32_t>& k_per, const Fs4Config& c) {
     Fs4Tables t;
     uint32_t q_base = 0;
+    std::vector<bool> second;  // merges whose rows index tiles2
     for (size_t i = 0; i < ixs.size(); i++) {
         const FsIndex* ix = ixs[i];
         t.idx.push_back(Fs4IndexDesc{ix->packed4, ix->n});
@@ -1104,21 +1120,31 @@ Fs4Tables fs4_tables(const std::vector<const FsIndex*>& ixs, const std::vector<u
         const uint32_t ng = (uint32_t)t.groups.size() - g_first;
         const uint32_t nblk = ix->blocks64();
         const uint32_t nseg = nblk ? (nblk + c.seg_blocks - 1) / c.seg_blocks : 0;
-        for (uint32_t gs = 0; gs < ng; gs += c.qg) {
-            const uint32_t t0 = (uint32_t)t.tiles.size();
-            const uint32_t gn = std::min(c.qg, ng - gs);
+        // groups [0, ng1) go in tiles of qg groups, the rest in tiles2 of rem_qg
+        const uint32_t ng1 = c.qg == 4 && c.rem_qg ? nq / 16 * 4 : ng;
+        auto emit = [&](uint32_t gs, uint32_t gn, uint32_t qg, bool two) {
+            std::vector<Fs4Tile>& tl = two ? t.tiles2 : t.tiles;
+            const uint32_t t0 = (uint32_t)tl.size();
             const uint32_t qset = (uint32_t)t.qsets.size();
-            if (c.qg == 4) t.qsets.push_back(Fs4Group{g_first + gs, gn});
+            if (qg == 4) t.qsets.push_back(Fs4Group{g_first + gs, gn});
             for (uint32_t seg = 0; seg < nseg; seg++)
-                t.tiles.push_back(Fs4Tile{(uint32_t)i, g_first + gs, gn, seg * c.seg_blocks,
-                                          std::min(nblk, (seg + 1) * c.seg_blocks), qset});
+                tl.push_back(Fs4Tile{(uint32_t)i, g_first + gs, gn, seg * c.seg_blocks,
+                                     std::min(nblk, (seg + 1) * c.seg_blocks), qg == 4 ? qset : 0u});
             for (uint32_t g = 0; g < gn; g++)
-                for (uint32_t s = 0; s < t.groups[g_first + gs + g].count; s++)
-                    t.merges.push_back(Fs4Merge{t0, nseg, g * 4 + s, nseg ? k : 0});
-        }
+                for (uint32_t s = 0; s < t.groups[g_first + gs + g].count; s++) {
+                    t.merges.push_back(Fs4Merge{t0 * 4 * qg + g * 4 + s, nseg, 4 * qg, nseg ? k : 0});
+                    second.push_back(two);
+                }
+        };
+        for (uint32_t gs = 0; gs < ng1; gs += c.qg) emit(gs, std::min(c.qg, ng1 - gs), c.qg, false);
+        for (uint32_t gs = ng1; gs < ng; gs += c.rem_qg) emit(gs, std::min(c.rem_qg, ng - gs), c.rem_qg, true);
         q_base += nq;
     }
     t.nq = q_base;
+    t.rows1 = (uint64_t)t.tiles.size() * 4 * c.qg;
+    t.rows2 = (uint64_t)t.tiles2.size() * 4 * (c.rem_qg ? c.rem_qg : 1);
+    for (size_t q = 0; q < t.merges.size(); q++)
+        if (second[q]) t.merges[q].prow += (uint32_t)t.rows1;
     return t;
 }
 
@@ -1128,6 +1154,7 @@ struct Fs4Dev {
     Fs4IndexDesc* idx = nullptr;
     Fs4Group* groups = nullptr;
     Fs4Tile* tiles = nullptr;
+    Fs4Tile* tiles2 = nullptr;
     Fs4Merge* merges = nullptr;
     float *dq = nullptr, *qhat = nullptr, *off = nullptr, *bq = nullptr, *out_s = nullptr;
     uint32_t *qstat = nullptr, *counter = nullptr;
@@ -1137,14 +1164,17 @@ struct Fs4Dev {
     uint64_t* partial = nullptr;
     int64_t* out_i = nullptr;
     size_t partial_bytes = 0;
-    uint32_t ntiles = 0, ngroups = 0, nqsets = 0, nq = 0, kcap = 1;
+    uint64_t rows1 = 0;
+    uint32_t ntiles = 0, ntiles2 = 0, ngroups = 0, nqsets = 0, nq = 0, kcap = 1;
 };
 
 size_t fs4_dev_bytes(const Fs4Tables& t, uint32_t d, uint32_t qg, size_t partial_floor) {
+    (void)qg;
     const uint64_t nq = t.nq;
-    const size_t part = std::max<size_t>(partial_floor, (size_t)t.tiles.size() * 4 * qg * t.kcap * 8);
+    const size_t part = std::max<size_t>(partial_floor, (size_t)(t.rows1 + t.rows2) * t.kcap * 8);
     return al(t.idx.size() * sizeof(Fs4IndexDesc)) + al(t.groups.size() * sizeof(Fs4Group)) +
-           al(t.tiles.size() * sizeof(Fs4Tile)) + al(t.merges.size() * sizeof(Fs4Merge)) + 3 * al(nq * d * 4) +
+           al(t.tiles.size() * sizeof(Fs4Tile)) + al(t.tiles2.size() * sizeof(Fs4Tile)) +
+           al(t.merges.size() * sizeof(Fs4Merge)) + 3 * al(nq * d * 4) +
            al(nq * 4) * 2 + al(nq * 8) + al(t.groups.size() * (uint64_t)d * 64) + al(part) + al(256) +
            al(nq * t.kcap * 4) + al(nq * t.kcap * 8) + al(t.qsets.size() * sizeof(Fs4Group)) +
            al(t.qsets.size() * fs4_frag_bytes(d));
@@ -1158,6 +1188,7 @@ void fs4_upload(Fs4Dev& dv, const Fs4Tables& t, uint32_t d, uint32_t qg, size_t 
     dv.idx = cv.take<Fs4IndexDesc>(t.idx.size());
     dv.groups = cv.take<Fs4Group>(t.groups.size());
     dv.tiles = cv.take<Fs4Tile>(t.tiles.size());
+    dv.tiles2 = cv.take<Fs4Tile>(t.tiles2.size());
     dv.merges = cv.take<Fs4Merge>(t.merges.size());
     dv.dq = cv.take<float>(nq * d);
     dv.qhat = cv.take<float>(nq * d);
@@ -1166,7 +1197,9 @@ void fs4_upload(Fs4Dev& dv, const Fs4Tables& t, uint32_t d, uint32_t qg, size_t 
     dv.qstat = cv.take<uint32_t>(nq);
     dv.bq = cv.take<float>(nq * 2);
     dv.lut = cv.take<uint8_t>(t.groups.size() * (uint64_t)d * 64);
-    dv.partial_bytes = std::max<size_t>(partial_floor, (size_t)t.tiles.size() * 4 * qg * t.kcap * 8);
+    (void)qg;
+    dv.partial_bytes = std::max<size_t>(partial_floor, (size_t)(t.rows1 + t.rows2) * t.kcap * 8);
+    dv.rows1 = t.rows1;
     dv.partial = cv.take<uint64_t>(dv.partial_bytes / 8);
     dv.counter = cv.take<uint32_t>(64);
     dv.out_s = cv.take<float>(nq * t.kcap);
@@ -1178,6 +1211,8 @@ void fs4_upload(Fs4Dev& dv, const Fs4Tables& t, uint32_t d, uint32_t qg, size_t 
     h2d(dv.idx, t.idx.data(), t.idx.size() * sizeof(Fs4IndexDesc));
     h2d(dv.groups, t.groups.data(), t.groups.size() * sizeof(Fs4Group));
     h2d(dv.tiles, t.tiles.data(), t.tiles.size() * sizeof(Fs4Tile));
+    if (!t.tiles2.empty()) h2d(dv.tiles2, t.tiles2.data(), t.tiles2.size() * sizeof(Fs4Tile));
+    dv.ntiles2 = (uint32_t)t.tiles2.size();
     h2d(dv.merges, t.merges.data(), t.merges.size() * sizeof(Fs4Merge));
     dv.ntiles = (uint32_t)t.tiles.size();
     dv.ngroups = (uint32_t)t.groups.size();
@@ -1198,12 +1233,19 @@ void fs4_enqueue(const Fs4Dev& dv, uint32_t d, const Fs4Config& c, bool persiste
     const uint8_t* tables = c.qg == 4 ? reinterpret_cast<const uint8_t*>(dv.frag) : dv.lut;
     launch_fs4_scan(c.qg, persistent ? c.grid : dv.ntiles, dv.idx, dv.tiles, dv.ntiles,
                     persistent ? dv.counter : nullptr, dv.groups, tables, dv.bq, d, inv_norm, dv.kcap, dv.partial, s);
-    launch_fs4_merge(dv.partial, dv.merges, dv.nq, 4 * c.qg, dv.kcap, dv.kcap, dv.out_s, dv.out_i, s);
+    if (dv.ntiles2)  // mixed form: each index's last queries on LUT tiles
+        launch_fs4_scan(c.rem_qg, persistent ? c.grid : dv.ntiles2, dv.idx, dv.tiles2, dv.ntiles2,
+                        persistent ? dv.counter + 32 : nullptr, dv.groups, dv.lut, dv.bq, d, inv_norm, dv.kcap,
+                        dv.partial + dv.rows1 * dv.kcap, s);
+    launch_fs4_merge(dv.partial, dv.merges, dv.nq, dv.kcap, dv.kcap, dv.out_s, dv.out_i, s);
 }
 
 bool fs4_config_fits(uint32_t qg, uint32_t d) {
     if (qg == 4 && !fs4_mfma_ok(d)) return false;
     return fs4_scan_lds_bytes(qg, d) + 1024 <= 160 * 1024;
+}
+bool fs4_config_fits(const Fs4Config& c, uint32_t d) {
+    return fs4_config_fits(c.qg, d) && (c.rem_qg == 0 || fs4_config_fits(c.rem_qg, d));
 }
 
 // Plain search of one b=4 index (nq queries, top-k).
@@ -3212,11 +3254,18 @@ ordvec_status_t ordvec_fastscan_batch_autotune(ordvec_fastscan_batch_t* batch, c
         for (uint32_t qg : {1u, 2u}) {
             if (!fs4_config_fits(qg, b->d)) continue;
             for (uint32_t sb : {16u, 32u, 64u, 128u, 256u})
-                for (uint32_t gm : {1u, 2u, 4u}) cands.push_back(Fs4Config{qg, sb, cus * gm});
+                for (uint32_t gm : {1u, 2u, 4u}) cands.push_back(fs4_config_of(4 * qg, sb, cus * gm));
         }
-        if (fs4_config_fits(4, b->d))  // matrix-core tiles: one 512-thread workgroup per CU
+        if (fs4_config_fits(4, b->d)) {  // matrix-core tiles: one 512-thread workgroup per CU
             for (uint32_t sb : {16u, 32u, 64u, 128u, 256u})
-                for (uint32_t gm : {1u, 2u}) cands.push_back(Fs4Config{4, sb, cus * gm});
+                for (uint32_t gm : {1u, 2u}) cands.push_back(fs4_config_of(16, sb, cus * gm));
+            // mixed: whole 16-query sets on the matrix cores, the rest on LUT tiles
+            for (uint32_t qpt : {20u, 24u}) {
+                if (!fs4_config_fits(fs4_config_of(qpt, 64, cus), b->d)) continue;
+                for (uint32_t sb : {64u, 128u, 256u})
+                    for (uint32_t gm : {1u, 2u}) cands.push_back(fs4_config_of(qpt, sb, cus * gm));
+            }
+        }
         size_t bytes = 0;
         for (const Fs4Config& c : cands) bytes = std::max(bytes, fs4_dev_bytes(fs4_tables(b->ixs, b->nq_per, b->k_per, c), b->d, c.qg, 0));
         b->drop_graph();
@@ -3303,7 +3352,7 @@ ordvec_status_t ordvec_fastscan_batch_config(ordvec_fastscan_batch_t* batch, uin
         Fs4Batch* b = FB(batch);
         if (total_queries) *total_queries = b->dv.nq;
         if (kout) *kout = b->dv.kcap;
-        if (queries_per_tile) *queries_per_tile = 4 * b->cfg.qg;
+        if (queries_per_tile) *queries_per_tile = b->cfg.qpt();
         if (blocks_per_tile) *blocks_per_tile = b->cfg.seg_blocks;
         if (grid) *grid = b->cfg.grid;
         if (n_tiles) *n_tiles = b->dv.ntiles;
@@ -3316,14 +3365,15 @@ ordvec_status_t ordvec_fastscan_batch_set_config(ordvec_fastscan_batch_t* batch,
     return boundary([&] {
         NONNULL(batch, "batch");
         Fs4Batch* b = FB(batch);
-        if (queries_per_tile != 4 && queries_per_tile != 8 && queries_per_tile != 16)
-            fail(kBadArgument, "queries_per_tile must be 4, 8 or 16");
+        if (queries_per_tile != 4 && queries_per_tile != 8 && queries_per_tile != 16 && queries_per_tile != 20 &&
+            queries_per_tile != 24)
+            fail(kBadArgument, "queries_per_tile must be 4, 8, 16, 20 or 24");
         if (blocks_per_tile == 0 || grid == 0) fail(kBadArgument, "blocks_per_tile and grid must be >= 1");
-        const uint32_t qg = queries_per_tile / 4;
-        if (!fs4_config_fits(qg, b->d))
+        const Fs4Config c = fs4_config_of(queries_per_tile, blocks_per_tile, grid);
+        if (!fs4_config_fits(c, b->d))
             fail(kBadArgument, std::to_string(queries_per_tile) + " queries per tile do not fit dim " + std::to_string(b->d));
         CtxScope scope(&b->ctx);
-        b->build(Fs4Config{qg, blocks_per_tile, grid});
+        b->build(c);
     });
 }
 
@@ -3334,7 +3384,7 @@ ordvec_status_t ordvec_fastscan_batch_tuning(ordvec_fastscan_batch_t* batch, uin
         Fs4Batch* b = FB(batch);
         if (i >= b->tuning.size()) fail(kBadArgument, "tuning row out of range");
         const Fs4TuneRow& r = b->tuning[i];
-        if (queries_per_tile) *queries_per_tile = 4 * r.c.qg;
+        if (queries_per_tile) *queries_per_tile = r.c.qpt();
         if (blocks_per_tile) *blocks_per_tile = r.c.seg_blocks;
         if (grid) *grid = r.c.grid;
         if (ms) *ms = r.ms;

@@ -228,9 +228,9 @@ struct Fs4Group {
 struct Fs4Tile {
     uint32_t index, g0, ng, blk0, blk1, pad;
 };
-// Per query: its partial top-k rows are ((t0 + seg) * 4*QG + slot) for seg < nseg.
+// Per query: its partial top-k rows are prow + seg * stride for seg < nseg.
 struct Fs4Merge {
-    uint32_t t0, nseg, slot, k;
+    uint32_t prow, nseg, stride, k;
 };
 constexpr uint32_t kFs4MaxK = 256;
 constexpr uint32_t kFs4MaxDim = 2048;
@@ -251,7 +251,7 @@ void launch_fs4_lut(const float* qhat, uint32_t d, const Fs4Group* groups, uint3
 void launch_fs4_scan(uint32_t qg, uint32_t grid, const Fs4IndexDesc* indexes, const Fs4Tile* tiles, uint32_t ntiles,
                      uint32_t* counter, const Fs4Group* groups, const uint8_t* lut, const float* bias_invq, uint32_t d,
                      float inv_norm, uint32_t kcap, uint64_t* partial, hipStream_t s);
-void launch_fs4_merge(const uint64_t* partial, const Fs4Merge* merges, uint32_t nq, uint32_t qt, uint32_t kcap,
+void launch_fs4_merge(const uint64_t* partial, const Fs4Merge* merges, uint32_t nq, uint32_t kcap,
                       uint32_t kout, float* out_s, int64_t* out_i, hipStream_t s);
 
 }  // namespace ovx

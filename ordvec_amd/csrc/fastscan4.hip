@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kFs4MfmaThreads, 1) void k_fs4_scan_mfma(
 // Per query: merge its segments' top-kcap lists into the final top-k row
 // (row stride kout; entries past k padded with -inf / -1).
 __global__ __launch_bounds__(256) void k_fs4_merge(const uint64_t* __restrict__ partial, const Fs4Merge* __restrict__ m,
-                                                   uint32_t qt, uint32_t kcap, uint32_t kout, float* __restrict__ out_s,
+                                                   uint32_t kcap, uint32_t kout, float* __restrict__ out_s,
                                                    int64_t* __restrict__ out_i) {
     __shared__ TopKShared tk;
     const uint64_t q = blockIdx.x;
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(256) void k_fs4_merge(const uint64_t* __restrict__ 
         const uint64_t i = r0 + threadIdx.x;
         if (i < total) {
             const uint64_t seg = i / kcap, j = i - seg * kcap;
-            const uint64_t key = partial[((mq.t0 + seg) * qt + mq.slot) * kcap + j];
+            const uint64_t key = partial[((uint64_t)mq.prow + seg * mq.stride) * kcap + j];
             if (key != kKeyMax) tk_offer(tk, k, key);
         }
     }
@@ -547,11 +547,11 @@ void launch_fs4_scan(uint32_t qg, uint32_t grid, const Fs4IndexDesc* indexes, co
     ktime_end(kFamFastScan, s);
 }
 
-void launch_fs4_merge(const uint64_t* partial, const Fs4Merge* merges, uint32_t nq, uint32_t qt, uint32_t kcap,
+void launch_fs4_merge(const uint64_t* partial, const Fs4Merge* merges, uint32_t nq, uint32_t kcap,
                       uint32_t kout, float* out_s, int64_t* out_i, hipStream_t s) {
     if (nq == 0) return;
     ktime_begin(kFamFastScan, s);
-    hipLaunchKernelGGL(k_fs4_merge, dim3(nq), dim3(256), 0, s, partial, merges, qt, kcap, kout, out_s, out_i);
+    hipLaunchKernelGGL(k_fs4_merge, dim3(nq), dim3(256), 0, s, partial, merges, kcap, kout, out_s, out_i);
     check();
     ktime_end(kFamFastScan, s);
 }

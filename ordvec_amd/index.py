@@ -810,7 +810,9 @@ class FastscanBatch:
 
     def set_config(self, queries_per_tile: int, blocks64_per_tile: int, grid: int) -> dict:
         """Pin the tile shape (4 or 8 queries per tile: LDS LUT lookups; 16:
-        matrix-core tiles, dim % 64 == 0) and re-capture the graph."""
+        matrix-core tiles, dim % 64 == 0; 20 / 24: whole 16-query sets on the
+        matrix cores and each index's remaining queries on LUT tiles of 4 / 8)
+        and re-capture the graph."""
         _host_stream()
         N.check(N.lib().ordvec_fastscan_batch_set_config(self._h, int(queries_per_tile), int(blocks64_per_tile),
                                                          int(grid)))

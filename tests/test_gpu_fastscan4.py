@@ -103,7 +103,10 @@ def test_batch_equals_plain_and_oracle(engine, oracle):
         r += nq
     # every tile shape gives the same rows: LDS-LUT tiles of 4 and 8 queries,
     # matrix-core tiles of 16 (ragged query sets, segments and last blocks)
-    for qpt, bpt, grid in ((16, 16, 256), (16, 5, 64), (16, 256, 512), (8, 32, 256), (4, 7, 100)):
+    # and the mixed form (20 / 24: whole 16-query sets on the matrix cores, the
+    # rest on LUT tiles of 4 / 8)
+    for qpt, bpt, grid in ((16, 16, 256), (16, 5, 64), (16, 256, 512), (8, 32, 256), (4, 7, 100), (20, 64, 256),
+                           (24, 13, 128), (20, 256, 512)):
         cfg = b.set_config(qpt, bpt, grid)
         assert (cfg["queries_per_tile"], cfg["blocks64_per_tile"], cfg["grid"]) == (qpt, bpt, grid)
         for use_graph in (True, False):
@@ -132,9 +135,10 @@ def test_graph_replays_eager_runs_and_autotune(engine):
         (x @ x).sum().item()
         torch.cuda.synchronize()
     rows = b.autotune(queries, reps=1)
-    # 30 LDS-LUT shapes + 10 matrix-core shapes (dim % 64 == 0)
-    assert len(rows) == 40 and all(r["ms"] > 0 for r in rows)
+    # 30 LDS-LUT shapes + 10 matrix-core shapes (dim % 64 == 0) + 12 mixed
+    assert len(rows) == 52 and all(r["ms"] > 0 for r in rows)
     assert sum(r["queries_per_tile"] == 16 for r in rows) == 10
+    assert sum(r["queries_per_tile"] in (20, 24) for r in rows) == 12
     best = min(rows, key=lambda r: r["ms"])
     cfg = b.config()
     assert (cfg["queries_per_tile"], cfg["blocks64_per_tile"], cfg["grid"]) == (
@@ -167,7 +171,7 @@ def test_batch_validation(engine):
     bt = engine.FastscanBatch([a], [2], [10])
     with pytest.raises(ValueError):
         bt.run(S.iid_unit(3, 64, seed=3))
-    for bad_cfg in ((12, 16, 64), (16, 0, 64), (16, 16, 0)):
+    for bad_cfg in ((12, 16, 64), (28, 16, 64), (16, 0, 64), (16, 16, 0)):
         with pytest.raises(ValueError):
             bt.set_config(*bad_cfg)
     d72 = engine.RankQuantFastscan(72, bits=4)  # matrix-core tiles need dim % 64 == 0
