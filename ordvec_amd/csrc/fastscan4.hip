@@ -211,8 +211,9 @@ __global__ __launch_bounds__(256) void k_fs4_scan(const Fs4IndexDesc* __restrict
 // s % 16.  The two formats split K differently over the lane halves (FP8 byte
 // b of half h is K = 32 h + b; FP4 slot s of half h is K = 16 (2 (s / 16) + h)
 // + s % 16, measured), so the FP8 byte b of half h carries coordinate
-// 4 ks + 2 (b / 16) + h, code b % 16.  B fragments ([query set][ks][lane][32 bytes]) are built once
-// per batch from the u8 LUTs (k_fs4_frag) and staged through LDS 16 K-steps
+// 4 ks + 2 (b / 16) + h, code b % 16.  B fragments ([query set][ks][half of
+// the 32 bytes][lane][16 bytes]) are built once per batch from the u8 LUTs
+// (k_fs4_frag) and staged through LDS 16 K-steps
 // (32 KiB) at a time; each wave scores four 32-document groups (two 64-doc
 // blocks) against every staged K-step.
 constexpr uint32_t kFs4MfmaThreads = 512;
@@ -227,15 +228,18 @@ __constant__ uint8_t kE4m3Int[16] = {0x00, 0x38, 0x40, 0x44, 0x48, 0x4A, 0x4C, 0
                                      0x50, 0x51, 0x52, 0x53, 0x54, 0x55, 0x56, 0x57};
 
 // Fragments of query set qs = groups [g0, g0 + gn) (up to 16 queries): block
-// (qs, ks), thread t writes dwords t and t + 256 of the 512 (64 lanes x 8).
+// (qs, ks), thread t writes dwords t and t + 256 of the 512 (2 halves x 64
+// lanes x 4).
 __global__ __launch_bounds__(256) void k_fs4_frag(const uint8_t* __restrict__ lut, const Fs4Group* __restrict__ qsets,
                                                   uint32_t d, uint32_t* __restrict__ frag) {
     const uint32_t nks = d / 4;
     const uint32_t qs = blockIdx.x / nks, ks = blockIdx.x % nks;
     const Fs4Group set = qsets[qs];
+    // K-step layout [half][lane][16 bytes]: a ds_read_b128 of one half reads
+    // 16 consecutive bytes per lane (no LDS bank conflicts)
     uint32_t* out = frag + ((uint64_t)qs * nks + ks) * 512;
     for (uint32_t e = threadIdx.x; e < 512; e += blockDim.x) {
-        const uint32_t lane = e >> 3, dw = e & 7;
+        const uint32_t half = e >> 8, lane = (e >> 2) & 63, dw = 4 * half + (e & 3);
         const uint32_t col = lane & 31, h = lane >> 5;
         const uint32_t slot = col & 15, hd = col >> 4, gi = slot >> 2, sq = slot & 3;
         uint32_t word = 0;
@@ -347,7 +351,7 @@ __global__ __launch_bounds__(kFs4MfmaThreads, 1) void k_fs4_scan_mfma(
                     for (int i = 0; i < 8; i++) xn[g][i] = dp[g][(uint64_t)(cn * 8 + i) * 64];
 #pragma unroll
                 for (int ks = 0; ks < (int)kFs4ChunkKs; ks++) {
-                    const uint4 q0 = sB[(ks * 64 + lane) * 2], q1 = sB[(ks * 64 + lane) * 2 + 1];
+                    const uint4 q0 = sB[ks * 128 + lane], q1 = sB[ks * 128 + 64 + lane];
                     const fs4_v8i bq = {(int)q0.x, (int)q0.y, (int)q0.z, (int)q0.w,
                                         (int)q1.x, (int)q1.y, (int)q1.z, (int)q1.w};
 #pragma unroll
