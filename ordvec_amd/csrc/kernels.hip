@@ -34,37 +34,40 @@ __global__ void k_check_finite(const float* __restrict__ x, uint64_t count, uint
 }
 
 // Sequential f32 sum of squares / of values over an LDS chunk by one thread
-// (the reference folds left to right starting at -0.0).
-__device__ __forceinline__ float seq_sum_sq(const float* xs, uint32_t len, float s) {
+// (the reference folds left to right starting at -0.0).  The next 32 values
+// are read from LDS while the current 32 are folded, so the fold runs at the
+// add chain's latency rather than LDS latency plus the chain.
+template <bool SQ>
+__device__ __forceinline__ float seq_fold(const float* xs, uint32_t len, float s) {
     uint32_t i = 0;
-    for (; i + 32 <= len; i += 32) {
-        float4 v[8];
+    if (len >= 32) {
+        float4 cur[8], nxt[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) v[j] = *reinterpret_cast<const float4*>(xs + i + 4 * j);
+        for (int j = 0; j < 8; j++) cur[j] = *reinterpret_cast<const float4*>(xs + 4 * j);
+        for (; i + 32 <= len; i += 32) {
+            const uint32_t in = i + 64 <= len ? i + 32 : i;  // in bounds; unused after the last block
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            s = __fadd_rn(s, __fmul_rn(v[j].x, v[j].x));
-            s = __fadd_rn(s, __fmul_rn(v[j].y, v[j].y));
-            s = __fadd_rn(s, __fmul_rn(v[j].z, v[j].z));
-            s = __fadd_rn(s, __fmul_rn(v[j].w, v[j].w));
+            for (int j = 0; j < 8; j++) nxt[j] = *reinterpret_cast<const float4*>(xs + in + 4 * j);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if (SQ) {
+                    s = __fadd_rn(s, __fmul_rn(cur[j].x, cur[j].x));
+                    s = __fadd_rn(s, __fmul_rn(cur[j].y, cur[j].y));
+                    s = __fadd_rn(s, __fmul_rn(cur[j].z, cur[j].z));
+                    s = __fadd_rn(s, __fmul_rn(cur[j].w, cur[j].w));
+                } else {
+                    s = __fadd_rn(__fadd_rn(__fadd_rn(__fadd_rn(s, cur[j].x), cur[j].y), cur[j].z), cur[j].w);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) cur[j] = nxt[j];
         }
     }
-    for (; i < len; i++) s = __fadd_rn(s, __fmul_rn(xs[i], xs[i]));
+    for (; i < len; i++) s = SQ ? __fadd_rn(s, __fmul_rn(xs[i], xs[i])) : __fadd_rn(s, xs[i]);
     return s;
 }
-
-__device__ __forceinline__ float seq_sum(const float* xs, uint32_t len, float s) {
-    uint32_t i = 0;
-    for (; i + 32 <= len; i += 32) {
-        float4 v[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) v[j] = *reinterpret_cast<const float4*>(xs + i + 4 * j);
-#pragma unroll
-        for (int j = 0; j < 8; j++) s = __fadd_rn(__fadd_rn(__fadd_rn(__fadd_rn(s, v[j].x), v[j].y), v[j].z), v[j].w);
-    }
-    for (; i < len; i++) s = __fadd_rn(s, xs[i]);
-    return s;
-}
+__device__ __forceinline__ float seq_sum_sq(const float* xs, uint32_t len, float s) { return seq_fold<true>(xs, len, s); }
+__device__ __forceinline__ float seq_sum(const float* xs, uint32_t len, float s) { return seq_fold<false>(xs, len, s); }
 
 // One workgroup per query: q_hat = x * (1/||x||) (zeros when ||x|| <= 1e-12)
 // and the SIMD tiers' TopK offset -centre * sum(q_hat) / R.  The two sums are
