@@ -2691,7 +2691,8 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
 
 uint32_t rerank_nsplit(uint64_t nq, uint64_t avg_m) {
     if (nq >= 128) return 1;
-    uint64_t want = (avg_m + 63) / 64;
+    // one round (128 candidates: two per lane quad) per split workgroup
+    uint64_t want = (avg_m + 127) / 128;
     uint64_t cap = std::max<uint64_t>(1, 512 / std::max<uint64_t>(1, nq));
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({want, cap, 32}));
 }
