@@ -157,9 +157,11 @@ def main():
     # ---- plain (one query per call) ------------------------------------------------
     lat = []
     P = plain.shape[0]
-    for i in range(min(20, P)):
-        ov.two_stage_search(sb, rq, plain[i:i + 1], m, k)
+    # untimed pass that keeps its results like the timed pass does, so the
+    # caching allocator already holds the segments the kept ids need
+    warm_ids = [ov.two_stage_search(sb, rq, plain[i:i + 1], m, k)[1] for i in range(P)]
     torch.cuda.synchronize()
+    del warm_ids
     tp0 = time.perf_counter()
     plain_ids = []
     for i in range(P):
@@ -171,7 +173,9 @@ def main():
     tp1 = time.perf_counter()
     lat_ms = np.array(lat) * 1e3
     plain_qps = P / (tp1 - tp0)
-    log(f"plain: {plain_qps:,.0f} QPS, p50 {np.percentile(lat_ms, 50):.3f} ms, p99 {np.percentile(lat_ms, 99):.3f} ms")
+    log(f"plain: {plain_qps:,.0f} QPS, p50 {np.percentile(lat_ms, 50):.3f} ms, p99 {np.percentile(lat_ms, 99):.3f} ms, "
+        f"mean {lat_ms.mean():.3f} ms, max {lat_ms.max():.3f} ms; "
+        f"slowest calls (index: ms) {[(int(j), round(float(lat_ms[j]), 2)) for j in np.argsort(lat_ms)[-5:]]}")
 
     # consistency: a query alone and inside a batch gives the same result
     bi = torch.cat([p for p in plain_ids[:8]]).cpu().numpy()
@@ -238,7 +242,8 @@ def main():
         "step_hbm_gbs": round(step_gbs, 1),
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "plain": {"qps": round(plain_qps, 1), "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
-                  "p99_ms": round(float(np.percentile(lat_ms, 99)), 4), "queries": P,
+                  "p99_ms": round(float(np.percentile(lat_ms, 99)), 4), "mean_ms": round(float(lat_ms.mean()), 4),
+                  "max_ms": round(float(lat_ms.max()), 3), "queries": P,
                   "batched_over_plain": round(qps / plain_qps, 2)},
         "roofline": {"kernel": "k_s1_scan_mfma (stage-1 scan: Hamming distances as FP4 MFMA dot products)",
                      "bound": "mfma", "achieved": round(mfma_tops, 1), "peak": MFMA_FP4_PEAK_TFLOPS,
