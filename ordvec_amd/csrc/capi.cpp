@@ -480,14 +480,14 @@ size_t stage1_bytes(const SbIndex* ix, uint64_t nq, uint64_t m_eff) {
 }
 
 Stage1Bufs stage1_dev(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t m_eff, uint32_t* cand,
-                      uint32_t* cand_h, uint32_t* qstat, Carver& cv) {
+                      uint32_t* cand_h, uint32_t* qstat, Carver& cv, const PrepFuse* prep = nullptr) {
     Stage1Bufs b{};
     b.qbits = cv.take<uint64_t>(nq * ix->qpv);
     b.plan = plan_stage1_fast(ix->n, ix->dim, nq, m_eff);
     void* base = cv.take<char>(stage1_fast_bytes(b.plan, nq));
     carve_stage1_fast(b.plan, nq, base, &b.work);
     launch_stage1_fast(ix->words, ix->n, ix->dim, dq, nq, m_eff, b.plan, b.work, b.qbits, cand, cand_h, qstat,
-                       ix->ntop, cur().stream);
+                       ix->ntop, cur().stream, prep);
     return b;
 }
 
@@ -865,8 +865,8 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
     const int tier = select_tier(d, ix->bits);
     const float inv_norm = rankquant_inv_norm(d, ix->bits);
     hipStream_t s = cur().stream;
-    launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, nullptr, s);
-    Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv);
+    const PrepFuse prep{dq, qhat, off, nullptr, d, ix->bits, tier, inv_norm};
+    Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv, &prep);
     launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, nullptr,
                   (uint32_t)out_k, ds, di, nsplit, rwork, s);
     if (!dev) {

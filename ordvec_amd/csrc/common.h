@@ -188,9 +188,23 @@ struct FastWork {
 FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff);
 size_t stage1_fast_bytes(const FastPlan& p, uint64_t nq);
 void carve_stage1_fast(const FastPlan& p, uint64_t nq, void* base, FastWork* w);
+// The rerank's query preparation (launch_prepare_queries' arguments for the
+// same nq queries).  Handed to launch_stage1_fast, it runs on extra
+// workgroups of the vector-ALU scan, beside the scan instead of before it;
+// where the matrix-core scan is used it is a launch of its own.
+struct PrepFuse {
+    const float* q;
+    float* qhat;
+    float* offset;
+    uint32_t* qstat;  // may be null
+    uint32_t d, bits;
+    int tier;
+    float inv_norm;
+};
 void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
                         uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
-                        uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s);
+                        uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s,
+                        const PrepFuse* prep = nullptr);
 
 // Stage 2 / subset rerank over candidate rows.  Rows are CSR (offsets != null)
 // or uniform (row q = cands[q*row_m .. (q+1)*row_m)).  Output nq x out_k.

@@ -71,19 +71,16 @@ __device__ __forceinline__ float seq_sum(const float* xs, uint32_t len, float s)
 
 // One workgroup per query: q_hat = x * (1/||x||) (zeros when ||x|| <= 1e-12)
 // and the SIMD tiers' TopK offset -centre * sum(q_hat) / R.  The two sums are
-// sequential like the reference's; chunks are staged in LDS so the single
-// summing thread reads on-chip memory.  qstat bit 0 records non-finite input.
+// sequential like the reference's; chunks are staged in LDS (xs, kPrepChunk
+// floats) so the single summing thread reads on-chip memory.  qstat bit 0
+// records non-finite input.
 constexpr uint32_t kPrepChunk = 2048;
 
-__global__ __launch_bounds__(256) void k_prepare(const float* __restrict__ q, uint64_t nq, uint32_t d, int tier,
-                                                 float centre, float inv_norm, float* __restrict__ qhat,
-                                                 float* __restrict__ offset, uint32_t* __restrict__ qstat) {
-    __shared__ __align__(16) float xs[kPrepChunk];
+__device__ __forceinline__ void prepare_row(const float* __restrict__ x, uint32_t d, int tier, float centre,
+                                            float inv_norm, float* __restrict__ out, float* __restrict__ offset,
+                                            uint32_t* __restrict__ qstat, float* xs) {
     __shared__ float s_inv;
     __shared__ int s_zero;
-    const uint64_t qi = blockIdx.x;
-    const float* x = q + qi * d;
-    float* out = qhat + qi * d;
     float sq = -0.0f;
     bool bad = false;
     for (uint32_t c0 = 0; c0 < d; c0 += kPrepChunk) {
@@ -102,7 +99,7 @@ __global__ __launch_bounds__(256) void k_prepare(const float* __restrict__ q, ui
         float norm = __fsqrt_rn(sq);
         s_zero = norm <= 1e-12f;
         s_inv = __fdiv_rn(1.0f, norm);
-        if (qstat) qstat[qi] = any_bad ? 1u : 0u;
+        if (qstat) *qstat = any_bad ? 1u : 0u;
     }
     __syncthreads();
     const float inv = s_inv;
@@ -119,7 +116,15 @@ __global__ __launch_bounds__(256) void k_prepare(const float* __restrict__ q, ui
         __syncthreads();
         if (threadIdx.x == 0 && tier != 0) qsum = seq_sum(xs, len, qsum);
     }
-    if (threadIdx.x == 0) offset[qi] = tier != 0 ? __fmul_rn(__fmul_rn(-centre, qsum), inv_norm) : 0.0f;
+    if (threadIdx.x == 0) *offset = tier != 0 ? __fmul_rn(__fmul_rn(-centre, qsum), inv_norm) : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_prepare(const float* __restrict__ q, uint64_t nq, uint32_t d, int tier,
+                                                 float centre, float inv_norm, float* __restrict__ qhat,
+                                                 float* __restrict__ offset, uint32_t* __restrict__ qstat) {
+    __shared__ __align__(16) float xs[kPrepChunk];
+    const uint64_t qi = blockIdx.x;
+    prepare_row(q + qi * d, d, tier, centre, inv_norm, qhat + qi * d, offset + qi, qstat ? qstat + qi : nullptr, xs);
 }
 
 // ---------------------------------------------------------------------------
@@ -723,12 +728,25 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
                                                  uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
                                                  uint32_t qt, uint32_t blocks_per_seg, const uint32_t* __restrict__ thr,
                                                  uint32_t cap, uint64_t* __restrict__ keys, uint32_t* __restrict__ cnt,
-                                                 uint32_t ntop, uint32_t wgl) {
+                                                 uint32_t ntop, uint32_t wgl, PrepFuse pf, uint32_t prep_rows) {
+    extern __shared__ uint64_t s_wl[];
+    __shared__ uint32_t s_wn, s_slot[4];
+    // The first prep_rows rows of the grid prepare query y * gridDim.x + x
+    // for the rerank (dispatched first, so they run beside the scan); the
+    // dynamic LDS holds at least kPrepChunk floats when prep_rows != 0.
+    if (blockIdx.y < prep_rows) {
+        const uint64_t qi = (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
+        if (qi < nq)
+            prepare_row(pf.q + qi * pf.d, pf.d, pf.tier, ((float)(1u << pf.bits) - 1.0f) / 2.0f, pf.inv_norm,
+                        pf.qhat + qi * pf.d, pf.offset + qi, pf.qstat ? pf.qstat + qi : nullptr,
+                        reinterpret_cast<float*>(s_wl));
+        return;
+    }
     const uint32_t q0 = blockIdx.x * qt;
     const uint32_t tq = min(qt, nq - q0);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t b_begin = blockIdx.y * blocks_per_seg;
+    const uint32_t b_begin = (blockIdx.y - prep_rows) * blocks_per_seg;
     const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
     const uint64_t* qb = qbits + (uint64_t)q0 * qpv;
     const uint32_t* tb = thr + q0;
@@ -737,8 +755,6 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
     // wgl != 0: survivors go to the workgroup's LDS list (wgl entries, at
     // least qt x 64 x blocks_per_seg, so it cannot overflow) and are drained
     // at the end; otherwise each appending wave takes global positions.
-    extern __shared__ uint64_t s_wl[];
-    __shared__ uint32_t s_wn, s_slot[4];
     if (wgl) {
         if (threadIdx.x == 0) s_wn = 0;
         if (threadIdx.x < 4) s_slot[threadIdx.x] = 0;
@@ -2582,9 +2598,16 @@ static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
     return g_mfma_scan && nq >= kMfmaMinQueries && (qpv == 16 || qpv == 12 || qpv == 8 || qpv == 4 || qpv == 2 || qpv == 1);
 }
 
+// Whether launch_scan runs a PrepFuse on the scan's own workgroups.
+static bool scan_fuses_prep(uint64_t nq, uint32_t qpv, const FastPlan& p) {
+    const uint64_t tiles = (nq + p.scan_qt - 1) / p.scan_qt;
+    return !use_mfma_scan(nq, qpv) && (uint64_t)p.nseg + (nq + tiles - 1) / tiles <= 65535;
+}
+
 template <int OP>
 static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
-                        const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
+                        const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s,
+                        const PrepFuse* prep = nullptr) {
     const uint32_t nblk = (uint32_t)((n + 63) / 64);
     if (use_mfma_scan(nq, qpv)) {
         // Matrix-core scan: 32, 64 or 128 queries per workgroup of 8 waves,
@@ -2624,10 +2647,15 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
         return;
     }
     const uint32_t tiles = (uint32_t)((nq + p.scan_qt - 1) / p.scan_qt);
-    dim3 grid(tiles, p.nseg);
+    const bool fuse = prep && scan_fuses_prep(nq, qpv, p);
+    const uint32_t prep_rows = fuse ? (uint32_t)((nq + tiles - 1) / tiles) : 0u;
+    const PrepFuse pf = fuse ? *prep : PrepFuse{};
+    size_t lds = (size_t)p.scan_wgl * 8;
+    if (fuse) lds = std::max<size_t>(lds, (size_t)kPrepChunk * 4);
+    dim3 grid(tiles, p.nseg + prep_rows);
 #define OVX_SCAN(W)                                                                                               \
-    hipLaunchKernelGGL((k_s1_scan<W, OP>), grid, dim3(256), (size_t)p.scan_wgl * 8, s, words, nblk, qpv, n, qbits, \
-                       (uint32_t)nq, p.scan_qt, p.scan_bps, w.thr, p.cap, w.keys, w.cnt, ntop, p.scan_wgl)
+    hipLaunchKernelGGL((k_s1_scan<W, OP>), grid, dim3(256), lds, s, words, nblk, qpv, n, qbits, (uint32_t)nq,     \
+                       p.scan_qt, p.scan_bps, w.thr, p.cap, w.keys, w.cnt, ntop, p.scan_wgl, pf, prep_rows)
     switch (qpv) {
         case 16: OVX_SCAN(16); break;
         case 12: OVX_SCAN(12); break;
@@ -2643,9 +2671,14 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
 
 void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
                         uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
-                        uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s) {
+                        uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s, const PrepFuse* prep) {
     if (nq == 0) return;
     const uint32_t qpv = d / 64;
+    if (prep && (m_eff == 0 || !scan_fuses_prep(nq, qpv, p))) {
+        launch_prepare_queries(prep->q, nq, prep->d, prep->bits, prep->tier, prep->inv_norm, prep->qhat, prep->offset,
+                               prep->qstat, s);
+        prep = nullptr;
+    }
     ktime_begin(kFamStage1Select, s);
     if (ntop) {
         const uint32_t P = next_pow2_host(d);
@@ -2668,8 +2701,8 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
     hipLaunchKernelGGL(k_s1_threshold, dim3((uint32_t)nq), dim3(64), 0, s, w.hist, p.nbins, p.s_target, w.thr);
     check_launch();
     ktime_begin(kFamSignScan, s);
-    if (ntop) launch_scan<1>(words, n, qpv, qbits, nq, p, w, ntop, s);
-    else launch_scan<0>(words, n, qpv, qbits, nq, p, w, 0, s);
+    if (ntop) launch_scan<1>(words, n, qpv, qbits, nq, p, w, ntop, s, prep);
+    else launch_scan<0>(words, n, qpv, qbits, nq, p, w, 0, s, prep);
     ktime_end(kFamSignScan, s);
     ktime_begin(kFamStage1Select, s);
     hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(1024), 0, s, w.keys, w.cnt, p.cap, m_eff, cand, cand_h,
