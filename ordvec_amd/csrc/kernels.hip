@@ -643,15 +643,12 @@ __global__ __launch_bounds__(256) void k_s1_sample(const uint64_t* __restrict__ 
 
 // T = smallest h with sample count(hamming <= h) >= s_target, or the largest
 // Hamming value (admit everything) when s_target is ~0u or unreachable.
-__global__ __launch_bounds__(64) void k_s1_threshold(const uint32_t* __restrict__ hist, uint32_t nbins,
-                                                     uint32_t s_target, uint32_t* __restrict__ thr) {
-    const uint64_t q = blockIdx.x;
-    const uint32_t lane = threadIdx.x;
-    if (s_target == 0xffffffffu) {
-        if (lane == 0) thr[q] = nbins - 1;
-        return;
-    }
-    const uint32_t* h = hist + q * nbins;
+// The smallest distance t whose sampled count (hist[0..t]) reaches s_target,
+// or nbins - 1 when the sample holds fewer (or every document is admitted).
+// One wave; every lane gets the value.
+__device__ __forceinline__ uint32_t sample_threshold(const uint32_t* __restrict__ h, uint32_t nbins,
+                                                     uint32_t s_target, uint32_t lane) {
+    if (s_target == 0xffffffffu) return nbins - 1;
     const uint32_t per = (nbins + 63) / 64;
     const uint32_t lo = min(nbins, lane * per), hi = min(nbins, lo + per);
     uint32_t local = 0;
@@ -662,18 +659,27 @@ __global__ __launch_bounds__(64) void k_s1_threshold(const uint32_t* __restrict_
         if ((int)lane >= off) incl += v;
     }
     const uint32_t excl = incl - local;
-    const bool here = excl < s_target && incl >= s_target;
+    const bool here = excl < s_target && incl >= s_target;  // one lane at most
+    uint32_t t = nbins - 1;
     if (here) {
         uint32_t cum = excl;
         for (uint32_t b = lo; b < hi; b++) {
             cum += h[b];
             if (cum >= s_target) {
-                thr[q] = b;
+                t = b;
                 break;
             }
         }
     }
-    if (__ballot(here) == 0 && lane == 0) thr[q] = nbins - 1;
+    const uint64_t m = __ballot(here);
+    return m ? (uint32_t)__shfl((int)t, __ffsll((long long)m) - 1) : nbins - 1;
+}
+
+__global__ __launch_bounds__(64) void k_s1_threshold(const uint32_t* __restrict__ hist, uint32_t nbins,
+                                                     uint32_t s_target, uint32_t* __restrict__ thr) {
+    const uint64_t q = blockIdx.x;
+    const uint32_t t = sample_threshold(hist + q * nbins, nbins, s_target, threadIdx.x);
+    if (threadIdx.x == 0) thr[q] = t;
 }
 
 // Full scan: every document with hamming <= T[q] is appended (wave-aggregated
@@ -728,9 +734,11 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
                                                  uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
                                                  uint32_t qt, uint32_t blocks_per_seg, const uint32_t* __restrict__ thr,
                                                  uint32_t cap, uint64_t* __restrict__ keys, uint32_t* __restrict__ cnt,
-                                                 uint32_t ntop, uint32_t wgl, PrepFuse pf, uint32_t prep_rows) {
+                                                 uint32_t ntop, uint32_t wgl, PrepFuse pf, uint32_t prep_rows,
+                                                 const uint32_t* __restrict__ hist, uint32_t nbins,
+                                                 uint32_t s_target) {
     extern __shared__ uint64_t s_wl[];
-    __shared__ uint32_t s_wn, s_slot[4];
+    __shared__ uint32_t s_wn, s_slot[4], s_thr[4];
     // The first prep_rows rows of the grid prepare query y * gridDim.x + x
     // for the rerank (dispatched first, so they run beside the scan); the
     // dynamic LDS holds at least kPrepChunk floats when prep_rows != 0.
@@ -755,11 +763,28 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
     // wgl != 0: survivors go to the workgroup's LDS list (wgl entries, at
     // least qt x 64 x blocks_per_seg, so it cannot overflow) and are drained
     // at the end; otherwise each appending wave takes global positions.
+    // hist != null (tiles of at most 4 queries): wave w derives query w's
+    // threshold from the sample histogram (k_s1_threshold's rule) instead of
+    // reading thr.
+    if (hist && wave < tq) {
+        const uint32_t t = sample_threshold(hist + (uint64_t)(q0 + wave) * nbins, nbins, s_target, lane);
+        if (lane == 0) s_thr[wave] = t;
+    }
     if (wgl) {
         if (threadIdx.x == 0) s_wn = 0;
         if (threadIdx.x < 4) s_slot[threadIdx.x] = 0;
-        __syncthreads();
     }
+    if (wgl || hist) __syncthreads();
+    uint32_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+    if (hist) {
+        t0 = s_thr[0];
+        t1 = tq > 1 ? s_thr[1] : 0u;
+        t2 = tq > 2 ? s_thr[2] : 0u;
+        t3 = tq > 3 ? s_thr[3] : 0u;
+    }
+    auto thr_of = [&](uint32_t i) -> uint32_t {
+        return hist ? (i == 0 ? t0 : (i == 1 ? t1 : (i == 2 ? t2 : t3))) : tb[i];
+    };
     auto append = [&](bool pass, uint32_t a, uint64_t doc, uint32_t i) {
         if (wgl) s1_append_lds(pass, a, doc, lane, lt_mask, i, s_wl, &s_wn);
         else s1_append(pass, a, doc, lane, lt_mask, cb + i, kb + (uint64_t)i * cap, cap);
@@ -785,13 +810,13 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
             for (; i + 1 < tq; i += 2) {
                 const uint32_t a0 = hamming_regs<WCH, OP>(dw, qb + (uint64_t)i * WCH, ntop);
                 const uint32_t a1 = hamming_regs<WCH, OP>(dw, qb + (uint64_t)(i + 1) * WCH, ntop);
-                const uint32_t t0 = tb[i], t1 = tb[i + 1];
-                append(valid && a0 <= t0, a0, doc, i);
-                append(valid && a1 <= t1, a1, doc, i + 1);
+                const uint32_t u0 = thr_of(i), u1 = thr_of(i + 1);
+                append(valid && a0 <= u0, a0, doc, i);
+                append(valid && a1 <= u1, a1, doc, i + 1);
             }
             if (i < tq) {
                 const uint32_t a0 = hamming_regs<WCH, OP>(dw, qb + (uint64_t)i * WCH, ntop);
-                append(valid && a0 <= tb[i], a0, doc, i);
+                append(valid && a0 <= thr_of(i), a0, doc, i);
             }
 #pragma unroll
             for (int j = 0; j < WCH; j++) dw[j] = nx[j];
@@ -806,7 +831,7 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
                 uint32_t acc = 0;
                 for (uint32_t j = 0; j < qpv; j++) acc += word_stat<OP>(base[(uint64_t)j * 64], qp[j]);
                 acc = to_dist<OP>(acc, ntop);
-                append(valid && acc <= tb[i], acc, doc, i);
+                append(valid && acc <= thr_of(i), acc, doc, i);
             }
         }
     }
@@ -2598,6 +2623,12 @@ static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
     return g_mfma_scan && nq >= kMfmaMinQueries && (qpv == 16 || qpv == 12 || qpv == 8 || qpv == 4 || qpv == 2 || qpv == 1);
 }
 
+// Whether the vector-ALU scan derives the thresholds itself (tiles of at most
+// 4 queries, one wave each), so that k_s1_threshold is not launched.
+static bool scan_fuses_threshold(uint64_t nq, uint32_t qpv, const FastPlan& p) {
+    return !use_mfma_scan(nq, qpv) && p.scan_qt <= 4;
+}
+
 // Whether launch_scan runs a PrepFuse on the scan's own workgroups.
 static bool scan_fuses_prep(uint64_t nq, uint32_t qpv, const FastPlan& p) {
     const uint64_t tiles = (nq + p.scan_qt - 1) / p.scan_qt;
@@ -2653,9 +2684,11 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
     size_t lds = (size_t)p.scan_wgl * 8;
     if (fuse) lds = std::max<size_t>(lds, (size_t)kPrepChunk * 4);
     dim3 grid(tiles, p.nseg + prep_rows);
+    const uint32_t* hist = scan_fuses_threshold(nq, qpv, p) ? w.hist : nullptr;
 #define OVX_SCAN(W)                                                                                               \
     hipLaunchKernelGGL((k_s1_scan<W, OP>), grid, dim3(256), lds, s, words, nblk, qpv, n, qbits, (uint32_t)nq,     \
-                       p.scan_qt, p.scan_bps, w.thr, p.cap, w.keys, w.cnt, ntop, p.scan_wgl, pf, prep_rows)
+                       p.scan_qt, p.scan_bps, w.thr, p.cap, w.keys, w.cnt, ntop, p.scan_wgl, pf, prep_rows, hist, \
+                       p.nbins, p.s_target)
     switch (qpv) {
         case 16: OVX_SCAN(16); break;
         case 12: OVX_SCAN(12); break;
@@ -2698,8 +2731,10 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
         else launch_sample<0>(words, n, qpv, qbits, nq, p, w, 0, s);
         ktime_end(kFamSample, s);
     }
-    hipLaunchKernelGGL(k_s1_threshold, dim3((uint32_t)nq), dim3(64), 0, s, w.hist, p.nbins, p.s_target, w.thr);
-    check_launch();
+    if (!scan_fuses_threshold(nq, qpv, p)) {
+        hipLaunchKernelGGL(k_s1_threshold, dim3((uint32_t)nq), dim3(64), 0, s, w.hist, p.nbins, p.s_target, w.thr);
+        check_launch();
+    }
     ktime_begin(kFamSignScan, s);
     if (ntop) launch_scan<1>(words, n, qpv, qbits, nq, p, w, ntop, s, prep);
     else launch_scan<0>(words, n, qpv, qbits, nq, p, w, 0, s, prep);
