@@ -78,7 +78,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
+        print(json.dumps({"error": "ordvec_amd runs on one GPU; multi-GPU sharding is not implemented "
                           "(documents shard by id range with one all-gather of per-shard top-m keys)",
                           "n_gpus": args.gpus}))
         sys.exit(2)

@@ -168,7 +168,7 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t *index, const ordvec_se
  * of through its captured HIP graph (same results). */
 #define ORDVEC_FLAG_NO_GRAPH 4
 
-/* Engine limits of this build. */
+/* Engine limits of this library. */
 #define ORDVEC_MAX_TOPK 1024
 #define ORDVEC_MAX_STAGE1_M 4096
 #define ORDVEC_MAX_ENCODE_DIM 4096
