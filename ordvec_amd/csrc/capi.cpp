@@ -101,6 +101,7 @@ struct Ctx {
     DevBuf ws, ws2;  // per-call workspace; ws2 serves the exact stage-1 recomputation
     uint32_t* pinned = nullptr;
     size_t pinned_count = 0;
+    hipEvent_t status_ev = nullptr;  // marks the copy of a call's status words to `pinned`
 
     void* workspace(size_t bytes) { return ws.get(bytes, stream); }
     void* workspace2(size_t bytes) { return ws2.get(bytes, stream); }
@@ -121,6 +122,8 @@ struct Ctx {
         if (ws.p) (void)hipFree(ws.p);
         if (ws2.p) (void)hipFree(ws2.p);
         if (pinned) (void)hipHostFree(pinned);
+        if (status_ev) (void)hipEventDestroy(status_ev);
+        status_ev = nullptr;
         ws.p = ws2.p = nullptr;
         ws.size = ws2.size = 0;
         pinned = nullptr;
@@ -407,6 +410,24 @@ std::vector<uint32_t> read_qstat(const uint32_t* dstat, uint64_t nq) {
     uint32_t* h = cur().pinned_u32(nq);
     d2h(h, dstat, nq * 4);
     sync();
+    return std::vector<uint32_t>(h, h + nq);
+}
+
+constexpr uint64_t kEarlyStatusQueries = 64;
+
+// The same words, copied back at this point of the stream; qstat_wait blocks
+// until that copy (not the work enqueued after it) has finished.
+uint32_t* qstat_copy_async(const uint32_t* dstat, uint64_t nq) {
+    Ctx& c = cur();
+    uint32_t* h = c.pinned_u32(nq);
+    d2h(h, dstat, nq * 4);
+    if (!c.status_ev) OVX_HIP(hipEventCreateWithFlags(&c.status_ev, hipEventDisableTiming));
+    OVX_HIP(hipEventRecord(c.status_ev, c.stream));
+    return h;
+}
+
+std::vector<uint32_t> qstat_wait(const uint32_t* h, uint64_t nq) {
+    OVX_HIP(hipEventSynchronize(cur().status_ev));
     return std::vector<uint32_t>(h, h + nq);
 }
 
@@ -867,13 +888,21 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
     hipStream_t s = cur().stream;
     const PrepFuse prep{dq, qhat, off, nullptr, d, ix->bits, tier, inv_norm};
     Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv, &prep);
+    // Device pointers, batches of kEarlyStatusQueries or more: the status
+    // words are final after stage 1, so the call waits for their copy only
+    // and returns while the rerank runs (results complete in stream order,
+    // like every device-pointer call); the next call's launches then overlap
+    // the rerank.  Smaller calls wait once, at the end (a second blocking wait
+    // costs a one-query call more than the overlap gains).
+    const bool early = dev && nq >= kEarlyStatusQueries;
+    const uint32_t* hst = early ? qstat_copy_async(qstat, nq) : nullptr;
     launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, nullptr,
                   (uint32_t)out_k, ds, di, nsplit, rwork, s);
     if (!dev) {
         d2h(out_scores, ds, nq * out_k * 4);
         d2h(out_ids, di, nq * out_k * 8);
     }
-    std::vector<uint32_t> st = read_qstat(qstat, nq);
+    std::vector<uint32_t> st = early ? qstat_wait(hst, nq) : read_qstat(qstat, nq);
     if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
         fail(kNonfiniteQuery, "queries contain NaN or infinity");
     std::vector<uint32_t> fl = flagged(st, kStatFallback);

@@ -287,6 +287,15 @@ def test_two_stage_large_batch_and_mixed_fallback(engine, oracle):
     ws, wi = oracle.subset_batched(oracle.rq_encode(docs, 2), dim, 2, qs, offs, cand.reshape(-1), 10)
     assert np.array_equal(i, wi)
     _assert_scores(s, ws)
+    # device tensors: the call returns once stage 1's status is back, with the
+    # rerank (and the exact rows' second rerank) still in stream order
+    import torch
+
+    e0 = engine.stage1_exact_queries()
+    ds, di = engine.two_stage_search(sb, rq, torch.from_numpy(qs).cuda(), 700, 10)
+    assert engine.stage1_exact_queries() > e0
+    assert np.array_equal(di.cpu().numpy(), wi)
+    assert np.array_equal(ds.cpu().numpy(), s)
 
 
 def test_nonfinite_device_query_is_rejected(engine):
@@ -305,6 +314,14 @@ def test_nonfinite_device_query_is_rejected(engine):
         rq.search_asymmetric(q, 5)
     with pytest.raises(ValueError):
         engine.two_stage_search(sb, rq, x[:3] * np.array([1, np.inf, 1], np.float32)[:, None], 50, 5)
+    big = torch.from_numpy(x[:100].copy()).cuda()  # a batch that waits for stage 1's status only
+    big[77, 3] = float("inf")
+    with pytest.raises(ValueError):
+        engine.two_stage_search(sb, rq, big, 50, 5)
+    big[77, 3] = 0.5
+    s, i = engine.two_stage_search(sb, rq, big, 50, 5)
+    s1, i1 = engine.two_stage_search(sb, rq, big[77:78], 50, 5)
+    assert torch.equal(i[77:78], i1) and torch.equal(s[77:78], s1)
 
 
 def test_fast_stage1_path_needs_no_exact_fallback_on_partial_tiles(engine, oracle):
