@@ -203,6 +203,10 @@ ordvec_status_t ordvec_stage1_exact_queries(uint64_t *count);
 #define ORDVEC_KERNEL_STAGE1_SAMPLE 7
 #define ORDVEC_KERNEL_BACKWARD 8
 ordvec_status_t ordvec_kernel_timing_enable(int enable);
+/* As ordvec_kernel_timing_enable, for the families whose bit (1 << family)
+ * is set in mask only, so that a timed run carries the events of the kernels
+ * it reports and no others; mask 0 turns timing off. */
+ordvec_status_t ordvec_kernel_timing_enable_families(uint32_t mask);
 ordvec_status_t ordvec_kernel_timing_read(uint32_t family, uint64_t *launches, double *total_ms);
 
 /* RankQuant: src/quant.rs:438-491 new, 587-614 add, 713-847 search_asymmetric. */

@@ -138,6 +138,7 @@ SIGNATURES = {
     "ordvec_synchronize": (ST, []),
     "ordvec_stage1_exact_queries": (ST, [P]),
     "ordvec_kernel_timing_enable": (ST, [C.c_int]),
+    "ordvec_kernel_timing_enable_families": (ST, [U32]),
     "ordvec_kernel_timing_read": (ST, [U32, P, P]),
     "ordvec_rankquant_new": (ST, [U64, U32, C.POINTER(P)]),
     "ordvec_rankquant_add": (ST, [P, P, U64, U64]),
@@ -267,8 +268,16 @@ KERNEL_FAMILIES = {"sign_scan": 0, "stage1_select": 1, "rerank": 2, "full_scan":
                    "encode": 5, "prepare": 6, "stage1_sample": 7, "backward": 8}
 
 
-def kernel_timing_enable(on: bool = True) -> None:
-    check(lib().ordvec_kernel_timing_enable(1 if on else 0))
+def kernel_timing_enable(on: bool = True, families=None) -> None:
+    """Reset the counters and record events around the launches of every
+    kernel family (``families`` None) or of the named families only."""
+    if families is None:
+        check(lib().ordvec_kernel_timing_enable(1 if on else 0))
+    else:
+        mask = 0
+        for f in families:
+            mask |= 1 << KERNEL_FAMILIES[f]
+        check(lib().ordvec_kernel_timing_enable_families(mask if on else 0))
 
 
 def kernel_timing_read(family: str):

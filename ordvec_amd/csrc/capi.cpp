@@ -1923,7 +1923,7 @@ constexpr uint64_t kAllCaps =
 // Optional event timing of kernel families (engine mutex held by callers)
 // ---------------------------------------------------------------------------
 struct KTimes {
-    bool enabled = false;
+    uint32_t mask = 0;  // families recorded (bit f = family f)
     struct Pending {
         hipEvent_t a, b;
         int fam;
@@ -1968,7 +1968,7 @@ KTimes& kt() {
 
 void ktime_begin(int family, hipStream_t s) {
     KTimes& t = kt();
-    if (!t.enabled || t_ctx) return;  // serving slots are not event-timed
+    if (!((t.mask >> family) & 1u) || t_ctx) return;  // serving slots are not event-timed
     hipEvent_t e = t.get();
     OVX_HIP(hipEventRecord(e, s));
     t.open[family] = e;
@@ -1976,7 +1976,7 @@ void ktime_begin(int family, hipStream_t s) {
 
 void ktime_end(int family, hipStream_t s) {
     KTimes& t = kt();
-    if (!t.enabled || t_ctx || !t.open[family]) return;
+    if (!((t.mask >> family) & 1u) || t_ctx || !t.open[family]) return;
     hipEvent_t e = t.get();
     OVX_HIP(hipEventRecord(e, s));
     t.pending.push_back({t.open[family], e, family});
@@ -2312,7 +2312,7 @@ ordvec_status_t ordvec_stage1_exact_queries(uint64_t* count) {
     });
 }
 
-ordvec_status_t ordvec_kernel_timing_enable(int enable) {
+ordvec_status_t ordvec_kernel_timing_enable_families(uint32_t mask) {
     return boundary([&] {
         LOCK;
         eng().init();
@@ -2323,8 +2323,12 @@ ordvec_status_t ordvec_kernel_timing_enable(int enable) {
             t.total_ms[f] = 0.0;
             t.open[f] = nullptr;
         }
-        t.enabled = enable != 0;
+        t.mask = mask & ((1u << kFamCount) - 1u);
     });
+}
+
+ordvec_status_t ordvec_kernel_timing_enable(int enable) {
+    return ordvec_kernel_timing_enable_families(enable ? ~0u : 0u);
 }
 
 ordvec_status_t ordvec_kernel_timing_read(uint32_t family, uint64_t* launches, double* total_ms) {

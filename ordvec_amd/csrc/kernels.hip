@@ -472,6 +472,35 @@ __global__ __launch_bounds__(64) void k_s1_prep(const float* __restrict__ q, uin
     }
 }
 
+// k_prepare and k_s1_prep in one workgroup per query (sign-bitmap stage 1
+// behind the matrix-core scan, where the rerank's query prep is a launch of
+// its own): the waves first write the query's sign words, zero its histogram
+// and counter and record non-finite input in qstat, then prepare q_hat.
+__global__ __launch_bounds__(256) void k_prepare_s1(const float* __restrict__ q, uint32_t d, int tier, float centre,
+                                                    float inv_norm, float* __restrict__ qhat,
+                                                    float* __restrict__ offset, uint64_t* __restrict__ qbits,
+                                                    uint32_t* __restrict__ hist, uint32_t nbins,
+                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ qstat) {
+    __shared__ __align__(16) float xs[kPrepChunk];
+    const uint64_t qi = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6, qpv = d / 64;
+    const float* x = q + qi * d;
+    bool bad = false;
+    for (uint32_t w = wave; w < qpv; w += nwave) {
+        const float v = x[w * 64 + lane];
+        bad |= !isfinite(v);
+        const uint64_t mask = __ballot(v > 0.0f);
+        if (lane == 0) qbits[qi * qpv + w] = mask;
+    }
+    for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) hist[qi * nbins + i] = 0;
+    const int any_bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) {
+        cnt[qi] = 0;
+        if (qstat) qstat[qi] = any_bad ? 1u : 0u;
+    }
+    prepare_row(x, d, tier, centre, inv_norm, qhat + qi * d, offset + qi, nullptr, xs);
+}
+
 // Top-bucket bitmaps (src/bitmap.rs:140-187): one workgroup per row, LDS
 // bitonic sort of 64-bit keys, bit j set for the chosen n_top coordinates.
 //  * document (QUERY == false): rank_transform's order (value ascending, -0 ==
@@ -2707,23 +2736,36 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
                         uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s, const PrepFuse* prep) {
     if (nq == 0) return;
     const uint32_t qpv = d / 64;
+    bool s1_prepped = false;
     if (prep && (m_eff == 0 || !scan_fuses_prep(nq, qpv, p))) {
-        launch_prepare_queries(prep->q, nq, prep->d, prep->bits, prep->tier, prep->inv_norm, prep->qhat, prep->offset,
-                               prep->qstat, s);
+        if (!ntop && !prep->qstat && prep->q == queries && prep->d == d) {
+            const float centre = ((float)(1u << prep->bits) - 1.0f) / 2.0f;
+            ktime_begin(kFamPrepare, s);
+            hipLaunchKernelGGL(k_prepare_s1, dim3((uint32_t)nq), dim3(256), 0, s, queries, d, prep->tier, centre,
+                               prep->inv_norm, prep->qhat, prep->offset, qbits, w.hist, p.nbins, w.cnt, qstat);
+            check_launch();
+            ktime_end(kFamPrepare, s);
+            s1_prepped = true;
+        } else {
+            launch_prepare_queries(prep->q, nq, prep->d, prep->bits, prep->tier, prep->inv_norm, prep->qhat,
+                                   prep->offset, prep->qstat, s);
+        }
         prep = nullptr;
     }
-    ktime_begin(kFamStage1Select, s);
-    if (ntop) {
-        const uint32_t P = next_pow2_host(d);
-        hipLaunchKernelGGL(k_topbits<true>, dim3((uint32_t)std::min<uint64_t>(nq, 65535)), dim3(256),
-                           (size_t)P * 8 + (size_t)qpv * 8, s, queries, nq, d, ntop, qbits, 0, w.hist, p.nbins, w.cnt,
-                           qstat);
-    } else {
-        hipLaunchKernelGGL(k_s1_prep, dim3((uint32_t)nq), dim3(64), 0, s, queries, d, qbits, w.hist, p.nbins, w.cnt,
-                           qstat);
+    if (!s1_prepped) {  // otherwise k_prepare_s1 did k_s1_prep's work
+        ktime_begin(kFamStage1Select, s);
+        if (ntop) {
+            const uint32_t P = next_pow2_host(d);
+            hipLaunchKernelGGL(k_topbits<true>, dim3((uint32_t)std::min<uint64_t>(nq, 65535)), dim3(256),
+                               (size_t)P * 8 + (size_t)qpv * 8, s, queries, nq, d, ntop, qbits, 0, w.hist, p.nbins,
+                               w.cnt, qstat);
+        } else {
+            hipLaunchKernelGGL(k_s1_prep, dim3((uint32_t)nq), dim3(64), 0, s, queries, d, qbits, w.hist, p.nbins,
+                               w.cnt, qstat);
+        }
+        check_launch();
+        ktime_end(kFamStage1Select, s);
     }
-    check_launch();
-    ktime_end(kFamStage1Select, s);
     if (m_eff == 0) return;
     if (p.s_target != 0xffffffffu) {
         ktime_begin(kFamSample, s);
