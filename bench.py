@@ -522,7 +522,7 @@ def grad_main(args, dev):
         q_raw = q_raw0.clone().requires_grad_(True)
         step(W, q_raw)
     torch.cuda.synchronize()
-    N.kernel_timing_enable(True)
+    N.kernel_timing_enable(True, families=["rerank", "backward"])
     t1 = time.perf_counter()
     for _ in range(args.steps):
         W = W0.clone().requires_grad_(True)
