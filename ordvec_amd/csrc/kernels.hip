@@ -451,22 +451,23 @@ __global__ __launch_bounds__(256) void k_stage1_finalize(const uint16_t* __restr
 
 // Per query: sign bits (wave ballot), zero the sample histogram and the
 // survivor counter, record non-finite input in qstat bit 0.
-__global__ __launch_bounds__(64) void k_s1_prep(const float* __restrict__ q, uint32_t d, uint64_t* __restrict__ qbits,
-                                                uint32_t* __restrict__ hist, uint32_t nbins, uint32_t* __restrict__ cnt,
-                                                uint32_t* __restrict__ qstat) {
+// One workgroup (4 waves) per query; wave w writes sign words w, w+4, ...
+__global__ __launch_bounds__(256) void k_s1_prep(const float* __restrict__ q, uint32_t d, uint64_t* __restrict__ qbits,
+                                                 uint32_t* __restrict__ hist, uint32_t nbins, uint32_t* __restrict__ cnt,
+                                                 uint32_t* __restrict__ qstat) {
     const uint64_t qi = blockIdx.x;
-    const uint32_t lane = threadIdx.x, qpv = d / 64;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6, qpv = d / 64;
     const float* x = q + qi * d;
     bool bad = false;
-    for (uint32_t w = 0; w < qpv; w++) {
-        float v = x[w * 64 + lane];
+    for (uint32_t w = wave; w < qpv; w += nwave) {
+        const float v = x[w * 64 + lane];
         bad |= !isfinite(v);
-        uint64_t mask = __ballot(v > 0.0f);
+        const uint64_t mask = __ballot(v > 0.0f);
         if (lane == 0) qbits[qi * qpv + w] = mask;
     }
-    for (uint32_t i = lane; i < nbins; i += 64) hist[qi * nbins + i] = 0;
-    bool any_bad = __any(bad);
-    if (lane == 0) {
+    for (uint32_t i = threadIdx.x; i < nbins; i += blockDim.x) hist[qi * nbins + i] = 0;
+    const int any_bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) {
         cnt[qi] = 0;
         if (qstat) qstat[qi] = any_bad ? 1u : 0u;
     }
@@ -2760,7 +2761,7 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
                                (size_t)P * 8 + (size_t)qpv * 8, s, queries, nq, d, ntop, qbits, 0, w.hist, p.nbins,
                                w.cnt, qstat);
         } else {
-            hipLaunchKernelGGL(k_s1_prep, dim3((uint32_t)nq), dim3(64), 0, s, queries, d, qbits, w.hist, p.nbins,
+            hipLaunchKernelGGL(k_s1_prep, dim3((uint32_t)nq), dim3(256), 0, s, queries, d, qbits, w.hist, p.nbins,
                                w.cnt, qstat);
         }
         check_launch();
