@@ -2030,6 +2030,40 @@ __global__ __launch_bounds__(256) void k_merge_topk(const uint64_t* __restrict__
     const uint64_t q = qmap ? qmap[qidx] : qidx;
     const uint64_t total = (uint64_t)nseg * k;
     const uint64_t* in = partial + qidx * total;
+    if (total <= 256 && k <= 32) {
+        // Few keys (one-query calls): wave 0 holds up to 4 per lane and takes
+        // the k smallest one at a time (wave minimum, one occurrence removed
+        // per round, so duplicate keys are kept like the LDS top-k keeps them).
+        if (threadIdx.x >= 64) return;
+        const uint32_t lane = threadIdx.x;
+        uint64_t r[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t i = lane + 64u * j;
+            r[j] = i < total ? in[i] : kKeyMax;
+        }
+        for (uint32_t j = 0; j < k; j++) {
+            const uint64_t lm = min(min(r[0], r[1]), min(r[2], r[3]));
+            uint64_t m = lm;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const uint64_t o = __shfl_xor(m, off);
+                m = o < m ? o : m;
+            }
+            const uint64_t own = __ballot(lm == m);
+            if ((int)lane == __ffsll((long long)own) - 1) {
+                if (r[0] == m) r[0] = kKeyMax;
+                else if (r[1] == m) r[1] = kKeyMax;
+                else if (r[2] == m) r[2] = kKeyMax;
+                else r[3] = kKeyMax;
+            }
+            if (lane == 0) {
+                out_s[q * k + j] = m == kKeyMax ? -INFINITY : key_score(m);
+                out_i[q * k + j] = m == kKeyMax ? -1 : (int64_t)(m & 0xffffffffu);
+            }
+        }
+        return;
+    }
     tk_init(tk, k);
     for (uint64_t r0 = 0; r0 < total; r0 += blockDim.x) {
         tk_reserve(tk, k, blockDim.x);

@@ -188,6 +188,16 @@ def test_subset_batched_csr_random(engine, oracle):
             ws, wi = oracle.subset_batched(oracle.rq_encode(docs, bits), dim, bits, qs, offs, cands, k)
             assert np.array_equal(i, wi), (bits, k)
             _assert_scores(s, ws)
+        # one query whose row repeats every candidate in another split's slice:
+        # each of the best rows comes back twice
+        row = np.concatenate([np.arange(300), np.arange(300)[::-1]]).astype(np.uint32)
+        one = np.array([0, len(row)], np.uint64)
+        for k in (7, 10, 32):
+            s, i = rq.search_asymmetric_subset_batched(qs[:1], one, row, k)
+            ws, wi = oracle.subset_batched(oracle.rq_encode(docs, bits), dim, bits, qs[:1], one, row, k)
+            assert np.array_equal(i, wi), (bits, k)
+            assert i[0, 0] == i[0, 1]
+            _assert_scores(s, ws)
 
 
 def test_full_candidates_equal_full_search(engine):
