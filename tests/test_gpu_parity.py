@@ -334,20 +334,85 @@ def test_nonfinite_device_query_is_rejected(engine):
     assert torch.equal(i[77:78], i1) and torch.equal(s[77:78], s1)
 
 
-def test_fast_stage1_path_needs_no_exact_fallback_on_partial_tiles(engine, oracle):
-    """Batches that fill their matrix-core query tiles (32, 64 or 128 queries)
-    partially; on ordinary data every query must finish on the fast path (a
-    wrong sampled threshold or survivor list would still give exact results,
-    but only through the exact fallback, which the counter exposes)."""
+@pytest.fixture(scope="module")
+def mixture_200k(engine):
+    """200,000 x 1024 mixture corpus (seed 21) with its sign bitmap and b=2
+    RankQuant index resident, shared by the C2-shape tests below."""
     d, n = 1024, 200_000
     gen = S.MixtureNP(d, seed=21)
     docs = gen.docs(n)
-    qs, _ = gen.queries(docs, 200)
+    qs, _ = gen.queries(docs, 1100)
     sb = engine.SignBitmap(d)
     sb.add(docs)
+    rq = engine.RankQuant(d, 2)
+    rq.add(docs)
+    return docs, qs, sb, rq
+
+
+def test_fast_stage1_path_needs_no_exact_fallback_on_partial_tiles(engine, oracle, mixture_200k):
+    """Batches that fill their matrix-core query tiles (32, 64 or 128 queries)
+    partially, and the one- and few-query VALU scan whose workgroups derive
+    the sampled threshold themselves (the sample is strided at 200k docs); on
+    ordinary data every query must finish on the fast path (a wrong sampled
+    threshold or survivor list would still give exact results, but only
+    through the exact fallback, which the counter exposes)."""
+    docs, qs, sb, _ = mixture_200k
     words = sb.words()
-    for nq in (2, 8, 31, 32, 33, 48, 64, 65, 100, 128, 129, 200):
+    for nq in (1, 3, 2, 8, 31, 32, 33, 48, 64, 65, 100, 128, 129, 200):
         e0 = engine.stage1_exact_queries()
         got = sb.top_m_candidates_batched(qs[:nq], 1000)
         assert engine.stage1_exact_queries() == e0, nq
         assert np.array_equal(got, oracle.sign_top_m(words, oracle.sign_encode(qs[:nq]), 1000, 8)), nq
+
+
+def _two_stage_oracle(oracle, docs_codes, words, qs, m, k):
+    dim = qs.shape[1]
+    cand = oracle.sign_top_m(words, oracle.sign_encode(qs), m, 8)
+    offs = np.arange(len(qs) + 1, dtype=np.uint64) * cand.shape[1]
+    return oracle.subset_batched(docs_codes, dim, 2, qs, offs, cand.reshape(-1), k, 8)
+
+
+def test_c2_one_query_calls_match_oracle(engine, oracle, mixture_200k):
+    """The plain one-query-per-call two-stage path at the C2 shape (1024-d,
+    m=1000 < n, k=10): VALU scan with the query prep and threshold inside its
+    grid, split rerank with the one-wave merge; host arrays and device
+    tensors, every query on the fast stage-1 path."""
+    import torch
+
+    docs, qs, sb, rq = mixture_200k
+    codes, words = rq.codes(), sb.words()
+    sel = qs[1000:1012]
+    ws, wi = _two_stage_oracle(oracle, codes, words, sel, 1000, 10)
+    e0 = engine.stage1_exact_queries()
+    for j in range(len(sel)):
+        s, i = engine.two_stage_search(sb, rq, sel[j:j + 1], 1000, 10)
+        assert np.array_equal(i[0], wi[j]), j
+        _assert_scores(s[0], ws[j])
+        tq = torch.from_numpy(sel[j:j + 1].copy()).cuda()
+        ds, di = engine.two_stage_search(sb, rq, tq, 1000, 10)
+        assert np.array_equal(di.cpu().numpy()[0], wi[j]), j
+        _assert_scores(ds.cpu().numpy()[0], ws[j])
+    assert engine.stage1_exact_queries() == e0
+    # the same queries in one call give the same rows
+    s, i = engine.two_stage_search(sb, rq, sel, 1000, 10)
+    assert np.array_equal(i, wi)
+    _assert_scores(s, ws)
+
+
+def test_c2_batch_1024_matches_oracle(engine, oracle, mixture_200k):
+    """A 1024-query two-stage batch (128-query matrix-core tiles, batched
+    rerank, the early stage-1 status wait on device tensors) against the
+    oracle at 200k x 1024, m=1000, k=10."""
+    import torch
+
+    docs, qs, sb, rq = mixture_200k
+    batch = qs[:1024]
+    ws, wi = _two_stage_oracle(oracle, rq.codes(), sb.words(), batch, 1000, 10)
+    e0 = engine.stage1_exact_queries()
+    s, i = engine.two_stage_search(sb, rq, batch, 1000, 10)
+    assert np.array_equal(i, wi)
+    _assert_scores(s, ws)
+    ds, di = engine.two_stage_search(sb, rq, torch.from_numpy(batch.copy()).cuda(), 1000, 10)
+    assert np.array_equal(di.cpu().numpy(), wi)
+    _assert_scores(ds.cpu().numpy(), ws)
+    assert engine.stage1_exact_queries() == e0

@@ -220,11 +220,43 @@ def test_fastscan_layout_and_lut(oracle):
                      for p in range(2)], np.float32)
     gmin, gmax = vals.min(), vals.max()
     assert bias == np.float32(2) * gmin
-    deq = bias + lut.reshape(2, 16).astype(np.float32).sum(0) * 0  # shape check only
-    assert deq.shape == (16,)
     # each dequantised entry within half a quantisation step of the float LUT
     step = (gmax - gmin) / 255
     assert np.max(np.abs(gmin + lut.reshape(2, 16) * invq - vals)) <= 0.5 * step * (1 + 1e-5) + 1e-7
+
+
+@pytest.mark.parametrize("dim", [64, 256, 1024])
+def test_fastscan_scores_within_quantisation_bound(oracle, dim):
+    """score = (bias_sum + acc * inv_q) / R (src/fastscan.rs:316-347) against
+    the float64 closed form sum_j q_hat[j] * centre(code_j) / R: each of the
+    d/2 pair entries is off by at most half a quantisation step, so every score
+    lies within pairs * 0.5 * inv_q / R of it.  A dropped 1/R or bias_sum
+    lands far outside that bound (checked on the same scores)."""
+    gen = S.MixtureNP(dim, seed=61)
+    n = 700
+    docs = gen.docs(n)
+    qs, _ = gen.queries(docs, 5)
+    codes = oracle.rq_encode(docs, 2)
+    buckets = np.stack([oracle.unpack_buckets(r, dim, 2) for r in codes])
+    packed = oracle.fs_pack(buckets, dim)
+    R = float(np.sqrt(dim * 15 / 12.0))
+    assert np.float32(R) == oracle.rankquant_norm(dim, 2)
+    cen = oracle.decode_centres(codes, dim, 2)
+    pairs = dim // 2
+    for q in qs:
+        qu = oracle.l2_normalise(q)
+        _, bias, invq = oracle.fs_query(qu)
+        s, i = oracle.fs_search(packed, n, dim, q[None], n)
+        assert sorted(i[0].tolist()) == list(range(n))
+        exact = cen[i[0]] @ qu.astype(np.float64) / R
+        bound = pairs * 0.5 * float(invq) / R * (1 + 1e-5) + 1e-6
+        err = np.abs(s[0].astype(np.float64) - exact)
+        assert np.max(err) <= bound
+        # sensitivity: the plausible slips fail the same bound
+        assert np.max(np.abs(s[0] * np.float64(R) - exact)) > bound
+        assert np.max(np.abs(s[0] - float(bias) / R - exact)) > bound
+        # order: score desc, id asc on exact ties
+        assert np.all(np.diff(s[0]) <= 0)
 
 
 def test_fastscan_top10_close_to_exact(oracle):
