@@ -2,10 +2,14 @@
 
 Produces ``ordvec_amd/lib/libordvec_amd.so`` in-tree from ``ordvec_amd/csrc``.
 Run ``python ordvec_amd/_build.py`` (by path: importing the package loads the
-library this builds) or call :func:`build`.
+library this builds) or call :func:`build`.  Sources compile in parallel to
+objects under ``_build/`` (reused while a source and the headers are older
+than its object, unless ``force``), then link into the shared library.
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
+import glob
 import os
 import shutil
 import subprocess
@@ -15,6 +19,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
+OBJDIR = os.path.join(ROOT, "_build")
 LIB = os.path.join(LIBDIR, "libordvec_amd.so")
 SOURCES = ["kernels.hip", "fastscan4.hip", "capi.cpp"]
 
@@ -23,7 +28,6 @@ FLAGS = [
     "-std=c++17",
     "--offload-arch=gfx950",
     "-fPIC",
-    "-shared",
     # Exactness: every fused multiply-add is explicit in the source.
     "-ffp-contract=off",
     "-fno-fast-math",
@@ -40,20 +44,44 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
+def _headers_mtime() -> float:
+    hs = glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
+    return max((os.path.getmtime(h) for h in hs), default=0.0)
+
+
+def _compile(src: str, force: bool, verbose: bool) -> str:
+    obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+    if (not force and os.path.exists(obj) and
+            os.path.getmtime(obj) > max(os.path.getmtime(src), _headers_mtime())):
+        return obj
+    cmd = [hipcc(), *FLAGS, "-c", "-I", os.path.join(ROOT, "include"), "-I", CSRC, src, "-o", obj + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout)
-        raise RuntimeError("hipcc failed building libordvec_amd.so")
+        raise RuntimeError(f"hipcc failed compiling {os.path.basename(src)}")
+    os.replace(obj + ".tmp", obj)
+    return obj
+
+
+def build(verbose: bool = False, force: bool = True) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    with cf.ThreadPoolExecutor(len(srcs)) as ex:
+        objs = list(ex.map(lambda s: _compile(s, force, verbose), srcs))
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout)
+        raise RuntimeError("hipcc failed linking libordvec_amd.so")
     os.replace(tmp, LIB)
     return LIB
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, force="--force" in sys.argv))
