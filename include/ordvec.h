@@ -339,6 +339,26 @@ ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t *sign, const 
                                         const float *queries, uint64_t nq, uint64_t m, uint64_t k,
                                         float *out_scores, int64_t *out_ids, uint64_t flags);
 
+/* Allocation-free two-stage search.  No reference counterpart: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same semantics and
+ * results as ordvec_two_stage_search with ORDVEC_FLAG_DEVICE_PTRS (implied:
+ * queries and outputs are device pointers), but the call allocates nothing:
+ * its scratch is a fixed array of ORDVEC_NOALLOC_SCRATCH_BYTES per GPU,
+ * reserved by the library until the process ends.  A call whose scratch
+ * (growing with nq, m and the index size) would not fit, or with more than
+ * ORDVEC_NOALLOC_MAX_QUERIES queries, fails with UNSUPPORTED_OPERATION before
+ * any work is enqueued.  Returns once the results are written.  Calls share
+ * the GPU's scratch: calls that overlap on one GPU (another thread, another
+ * stream) race; the library's own calls are serialised by its lock.  Call
+ * ordvec_init once first so that the scratch address is looked up at set-up
+ * (otherwise the first call on each device does it). */
+#define ORDVEC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define ORDVEC_NOALLOC_MAX_QUERIES 1024
+ordvec_status_t ordvec_two_stage_search_noalloc(const ordvec_sign_bitmap_t *sign, const ordvec_rankquant_t *rq,
+                                                const float *queries, uint64_t nq, uint64_t m, uint64_t k,
+                                                float *out_scores, int64_t *out_ids, uint64_t flags);
+
 /* RankQuantFastscan b=2: src/fastscan.rs:571-705. */
 ordvec_status_t ordvec_fastscan_new(uint64_t dim, ordvec_fastscan_t **out);
 ordvec_status_t ordvec_fastscan_add(ordvec_fastscan_t *index, const float *vectors, uint64_t n, uint64_t flags);

@@ -159,6 +159,7 @@ SIGNATURES = {
     "ordvec_sign_bitmap_load": (ST, [C.c_char_p, C.POINTER(P)]),
     "ordvec_sign_bitmap_free": (None, [P]),
     "ordvec_two_stage_search": (ST, [P, P, P, U64, U64, U64, P, P, U64]),
+    "ordvec_two_stage_search_noalloc": (ST, [P, P, P, U64, U64, U64, P, P, U64]),
     "ordvec_bitmap_new": (ST, [U64, U64, C.POINTER(P)]),
     "ordvec_bitmap_add": (ST, [P, P, U64, U64]),
     "ordvec_bitmap_info": (ST, [P, P, P, P]),

@@ -827,14 +827,20 @@ void bm_subset(const SbIndex* ix, const float* queries, uint64_t nq, const uint6
     finish_status(qstat, nq, flags, kNonfiniteQuery, "queries");
 }
 
-// Exact recomputation of flagged queries (histogram pipeline, workspace 2);
-// rewrites their rows of cand / cand_h in place.
-void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint32_t>& fl, uint64_t m_eff,
-                uint32_t* cand, uint32_t* cand_h) {
-    const uint64_t nf = fl.size(), qpv = ix->qpv;
+// Device bytes the exact recomputation of nf queries needs.
+size_t stage1_fix_bytes(const SbIndex* ix, uint64_t nf, uint64_t m_eff) {
     Stage1Plan plan = plan_stage1(ix->n, ix->dim, nf);
-    size_t bytes = al(nf * qpv * 8) + al(stage1_work_bytes(plan, nf, m_eff)) + 2 * al(nf * m_eff * 4);
-    Carver cv{static_cast<char*>(cur().workspace2(bytes))};
+    return al(nf * ix->qpv * 8) + al(stage1_work_bytes(plan, nf, m_eff)) + 2 * al(nf * m_eff * 4);
+}
+
+// Exact recomputation of the nf queries listed in fl (histogram pipeline) in
+// the scratch at `base` (stage1_fix_bytes); rewrites their rows of cand /
+// cand_h in place.
+void stage1_fix_into(const SbIndex* ix, const uint64_t* qbits, const uint32_t* fl, uint64_t nf, uint64_t m_eff,
+                     uint32_t* cand, uint32_t* cand_h, char* base) {
+    const uint64_t qpv = ix->qpv;
+    Stage1Plan plan = plan_stage1(ix->n, ix->dim, nf);
+    Carver cv{base};
     uint64_t* qb = cv.take<uint64_t>(nf * qpv);
     for (uint64_t i = 0; i < nf; i++) d2d(qb + i * qpv, qbits + (uint64_t)fl[i] * qpv, qpv * 8);
     Stage1Work w{};
@@ -847,6 +853,13 @@ void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint
         d2d(cand + (uint64_t)fl[i] * m_eff, cc + i * m_eff, m_eff * 4);
         if (cand_h) d2d(cand_h + (uint64_t)fl[i] * m_eff, ch + i * m_eff, m_eff * 4);
     }
+}
+
+// The same in the context's second workspace.
+void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint32_t>& fl, uint64_t m_eff,
+                uint32_t* cand, uint32_t* cand_h) {
+    char* base = static_cast<char*>(cur().workspace2(stage1_fix_bytes(ix, fl.size(), m_eff)));
+    stage1_fix_into(ix, qbits, fl.data(), fl.size(), m_eff, cand, cand_h, base);
 }
 
 // Two-stage search on the calling thread's context (src/sign_bitmap.rs:453-483
@@ -918,6 +931,136 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
                 d2h(out_ids + (uint64_t)q * out_k, di + (uint64_t)q * out_k, out_k * 8);
             }
         sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free two-stage search, for callers that cannot allocate device
+// memory (code under HIP graph capture, tight serving loops).  Every scratch
+// buffer of two_stage_run (stage-1 histograms, thresholds, counters and
+// survivor keys, candidates, q_hat, rerank partials, and the exact
+// recomputation of flagged queries) is carved from `scratch`, a fixed
+// per-device array; the status words come back through a fixed host buffer.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+constexpr uint64_t kNoAllocMaxQueries = 1024;
+static_assert(kNoAllocScratchBytes == (size_t)ORDVEC_NOALLOC_SCRATCH_BYTES, "header and library disagree");
+static_assert(kNoAllocMaxQueries == (uint64_t)ORDVEC_NOALLOC_MAX_QUERIES, "header and library disagree");
+
+// One copy per GPU, held until the process ends.  In host code `scratch` only
+// names the symbol: its device address comes from hipGetSymbolAddress, once
+// per device (noalloc_scratch), and only that address is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (engine mutex held).
+char* noalloc_scratch() {
+    int dev = 0;
+    OVX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) fail(kInternal, "device ordinal out of range for the scratch table");
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(kInternal, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Kernel-family timing records events (which it creates on first use); an
+// allocation-free call records none.
+thread_local bool t_no_events = false;
+struct NoEventsScope {
+    bool prev;
+    NoEventsScope() : prev(t_no_events) { t_no_events = true; }
+    ~NoEventsScope() { t_no_events = prev; }
+};
+
+thread_local uint32_t t_na_status[kNoAllocMaxQueries];
+thread_local uint32_t t_na_flagged[kNoAllocMaxQueries];
+
+// Main-path scratch of one allocation-free call (everything but the exact
+// recomputation).
+size_t noalloc_main_bytes(const SbIndex* sb, const RqIndex* ix, uint64_t nq, uint64_t m_eff, uint64_t out_k) {
+    const uint32_t nsplit = rerank_nsplit(nq, m_eff);
+    return al(nq * ix->dim * 4) + 3 * al(nq * 4) + al(nq * m_eff * 4) + stage1_bytes(sb, nq, m_eff) +
+           rerank_work_bytes(nq, (uint32_t)out_k, nsplit);
+}
+
+// two_stage_run's semantics on device pointers, with no allocation: returns
+// once the results are written.
+void two_stage_noalloc(const SbIndex* sb, const RqIndex* ix, const float* queries, uint64_t nq, uint64_t m,
+                       uint64_t k, float* out_scores, int64_t* out_ids, uint64_t flags) {
+    if (sb->dim != ix->dim) fail(kBadArgument, "sign_probe.dim must match RankQuant dim " + std::to_string(ix->dim));
+    if (sb->n != ix->n) fail(kBadArgument, "sign_probe.len must match RankQuant len " + std::to_string(ix->n));
+    const uint64_t out_k = std::min<uint64_t>(k, ix->n);
+    const uint64_t m_eff = std::min<uint64_t>(m, ix->n);
+    if (nq == 0 || out_k == 0) return;
+    NONNULL(queries, "queries");
+    NONNULL(out_scores, "out_scores");
+    NONNULL(out_ids, "out_ids");
+    check_m(m_eff);
+    check_k(out_k);
+    check_search_dim(ix->dim);
+    if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
+    if (nq > kNoAllocMaxQueries)
+        fail(kUnsupportedOperation, "the allocation-free two-stage search takes at most " +
+                                        std::to_string(kNoAllocMaxQueries) + " queries per call, got " +
+                                        std::to_string(nq));
+    const size_t main_bytes = noalloc_main_bytes(sb, ix, nq, m_eff, out_k);
+    const size_t fix1 = stage1_fix_bytes(sb, 1, m_eff);
+    if (main_bytes + fix1 > kNoAllocScratchBytes)
+        fail(kUnsupportedOperation, "the allocation-free two-stage search needs " +
+                                        std::to_string(main_bytes + fix1) + " bytes of scratch for " +
+                                        std::to_string(nq) + " queries over " + std::to_string(ix->n) +
+                                        " documents (m = " + std::to_string(m_eff) + "); its scratch holds " +
+                                        std::to_string(kNoAllocScratchBytes) + " bytes: use fewer queries per call");
+    char* base = noalloc_scratch();
+    NoEventsScope no_events;
+    const uint32_t d = ix->dim;
+    const uint32_t nsplit = rerank_nsplit(nq, m_eff);
+    Carver cv{base};
+    float* qhat = cv.take<float>(nq * d);
+    float* off = cv.take<float>(nq);
+    uint32_t* qstat = cv.take<uint32_t>(nq);
+    uint32_t* qmap = cv.take<uint32_t>(nq);
+    uint32_t* cand = cv.take<uint32_t>(nq * m_eff);
+    void* rwork = cv.take<char>(rerank_work_bytes(nq, (uint32_t)out_k, nsplit));
+    const int tier = select_tier(d, ix->bits);
+    const float inv_norm = rankquant_inv_norm(d, ix->bits);
+    hipStream_t s = cur().stream;
+    if (cv.off + stage1_bytes(sb, nq, m_eff) != main_bytes) fail(kInternal, "allocation-free scratch plan out of step");
+    const PrepFuse prep{queries, qhat, off, nullptr, d, ix->bits, tier, inv_norm};
+    Stage1Bufs b = stage1_dev(sb, queries, nq, m_eff, cand, nullptr, qstat, cv, &prep);
+    launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, nullptr,
+                  (uint32_t)out_k, out_scores, out_ids, nsplit, rwork, s);
+    OVX_HIP(hipMemcpyAsync(t_na_status, qstat, nq * 4, hipMemcpyDeviceToHost, s));
+    OVX_HIP(hipStreamSynchronize(s));
+    uint64_t nf = 0;
+    bool nonfinite = false;
+    for (uint64_t q = 0; q < nq; q++) {
+        nonfinite |= (t_na_status[q] & kStatNonfinite) != 0;
+        if (t_na_status[q] & kStatFallback) t_na_flagged[nf++] = (uint32_t)q;
+    }
+    if (!(flags & ORDVEC_FLAG_ASSUME_FINITE) && nonfinite) fail(kNonfiniteQuery, "queries contain NaN or infinity");
+    if (nf == 0) return;
+    cur().exact_queries += nf;
+    // Recompute the flagged queries in as few chunks as the scratch left over
+    // holds (at least one query: checked above).
+    char* fix_base = base + al(main_bytes);
+    const size_t room = kNoAllocScratchBytes - al(main_bytes);
+    for (uint64_t f0 = 0; f0 < nf;) {
+        uint64_t c = nf - f0;
+        while (c > 1 && stage1_fix_bytes(sb, c, m_eff) > room) c = (c + 1) / 2;
+        stage1_fix_into(sb, b.qbits, t_na_flagged + f0, c, m_eff, cand, nullptr, fix_base);
+        h2d(qmap, t_na_flagged + f0, c * 4);
+        launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, c, nullptr, cand, m_eff, ix->n, qmap,
+                      (uint32_t)out_k, out_scores, out_ids, 1, nullptr, s);
+        OVX_HIP(hipStreamSynchronize(s));
+        f0 += c;
     }
 }
 
@@ -1968,7 +2111,7 @@ KTimes& kt() {
 
 void ktime_begin(int family, hipStream_t s) {
     KTimes& t = kt();
-    if (!((t.mask >> family) & 1u) || t_ctx) return;  // serving slots are not event-timed
+    if (!((t.mask >> family) & 1u) || t_ctx || t_no_events) return;  // serving slots are not event-timed
     hipEvent_t e = t.get();
     OVX_HIP(hipEventRecord(e, s));
     t.open[family] = e;
@@ -1976,7 +2119,7 @@ void ktime_begin(int family, hipStream_t s) {
 
 void ktime_end(int family, hipStream_t s) {
     KTimes& t = kt();
-    if (!((t.mask >> family) & 1u) || t_ctx || !t.open[family]) return;
+    if (!((t.mask >> family) & 1u) || t_ctx || t_no_events || !t.open[family]) return;
     hipEvent_t e = t.get();
     OVX_HIP(hipEventRecord(e, s));
     t.pending.push_back({t.open[family], e, family});
@@ -2036,6 +2179,7 @@ ordvec_status_t ordvec_init(void) {
     return boundary([&] {
         LOCK;
         eng().init();
+        (void)noalloc_scratch();  // the allocation-free entry point's scratch address on this device
     });
 }
 
@@ -3043,6 +3187,17 @@ ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const 
         NONNULL(rq, "rq");
         LOCK;
         two_stage_run(SB(sign), RQ(rq), queries, nq, m, k, out_scores, out_ids, flags);
+    });
+}
+
+ordvec_status_t ordvec_two_stage_search_noalloc(const ordvec_sign_bitmap_t* sign, const ordvec_rankquant_t* rq,
+                                                const float* queries, uint64_t nq, uint64_t m, uint64_t k,
+                                                float* out_scores, int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(sign, "sign");
+        NONNULL(rq, "rq");
+        LOCK;
+        two_stage_noalloc(SB(sign), RQ(rq), queries, nq, m, k, out_scores, out_ids, flags);
     });
 }
 
