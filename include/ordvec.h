@@ -168,7 +168,11 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t *index, const ordvec_se
  * of through its captured HIP graph (same results). */
 #define ORDVEC_FLAG_NO_GRAPH 4
 
-/* Engine limits of this library. */
+/* Engine limits of this library.  Searches with k above ORDVEC_MAX_TOPK and
+ * stage-1 candidate counts above ORDVEC_MAX_STAGE1_M run too, on slower
+ * paths (whole candidate rows sorted; every query on the exact stage-1
+ * pipeline); the limits still bind the serving loop, the allocation-free
+ * entry point, Rank searches and FastScan. */
 #define ORDVEC_MAX_TOPK 1024
 #define ORDVEC_MAX_STAGE1_M 4096
 #define ORDVEC_MAX_ENCODE_DIM 4096
@@ -338,6 +342,42 @@ ordvec_status_t ordvec_bitmap_swap_remove(ordvec_bitmap_t *index, uint64_t idx, 
 ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t *sign, const ordvec_rankquant_t *rq,
                                         const float *queries, uint64_t nq, uint64_t m, uint64_t k,
                                         float *out_scores, int64_t *out_ids, uint64_t flags);
+
+/* TwoStageCandidatePolicy (src/quant.rs:235-262): how many stage-1
+ * candidates a two-stage search takes for a given k.  max_candidates applies
+ * only when has_max_candidates != 0 (the reference's Option).  The default is
+ * {256, 32, none}. */
+typedef struct {
+    uint64_t min_candidates;
+    uint64_t k_multiplier;
+    uint64_t max_candidates;
+    uint32_t has_max_candidates;
+    uint32_t reserved;
+} ordvec_two_stage_policy_t;
+/* src/quant.rs:252-262 Default. */
+void ordvec_two_stage_policy_default(ordvec_two_stage_policy_t *policy);
+/* src/quant.rs:241-250 candidate_count: 0 when k or search_space is 0, else
+ * min(max(min_candidates, k * k_multiplier (saturating)), max_candidates if
+ * set, search_space).  A NULL policy is the default. */
+uint64_t ordvec_two_stage_policy_candidate_count(const ordvec_two_stage_policy_t *policy, uint64_t k,
+                                                 uint64_t search_space);
+/* RankQuant::try_search_with_sign_probe[_with_policy] (src/quant.rs:1453-1497):
+ * one query of `dim` values; m = candidate_count(k, n) sign-bitmap candidates,
+ * then search_asymmetric_subset over them.  Writes min(k, m) hits (capacity
+ * k) and their count to *returned.  policy NULL = the default policy.
+ * Errors in the reference's order: sign dim or length mismatch (BAD_ARGUMENT),
+ * dim != index dim (DIM_MISMATCH), non-finite query (NONFINITE_QUERY). */
+ordvec_status_t ordvec_rankquant_search_with_sign_probe(const ordvec_rankquant_t *rq, const ordvec_sign_bitmap_t *sign,
+                                                        const float *query, uint64_t dim, uint64_t k,
+                                                        const ordvec_two_stage_policy_t *policy, float *out_scores,
+                                                        int64_t *out_ids, uint64_t *returned, uint64_t flags);
+/* ordvec_two_stage_search with m = candidate_count(k, n) of the policy (NULL
+ * = default) for a batch of queries: outputs nq x min(k, n), rows padded with
+ * -inf / -1 past min(k, m). */
+ordvec_status_t ordvec_two_stage_search_policy(const ordvec_sign_bitmap_t *sign, const ordvec_rankquant_t *rq,
+                                               const float *queries, uint64_t nq, uint64_t k,
+                                               const ordvec_two_stage_policy_t *policy, float *out_scores,
+                                               int64_t *out_ids, uint64_t flags);
 
 /* Allocation-free two-stage search.  No reference counterpart: for callers
  * that cannot allocate device memory (code under HIP graph capture, tight

@@ -18,9 +18,11 @@ from .index import (  # noqa: E402,F401
     RankQuant,
     RankQuantFastscan,
     SignBitmap,
+    TwoStageCandidatePolicy,
     stage1_exact_queries,
     synchronize,
     two_stage_search,
+    two_stage_search_policy,
 )
 from .serving import Server  # noqa: E402,F401
 
@@ -37,6 +39,8 @@ __all__ = [
     "SignBitmap",
     "RankQuantFastscan",
     "two_stage_search",
+    "two_stage_search_policy",
+    "TwoStageCandidatePolicy",
     "Server",
     "rerank_scores",
     "stage1_exact_queries",

@@ -21,7 +21,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 OBJDIR = os.path.join(ROOT, "_build")
 LIB = os.path.join(LIBDIR, "libordvec_amd.so")
-SOURCES = ["kernels.hip", "fastscan4.hip", "capi.cpp"]
+SOURCES = ["kernels.hip", "fastscan4.hip", "segsort.hip", "capi.cpp"]
 
 FLAGS = [
     "-O3",

@@ -110,6 +110,11 @@ class ServeReport(C.Structure):
     ]
 
 
+class TwoStagePolicy(C.Structure):
+    _fields_ = [("min_candidates", C.c_uint64), ("k_multiplier", C.c_uint64), ("max_candidates", C.c_uint64),
+                ("has_max_candidates", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class Hit(C.Structure):
     _fields_ = [("row_id", C.c_uint64), ("id", C.c_uint64), ("score", C.c_float), ("reserved", C.c_uint32)]
 
@@ -160,6 +165,10 @@ SIGNATURES = {
     "ordvec_sign_bitmap_free": (None, [P]),
     "ordvec_two_stage_search": (ST, [P, P, P, U64, U64, U64, P, P, U64]),
     "ordvec_two_stage_search_noalloc": (ST, [P, P, P, U64, U64, U64, P, P, U64]),
+    "ordvec_two_stage_policy_default": (None, [P]),
+    "ordvec_two_stage_policy_candidate_count": (U64, [P, U64, U64]),
+    "ordvec_rankquant_search_with_sign_probe": (ST, [P, P, P, U64, U64, P, P, P, P, U64]),
+    "ordvec_two_stage_search_policy": (ST, [P, P, P, U64, U64, P, P, P, U64]),
     "ordvec_bitmap_new": (ST, [U64, U64, C.POINTER(P)]),
     "ordvec_bitmap_add": (ST, [P, P, U64, U64]),
     "ordvec_bitmap_info": (ST, [P, P, P, P]),

@@ -539,13 +539,52 @@ void finish_status(const uint32_t* qstat, uint64_t nq, uint64_t flags, uint32_t 
     if (dev && any_bit(st, kStatNonfinite)) fail(status, std::string(what) + " contains NaN or infinity");
 }
 
+// Full search with k above kTopKMax: every document's score key, a per-query
+// sort, the first k gathered; queries in chunks whose keys fit kLargeKBytes.
+constexpr size_t kLargeKBytes = 1ull << 30;
+
+void rq_full_search_large(const RqIndex* ix, const float* queries, uint64_t nq, uint64_t kk, float* out_s,
+                          int64_t* out_i, uint64_t flags) {
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    const uint32_t d = ix->dim;
+    hipStream_t s = cur().stream;
+    const uint64_t qc = std::max<uint64_t>(1, std::min<uint64_t>({nq, 65535, kLargeKBytes / (16 * ix->n + 1)}));
+    const int tier = select_tier(d, ix->bits);
+    const float inv_norm = rankquant_inv_norm(d, ix->bits);
+    for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
+        const uint64_t cn = std::min(qc, nq - q0);
+        const size_t lk = large_k_work_bytes(cn * ix->n, cn, s);
+        size_t bytes = al(cn * d * 4) + 2 * al(cn * 4) + lk +
+                       (dev ? 0 : al(cn * d * 4) + al(cn * kk * 4) + al(cn * kk * 8));
+        Carver cv{static_cast<char*>(cur().workspace(bytes))};
+        const float* dq = stage_queries(cv, queries + q0 * d, cn * d, flags);
+        float* qhat = cv.take<float>(cn * d);
+        float* off = cv.take<float>(cn);
+        uint32_t* qstat = cv.take<uint32_t>(cn);
+        void* work = cv.take<char>(lk);
+        float* ds = dev ? out_s + q0 * kk : cv.take<float>(cn * kk);
+        int64_t* di = dev ? out_i + q0 * kk : cv.take<int64_t>(cn * kk);
+        launch_prepare_queries(dq, cn, d, ix->bits, tier, inv_norm, qhat, off, qstat, s);
+        launch_rerank_large(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, cn, nullptr, nullptr, ix->n,
+                            cn * ix->n, ix->n, (uint32_t)kk, ds, di, work, s);
+        if (!dev) {
+            d2h(out_s + q0 * kk, ds, cn * kk * 4);
+            d2h(out_i + q0 * kk, di, cn * kk * 8);
+        }
+        finish_status(qstat, cn, flags, kNonfiniteQuery, "queries");
+    }
+}
+
 void rq_full_search(const RqIndex* ix, const float* queries, uint64_t nq, uint64_t k, float* out_s, int64_t* out_i,
                     uint64_t flags) {
     const uint64_t kk = std::min<uint64_t>(k, ix->n);
     if (nq == 0 || kk == 0) return;
-    check_k(kk);
     check_search_dim(ix->dim);
     host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
+    if (kk > kTopKMax) {
+        rq_full_search_large(ix, queries, nq, kk, out_s, out_i, flags);
+        return;
+    }
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
     size_t bytes = full_search_bytes(ix, nq, kk) + al(nq * 4) +
                    (dev ? 0 : al(nq * ix->dim * 4) + al(nq * kk * 4) + al(nq * kk * 8));
@@ -563,16 +602,34 @@ void rq_full_search(const RqIndex* ix, const float* queries, uint64_t nq, uint64
 }
 
 // CSR (offsets != null) or uniform rows; outputs nq x out_k.
+// The large-k paths size their key buffers by the entry count and hand the
+// offsets to a per-row sort, so device CSR offsets are read back and checked
+// (monotone, from 0 to n_entries) before any kernel uses them.
+void check_device_csr(const uint64_t* offsets, uint64_t nq, uint64_t n_entries) {
+    std::vector<uint64_t> h(nq + 1);
+    d2h(h.data(), offsets, (nq + 1) * 8);
+    sync();
+    if (h[0] != 0) fail(kBadArgument, "candidate_offsets[0] must be 0");
+    if (h[nq] != n_entries) fail(kBadArgument, "candidate_offsets[nq] must equal candidates.len()");
+    for (uint64_t q = 0; q < nq; q++)
+        if (h[q + 1] < h[q]) fail(kBadArgument, "candidate_offsets must be monotonic non-decreasing");
+}
+
 void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, const uint64_t* offsets,
                       const uint32_t* cands, uint64_t n_cands, uint64_t row_m, uint64_t out_k, float* out_s,
                       int64_t* out_i, uint64_t flags) {
     if (nq == 0 || out_k == 0) return;
-    check_k(out_k);
     check_search_dim(ix->dim);
+    if (out_k > kTopKMax && nq > 65535)
+        fail(kUnsupportedOperation, "subset search with k above " + std::to_string(kTopKMax) +
+                                        " takes at most 65535 queries per call");
     host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    const bool big_k = out_k > kTopKMax;
+    if (big_k && dev && offsets) check_device_csr(offsets, nq, n_cands);
     const uint64_t avg_m = offsets ? (n_cands + nq - 1) / nq : row_m;
-    size_t bytes = subset_search_bytes(ix, nq, out_k, avg_m) + al(nq * 4) +
+    const size_t lk = big_k ? large_k_work_bytes(n_cands, nq, cur().stream) : 0;
+    size_t bytes = subset_search_bytes(ix, nq, out_k, avg_m) + al(nq * 4) + lk +
                    (dev ? 0 : al(nq * ix->dim * 4) + al((nq + 1) * 8) + al(n_cands * 4) + al(nq * out_k * 4) +
                                   al(nq * out_k * 8));
     Carver cv{static_cast<char*>(cur().workspace(bytes))};
@@ -594,7 +651,19 @@ void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, cons
         ds = cv.take<float>(nq * out_k);
         di = cv.take<int64_t>(nq * out_k);
     }
-    subset_search_dev(ix, dq, nq, doff, dc, row_m, avg_m, out_k, ds, di, qstat, cv);
+    if (big_k) {
+        const uint32_t d = ix->dim;
+        float* qhat = cv.take<float>(nq * d);
+        float* off = cv.take<float>(nq);
+        void* work = cv.take<char>(lk);
+        const int tier = select_tier(d, ix->bits);
+        const float inv_norm = rankquant_inv_norm(d, ix->bits);
+        launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, qstat, cur().stream);
+        launch_rerank_large(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, doff, dc, row_m, n_cands, ix->n,
+                            (uint32_t)out_k, ds, di, work, cur().stream);
+    } else {
+        subset_search_dev(ix, dq, nq, doff, dc, row_m, avg_m, out_k, ds, di, qstat, cv);
+    }
     if (!dev) {
         d2h(out_s, ds, nq * out_k * 4);
         d2h(out_i, di, nq * out_k * 8);
@@ -730,28 +799,50 @@ void check_bitmap_query_dim(const SbIndex* ix) {
 void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint32_t>& fl, uint64_t m_eff,
                 uint32_t* cand, uint32_t* cand_h);
 
+// The fast stage-1 path handles m_eff up to kStage1MaxM; above it every query
+// takes the exact pipeline (ordered scatter, no cap on m).
+uint64_t stage1_fast_m(uint64_t m_eff) { return m_eff > kStage1MaxM ? 0 : m_eff; }
+
+// Stage 1 to completion on the stream: the fast path, then the exact
+// recomputation of the queries it flagged (or of every query when m_eff >
+// kStage1MaxM).  Synchronises once to read the status words; fails on
+// non-finite device queries unless the caller vouched for them.
+void stage1_complete(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t m_eff, uint32_t* cand,
+                     uint32_t* cand_h, uint32_t* qstat, Carver& cv, uint64_t flags, const PrepFuse* prep = nullptr) {
+    Stage1Bufs b = stage1_dev(ix, dq, nq, stage1_fast_m(m_eff), cand, cand_h, qstat, cv, prep);
+    std::vector<uint32_t> st = read_qstat(qstat, nq);
+    if ((flags & ORDVEC_FLAG_DEVICE_PTRS) && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
+        fail(kNonfiniteQuery, "queries contain NaN or infinity");
+    std::vector<uint32_t> fl;
+    if (stage1_fast_m(m_eff) == 0) {
+        fl.resize(nq);
+        for (uint64_t q = 0; q < nq; q++) fl[q] = (uint32_t)q;
+    } else {
+        fl = flagged(st, kStatFallback);
+    }
+    if (!fl.empty()) {
+        cur().exact_queries += fl.size();
+        stage1_fix(ix, b.qbits, fl, m_eff, cand, cand_h);
+    }
+}
+
 // Exact top-m by (distance, id) for a sign bitmap (Hamming) or a Bitmap
 // (n_top - overlap): src/sign_bitmap.rs:231-483, src/bitmap.rs:264-362.
 // Outputs nq x m_eff ids and, when requested, distances.
 void sb_top_m(const SbIndex* ix, const float* queries, uint64_t nq, uint64_t m_eff, uint32_t* out_c, uint32_t* out_h,
               uint64_t flags) {
-    check_m(m_eff);
     if (ix->ntop) check_bitmap_query_dim(ix);
     else if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
     host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-    size_t bytes = stage1_bytes(ix, nq, m_eff) + al(nq * 4) + (dev ? 0 : al(nq * ix->dim * 4) + 2 * al(nq * m_eff * 4));
+    size_t bytes = stage1_bytes(ix, nq, stage1_fast_m(m_eff)) + al(nq * 4) +
+                   (dev ? 0 : al(nq * ix->dim * 4) + 2 * al(nq * m_eff * 4));
     Carver cv{static_cast<char*>(cur().workspace(bytes))};
     const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
     uint32_t* qstat = cv.take<uint32_t>(nq);
     uint32_t* dc = dev ? out_c : cv.take<uint32_t>(nq * m_eff);
     uint32_t* dh = dev ? out_h : (out_h ? cv.take<uint32_t>(nq * m_eff) : nullptr);
-    Stage1Bufs b = stage1_dev(ix, dq, nq, m_eff, dc, dh, qstat, cv);
-    std::vector<uint32_t> st = read_qstat(qstat, nq);
-    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
-        fail(kNonfiniteQuery, "queries contain NaN or infinity");
-    std::vector<uint32_t> fl = flagged(st, kStatFallback);
-    if (!fl.empty()) stage1_fix(ix, b.qbits, fl, m_eff, dc, dh);
+    stage1_complete(ix, dq, nq, m_eff, dc, dh, qstat, cv, flags);
     if (!dev) {
         d2h(out_c, dc, nq * m_eff * 4);
         if (out_h) d2h(out_h, dh, nq * m_eff * 4);
@@ -763,11 +854,10 @@ void sb_top_m(const SbIndex* ix, const float* queries, uint64_t nq, uint64_t m_e
 // scores = overlap as f32.  k_eff = min(k, n) > 0.
 void bm_search(const SbIndex* ix, const float* queries, uint64_t nq, uint64_t k_eff, float* out_s, int64_t* out_i,
                uint64_t flags) {
-    if (k_eff > kStage1MaxM) fail(kUnsupportedOperation, "GPU Bitmap search supports k <= " + std::to_string(kStage1MaxM));
     check_bitmap_query_dim(ix);
     host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-    size_t bytes = stage1_bytes(ix, nq, k_eff) + al(nq * 4) + 2 * al(nq * k_eff * 4) +
+    size_t bytes = stage1_bytes(ix, nq, stage1_fast_m(k_eff)) + al(nq * 4) + 2 * al(nq * k_eff * 4) +
                    (dev ? 0 : al(nq * ix->dim * 4) + al(nq * k_eff * 4) + al(nq * k_eff * 8));
     Carver cv{static_cast<char*>(cur().workspace(bytes))};
     const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
@@ -776,12 +866,7 @@ void bm_search(const SbIndex* ix, const float* queries, uint64_t nq, uint64_t k_
     uint32_t* h = cv.take<uint32_t>(nq * k_eff);
     float* ds = dev ? out_s : cv.take<float>(nq * k_eff);
     int64_t* di = dev ? out_i : cv.take<int64_t>(nq * k_eff);
-    Stage1Bufs b = stage1_dev(ix, dq, nq, k_eff, c, h, qstat, cv);
-    std::vector<uint32_t> st = read_qstat(qstat, nq);
-    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
-        fail(kNonfiniteQuery, "queries contain NaN or infinity");
-    std::vector<uint32_t> fl = flagged(st, kStatFallback);
-    if (!fl.empty()) stage1_fix(ix, b.qbits, fl, k_eff, c, h);
+    stage1_complete(ix, dq, nq, k_eff, c, h, qstat, cv, flags);
     launch_bm_finish(c, h, nq * k_eff, ix->ntop, ds, di, cur().stream);
     if (!dev) {
         d2h(out_s, ds, nq * k_eff * 4);
@@ -794,11 +879,16 @@ void bm_search(const SbIndex* ix, const float* queries, uint64_t nq, uint64_t k_
 void bm_subset(const SbIndex* ix, const float* queries, uint64_t nq, const uint64_t* offsets, const uint32_t* rows,
                uint64_t n_rows, uint64_t out_k, float* out_s, int64_t* out_i, uint64_t flags) {
     if (nq == 0 || out_k == 0) return;
-    check_k(out_k);
     check_bitmap_query_dim(ix);
+    const bool big_k = out_k > kTopKMax;
+    if (big_k && nq > 65535)
+        fail(kUnsupportedOperation, "Bitmap subset search with k above " + std::to_string(kTopKMax) +
+                                        " takes at most 65535 queries per call");
     host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
-    size_t bytes = al(nq * ix->qpv * 8) + al(nq * 4) +
+    if (big_k && dev) check_device_csr(offsets, nq, n_rows);
+    const size_t lk = big_k ? large_k_work_bytes(n_rows, nq, cur().stream) : 0;
+    size_t bytes = al(nq * ix->qpv * 8) + al(nq * 4) + lk +
                    (dev ? 0 : al(nq * ix->dim * 4) + al((nq + 1) * 8) + al(n_rows * 4) + al(nq * out_k * 12));
     Carver cv{static_cast<char*>(cur().workspace(bytes))};
     const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
@@ -819,7 +909,11 @@ void bm_subset(const SbIndex* ix, const float* queries, uint64_t nq, const uint6
         di = cv.take<int64_t>(nq * out_k);
     }
     launch_bitmap_query_bits(dq, nq, ix->dim, ix->ntop, qb, qstat, cur().stream);
-    launch_bm_subset(ix->words, ix->dim, ix->n, qb, nq, doff, dr, ix->ntop, (uint32_t)out_k, ds, di, cur().stream);
+    if (big_k)
+        launch_bm_subset_large(ix->words, ix->dim, ix->n, qb, nq, doff, dr, n_rows, ix->ntop, (uint32_t)out_k, ds, di,
+                               cv.take<char>(lk), cur().stream);
+    else
+        launch_bm_subset(ix->words, ix->dim, ix->n, qb, nq, doff, dr, ix->ntop, (uint32_t)out_k, ds, di, cur().stream);
     if (!dev) {
         d2h(out_s, ds, nq * out_k * 4);
         d2h(out_i, di, nq * out_k * 8);
@@ -855,11 +949,59 @@ void stage1_fix_into(const SbIndex* ix, const uint64_t* qbits, const uint32_t* f
     }
 }
 
-// The same in the context's second workspace.
+// The same in the context's second workspace, in chunks of queries whose
+// scratch (2 bytes per document per query and the segment histograms) stays
+// within kFixChunkBytes (at least one query per chunk).
+constexpr size_t kFixChunkBytes = 1ull << 30;
+
 void stage1_fix(const SbIndex* ix, const uint64_t* qbits, const std::vector<uint32_t>& fl, uint64_t m_eff,
                 uint32_t* cand, uint32_t* cand_h) {
-    char* base = static_cast<char*>(cur().workspace2(stage1_fix_bytes(ix, fl.size(), m_eff)));
-    stage1_fix_into(ix, qbits, fl.data(), fl.size(), m_eff, cand, cand_h, base);
+    const uint64_t nf = fl.size();
+    uint64_t c = nf;
+    while (c > 1 && stage1_fix_bytes(ix, c, m_eff) > kFixChunkBytes) c = (c + 1) / 2;
+    char* base = static_cast<char*>(cur().workspace2(stage1_fix_bytes(ix, c, m_eff)));
+    for (uint64_t f0 = 0; f0 < nf; f0 += c)
+        stage1_fix_into(ix, qbits, fl.data() + f0, std::min(c, nf - f0), m_eff, cand, cand_h, base);
+}
+
+// Two-stage search past the fast path's limits: m_eff > kStage1MaxM (every
+// query takes the exact stage-1 pipeline) or out_k > kTopKMax (the rerank
+// sorts whole candidate rows).  Stage 1 completes (one status wait) before
+// the rerank is enqueued.
+void two_stage_big(const SbIndex* sb, const RqIndex* ix, const float* queries, uint64_t nq, uint64_t m_eff,
+                   uint64_t out_k, float* out_scores, int64_t* out_ids, uint64_t flags) {
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    const bool big_k = out_k > kTopKMax;
+    const uint32_t d = ix->dim;
+    hipStream_t s = cur().stream;
+    const uint32_t nsplit = big_k ? 1 : rerank_nsplit(nq, m_eff);
+    const size_t rr = big_k ? large_k_work_bytes(nq * m_eff, nq, s) : rerank_work_bytes(nq, (uint32_t)out_k, nsplit);
+    size_t bytes = al(nq * d * 4) + 2 * al(nq * 4) + al(nq * m_eff * 4) + stage1_bytes(sb, nq, stage1_fast_m(m_eff)) +
+                   (dev ? 0 : al(nq * d * 4) + al(nq * out_k * 4) + al(nq * out_k * 8)) + rr;
+    Carver cv{static_cast<char*>(cur().workspace(bytes))};
+    const float* dq = stage_queries(cv, queries, nq * d, flags);
+    float* qhat = cv.take<float>(nq * d);
+    float* off = cv.take<float>(nq);
+    uint32_t* qstat = cv.take<uint32_t>(nq);
+    uint32_t* cand = cv.take<uint32_t>(nq * m_eff);
+    void* rwork = cv.take<char>(rr);
+    float* ds = dev ? out_scores : cv.take<float>(nq * out_k);
+    int64_t* di = dev ? out_ids : cv.take<int64_t>(nq * out_k);
+    const int tier = select_tier(d, ix->bits);
+    const float inv_norm = rankquant_inv_norm(d, ix->bits);
+    const PrepFuse prep{dq, qhat, off, nullptr, d, ix->bits, tier, inv_norm};
+    stage1_complete(sb, dq, nq, m_eff, cand, nullptr, qstat, cv, flags, &prep);
+    if (big_k)
+        launch_rerank_large(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, nq * m_eff,
+                            ix->n, (uint32_t)out_k, ds, di, rwork, s);
+    else
+        launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, nullptr,
+                      (uint32_t)out_k, ds, di, nsplit, rwork, s);
+    if (!dev) {
+        d2h(out_scores, ds, nq * out_k * 4);
+        d2h(out_ids, di, nq * out_k * 8);
+        sync();
+    }
 }
 
 // Two-stage search on the calling thread's context (src/sign_bitmap.rs:453-483
@@ -874,12 +1016,14 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
     NONNULL(queries, "queries");
     NONNULL(out_scores, "out_scores");
     NONNULL(out_ids, "out_ids");
-    check_m(m_eff);
-    check_k(out_k);
     check_search_dim(ix->dim);
     if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
     host_finite_or_fail(queries, nq * ix->dim, flags, kNonfiniteQuery, "queries");
     eng().init();
+    if (m_eff > kStage1MaxM || out_k > kTopKMax) {
+        two_stage_big(sb, ix, queries, nq, m_eff, out_k, out_scores, out_ids, flags);
+        return;
+    }
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
     const uint32_t d = ix->dim;
     const uint32_t nsplit = rerank_nsplit(nq, m_eff);
@@ -3185,6 +3329,71 @@ ordvec_status_t ordvec_two_stage_search(const ordvec_sign_bitmap_t* sign, const 
     return boundary([&] {
         NONNULL(sign, "sign");
         NONNULL(rq, "rq");
+        LOCK;
+        two_stage_run(SB(sign), RQ(rq), queries, nq, m, k, out_scores, out_ids, flags);
+    });
+}
+
+void ordvec_two_stage_policy_default(ordvec_two_stage_policy_t* policy) {
+    if (!policy) return;
+    *policy = ordvec_two_stage_policy_t{};
+    policy->min_candidates = 256;
+    policy->k_multiplier = 32;
+}
+
+uint64_t ordvec_two_stage_policy_candidate_count(const ordvec_two_stage_policy_t* policy, uint64_t k,
+                                                 uint64_t search_space) {
+    ordvec_two_stage_policy_t def;
+    ordvec_two_stage_policy_default(&def);
+    const ordvec_two_stage_policy_t& p = policy ? *policy : def;
+    if (k == 0 || search_space == 0) return 0;
+    uint64_t prod = 0;
+    if (__builtin_mul_overflow(k, p.k_multiplier, &prod)) prod = UINT64_MAX;  // saturating_mul
+    uint64_t count = std::max(p.min_candidates, prod);
+    if (p.has_max_candidates) count = std::min(count, p.max_candidates);
+    return std::min(count, search_space);
+}
+
+ordvec_status_t ordvec_rankquant_search_with_sign_probe(const ordvec_rankquant_t* rq, const ordvec_sign_bitmap_t* sign,
+                                                        const float* query, uint64_t dim, uint64_t k,
+                                                        const ordvec_two_stage_policy_t* policy, float* out_scores,
+                                                        int64_t* out_ids, uint64_t* returned, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(rq, "rq");
+        NONNULL(sign, "sign");
+        NONNULL(returned, "returned");
+        *returned = 0;
+        const RqIndex* ix = RQ(rq);
+        const SbIndex* sb = SB(sign);
+        // src/quant.rs:1474-1492: parameter checks in the reference's order.
+        if (sb->dim != ix->dim) fail(kBadArgument, "sign_probe.dim must match RankQuant dim " + std::to_string(ix->dim));
+        if (sb->n != ix->n) fail(kBadArgument, "sign_probe.len must match RankQuant len " + std::to_string(ix->n));
+        if (dim != ix->dim)
+            fail(kDimMismatch, "query has length " + std::to_string(dim) + ", expected " + std::to_string(ix->dim));
+        NONNULL(query, "query");
+        if (!(flags & (ORDVEC_FLAG_DEVICE_PTRS | ORDVEC_FLAG_ASSUME_FINITE)) && !host_all_finite(query, dim))
+            fail(kNonfiniteQuery, "query must contain only finite values");
+        const uint64_t m = ordvec_two_stage_policy_candidate_count(policy, k, ix->n);
+        const uint64_t ret = std::min(k, m);
+        if (ret == 0) return;
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        LOCK;
+        // The top min(k, m) of the m candidates: a two-stage row of width
+        // min(k, m) (= search_asymmetric_subset's output length).
+        two_stage_run(sb, ix, query, 1, m, ret, out_scores, out_ids, flags);
+        *returned = ret;
+    });
+}
+
+ordvec_status_t ordvec_two_stage_search_policy(const ordvec_sign_bitmap_t* sign, const ordvec_rankquant_t* rq,
+                                               const float* queries, uint64_t nq, uint64_t k,
+                                               const ordvec_two_stage_policy_t* policy, float* out_scores,
+                                               int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(sign, "sign");
+        NONNULL(rq, "rq");
+        const uint64_t m = ordvec_two_stage_policy_candidate_count(policy, k, RQ(rq)->n);
         LOCK;
         two_stage_run(SB(sign), RQ(rq), queries, nq, m, k, out_scores, out_ids, flags);
     });

@@ -150,8 +150,9 @@ struct Stage1Work {
     uint16_t* hist_part;  // [nq][nseg][nbins]
     uint32_t* thr;        // [nq][4]
     uint32_t* counters;   // [nq][2]
-    uint64_t* below;      // [nq][m_eff]
-    uint32_t* ties;       // [nq][kTieCap]
+    uint64_t* below;      // [nq][m_eff]       (m_eff <= kStage1MaxM)
+    uint32_t* ties;       // [nq][kTieCap]     (m_eff <= kStage1MaxM)
+    uint32_t* run_off;    // [nq][nseg][nbins] (m_eff > kStage1MaxM: ordered scatter)
 };
 struct Stage1Plan {
     uint32_t qt, nseg, blocks_per_seg, nbins, hwords, lds_bytes, nseg3;
@@ -216,6 +217,26 @@ void launch_rerank(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, fl
                    const float* offset, uint64_t nq, const uint64_t* offsets, const uint32_t* cands, uint64_t row_m,
                    uint64_t n_docs, const uint32_t* qmap, uint32_t out_k, float* out_s, int64_t* out_i,
                    uint32_t nsplit, void* work, hipStream_t s);
+
+// Per-row ascending sort of 64-bit keys (segsort.hip); rows are
+// [offsets[i], offsets[i+1]) of a device array of nseg + 1 offsets.
+size_t segsort_temp_bytes(uint64_t total, uint32_t nseg, const uint64_t* offsets, hipStream_t s);
+void segsort_keys(void* temp, size_t temp_bytes, const uint64_t* in, uint64_t* out, uint64_t total, uint32_t nseg,
+                  const uint64_t* offsets, hipStream_t s);
+
+// Large k (out_k above kTopKMax): score keys of whole rows, per-row sort,
+// gather of the first out_k.  Rows are CSR (offsets, device, nq + 1 entries),
+// uniform rows of row_m candidates, or every document (cands == null, row_m =
+// n_docs); total = number of entries over all rows; work =
+// large_k_work_bytes(total, nq).
+size_t large_k_work_bytes(uint64_t total, uint64_t nq, hipStream_t s);
+void launch_rerank_large(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat,
+                         const float* offset, uint64_t nq, const uint64_t* offsets, const uint32_t* cands,
+                         uint64_t row_m, uint64_t total, uint64_t n_docs, uint32_t out_k, float* out_s,
+                         int64_t* out_i, void* work, hipStream_t s);
+void launch_bm_subset_large(const uint64_t* words, uint32_t d, uint64_t n_docs, const uint64_t* qbits, uint64_t nq,
+                            const uint64_t* offsets, const uint32_t* rows, uint64_t total, uint32_t ntop,
+                            uint32_t out_k, float* out_s, int64_t* out_i, void* work, hipStream_t s);
 
 // Full scans: per (query, segment) block top-k into partial keys, then merge.
 size_t fullscan_work_bytes(uint64_t n, uint64_t nq, uint32_t k);

@@ -437,6 +437,109 @@ __global__ __launch_bounds__(256) void k_stage1_finalize(const uint16_t* __restr
     }
 }
 
+// Large m (m_eff > kStage1MaxM): the exact top-m by an ordered scatter
+// instead of compact + sort.  The documents with hamming == h of segment s
+// take the output positions starting at
+//   run(s, h) = sum_{h' < h} count(h') + sum_{s' < s} hist_part[s'][h],
+// in id order, so every document with hamming <= h* and position < m_eff
+// lands at its place in the (hamming asc, id asc) order with no sort.
+// K5: per query, run(s, h) for h <= h* into off[q][s][h] (u32).
+__global__ __launch_bounds__(256) void k_stage1_run_offsets(const uint16_t* __restrict__ hist_part, uint32_t nseg,
+                                                            uint32_t nbins, const uint32_t* __restrict__ thr,
+                                                            uint32_t* __restrict__ off) {
+    extern __shared__ uint32_t start[];
+    __shared__ uint32_t part[256];
+    const uint64_t q = blockIdx.x;
+    const uint16_t* hp = hist_part + q * nseg * nbins;
+    const uint32_t hstar = thr[q * 4 + 0];
+    for (uint32_t bin = threadIdx.x; bin < nbins; bin += blockDim.x) {
+        uint32_t c = 0;
+        if (bin <= hstar)
+            for (uint32_t g = 0; g < nseg; g++) c += hp[(uint64_t)g * nbins + bin];
+        start[bin] = c;
+    }
+    __syncthreads();
+    // exclusive prefix over bins: per-thread runs, then one pass over the runs
+    const uint32_t per = (nbins + blockDim.x - 1) / blockDim.x;
+    const uint32_t lo = min(nbins, threadIdx.x * per), hi = min(nbins, lo + per);
+    uint32_t local = 0;
+    for (uint32_t b = lo; b < hi; b++) local += start[b];
+    part[threadIdx.x] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t t = 0; t < blockDim.x; t++) {
+            const uint32_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    uint32_t cum = part[threadIdx.x];
+    for (uint32_t b = lo; b < hi; b++) {
+        const uint32_t c = start[b];
+        start[b] = cum;
+        cum += c;
+    }
+    __syncthreads();
+    uint32_t* oq = off + q * (uint64_t)nseg * nbins;
+    for (uint32_t bin = threadIdx.x; bin <= hstar && bin < nbins; bin += blockDim.x) {
+        uint32_t run = start[bin];
+        for (uint32_t g = 0; g < nseg; g++) {
+            oq[(uint64_t)g * nbins + bin] = run;
+            run += hp[(uint64_t)g * nbins + bin];
+        }
+    }
+}
+
+// K6: grid (segments, queries), one wave per workgroup walking its segment's
+// documents in id order, 64 at a time.  Lanes with hamming <= h* are placed
+// one distinct hamming value at a time: a ballot of the lanes sharing the
+// leading lane's value gives each its order, the run counter in LDS its base.
+__global__ __launch_bounds__(64) void k_stage1_scatter(const uint16_t* __restrict__ H, uint64_t n, uint64_t n_pad,
+                                                       uint32_t blocks_per_seg, uint32_t nseg, uint32_t nbins,
+                                                       const uint32_t* __restrict__ thr,
+                                                       const uint32_t* __restrict__ off, uint64_t m_eff,
+                                                       uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_h) {
+    extern __shared__ uint32_t run[];
+    const uint64_t q = blockIdx.y;
+    const uint32_t seg = blockIdx.x;
+    const uint32_t hstar = thr[q * 4 + 0];
+    const uint32_t lane = threadIdx.x;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t* oq = off + (q * nseg + seg) * (uint64_t)nbins;
+    for (uint32_t b = lane; b <= hstar && b < nbins; b += 64) run[b] = oq[b];
+    __syncthreads();
+    const uint64_t begin = (uint64_t)seg * blocks_per_seg * 64;
+    const uint64_t end = min(n, begin + (uint64_t)blocks_per_seg * 64);
+    const uint16_t* row = H + q * n_pad;
+    uint32_t* cq = cand + q * m_eff;
+    uint32_t* hq = cand_h ? cand_h + q * m_eff : nullptr;
+    for (uint64_t base = begin; base < end; base += 64) {
+        const uint64_t idx = base + lane;
+        const uint32_t h = idx < end ? (uint32_t)row[idx] : 0xffffffffu;
+        bool active = h <= hstar;
+        uint64_t mask = __ballot(active);
+        while (mask) {
+            const int lead = __ffsll((long long)mask) - 1;
+            const uint32_t hv = __shfl(h, lead);
+            const bool mine = active && h == hv;
+            const uint64_t same = __ballot(mine);
+            const uint32_t basepos = run[hv];
+            __syncthreads();
+            const uint64_t pos = (uint64_t)basepos + (uint32_t)__popcll(same & lt_mask);
+            if (mine && pos < m_eff) {
+                cq[pos] = (uint32_t)idx;
+                if (hq) hq[pos] = h;
+            }
+            if (lane == (uint32_t)lead) run[hv] = basepos + (uint32_t)__popcll(same);
+            __syncthreads();
+            active = active && !mine;
+            mask &= ~same;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Stage 1, fast path: sampled threshold + survivor compaction.
 //
@@ -2137,6 +2240,95 @@ __global__ __launch_bounds__(256) void k_score_ids(const uint8_t* __restrict__ c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Large k: score keys of whole candidate rows (or of every document), sorted
+// per row by the caller, then gathered.  The keys order (score desc, id asc)
+// exactly as TopK does, and the scores are the search kernels' own.
+// ---------------------------------------------------------------------------
+// keys[j] for j in query q's row [b, e): CSR (offsets), uniform rows of
+// row_m candidates, or every document (cands == null: row_m = n_docs, id = j - b).
+// Candidates outside [0, n_docs) get kKeyMax (dropped, as the rerank drops them).
+template <int TIER, int BITS>
+__global__ __launch_bounds__(256) void k_score_keys(const uint8_t* __restrict__ codes, uint64_t n_docs, uint32_t d,
+                                                    float inv_norm, const float* __restrict__ qhat,
+                                                    const float* __restrict__ offset,
+                                                    const uint64_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ cands, uint64_t row_m,
+                                                    uint64_t* __restrict__ keys) {
+    extern __shared__ float qs[];
+    const uint64_t q = blockIdx.y;
+    if (TIER == 2) stage_q_mix<BITS>(qs, qhat + q * d, d);
+    else stage_q(qs, qhat + q * d, d);
+    __syncthreads();
+    const float off = offset[q];
+    const uint32_t bpv = d * BITS / 8;
+    const uint64_t b = offsets ? offsets[q] : q * row_m;
+    const uint64_t e = offsets ? offsets[q + 1] : (q + 1) * row_m;
+    const uint64_t per = (e - b + gridDim.x - 1) / gridDim.x;
+    const uint64_t sb = min(e, b + blockIdx.x * per), se = min(e, sb + per);
+    if (TIER == 2) {
+        const uint32_t quad = threadIdx.x >> 2, jl = threadIdx.x & 3;
+        for (uint64_t r0 = sb; r0 < se; r0 += 64) {
+            const uint64_t j = r0 + quad;
+            const uint64_t id = j < se ? (cands ? (uint64_t)cands[j] : j - b) : 0;
+            const bool ok = j < se && id < n_docs;
+            const float raw = raw_avx512_quad<BITS>(codes + (ok ? id : 0) * bpv, d, qs, jl);
+            if (j < se && jl == 0)
+                keys[j] = ok ? score_key(__fadd_rn(__fmul_rn(raw, inv_norm), off), (uint32_t)id) : kKeyMax;
+        }
+    } else {
+        for (uint64_t j = sb + threadIdx.x; j < se; j += blockDim.x) {
+            const uint64_t id = cands ? (uint64_t)cands[j] : j - b;
+            keys[j] = id < n_docs ? score_key(doc_score<TIER, BITS>(codes + id * bpv, d, qs, inv_norm, off), (uint32_t)id)
+                                  : kKeyMax;
+        }
+    }
+}
+
+// Top-bucket Bitmap keys ((n_top - overlap) << 32 | row) of CSR rows; rows
+// outside [0, n_docs) get kKeyMax.
+__global__ __launch_bounds__(256) void k_bm_keys(const uint64_t* __restrict__ words, uint32_t qpv, uint64_t n_docs,
+                                                 const uint64_t* __restrict__ qbits,
+                                                 const uint64_t* __restrict__ offsets,
+                                                 const uint32_t* __restrict__ rows, uint32_t ntop,
+                                                 uint64_t* __restrict__ keys) {
+    const uint64_t q = blockIdx.y;
+    const uint64_t b = offsets[q], e = offsets[q + 1];
+    const uint64_t* qb = qbits + q * qpv;
+    for (uint64_t j = b + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < e; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = rows[j];
+        if (r >= n_docs) {
+            keys[j] = kKeyMax;
+            continue;
+        }
+        const uint64_t* dw = words + (r / 64) * (uint64_t)qpv * 64 + (r % 64);
+        uint32_t acc = 0;
+        for (uint32_t w = 0; w < qpv; w++) acc += (uint32_t)__popcll(dw[(uint64_t)w * 64] & qb[w]);
+        keys[j] = ((uint64_t)(ntop - acc) << 32) | r;
+    }
+}
+
+// Row q of the output: the first min(out_k, row length) sorted keys of its
+// row, decoded; the rest (and kKeyMax keys) padded -inf / -1.  bitmap_ntop > 0
+// decodes Bitmap keys (score = overlap as f32), else score keys.
+__global__ __launch_bounds__(256) void k_gather_keys(const uint64_t* __restrict__ sorted,
+                                                     const uint64_t* __restrict__ offsets, uint64_t row_m,
+                                                     uint32_t out_k, uint32_t bitmap_ntop, float* __restrict__ out_s,
+                                                     int64_t* __restrict__ out_i) {
+    const uint64_t q = blockIdx.y;
+    const uint64_t b = offsets ? offsets[q] : q * row_m;
+    const uint64_t e = offsets ? offsets[q + 1] : (q + 1) * row_m;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < out_k; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = b + j < e ? sorted[b + j] : kKeyMax;
+        float sc;
+        if (key == kKeyMax) sc = -INFINITY;
+        else if (bitmap_ntop) sc = (float)(bitmap_ntop - (uint32_t)(key >> 32));
+        else sc = key_score(key);
+        out_s[q * out_k + j] = sc;
+        out_i[q * out_k + j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
+    }
+}
+
 // Backward of k_score_ids through the query normalisation.  With
 // s_i = sum_j qhat_j * c(code_ij) / R and qhat = x / ||x||:
 //   g_j  = sum_i G_i * c(code_ij) / R          (re-decoded from the codes)
@@ -2441,18 +2633,22 @@ Stage1Plan plan_stage1(uint64_t n, uint32_t d, uint64_t nq) {
     return p;
 }
 
+// m_eff above kStage1MaxM takes the ordered scatter (run offsets, no below /
+// ties buffers); smaller m the compact + sort.
 size_t stage1_work_bytes(const Stage1Plan& p, uint64_t nq, uint64_t m_eff) {
+    const bool big = m_eff > kStage1MaxM;
     size_t b = 0;
     b += align256(nq * p.n_pad * 2);
     b += align256(nq * (uint64_t)p.nseg * p.nbins * 2);
     b += align256(nq * 16);
     b += align256(nq * 8);
-    b += align256(nq * m_eff * 8);
-    b += align256(nq * (uint64_t)kTieCap * 4);
+    b += big ? align256(nq * (uint64_t)p.nseg * p.nbins * 4) : align256(nq * m_eff * 8);
+    b += big ? 0 : align256(nq * (uint64_t)kTieCap * 4);
     return b;
 }
 
 void carve_stage1(const Stage1Plan& p, uint64_t nq, uint64_t m_eff, void* base, Stage1Work* w) {
+    const bool big = m_eff > kStage1MaxM;
     char* c = static_cast<char*>(base);
     w->H = reinterpret_cast<uint16_t*>(c);
     c += align256(nq * p.n_pad * 2);
@@ -2462,6 +2658,13 @@ void carve_stage1(const Stage1Plan& p, uint64_t nq, uint64_t m_eff, void* base, 
     c += align256(nq * 16);
     w->counters = reinterpret_cast<uint32_t*>(c);
     c += align256(nq * 8);
+    w->below = nullptr;
+    w->ties = nullptr;
+    w->run_off = nullptr;
+    if (big) {
+        w->run_off = reinterpret_cast<uint32_t*>(c);
+        return;
+    }
     w->below = reinterpret_cast<uint64_t*>(c);
     c += align256(nq * m_eff * 8);
     w->ties = reinterpret_cast<uint32_t*>(c);
@@ -2503,6 +2706,16 @@ void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t
     hipLaunchKernelGGL(k_stage1_threshold, dim3((uint32_t)nq), dim3(256), (size_t)p.nbins * 4, s, w.hist_part, p.nseg,
                        p.nbins, m_eff, w.thr, w.counters);
     check_launch();
+    if (m_eff > kStage1MaxM) {
+        hipLaunchKernelGGL(k_stage1_run_offsets, dim3((uint32_t)nq), dim3(256), (size_t)p.nbins * 4, s, w.hist_part,
+                           p.nseg, p.nbins, w.thr, w.run_off);
+        check_launch();
+        hipLaunchKernelGGL(k_stage1_scatter, dim3(p.nseg, (uint32_t)nq), dim3(64), (size_t)p.nbins * 4, s, w.H, n,
+                           p.n_pad, p.blocks_per_seg, p.nseg, p.nbins, w.thr, w.run_off, m_eff, cand, cand_h);
+        check_launch();
+        ktime_end(kFamStage1Select, s);
+        return;
+    }
     uint64_t per3 = (n + p.nseg3 - 1) / p.nseg3;
     per3 = (per3 + 7) & ~7ull;
     hipLaunchKernelGGL(k_stage1_compact, dim3(p.nseg3, (uint32_t)nq), dim3(256), 0, s, w.H, n, p.n_pad, per3, w.thr,
@@ -3029,6 +3242,86 @@ void launch_fastscan_prepare(const float* qhat, uint64_t nq, uint32_t d, uint8_t
     hipLaunchKernelGGL(k_fastscan_prepare, dim3((uint32_t)nq), dim3(256), 0, s, qhat, d, lut, bias_invq);
     check_launch();
     ktime_end(kFamFastScan, s);
+}
+
+// Large k: uniform row offsets q * row_m for the segmented sort.
+__global__ void k_row_offsets(uint64_t* __restrict__ offs, uint64_t nq, uint64_t row_m) {
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= nq; q += (uint64_t)gridDim.x * blockDim.x)
+        offs[q] = q * row_m;
+}
+
+struct LargeKWork {
+    uint64_t *keys, *sorted, *offs;
+    void* temp;
+    size_t temp_bytes;
+};
+
+static LargeKWork carve_large_k(void* work, uint64_t total, uint64_t nq, hipStream_t s) {
+    LargeKWork w{};
+    char* c = static_cast<char*>(work);
+    w.keys = reinterpret_cast<uint64_t*>(c);
+    c += align256(total * 8);
+    w.sorted = reinterpret_cast<uint64_t*>(c);
+    c += align256(total * 8);
+    w.offs = reinterpret_cast<uint64_t*>(c);
+    c += align256((nq + 1) * 8);
+    w.temp = c;
+    w.temp_bytes = segsort_temp_bytes(total, (uint32_t)nq, nullptr, s);
+    return w;
+}
+
+size_t large_k_work_bytes(uint64_t total, uint64_t nq, hipStream_t s) {
+    return 2 * align256(total * 8) + align256((nq + 1) * 8) + segsort_temp_bytes(total, (uint32_t)nq, nullptr, s);
+}
+
+// Rows split across workgroups of about 4096 entries each.
+static uint32_t large_k_chunks(uint64_t total, uint64_t nq) {
+    const uint64_t avg = (total + nq - 1) / std::max<uint64_t>(1, nq);
+    return (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(1, (avg + 4095) / 4096));
+}
+
+void launch_rerank_large(const uint8_t* codes, uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat,
+                         const float* offset, uint64_t nq, const uint64_t* offsets, const uint32_t* cands,
+                         uint64_t row_m, uint64_t total, uint64_t n_docs, uint32_t out_k, float* out_s,
+                         int64_t* out_i, void* work, hipStream_t s) {
+    if (nq == 0 || out_k == 0) return;
+    if (nq > 65535) throw Error(kInternal, "large-k rerank: more than 65535 queries in one launch");
+    LargeKWork w = carve_large_k(work, total, nq, s);
+    const uint64_t* segoff = offsets;
+    if (!offsets) {
+        hipLaunchKernelGGL(k_row_offsets, dim3((uint32_t)std::min<uint64_t>(1024, (nq + 256) / 256)), dim3(256), 0, s,
+                           w.offs, nq, row_m);
+        check_launch();
+        segoff = w.offs;
+    }
+    ktime_begin(kFamRerank, s);
+    const dim3 grid(large_k_chunks(total, nq), (uint32_t)nq);
+#define OVX_KEYS(T, B, _)                                                                                           \
+    hipLaunchKernelGGL((k_score_keys<T, B>), grid, dim3(256), (size_t)d * 4, s, codes, n_docs, d, inv_norm, qhat, \
+                       offset, offsets, cands, row_m, w.keys)
+    OVX_TIER_BITS_DISPATCH(OVX_KEYS, 0);
+#undef OVX_KEYS
+    check_launch();
+    segsort_keys(w.temp, w.temp_bytes, w.keys, w.sorted, total, (uint32_t)nq, segoff, s);
+    hipLaunchKernelGGL(k_gather_keys, dim3((out_k + 255) / 256, (uint32_t)nq), dim3(256), 0, s, w.sorted, segoff,
+                       row_m, out_k, 0u, out_s, out_i);
+    check_launch();
+    ktime_end(kFamRerank, s);
+}
+
+void launch_bm_subset_large(const uint64_t* words, uint32_t d, uint64_t n_docs, const uint64_t* qbits, uint64_t nq,
+                            const uint64_t* offsets, const uint32_t* rows, uint64_t total, uint32_t ntop,
+                            uint32_t out_k, float* out_s, int64_t* out_i, void* work, hipStream_t s) {
+    if (nq == 0 || out_k == 0) return;
+    if (nq > 65535) throw Error(kInternal, "large-k Bitmap subset: more than 65535 queries in one launch");
+    LargeKWork w = carve_large_k(work, total, nq, s);
+    const dim3 grid(large_k_chunks(total, nq), (uint32_t)nq);
+    hipLaunchKernelGGL(k_bm_keys, grid, dim3(256), 0, s, words, d / 64, n_docs, qbits, offsets, rows, ntop, w.keys);
+    check_launch();
+    segsort_keys(w.temp, w.temp_bytes, w.keys, w.sorted, total, (uint32_t)nq, offsets, s);
+    hipLaunchKernelGGL(k_gather_keys, dim3((out_k + 255) / 256, (uint32_t)nq), dim3(256), 0, s, w.sorted, offsets,
+                       (uint64_t)0, out_k, ntop, out_s, out_i);
+    check_launch();
 }
 
 void launch_fullscan_fs(const uint8_t* packed, uint64_t n, uint32_t d, const uint8_t* lut, const float* bias_invq,

@@ -122,6 +122,44 @@ def _full_search(fn, h, queries, d: int, n: int, k: int):
     return s, i
 
 
+class TwoStageCandidatePolicy:
+    """TwoStageCandidatePolicy (src/quant.rs:235-262): stage-1 candidate count
+    for a two-stage search, ``min(max(min_candidates, k * k_multiplier),
+    max_candidates, search_space)`` (0 when k or the search space is 0).  The
+    default is 256 / 32 / no maximum."""
+
+    def __init__(self, min_candidates: int = 256, k_multiplier: int = 32, max_candidates=None):
+        for name, v in (("min_candidates", min_candidates), ("k_multiplier", k_multiplier)):
+            if int(v) < 0:
+                raise ValueError(f"{name} must be non-negative")
+        if max_candidates is not None and int(max_candidates) < 0:
+            raise ValueError("max_candidates must be non-negative")
+        self.min_candidates = int(min_candidates)
+        self.k_multiplier = int(k_multiplier)
+        self.max_candidates = None if max_candidates is None else int(max_candidates)
+
+    def _c(self) -> N.TwoStagePolicy:
+        p = N.TwoStagePolicy()
+        p.min_candidates = self.min_candidates
+        p.k_multiplier = self.k_multiplier
+        p.max_candidates = self.max_candidates or 0
+        p.has_max_candidates = 0 if self.max_candidates is None else 1
+        return p
+
+    def candidate_count(self, k: int, search_space: int) -> int:
+        p = self._c()
+        return int(N.lib().ordvec_two_stage_policy_candidate_count(C.byref(p), int(k), int(search_space)))
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, TwoStageCandidatePolicy) and (
+            self.min_candidates, self.k_multiplier, self.max_candidates) == (
+            other.min_candidates, other.k_multiplier, other.max_candidates)
+
+    def __repr__(self) -> str:
+        return (f"TwoStageCandidatePolicy(min_candidates={self.min_candidates}, "
+                f"k_multiplier={self.k_multiplier}, max_candidates={self.max_candidates})")
+
+
 class RankQuant:
     """B-bit bucketed-rank index resident in HBM (src/quant.rs:210-1017)."""
 
@@ -232,6 +270,40 @@ class RankQuant:
                                                        _ptr(s) if out_k else None, _ptr(i) if out_k else None,
                                                        C.byref(ret), qflag))
         return s, i
+
+    def try_search_with_sign_probe(self, sign_probe, query, k: int, policy=None):
+        """try_search_with_sign_probe[_with_policy] (src/quant.rs:1453-1497):
+        the policy's candidate count of sign-bitmap candidates, then
+        search_asymmetric_subset over them; 1-D arrays of length min(k, m).
+        ValueError on a sign probe of another dim or length, a query of the
+        wrong length or a non-finite query."""
+        if not isinstance(sign_probe, SignBitmap):
+            raise TypeError("sign_probe must be a SignBitmap")
+        d, _, n = self._info()
+        if k < 0:
+            raise ValueError("k must be non-negative")
+        if _is_cuda_tensor(query):
+            q = query.to(torch.float32).contiguous().reshape(-1)
+            flag = N.FLAG_DEVICE_PTRS
+        else:
+            q = np.ascontiguousarray(np.asarray(query, dtype=np.float32).reshape(-1))
+            flag = 0
+        qlen = int(q.shape[0])
+        pol = (policy or TwoStageCandidatePolicy())._c()
+        m = int(N.lib().ordvec_two_stage_policy_candidate_count(C.byref(pol), int(k), n))
+        out_k = min(int(k), m)
+        s = _out((out_k,), np.float32, flag)
+        i = _out((out_k,), np.int64, flag)
+        ret = C.c_uint64()
+        _bind_stream() if flag else _host_stream()
+        N.check(N.lib().ordvec_rankquant_search_with_sign_probe(
+            self._h, sign_probe._h, _ptr(q), qlen, int(k), C.byref(pol), _ptr(s) if out_k else None,
+            _ptr(i) if out_k else None, C.byref(ret), flag))
+        return s, i
+
+    def search_with_sign_probe(self, sign_probe, query, k: int):
+        """search_with_sign_probe (src/quant.rs:1499-1507): the default policy."""
+        return self.try_search_with_sign_probe(sign_probe, query, k)
 
     def search_asymmetric_subset_batched(self, queries, offsets, candidates, k: int):
         """search_asymmetric_subset_batched_serial (src/quant.rs:1417-1451):
@@ -872,6 +944,25 @@ def two_stage_search(sign: SignBitmap, rq: RankQuant, queries, m: int, k: int,
         _bind_stream() if flag else _host_stream()
         f = flag | (N.FLAG_ASSUME_FINITE if assume_finite else 0)
         N.check(N.lib().ordvec_two_stage_search(sign._h, rq._h, _ptr(q), nq, int(m), int(k), _ptr(s), _ptr(i), f))
+    return s, i
+
+
+def two_stage_search_policy(sign: SignBitmap, rq: RankQuant, queries, k: int, policy=None,
+                            assume_finite: bool = False):
+    """two_stage_search with m = policy.candidate_count(k, n) (default policy
+    256 / 32 / none, src/quant.rs:235-262); rows nq x min(k, n), padded
+    -inf / -1 past min(k, m)."""
+    d, _, n = rq._info()
+    q, nq, flag = _float_rows(queries, d, "queries")
+    out_k = min(int(k), n)
+    s = _out((nq, out_k), np.float32, flag)
+    i = _out((nq, out_k), np.int64, flag)
+    if nq and out_k:
+        pol = (policy or TwoStageCandidatePolicy())._c()
+        _bind_stream() if flag else _host_stream()
+        f = flag | (N.FLAG_ASSUME_FINITE if assume_finite else 0)
+        N.check(N.lib().ordvec_two_stage_search_policy(sign._h, rq._h, _ptr(q), nq, int(k), C.byref(pol), _ptr(s),
+                                                       _ptr(i), f))
     return s, i
 
 

@@ -177,3 +177,27 @@ def test_rank_validation_and_forged_ovr_need_no_device(tmp_path):
         assert lib.ordvec_rank_load(str(p).encode(), C.byref(h)) == N.CORRUPT_INDEX, j
         assert msg in N.last_error(), (j, N.last_error())
     assert lib.ordvec_rank_load(str(tmp_path / "missing.ovr").encode(), C.byref(h)) == N.IO
+
+
+def test_two_stage_candidate_policy_kats():
+    """tests/index/two_stage.rs:76-89 (N = 256), through the C ABI."""
+    import ordvec_amd as ov
+
+    n = 256
+    default = ov.TwoStageCandidatePolicy()
+    assert default.candidate_count(0, n) == 0
+    assert default.candidate_count(1, n) == min(256, n)
+    assert default.candidate_count(10, n) == n
+    assert default.candidate_count(10, 10**6) == 320
+    assert default.candidate_count(5, 0) == 0
+    capped = ov.TwoStageCandidatePolicy(min_candidates=4, k_multiplier=2**64 - 1, max_candidates=37)
+    assert capped.candidate_count(2**64 - 1, n) == 37  # k * multiplier saturates
+    assert capped.candidate_count(10, 9) == 9
+    assert ov.TwoStageCandidatePolicy(0, 4, 37).candidate_count(5, n) == 20
+    assert ov.TwoStageCandidatePolicy(0, 0).candidate_count(3, 10) == 0
+    assert ov.TwoStageCandidatePolicy(7, 0, 0).candidate_count(3, 10) == 0  # Some(0) caps at zero
+    # NULL policy = the default
+    assert N.lib().ordvec_two_stage_policy_candidate_count(None, 10, 10**6) == 320
+    p = N.TwoStagePolicy()
+    N.lib().ordvec_two_stage_policy_default(C.byref(p))
+    assert (p.min_candidates, p.k_multiplier, p.has_max_candidates) == (256, 32, 0)
