@@ -14,7 +14,9 @@
  *
  * Threading: every function may be called from any thread.  Calls are
  * serialised on one HIP stream per process; index handles may be shared for
- * concurrent searches but must not be freed while in use.
+ * concurrent searches but must not be freed while a search runs.  Indexes a
+ * running server reads are pinned: add and swap_remove on them fail with
+ * BAD_ARGUMENT and free is deferred until the last such server stops.
  */
 #ifndef ORDVEC_H
 #define ORDVEC_H

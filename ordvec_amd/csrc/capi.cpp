@@ -232,7 +232,21 @@ void grow_rows(T*& buf, uint64_t& cap, uint64_t n, uint64_t row_bytes, uint64_t 
     cap = nc;
 }
 
-struct RqIndex {
+// Every index counts the running servers that read it (engine mutex held):
+// while the count is nonzero, add and swap_remove fail and free is deferred
+// to the last server's stop, so no in-flight batch sees its rows move.
+struct Served {
+    uint32_t serving = 0;
+    bool free_pending = false;
+};
+
+void check_not_serving(const Served& ix, const char* what) {
+    if (ix.serving)
+        fail(kBadArgument, std::string(what) + " is not allowed while a running server reads this index "
+                                               "(stop the server first)");
+}
+
+struct RqIndex : Served {
     uint32_t dim = 0, bits = 0, bpv = 0;
     uint64_t n = 0, cap = 0;
     uint8_t* codes = nullptr;
@@ -256,7 +270,7 @@ struct RkIndex {
 
 // Sign bitmap (ntop == 0) or top-bucket Bitmap (ntop > 0): same interleaved
 // storage, different per-word statistic.
-struct SbIndex {
+struct SbIndex : Served {
     uint32_t dim = 0, qpv = 0, ntop = 0;
     uint64_t n = 0, cap_blocks = 0;
     uint64_t* words = nullptr;  // interleaved [block][word][64]
@@ -2651,6 +2665,7 @@ ordvec_status_t ordvec_rankquant_add(ordvec_rankquant_t* index, const float* vec
         if (n == 0) return;
         NONNULL(vectors, "vectors");
         LOCK;
+        check_not_serving(*RQ(index), "add");
         rq_add(RQ(index), vectors, n, flags);
     });
 }
@@ -2876,7 +2891,8 @@ ordvec_status_t ordvec_rankquant_load(const char* path, ordvec_rankquant_t** out
 void ordvec_rankquant_free(ordvec_rankquant_t* index) {
     if (!index) return;
     LOCK;
-    delete RQ(index);
+    if (RQ(index)->serving) RQ(index)->free_pending = true;
+    else delete RQ(index);
 }
 
 ordvec_status_t ordvec_rankquant_search(const ordvec_rankquant_t* index, const float* queries, uint64_t nq, uint64_t k,
@@ -2898,6 +2914,7 @@ ordvec_status_t ordvec_rankquant_swap_remove(ordvec_rankquant_t* index, uint64_t
         NONNULL(index, "index");
         LOCK;
         RqIndex* ix = RQ(index);
+        check_not_serving(*ix, "swap_remove");
         uint64_t last = swap_remove_rows(ix->codes, ix->n, ix->bpv, idx);
         if (moved_from) *moved_from = last;
     });
@@ -3035,6 +3052,7 @@ ordvec_status_t ordvec_sign_bitmap_add(ordvec_sign_bitmap_t* index, const float*
         if (n == 0) return;
         NONNULL(vectors, "vectors");
         LOCK;
+        check_not_serving(*SB(index), "add");
         sb_add(SB(index), vectors, n, flags);
     });
 }
@@ -3114,6 +3132,7 @@ ordvec_status_t ordvec_sign_bitmap_swap_remove(ordvec_sign_bitmap_t* index, uint
     return boundary([&] {
         NONNULL(index, "index");
         LOCK;
+        check_not_serving(*SB(index), "swap_remove");
         uint64_t last = sb_swap_remove(SB(index), idx);
         if (moved_from) *moved_from = last;
     });
@@ -3122,7 +3141,8 @@ ordvec_status_t ordvec_sign_bitmap_swap_remove(ordvec_sign_bitmap_t* index, uint
 void ordvec_sign_bitmap_free(ordvec_sign_bitmap_t* index) {
     if (!index) return;
     LOCK;
-    delete SB(index);
+    if (SB(index)->serving) SB(index)->free_pending = true;
+    else delete SB(index);
 }
 
 // ---- Bitmap (top-bucket, AND-popcount) ------------------------------------------
@@ -3847,6 +3867,8 @@ ordvec_status_t ordvec_server_start(const ordvec_sign_bitmap_t* sign, const ordv
             sv->slots.push_back(std::move(sl));
         }
         Server* raw = sv.get();
+        const_cast<SbIndex*>(sb)->serving++;
+        const_cast<RqIndex*>(ix)->serving++;
         for (auto& sl : raw->slots) sl->th = std::thread(&Server::worker, raw, sl.get());
         *out = reinterpret_cast<ordvec_server_t*>(sv.release());
     });
@@ -4027,6 +4049,13 @@ void ordvec_server_stop(ordvec_server_t* server) {
     if (!server) return;
     Server* sv = SV(server);
     sv->stop();
+    {
+        LOCK;
+        SbIndex* sb = const_cast<SbIndex*>(sv->sb);
+        RqIndex* rq = const_cast<RqIndex*>(sv->rq);
+        if (--sb->serving == 0 && sb->free_pending) delete sb;
+        if (--rq->serving == 0 && rq->free_pending) delete rq;
+    }
     delete sv;
 }
 

@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 
 namespace ovx {
@@ -473,8 +475,8 @@ void launch_fs4_frag(const uint8_t* lut, const Fs4Group* qsets, uint32_t nqsets,
 }
 
 void fs4_configure_lds() {
-    static bool done = false;
-    if (done) return;
+    static std::once_flag done;
+    std::call_once(done, [] {
     // Larger dynamic LDS than the default 64 KiB (gfx950 has 160 KiB per CU).
 #define OVX_FS4_LDS(QG, CH)                                                                       \
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fs4_scan<QG, CH>),                \
@@ -491,7 +493,7 @@ void fs4_configure_lds() {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fs4_scan_mfma),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipGetLastError();
-    done = true;
+    });
 }
 
 void launch_fs4_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint32_t* packed, hipStream_t s) {

@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 
@@ -2886,8 +2888,9 @@ static void set_sample_lds() {
 // Query fragments live in LDS (16 * d bytes per 32-query tile, <= 64 KiB).
 constexpr uint64_t kMfmaMinQueries = 2;
 static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
-    static bool configured = false;
-    if (!configured) {
+    // once per process, also when serving slots reach here concurrently
+    static std::once_flag configured;
+    std::call_once(configured, [] {
         set_mfma_lds<4, 0>();
         set_mfma_lds<4, 1>();
         set_sample_lds<0>();
@@ -2895,8 +2898,7 @@ static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
         set_mfma_lds<2, 0>();
         set_mfma_lds<2, 1>();
         (void)hipGetLastError();
-        configured = true;
-    }
+    });
     return g_mfma_scan && nq >= kMfmaMinQueries && (qpv == 16 || qpv == 12 || qpv == 8 || qpv == 4 || qpv == 2 || qpv == 1);
 }
 

@@ -110,3 +110,55 @@ def test_open_loop_completes_every_request(engine, corpus):
     assert rep["queries"] >= rep["requests"]
     assert 0 < rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
     assert rep["batches"] >= 1 and rep["mean_batch_queries"] >= 1
+
+
+def test_indexes_are_pinned_while_a_server_runs(engine, corpus):
+    """add / swap_remove of a served index fail while the server runs, and
+    freeing it is deferred to the server's stop (in-flight batches never see
+    its rows move or its memory go)."""
+    import ctypes as C
+
+    from ordvec_amd import _native as N
+
+    docs, qs, sb, rq = corpus
+    with engine.Server(sb, rq, m=1000, max_k=10, slots=1) as srv:
+        with pytest.raises(ValueError, match="running server"):
+            sb.add(docs[:3])
+        with pytest.raises(ValueError, match="running server"):
+            rq.add(docs[:3])
+        with pytest.raises(ValueError, match="running server"):
+            rq.swap_remove(5)
+        with pytest.raises(ValueError, match="running server"):
+            sb.swap_remove(5)
+        s, i = srv.search(qs[:4], 10)
+    assert len(rq) == len(docs) and len(sb) == len(docs)
+    ws, wi = engine.two_stage_search(sb, rq, qs[:4], 1000, 10)
+    assert np.array_equal(i, wi)
+    ref_rq = engine.RankQuant(1024, 2)
+    ref_rq.add(docs[:3000])
+    ref_sb = engine.SignBitmap(1024)
+    ref_sb.add(docs[:3000])
+    # a raw handle freed while served: the server keeps answering, stop frees it
+    lib = N.lib()
+    h_rq, h_sb = C.c_void_p(), C.c_void_p()
+    N.check(lib.ordvec_rankquant_new(1024, 2, C.byref(h_rq)))
+    N.check(lib.ordvec_sign_bitmap_new(1024, C.byref(h_sb)))
+    x = np.ascontiguousarray(docs[:3000])
+    N.check(lib.ordvec_rankquant_add(h_rq, x.ctypes.data_as(C.c_void_p), 3000, 0))
+    N.check(lib.ordvec_sign_bitmap_add(h_sb, x.ctypes.data_as(C.c_void_p), 3000, 0))
+    h_srv = C.c_void_p()
+    N.check(lib.ordvec_server_start(h_sb, h_rq, 500, 10, 1, 64, 1 << 30, C.byref(h_srv)))
+    lib.ordvec_rankquant_free(h_rq)
+    lib.ordvec_sign_bitmap_free(h_sb)
+    q = np.ascontiguousarray(qs[:2])
+    s = np.empty((2, 10), np.float32)
+    ids = np.empty((2, 10), np.int64)
+    t = C.c_uint64()
+    N.check(lib.ordvec_server_submit(h_srv, q.ctypes.data_as(C.c_void_p), 2, 10, s.ctypes.data_as(C.c_void_p),
+                                     ids.ctypes.data_as(C.c_void_p), C.byref(t)))
+    lat, bq = C.c_uint64(), C.c_uint64()
+    N.check(lib.ordvec_server_wait(h_srv, t.value, C.byref(lat), C.byref(bq)))
+    lib.ordvec_server_stop(h_srv)
+    ws, wi = engine.two_stage_search(ref_sb, ref_rq, qs[:2], 500, 10)
+    assert np.array_equal(ids, wi)
+    assert np.array_equal(s.view(np.uint32), ws.view(np.uint32))
