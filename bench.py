@@ -194,43 +194,7 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
-        import oracle as O
-
-        O.build()
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-        codes = rq.codes()
-        words = sb.words()
-        qhost = batches[args.warmup].cpu().numpy()
-        gpu_s, gpu_i = outs[0][0].cpu().numpy(), outs[0][1].cpu().numpy()
-
-        def run(nq):
-            q = qhost[:nq]
-            a = time.perf_counter()
-            cand = O.sign_top_m(words, O.sign_encode(q), m, threads)
-            offs = np.arange(nq + 1, dtype=np.uint64) * cand.shape[1]
-            s_, i_ = O.subset_batched(codes, d, 2, q, offs, cand.reshape(-1), k, threads)
-            return time.perf_counter() - a, s_, i_
-
-        nq = threads
-        dt, _, _ = run(nq)
-        while dt < args.cpu_seconds / 4 and nq < B:
-            nq = min(B, nq * 4)
-            dt, cs, ci = run(nq)
-        if nq == threads:
-            dt, cs, ci = run(nq)
-        # the whole batch fits in the sample: repeat it until the sample is long enough
-        reps, total = 1, dt
-        while total < args.cpu_seconds and nq == B:
-            more, _, _ = run(nq)
-            total += more
-            reps += 1
-        cpu = {"value": round(reps * nq / total, 2), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"{nq} queries of the timed batch x {reps} passes, same 1M x 1024 index, two-stage "
-                         f"m={m} k={k}, {total:.2f} s wall on {threads} host threads"}
-        parity = {"queries": nq, "ids_identical": bool(np.array_equal(ci, gpu_i[:nq])),
-                  "scores_bit_identical": bool(np.array_equal(cs.view(np.uint32), gpu_s[:nq].view(np.uint32)))}
-        log(f"cpu baseline: {cpu['value']:,.1f} QPS on {threads} threads; parity {parity}")
+        cpu, parity = cpu_baseline(args, rq, sb, batches[args.warmup], outs[0], d, m, k, B)
 
     out = {
         "metric": "QPS + HBM GB/s (% of 8 TB/s), 1M x 1024-d two-stage k=10, batched vs plain; p99 ms",
@@ -268,6 +232,102 @@ def main():
         "parity_full_size": parity,
     }
     print(json.dumps(out), flush=True)
+
+
+def host_cpu_info():
+    """CPU model (from /proc/cpuinfo), logical CPU count and this process's
+    affinity set size."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count() or 1, "affinity_cpus": aff}
+
+
+def cpu_baseline(args, rq, sb, qbatch, gpu_out, d, m, k, B):
+    """The oracle (C restatement of the reference CPU path, with its AVX-512
+    VPOPCNTDQ stage-1 scan and AVX-512 rerank tier where the host has them)
+    on the same index and queries: one thread pinned to one core (as the
+    reference's two_stage_caller_owned_dim1024 benchmark), then every core of
+    this process's CPU share, one query range per thread.  Phases timed as
+    two_stage_bench does (stage 1, rerank, full); bounded samples of about
+    --cpu-seconds in total.  Also compares the sample's top-k with the GPU's."""
+    import numpy as np
+
+    import oracle as O
+
+    O.build()
+    simd = O.simd(True)
+    host = host_cpu_info()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 16
+    threads = args.cpu_threads or max(1, min(host["affinity_cpus"], share))
+    codes, words = rq.codes(), sb.words()
+    qhost = qbatch.cpu().numpy()
+    # query sign words up front: this also starts the OpenMP pool on the
+    # whole CPU share before one thread is pinned (workers inherit the
+    # affinity of the thread that starts them)
+    qbits = O.sign_encode(qhost)
+    gpu_s, gpu_i = gpu_out[0].cpu().numpy(), gpu_out[1].cpu().numpy()
+
+    def run(nq, th):
+        q = qhost[:nq]
+        a = time.perf_counter()
+        cand = O.sign_top_m(words, qbits[:nq], m, th)
+        b = time.perf_counter()
+        offs = np.arange(nq + 1, dtype=np.uint64) * cand.shape[1]
+        s_, i_ = O.subset_batched(codes, d, 2, q, offs, cand.reshape(-1), k, th)
+        c = time.perf_counter()
+        return b - a, c - b, s_, i_
+
+    def leg(th, nq, budget):
+        t1 = t2 = 0.0
+        reps = 0
+        s_ = i_ = None
+        while reps == 0 or (t1 + t2 < budget and reps < 50):
+            a, b, s_, i_ = run(nq, th)
+            t1, t2, reps = t1 + a, t2 + b, reps + 1
+        per = reps * nq
+        return {"qps": round(per / (t1 + t2), 2), "stage1_us_per_query": round(t1 / per * 1e6, 1),
+                "rerank_us_per_query": round(t2 / per * 1e6, 1), "queries": nq, "passes": reps,
+                "seconds": round(t1 + t2, 2)}, s_, i_
+
+    # one thread, pinned to one core of the affinity set
+    saved = None
+    try:
+        saved = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(saved)})
+    except (AttributeError, OSError):
+        saved = None
+    try:
+        single, _, _ = leg(1, 64, args.cpu_seconds * 0.4)
+    finally:
+        if saved is not None:
+            os.sched_setaffinity(0, saved)
+    # every core of the share: 32-query tiles per thread
+    nq_all = min(B, 32 * threads * 2)
+    allc, cs, ci = leg(threads, nq_all, args.cpu_seconds * 0.6)
+    cpu = {"value": allc["qps"], "unit": "queries/s", "cores": threads, "kind": "port",
+           "sample": f"{nq_all} queries of the timed batch x {allc['passes']} passes on {threads} threads "
+                     f"({allc['seconds']} s), same 1M x 1024 index, two-stage m={m} k={k}; 1 pinned thread: "
+                     f"{single['queries']} queries x {single['passes']} passes ({single['seconds']} s)",
+           "simd": "avx512f+dq+vpopcntdq" if simd else "scalar (x86-64-v3)",
+           "host": host, "all_cores": allc, "single_thread": single}
+    parity = {"queries": nq_all, "ids_identical": bool(np.array_equal(ci, gpu_i[:nq_all])),
+              "scores_bit_identical": bool(np.array_equal(cs.view(np.uint32), gpu_s[:nq_all].view(np.uint32)))}
+    log(f"cpu baseline ({host['model']}, {host['nproc']} cpus, simd {simd}): {allc['qps']:,.1f} QPS on {threads} "
+        f"threads (stage 1 {allc['stage1_us_per_query']} us/query); 1 thread {single['qps']:,.1f} QPS "
+        f"(stage 1 {single['stage1_us_per_query']} us, rerank {single['rerank_us_per_query']} us per query); "
+        f"parity {parity}")
+    return cpu, parity
 
 
 def serve_main(args, dev):

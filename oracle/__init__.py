@@ -52,6 +52,8 @@ U32 = C.c_uint32
 
 def _declare(L):
     sig = {
+        "orc_simd_supported": (C.c_int, []),
+        "orc_set_simd": (C.c_int, [C.c_int]),
         "orc_rank_transform": (None, [P, U32, P]),
         "orc_rank_transform_qsort": (None, [P, U32, P]),
         "orc_rank_to_bucket": (C.c_uint8, [U32, U32, U32]),
@@ -101,6 +103,17 @@ def _f32(x):
 
 
 # --- src/rank.rs -------------------------------------------------------------
+
+def simd(enable=None) -> bool:
+    """AVX-512 forms of the stage-1 scan and the AVX-512 scoring tier
+    (identical results): None queries, False forces the scalar restatements,
+    True uses AVX-512 where the CPU has it.  Returns whether they are in use."""
+    return bool(lib().orc_set_simd(-1 if enable is None else (1 if enable else 0)))
+
+
+def simd_supported() -> bool:
+    return bool(lib().orc_simd_supported())
+
 
 def rank_transform(v) -> np.ndarray:
     """rank_transform (src/rank.rs:38-55)."""

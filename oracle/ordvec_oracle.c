@@ -24,6 +24,13 @@
 #include <omp.h>
 #endif
 
+/* AVX-512 forms of the scan and the AVX-512 tier (ordvec_cpu_simd.c), used
+ * when the CPU has them; identical results. */
+int orc_simd_active(void);
+void orc_simd_scan_block(const uint64_t *block, uint64_t bn, uint32_t qpv, const uint64_t *qb, uint64_t tq,
+                         uint32_t *scores);
+float orc_simd_score_avx512(const uint8_t *row, uint32_t d, uint32_t bits, const float *q, float scale);
+
 #define ORC_TIER_SCALAR 0
 #define ORC_TIER_AVX2 1
 #define ORC_TIER_AVX512 2
@@ -325,7 +332,8 @@ static orc_qconst query_consts(const float *q_unit, uint32_t d, uint32_t bits, f
 static inline float doc_score(const uint8_t *row, uint32_t d, uint32_t bits, const float *q_unit,
                               const float *lut, const orc_qconst *c) {
     float s;
-    if (c->tier == ORC_TIER_AVX512) s = score_avx512(row, d, bits, q_unit, c->inv_norm);
+    if (c->tier == ORC_TIER_AVX512 && orc_simd_active()) s = orc_simd_score_avx512(row, d, bits, q_unit, c->inv_norm);
+    else if (c->tier == ORC_TIER_AVX512) s = score_avx512(row, d, bits, q_unit, c->inv_norm);
     else if (c->tier == ORC_TIER_AVX2) s = score_avx2(row, d, bits, q_unit, c->inv_norm);
     else s = score_scalar(row, d, bits, lut, c->inv_norm);
     return s + c->offset;
@@ -646,6 +654,10 @@ static int cmp_u64(const void *a, const void *b) {
  * (src/sign_bitmap.rs:763-955): scores[ti*bn + d]. */
 static void scan_block(const uint64_t *block, uint64_t bn, uint32_t qpv, const uint64_t *qb, uint64_t tq,
                        uint32_t *scores) {
+    if (orc_simd_active()) {
+        orc_simd_scan_block(block, bn, qpv, qb, tq, scores);
+        return;
+    }
     for (uint64_t t0 = 0; t0 < tq; t0 += 8) {
         uint64_t tc = tq - t0 < 8 ? tq - t0 : 8;
         for (uint64_t d = 0; d < bn; d++) {

@@ -543,3 +543,34 @@ def test_rq_search_symmetric_closed_form(oracle, bits):
             assert s[qi, j] > s[qi, j + 1] or (s[qi, j] == s[qi, j + 1] and i[qi, j] < i[qi, j + 1])
     s, i = oracle.rq_search_symmetric(codes, d, bits, docs[5:6], 1)
     assert i[0, 0] == 5 and abs(float(s[0, 0]) - 1.0) < 1e-6
+
+
+# ---- AVX-512 forms (the CPU baseline's kernels) -------------------------------
+
+@pytest.mark.skipif("not __import__('oracle').simd_supported()", reason="CPU lacks AVX-512 F/DQ/VPOPCNTDQ")
+@pytest.mark.parametrize("dim", [64, 128, 576, 1024])
+def test_simd_forms_equal_scalar_restatements(oracle, dim):
+    """The AVX-512 VPOPCNTDQ scan (src/sign_bitmap.rs:845-955) and the AVX-512
+    scoring tier (src/quant_kernels.rs:406-581) give the scalar restatements'
+    results bit for bit, including masked tail words (qpv % 8 != 0)."""
+    gen = S.MixtureNP(dim, seed=71)
+    docs = gen.docs(3000)
+    qs, _ = gen.queries(docs, 13)
+    bm, qb = oracle.sign_encode(docs), oracle.sign_encode(qs)
+    try:
+        assert oracle.simd(True)
+        fast = oracle.sign_top_m(bm, qb, 300, 2)
+        assert oracle.simd(False) is False
+        slow = oracle.sign_top_m(bm, qb, 300, 2)
+        assert np.array_equal(fast, slow)
+        assert np.array_equal(fast, oracle.sign_top_m_bruteforce(bm, qb, 300))
+        for bits in (2, 4):
+            codes = oracle.rq_encode(docs, bits)
+            oracle.simd(False)
+            ss, si = oracle.search_asymmetric(codes, dim, bits, qs, 50)
+            oracle.simd(True)
+            fs, fi = oracle.search_asymmetric(codes, dim, bits, qs, 50)
+            assert np.array_equal(si, fi)
+            assert np.array_equal(ss.view(np.uint32), fs.view(np.uint32))
+    finally:
+        oracle.simd(True)
