@@ -72,6 +72,8 @@ def main():
                     help="batched multi-index FastScan b=4 run instead: 64 indexes of 768-d, Poisson(16) queries")
     ap.add_argument("--grad", action="store_true",
                     help="differentiable rerank run instead: 1M x 1024 index, 4096 queries through an adapter")
+    ap.add_argument("--encode", action="store_true",
+                    help="ingest run instead: 1M x 1024 rows encoded into a SignBitmap + RankQuant b=2 in chunks")
     ap.add_argument("--ref-shapes", action="store_true",
                     help="every index form at the reference's rank-modes shape (30k x 256, 200 queries, k=10)")
     args = ap.parse_args()
@@ -94,6 +96,8 @@ def main():
     torch.cuda.set_device(dev)
     if args.serve:
         return serve_main(args, dev)
+    if args.encode:
+        return encode_main(args, dev)
     if args.grad:
         return grad_main(args, dev)
     if args.multi:
@@ -420,6 +424,64 @@ REF_RANK_MODES = {
     "TwoStage b=2 M=1000": (0.082, 0.091, 2256172), "TwoStage b=2 M=5000": (0.186, 0.198, 2339041),
     "SignTwoStage b=2 M=500": (0.064, 0.074, 3605670),
 }
+
+
+def encode_main(args, dev):
+    """Ingest: the 1M x 1024 mixture (seed 2) in chunks of 131,072 f32 rows
+    resident in HBM, encoded (a) in one pass into both indexes
+    (two_stage_add: sign words, b=2 codes, finite check) and (b) by the two
+    adds.  Per launch: algorithmic bytes = rows x (4d read + d/8 + d/4
+    written); achieved GB/s from the encode family's HIP events."""
+    import numpy as np
+    import torch
+
+    import ordvec_amd as ov
+    from ordvec_amd import _native as N
+    from ordvec_amd.synthetic import MixtureTorch
+
+    d, n, chunk = args.dim, args.n_docs, 131072
+    gen = MixtureTorch(d, seed=2, device=dev)
+    chunks = [gen.docs(min(chunk, n - s0)) for s0 in range(0, n, chunk)]
+    torch.cuda.synchronize()
+    res = {}
+    for mode in ("one_pass", "separate"):
+        for rep in range(2):  # the first pass warms up (allocations, code objects)
+            sb, rq = ov.SignBitmap(d), ov.RankQuant(d, 2)
+            N.kernel_timing_enable(True, families=["encode"])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for c in chunks:
+                if mode == "one_pass":
+                    ov.two_stage_add(sb, rq, c)
+                else:
+                    sb.add(c)
+                    rq.add(c)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            launches, ms = N.kernel_timing_read("encode")
+            N.kernel_timing_enable(False)
+        rows = sum(c.shape[0] for c in chunks)
+        kern_s = ms / 1e3
+        alg = rows * (4 * d + d // 8 + d // 4)
+        read_passes = 1 if mode == "one_pass" else 2
+        res[mode] = {"rows": rows, "wall_s": round(wall, 4), "kernel_s": round(kern_s, 5), "launches": launches,
+                     "rows_per_s": round(rows / wall, 1), "kernel_rows_per_s": round(rows / kern_s, 1),
+                     "algorithmic_bytes": alg, "f32_row_reads": read_passes,
+                     "kernel_gbs": round(alg / kern_s / 1e9, 1), "frac_of_hbm": round(alg / kern_s / 1e12 / 8.0, 4)}
+        log(f"{mode}: {rows:,} rows, kernels {kern_s * 1e3:.2f} ms ({launches} launches) = "
+            f"{res[mode]['kernel_gbs']:.0f} GB/s algorithmic ({res[mode]['frac_of_hbm']:.1%} of 8 TB/s), "
+            f"wall {wall * 1e3:.1f} ms")
+    # the one-pass indexes equal the two adds
+    sb1, rq1, sb2, rq2 = ov.SignBitmap(d), ov.RankQuant(d, 2), ov.SignBitmap(d), ov.RankQuant(d, 2)
+    ov.two_stage_add(sb1, rq1, chunks[0][:20000])
+    sb2.add(chunks[0][:20000])
+    rq2.add(chunks[0][:20000])
+    same = bool(np.array_equal(rq1.codes(), rq2.codes()) and np.array_equal(sb1.words(), sb2.words()))
+    out = {"metric": "encode rows/s and HBM GB/s (1M x 1024 f32 rows -> sign bitmap + RankQuant b=2)",
+           "value": res["one_pass"]["kernel_rows_per_s"], "unit": "rows/s", "n_gpus": 1, "higher_is_better": True,
+           "data": "synthetic (Gaussian mixture, seed 2)", "config": {"n_docs": n, "dim": d, "chunk_rows": chunk},
+           "one_pass": res["one_pass"], "separate": res["separate"], "one_pass_equals_separate": same}
+    print(json.dumps(out), flush=True)
 
 
 def ref_shapes_main(args, dev):

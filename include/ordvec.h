@@ -337,6 +337,15 @@ void ordvec_bitmap_free(ordvec_bitmap_t *index);
 /* src/bitmap.rs:515-527 swap_remove (as ordvec_rankquant_swap_remove). */
 ordvec_status_t ordvec_bitmap_swap_remove(ordvec_bitmap_t *index, uint64_t idx, uint64_t *moved_from);
 
+/* SignBitmap::add then RankQuant::add (src/sign_bitmap.rs:180-200,
+ * src/quant.rs:587-614) of the same vectors in one pass over them: each row is
+ * read once for its sign bits, its rank codes and the finite check (no
+ * reference counterpart as a single call; the results equal the two adds).
+ * Both indexes must have the same dim; a non-finite value leaves both
+ * unchanged (BAD_ARGUMENT). */
+ordvec_status_t ordvec_two_stage_add(ordvec_sign_bitmap_t *sign, ordvec_rankquant_t *rq, const float *vectors,
+                                     uint64_t n, uint64_t flags);
+
 /* Two-stage search in one call: SignBitmap::top_m_candidates_batched_serial_csr
  * followed by RankQuant::search_asymmetric_subset_batched_serial_into
  * (src/sign_bitmap.rs:453-483, src/quant.rs:1336-1394).  Both indexes must

@@ -21,6 +21,7 @@ from .index import (  # noqa: E402,F401
     TwoStageCandidatePolicy,
     stage1_exact_queries,
     synchronize,
+    two_stage_add,
     two_stage_search,
     two_stage_search_policy,
 )
@@ -38,6 +39,7 @@ __all__ = [
     "FastscanBatch",
     "SignBitmap",
     "RankQuantFastscan",
+    "two_stage_add",
     "two_stage_search",
     "two_stage_search_policy",
     "TwoStageCandidatePolicy",

@@ -164,6 +164,7 @@ SIGNATURES = {
     "ordvec_sign_bitmap_load": (ST, [C.c_char_p, C.POINTER(P)]),
     "ordvec_sign_bitmap_free": (None, [P]),
     "ordvec_two_stage_search": (ST, [P, P, P, U64, U64, U64, P, P, U64]),
+    "ordvec_two_stage_add": (ST, [P, P, P, U64, U64]),
     "ordvec_two_stage_search_noalloc": (ST, [P, P, P, U64, U64, U64, P, P, U64]),
     "ordvec_two_stage_policy_default": (None, [P]),
     "ordvec_two_stage_policy_candidate_count": (U64, [P, U64, U64]),

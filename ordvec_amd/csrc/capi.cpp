@@ -377,9 +377,50 @@ void for_each_chunk(const float* vectors, uint64_t n, uint32_t dim, uint64_t fla
     }
 }
 
+// One pass over the vectors into the rows [n, n + count) of a RankQuant
+// and/or a sign bitmap (k_encode_rows: codes, sign words and the non-finite
+// check together).  The indexes' row counts move only once every value was
+// found finite, so a failed add leaves them unchanged.
+void add_one_pass(RqIndex* rq, SbIndex* sb, const float* vectors, uint64_t n, uint64_t flags) {
+    const uint32_t dim = rq ? rq->dim : sb->dim;
+    if (!(flags & (ORDVEC_FLAG_DEVICE_PTRS | ORDVEC_FLAG_ASSUME_FINITE)) && !host_all_finite(vectors, n * dim))
+        fail(kBadArgument, "vectors contains NaN or infinity");
+    if (rq) check_new_count(rq->n, n);
+    if (sb) check_new_count(sb->n, n);
+    if (rq) rq->reserve(rq->n + n);
+    if (sb) sb->reserve(sb->n + n);
+    Engine& E = eng();
+    hipStream_t s = cur().stream;
+    const uint64_t max_rows = (flags & ORDVEC_FLAG_DEVICE_PTRS) ? n : std::max<uint64_t>(1, kChunkBytes / ((uint64_t)dim * 4));
+    void* work = cur().workspace2(encode_rows_work_bytes(std::min(n, max_rows)));
+    OVX_HIP(hipMemsetAsync(E.d_flag, 0, 4, s));
+    for_each_chunk(vectors, n, dim, flags, [&](const float* dv, uint64_t r0, uint64_t cnt) {
+        launch_encode_rows(dv, cnt, dim, rq ? rq->bits : 1, rq ? rq->codes + (rq->n + r0) * rq->bpv : nullptr,
+                           sb ? sb->words : nullptr, sb ? sb->n + r0 : 0, E.d_flag, work, s);
+    });
+    uint32_t flag = 0;
+    OVX_HIP(hipMemcpyAsync(&flag, E.d_flag, 4, hipMemcpyDeviceToHost, s));
+    OVX_HIP(hipStreamSynchronize(s));
+    if (flag && !(flags & ORDVEC_FLAG_ASSUME_FINITE)) fail(kBadArgument, "vectors contains NaN or infinity");
+    if (rq) rq->n += n;
+    if (sb) sb->n += n;
+}
+
+// The one-pass kernel reads rows with 16-byte loads: device rows must start
+// 16-byte aligned (host rows are staged into the aligned workspace).
+bool one_pass_ok(uint32_t dim, uint32_t bits, bool codes, bool words, const float* vectors, uint64_t flags) {
+    return encode_rows_supported(dim, bits, codes, words) &&
+           (!(flags & ORDVEC_FLAG_DEVICE_PTRS) || reinterpret_cast<uintptr_t>(vectors) % 16 == 0);
+}
+
 void rq_add(RqIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
     if (ix->dim > kEncodeMaxDim)
         fail(kUnsupportedOperation, "GPU rank encode supports dim <= " + std::to_string(kEncodeMaxDim));
+    if (one_pass_ok(ix->dim, ix->bits, true, false, vectors, flags)) {
+        eng().init();
+        add_one_pass(ix, nullptr, vectors, n, flags);
+        return;
+    }
     check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
     check_new_count(ix->n, n);
     ix->reserve(ix->n + n);
@@ -393,6 +434,11 @@ void rq_add(RqIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
 
 void sb_add(SbIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
     if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= " + std::to_string(kSignMaxDim));
+    if (one_pass_ok(ix->dim, 1, false, true, vectors, flags)) {
+        eng().init();
+        add_one_pass(nullptr, ix, vectors, n, flags);
+        return;
+    }
     check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
     check_new_count(ix->n, n);
     ix->reserve(ix->n + n);
@@ -3054,6 +3100,35 @@ ordvec_status_t ordvec_sign_bitmap_add(ordvec_sign_bitmap_t* index, const float*
         LOCK;
         check_not_serving(*SB(index), "add");
         sb_add(SB(index), vectors, n, flags);
+    });
+}
+
+ordvec_status_t ordvec_two_stage_add(ordvec_sign_bitmap_t* sign, ordvec_rankquant_t* rq, const float* vectors,
+                                     uint64_t n, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(sign, "sign");
+        NONNULL(rq, "rq");
+        SbIndex* sb = SB(sign);
+        RqIndex* ix = RQ(rq);
+        if (sb->dim != ix->dim) fail(kBadArgument, "sign_probe.dim must match RankQuant dim " + std::to_string(ix->dim));
+        if (n == 0) return;
+        NONNULL(vectors, "vectors");
+        LOCK;
+        check_not_serving(*sb, "add");
+        check_not_serving(*ix, "add");
+        if (one_pass_ok(ix->dim, ix->bits, true, true, vectors, flags)) {
+            eng().init();
+            add_one_pass(ix, sb, vectors, n, flags);
+            return;
+        }
+        // no one-pass form for this (dim, bits): the two adds, vectors checked once
+        if (ix->dim > kEncodeMaxDim)
+            fail(kUnsupportedOperation, "GPU rank encode supports dim <= " + std::to_string(kEncodeMaxDim));
+        check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
+        check_new_count(ix->n, n);
+        check_new_count(sb->n, n);
+        sb_add(sb, vectors, n, flags | ORDVEC_FLAG_ASSUME_FINITE);
+        rq_add(ix, vectors, n, flags | ORDVEC_FLAG_ASSUME_FINITE);
     });
 }
 

@@ -100,6 +100,18 @@ void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bi
 void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, uint64_t first_doc, bool interleaved,
                         hipStream_t s);
 void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, hipStream_t s);
+// One pass over n f32 rows: RankQuant codes (codes != null, row-major; rows
+// whose values do not bin finely enough are finished by the whole-row sort
+// in a second launch), sign words (words != null, interleaved layout from
+// document first_doc), and flag |= 1 when any value is non-finite.  Returns false (nothing launched)
+// for a (d, bits) without a one-pass form; the callers then use the kernels
+// above.
+bool launch_encode_rows(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, uint64_t* words,
+                        uint64_t first_doc, uint32_t* flag, void* work, hipStream_t s);
+// Device scratch of launch_encode_rows for n rows (rows handed to the
+// whole-row sort).
+size_t encode_rows_work_bytes(uint64_t n);
+bool encode_rows_supported(uint32_t d, uint32_t bits, bool codes, bool words);
 void launch_fastscan_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint8_t* packed, hipStream_t s);
 // Rank vectors (u16 [n][d]) of rows; centre(bucket(rank)) of rank vectors.
 void launch_rank_u16(const float* x, uint64_t n, uint32_t d, uint16_t* ranks, hipStream_t s);

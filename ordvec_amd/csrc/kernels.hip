@@ -157,16 +157,22 @@ __global__ __launch_bounds__(256) void k_sign_encode(const float* __restrict__ x
 // -0.0 is canonicalised to +0.0 first: the reference compares with `==`
 // before total_cmp, so the two zeros tie and fall back to index order.
 // RANKS: write the u16 rank vector (Rank::add, src/rank.rs:329-350) instead.
+// list != null: rows list[0 .. *list_count) only (rows the one-pass encode
+// handed over).
 template <bool RANKS>
 __global__ __launch_bounds__(256) void k_rank_encode(const float* __restrict__ x, uint64_t n, uint32_t d,
                                                      uint32_t bits, uint8_t* __restrict__ codes,
-                                                     uint16_t* __restrict__ ranks) {
+                                                     uint16_t* __restrict__ ranks,
+                                                     const uint32_t* __restrict__ list = nullptr,
+                                                     const uint32_t* __restrict__ list_count = nullptr) {
     extern __shared__ uint64_t smem[];
     const uint32_t P = next_pow2(d);
     uint64_t* keys = smem;
     uint8_t* buck = reinterpret_cast<uint8_t*>(smem + P);
     const uint32_t cpb = RANKS ? 1 : 8 / bits, bpv = d / cpb, nb = 1u << bits;
-    for (uint64_t r = blockIdx.x; r < n; r += gridDim.x) {
+    const uint64_t total = list ? min((uint64_t)*list_count, n) : n;
+    for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        const uint64_t r = list ? list[t] : t;
         const float* v = x + r * d;
         for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
             if (i < d) {
@@ -197,6 +203,269 @@ __global__ __launch_bounds__(256) void k_rank_encode(const float* __restrict__ x
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// One-pass row encode (src/rank.rs:38-156 with src/quant.rs:587-614, and
+// src/sign_bitmap.rs:180-200): one wave per row, lane l holding the EPL
+// consecutive coordinates l*EPL .. l*EPL+EPL-1 in registers.  In the same
+// pass over the f32 row it records non-finite values, writes the sign bits
+// and the RankQuant codes.
+//
+// The codes need each coordinate's bucket floor(rank * 2^b / d), not the
+// full ranking.  The row's values are binned into d equal-width bins over
+// [min, max] (the map v -> bin is monotone, and equal values share a bin),
+// a histogram gives every bin's rank range [cum, cum + cnt), and a bin whose
+// whole range falls in one bucket gives its elements that bucket whatever
+// their order inside the bin.  Only bins whose range straddles a bucket
+// boundary (at most 2^b - 1 of them) need the exact order: their elements
+// are ranked inside the bin by (value, index), the reference's order
+// (cmp_finite_f32_then_index, src/util.rs:19-30; -0.0 and +0.0 tie).  A row
+// whose straddling bins hold more than kEncSpecial elements, or whose range
+// does not bin (overflowing max - min), is appended to a list that
+// k_rank_encode (the whole-row sort) finishes in a second launch, so every
+// row is exact.  Each wave requests its next row before working on the
+// current one.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kEncSpecial = 128;
+
+template <int EPL>
+__device__ __forceinline__ void load_row(const float* __restrict__ p, float (&f)[EPL]) {
+    if constexpr (EPL % 4 == 0) {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int i = 0; i < EPL / 4; i++) {
+            const f32x4 v = *(reinterpret_cast<const f32x4*>(p) + i);
+            f[4 * i] = v.x;
+            f[4 * i + 1] = v.y;
+            f[4 * i + 2] = v.z;
+            f[4 * i + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < EPL; i++) f[i] = p[i];
+    }
+}
+
+// Wave-wide min / max: DPP steps inside each 16-lane row (quad swaps, half
+// and full row mirrors), then the four row results read as scalars.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float a) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(a), __float_as_int(a), CTRL, 0xF, 0xF, false));
+}
+
+template <bool MAX>
+__device__ __forceinline__ float wave_minmax(float v) {
+    auto op = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
+    v = op(v, dpp_f32<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = op(v, dpp_f32<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = op(v, dpp_f32<0x141>(v));  // row_half_mirror
+    v = op(v, dpp_f32<0x140>(v));  // row_mirror
+    const float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return op(op(a, b), op(c, d));
+}
+
+// Exclusive prefix sum over the wave's lanes: in-row inclusive scan by DPP
+// row shifts (lanes shifted in from outside the row read 0), then the row
+// totals added as scalars.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t lane) {
+    uint32_t x = v;
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);  // row_shr:8
+    const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)x, 15);
+    const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)x, 31);
+    const uint32_t t2 = (uint32_t)__builtin_amdgcn_readlane((int)x, 47);
+    const uint32_t row = lane >> 4;
+    const uint32_t base = (row > 0 ? t0 : 0u) + (row > 1 ? t1 : 0u) + (row > 2 ? t2 : 0u);
+    return base + x - v;
+}
+
+// Writes NB bytes held little-endian in w[] at p.
+template <int NB>
+__device__ __forceinline__ void store_bytes(uint8_t* p, const uint32_t (&w)[(NB + 3) / 4]) {
+    if constexpr (NB == 16) {
+        *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+    } else if constexpr (NB == 8) {
+        *reinterpret_cast<uint2*>(p) = make_uint2(w[0], w[1]);
+    } else if constexpr (NB == 4) {
+        *reinterpret_cast<uint32_t*>(p) = w[0];
+    } else if constexpr (NB == 2) {
+        *reinterpret_cast<uint16_t*>(p) = (uint16_t)w[0];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NB; i++) p[i] = (uint8_t)(w[i / 4] >> ((i % 4) * 8));
+    }
+}
+
+// The encode workgroup is a single wave, and LDS instructions of a wave run
+// in order: its lanes share LDS data with no hardware barrier, only a code
+// motion barrier (so the next row's global loads stay in flight).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
+// LDS per workgroup (one wave): the histogram and the straddling-bin list.
+template <int EPL>
+__host__ __device__ constexpr size_t enc_lds_bytes() {
+    return (size_t)64 * EPL * 4 + kEncSpecial * (8 + 4 + 4);
+}
+
+template <int EPL, int BITS, bool CODES, bool SIGNS>
+__global__ __launch_bounds__(64) void k_encode_rows(const float* __restrict__ x, uint64_t n,
+                                                    uint8_t* __restrict__ codes, uint64_t* __restrict__ words,
+                                                    uint64_t first_doc, uint32_t* __restrict__ flag,
+                                                    uint32_t* __restrict__ fb_rows, uint32_t* __restrict__ fb_count) {
+    constexpr uint32_t D = 64 * EPL, NB = 1u << BITS, CPB = 8 / BITS, LANE_BYTES = EPL * BITS / 8;
+    constexpr uint32_t NW = (LANE_BYTES + 3) / 4;
+    static_assert(!CODES || (EPL * BITS) % 8 == 0, "a lane's codes must fill whole bytes");
+    static_assert(!SIGNS || EPL % 8 == 0, "a lane's sign bits must fill whole bytes");
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint32_t* hist = reinterpret_cast<uint32_t*>(lds);                       // [D] counts, then cum << 16 | cnt
+    uint64_t* spec_key = reinterpret_cast<uint64_t*>(lds + (size_t)D * 4);   // [kEncSpecial]
+    uint32_t* spec_bin = reinterpret_cast<uint32_t*>(spec_key + kEncSpecial);  // [kEncSpecial]
+    uint32_t* spec_out = spec_bin + kEncSpecial;                             // [kEncSpecial] exact buckets
+    const uint32_t lane = threadIdx.x;
+    bool bad = false;
+    float nxt[EPL];
+    if (blockIdx.x < n) load_row<EPL>(x + (uint64_t)blockIdx.x * D + (uint64_t)lane * EPL, nxt);
+    for (uint64_t r = blockIdx.x; r < n; r += gridDim.x) {
+        float f[EPL];
+#pragma unroll
+        for (int i = 0; i < EPL; i++) f[i] = nxt[i];
+        if (r + gridDim.x < n) load_row<EPL>(x + (r + gridDim.x) * D + (uint64_t)lane * EPL, nxt);
+#pragma unroll
+        for (int i = 0; i < EPL; i++) bad |= !isfinite(f[i]);
+        if constexpr (SIGNS) {
+            // bit j of word j / 64 is (v_j > 0); the lane's EPL bits sit at
+            // bit (lane * EPL) % 64 of word lane * EPL / 64
+            uint32_t sw[(EPL / 8 + 3) / 4] = {};
+#pragma unroll
+            for (int i = 0; i < EPL; i++) sw[i / 32] |= (f[i] > 0.0f ? 1u : 0u) << (i % 32);
+            const uint64_t doc = first_doc + r;
+            const uint32_t w = lane * EPL / 64, qpv = D / 64;
+            const uint64_t widx = (((doc >> 6) * qpv + w) << 6) + (doc & 63);
+            store_bytes<EPL / 8>(reinterpret_cast<uint8_t*>(words + widx) + (lane * EPL % 64) / 8, sw);
+        }
+        if constexpr (CODES) {
+            float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+            for (int i = 0; i < EPL; i++) {
+                f[i] = __fadd_rn(f[i], 0.0f);  // -0.0 + 0.0 = +0.0: the two zeros tie
+                lo = fminf(lo, f[i]);
+                hi = fmaxf(hi, f[i]);
+            }
+            lo = wave_minmax<false>(lo);
+            hi = wave_minmax<true>(hi);
+            const float scale = (float)D / (hi - lo);
+            const bool all_equal = !(hi > lo);  // every element equal: rank = index order
+            uint32_t cw[NW] = {};
+            // code of element i lands in byte i / CPB, MSB-first inside the byte
+            auto put = [&](int i, uint32_t b) { cw[(i / CPB) / 4] |= b << (((i / CPB) % 4) * 8 + (CPB - 1 - i % CPB) * BITS); };
+            bool listed = false;
+            if (all_equal) {
+#pragma unroll
+                for (int i = 0; i < EPL; i++) put(i, ((lane * EPL + i) * NB) / D);
+            } else if (!(isfinite(scale) && scale > 0.0f)) {
+                listed = true;
+            } else {
+                uint32_t bin[EPL];
+#pragma unroll
+                for (int i = 0; i < EPL; i++) {
+                    const float t = __fmul_rn(__fsub_rn(f[i], lo), scale);
+                    bin[i] = t >= (float)D ? D - 1 : (t > 0.0f ? (uint32_t)t : 0u);
+                }
+#pragma unroll
+                for (int i = 0; i < EPL; i += 4)
+                    *reinterpret_cast<uint4*>(hist + lane * EPL + i) = make_uint4(0, 0, 0, 0);
+                wave_lds_sync();
+#pragma unroll
+                for (int i = 0; i < EPL; i++) atomicAdd(&hist[bin[i]], 1u);
+                wave_lds_sync();
+                // exclusive prefix over bins (lane owns bins lane*EPL ..)
+                uint32_t own[EPL], tot = 0;
+#pragma unroll
+                for (int i = 0; i < EPL; i += 4) {
+                    const uint4 c = *reinterpret_cast<const uint4*>(hist + lane * EPL + i);
+                    own[i] = c.x;
+                    own[i + 1] = c.y;
+                    own[i + 2] = c.z;
+                    own[i + 3] = c.w;
+                    tot += c.x + c.y + c.z + c.w;
+                }
+                uint32_t cum = wave_excl_scan(tot, lane);
+#pragma unroll
+                for (int i = 0; i < EPL; i++) {
+                    const uint32_t c = own[i];
+                    own[i] = (cum << 16) | c;
+                    cum += c;
+                }
+#pragma unroll
+                for (int i = 0; i < EPL; i += 4)
+                    *reinterpret_cast<uint4*>(hist + lane * EPL + i) = make_uint4(own[i], own[i + 1], own[i + 2], own[i + 3]);
+                wave_lds_sync();
+                // a bin whose rank range stays in one bucket gives it to all its
+                // elements; the others' elements are listed for exact ranking
+                uint32_t nspec = 0, spec_mask = 0;
+#pragma unroll
+                for (int i = 0; i < EPL; i++) {
+                    const uint32_t h = hist[bin[i]];
+                    const uint32_t c0 = h >> 16, c1 = c0 + (h & 0xffffu) - 1;
+                    const uint32_t b0 = (c0 * NB) / D, b1 = (c1 * NB) / D;
+                    put(i, b0);
+                    if (b0 != b1) {
+                        spec_mask |= 1u << i;
+                        nspec++;
+                    }
+                }
+                const uint32_t sbase = wave_excl_scan(nspec, lane);
+                const uint32_t stotal = (uint32_t)__builtin_amdgcn_readlane((int)(sbase + nspec), 63);
+                if (stotal > kEncSpecial) {
+                    listed = true;
+                } else if (stotal > 0) {
+                    uint32_t pos = sbase;
+#pragma unroll
+                    for (int i = 0; i < EPL; i++)
+                        if (spec_mask & (1u << i)) {
+                            spec_key[pos] = ((uint64_t)f32_ord(f[i]) << 16) | (lane * EPL + i);
+                            spec_bin[pos] = bin[i];
+                            pos++;
+                        }
+                    wave_lds_sync();
+                    // rank inside the bin by (value, index), from the bin's first rank
+                    for (uint32_t e = lane; e < stotal; e += 64) {
+                        const uint64_t ke = spec_key[e];
+                        const uint32_t be = spec_bin[e];
+                        uint32_t below = 0;
+                        for (uint32_t g = 0; g < stotal; g++)
+                            below += (spec_bin[g] == be && spec_key[g] < ke) ? 1u : 0u;
+                        spec_out[e] = (((hist[be] >> 16) + below) * NB) / D;
+                    }
+                    wave_lds_sync();
+                    pos = sbase;
+#pragma unroll
+                    for (int i = 0; i < EPL; i++)
+                        if (spec_mask & (1u << i)) {
+                            const int sh = ((i / CPB) % 4) * 8 + (CPB - 1 - i % CPB) * BITS;
+                            cw[(i / CPB) / 4] = (cw[(i / CPB) / 4] & ~((NB - 1) << sh)) | (spec_out[pos] << sh);
+                            pos++;
+                        }
+                }
+                wave_lds_sync();  // the histogram is reused by the next row
+            }
+            if (listed) {  // wave-uniform: the whole-row sort finishes this row
+                if (lane == 0) fb_rows[atomicAdd(fb_count, 1u)] = (uint32_t)r;
+            } else {
+                store_bytes<LANE_BYTES>(codes + r * (D * BITS / 8) + lane * LANE_BYTES, cw);
+            }
+        }
+    }
+    if (__any(bad) && lane == 0) atomicOr(flag, 1u);
 }
 
 // src/fastscan.rs:87-110: out[b*(d/2)*32 + p*32 + lane] = code[2p]<<2 | code[2p+1]
@@ -2532,6 +2801,91 @@ void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, u
     ktime_end(kFamEncode, s);
 }
 
+static uint32_t next_pow2_host(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// One-pass encode of n rows (codes and/or interleaved sign words, non-finite
+// flag), then the whole-row sort for the rows it listed.
+template <int EPL, int BITS>
+static void enc_launch(const float* x, uint64_t n, uint8_t* codes, uint64_t* words, uint64_t first_doc,
+                       uint32_t* flag, uint32_t* fb_rows, uint32_t* fb_count, uint32_t grid, hipStream_t s) {
+    const size_t lds = enc_lds_bytes<EPL>();
+#define OVX_ENC(C, W)                                                                                               \
+    hipLaunchKernelGGL((k_encode_rows<EPL, BITS, C, W>), dim3(grid), dim3(64), lds, s, x, n, codes, words, first_doc, \
+                       flag, fb_rows, fb_count)
+    if constexpr ((EPL * BITS) % 8 == 0 && EPL % 8 == 0) {
+        if (codes && words) {
+            OVX_ENC(true, true);
+            return;
+        }
+    }
+    if constexpr ((EPL * BITS) % 8 == 0) {
+        if (codes) {
+            OVX_ENC(true, false);
+            return;
+        }
+    }
+    if constexpr (EPL % 8 == 0 && BITS == 1) OVX_ENC(false, true);
+#undef OVX_ENC
+}
+
+template <int EPL>
+static void enc_epl(const float* x, uint64_t n, uint32_t bits, uint8_t* codes, uint64_t* words, uint64_t first_doc,
+                    uint32_t* flag, uint32_t* fb_rows, uint32_t* fb_count, uint32_t grid, hipStream_t s) {
+    if (!codes) bits = 1;
+    if (bits == 1) enc_launch<EPL, 1>(x, n, codes, words, first_doc, flag, fb_rows, fb_count, grid, s);
+    else if (bits == 2) enc_launch<EPL, 2>(x, n, codes, words, first_doc, flag, fb_rows, fb_count, grid, s);
+    else enc_launch<EPL, 4>(x, n, codes, words, first_doc, flag, fb_rows, fb_count, grid, s);
+}
+
+bool encode_rows_supported(uint32_t d, uint32_t bits, bool codes, bool words) {
+    if (!codes && !words) return false;
+    if (d != 512 && d != 768 && d != 1024 && d != 2048) return false;
+    const uint32_t epl = d / 64;
+    if (codes && (bits != 1 && bits != 2 && bits != 4)) return false;
+    if (codes && (epl * bits) % 8 != 0) return false;
+    if (words && epl % 8 != 0) return false;
+    return true;
+}
+
+size_t encode_rows_work_bytes(uint64_t n) { return align256((n + 64) * 4); }
+
+bool launch_encode_rows(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, uint64_t* words,
+                        uint64_t first_doc, uint32_t* flag, void* work, hipStream_t s) {
+    if (!encode_rows_supported(d, bits, codes != nullptr, words != nullptr)) return false;
+    if (n == 0) return true;
+    if (n > 0xffffffffull) throw Error(kInternal, "one-pass encode: more than 2^32 rows in one launch");
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
+    }
+    uint32_t* fb_count = static_cast<uint32_t*>(work);
+    uint32_t* fb_rows = fb_count + 64;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(n, (uint64_t)cus * 32);
+    ktime_begin(kFamEncode, s);
+    if (codes) OVX_HIP(hipMemsetAsync(fb_count, 0, 4, s));
+    switch (d) {
+        case 512: enc_epl<8>(x, n, bits, codes, words, first_doc, flag, fb_rows, fb_count, grid, s); break;
+        case 768: enc_epl<12>(x, n, bits, codes, words, first_doc, flag, fb_rows, fb_count, grid, s); break;
+        case 1024: enc_epl<16>(x, n, bits, codes, words, first_doc, flag, fb_rows, fb_count, grid, s); break;
+        default: enc_epl<32>(x, n, bits, codes, words, first_doc, flag, fb_rows, fb_count, grid, s); break;
+    }
+    check_launch();
+    if (codes) {  // rows the one-pass kernel listed: the whole-row sort
+        const uint32_t P = next_pow2_host(d);
+        const uint32_t g2 = (uint32_t)std::min<uint64_t>(n, (uint64_t)cus * 2);
+        hipLaunchKernelGGL(k_rank_encode<false>, dim3(g2), dim3(256), (size_t)P * 8 + d, s, x, n, d, bits, codes,
+                           nullptr, fb_rows, fb_count);
+        check_launch();
+    }
+    ktime_end(kFamEncode, s);
+    return true;
+}
+
 void launch_rank_u16(const float* x, uint64_t n, uint32_t d, uint16_t* ranks, hipStream_t s) {
     if (n == 0) return;
     uint32_t P = 1;
@@ -2554,11 +2908,6 @@ void launch_fastscan_pack(const uint8_t* codes, uint64_t n, uint32_t d, uint8_t*
     ktime_end(kFamEncode, s);
 }
 
-static uint32_t next_pow2_host(uint32_t x) {
-    uint32_t p = 1;
-    while (p < x) p <<= 1;
-    return p;
-}
 
 void launch_topbits_docs(const float* x, uint64_t n, uint32_t d, uint32_t ntop, uint64_t* out, uint64_t first_doc,
                          hipStream_t s) {

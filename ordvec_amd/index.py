@@ -947,6 +947,18 @@ def two_stage_search(sign: SignBitmap, rq: RankQuant, queries, m: int, k: int,
     return s, i
 
 
+def two_stage_add(sign: SignBitmap, rq: RankQuant, vectors) -> None:
+    """SignBitmap.add and RankQuant.add of the same vectors in one pass over
+    them (same results as the two adds; a non-finite value leaves both
+    indexes unchanged)."""
+    d, _, _ = rq._info()
+    x, n, flag = _float_rows(vectors, d, "vectors")
+    if n == 0:
+        return
+    _bind_stream() if flag else _host_stream()
+    N.check(N.lib().ordvec_two_stage_add(sign._h, rq._h, _ptr(x), n, flag))
+
+
 def two_stage_search_policy(sign: SignBitmap, rq: RankQuant, queries, k: int, policy=None,
                             assume_finite: bool = False):
     """two_stage_search with m = policy.candidate_count(k, n) (default policy
