@@ -236,6 +236,19 @@ ordvec_status_t ordvec_rankquant_search_subset_batched(const ordvec_rankquant_t 
                                                        const uint32_t *candidates, uint64_t n_candidates,
                                                        uint64_t k, float *out_scores, int64_t *out_ids,
                                                        uint64_t flags);
+/* src/rank.rs:38-72 rank_transform of n rows of `dim` values (dim <= 65535):
+ * out_ranks[r][k] = position of v[k] in the row's stable ascending order
+ * (ties, including -0.0 / +0.0, by index). */
+ordvec_status_t ordvec_rank_transform(const float *vectors, uint64_t n, uint64_t dim, uint16_t *out_ranks,
+                                      uint64_t flags);
+/* src/quant.rs:1541-1610 rankquant_eval_search: the eval-only symmetric
+ * search at any bit width 1..8 (non-persistable widths included), corpus and
+ * queries rank-bucketed on the fly; score = sum_j centre(bq_j) centre(bd_j) /
+ * norm^2 with the empirical norm; outputs nq x min(k, n) by (score desc, id
+ * asc).  dim <= 8192 on the GPU. */
+ordvec_status_t ordvec_rankquant_eval_search(const float *corpus, uint64_t n, const float *queries, uint64_t nq,
+                                             uint64_t dim, uint32_t bits, uint64_t k, float *out_scores,
+                                             int64_t *out_ids, uint64_t flags);
 /* Copy the packed codes (n x dim*bits/8, .ovrq payload order). */
 ordvec_status_t ordvec_rankquant_codes(const ordvec_rankquant_t *index, uint8_t *out, uint64_t capacity,
                                        uint64_t flags);
@@ -286,6 +299,15 @@ ordvec_status_t ordvec_sign_bitmap_info(const ordvec_sign_bitmap_t *index, uint6
 ordvec_status_t ordvec_sign_bitmap_top_m(const ordvec_sign_bitmap_t *index, const float *queries, uint64_t nq,
                                          uint64_t m, uint32_t *out_candidates, uint32_t *out_hamming,
                                          uint64_t flags);
+/* src/sign_bitmap.rs:204-214 build_query_bitmap for nq queries: out_words
+ * nq x dim/64 u64 (bit j of word j/64 set iff q[j] > 0).  Non-finite queries
+ * are BAD_ARGUMENT (the reference asserts finiteness). */
+ordvec_status_t ordvec_sign_bitmap_query_bits(const ordvec_sign_bitmap_t *index, const float *queries, uint64_t nq,
+                                              uint64_t *out_words, uint64_t flags);
+/* src/sign_bitmap.rs:492-548 score_all / score_all_batched_flat: out_scores
+ * nq x n u32 sign agreements dim - popcount(doc XOR q), by document id. */
+ordvec_status_t ordvec_sign_bitmap_score_all(const ordvec_sign_bitmap_t *index, const float *queries, uint64_t nq,
+                                             uint32_t *out_scores, uint64_t flags);
 /* Copy the bitmaps row-major (n x dim/64 u64, .ovsb payload order). */
 ordvec_status_t ordvec_sign_bitmap_words(const ordvec_sign_bitmap_t *index, uint64_t *out, uint64_t capacity,
                                          uint64_t flags);

@@ -53,6 +53,7 @@ U32 = C.c_uint32
 def _declare(L):
     sig = {
         "orc_simd_supported": (C.c_int, []),
+        "orc_eval_search": (None, [P, U64, P, U64, U32, U32, U64, P, P]),
         "orc_set_simd": (C.c_int, [C.c_int]),
         "orc_rank_transform": (None, [P, U32, P]),
         "orc_rank_transform_qsort": (None, [P, U32, P]),
@@ -103,6 +104,20 @@ def _f32(x):
 
 
 # --- src/rank.rs -------------------------------------------------------------
+
+def eval_search(corpus, queries, bits: int, k: int):
+    """rankquant_eval_search (src/quant.rs:1541-1610)."""
+    c = _f32(corpus)
+    q = _f32(queries).reshape(-1, c.shape[1])
+    n, d = c.shape
+    nq = q.shape[0]
+    kk = min(k, n)
+    os_ = np.zeros((nq, kk), np.float32)
+    oi = np.full((nq, kk), -1, np.int64)
+    if kk and nq:
+        lib().orc_eval_search(_p(c), n, _p(q), nq, d, bits, kk, _p(os_), _p(oi))
+    return os_, oi
+
 
 def simd(enable=None) -> bool:
     """AVX-512 forms of the stage-1 scan and the AVX-512 scoring tier

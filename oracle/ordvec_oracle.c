@@ -895,6 +895,47 @@ void orc_rq_search_symmetric(const uint8_t *codes, uint64_t n, uint32_t d, uint3
     }
 }
 
+/* src/quant.rs:1541-1610 rankquant_eval_search: corpus and queries
+ * rank-bucketed at `bits` (1..8); score = (sum_j centre(bq_j) * centre(bd_j),
+ * sequential) * (1 / norm^2) with norm = sqrt(sum_rank centre(bucket)^2) in
+ * f64 (src/quant.rs:106-117); TopK by (score desc, id asc). */
+void orc_eval_search(const float *corpus, uint64_t n, const float *queries, uint64_t nq, uint32_t d, uint32_t bits,
+                     uint64_t k, float *os, int64_t *oi) {
+    if (k > n) k = n;
+    if (k == 0) return;
+    double acc2 = 0.0;
+    for (uint32_t r = 0; r < d; r++) {
+        double c = (double)orc_bucket_centre(orc_rank_to_bucket(r, d, bits), bits);
+        acc2 += c * c;
+    }
+    float norm = (float)sqrt(acc2);
+    float inv_norm_sq = 1.0f / (norm * norm);
+    uint8_t *db = (uint8_t *)malloc(n * d);
+    uint16_t *rk = (uint16_t *)malloc(d * sizeof(uint16_t));
+    for (uint64_t i = 0; i < n; i++) {
+        orc_rank_transform(corpus + i * d, d, rk);
+        for (uint32_t j = 0; j < d; j++) db[i * d + j] = orc_rank_to_bucket(rk[j], d, bits);
+    }
+    float *qc = (float *)malloc(d * sizeof(float));
+    topk_t t;
+    topk_init(&t, k);
+    for (uint64_t qi = 0; qi < nq; qi++) {
+        orc_rank_transform(queries + qi * d, d, rk);
+        for (uint32_t j = 0; j < d; j++) qc[j] = orc_bucket_centre(orc_rank_to_bucket(rk[j], d, bits), bits);
+        topk_reset(&t);
+        for (uint64_t i = 0; i < n; i++) {
+            float acc = -0.0f;
+            for (uint32_t j = 0; j < d; j++) acc = acc + qc[j] * orc_bucket_centre(db[i * d + j], bits);
+            topk_insert(&t, acc * inv_norm_sq, (int64_t)i, (int64_t)i);
+        }
+        topk_finalize(&t, os + qi * k, oi + qi * k, k);
+    }
+    topk_free(&t);
+    free(db);
+    free(rk);
+    free(qc);
+}
+
 /* src/rank.rs:249-252 rank_norm = sqrt(d (d^2 - 1) / 12), f64 then f32. */
 float orc_rank_norm(uint32_t d) {
     double x = (double)d;

@@ -25,6 +25,18 @@ from .index import (  # noqa: E402,F401
     two_stage_search,
     two_stage_search_policy,
 )
+from .primitives import (  # noqa: E402,F401
+    bucket_centre,
+    bucket_ranks,
+    pack_buckets,
+    rank_norm,
+    rank_to_bucket,
+    rank_transform,
+    rankquant_bytes_per_vec,
+    rankquant_eval_search,
+    rankquant_norm,
+    unpack_buckets,
+)
 from .serving import Server  # noqa: E402,F401
 
 try:  # the autograd binding needs torch
@@ -51,4 +63,14 @@ __all__ = [
     "MAX_DIM",
     "MAX_SIGN_BITMAP_DIM",
     "MAX_VECTORS",
+    "rank_transform",
+    "rank_to_bucket",
+    "bucket_ranks",
+    "pack_buckets",
+    "unpack_buckets",
+    "rankquant_bytes_per_vec",
+    "bucket_centre",
+    "rank_norm",
+    "rankquant_norm",
+    "rankquant_eval_search",
 ]

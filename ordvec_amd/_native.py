@@ -152,6 +152,8 @@ SIGNATURES = {
     "ordvec_rankquant_search_subset": (ST, [P, P, P, U64, U64, P, P, P, U64]),
     "ordvec_rankquant_search_subset_batched": (ST, [P, P, U64, P, P, U64, U64, P, P, U64]),
     "ordvec_rankquant_codes": (ST, [P, P, U64, U64]),
+    "ordvec_rankquant_eval_search": (ST, [P, U64, P, U64, U64, U32, U64, P, P, U64]),
+    "ordvec_rank_transform": (ST, [P, U64, U64, P, U64]),
     "ordvec_rankquant_write": (ST, [P, C.c_char_p]),
     "ordvec_rankquant_load": (ST, [C.c_char_p, C.POINTER(P)]),
     "ordvec_rankquant_free": (None, [P]),
@@ -160,6 +162,8 @@ SIGNATURES = {
     "ordvec_sign_bitmap_info": (ST, [P, P, P]),
     "ordvec_sign_bitmap_top_m": (ST, [P, P, U64, U64, P, P, U64]),
     "ordvec_sign_bitmap_words": (ST, [P, P, U64, U64]),
+    "ordvec_sign_bitmap_query_bits": (ST, [P, P, U64, P, U64]),
+    "ordvec_sign_bitmap_score_all": (ST, [P, P, U64, P, U64]),
     "ordvec_sign_bitmap_write": (ST, [P, C.c_char_p]),
     "ordvec_sign_bitmap_load": (ST, [C.c_char_p, C.POINTER(P)]),
     "ordvec_sign_bitmap_free": (None, [P]),

@@ -413,9 +413,26 @@ bool one_pass_ok(uint32_t dim, uint32_t bits, bool codes, bool words, const floa
            (!(flags & ORDVEC_FLAG_DEVICE_PTRS) || reinterpret_cast<uintptr_t>(vectors) % 16 == 0);
 }
 
+// Rank transform of cnt device rows into u16 ranks (ranks != null) or
+// RankQuant codes: the per-row LDS sort up to kEncodeMaxDim, a per-row
+// segmented sort (second workspace) above, up to the format limit of 65535.
+void rank_rows(const float* dv, uint64_t cnt, uint32_t d, uint16_t* ranks, uint32_t bits, uint8_t* codes) {
+    hipStream_t s = cur().stream;
+    if (d <= kEncodeMaxDim) {
+        if (ranks) launch_rank_u16(dv, cnt, d, ranks, s);
+        else launch_rank_encode(dv, cnt, d, bits, codes, s);
+        return;
+    }
+    const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(65535, (1ull << 30) / (16ull * d)));
+    for (uint64_t r0 = 0; r0 < cnt; r0 += per) {
+        const uint64_t c = std::min(per, cnt - r0);
+        void* work = cur().workspace2(rank_sort_work_bytes(c, d, s));
+        launch_rank_sort(dv + r0 * d, c, d, ranks ? ranks + r0 * d : nullptr, bits,
+                         codes ? codes + r0 * (d * bits / 8) : nullptr, work, s);
+    }
+}
+
 void rq_add(RqIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
-    if (ix->dim > kEncodeMaxDim)
-        fail(kUnsupportedOperation, "GPU rank encode supports dim <= " + std::to_string(kEncodeMaxDim));
     if (one_pass_ok(ix->dim, ix->bits, true, false, vectors, flags)) {
         eng().init();
         add_one_pass(ix, nullptr, vectors, n, flags);
@@ -426,7 +443,7 @@ void rq_add(RqIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
     ix->reserve(ix->n + n);
     uint64_t base = ix->n;
     for_each_chunk(vectors, n, ix->dim, flags, [&](const float* dv, uint64_t r0, uint64_t cnt) {
-        launch_rank_encode(dv, cnt, ix->dim, ix->bits, ix->codes + (base + r0) * ix->bpv, cur().stream);
+        rank_rows(dv, cnt, ix->dim, nullptr, ix->bits, ix->codes + (base + r0) * ix->bpv);
     });
     ix->n += n;
     if (!(flags & ORDVEC_FLAG_DEVICE_PTRS)) sync();
@@ -741,14 +758,12 @@ void rq_subset_search(const RqIndex* ix, const float* queries, uint64_t nq, cons
 constexpr uint64_t kQueryChunk = 16384;
 
 void rank_add(RkIndex* ix, const float* vectors, uint64_t n, uint64_t flags) {
-    if (ix->dim > kEncodeMaxDim)
-        fail(kUnsupportedOperation, "GPU rank transform supports dim <= " + std::to_string(kEncodeMaxDim));
     check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
     check_new_count(ix->n, n);
     ix->reserve(ix->n + n);
     const uint64_t base = ix->n;
     for_each_chunk(vectors, n, ix->dim, flags, [&](const float* dv, uint64_t r0, uint64_t cnt) {
-        launch_rank_u16(dv, cnt, ix->dim, ix->ranks + (base + r0) * ix->dim, cur().stream);
+        rank_rows(dv, cnt, ix->dim, ix->ranks + (base + r0) * ix->dim, 0, nullptr);
     });
     ix->n += n;
     if (!(flags & ORDVEC_FLAG_DEVICE_PTRS)) sync();
@@ -790,8 +805,6 @@ void chunked_full_search(uint32_t d, uint64_t n, const float* queries, uint64_t 
 void rank_search(const RkIndex* ix, const float* queries, uint64_t nq, uint64_t k, bool symmetric, float* out_s,
                  int64_t* out_i, uint64_t flags) {
     const uint32_t d = ix->dim;
-    if (symmetric && d > kEncodeMaxDim)
-        fail(kUnsupportedOperation, "GPU query rank transform supports dim <= " + std::to_string(kEncodeMaxDim));
     check_search_dim(d);
     const size_t extra = symmetric ? (size_t)d * 2 : (size_t)d * 4 + 4;
     chunked_full_search(d, ix->n, queries, nq, k, out_s, out_i, flags, !symmetric, extra,
@@ -801,7 +814,7 @@ void rank_search(const RkIndex* ix, const float* queries, uint64_t nq, uint64_t 
                             void* work = cv.take<char>(fullscan_work_bytes(ix->n, cn, kk));
                             if (symmetric) {
                                 uint16_t* qr = cv.take<uint16_t>(cn * d);
-                                launch_rank_u16(dq, cn, d, qr, s);
+                                rank_rows(dq, cn, d, qr, 0, nullptr);
                                 launch_rank_search(ix->ranks, ix->n, d, true, qr, nullptr, nullptr, cn, kk, work, ds,
                                                    di, s);
                             } else {
@@ -819,8 +832,6 @@ void rank_search(const RkIndex* ix, const float* queries, uint64_t nq, uint64_t 
 void rq_symmetric_search(const RqIndex* ix, const float* queries, uint64_t nq, uint64_t k, float* out_s,
                          int64_t* out_i, uint64_t flags) {
     const uint32_t d = ix->dim;
-    if (d > kEncodeMaxDim)
-        fail(kUnsupportedOperation, "GPU query rank transform supports dim <= " + std::to_string(kEncodeMaxDim));
     const double nb = (double)(1u << ix->bits);  // src/rank.rs:275-280 rankquant_norm
     const float R = (float)std::sqrt((double)d * ((nb * nb - 1.0) / 12.0));
     const float inv_norm_sq = 1.0f / (R * R);
@@ -831,7 +842,7 @@ void rq_symmetric_search(const RqIndex* ix, const float* queries, uint64_t nq, u
                             uint16_t* qr = cv.take<uint16_t>(cn * d);
                             float* qc = cv.take<float>(cn * d);
                             float* zero = cv.take<float>(cn);
-                            launch_rank_u16(dq, cn, d, qr, s);
+                            rank_rows(dq, cn, d, qr, 0, nullptr);
                             launch_rank_centres(qr, cn * d, d, ix->bits, qc, s);
                             OVX_HIP(hipMemsetAsync(zero, 0, cn * 4, s));
                             launch_fullscan_rq(ix->codes, ix->n, d, ix->bits, 0, inv_norm_sq, qc, zero, cn, kk, work,
@@ -3122,8 +3133,6 @@ ordvec_status_t ordvec_two_stage_add(ordvec_sign_bitmap_t* sign, ordvec_rankquan
             return;
         }
         // no one-pass form for this (dim, bits): the two adds, vectors checked once
-        if (ix->dim > kEncodeMaxDim)
-            fail(kUnsupportedOperation, "GPU rank encode supports dim <= " + std::to_string(kEncodeMaxDim));
         check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
         check_new_count(ix->n, n);
         check_new_count(sb->n, n);
@@ -3151,6 +3160,174 @@ ordvec_status_t ordvec_sign_bitmap_top_m(const ordvec_sign_bitmap_t* index, cons
         NONNULL(out_candidates, "out_candidates");
         LOCK;
         sb_top_m(ix, queries, nq, m_eff, out_candidates, out_hamming, flags);
+    });
+}
+
+ordvec_status_t ordvec_rank_transform(const float* vectors, uint64_t n, uint64_t dim, uint16_t* out_ranks,
+                                      uint64_t flags) {
+    return boundary([&] {
+        if (dim > kMaxDim) fail(kBadArgument, "length " + std::to_string(dim) + " exceeds u16::MAX (65535)");
+        if (n == 0 || dim == 0) return;
+        NONNULL(vectors, "vectors");
+        NONNULL(out_ranks, "out_ranks");
+        check_finite(vectors, n * dim, flags, kBadArgument, "vectors");
+        LOCK;
+        eng().init();
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        const uint32_t d = (uint32_t)dim;
+        if (d == 1) {  // a single value has rank 0
+            if (dev) OVX_HIP(hipMemsetAsync(out_ranks, 0, n * 2, cur().stream));
+            else std::memset(out_ranks, 0, n * 2);
+            return;
+        }
+        const uint64_t rows = dev ? n : std::max<uint64_t>(1, (256ull << 20) / (4ull * d));
+        for (uint64_t r0 = 0; r0 < n; r0 += rows) {
+            const uint64_t c = std::min(rows, n - r0);
+            Carver cv{static_cast<char*>(cur().workspace(dev ? 256 : al(c * d * 4) + al(c * d * 2)))};
+            const float* dv = stage_queries(cv, vectors + r0 * d, c * d, flags);
+            uint16_t* rk = dev ? out_ranks + r0 * d : cv.take<uint16_t>(c * d);
+            rank_rows(dv, c, d, rk, 0, nullptr);
+            if (!dev) {
+                d2h(out_ranks + r0 * d, rk, c * d * 2);
+                sync();
+            }
+        }
+    });
+}
+
+ordvec_status_t ordvec_rankquant_eval_search(const float* corpus, uint64_t n, const float* queries, uint64_t nq,
+                                             uint64_t dim, uint32_t bits, uint64_t k, float* out_scores,
+                                             int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        // src/quant.rs:96-117,1541-1575 parameter checks
+        if (bits < 1 || bits > 8) fail(kBadArgument, "bits must be in 1..=8");
+        if (dim < 2) fail(kBadArgument, "dim must be >= 2");
+        if (dim > kMaxDim) fail(kBadArgument, "dim must fit in u16");
+        if (dim > 8192) fail(kUnsupportedOperation, "GPU eval search stages the query in LDS and supports dim <= 8192");
+        const uint64_t kk = std::min<uint64_t>(k, n);
+        if (nq == 0 || kk == 0) return;
+        NONNULL(corpus, "corpus");
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        check_finite(corpus, n * dim, flags, kBadArgument, "corpus");
+        check_finite(queries, nq * dim, flags, kBadArgument, "queries");
+        LOCK;
+        eng().init();
+        const uint32_t d = (uint32_t)dim;
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        hipStream_t s = cur().stream;
+        // norm: sqrt of sum_rank centre(bucket(rank))^2 in f64 (src/quant.rs:106-117)
+        double acc = 0.0;
+        const double half = (double)((1u << bits) - 1) / 2.0;
+        for (uint64_t r = 0; r < d; r++) {
+            const double c = (double)(float)((double)((r << bits) / d) - half);
+            acc += c * c;
+        }
+        const float norm = (float)std::sqrt(acc);
+        const float inv_norm_sq = 1.0f / (norm * norm);
+        // corpus buckets, row chunks through the rank transform
+        DevBuf buckets;
+        struct FreeOnExit {  // the bucket matrix lives for this call only, also on the error paths
+            DevBuf& b;
+            hipStream_t s;
+            ~FreeOnExit() {
+                if (b.p) {
+                    (void)hipStreamSynchronize(s);
+                    (void)hipFree(b.p);
+                }
+            }
+        } release{buckets, s};
+        uint8_t* db = static_cast<uint8_t*>(buckets.get(std::max<uint64_t>(256, n * d), s));
+        const uint64_t rows = std::max<uint64_t>(1, (256ull << 20) / (4ull * d));
+        for (uint64_t r0 = 0; r0 < n; r0 += rows) {
+            const uint64_t c = std::min(rows, n - r0);
+            Carver cv{static_cast<char*>(cur().workspace(al(c * d * 2) + (dev ? 0 : al(c * d * 4))))};
+            const float* dv = stage_queries(cv, corpus + r0 * d, c * d, flags);
+            uint16_t* rk = cv.take<uint16_t>(c * d);
+            rank_rows(dv, c, d, rk, 0, nullptr);
+            launch_eval_buckets(rk, c * d, d, bits, db + r0 * d, s);
+            sync();
+        }
+        // queries in chunks whose score keys stay within 1 GiB
+        const uint64_t qc = std::max<uint64_t>(1, std::min<uint64_t>({nq, 65535, (1ull << 30) / (16 * n + 1)}));
+        for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
+            const uint64_t cn = std::min(qc, nq - q0);
+            const size_t wb = eval_search_work_bytes(n, cn, d, s);
+            Carver cv{static_cast<char*>(cur().workspace(wb + (dev ? 0 : al(cn * d * 4) + al(cn * kk * 12))))};
+            char* work = cv.take<char>(wb);
+            const float* dq = stage_queries(cv, queries + q0 * d, cn * d, flags);
+            float* ds = dev ? out_scores + q0 * kk : cv.take<float>(cn * kk);
+            int64_t* di = dev ? out_ids + q0 * kk : cv.take<int64_t>(cn * kk);
+            uint16_t* qr = reinterpret_cast<uint16_t*>(work);
+            rank_rows(dq, cn, d, qr, 0, nullptr);
+            launch_eval_search(db, n, d, bits, inv_norm_sq, qr, cn, (uint32_t)kk, ds, di, work, s);
+            if (!dev) {
+                d2h(out_scores + q0 * kk, ds, cn * kk * 4);
+                d2h(out_ids + q0 * kk, di, cn * kk * 8);
+            }
+            sync();
+        }
+        sync();
+    });
+}
+
+ordvec_status_t ordvec_sign_bitmap_query_bits(const ordvec_sign_bitmap_t* index, const float* queries, uint64_t nq,
+                                              uint64_t* out_words, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = SB(index);
+        if (nq == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_words, "out_words");
+        host_finite_or_fail(queries, nq * ix->dim, flags, kBadArgument, "query");
+        LOCK;
+        eng().init();
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        const uint64_t qpv = ix->qpv;
+        Carver cv{static_cast<char*>(cur().workspace(dev ? 256 : al(nq * ix->dim * 4) + al(nq * qpv * 8)))};
+        const float* dq = stage_queries(cv, queries, nq * ix->dim, flags);
+        uint64_t* dw = dev ? out_words : cv.take<uint64_t>(nq * qpv);
+        if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE)) check_finite(queries, nq * ix->dim, flags, kBadArgument, "query");
+        launch_sign_encode(dq, nq, ix->dim, dw, 0, false, cur().stream);
+        if (!dev) {
+            d2h(out_words, dw, nq * qpv * 8);
+            sync();
+        }
+    });
+}
+
+ordvec_status_t ordvec_sign_bitmap_score_all(const ordvec_sign_bitmap_t* index, const float* queries, uint64_t nq,
+                                             uint32_t* out_scores, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(index, "index");
+        const SbIndex* ix = SB(index);
+        if (nq == 0 || ix->n == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        if (ix->ntop) fail(kBadArgument, "score_all is a SignBitmap method");
+        host_finite_or_fail(queries, nq * ix->dim, flags, kBadArgument, "query");
+        LOCK;
+        eng().init();
+        const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+        if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE)) check_finite(queries, nq * ix->dim, flags, kBadArgument, "query");
+        const uint64_t qpv = ix->qpv;
+        // host outputs go back in query chunks whose scores fit 1 GiB of workspace
+        const uint64_t qc = dev ? nq : std::max<uint64_t>(1, std::min<uint64_t>(nq, (1ull << 30) / (ix->n * 4)));
+        for (uint64_t q0 = 0; q0 < nq; q0 += qc) {
+            const uint64_t cn = std::min(qc, nq - q0);
+            Carver cv{static_cast<char*>(cur().workspace(al(cn * qpv * 8) + (dev ? 0 : al(cn * ix->dim * 4) +
+                                                                                        al(cn * ix->n * 4))))};
+            const float* dq = stage_queries(cv, queries + q0 * ix->dim, cn * ix->dim, flags);
+            uint64_t* qb = cv.take<uint64_t>(cn * qpv);
+            uint32_t* ds = dev ? out_scores + q0 * ix->n : cv.take<uint32_t>(cn * ix->n);
+            launch_sign_encode(dq, cn, ix->dim, qb, 0, false, cur().stream);
+            launch_sign_score_all(ix->words, ix->n, ix->dim, qb, cn, ds, cur().stream);
+            if (!dev) {
+                d2h(out_scores + q0 * ix->n, ds, cn * ix->n * 4);
+                sync();
+            }
+        }
     });
 }
 
@@ -3554,7 +3731,6 @@ ordvec_status_t ordvec_fastscan_add(ordvec_fastscan_t* index, const float* vecto
         if (ix->n != 0)
             fail(kBadArgument, "FastScan v1: incremental add() not supported (block-32 layout has tail-padding semantics "
                                "that don't compose); construct a new RankQuantFastscan instead");
-        if (ix->dim > kEncodeMaxDim) fail(kUnsupportedOperation, "GPU rank encode supports dim <= 4096");
         LOCK;
         check_finite(vectors, n * ix->dim, flags, kBadArgument, "vectors");
         check_new_count(0, n);

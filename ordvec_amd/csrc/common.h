@@ -99,6 +99,24 @@ void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bi
                             float* qhat, float* offset, uint32_t* qstat, hipStream_t s);
 void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, uint64_t first_doc, bool interleaved,
                         hipStream_t s);
+// Rank transform of n rows of any d <= 65535 by a per-row sort: u16 ranks
+// (ranks != null) or RankQuant codes of `bits` (codes != null); work =
+// rank_sort_work_bytes(n, d), n <= 65535 rows per launch.
+size_t rank_sort_work_bytes(uint64_t n, uint32_t d, hipStream_t s);
+void launch_rank_sort(const float* x, uint64_t n, uint32_t d, uint16_t* ranks, uint32_t bits, uint8_t* codes,
+                      void* work, hipStream_t s);
+// rankquant_eval_search: u8 buckets of width `bits` from u16 ranks; the
+// search itself (query ranks qranks [nq][d], placed by the caller at the start
+// of work = eval_search_work_bytes; top-k by (score desc, id asc)).
+void launch_eval_buckets(const uint16_t* ranks, uint64_t count, uint32_t d, uint32_t bits, uint8_t* out,
+                         hipStream_t s);
+size_t eval_search_work_bytes(uint64_t n, uint64_t nq, uint32_t d, hipStream_t s);
+void launch_eval_search(const uint8_t* buckets, uint64_t n, uint32_t d, uint32_t bits, float inv_norm_sq,
+                        const uint16_t* qranks, uint64_t nq, uint32_t k, float* out_s, int64_t* out_i, void* work,
+                        hipStream_t s);
+// out[q][doc] = d - popcount(doc XOR q) for every document (interleaved words).
+void launch_sign_score_all(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t* qbits, uint64_t nq,
+                           uint32_t* out, hipStream_t s);
 void launch_rank_encode(const float* x, uint64_t n, uint32_t d, uint32_t bits, uint8_t* codes, hipStream_t s);
 // One pass over n f32 rows: RankQuant codes (codes != null, row-major; rows
 // whose values do not bin finely enough are finished by the whole-row sort

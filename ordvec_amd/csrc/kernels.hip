@@ -468,6 +468,105 @@ __global__ __launch_bounds__(64) void k_encode_rows(const float* __restrict__ x,
     if (__any(bad) && lane == 0) atomicOr(flag, 1u);
 }
 
+// Uniform row offsets q * row_m for the segmented sort (large k, long rows).
+__global__ void k_row_offsets(uint64_t* __restrict__ offs, uint64_t nq, uint64_t row_m) {
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= nq; q += (uint64_t)gridDim.x * blockDim.x)
+        offs[q] = q * row_m;
+}
+
+// Rank transform of long rows (d above the LDS sort's capacity, up to
+// 65535): keys (orderable value << 16 | index) per element, a per-row sort,
+// then each sorted position scattered to its element (rank[idx] = position);
+// -0.0 ties +0.0 as in the one-pass kernels.
+__global__ void k_rank_keys(const float* __restrict__ x, uint64_t count, uint32_t d, uint64_t* __restrict__ keys) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+        keys[i] = ((uint64_t)f32_ord(__fadd_rn(x[i], 0.0f)) << 16) | (uint32_t)(i % d);
+}
+
+// ranks != null: ranks[row][idx] = position; else codes (packed MSB-first,
+// bucket = position * 2^bits / d) built through an LDS row of buckets.
+__global__ __launch_bounds__(256) void k_rank_scatter(const uint64_t* __restrict__ sorted, uint64_t n, uint32_t d,
+                                                      uint16_t* __restrict__ ranks, uint32_t bits,
+                                                      uint8_t* __restrict__ codes) {
+    extern __shared__ uint8_t rowb[];
+    for (uint64_t r = blockIdx.x; r < n; r += gridDim.x) {
+        const uint64_t* sr = sorted + r * d;
+        if (ranks) {
+            for (uint32_t p = threadIdx.x; p < d; p += blockDim.x) ranks[r * d + (sr[p] & 0xffffu)] = (uint16_t)p;
+            continue;
+        }
+        const uint32_t nb = 1u << bits, cpb = 8 / bits, bpv = d / cpb;
+        for (uint32_t p = threadIdx.x; p < d; p += blockDim.x)
+            rowb[sr[p] & 0xffffu] = (uint8_t)(((uint64_t)p * nb) / d);
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < bpv; t += blockDim.x) {
+            uint32_t byte = 0;
+            for (uint32_t j = 0; j < cpb; j++) byte |= (uint32_t)rowb[t * cpb + j] << ((cpb - 1 - j) * bits);
+            codes[r * bpv + t] = (uint8_t)byte;
+        }
+        __syncthreads();
+    }
+}
+
+// rankquant_eval_search (src/quant.rs:1541-1610): rank buckets of any width
+// 1..8 computed on the fly.  Buckets from u16 ranks; query centres; scores
+// (sum over coordinates, in order, of q_centre * centre(doc bucket)) times
+// 1 / norm^2 as score keys for every document.
+__global__ void k_eval_buckets(const uint16_t* __restrict__ ranks, uint64_t count, uint32_t d, uint32_t bits,
+                               uint8_t* __restrict__ out, float* __restrict__ centres) {
+    const float half = (float)((1u << bits) - 1) / 2.0f;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)(((uint64_t)ranks[i] << bits) / d);
+        if (out) out[i] = (uint8_t)b;
+        if (centres) centres[i] = __fsub_rn((float)b, half);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_eval_keys(const uint8_t* __restrict__ buckets, uint64_t n, uint32_t d,
+                                                   uint32_t bits, const float* __restrict__ qcent, float inv_norm_sq,
+                                                   uint64_t* __restrict__ keys) {
+    extern __shared__ float eval_qc[];
+    float* qc = eval_qc;
+    const uint64_t q = blockIdx.y;
+    const float half = (float)((1u << bits) - 1) / 2.0f;
+    for (uint32_t j = threadIdx.x; j < d; j += blockDim.x) qc[j] = qcent[q * d + j];
+    __syncthreads();
+    for (uint64_t doc = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; doc < n; doc += (uint64_t)gridDim.x * blockDim.x) {
+        const uint8_t* row = buckets + doc * d;
+        float acc = -0.0f;
+        for (uint32_t j = 0; j < d; j++) acc = __fadd_rn(acc, __fmul_rn(qc[j], __fsub_rn((float)row[j], half)));
+        keys[q * n + doc] = score_key(__fmul_rn(acc, inv_norm_sq), (uint32_t)doc);
+    }
+}
+
+// SignBitmap::score_all / score_all_batched_flat (src/sign_bitmap.rs:
+// 492-548): out[q][doc] = dim - popcount(doc XOR q), the sign agreement,
+// for every document.  Grid (64-document blocks / 4, query tiles of 8); a
+// wave holds one block's words (coalesced 512-byte loads) and scores them
+// against the tile's queries (wave-uniform words).
+__global__ __launch_bounds__(256) void k_sign_score_all(const uint64_t* __restrict__ words, uint64_t n, uint32_t qpv,
+                                                        const uint64_t* __restrict__ qbits, uint32_t nq,
+                                                        uint32_t* __restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t blk = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t q0 = blockIdx.y * 8, qn = min(8u, nq - q0);
+    const uint64_t doc = blk * 64 + lane;
+    if (blk * 64 >= n) return;
+    uint32_t acc[8] = {};
+    const uint64_t* base = words + blk * (uint64_t)qpv * 64 + lane;
+    for (uint32_t w = 0; w < qpv; w++) {
+        const uint64_t dw = base[(uint64_t)w * 64];
+#pragma unroll
+        for (uint32_t t = 0; t < 8; t++)
+            if (t < qn) acc[t] += (uint32_t)__popcll(dw ^ qbits[(uint64_t)(q0 + t) * qpv + w]);
+    }
+    if (doc < n) {
+#pragma unroll
+        for (uint32_t t = 0; t < 8; t++)
+            if (t < qn) out[(uint64_t)(q0 + t) * n + doc] = qpv * 64 - acc[t];
+    }
+}
+
 // src/fastscan.rs:87-110: out[b*(d/2)*32 + p*32 + lane] = code[2p]<<2 | code[2p+1]
 // from packed b=2 rows (tail lanes of the last block stay zero).
 __global__ void k_fastscan_pack(const uint8_t* __restrict__ codes, uint64_t n, uint32_t d, uint8_t* __restrict__ out) {
@@ -2886,6 +2985,94 @@ bool launch_encode_rows(const float* x, uint64_t n, uint32_t d, uint32_t bits, u
     return true;
 }
 
+size_t rank_sort_work_bytes(uint64_t n, uint32_t d, hipStream_t s) {
+    return 2 * align256(n * d * 8) + align256((n + 1) * 8) + segsort_temp_bytes(n * d, (uint32_t)n, nullptr, s);
+}
+
+void launch_rank_sort(const float* x, uint64_t n, uint32_t d, uint16_t* ranks, uint32_t bits, uint8_t* codes,
+                      void* work, hipStream_t s) {
+    if (n == 0) return;
+    if (n > 65535) throw Error(kInternal, "rank sort: more than 65535 rows in one launch");
+    char* c = static_cast<char*>(work);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(c);
+    c += align256(n * d * 8);
+    uint64_t* sorted = reinterpret_cast<uint64_t*>(c);
+    c += align256(n * d * 8);
+    uint64_t* offs = reinterpret_cast<uint64_t*>(c);
+    c += align256((n + 1) * 8);
+    const size_t tb = segsort_temp_bytes(n * d, (uint32_t)n, nullptr, s);
+    ktime_begin(kFamEncode, s);
+    hipLaunchKernelGGL(k_rank_keys, dim3((uint32_t)std::min<uint64_t>(65536, (n * d + 255) / 256)), dim3(256), 0, s,
+                       x, n * d, d, keys);
+    check_launch();
+    hipLaunchKernelGGL(k_row_offsets, dim3((uint32_t)std::min<uint64_t>(1024, (n + 256) / 256)), dim3(256), 0, s,
+                       offs, n, (uint64_t)d);
+    check_launch();
+    segsort_keys(c, tb, keys, sorted, n * d, (uint32_t)n, offs, s);
+    hipLaunchKernelGGL(k_rank_scatter, dim3((uint32_t)std::min<uint64_t>(n, 65535)), dim3(256), ranks ? 0 : (size_t)d,
+                       s, sorted, n, d, ranks, bits, codes);
+    check_launch();
+    ktime_end(kFamEncode, s);
+}
+
+size_t eval_search_work_bytes(uint64_t n, uint64_t nq, uint32_t d, hipStream_t s) {
+    return align256(nq * (uint64_t)d * 2) + align256(nq * (uint64_t)d * 4) + 2 * align256(nq * n * 8) +
+           align256((nq + 1) * 8) + segsort_temp_bytes(nq * n, (uint32_t)nq, nullptr, s);
+}
+
+void launch_eval_buckets(const uint16_t* ranks, uint64_t count, uint32_t d, uint32_t bits, uint8_t* out,
+                         hipStream_t s) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(k_eval_buckets, dim3((uint32_t)std::min<uint64_t>(65536, (count + 255) / 256)), dim3(256), 0,
+                       s, ranks, count, d, bits, out, nullptr);
+    check_launch();
+}
+
+void launch_eval_search(const uint8_t* buckets, uint64_t n, uint32_t d, uint32_t bits, float inv_norm_sq,
+                        const uint16_t* qranks, uint64_t nq, uint32_t k, float* out_s, int64_t* out_i, void* work,
+                        hipStream_t s) {
+    if (n == 0 || nq == 0 || k == 0) return;
+    if (nq > 65535) throw Error(kInternal, "eval search: more than 65535 queries in one launch");
+    char* c = static_cast<char*>(work);
+    float* qcent = reinterpret_cast<float*>(c + align256(nq * (uint64_t)d * 2));
+    c += align256(nq * (uint64_t)d * 2) + align256(nq * (uint64_t)d * 4);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(c);
+    c += align256(nq * n * 8);
+    uint64_t* sorted = reinterpret_cast<uint64_t*>(c);
+    c += align256(nq * n * 8);
+    uint64_t* offs = reinterpret_cast<uint64_t*>(c);
+    c += align256((nq + 1) * 8);
+    const size_t tb = segsort_temp_bytes(nq * n, (uint32_t)nq, nullptr, s);
+    ktime_begin(kFamFullScan, s);
+    hipLaunchKernelGGL(k_eval_buckets, dim3((uint32_t)std::min<uint64_t>(65536, (nq * d + 255) / 256)), dim3(256), 0,
+                       s, qranks, nq * d, d, bits, nullptr, qcent);
+    check_launch();
+    const uint32_t gx = (uint32_t)std::min<uint64_t>(65535, std::max<uint64_t>(1, (n + 255) / 256 / 4));
+    hipLaunchKernelGGL(k_eval_keys, dim3(gx, (uint32_t)nq), dim3(256), (size_t)d * 4, s, buckets, n, d, bits, qcent,
+                       inv_norm_sq, keys);
+    check_launch();
+    hipLaunchKernelGGL(k_row_offsets, dim3((uint32_t)std::min<uint64_t>(1024, (nq + 256) / 256)), dim3(256), 0, s,
+                       offs, nq, n);
+    check_launch();
+    segsort_keys(c, tb, keys, sorted, nq * n, (uint32_t)nq, offs, s);
+    hipLaunchKernelGGL(k_gather_keys, dim3((k + 255) / 256, (uint32_t)nq), dim3(256), 0, s, sorted, offs, n, k, 0u,
+                       out_s, out_i);
+    check_launch();
+    ktime_end(kFamFullScan, s);
+}
+
+void launch_sign_score_all(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t* qbits, uint64_t nq,
+                           uint32_t* out, hipStream_t s) {
+    if (n == 0 || nq == 0) return;
+    const uint64_t nblk = (n + 63) / 64;
+    for (uint64_t q0 = 0; q0 < nq; q0 += 8 * 65535ull) {
+        const uint64_t qc = std::min<uint64_t>(nq - q0, 8 * 65535ull);
+        hipLaunchKernelGGL(k_sign_score_all, dim3((uint32_t)((nblk + 3) / 4), (uint32_t)((qc + 7) / 8)), dim3(256), 0,
+                           s, words, n, d / 64, qbits + q0 * (d / 64), (uint32_t)qc, out + q0 * n);
+        check_launch();
+    }
+}
+
 void launch_rank_u16(const float* x, uint64_t n, uint32_t d, uint16_t* ranks, hipStream_t s) {
     if (n == 0) return;
     uint32_t P = 1;
@@ -3595,11 +3782,6 @@ void launch_fastscan_prepare(const float* qhat, uint64_t nq, uint32_t d, uint8_t
     ktime_end(kFamFastScan, s);
 }
 
-// Large k: uniform row offsets q * row_m for the segmented sort.
-__global__ void k_row_offsets(uint64_t* __restrict__ offs, uint64_t nq, uint64_t row_m) {
-    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= nq; q += (uint64_t)gridDim.x * blockDim.x)
-        offs[q] = q * row_m;
-}
 
 struct LargeKWork {
     uint64_t *keys, *sorted, *offs;

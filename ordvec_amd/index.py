@@ -538,6 +538,34 @@ class SignBitmap:
         _bind_stream() if flag else _host_stream()
         N.check(N.lib().ordvec_sign_bitmap_add(self._h, _ptr(x), n, flag))
 
+    def score_all_batched(self, queries):
+        """score_all_batched (src/sign_bitmap.rs:507-560): u32 [nq, n] sign
+        agreements (dim - Hamming distance), columns by document id."""
+        d, n = self._info()
+        q, nq, flag = _float_rows(queries, d, "queries")
+        out = _out((nq, n), np.uint32, flag)
+        if nq and n:
+            _bind_stream() if flag else _host_stream()
+            N.check(N.lib().ordvec_sign_bitmap_score_all(self._h, _ptr(q), nq, _ptr(out), flag))
+        return out
+
+    def score_all(self, query):
+        """score_all (src/sign_bitmap.rs:492-505): u32 [n] sign agreements."""
+        d, _ = self._info()
+        q, _ = _float_vec(query, d, "query")
+        return self.score_all_batched(q.reshape(1, d))[0]
+
+    def build_query_bitmap(self, query):
+        """build_query_bitmap (src/sign_bitmap.rs:204-214): u64 [dim / 64],
+        bit j set iff q[j] > 0."""
+        d, _ = self._info()
+        q, flag = _float_vec(query, d, "query")
+        out = _out((d // 64,), np.uint64, flag) if not flag else torch.empty((d // 64,), dtype=torch.int64,
+                                                                              device="cuda")
+        _bind_stream() if flag else _host_stream()
+        N.check(N.lib().ordvec_sign_bitmap_query_bits(self._h, _ptr(q), 1, _ptr(out), flag))
+        return out
+
     def top_m_candidates_batched(self, queries, m: int, return_hamming: bool = False):
         """top_m_candidates_batched (src/sign_bitmap.rs:369-415): nq x min(m, n)
         candidate ids ordered (hamming asc, id asc)."""

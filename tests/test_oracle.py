@@ -574,3 +574,56 @@ def test_simd_forms_equal_scalar_restatements(oracle, dim):
             assert np.array_equal(ss.view(np.uint32), fs.view(np.uint32))
     finally:
         oracle.simd(True)
+
+
+def _eval_bruteforce(corpus, queries, bits, k):
+    """Eval search restated with numpy: stable-argsort ranks, bucket
+    floor(r * 2^bits / d), centres b - (2^bits - 1)/2, f64 dot products over
+    the f64 empirical norm, top-k by (score desc, id asc)."""
+    d = corpus.shape[1]
+
+    def centres(x):
+        r = np.empty(x.shape, np.int64)
+        order = np.argsort(np.where(x == 0, 0, x), axis=1, kind="stable")
+        np.put_along_axis(r, order, np.arange(d)[None, :].repeat(len(x), 0), axis=1)
+        return (r << bits) // d - ((1 << bits) - 1) / 2.0
+
+    cc, qc = centres(corpus), centres(queries)
+    norm2 = float(np.sum((((np.arange(d) << bits) // d) - ((1 << bits) - 1) / 2.0) ** 2))
+    s = qc @ cc.T / norm2
+    ids = np.stack([np.lexsort((np.arange(len(corpus)), -row))[:k] for row in s])
+    return np.take_along_axis(s, ids, 1), ids
+
+
+@pytest.mark.parametrize("bits", [1, 2, 3, 4, 5, 7, 8])
+def test_eval_search_matches_bruteforce_and_symmetric(oracle, bits):
+    rng = np.random.default_rng(60 + bits)
+    d = 128
+    corpus = rng.standard_normal((45, d)).astype(np.float32)
+    corpus[3] = corpus[7]                      # duplicate documents tie on score
+    queries = rng.standard_normal((5, d)).astype(np.float32)
+    s, i = oracle.eval_search(corpus, queries, bits, 9)
+    ws, wi = _eval_bruteforce(corpus, queries, bits, 9)
+    assert np.array_equal(i, wi)
+    np.testing.assert_allclose(s, ws, rtol=0, atol=1e-6)
+    if bits in (1, 2, 4):  # tests/index/quant.rs:91-117: equal to the packed symmetric search
+        ps, pi = oracle.rq_search_symmetric(oracle.rq_encode(corpus, bits), d, bits, queries, 9)
+        assert np.array_equal(i, pi)
+        np.testing.assert_allclose(s, ps, rtol=0, atol=1e-6)
+    # a plausible slip (closed-form norm where occupancy is uneven) is visible at d = 100
+    if bits == 3:
+        c100 = rng.standard_normal((20, 100)).astype(np.float32)
+        s100, _ = oracle.eval_search(c100, c100[:2], 3, 1)
+        assert abs(float(s100[0, 0]) - 1.0) < 1e-6  # a document scores exactly 1 against itself
+    assert oracle.eval_search(corpus, queries[:0], bits, 9)[0].shape == (0, 9)
+
+
+def test_eval_search_b8_any_dim(oracle):
+    # tests/index/quant_b8.rs:527-556: b = 8 at dim 384 returns valid ids
+    dim, n, nq = 384, 32, 2
+    corpus = ((np.arange(n * dim) * 7 % 101).astype(np.float32) - 50.0).reshape(n, dim)
+    queries = ((np.arange(nq * dim) * 13 % 97).astype(np.float32) - 48.0).reshape(nq, dim)
+    s, i = oracle.eval_search(corpus, queries, 8, 5)
+    assert i.shape == (2, 5) and ((i >= 0) & (i < n)).all()
+    ws, wi = _eval_bruteforce(corpus, queries, 8, 5)
+    np.testing.assert_allclose(s, ws, rtol=0, atol=1e-6)
