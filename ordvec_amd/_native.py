@@ -8,7 +8,10 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libordvec_amd.so")
+# ORDVEC_AMD_LIB names another build of the same library (the CPU sanitizer
+# build of the host code is one).
+LIB_PATH = os.environ.get("ORDVEC_AMD_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib",
+                                                            "libordvec_amd.so")
 
 # Status codes (include/ordvec.h).
 OK = 0

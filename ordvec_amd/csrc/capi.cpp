@@ -24,6 +24,8 @@
 #include <string>
 #include <vector>
 
+#include <sys/stat.h>
+
 #include "common.h"
 
 namespace ovx {
@@ -1814,27 +1816,41 @@ struct Reader {
     }
 };
 
-std::vector<uint8_t> read_file(const char* path, size_t limit = SIZE_MAX) {
+// Largest index file: the longest header (17 B, .ovbm) plus MAX_PAYLOAD.
+constexpr uint64_t kMaxFileBytes = kMaxPayload + 17;
+
+// Opens a regular file for reading and returns its size.  Directories, FIFOs
+// and devices are IO errors (a read of them cannot fill an index).
+FILE* open_index_file(const char* path, uint64_t& size) {
     FILE* f = std::fopen(path, "rb");
     if (!f) fail(kIo, std::string("open index: ") + std::strerror(errno));
-    std::fseek(f, 0, SEEK_END);
-    long sz = std::ftell(f);
-    std::fseek(f, 0, SEEK_SET);
-    size_t want = std::min<size_t>((size_t)std::max<long>(sz, 0), limit);
+    struct stat st;
+    if (fstat(fileno(f), &st) != 0 || !S_ISREG(st.st_mode)) {
+        std::fclose(f);
+        fail(kIo, "open index: not a regular file");
+    }
+    size = (uint64_t)st.st_size;
+    return f;
+}
+
+// The first min(size, limit) bytes of a file; a whole-file read (no limit)
+// refuses files larger than any valid index before allocating.
+std::vector<uint8_t> read_file(const char* path, size_t limit = SIZE_MAX) {
+    uint64_t sz = 0;
+    std::unique_ptr<FILE, int (*)(FILE*)> f(open_index_file(path, sz), &std::fclose);
+    if (limit == SIZE_MAX && sz > kMaxFileBytes)
+        fail(kCorruptIndex, "index file of " + std::to_string(sz) + " B exceeds header + MAX_PAYLOAD");
+    const size_t want = (size_t)std::min<uint64_t>(sz, limit);
     std::vector<uint8_t> b(want);
-    size_t got = want ? std::fread(b.data(), 1, want, f) : 0;
-    std::fclose(f);
+    const size_t got = want ? std::fread(b.data(), 1, want, f.get()) : 0;
     if (got != want) fail(kIo, "short read");
     return b;
 }
 
 uint64_t file_size(const char* path) {
-    FILE* f = std::fopen(path, "rb");
-    if (!f) fail(kIo, std::string("open index: ") + std::strerror(errno));
-    std::fseek(f, 0, SEEK_END);
-    long sz = std::ftell(f);
-    std::fclose(f);
-    return (uint64_t)std::max<long>(sz, 0);
+    uint64_t sz = 0;
+    std::fclose(open_index_file(path, sz));
+    return sz;
 }
 
 void write_file(const char* path, const std::vector<uint8_t>& hdr, const void* payload, uint64_t bytes) {
@@ -2100,11 +2116,12 @@ SbIndex* load_ovsb(const char* path) {
     uint32_t n = r.u32("n_vectors");
     if (n > kMaxVectors) fail(kCorruptIndex, "n_vectors exceeds MAX_VECTORS");
     r.payload_exact((uint64_t)n * (dim / 64) * 8);
+    std::vector<uint64_t> rows((uint64_t)n * (dim / 64));
+    std::memcpy(rows.data(), r.buf.data() + r.pos, rows.size() * 8);  // little-endian host
+    eng().init();
     SbIndex* ix = new SbIndex();
     ix->dim = dim;
     ix->qpv = dim / 64;
-    std::vector<uint64_t> rows((uint64_t)n * ix->qpv);
-    std::memcpy(rows.data(), r.buf.data() + r.pos, rows.size() * 8);  // little-endian host
     try {
         sb_upload_rowmajor(ix, rows.data(), n);
     } catch (...) {
@@ -2218,6 +2235,7 @@ FsIndex* load_ovfs(const char* path) {
                         fail(kCorruptIndex, "OVFS row " + std::to_string(b * 32 + l) + " violates b=2 constant composition");
         }
     }
+    eng().init();
     FsIndex* ix = new FsIndex();
     ix->dim = dim;
     ix->n = n;
@@ -2419,7 +2437,6 @@ ordvec_status_t ordvec_index_load(const char* path, uint64_t flags, ordvec_index
         uint64_t fsz = file_size(path);
         if (magic_is(head, "OVRQ", "TVRQ")) {
             LOCK;
-            eng().init();
             IndexHandle* h = new IndexHandle();
             try {
                 h->rq = load_ovrq(path);
@@ -2939,8 +2956,7 @@ ordvec_status_t ordvec_rankquant_load(const char* path, ordvec_rankquant_t** out
         NONNULL(path, "path");
         NONNULL(out, "out");
         *out = nullptr;
-        LOCK;
-        eng().init();
+        LOCK;  // the loader checks the file before any device set-up
         *out = reinterpret_cast<ordvec_rankquant_t*>(load_ovrq(path));
     });
 }
@@ -3361,8 +3377,7 @@ ordvec_status_t ordvec_sign_bitmap_load(const char* path, ordvec_sign_bitmap_t**
         NONNULL(path, "path");
         NONNULL(out, "out");
         *out = nullptr;
-        LOCK;
-        eng().init();
+        LOCK;  // the loader checks the file before any device set-up
         *out = reinterpret_cast<ordvec_sign_bitmap_t*>(load_ovsb(path));
     });
 }
@@ -3836,8 +3851,7 @@ ordvec_status_t ordvec_fastscan_load(const char* path, ordvec_fastscan_t** out) 
         NONNULL(path, "path");
         NONNULL(out, "out");
         *out = nullptr;
-        LOCK;
-        eng().init();
+        LOCK;  // the loader checks the file before any device set-up
         *out = reinterpret_cast<ordvec_fastscan_t*>(load_ovfs(path));
     });
 }
