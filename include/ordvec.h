@@ -214,6 +214,19 @@ ordvec_status_t ordvec_kernel_timing_enable(int enable);
  * it reports and no others; mask 0 turns timing off. */
 ordvec_status_t ordvec_kernel_timing_enable_families(uint32_t mask);
 ordvec_status_t ordvec_kernel_timing_read(uint32_t family, uint64_t *launches, double *total_ms);
+/* Execution-path switches (no counterpart in the reference): each selects
+ * between two paths of this engine that return identical results, for tests
+ * that compare them and for measurement.  Sets option to value (0 or 1) and,
+ * when previous is not NULL, stores the value it replaces; value -1 only
+ * reads.  Process-wide; calls in flight keep the path they started on.
+ *  ORDVEC_OPTION_ONE_QUERY_PATH: 1 (default) runs a one-query two-stage call
+ *    in three launches (stage-1 head, scan, select+rerank+merge tail); 0 runs
+ *    it through the general batched pipeline.
+ *  ORDVEC_OPTION_MFMA_SCAN: 1 (default) runs batched stage-1 scans of two or
+ *    more queries on the matrix cores; 0 uses the vector-ALU scan. */
+#define ORDVEC_OPTION_ONE_QUERY_PATH 1
+#define ORDVEC_OPTION_MFMA_SCAN 2
+ordvec_status_t ordvec_engine_option(uint32_t option, int64_t value, int64_t *previous);
 
 /* RankQuant: src/quant.rs:438-491 new, 587-614 add, 713-847 search_asymmetric. */
 ordvec_status_t ordvec_rankquant_new(uint64_t dim, uint32_t bits, ordvec_rankquant_t **out);

@@ -146,6 +146,7 @@ SIGNATURES = {
     "ordvec_synchronize": (ST, []),
     "ordvec_stage1_exact_queries": (ST, [P]),
     "ordvec_kernel_timing_enable": (ST, [C.c_int]),
+    "ordvec_engine_option": (ST, [U32, C.c_int64, C.POINTER(C.c_int64)]),
     "ordvec_kernel_timing_enable_families": (ST, [U32]),
     "ordvec_kernel_timing_read": (ST, [U32, P, P]),
     "ordvec_rankquant_new": (ST, [U64, U32, C.POINTER(P)]),
@@ -284,6 +285,18 @@ def check(status: int) -> None:
 
 KERNEL_FAMILIES = {"sign_scan": 0, "stage1_select": 1, "rerank": 2, "full_scan": 3, "fastscan": 4,
                    "encode": 5, "prepare": 6, "stage1_sample": 7, "backward": 8}
+
+
+ENGINE_OPTIONS = {"one_query_path": 1, "mfma_scan": 2}
+
+
+def engine_option(name: str, value=None) -> bool:
+    """Switch between equivalent execution paths (ordvec_engine_option):
+    value None only reads; returns the setting before the call."""
+    prev = C.c_int64()
+    check(lib().ordvec_engine_option(ENGINE_OPTIONS[name], -1 if value is None else int(bool(value)),
+                                     C.byref(prev)))
+    return bool(prev.value)
 
 
 def kernel_timing_enable(on: bool = True, families=None) -> None:

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -104,8 +105,32 @@ struct Ctx {
     uint32_t* pinned = nullptr;
     size_t pinned_count = 0;
     hipEvent_t status_ev = nullptr;  // marks the copy of a call's status words to `pinned`
+    // Device words that stay zero between calls: the one-query path's sample
+    // histogram and arrival counters (the kernels that use them clear them
+    // before they exit).  Zeroed once, when allocated.
+    uint32_t* zw = nullptr;
+    size_t zw_count = 0;
 
     void* workspace(size_t bytes) { return ws.get(bytes, stream); }
+    uint32_t* zero_words(size_t count) {
+        if (count > zw_count) {
+            if (zw) {
+                OVX_HIP(hipStreamSynchronize(stream));
+                OVX_HIP(hipFree(zw));
+                zw = nullptr;
+                zw_count = 0;
+            }
+            const size_t want = std::max<size_t>(count, 16384);
+            if (hipMalloc(&zw, want * 4) != hipSuccess) {
+                (void)hipGetLastError();
+                zw = nullptr;
+                fail(kInternal, "device allocation of the zeroed counter words failed");
+            }
+            OVX_HIP(hipMemsetAsync(zw, 0, want * 4, stream));
+            zw_count = want;
+        }
+        return zw;
+    }
     void* workspace2(size_t bytes) { return ws2.get(bytes, stream); }
     uint32_t* pinned_u32(size_t count) {
         if (count > pinned_count) {
@@ -124,6 +149,9 @@ struct Ctx {
         if (ws.p) (void)hipFree(ws.p);
         if (ws2.p) (void)hipFree(ws2.p);
         if (pinned) (void)hipHostFree(pinned);
+        if (zw) (void)hipFree(zw);
+        zw = nullptr;
+        zw_count = 0;
         if (status_ev) (void)hipEventDestroy(status_ev);
         status_ev = nullptr;
         ws.p = ws2.p = nullptr;
@@ -1079,6 +1107,72 @@ void two_stage_big(const SbIndex* sb, const RqIndex* ix, const float* queries, u
 
 // Two-stage search on the calling thread's context (src/sign_bitmap.rs:453-483
 // then src/quant.rs:1336-1394).  Outputs nq x min(k, n).
+// ORDVEC_OPTION_ONE_QUERY_PATH: one-query calls take two_stage_plain.
+std::atomic<bool> g_plain_path{true};
+
+// One query (the plain form): three launches (stage-1 head, scan, tail) and
+// one wait; the tail writes the status word straight to pinned host memory.
+// A query whose survivors cannot prove exactness is recomputed as in
+// two_stage_run.
+void two_stage_plain(const SbIndex* sb, const RqIndex* ix, const float* queries, uint64_t m_eff, uint64_t out_k,
+                     float* out_scores, int64_t* out_ids, uint64_t flags) {
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    const uint32_t d = ix->dim, qpv = d / 64;
+    const FastPlan p = plan_stage1_fast(sb->n, d, 1, m_eff);
+    const uint32_t nsplit = rerank_nsplit(1, m_eff);
+    Ctx& c = cur();
+    uint32_t* zw = c.zero_words((size_t)p.nbins + 64);
+    uint32_t* hist = zw;
+    uint32_t* done_head = zw + p.nbins;
+    uint32_t* done_tail = done_head + 1;
+    uint32_t* hstat = c.pinned_u32(1);
+    hstat[0] = 0xffffffffu;  // overwritten by the tail
+    const size_t bytes = al(d * 4) + 4 * al(4) + al((uint64_t)p.cap * 8) + al(qpv * 8) + al(m_eff * 4) +
+                         rerank_work_bytes(1, (uint32_t)out_k, nsplit) +
+                         (dev ? 0 : al(d * 4) + al(out_k * 4) + al(out_k * 8));
+    Carver cv{static_cast<char*>(c.workspace(bytes))};
+    const float* dq = stage_queries(cv, queries, d, flags);
+    float* qhat = cv.take<float>(d);
+    float* off = cv.take<float>(1);
+    uint32_t* qstat = cv.take<uint32_t>(1);
+    uint32_t* thr = cv.take<uint32_t>(1);
+    uint32_t* cnt = cv.take<uint32_t>(1);
+    uint64_t* skeys = cv.take<uint64_t>(p.cap);
+    uint64_t* qbits = cv.take<uint64_t>(qpv);
+    uint32_t* cand = cv.take<uint32_t>(m_eff);
+    void* rwork = cv.take<char>(rerank_work_bytes(1, (uint32_t)out_k, nsplit));
+    float* ds = dev ? out_scores : cv.take<float>(out_k);
+    int64_t* di = dev ? out_ids : cv.take<int64_t>(out_k);
+    const int tier = select_tier(d, ix->bits);
+    const float inv_norm = rankquant_inv_norm(d, ix->bits);
+    hipStream_t s = c.stream;
+    const PrepFuse prep{dq, qhat, off, nullptr, d, ix->bits, tier, inv_norm};
+    launch_stage1_plain(sb->words, sb->n, d, dq, p, hist, done_head, thr, cnt, skeys, qbits, qstat, prep, s);
+    launch_s1_tail(skeys, cnt, p.cap, m_eff, ix->codes, d, ix->bits, tier, inv_norm, qhat, off, ix->n, 1,
+                   (uint32_t)out_k, ds, di, nsplit, rwork, done_tail, qstat, hstat, s);
+    if (!dev) {
+        d2h(out_scores, ds, out_k * 4);
+        d2h(out_ids, di, out_k * 8);
+    }
+    sync();
+    const uint32_t st = __atomic_load_n(&hstat[0], __ATOMIC_ACQUIRE);
+    if (st == 0xffffffffu) fail(kInternal, "one-query two-stage path: the tail did not report a status");
+    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && (st & kStatNonfinite))
+        fail(kNonfiniteQuery, "queries contain NaN or infinity");
+    if (st & kStatFallback) {
+        cur().exact_queries += 1;
+        const std::vector<uint32_t> fl{0u};
+        stage1_fix(sb, qbits, fl, m_eff, cand, nullptr);
+        launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, 1, nullptr, cand, m_eff, ix->n, nullptr,
+                      (uint32_t)out_k, ds, di, 1, nullptr, s);
+        if (!dev) {
+            d2h(out_scores, ds, out_k * 4);
+            d2h(out_ids, di, out_k * 8);
+        }
+        sync();
+    }
+}
+
 void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, uint64_t nq, uint64_t m, uint64_t k,
                    float* out_scores, int64_t* out_ids, uint64_t flags) {
     if (sb->dim != ix->dim) fail(kBadArgument, "sign_probe.dim must match RankQuant dim " + std::to_string(ix->dim));
@@ -1095,6 +1189,10 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
     eng().init();
     if (m_eff > kStage1MaxM || out_k > kTopKMax) {
         two_stage_big(sb, ix, queries, nq, m_eff, out_k, out_scores, out_ids, flags);
+        return;
+    }
+    if (nq == 1 && sb->ntop == 0 && g_plain_path && stage1_plain_supported(sb->n, ix->dim, m_eff, out_k)) {
+        two_stage_plain(sb, ix, queries, m_eff, out_k, out_scores, out_ids, flags);
         return;
     }
     const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
@@ -2705,6 +2803,22 @@ ordvec_status_t ordvec_kernel_timing_enable_families(uint32_t mask) {
 
 ordvec_status_t ordvec_kernel_timing_enable(int enable) {
     return ordvec_kernel_timing_enable_families(enable ? ~0u : 0u);
+}
+
+ordvec_status_t ordvec_engine_option(uint32_t option, int64_t value, int64_t* previous) {
+    return boundary([&] {
+        if (value < -1 || value > 1) fail(kBadArgument, "option value must be -1 (read), 0 or 1");
+        int64_t prev = 0;
+        if (option == ORDVEC_OPTION_ONE_QUERY_PATH) {
+            prev = g_plain_path.load() ? 1 : 0;
+            if (value >= 0) g_plain_path.store(value != 0);
+        } else if (option == ORDVEC_OPTION_MFMA_SCAN) {
+            prev = set_mfma_scan((int)value) ? 1 : 0;
+        } else {
+            fail(kBadArgument, "unknown engine option " + std::to_string(option));
+        }
+        if (previous) *previous = prev;
+    });
 }
 
 ordvec_status_t ordvec_kernel_timing_read(uint32_t family, uint64_t* launches, double* total_ms) {

@@ -237,6 +237,25 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
                         uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s,
                         const PrepFuse* prep = nullptr);
 
+// One-query two-stage path: stage 1 in two launches (head: sign words,
+// sampled histogram and threshold; then the vector-ALU scan with the rerank's
+// query preparation beside it) and the tail (select + rerank + merge) in one.
+// hist ([nbins]) and done (one word for the head, nq for the tail) are device
+// words that are zero before and after each launch.  The tail writes each
+// query's final status word to qstat and, when hstat is not null, to that
+// host-visible array.
+bool stage1_plain_supported(uint64_t n, uint32_t d, uint64_t m_eff, uint64_t out_k);
+// Whether batched stage-1 scans use the matrix cores; value 0 / 1 sets it,
+// -1 only reads.  Returns the value before the call.
+bool set_mfma_scan(int value);
+void launch_stage1_plain(const uint64_t* words, uint64_t n, uint32_t d, const float* query, const FastPlan& p,
+                         uint32_t* hist, uint32_t* done, uint32_t* thr, uint32_t* cnt, uint64_t* keys,
+                         uint64_t* qbits, uint32_t* qstat, const PrepFuse& prep, hipStream_t s);
+void launch_s1_tail(const uint64_t* keys, const uint32_t* cnt, uint32_t cap, uint64_t m_eff, const uint8_t* codes,
+                    uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat, const float* offset,
+                    uint64_t n_docs, uint64_t nq, uint32_t out_k, float* out_s, int64_t* out_i, uint32_t nsplit,
+                    void* work, uint32_t* done, uint32_t* qstat, uint32_t* hstat, hipStream_t s);
+
 // Stage 2 / subset rerank over candidate rows.  Rows are CSR (offsets != null)
 // or uniform (row q = cands[q*row_m .. (q+1)*row_m)).  Output nq x out_k.
 // Small batches split each row across `nsplit` workgroups (partial top-k in

@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <mutex>
 
 #include <algorithm>
@@ -266,6 +268,31 @@ __device__ __forceinline__ float wave_minmax(float v) {
     const float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
     const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
     return op(op(a, b), op(c, d));
+}
+
+// Wave minimum of a 64-bit value by DPP (both halves moved with the same
+// control; no LDS round trips), then the four row results as scalars.
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t a) {
+    const uint32_t lo = (uint32_t)a, hi = (uint32_t)(a >> 32);
+    const uint32_t l2 = (uint32_t)__builtin_amdgcn_update_dpp((int)lo, (int)lo, CTRL, 0xF, 0xF, false);
+    const uint32_t h2 = (uint32_t)__builtin_amdgcn_update_dpp((int)hi, (int)hi, CTRL, 0xF, 0xF, false);
+    return ((uint64_t)h2 << 32) | l2;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+    v = min(v, dpp_u64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = min(v, dpp_u64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = min(v, dpp_u64<0x141>(v));  // row_half_mirror
+    v = min(v, dpp_u64<0x140>(v));  // row_mirror
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 16 * r);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 16 * r);
+        m = min(m, ((uint64_t)hi << 32) | lo);
+    }
+    return m;
 }
 
 // Exclusive prefix sum over the wave's lanes: in-row inclusive scan by DPP
@@ -1185,6 +1212,86 @@ __global__ __launch_bounds__(64) void k_s1_threshold(const uint32_t* __restrict_
     if (threadIdx.x == 0) thr[q] = t;
 }
 
+// Stage-1 head of a one-query sign-bitmap call: the query's sign words
+// (k_s1_prep: ballots of q > 0, non-finite check in qstat bit 0), the sampled
+// Hamming histogram (k_s1_sample's blocks and packed LDS counting), and, in
+// the workgroup that arrives last, the survivor threshold (k_s1_threshold's
+// rule).  That workgroup also leaves what the next call expects at zero: the
+// histogram `hist` and the arrival counter `done`, both zero when the kernel
+// starts; and it clears this call's survivor counter.  Dynamic LDS:
+// max(hwords, nbins) words.
+template <int WCH>
+__global__ __launch_bounds__(256) void k_s1_head(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
+                                                 uint64_t n, const float* __restrict__ q, uint32_t stride,
+                                                 uint32_t nsample, uint32_t blocks_per_seg, uint32_t* __restrict__ hist,
+                                                 uint32_t nbins, uint32_t hwords, uint32_t s_target,
+                                                 uint32_t* __restrict__ thr, uint32_t* __restrict__ cnt,
+                                                 uint64_t* __restrict__ qbits, uint32_t* __restrict__ qstat,
+                                                 uint32_t* __restrict__ done) {
+    extern __shared__ uint32_t lds_h[];
+    __shared__ uint64_t s_qb[kSignMaxDim / 64];
+    __shared__ int s_last;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    bool bad = false;
+    for (uint32_t w = wave; w < qpv; w += 4) {
+        const float v = q[w * 64 + lane];
+        bad |= !isfinite(v);
+        const uint64_t m = __ballot(v > 0.0f);
+        if (lane == 0) s_qb[w] = m;
+    }
+    for (uint32_t i = threadIdx.x; i < hwords; i += blockDim.x) lds_h[i] = 0;
+    const int any_bad = __syncthreads_or(bad);
+    if (blockIdx.x == 0) {
+        for (uint32_t w = threadIdx.x; w < qpv; w += blockDim.x) qbits[w] = s_qb[w];
+        if (threadIdx.x == 0) *qstat = any_bad ? 1u : 0u;
+    }
+    const uint32_t j0 = blockIdx.x * blocks_per_seg, j1 = min(nsample, j0 + blocks_per_seg);
+    for (uint32_t j = j0 + wave; j < j1; j += 4) {
+        const uint32_t b = j * stride;
+        if (b >= nblk) break;
+        const uint64_t* base = words + (uint64_t)b * qpv * 64 + lane;
+        uint32_t acc = 0;
+        if (WCH > 0) {
+            uint64_t dw[WCH > 0 ? WCH : 1];
+#pragma unroll
+            for (int jj = 0; jj < WCH; jj++) dw[jj] = base[(uint64_t)jj * 64];
+#pragma unroll
+            for (int jj = 0; jj < WCH; jj++) acc += (uint32_t)__popcll(dw[jj] ^ s_qb[jj]);
+        } else {
+            for (uint32_t w = 0; w < qpv; w++) acc += (uint32_t)__popcll(base[(uint64_t)w * 64] ^ s_qb[w]);
+        }
+        if ((uint64_t)b * 64 + lane < n) atomicAdd(&lds_h[acc >> 1], 1u << ((acc & 1) * 16));
+    }
+    __syncthreads();
+    // Cross-workgroup hand-off without device-scope fences (each would write
+    // back the XCD's L2): the histogram and the counter change only through
+    // device-scope atomics, and the barrier below waits for this
+    // workgroup's atomics to complete before its arrival is counted; the
+    // last workgroup then reads and clears them with device-scope atomics.
+    for (uint32_t bin = threadIdx.x; bin < nbins; bin += blockDim.x) {
+        const uint32_t w = lds_h[bin >> 1];
+        const uint32_t c = (bin & 1) ? (w >> 16) : (w & 0xffffu);
+        if (c) __hip_atomic_fetch_add(&hist[bin], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_last = __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    for (uint32_t bin = threadIdx.x; bin < nbins; bin += blockDim.x)
+        lds_h[bin] = __hip_atomic_exchange(&hist[bin], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t t = sample_threshold(lds_h, nbins, s_target, lane);
+        if (lane == 0) *thr = t;
+    }
+    if (threadIdx.x == 0) {
+        *cnt = 0;
+        __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Full scan: every document with hamming <= T[q] is appended (wave-aggregated
 // atomics) to the query's key buffer; counts past `cap` are counted, not stored.
 // Each wave holds one 64-doc block's sign words in VGPRs (WCH = words per
@@ -1774,7 +1881,8 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
 
 // Sort each query's survivors by (hamming, id) and keep the first m; flag the
 // query (qstat bit 1) when the buffer cannot prove exactness.  A histogram of
-// min(hamming, kSelBins-1) first narrows the sort to the keys at or below the
+// min(hamming - hmin, kSelBins-1) (hmin: the smallest survivor distance)
+// first narrows the sort to the keys at or below the
 // bin that reaches m (a superset of the answer that keeps every tie).
 constexpr uint32_t kSelBins = 2048, kSelBuf = 4096;
 
@@ -1787,7 +1895,7 @@ __global__ __launch_bounds__(1024) void k_s1_select(const uint64_t* __restrict__
     __shared__ uint32_t fill[kSelBins];
     __shared__ uint64_t sk[kSelBuf];           // selected keys grouped by bin
     __shared__ uint32_t wsum[16];
-    __shared__ uint32_t s_cut, s_big;
+    __shared__ uint32_t s_cut, s_big, s_hmin;
     const uint64_t q = blockIdx.x;
     const uint32_t count = cnt[q];
     if (count < m_eff || count > cap) {
@@ -1806,11 +1914,25 @@ __global__ __launch_bounds__(1024) void k_s1_select(const uint64_t* __restrict__
         hist[i] = 0;
         fill[i] = 0;
     }
-    if (threadIdx.x == 0) s_big = 0;
+    // bins count from the smallest survivor distance, so that wide rows
+    // (distances above kSelBins) keep distinct bins
+    uint64_t kmin = kKeyMax;
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++) kmin = min(kmin, kr[j]);
+    uint32_t hmin = (uint32_t)(kmin >> 32);
+    for (int off = 32; off >= 1; off >>= 1) hmin = min(hmin, (uint32_t)__shfl_xor((int)hmin, off));
+    if (threadIdx.x == 0) {
+        s_big = 0;
+        s_hmin = 0xffffffffu;
+    }
     __syncthreads();
+    if ((threadIdx.x & 63) == 0) atomicMin(&s_hmin, hmin);
+    __syncthreads();
+    const uint32_t hbase = count ? s_hmin : 0u;
+    auto bin_of = [&](uint64_t key) { return min((uint32_t)(key >> 32) - hbase, kSelBins - 1); };
 #pragma unroll
     for (uint32_t j = 0; j < KPT; j++)
-        if (kr[j] != kKeyMax) atomicAdd(&hist[min((uint32_t)(kr[j] >> 32), kSelBins - 1)], 1u);
+        if (kr[j] != kKeyMax) atomicAdd(&hist[bin_of(kr[j])], 1u);
     __syncthreads();
     // Exclusive bin starts (2 bins per thread): wave scans, then the 16 wave totals.
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lo = threadIdx.x * 2;
@@ -1847,7 +1969,7 @@ __global__ __launch_bounds__(1024) void k_s1_select(const uint64_t* __restrict__
     if (lo <= cut) atomicMax(&s_big, max(h0, lo + 1 <= cut ? h1 : 0u));
 #pragma unroll
     for (uint32_t j = 0; j < KPT; j++) {
-        const uint32_t bin = min((uint32_t)(kr[j] >> 32), kSelBins - 1);
+        const uint32_t bin = bin_of(kr[j]);
         if (kr[j] != kKeyMax && bin <= cut) sk[start[bin] + atomicAdd(&fill[bin], 1u)] = kr[j];
     }
     __syncthreads();
@@ -1857,7 +1979,7 @@ __global__ __launch_bounds__(1024) void k_s1_select(const uint64_t* __restrict__
         // below the clamp and ids are unique).
         for (uint32_t i = threadIdx.x; i < nsel; i += blockDim.x) {
             const uint64_t k = sk[i];
-            const uint32_t bin = min((uint32_t)(k >> 32), kSelBins - 1), b0 = start[bin], sz = hist[bin];
+            const uint32_t bin = bin_of(k), b0 = start[bin], sz = hist[bin];
             uint32_t rank = 0;
             for (uint32_t t = 0; t < sz; t++) rank += sk[b0 + t] < k ? 1u : 0u;
             const uint32_t pos = b0 + rank;
@@ -2251,6 +2373,367 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
     } else {
         uint64_t* pp = partial + (qidx * nsplit + split) * out_k;
         for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) pp[j] = tk.keys[j];
+    }
+}
+
+// Exclusive prefix sum over a 256-thread workgroup (wtot: 4 LDS words);
+// *total (optional) gets the sum.  Synchronises.
+__device__ uint32_t block_excl_scan256(uint32_t v, uint32_t* wtot, uint32_t* total = nullptr) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_excl_scan(v, lane) + v;
+    __syncthreads();  // wtot may still be read by an earlier use
+    if (lane == 63) wtot[wave] = incl;
+    __syncthreads();
+    uint32_t base = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; w++) {
+        const uint32_t t = wtot[w];
+        base += w < wave ? t : 0u;
+        all += t;
+    }
+    if (total) *total = all;
+    return base + incl - v;
+}
+
+// Wave-level selection helpers (one wave, no barriers).
+// The k smallest of cnt keys a[0..cnt) (cnt <= 64 * KW), ascending, into
+// out[0..k) (kKeyMax past the last key): k rounds of a wave minimum, one
+// occurrence removed per round.
+template <int KW>
+__device__ void wave_kmin(const uint64_t* a, uint32_t cnt, uint32_t k, uint64_t* out, uint32_t lane) {
+    uint64_t r[KW];
+#pragma unroll
+    for (int j = 0; j < KW; j++) {
+        const uint32_t i = lane + 64u * j;
+        r[j] = i < cnt ? a[i] : kKeyMax;
+    }
+    for (uint32_t t = 0; t < k; t++) {
+        uint64_t lm = r[0];
+#pragma unroll
+        for (int j = 1; j < KW; j++) lm = min(lm, r[j]);
+        const uint64_t m = wave_min_u64(lm);
+        const uint64_t own = __ballot(lm == m);
+        if ((int)lane == __ffsll((long long)own) - 1) {
+            bool done_one = false;
+#pragma unroll
+            for (int j = 0; j < KW; j++)
+                if (!done_one && r[j] == m) {
+                    r[j] = kKeyMax;
+                    done_one = true;
+                }
+        }
+        if (lane == 0) out[t] = m;
+    }
+}
+
+// The take-th smallest (1-based) of cnt <= 256 keys that share their upper
+// 32 bits (one hamming value): a select over the low (document id) bits, most
+// significant first, by wave ballots.
+__device__ uint64_t wave_select_id(const uint64_t* a, uint32_t cnt, uint32_t take, uint32_t nbits, uint32_t lane) {
+    uint32_t id[4];
+    bool act[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t i = lane + 64u * j;
+        act[j] = i < cnt;
+        id[j] = act[j] ? (uint32_t)(a[i] & 0xffffffffu) : 0u;
+    }
+    const uint64_t hi = a[0] & 0xffffffff00000000ull;
+    uint32_t r = take, prefix = 0;
+    for (int b = (int)nbits - 1; b >= 0; b--) {
+        uint32_t c0 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) c0 += (uint32_t)__popcll(__ballot(act[j] && !((id[j] >> b) & 1u)));
+        const uint32_t want = r <= c0 ? 0u : 1u;
+        if (want) {
+            r -= c0;
+            prefix |= 1u << b;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) act[j] = act[j] && ((id[j] >> b) & 1u) == want;
+    }
+    return hi | prefix;
+}
+
+// Stage-1 select, stage-2 rerank and the top-k merge of a small two-stage
+// call in one launch: grid (nsplit, nq), 256 threads.
+//  * select (k_s1_select's rule): every workgroup of query q reads the
+//    survivor keys (hamming << 32 | id) into registers, histograms them by
+//    min(hamming - hmin, kSelBins - 1) (hmin: the smallest survivor
+//    distance, so that wide rows keep distinct bins), finds the cut bin where
+//    the cumulative count reaches m_eff, and, when the cut bin holds more keys
+//    than the top-m needs, the largest key of the bin that is still inside
+//    (a wave select over the ids, or a sort of the bin's keys in LDS).  A key
+//    is a candidate iff its bin is below the cut, or it is in the cut bin and
+//    not above that key: the exact top-m by (hamming, id).
+//  * rerank (k_rerank's scoring): workgroup `split` scores the candidates
+//    held by threads t with t mod nsplit == split, and keeps their top-k.
+//  * merge (k_merge_topk): the workgroup that arrives last at query q's
+//    counter done[q] (zero at the start, reset by it) merges the nsplit
+//    partial top-k rows and writes the output row.
+// Small k (out_k <= kTailWaveK and at most 256 keys to choose from) is
+// chosen by one wave (wave_kmin); larger ones by the block top-k.  Queries
+// whose survivors cannot prove exactness (fewer than m_eff, more than cap,
+// or a cut bin above kTailTieCap keys) get qstat bit 1 and no output, as
+// from k_s1_select.  The last workgroup writes the query's final status word
+// to qstat and, when hstat is not null, to that host-visible word.  Partial
+// rows and counters pass between workgroups through device-scope atomics
+// (no device-scope fence, which would write back the XCD's L2).  Dynamic
+// LDS: s1_tail_lds(d) bytes (cut-bin keys, then q_hat).
+constexpr uint32_t kTailTieCap = 2048, kTailWaveK = 32;
+
+template <int TIER, int BITS>
+__global__ __launch_bounds__(256) void k_s1_tail(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ cnt,
+                                                 uint32_t cap, uint32_t m_eff, const uint8_t* __restrict__ codes,
+                                                 uint32_t d, float inv_norm, const float* __restrict__ qhat,
+                                                 const float* __restrict__ offset, uint64_t n_docs, uint32_t out_k,
+                                                 float* __restrict__ out_s, int64_t* __restrict__ out_i,
+                                                 uint64_t* __restrict__ partial, uint32_t* __restrict__ done,
+                                                 uint32_t* __restrict__ qstat, uint32_t* __restrict__ hstat) {
+    constexpr uint32_t KPT = kFastCap / 256;
+    __shared__ TopKShared tk;  // block top-k (large k); otherwise the scored keys
+    __shared__ uint32_t bh[kSelBins];
+    __shared__ uint32_t s_list[kStage1MaxM];
+    __shared__ uint64_t s_res[kTailWaveK];
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t s_cut, s_before, s_nlist, s_hmin;
+    __shared__ uint64_t s_cutkey;
+    __shared__ int s_last;
+    extern __shared__ uint64_t tail_dyn[];  // [kTailTieCap] cut-bin keys, then q_hat (d floats)
+    float* qs = reinterpret_cast<float*>(tail_dyn + kTailTieCap);
+    const uint32_t split = blockIdx.x, nsplit = gridDim.x;
+    const uint64_t q = blockIdx.y;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // One round trip to memory: the count, every slot of the survivor
+    // buffer (cap entries; those at or past the count are masked below) and
+    // q_hat are requested together.
+    const uint32_t count = cnt[q];
+    const uint64_t* kq = keys + q * cap;
+    uint64_t kr[KPT];
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++) kr[j] = kq[threadIdx.x + j * 256];
+    if (TIER == 2) stage_q_mix<BITS>(qs, qhat + q * d, d);
+    else stage_q(qs, qhat + q * d, d);
+    bool fb = count < m_eff || count > cap;
+    if (!fb) {
+#pragma unroll
+        for (uint32_t j = 0; j < KPT; j++)
+            if (threadIdx.x + j * 256 >= count) kr[j] = kKeyMax;
+        for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x) bh[i] = 0;
+        uint64_t kmin = kKeyMax;
+#pragma unroll
+        for (uint32_t j = 0; j < KPT; j++) kmin = min(kmin, kr[j]);
+        const uint32_t hmin = (uint32_t)(wave_min_u64(kmin) >> 32);
+        if (threadIdx.x == 0) {
+            s_nlist = 0;
+            s_cutkey = kKeyMax;
+            s_hmin = 0xffffffffu;
+        }
+        __syncthreads();
+        if (lane == 0) atomicMin(&s_hmin, hmin);
+        __syncthreads();
+        const uint32_t hbase = s_hmin;  // count >= m_eff >= 1: some key is valid
+        auto bin_of = [&](uint64_t key) { return min((uint32_t)(key >> 32) - hbase, kSelBins - 1); };
+#pragma unroll
+        for (uint32_t j = 0; j < KPT; j++)
+            if (kr[j] != kKeyMax) atomicAdd(&bh[bin_of(kr[j])], 1u);
+        __syncthreads();
+        // exclusive bin starts: 8 bins per thread, wave scans, then 4 wave totals
+        constexpr uint32_t BPT = kSelBins / 256;
+        uint32_t c[BPT], local = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < BPT; b++) {
+            c[b] = bh[threadIdx.x * BPT + b];
+            local += c[b];
+        }
+        const uint32_t incl = wave_excl_scan(local, lane) + local;
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t wbase = 0;
+        for (uint32_t w = 0; w < wave; w++) wbase += wsum[w];
+        uint32_t run = wbase + incl - local;
+        if (run < m_eff && run + local >= m_eff) {
+#pragma unroll
+            for (uint32_t b = 0; b < BPT; b++) {
+                if (run < m_eff && run + c[b] >= m_eff) {
+                    s_cut = threadIdx.x * BPT + b;
+                    s_before = run;
+                }
+                run += c[b];
+            }
+        }
+        __syncthreads();
+        const uint32_t cut = s_cut, take = m_eff - s_before, ncut = bh[cut];
+        if (take < ncut) {
+            // the cut bin is split: its keys to LDS, and the take-th smallest
+            if (ncut > kTailTieCap) {
+                fb = true;
+            } else {
+                // positions by a block scan of per-thread counts (no per-key atomics)
+                uint32_t mine = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < KPT; j++) mine += (kr[j] != kKeyMax && bin_of(kr[j]) == cut) ? 1u : 0u;
+                uint32_t pos = block_excl_scan256(mine, wsum);
+#pragma unroll
+                for (uint32_t j = 0; j < KPT; j++)
+                    if (kr[j] != kKeyMax && bin_of(kr[j]) == cut) tail_dyn[pos++] = kr[j];
+                __syncthreads();
+                if (cut < kSelBins - 1 && ncut <= 256) {
+                    // one hamming value: select on the ids
+                    if (wave == 0) {
+                        const uint32_t nbits = n_docs > 1 ? 64u - (uint32_t)__clzll((long long)(n_docs - 1)) : 1u;
+                        const uint64_t ck = wave_select_id(tail_dyn, ncut, take, min(nbits, 32u), lane);
+                        if (lane == 0) s_cutkey = ck;
+                    }
+                } else {
+                    const uint32_t P = next_pow2(ncut);
+                    for (uint32_t i = ncut + threadIdx.x; i < P; i += blockDim.x) tail_dyn[i] = kKeyMax;
+                    __syncthreads();
+                    bitonic_sort(tail_dyn, P);
+                    if (threadIdx.x == 0) s_cutkey = tail_dyn[take - 1];
+                }
+                __syncthreads();
+            }
+        }
+        if (!fb) {
+            const uint64_t cutkey = s_cutkey;
+            const bool take_mine = threadIdx.x % nsplit == split;
+            auto in = [&](uint64_t key) {
+                const uint32_t bn = bin_of(key);
+                return take_mine && key != kKeyMax && (bn < cut || (bn == cut && key <= cutkey));
+            };
+            uint32_t mine = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < KPT; j++) mine += in(kr[j]) ? 1u : 0u;
+            uint32_t total = 0;
+            uint32_t pos = block_excl_scan256(mine, wsum, &total);
+#pragma unroll
+            for (uint32_t j = 0; j < KPT; j++)
+                if (in(kr[j])) s_list[pos++] = (uint32_t)(kr[j] & 0xffffffffu);
+            if (threadIdx.x == 0) s_nlist = total;
+        }
+        __syncthreads();  // s_list and s_nlist complete (also orders qs)
+    }
+    const bool wave_k = out_k <= kTailWaveK;
+    uint64_t* sk = tk.keys;  // scored keys of this workgroup (wave-selected path)
+    if (!fb) {
+        const float off = offset[q];
+        const uint32_t bpv = d * BITS / 8;
+        const uint32_t nl = s_nlist;
+        const bool small = wave_k && nl <= 256;
+        if (!small) tk_init(tk, out_k);
+        if (TIER == 2) {
+            constexpr uint32_t NC = 2;
+            const uint32_t nquad = blockDim.x / 4, per_round = NC * nquad, quad = threadIdx.x >> 2, jl = threadIdx.x & 3;
+            for (uint32_t r0 = 0; r0 < nl; r0 += per_round) {
+                if (!small) tk_reserve(tk, out_k, per_round);
+                const uint8_t* rows[NC];
+                uint32_t id[NC];
+                bool ok[NC];
+#pragma unroll
+                for (uint32_t r = 0; r < NC; r++) {
+                    const uint32_t j = r0 + quad + r * nquad;
+                    id[r] = j < nl ? s_list[j] : 0u;
+                    ok[r] = j < nl && id[r] < n_docs;
+                    rows[r] = codes + (uint64_t)(ok[r] ? id[r] : 0u) * bpv;
+                }
+                float raw[NC];
+                raw_avx512_quad_n<BITS, NC>(rows, d, qs, jl, raw);
+#pragma unroll
+                for (uint32_t r = 0; r < NC; r++) {
+                    const uint32_t j = r0 + quad + r * nquad;
+                    const uint64_t key =
+                        ok[r] ? score_key(__fadd_rn(__fmul_rn(raw[r], inv_norm), off), id[r]) : kKeyMax;
+                    if (jl == 0 && j < nl) {
+                        if (small) sk[j] = key;
+                        else if (ok[r]) tk_offer(tk, out_k, key);
+                    }
+                }
+            }
+        } else {
+            for (uint32_t r0 = 0; r0 < nl; r0 += blockDim.x) {
+                if (!small) tk_reserve(tk, out_k, blockDim.x);
+                const uint32_t j = r0 + threadIdx.x;
+                const uint32_t id = j < nl ? s_list[j] : 0u;
+                if (j < nl) {
+                    const uint64_t key =
+                        id < n_docs
+                            ? score_key(doc_score<TIER, BITS>(codes + (uint64_t)id * bpv, d, qs, inv_norm, off), id)
+                            : kKeyMax;
+                    if (small) sk[j] = key;
+                    else if (id < n_docs) tk_offer(tk, out_k, key);
+                }
+            }
+        }
+        __syncthreads();
+        if (small) {
+            if (wave == 0) wave_kmin<4>(sk, nl, out_k, s_res, lane);
+            __syncthreads();
+        } else {
+            tk_flush(tk, out_k);
+        }
+        const uint64_t* best = small ? s_res : tk.keys;
+        if (nsplit > 1) {
+            // device-scope stores: written through to where the last
+            // workgroup, on any XCD, reads them
+            uint64_t* pp = partial + (q * nsplit + split) * out_k;
+            for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x)
+                __hip_atomic_store(&pp[j], best[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
+                const uint64_t key = best[j];
+                out_s[q * out_k + j] = key == kKeyMax ? -INFINITY : key_score(key);
+                out_i[q * out_k + j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
+            }
+        }
+    }
+    // arrival (after this workgroup's stores completed); the last workgroup
+    // of the query merges the partial rows
+    if (nsplit > 1) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            s_last = __hip_atomic_fetch_add(&done[q], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nsplit - 1;
+        __syncthreads();
+        if (!s_last) return;
+        if (!fb) {
+            const uint64_t* in = partial + q * nsplit * out_k;
+            const uint32_t total = nsplit * out_k;
+            if (wave_k && total <= 1024) {
+                for (uint32_t j = threadIdx.x; j < total; j += blockDim.x)
+                    sk[j] = __hip_atomic_load(&in[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __syncthreads();
+                if (wave == 0) {
+                    if (total <= 256) wave_kmin<4>(sk, total, out_k, s_res, lane);
+                    else wave_kmin<16>(sk, total, out_k, s_res, lane);
+                }
+                __syncthreads();
+            } else {
+                tk_init(tk, out_k);
+                for (uint32_t r0 = 0; r0 < total; r0 += blockDim.x) {
+                    tk_reserve(tk, out_k, blockDim.x);
+                    const uint32_t j = r0 + threadIdx.x;
+                    if (j < total) {
+                        const uint64_t key = __hip_atomic_load(&in[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (key != kKeyMax) tk_offer(tk, out_k, key);
+                    }
+                }
+                __syncthreads();
+                tk_flush(tk, out_k);
+            }
+            const uint64_t* best = (wave_k && total <= 1024) ? s_res : tk.keys;
+            for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
+                const uint64_t key = best[j];
+                out_s[q * out_k + j] = key == kKeyMax ? -INFINITY : key_score(key);
+                out_i[q * out_k + j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
+            }
+        }
+        if (threadIdx.x == 0) __hip_atomic_store(&done[q], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t st = qstat[q] | (fb ? 2u : 0u);
+        qstat[q] = st;
+        if (hstat) __hip_atomic_store(&hstat[q], st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -3390,7 +3873,12 @@ static void launch_sample(const uint64_t* words, uint64_t n, uint32_t qpv, const
 
 // The matrix-core scan pays off once a workgroup has a full 32-query tile;
 // its query fragments live in LDS (nqt x d bytes).
-static bool g_mfma_scan = true;
+static std::atomic<bool> g_mfma_scan{true};
+bool set_mfma_scan(int value) {
+    const bool prev = g_mfma_scan.load();
+    if (value >= 0) g_mfma_scan.store(value != 0);
+    return prev;
+}
 template <int NQT, int OP>
 static void set_mfma_lds() {
 #define OVX_SET(QPV)                                                                          \
@@ -3584,6 +4072,75 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
         else if (bits == 2) KERNEL(0, 2, __VA_ARGS__);                          \
         else KERNEL(0, 4, __VA_ARGS__);                                         \
     } while (0)
+
+// One-query sign-bitmap stage 1 (see k_s1_head): the head, then the
+// vector-ALU scan reading the head's threshold, with the rerank's query
+// preparation on extra workgroups.  hist and done are zero on entry and on
+// return (in stream order).
+void launch_stage1_plain(const uint64_t* words, uint64_t n, uint32_t d, const float* query, const FastPlan& p,
+                         uint32_t* hist, uint32_t* done, uint32_t* thr, uint32_t* cnt, uint64_t* keys,
+                         uint64_t* qbits, uint32_t* qstat, const PrepFuse& prep, hipStream_t s) {
+    const uint32_t qpv = d / 64, nblk = (uint32_t)((n + 63) / 64);
+    const bool sampled = p.s_target != 0xffffffffu;
+    const uint32_t grid = sampled ? std::max<uint32_t>(1, p.nsseg) : 1u;
+    const size_t lds = (size_t)std::max(p.hwords, p.nbins) * 4;
+    const uint32_t nsample = sampled ? p.nsample : 0u, bps = sampled ? p.sample_bps : 1u;
+    ktime_begin(kFamSample, s);
+#define OVX_HEAD(W)                                                                                                \
+    hipLaunchKernelGGL(k_s1_head<W>, dim3(grid), dim3(256), lds, s, words, nblk, qpv, n, query, p.stride, nsample, \
+                       bps, hist, p.nbins, p.hwords, p.s_target, thr, cnt, qbits, qstat, done)
+    if (qpv == 16) OVX_HEAD(16);
+    else if (qpv == 8) OVX_HEAD(8);
+    else OVX_HEAD(0);
+#undef OVX_HEAD
+    check_launch();
+    ktime_end(kFamSample, s);
+    const FastWork w{nullptr, thr, cnt, keys};
+    ktime_begin(kFamSignScan, s);
+    launch_scan<0>(words, n, qpv, qbits, 1, p, w, 0, s, &prep);
+    ktime_end(kFamSignScan, s);
+}
+
+bool stage1_plain_supported(uint64_t n, uint32_t d, uint64_t m_eff, uint64_t out_k) {
+    if (d % 64 || d > kSignMaxDim || m_eff < 1 || m_eff > kStage1MaxM || out_k < 1 || out_k > kTopKMax) return false;
+    const FastPlan p = plan_stage1_fast(n, d, 1, m_eff);
+    return p.cap == kFastCap && !use_mfma_scan(1, d / 64) && scan_fuses_prep(1, d / 64, p);
+}
+
+size_t s1_tail_lds(uint32_t d) { return (size_t)kTailTieCap * 8 + (size_t)d * 4; }
+
+void launch_s1_tail(const uint64_t* keys, const uint32_t* cnt, uint32_t cap, uint64_t m_eff, const uint8_t* codes,
+                    uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat, const float* offset,
+                    uint64_t n_docs, uint64_t nq, uint32_t out_k, float* out_s, int64_t* out_i, uint32_t nsplit,
+                    void* work, uint32_t* done, uint32_t* qstat, uint32_t* hstat, hipStream_t s) {
+    if (nq == 0) return;
+    if (nsplit < 1) nsplit = 1;
+    const size_t lds = s1_tail_lds(d);
+    static std::once_flag configured;
+    std::call_once(configured, [] {
+        const int bytes = (int)s1_tail_lds(kQInLds);
+#define OVX_TAIL_ATTR(T, B, _) \
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_s1_tail<T, B>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
+        OVX_TAIL_ATTR(2, 2, 0);
+        OVX_TAIL_ATTR(2, 4, 0);
+        OVX_TAIL_ATTR(1, 2, 0);
+        OVX_TAIL_ATTR(1, 4, 0);
+        OVX_TAIL_ATTR(0, 1, 0);
+        OVX_TAIL_ATTR(0, 2, 0);
+        OVX_TAIL_ATTR(0, 4, 0);
+#undef OVX_TAIL_ATTR
+        (void)hipGetLastError();
+    });
+    ktime_begin(kFamRerank, s);
+#define OVX_TAIL(T, B, _)                                                                                          \
+    hipLaunchKernelGGL((k_s1_tail<T, B>), dim3(nsplit, (uint32_t)nq), dim3(256), lds, s, keys, cnt, cap,          \
+                       (uint32_t)m_eff, codes, d, inv_norm, qhat, offset, n_docs, out_k, out_s, out_i,            \
+                       static_cast<uint64_t*>(work), done, qstat, hstat)
+    OVX_TIER_BITS_DISPATCH(OVX_TAIL, 0);
+#undef OVX_TAIL
+    check_launch();
+    ktime_end(kFamRerank, s);
+}
 
 uint32_t rerank_nsplit(uint64_t nq, uint64_t avg_m) {
     if (nq >= 128) return 1;
