@@ -223,9 +223,17 @@ ordvec_status_t ordvec_kernel_timing_read(uint32_t family, uint64_t *launches, d
  *    in three launches (stage-1 head, scan, select+rerank+merge tail); 0 runs
  *    it through the general batched pipeline.
  *  ORDVEC_OPTION_MFMA_SCAN: 1 (default) runs batched stage-1 scans of two or
- *    more queries on the matrix cores; 0 uses the vector-ALU scan. */
+ *    more queries on the matrix cores; 0 uses the vector-ALU scan.
+ *  ORDVEC_OPTION_BATCH_TAIL: 1 (default) runs the stage-1 select and the
+ *    rerank of a two-stage batch of 2 to 127 queries in one launch; 0 in
+ *    separate launches (larger batches always use separate launches).
+ *  ORDVEC_OPTION_SIDE_PREP: 1 (default) prepares the queries of a batch of
+ *    256 or more for the rerank on a second stream beside the stage-1 sample;
+ *    0 in line. */
 #define ORDVEC_OPTION_ONE_QUERY_PATH 1
 #define ORDVEC_OPTION_MFMA_SCAN 2
+#define ORDVEC_OPTION_BATCH_TAIL 3
+#define ORDVEC_OPTION_SIDE_PREP 4
 ordvec_status_t ordvec_engine_option(uint32_t option, int64_t value, int64_t *previous);
 
 /* RankQuant: src/quant.rs:438-491 new, 587-614 add, 713-847 search_asymmetric. */

@@ -287,7 +287,7 @@ KERNEL_FAMILIES = {"sign_scan": 0, "stage1_select": 1, "rerank": 2, "full_scan":
                    "encode": 5, "prepare": 6, "stage1_sample": 7, "backward": 8}
 
 
-ENGINE_OPTIONS = {"one_query_path": 1, "mfma_scan": 2}
+ENGINE_OPTIONS = {"one_query_path": 1, "mfma_scan": 2, "batch_tail": 3, "side_prep": 4}
 
 
 def engine_option(name: str, value=None) -> bool:

@@ -111,7 +111,28 @@ struct Ctx {
     uint32_t* zw = nullptr;
     size_t zw_count = 0;
 
+    // A second stream for work that runs beside the main one within a call
+    // (forked from and joined back into `stream`; default priority).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
     void* workspace(size_t bytes) { return ws.get(bytes, stream); }
+    hipStream_t fork() {
+        if (!side) {
+            OVX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+            OVX_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+            OVX_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        }
+        OVX_HIP(hipEventRecord(ev_fork, stream));
+        OVX_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+        return side;
+    }
+    // Marks the end of the side stream's work; the main stream waits for it
+    // where the returned event is waited on.
+    hipEvent_t join_event() {
+        OVX_HIP(hipEventRecord(ev_join, side));
+        return ev_join;
+    }
     uint32_t* zero_words(size_t count) {
         if (count > zw_count) {
             if (zw) {
@@ -153,6 +174,14 @@ struct Ctx {
         zw = nullptr;
         zw_count = 0;
         if (status_ev) (void)hipEventDestroy(status_ev);
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+            (void)hipEventDestroy(ev_fork);
+            (void)hipEventDestroy(ev_join);
+        }
+        side = nullptr;
+        ev_fork = ev_join = nullptr;
         status_ev = nullptr;
         ws.p = ws2.p = nullptr;
         ws.size = ws2.size = 0;
@@ -608,14 +637,15 @@ size_t stage1_bytes(const SbIndex* ix, uint64_t nq, uint64_t m_eff) {
 }
 
 Stage1Bufs stage1_dev(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t m_eff, uint32_t* cand,
-                      uint32_t* cand_h, uint32_t* qstat, Carver& cv, const PrepFuse* prep = nullptr) {
+                      uint32_t* cand_h, uint32_t* qstat, Carver& cv, const PrepFuse* prep = nullptr,
+                      bool select = true, hipEvent_t before_scan = nullptr) {
     Stage1Bufs b{};
     b.qbits = cv.take<uint64_t>(nq * ix->qpv);
     b.plan = plan_stage1_fast(ix->n, ix->dim, nq, m_eff);
     void* base = cv.take<char>(stage1_fast_bytes(b.plan, nq));
     carve_stage1_fast(b.plan, nq, base, &b.work);
     launch_stage1_fast(ix->words, ix->n, ix->dim, dq, nq, m_eff, b.plan, b.work, b.qbits, cand, cand_h, qstat,
-                       ix->ntop, cur().stream, prep);
+                       ix->ntop, cur().stream, prep, select, before_scan);
     return b;
 }
 
@@ -1107,8 +1137,51 @@ void two_stage_big(const SbIndex* sb, const RqIndex* ix, const float* queries, u
 
 // Two-stage search on the calling thread's context (src/sign_bitmap.rs:453-483
 // then src/quant.rs:1336-1394).  Outputs nq x min(k, n).
+// After a two-stage batch's status words are back: the non-finite error for
+// device queries, then the exact recomputation of the queries whose stage-1
+// survivors could not prove exactness (their rows rewritten in ds/di and,
+// for host outputs, copied back).
+void finish_two_stage(const SbIndex* sb, const RqIndex* ix, const uint64_t* qbits, const std::vector<uint32_t>& st,
+                      uint64_t nq, uint64_t m_eff, uint64_t out_k, int tier, float inv_norm, const float* qhat,
+                      const float* off, uint32_t* cand, uint32_t* qmap, float* ds, int64_t* di, float* out_scores,
+                      int64_t* out_ids, uint64_t flags) {
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
+        fail(kNonfiniteQuery, "queries contain NaN or infinity");
+    std::vector<uint32_t> fl = flagged(st, kStatFallback);
+    if (fl.empty()) return;
+    cur().exact_queries += fl.size();
+    stage1_fix(sb, qbits, fl, m_eff, cand, nullptr);
+    h2d(qmap, fl.data(), fl.size() * 4);
+    launch_rerank(ix->codes, ix->dim, ix->bits, tier, inv_norm, qhat, off, fl.size(), nullptr, cand, m_eff, ix->n,
+                  qmap, (uint32_t)out_k, ds, di, 1, nullptr, cur().stream);
+    if (!dev)
+        for (uint32_t q : fl) {
+            d2h(out_scores + (uint64_t)q * out_k, ds + (uint64_t)q * out_k, out_k * 4);
+            d2h(out_ids + (uint64_t)q * out_k, di + (uint64_t)q * out_k, out_k * 8);
+        }
+    sync();
+}
+
 // ORDVEC_OPTION_ONE_QUERY_PATH: one-query calls take two_stage_plain.
 std::atomic<bool> g_plain_path{true};
+// ORDVEC_OPTION_BATCH_TAIL: batches select and rerank in one launch.
+std::atomic<bool> g_batch_tail{true};
+// ORDVEC_OPTION_SIDE_PREP: batches prepare their queries on the side stream.
+std::atomic<bool> g_side_prep{true};
+
+// Batches of this many queries or more keep separate select and rerank
+// launches: the select then runs with 1024 threads per query, and the rerank
+// with every query's workgroup resident at once.
+constexpr uint64_t kTailMaxBatch = 128;
+
+// From this batch size the rerank's query preparation runs on the side
+// stream (smaller batches lose more to the fork and join than they overlap).
+constexpr uint64_t kSidePrepMin = 256;
+
+// Layout of a context's zero words: the head's histogram (up to 8193 bins),
+// the head's arrival counter, then one tail counter per query.
+constexpr size_t kZwHeadDone = 8200, kZwTailDone = 8208;
 
 // One query (the plain form): three launches (stage-1 head, scan, tail) and
 // one wait; the tail writes the status word straight to pinned host memory.
@@ -1121,10 +1194,10 @@ void two_stage_plain(const SbIndex* sb, const RqIndex* ix, const float* queries,
     const FastPlan p = plan_stage1_fast(sb->n, d, 1, m_eff);
     const uint32_t nsplit = rerank_nsplit(1, m_eff);
     Ctx& c = cur();
-    uint32_t* zw = c.zero_words((size_t)p.nbins + 64);
+    uint32_t* zw = c.zero_words(kZwTailDone + 1);
     uint32_t* hist = zw;
-    uint32_t* done_head = zw + p.nbins;
-    uint32_t* done_tail = done_head + 1;
+    uint32_t* done_head = zw + kZwHeadDone;
+    uint32_t* done_tail = zw + kZwTailDone;
     uint32_t* hstat = c.pinned_u32(1);
     hstat[0] = 0xffffffffu;  // overwritten by the tail
     const size_t bytes = al(d * 4) + 4 * al(4) + al((uint64_t)p.cap * 8) + al(qpv * 8) + al(m_eff * 4) +
@@ -1215,7 +1288,46 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
     const float inv_norm = rankquant_inv_norm(d, ix->bits);
     hipStream_t s = cur().stream;
     const PrepFuse prep{dq, qhat, off, nullptr, d, ix->bits, tier, inv_norm};
-    Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv, &prep);
+    // Large batches: the rerank's query preparation (sequential sums, one
+    // workgroup per query) runs on a second stream beside the stage-1 sample,
+    // and the scan waits for it (beside the scan itself it slows the scan more
+    // than it saves).
+    const bool side_prep = nq >= kSidePrepMin && sb->ntop == 0 && g_side_prep;
+    if (sb->ntop == 0 && g_batch_tail && nq < kTailMaxBatch && s1_tail_supported(sb->n, d, nq, m_eff, out_k)) {
+        // stage 1 without its select, then select + rerank (+ merge) in one
+        // launch that writes the status words straight to pinned host memory
+        hipEvent_t prepped = nullptr;
+        if (side_prep) {
+            launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, nullptr, cur().fork());
+            prepped = cur().join_event();
+        }
+        Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv, side_prep ? nullptr : &prep, false,
+                                  prepped);
+        Ctx& c = cur();
+        uint32_t* zw = c.zero_words(kZwTailDone + nq);
+        uint32_t* hst = c.pinned_u32(nq);
+        std::fill(hst, hst + nq, 0xffffffffu);
+        launch_s1_tail(b.work.keys, b.work.cnt, b.plan.cap, m_eff, ix->codes, d, ix->bits, tier, inv_norm, qhat, off,
+                       ix->n, nq, (uint32_t)out_k, ds, di, nsplit, rwork, zw + kZwTailDone, qstat, hst, s);
+        if (!dev) {
+            d2h(out_scores, ds, nq * out_k * 4);
+            d2h(out_ids, di, nq * out_k * 8);
+        }
+        sync();
+        std::vector<uint32_t> st(nq);
+        for (uint64_t q = 0; q < nq; q++) st[q] = __atomic_load_n(&hst[q], __ATOMIC_ACQUIRE);
+        if (std::find(st.begin(), st.end(), 0xffffffffu) != st.end())
+            fail(kInternal, "two-stage tail: a query has no status");
+        finish_two_stage(sb, ix, b.qbits, st, nq, m_eff, out_k, tier, inv_norm, qhat, off, cand, qmap, ds, di,
+                         out_scores, out_ids, flags);
+        return;
+    }
+    hipEvent_t prepped = nullptr;
+    if (side_prep) {
+        launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, nullptr, cur().fork());
+        prepped = cur().join_event();
+    }
+    Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv, side_prep ? nullptr : &prep, true, prepped);
     // Device pointers, batches of kEarlyStatusQueries or more: the status
     // words are final after stage 1, so the call waits for their copy only
     // and returns while the rerank runs (results complete in stream order,
@@ -1231,22 +1343,8 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
         d2h(out_ids, di, nq * out_k * 8);
     }
     std::vector<uint32_t> st = early ? qstat_wait(hst, nq) : read_qstat(qstat, nq);
-    if (dev && !(flags & ORDVEC_FLAG_ASSUME_FINITE) && any_bit(st, kStatNonfinite))
-        fail(kNonfiniteQuery, "queries contain NaN or infinity");
-    std::vector<uint32_t> fl = flagged(st, kStatFallback);
-    if (!fl.empty()) {
-        cur().exact_queries += fl.size();
-        stage1_fix(sb, b.qbits, fl, m_eff, cand, nullptr);
-        h2d(qmap, fl.data(), fl.size() * 4);
-        launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, fl.size(), nullptr, cand, m_eff, ix->n,
-                      qmap, (uint32_t)out_k, ds, di, 1, nullptr, s);
-        if (!dev)
-            for (uint32_t q : fl) {
-                d2h(out_scores + (uint64_t)q * out_k, ds + (uint64_t)q * out_k, out_k * 4);
-                d2h(out_ids + (uint64_t)q * out_k, di + (uint64_t)q * out_k, out_k * 8);
-            }
-        sync();
-    }
+    finish_two_stage(sb, ix, b.qbits, st, nq, m_eff, out_k, tier, inv_norm, qhat, off, cand, qmap, ds, di, out_scores,
+                     out_ids, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -2814,6 +2912,12 @@ ordvec_status_t ordvec_engine_option(uint32_t option, int64_t value, int64_t* pr
             if (value >= 0) g_plain_path.store(value != 0);
         } else if (option == ORDVEC_OPTION_MFMA_SCAN) {
             prev = set_mfma_scan((int)value) ? 1 : 0;
+        } else if (option == ORDVEC_OPTION_BATCH_TAIL) {
+            prev = g_batch_tail.load() ? 1 : 0;
+            if (value >= 0) g_batch_tail.store(value != 0);
+        } else if (option == ORDVEC_OPTION_SIDE_PREP) {
+            prev = g_side_prep.load() ? 1 : 0;
+            if (value >= 0) g_side_prep.store(value != 0);
         } else {
             fail(kBadArgument, "unknown engine option " + std::to_string(option));
         }

@@ -235,7 +235,7 @@ struct PrepFuse {
 void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
                         uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
                         uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s,
-                        const PrepFuse* prep = nullptr);
+                        const PrepFuse* prep = nullptr, bool select = true, hipEvent_t before_scan = nullptr);
 
 // One-query two-stage path: stage 1 in two launches (head: sign words,
 // sampled histogram and threshold; then the vector-ALU scan with the rerank's
@@ -245,6 +245,9 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
 // query's final status word to qstat and, when hstat is not null, to that
 // host-visible array.
 bool stage1_plain_supported(uint64_t n, uint32_t d, uint64_t m_eff, uint64_t out_k);
+// Whether a batch can take stage 1 without its select and the tail after it
+// (survivor buffers of kFastCap keys, m_eff and out_k within the tail's limits).
+bool s1_tail_supported(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff, uint64_t out_k);
 // Whether batched stage-1 scans use the matrix cores; value 0 / 1 sets it,
 // -1 only reads.  Returns the value before the call.
 bool set_mfma_scan(int value);

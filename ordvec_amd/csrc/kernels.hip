@@ -3357,7 +3357,8 @@ void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bi
     if (nq == 0) return;
     float centre = ((float)(1u << bits) - 1.0f) / 2.0f;
     ktime_begin(kFamPrepare, s);
-    hipLaunchKernelGGL(k_prepare, dim3((uint32_t)nq), dim3(256), 0, s, q, nq, d, tier, centre, inv_norm, qhat, offset, qstat);
+    hipLaunchKernelGGL(k_prepare, dim3((uint32_t)nq), dim3(256), 0, s, q, nq, d, tier, centre, inv_norm, qhat, offset,
+                       qstat);
     check_launch();
     ktime_end(kFamPrepare, s);
 }
@@ -4007,7 +4008,8 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
 
 void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
                         uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
-                        uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s, const PrepFuse* prep) {
+                        uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s, const PrepFuse* prep,
+                        bool select, hipEvent_t before_scan) {
     if (nq == 0) return;
     const uint32_t qpv = d / 64;
     bool s1_prepped = false;
@@ -4051,10 +4053,12 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
         hipLaunchKernelGGL(k_s1_threshold, dim3((uint32_t)nq), dim3(64), 0, s, w.hist, p.nbins, p.s_target, w.thr);
         check_launch();
     }
+    if (before_scan) OVX_HIP(hipStreamWaitEvent(s, before_scan, 0));
     ktime_begin(kFamSignScan, s);
     if (ntop) launch_scan<1>(words, n, qpv, qbits, nq, p, w, ntop, s, prep);
     else launch_scan<0>(words, n, qpv, qbits, nq, p, w, 0, s, prep);
     ktime_end(kFamSignScan, s);
+    if (!select) return;  // the caller's tail selects from the survivor keys
     ktime_begin(kFamStage1Select, s);
     hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(1024), 0, s, w.keys, w.cnt, p.cap, m_eff, cand, cand_h,
                        qstat);
@@ -4105,6 +4109,11 @@ bool stage1_plain_supported(uint64_t n, uint32_t d, uint64_t m_eff, uint64_t out
     if (d % 64 || d > kSignMaxDim || m_eff < 1 || m_eff > kStage1MaxM || out_k < 1 || out_k > kTopKMax) return false;
     const FastPlan p = plan_stage1_fast(n, d, 1, m_eff);
     return p.cap == kFastCap && !use_mfma_scan(1, d / 64) && scan_fuses_prep(1, d / 64, p);
+}
+
+bool s1_tail_supported(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff, uint64_t out_k) {
+    if (d > kQInLds || m_eff < 1 || m_eff > kStage1MaxM || out_k < 1 || out_k > kTopKMax || nq > 65535) return false;
+    return plan_stage1_fast(n, d, nq, m_eff).cap == kFastCap;
 }
 
 size_t s1_tail_lds(uint32_t d) { return (size_t)kTailTieCap * 8 + (size_t)d * 4; }

@@ -104,3 +104,48 @@ def test_one_query_path_nonfinite_and_option(engine, oracle):
     assert N.engine_option("mfma_scan") is True
     with pytest.raises(ValueError):
         N.check(N.lib().ordvec_engine_option(99, 1, None))
+
+
+BATCH_CASES = [
+    # (n, dim, bits, m, k, nq, data)
+    (120_000, 1024, 2, 1000, 10, 100, "mixture"),   # rows split over 5 workgroups, wave top-k
+    (60_000, 768, 4, 300, 100, 40, "iid"),          # split rows, block top-k (k > 32)
+    (40_000, 256, 1, 64, 5, 7, "iid"),
+    (30_000, 512, 2, 4096, 1024, 3, "mixture"),
+    (24_000, 1024, 2, 700, 10, 16, "dups"),
+    (8_000, 8192, 2, 500, 10, 5, "iid"),
+]
+
+
+@pytest.mark.parametrize("n,dim,bits,m,k,nq,data", BATCH_CASES)
+def test_batch_tail_matches_oracle_and_separate_launches(engine, oracle, n, dim, bits, m, k, nq, data):
+    """Batches whose stage-1 select and rerank run in one launch (the tail)
+    against the oracle and against the same batch with separate select and
+    rerank launches."""
+    import torch
+
+    if data == "mixture":
+        gen = S.MixtureNP(dim, seed=n % 89)
+        docs = gen.docs(n)
+        qs = gen.queries(docs, nq)[0]
+    elif data == "dups":
+        docs = np.repeat(S.iid_unit(n // 40, dim, seed=19), 40, axis=0)
+        qs = S.iid_unit(nq, dim, seed=20)
+    else:
+        docs = S.iid_unit(n, dim, seed=n + 3)
+        qs = S.iid_unit(nq, dim, seed=n + 4)
+    sb = engine.SignBitmap(dim)
+    rq = engine.RankQuant(dim, bits)
+    engine.two_stage_add(sb, rq, docs)
+    print(f"batch case n={n} dim={dim} bits={bits} m={m} k={k} nq={nq} {data}", flush=True)
+    ws, wi = _oracle(oracle, rq.codes(), sb.words(), qs, bits, m, k)
+    got = engine.two_stage_search(sb, rq, qs, m, k)
+    assert np.array_equal(got[1], wi)
+    assert np.array_equal(got[0].view(np.uint32), ws.view(np.uint32))
+    dev = engine.two_stage_search(sb, rq, torch.from_numpy(qs).cuda(), m, k)
+    _same((dev[0].cpu().numpy(), dev[1].cpu().numpy()), got)
+    prev = N.engine_option("batch_tail", 0)
+    try:
+        _same(engine.two_stage_search(sb, rq, qs, m, k), got)
+    finally:
+        N.engine_option("batch_tail", prev)
