@@ -1607,154 +1607,165 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     constexpr uint32_t kListCap = 256 * NQT;
     __shared__ uint64_t s_list[NWAVE * (kListCap + 64)];
     __shared__ uint32_t s_wcnt[NWAVE * QW];  // per-wave slot counters of the list drain
-    // Workgroups are dealt to the 8 XCDs round-robin by id; map ids so that
-    // every query tile of a document segment lands on the same XCD and the
-    // segment's codes are read from HBM once per XCD L2 rather than per tile.
-    uint32_t tile, seg;
-    {
-        const uint32_t f = blockIdx.x;
-        if (nsegs % 8 == 0) {
-            const uint32_t xcd = f % 8, k = f / 8;
-            seg = xcd + 8 * (k / ntiles);
-            tile = k % ntiles;
-        } else {
-            tile = f % ntiles;
-            seg = f / ntiles;
-        }
-    }
-    const uint32_t q0 = tile * QW;
-    uint32_t wbase = 0;  // entries in this wave's list (wave-uniform)
-    for (uint32_t e = threadIdx.x; e < NQT * QPV * 64; e += blockDim.x) {
-        const uint32_t ln = e & 63, w = (e >> 6) % QPV, t = (e >> 6) / QPV;
-        const uint32_t q = q0 + t * 32 + (ln & 31);
-        if (q < nq) {
-            const uint32_t bits = (uint32_t)(qbits[(uint64_t)q * QPV + w] >> (32 * (ln >> 5)));
-            if (OP) {
-                const ovx_v8i y = fp4_01(bits);
-                sB[e] = make_uint4((uint32_t)y[0], (uint32_t)y[1], (uint32_t)y[2], (uint32_t)y[3]);
-            } else {
-                sB[e] = fp4_pm(bits);
-            }
-        } else {
-            sB[e] = make_uint4(0, 0, 0, 0);
-        }
-    }
-    for (uint32_t j = threadIdx.x; j < QW; j += blockDim.x) {
-        const uint32_t q = q0 + j;
-        uint32_t pc = 0;
-        if (q < nq && !OP)
-            for (uint32_t w = 0; w < QPV; w++) pc += (uint32_t)__popcll(qbits[(uint64_t)q * QPV + w]);
-        s_base[j] = (int)(OP ? ntop : pc);
-        s_thr[j] = q < nq ? thr[q] : 0u;
-    }
-    __syncthreads();
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t col = lane & 31, g = lane >> 5;
+    // A workgroup takes (document segment, query tile) pairs p0, p0 + pstep,
+    // ... .  Workgroups are dealt to the 8 XCDs round-robin by id; with
+    // nsegs % 8 == 0 the pairs of XCD x are its segments x, x + 8, ... times
+    // every tile, so the tiles of a segment run on one XCD (its codes come
+    // from HBM once per XCD L2, not once per tile), and when the workgroups
+    // per XCD are a multiple of ntiles a workgroup keeps one tile, whose
+    // query fragments it expands into LDS once for all its segments.
+    const uint32_t f = blockIdx.x;
+    const bool grouped = nsegs % 8 == 0;
+    const uint32_t xcd = grouped ? f % 8 : 0u;
+    const uint32_t p0 = grouped ? f / 8 : f, pstep = grouped ? gridDim.x / 8 : gridDim.x;
+    const uint32_t pend = grouped ? ntiles * (nsegs / 8) : ntiles * nsegs;
+    uint32_t cur_tile = 0xffffffffu, q0 = 0;
+    uint32_t wbase = 0;  // entries in this wave's list (wave-uniform)
+    uint64_t* const wlist = s_list + wave * (kListCap + 64);
     // pass <=> dot >= base - T; columns past nq never pass.
     float need[NQT];
     int basev[NQT];
-#pragma unroll
-    for (int t = 0; t < NQT; t++) {
-        const uint32_t j = t * 32 + col;
-        basev[t] = s_base[j];
-        need[t] = q0 + j < nq ? (float)(basev[t] - (int)s_thr[j]) : 3.0e38f;
-    }
-    uint32_t* const cnt_col = cnt + q0 + col;
-    const uint32_t b_begin = seg * blocks_per_seg;
-    const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
     const uint32_t* words32 = reinterpret_cast<const uint32_t*>(words);
-    // dword g of documents b*64 + col (+32): word w at [w * 128] (+64).  The
-    // ring runs across blocks: the last RING words of a block prefetch the
-    // first RING words of the wave's next block.
-    uint32_t r0[RING], r1[RING];
-    if (b_begin + wave < b_end) {
-        const uint32_t* first = words32 + ((uint64_t)(b_begin + wave) * QPV * 64 + col) * 2 + g;
-#pragma unroll
-        for (int i = 0; i < RING; i++) {
-            r0[i] = first[i * 128];
-            r1[i] = first[i * 128 + 64];
-        }
-    }
-    for (uint32_t b = b_begin + wave; b < b_end; b += NWAVE) {
-        ovx_v16f acc[2][NQT];
-        // The query fragments do not depend on b; an opaque lane index keeps
-        // the compiler from hoisting all of them out of the loop (they would
-        // not fit in registers).
-        uint32_t slane = lane;
-        asm volatile("" : "+v"(slane));
-        const uint32_t* base = words32 + ((uint64_t)b * QPV * 64 + col) * 2 + g;
-        // the wave's next block, or (after its last block) this block again:
-        // an in-bounds load whose value is never used, which keeps the
-        // unrolled word loop free of branches
-        const uint32_t* next = b + NWAVE < b_end ? base + (uint64_t)NWAVE * QPV * 128 : base;
-#pragma unroll
-        for (int w0 = 0; w0 < QPV; w0 += RING) {
-#pragma unroll
-            for (int i = 0; i < RING; i++) {
-                const int w = w0 + i;
-                const uint32_t x0 = r0[i], x1 = r1[i];
-                const int wn = w + RING;
-                if (wn < QPV) {
-                    r0[i] = base[wn * 128];
-                    r1[i] = base[wn * 128 + 64];
+    for (uint32_t pr = p0; pr < pend; pr += pstep) {
+        const uint32_t tile = pr % ntiles;
+        const uint32_t seg = grouped ? xcd + 8 * (pr / ntiles) : pr / ntiles;
+        if (tile != cur_tile) {
+            if (cur_tile != 0xffffffffu) {
+                // the list's query slots belong to the old tile
+                s1_flush_list<QW>(wlist, wbase, lane, q0, n, cap, keys, cnt, s_wcnt + wave * QW);
+                wbase = 0;
+                __syncthreads();  // every wave is done with the old tile's fragments
+            }
+            cur_tile = tile;
+            q0 = tile * QW;
+            for (uint32_t e = threadIdx.x; e < NQT * QPV * 64; e += blockDim.x) {
+                const uint32_t ln = e & 63, w = (e >> 6) % QPV, t = (e >> 6) / QPV;
+                const uint32_t q = q0 + t * 32 + (ln & 31);
+                if (q < nq) {
+                    const uint32_t bits = (uint32_t)(qbits[(uint64_t)q * QPV + w] >> (32 * (ln >> 5)));
+                    if (OP) {
+                        const ovx_v8i y = fp4_01(bits);
+                        sB[e] = make_uint4((uint32_t)y[0], (uint32_t)y[1], (uint32_t)y[2], (uint32_t)y[3]);
+                    } else {
+                        sB[e] = fp4_pm(bits);
+                    }
                 } else {
-                    r0[i] = next[(wn - QPV) * 128];
-                    r1[i] = next[(wn - QPV) * 128 + 64];
+                    sB[e] = make_uint4(0, 0, 0, 0);
                 }
-                if (w == 0) mfma_word<NQT, QPV, true>(x0, x1, 0, slane, sB, acc);
-                else mfma_word<NQT, QPV, false>(x0, x1, w, slane, sB, acc);
-                __builtin_amdgcn_sched_barrier(0);
+            }
+            for (uint32_t j = threadIdx.x; j < QW; j += blockDim.x) {
+                const uint32_t q = q0 + j;
+                uint32_t pc = 0;
+                if (q < nq && !OP)
+                    for (uint32_t w = 0; w < QPV; w++) pc += (uint32_t)__popcll(qbits[(uint64_t)q * QPV + w]);
+                s_base[j] = (int)(OP ? ntop : pc);
+                s_thr[j] = q < nq ? thr[q] : 0u;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < NQT; t++) {
+                const uint32_t j = t * 32 + col;
+                basev[t] = s_base[j];
+                need[t] = q0 + j < nq ? (float)(basev[t] - (int)s_thr[j]) : 3.0e38f;
             }
         }
-        // Every accumulator is consumed here (an empty asm reading one element
-        // of each), so the compiler cannot sink MFMA chains into the
-        // conditional code below: MFMAs read every lane's operands and must
-        // run with the full exec mask.
+        uint32_t* const cnt_col = cnt + q0 + col;
+        const uint32_t b_begin = seg * blocks_per_seg;
+        const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
+        // dword g of documents b*64 + col (+32): word w at [w * 128] (+64).  The
+        // ring runs across blocks: the last RING words of a block prefetch the
+        // first RING words of the wave's next block.
+        uint32_t r0[RING], r1[RING];
+        if (b_begin + wave < b_end) {
+            const uint32_t* first = words32 + ((uint64_t)(b_begin + wave) * QPV * 64 + col) * 2 + g;
 #pragma unroll
-        for (int t = 0; t < NQT; t++) asm volatile("" ::"v"(acc[0][t][0]), "v"(acc[1][t][0]));
-        // Survivors: for each 16-slot row group, 16 compares give the wave
-        // masks (scalar registers); each non-empty mask appends (query, key)
-        // entries to this wave's LDS list at ballot offsets.  The append has no
-        // branch or exec change inside: lanes that do not pass, and entries
-        // past the list's capacity, write a spare slot of their own.  Rows past
-        // n (last block only) are dropped when the list is drained.
-        const uint32_t dbase = (uint32_t)b * 64 + 4 * g;
-        uint64_t* const wlist = s_list + wave * (kListCap + 64);
+            for (int i = 0; i < RING; i++) {
+                r0[i] = first[i * 128];
+                r1[i] = first[i * 128 + 64];
+            }
+        }
+        for (uint32_t b = b_begin + wave; b < b_end; b += NWAVE) {
+            ovx_v16f acc[2][NQT];
+            // The query fragments do not depend on b; an opaque lane index keeps
+            // the compiler from hoisting all of them out of the loop (they would
+            // not fit in registers).
+            uint32_t slane = lane;
+            asm volatile("" : "+v"(slane));
+            const uint32_t* base = words32 + ((uint64_t)b * QPV * 64 + col) * 2 + g;
+            // the wave's next block, or (after its last block) this block again:
+            // an in-bounds load whose value is never used, which keeps the
+            // unrolled word loop free of branches
+            const uint32_t* next = b + NWAVE < b_end ? base + (uint64_t)NWAVE * QPV * 128 : base;
 #pragma unroll
-        for (int t = 0; t < NQT; t++) {
-            const uint32_t hbase = ((uint32_t)(t * 32 + col) << 24) | (uint32_t)basev[t];
+            for (int w0 = 0; w0 < QPV; w0 += RING) {
 #pragma unroll
-            for (int G = 0; G < 2; G++) {
-                uint64_t m[16];
+                for (int i = 0; i < RING; i++) {
+                    const int w = w0 + i;
+                    const uint32_t x0 = r0[i], x1 = r1[i];
+                    const int wn = w + RING;
+                    if (wn < QPV) {
+                        r0[i] = base[wn * 128];
+                        r1[i] = base[wn * 128 + 64];
+                    } else {
+                        r0[i] = next[(wn - QPV) * 128];
+                        r1[i] = next[(wn - QPV) * 128 + 64];
+                    }
+                    if (w == 0) mfma_word<NQT, QPV, true>(x0, x1, 0, slane, sB, acc);
+                    else mfma_word<NQT, QPV, false>(x0, x1, w, slane, sB, acc);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            // Every accumulator is consumed here (an empty asm reading one element
+            // of each), so the compiler cannot sink MFMA chains into the
+            // conditional code below: MFMAs read every lane's operands and must
+            // run with the full exec mask.
 #pragma unroll
-                for (int v = 0; v < 16; v++) m[v] = __ballot(acc[G][t][v] >= need[t]);
+            for (int t = 0; t < NQT; t++) asm volatile("" ::"v"(acc[0][t][0]), "v"(acc[1][t][0]));
+            // Survivors: for each 16-slot row group, 16 compares give the wave
+            // masks (scalar registers); each non-empty mask appends (query, key)
+            // entries to this wave's LDS list at ballot offsets.  The append has no
+            // branch or exec change inside: lanes that do not pass, and entries
+            // past the list's capacity, write a spare slot of their own.  Rows past
+            // n (last block only) are dropped when the list is drained.
+            const uint32_t dbase = (uint32_t)b * 64 + 4 * g;
 #pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    if (__builtin_expect(m[v] != 0, 0)) {  // out of line: the common path falls through
-                        const uint32_t off =
-                            __builtin_amdgcn_mbcnt_hi((uint32_t)(m[v] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[v], 0u));
-                        const bool p = acc[G][t][v] >= need[t];
-                        const uint32_t slot = min(p ? wbase + off : 0xffffffffu, kListCap + lane);
-                        // (query slot << 24 | base) - dot = query slot << 24 | hamming
-                        const uint32_t hi = hbase - (uint32_t)(int)acc[G][t][v];
-                        wlist[slot] = ((uint64_t)hi << 32) | (dbase + (G * 32 + 8 * (v / 4) + (v % 4)));
-                        wbase += (uint32_t)__popcll(m[v]);
+            for (int t = 0; t < NQT; t++) {
+                const uint32_t hbase = ((uint32_t)(t * 32 + col) << 24) | (uint32_t)basev[t];
+#pragma unroll
+                for (int G = 0; G < 2; G++) {
+                    uint64_t m[16];
+#pragma unroll
+                    for (int v = 0; v < 16; v++) m[v] = __ballot(acc[G][t][v] >= need[t]);
+#pragma unroll
+                    for (int v = 0; v < 16; v++) {
+                        if (__builtin_expect(m[v] != 0, 0)) {  // out of line: the common path falls through
+                            const uint32_t off = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[v] >> 32),
+                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m[v], 0u));
+                            const bool p = acc[G][t][v] >= need[t];
+                            const uint32_t slot = min(p ? wbase + off : 0xffffffffu, kListCap + lane);
+                            // (query slot << 24 | base) - dot = query slot << 24 | hamming
+                            const uint32_t hi = hbase - (uint32_t)(int)acc[G][t][v];
+                            wlist[slot] = ((uint64_t)hi << 32) | (dbase + (G * 32 + 8 * (v / 4) + (v % 4)));
+                            wbase += (uint32_t)__popcll(m[v]);
+                        }
                     }
                 }
             }
-        }
-        if (wbase > kListCap) {  // entries were dropped: send every query of the tile to the exact path
+            if (wbase > kListCap) {  // entries were dropped: send every query of the tile to the exact path
 #pragma unroll
-            for (int t = 0; t < NQT; t++)
-                if (q0 + t * 32 + col < nq) atomicAdd(cnt_col + t * 32, cap + 1);
-            wbase = kListCap;
-        }
-        if (wbase >= kListCap / 2) {
-            s1_flush_list<QW>(wlist, wbase, lane, q0, n, cap, keys, cnt, s_wcnt + wave * QW);
-            wbase = 0;
+                for (int t = 0; t < NQT; t++)
+                    if (q0 + t * 32 + col < nq) atomicAdd(cnt_col + t * 32, cap + 1);
+                wbase = kListCap;
+            }
+            if (wbase >= kListCap / 2) {
+                s1_flush_list<QW>(wlist, wbase, lane, q0, n, cap, keys, cnt, s_wcnt + wave * QW);
+                wbase = 0;
+            }
         }
     }
-    s1_flush_list<QW>(s_list + wave * (kListCap + 64), wbase, lane, q0, n, cap, keys, cnt, s_wcnt + wave * QW);
+    if (cur_tile != 0xffffffffu) s1_flush_list<QW>(wlist, wbase, lane, q0, n, cap, keys, cnt, s_wcnt + wave * QW);
 }
 
 // Sampled Hamming histogram on the matrix cores (the k_s1_sample contract):
@@ -3957,9 +3968,19 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
         const uint32_t bps = (uint32_t)((nblk + nseg - 1) / nseg);
         const uint32_t segs = (nblk + bps - 1) / bps;
         const size_t lds = (size_t)nqt * qpv * 64 * 16;
+        // Persistent pairs: with grouped segments, the workgroups per XCD are a
+        // multiple of the tile count (each keeps one tile), at most the XCD's
+        // resident capacity (32 CUs x workgroups per CU).
+        uint32_t grid = tiles_m * segs;
+        if (segs % 8 == 0) {
+            const uint32_t per_cu = nqt == 4 ? 1u : 2u;
+            const uint32_t c = std::max<uint32_t>(1, 32 * per_cu / tiles_m);
+            const uint32_t pairs = tiles_m * (segs / 8);
+            grid = 8 * std::min<uint32_t>(pairs, tiles_m * c);
+        }
 #define OVX_MFMA(NQT, QPV)                                                                                         \
-    hipLaunchKernelGGL((k_s1_scan_mfma<NQT, OP, QPV>), dim3(tiles_m * segs), dim3(kMfmaThreads), lds, s, words,   \
-                       nblk, n, qbits, (uint32_t)nq, bps, w.thr, p.cap, w.keys, w.cnt, ntop, tiles_m, segs)
+    hipLaunchKernelGGL((k_s1_scan_mfma<NQT, OP, QPV>), dim3(grid), dim3(kMfmaThreads), lds, s, words, nblk, n,     \
+                       qbits, (uint32_t)nq, bps, w.thr, p.cap, w.keys, w.cnt, ntop, tiles_m, segs)
 #define OVX_MFMA_Q(NQT)                        \
     switch (qpv) {                             \
         case 16: OVX_MFMA(NQT, 16); break;     \
