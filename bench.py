@@ -76,6 +76,13 @@ def main():
                     help="ingest run instead: 1M x 1024 rows encoded into a SignBitmap + RankQuant b=2 in chunks")
     ap.add_argument("--ref-shapes", action="store_true",
                     help="every index form at the reference's rank-modes shape (30k x 256, 200 queries, k=10)")
+    ap.add_argument("--shapes", type=str, default="",
+                    help="two-stage batch sizes to time one by one (comma separated) on a --shape-docs index: one "
+                         "JSON line per batch size with per-kernel times and HBM rates (the launch shapes profiled "
+                         "with rocprofv3)")
+    ap.add_argument("--shape-docs", type=int, default=20_000_000)
+    ap.add_argument("--shape-calls", type=int, default=0, help="timed calls per batch size (0: 100 for one query, "
+                                                              "else 10)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -104,6 +111,8 @@ def main():
         return multi_main(args, dev)
     if args.ref_shapes:
         return ref_shapes_main(args, dev)
+    if args.shapes:
+        return shapes_main(args, dev)
     d, n, B, m, k = args.dim, args.n_docs, args.batch, args.m, args.k
 
     # ---- build the resident index ------------------------------------------------
@@ -332,6 +341,82 @@ def cpu_baseline(args, rq, sb, qbatch, gpu_out, d, m, k, B):
         f"(stage 1 {single['stage1_us_per_query']} us, rerank {single['rerank_us_per_query']} us per query); "
         f"parity {parity}")
     return cpu, parity
+
+
+def shapes_main(args, dev):
+    """Two-stage search (sign top-m, RankQuant b=2 rerank) at each batch size
+    of --shapes on an index of --shape-docs mixture docs (seed 4; 20M x 1024
+    by default: 2.56 GB of sign words and 5.12 GB of codes, above the 256 MB
+    MALL).  Per batch size: wall time per call, per-kernel-family average GPU
+    time (hipEvents, from a second pass), and the algorithmic HBM rate of the
+    stage-1 scan (corpus sign words + query words per launch) and of the
+    rerank (B x m code rows + q_hat)."""
+    import numpy as np
+    import torch
+
+    import ordvec_amd as ov
+    from ordvec_amd import _native as N
+    from ordvec_amd.synthetic import MixtureTorch
+
+    d, n, m, k = args.dim, args.shape_docs, args.m, args.k
+    shapes = [int(x) for x in args.shapes.split(",") if x]
+    t0 = time.time()
+    gen = MixtureTorch(d, seed=4, device=dev)
+    sb = ov.SignBitmap(d)
+    rq = ov.RankQuant(d, 2)
+    chunk = 262144
+    need_q = 3 * max(shapes) + 200
+    pool = []
+    per_chunk = max(1, need_q * chunk // n + 1)
+    for s0 in range(0, n, chunk):
+        c = gen.docs(min(chunk, n - s0))
+        ov.two_stage_add(sb, rq, c)
+        q, _ = gen.queries(c, per_chunk)
+        pool.append(q)
+        if (s0 // chunk) % 16 == 0:
+            torch.cuda.synchronize()
+            log(f"  built {s0 + c.shape[0]:,} docs ({time.time() - t0:.0f}s)")
+        del c
+    torch.cuda.synchronize()
+    pool = torch.cat(pool)[:need_q].contiguous()
+    log(f"index built: {n:,} docs x {d} in {time.time() - t0:.1f}s")
+    scan_bytes = lambda B: n * d // 8 + B * d // 8
+    rerank_bytes = lambda B: B * (m * d * 2 // 8 + d * 4)
+    for B in shapes:
+        calls = args.shape_calls or (100 if B == 1 else 10)
+        qs = [pool[(i * B) % (pool.shape[0] - B):][:B] for i in range(calls + 2)]
+        for i in range(2):
+            ov.two_stage_search(sb, rq, qs[i], m, k)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for i in range(calls):
+            ov.two_stage_search(sb, rq, qs[2 + i], m, k)
+            if B == 1:
+                torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t) / calls
+        N.kernel_timing_enable(True)
+        for i in range(calls):
+            ov.two_stage_search(sb, rq, qs[2 + i], m, k)
+        torch.cuda.synchronize()
+        fam = {f: N.kernel_timing_read(f) for f in N.KERNEL_FAMILIES}
+        N.kernel_timing_enable(False)
+        avg = {f: ms / c * 1e3 for f, (c, ms) in fam.items() if c}
+        out = {"shape": "two-stage", "n_docs": n, "dim": d, "batch": B, "m": m, "k": k, "calls": calls,
+               "ms_per_call": round(wall * 1e3, 4), "qps": round(B / wall, 1),
+               "kernel_avg_us": {f: round(v, 2) for f, v in avg.items()}}
+        if "sign_scan" in avg:
+            g = scan_bytes(B) / (avg["sign_scan"] * 1e-6) / 1e9
+            out["scan"] = {"algorithmic_bytes": scan_bytes(B), "gbs": round(g, 1), "hbm_frac": round(g / HBM_PEAK_GBS, 4)}
+            if B > 1:
+                t_ops = 2.0 * d * n * B / (avg["sign_scan"] * 1e-6) / 1e12
+                out["scan"]["fp4_tflops"] = round(t_ops, 1)
+                out["scan"]["mfma_frac"] = round(t_ops / MFMA_FP4_PEAK_TFLOPS, 4)
+        if "rerank" in avg:
+            g = rerank_bytes(B) / (avg["rerank"] * 1e-6) / 1e9
+            out["rerank"] = {"algorithmic_bytes": rerank_bytes(B), "gbs": round(g, 1),
+                             "hbm_frac": round(g / HBM_PEAK_GBS, 4)}
+        print(json.dumps(out), flush=True)
 
 
 def serve_main(args, dev):
