@@ -551,7 +551,11 @@ typedef struct {
     double p99_queue_ms;           /* arrival -> its batch starts */
     double p99_service_ms;         /* batch start -> results copied back */
     uint64_t exact_path_queries;   /* queries whose stage 1 was recomputed exactly */
-    uint64_t reserved[5];
+    uint64_t pool_bytes;           /* device memory the server lends to batches (its budget) */
+    uint64_t peak_pool_bytes;      /* most of it held by batches at one time during the run */
+    uint64_t overlapped_batches;   /* batches started while another was still on the GPU */
+    uint64_t max_batches_in_flight;
+    uint64_t workspace_overflows;  /* batch calls that needed more than their share (allocated) */
 } ordvec_serve_report_t;
 
 ordvec_status_t ordvec_server_start(const ordvec_sign_bitmap_t *sign, const ordvec_rankquant_t *rq, uint64_t m,

@@ -109,7 +109,11 @@ class ServeReport(C.Structure):
         ("p99_queue_ms", C.c_double),
         ("p99_service_ms", C.c_double),
         ("exact_path_queries", C.c_uint64),
-        ("reserved", C.c_uint64 * 5),
+        ("pool_bytes", C.c_uint64),
+        ("peak_pool_bytes", C.c_uint64),
+        ("overlapped_batches", C.c_uint64),
+        ("max_batches_in_flight", C.c_uint64),
+        ("workspace_overflows", C.c_uint64),
     ]
 
 

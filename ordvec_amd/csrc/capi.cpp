@@ -116,7 +116,19 @@ struct Ctx {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
-    void* workspace(size_t bytes) { return ws.get(bytes, stream); }
+    // A region lent to this context for one call (a serving batch's share of
+    // the server's pool); calls that need more fall back to `ws`.
+    char* lent = nullptr;
+    size_t lent_size = 0;
+    uint64_t lent_overflows = 0;
+
+    void* workspace(size_t bytes) {
+        if (lent) {
+            if (bytes <= lent_size) return lent;
+            lent_overflows++;
+        }
+        return ws.get(bytes, stream);
+    }
     hipStream_t fork() {
         if (!side) {
             OVX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
@@ -1517,6 +1529,65 @@ struct ServeSlot {
     int64_t* i_stage = nullptr;
 };
 
+// The server's device memory: one allocation of the memory budget, lent
+// out in 256-byte-aligned regions (first fit over a sorted free list) to
+// batches in flight and taken back when they finish.
+struct ServePool {
+    char* base = nullptr;
+    size_t size = 0, in_use = 0, peak = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    void init(size_t bytes) {
+        if (hipMalloc(&base, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            base = nullptr;
+            fail(kInternal, "device allocation of the serving pool (" + std::to_string(bytes) + " bytes) failed");
+        }
+        size = bytes;
+        free_[0] = bytes;
+    }
+    size_t largest() const {
+        size_t b = 0;
+        for (auto& f : free_) b = std::max(b, f.second);
+        return b;
+    }
+    char* take(size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, len = it->second;
+            free_.erase(it);
+            if (len > bytes) free_[off + bytes] = len - bytes;
+            in_use += bytes;
+            peak = std::max(peak, in_use);
+            return base + off;
+        }
+        return nullptr;
+    }
+    void give(char* p, size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        size_t off = (size_t)(p - base);
+        in_use -= bytes;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += bytes;
+                return;
+            }
+        }
+        free_[off] = bytes;
+    }
+    void release() {
+        if (base) (void)hipFree(base);
+        base = nullptr;
+        free_.clear();
+    }
+};
+
 struct Server {
     const SbIndex* sb = nullptr;
     const RqIndex* rq = nullptr;
@@ -1529,6 +1600,12 @@ struct Server {
     uint64_t next_ticket = 1;
     bool stopping = false;
     uint64_t batches = 0, batched_queries = 0;
+    // Admission by device memory: a batch of q queries holds ws_need[q] bytes
+    // of the pool while it runs; a free slot takes pending requests in order
+    // while the batch's need fits the largest free region.
+    ServePool pool;
+    std::vector<size_t> ws_need;  // [0, max_batch]
+    uint64_t in_flight = 0, max_in_flight = 0, overlapped = 0;
 
     void worker(ServeSlot* slot) {
         CtxScope scope(&slot->ctx);
@@ -1536,21 +1613,36 @@ struct Server {
         for (;;) {
             std::vector<ServeRequest*> batch;
             uint64_t q = 0, kmax = 0;
+            char* region = nullptr;
+            size_t region_bytes = 0;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return stopping || !pending.empty(); });
-                if (pending.empty()) return;  // stopping and drained
-                while (!pending.empty()) {
-                    ServeRequest* r = pending.front();
-                    if (!batch.empty() && q + r->nq > max_batch) break;
-                    batch.push_back(r);
-                    pending.pop_front();
-                    q += r->nq;
-                    kmax = std::max(kmax, r->k);
+                for (;;) {
+                    cv_work.wait(lk, [&] { return stopping || !pending.empty(); });
+                    if (pending.empty()) return;  // stopping and drained
+                    const size_t room = pool.largest();
+                    while (!pending.empty()) {
+                        ServeRequest* r = pending.front();
+                        if (q + r->nq > max_batch || ws_need[q + r->nq] > room) break;
+                        batch.push_back(r);
+                        pending.pop_front();
+                        q += r->nq;
+                        kmax = std::max(kmax, r->k);
+                    }
+                    if (!batch.empty()) break;
+                    // the pool is held by batches in flight: wait for one to end
+                    cv_work.wait(lk);
                 }
+                region_bytes = ws_need[q];
+                region = pool.take(region_bytes);
                 batches++;
                 batched_queries += q;
+                if (in_flight > 0) overlapped++;
+                in_flight++;
+                max_in_flight = std::max(max_in_flight, in_flight);
             }
+            slot->ctx.lent = region;
+            slot->ctx.lent_size = region_bytes;
             uint32_t status = 0;
             std::string err;
             const uint64_t t_start = now_ns();
@@ -1581,6 +1673,8 @@ struct Server {
                 err = e.what();
             }
             const uint64_t t = now_ns();
+            slot->ctx.lent = nullptr;
+            slot->ctx.lent_size = 0;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 for (ServeRequest* r : batch) {
@@ -1591,8 +1685,11 @@ struct Server {
                     r->batch_queries = q;
                     r->done = true;
                 }
+                if (region) pool.give(region, region_bytes);  // the batch's work has completed
+                in_flight--;
             }
             cv_done.notify_all();
+            cv_work.notify_all();  // memory came back: waiting slots may admit now
         }
     }
 
@@ -1624,6 +1721,7 @@ struct Server {
             if (sl->ctx.stream) (void)hipStreamDestroy(sl->ctx.stream);
         }
         slots.clear();
+        pool.release();
     }
 };
 
@@ -4323,19 +4421,23 @@ ordvec_status_t ordvec_server_start(const ordvec_sign_bitmap_t* sign, const ordv
         check_m(std::min<uint64_t>(m, ix->n));
         LOCK;
         eng().init();
-        // Admission by device memory: the largest batch whose workspace fits
-        // each slot's share of the budget.
-        const uint64_t per_slot = memory_budget_bytes / slots;
+        // Admission by device memory: the budget is one pool; the largest
+        // batch is the largest whose workspace fits all of it (batches that
+        // run together share it, each holding its own need until it ends).
         uint64_t cap = std::max<uint64_t>(1, max_batch_queries);
-        while (cap > 1 && serve_ws_bytes(sb, ix, cap, m, max_k) > per_slot) cap = cap * 3 / 4;
-        if (serve_ws_bytes(sb, ix, cap, m, max_k) > per_slot)
-            fail(kBadArgument, "memory budget is too small for a single query per slot");
+        while (cap > 1 && serve_ws_bytes(sb, ix, cap, m, max_k) > memory_budget_bytes) cap = cap * 3 / 4;
+        if (serve_ws_bytes(sb, ix, cap, m, max_k) > memory_budget_bytes)
+            fail(kBadArgument, "memory budget is too small for a single query");
         std::unique_ptr<Server> sv(new Server());
         sv->sb = sb;
         sv->rq = ix;
         sv->m = m;
         sv->max_batch = cap;
         sv->kcap = max_k;
+        sv->ws_need.assign(cap + 1, 0);
+        for (uint64_t q = 1; q <= cap; q++)
+            sv->ws_need[q] = std::max(sv->ws_need[q - 1], two_stage_bytes(sb, ix, q, m, max_k));
+        sv->pool.init(std::max<size_t>(sv->ws_need[cap], memory_budget_bytes));
         for (uint32_t i = 0; i < slots; i++) {
             std::unique_ptr<ServeSlot> sl(new ServeSlot());
             OVX_HIP(hipStreamCreateWithFlags(&sl->ctx.stream, hipStreamNonBlocking));
@@ -4344,8 +4446,8 @@ ordvec_status_t ordvec_server_start(const ordvec_sign_bitmap_t* sign, const ordv
             OVX_HIP(hipHostMalloc((void**)&sl->i_stage, cap * max_k * 8, hipHostMallocDefault));
             {
                 CtxScope scope(&sl->ctx);
-                (void)cur().workspace(serve_ws_bytes(sb, ix, cap, m, max_k));
                 (void)cur().pinned_u32(cap);
+                (void)cur().zero_words(kZwTailDone + cap);
             }
             sv->slots.push_back(std::move(sl));
         }
@@ -4444,9 +4546,18 @@ ordvec_status_t ordvec_server_open_loop(ordvec_server_t* server, const float* qu
         const double mean_nq = 0.5 * (min_nq + max_nq);
         std::exponential_distribution<double> gap(offered_qps / mean_nq);  // request arrivals per second
         std::vector<std::unique_ptr<ServeRequest>> reqs;
-        const uint64_t b0 = sv->batches;
-        uint64_t exact0 = 0;
-        for (auto& sl : sv->slots) exact0 += sl->ctx.exact_queries;
+        uint64_t b0 = 0, exact0 = 0, ov0 = 0, of0 = 0;
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            b0 = sv->batches;
+            ov0 = sv->overlapped;
+            sv->max_in_flight = sv->in_flight;
+            sv->pool.peak = sv->pool.in_use;
+            for (auto& sl : sv->slots) {
+                exact0 += sl->ctx.exact_queries;
+                of0 += sl->ctx.lent_overflows;
+            }
+        }
         const uint64_t t0 = now_ns() + 2000000;  // start 2 ms from now
         double t = 0.0;
         for (;;) {
@@ -4519,9 +4630,17 @@ ordvec_status_t ordvec_server_open_loop(ordvec_server_t* server, const float* qu
             rep.p99_service_ms = service[i99];
         }
         std::lock_guard<std::mutex> lk(sv->mu);
-        uint64_t exact1 = 0;
-        for (auto& sl : sv->slots) exact1 += sl->ctx.exact_queries;
+        uint64_t exact1 = 0, of1 = 0;
+        for (auto& sl : sv->slots) {
+            exact1 += sl->ctx.exact_queries;
+            of1 += sl->ctx.lent_overflows;
+        }
         rep.exact_path_queries = exact1 - exact0;
+        rep.pool_bytes = sv->pool.size;
+        rep.peak_pool_bytes = sv->pool.peak;
+        rep.overlapped_batches = sv->overlapped - ov0;
+        rep.max_batches_in_flight = sv->max_in_flight;
+        rep.workspace_overflows = of1 - of0;
         rep.batches = (double)(sv->batches - b0);
         rep.mean_batch_queries = rep.batches > 0 ? (double)queries / rep.batches : 0;
         *report = rep;

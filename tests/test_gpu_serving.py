@@ -110,6 +110,10 @@ def test_open_loop_completes_every_request(engine, corpus):
     assert rep["queries"] >= rep["requests"]
     assert 0 < rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
     assert rep["batches"] >= 1 and rep["mean_batch_queries"] >= 1
+    # every batch ran inside its share of the pool
+    assert rep["workspace_overflows"] == 0
+    assert 0 < rep["peak_pool_bytes"] <= rep["pool_bytes"]
+    assert rep["max_batches_in_flight"] >= 1
 
 
 def test_indexes_are_pinned_while_a_server_runs(engine, corpus):
