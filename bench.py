@@ -30,16 +30,17 @@ MFMA_FP4_PEAK_TFLOPS = 10100.0           # MI355X dense FP4 matrix peak (10.1 PF
 VALU_INT32_PEAK_OPS = 256 * 64 * 2.4e9  # CUs x lanes/clk x clock (estimate)
 
 
-def measured_traffic(kernel_prefix):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel (name with its template arguments, as
+    rocprofv3 prints it) from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
     path = os.path.join(ROOT, "profiles", "traffic_two_stage_1Mx1024.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    k = t.get("kernels", {}).get(kernel_prefix)
+    k = t.get("kernels", {}).get(kernel)
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
@@ -174,6 +175,7 @@ def main():
     mfma_tops = mfma_ops / scan_avg_s / 1e12
     step_bytes = n * d // 8 + B * (m * d * 2 // 8 + d * 4 + 12 * k)
     step_gbs = step_bytes / (elapsed / args.steps) / 1e9
+    traffic = measured_traffic("k_s1_scan_mfma<4, 0, 16>")
 
     # ---- plain (one query per call) ------------------------------------------------
     lat = []
@@ -234,11 +236,16 @@ def main():
                      "bound": "mfma", "achieved": round(mfma_tops, 1), "peak": MFMA_FP4_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "op": "FP4 multiply-add = 2 ops, d per (query, doc) pair",
                      "frac": round(mfma_tops / MFMA_FP4_PEAK_TFLOPS, 4),
-                     "traffic": measured_traffic("k_s1_scan_mfma"),
+                     "traffic": traffic,
                      "algorithmic_ops_per_launch": mfma_ops, "algorithmic_bytes_per_launch": alg_bytes,
-                     "avg_launch_us": round(scan_avg_s * 1e6, 2)},
-        "hbm_view": {"kernel": "k_s1_scan_mfma", "achieved_gbs": round(scan_gbs, 2), "peak_gbs": HBM_PEAK_GBS,
-                     "frac": round(scan_gbs / HBM_PEAK_GBS, 5)},
+                     "avg_launch_us": round(scan_avg_s * 1e6, 2),
+                     # the same launch against HBM: algorithmic bytes (sign corpus + query bitmaps) and the
+                     # measured FETCH+WRITE traffic, each over the launch time, as fractions of 8 TB/s
+                     "hbm_achieved_gbs": round(scan_gbs, 2), "hbm_peak_gbs": HBM_PEAK_GBS,
+                     "hbm_frac": round(scan_gbs / HBM_PEAK_GBS, 5),
+                     "hbm_traffic_gbs": None if traffic is None else round(traffic / scan_avg_s / 1e9, 1),
+                     "hbm_traffic_frac": None if traffic is None else round(traffic / scan_avg_s / 1e9
+                                                                              / HBM_PEAK_GBS, 4)},
         "kernel_avg_us": {f: round(ms / c * 1e3, 2) for f, (c, ms) in fam.items() if c},
         "alone_equals_batched": same_alone_batched,
         "cpu_baseline": cpu,
