@@ -166,8 +166,9 @@ ordvec_status_t ordvec_index_search(const ordvec_index_t *index, const ordvec_se
 #define ORDVEC_FLAG_DEVICE_PTRS 1
 /* Caller guarantees inputs are finite; skips the NaN/Inf scan. */
 #define ORDVEC_FLAG_ASSUME_FINITE 2
-/* ordvec_fastscan_batch_run only: replay the launch sequence eagerly instead
- * of through its captured HIP graph (same results). */
+/* ordvec_fastscan_batch_run and ordvec_two_stage_batch_run only: enqueue the
+ * launch sequence eagerly instead of replaying its captured HIP graphs (same
+ * results). */
 #define ORDVEC_FLAG_NO_GRAPH 4
 
 /* Engine limits of this library.  Searches with k above ORDVEC_MAX_TOPK and
@@ -508,6 +509,56 @@ ordvec_status_t ordvec_fastscan_batch_set_config(ordvec_fastscan_batch_t *batch,
 ordvec_status_t ordvec_fastscan_batch_tuning(ordvec_fastscan_batch_t *batch, uint64_t i, uint32_t *queries_per_tile,
                                              uint32_t *blocks_per_tile, uint32_t *grid, double *ms);
 void ordvec_fastscan_batch_free(ordvec_fastscan_batch_t *batch);
+
+/* ---- Two-stage batch plans (HIP graphs, autotuned launch shapes) -----------
+ * No reference counterpart: the batched form of ordvec_two_stage_search
+ * (src/sign_bitmap.rs:453-483 then src/quant.rs:1336-1394) for a fixed
+ * (sign bitmap, RankQuant, m, k).  A plan owns a workspace sized once for
+ * max_queries and, per (query count, query buffer, output buffers), the call's
+ * launch sequence captured in two HIP graphs: stage 1 (query preparation,
+ * sign words, sampled threshold, persistent matrix-core scan, select) up to
+ * the copy of the status words, then the rerank.  run() replays them on the
+ * library stream, waits for the status copy only (device pointers: the rerank
+ * completes in stream order), and recomputes queries whose stage-1 survivors
+ * could not prove exactness with the exact pipeline, as the one-call search
+ * does; results equal ordvec_two_stage_search's.  The last four graph pairs
+ * are kept.  autotune() times the stage-1 launch shapes on the given queries
+ * (nq <= max_queries; host or device per flags), keeps the fastest and drops
+ * the graphs.  While a plan lives, add and swap_remove on its two indexes
+ * fail and their free is deferred to the plan's free.  One thread at a time
+ * per plan.  m <= ORDVEC_MAX_STAGE1_M, k <= ORDVEC_MAX_TOPK,
+ * max_queries <= 65536; outputs nq x min(k, n). */
+typedef struct ordvec_two_stage_batch_t ordvec_two_stage_batch_t;
+/* Launch shapes of a plan (0 = the built-in rule; none changes results). */
+typedef struct ordvec_two_stage_shape_t {
+    uint32_t scan_tiles;  /* 32-query tiles per matrix-core scan workgroup: 1, 2 or 4 */
+    uint32_t scan_pairs;  /* target (document segment, query tile) pairs of the scan (rule: 1024) */
+    uint32_t scan_flat;   /* 1: one scan workgroup per pair instead of the persistent XCD-grouped grid */
+    uint32_t sample_wgs;  /* target workgroups of the matrix-core sample (rule: 256) */
+    uint32_t side_prep;   /* 1: batches of 256+ prepare the rerank's queries on a second stream */
+    uint32_t reserved[3]; /* zero */
+} ordvec_two_stage_shape_t;
+ordvec_status_t ordvec_two_stage_batch_new(const ordvec_sign_bitmap_t *sign, const ordvec_rankquant_t *rq,
+                                           uint64_t max_queries, uint64_t m, uint64_t k,
+                                           ordvec_two_stage_batch_t **out);
+ordvec_status_t ordvec_two_stage_batch_run(ordvec_two_stage_batch_t *batch, const float *queries, uint64_t nq,
+                                           float *out_scores, int64_t *out_ids, uint64_t flags);
+ordvec_status_t ordvec_two_stage_batch_autotune(ordvec_two_stage_batch_t *batch, const float *queries, uint64_t nq,
+                                                uint32_t reps, uint64_t flags);
+ordvec_status_t ordvec_two_stage_batch_shape(const ordvec_two_stage_batch_t *batch, ordvec_two_stage_shape_t *shape);
+/* Pins the shapes (BAD_ARGUMENT for scan_tiles outside {0, 1, 2, 4}, flags
+ * other than 0 / 1 or nonzero reserved words) and drops the graphs. */
+ordvec_status_t ordvec_two_stage_batch_set_shape(ordvec_two_stage_batch_t *batch,
+                                                 const ordvec_two_stage_shape_t *shape);
+/* Row i of the last autotune: the shapes and the time per call (ms). */
+ordvec_status_t ordvec_two_stage_batch_tuning(const ordvec_two_stage_batch_t *batch, uint64_t i,
+                                              ordvec_two_stage_shape_t *shape, double *ms);
+/* Counters: graph pairs captured, graph calls, eager calls (ORDVEC_FLAG_NO_GRAPH),
+ * queries recomputed by the exact stage-1 pipeline, rows of the last autotune. */
+ordvec_status_t ordvec_two_stage_batch_stats(const ordvec_two_stage_batch_t *batch, uint64_t *graph_captures,
+                                             uint64_t *graph_calls, uint64_t *eager_calls, uint64_t *exact_queries,
+                                             uint64_t *tuned_shapes);
+void ordvec_two_stage_batch_free(ordvec_two_stage_batch_t *batch);
 
 /* ---- Differentiable rerank -------------------------------------------------
  * No reference counterpart: the forward is the reference's asymmetric score

@@ -18,6 +18,7 @@ from .index import (  # noqa: E402,F401
     RankQuant,
     RankQuantFastscan,
     SignBitmap,
+    TwoStageBatch,
     TwoStageCandidatePolicy,
     stage1_exact_queries,
     synchronize,
@@ -55,6 +56,7 @@ __all__ = [
     "two_stage_search",
     "two_stage_search_policy",
     "TwoStageCandidatePolicy",
+    "TwoStageBatch",
     "Server",
     "rerank_scores",
     "stage1_exact_queries",

@@ -122,6 +122,11 @@ class TwoStagePolicy(C.Structure):
                 ("has_max_candidates", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class TwoStageShape(C.Structure):
+    _fields_ = [("scan_tiles", C.c_uint32), ("scan_pairs", C.c_uint32), ("scan_flat", C.c_uint32),
+                ("sample_wgs", C.c_uint32), ("side_prep", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
 class Hit(C.Structure):
     _fields_ = [("row_id", C.c_uint64), ("id", C.c_uint64), ("score", C.c_float), ("reserved", C.c_uint32)]
 
@@ -210,6 +215,14 @@ SIGNATURES = {
     "ordvec_fastscan_batch_tuning": (ST, [P, U64, P, P, P, P]),
     "ordvec_fastscan_batch_set_config": (ST, [P, U32, U32, U32]),
     "ordvec_fastscan_batch_free": (None, [P]),
+    "ordvec_two_stage_batch_new": (ST, [P, P, U64, U64, U64, C.POINTER(P)]),
+    "ordvec_two_stage_batch_run": (ST, [P, P, U64, P, P, U64]),
+    "ordvec_two_stage_batch_autotune": (ST, [P, P, U64, U32, U64]),
+    "ordvec_two_stage_batch_shape": (ST, [P, C.POINTER(TwoStageShape)]),
+    "ordvec_two_stage_batch_set_shape": (ST, [P, C.POINTER(TwoStageShape)]),
+    "ordvec_two_stage_batch_tuning": (ST, [P, U64, C.POINTER(TwoStageShape), P]),
+    "ordvec_two_stage_batch_stats": (ST, [P, P, P, P, P, P]),
+    "ordvec_two_stage_batch_free": (None, [P]),
     "ordvec_rankquant_score_ids": (ST, [P, P, U64, P, U64, P, U64]),
     "ordvec_rankquant_score_ids_backward": (ST, [P, P, U64, P, U64, P, P, U64]),
     "ordvec_server_start": (ST, [P, P, U64, U64, U32, U64, U64, C.POINTER(P)]),

@@ -101,6 +101,7 @@ struct DevBuf {
 struct Ctx {
     hipStream_t stream = nullptr;
     uint64_t exact_queries = 0;  // queries whose stage 1 went through the exact recomputation
+    bool timed = false;  // kernel-family timing applies (API calls and two-stage batch plans; not serving slots)
     DevBuf ws, ws2;  // per-call workspace; ws2 serves the exact stage-1 recomputation
     uint32_t* pinned = nullptr;
     size_t pinned_count = 0;
@@ -303,9 +304,10 @@ void grow_rows(T*& buf, uint64_t& cap, uint64_t n, uint64_t row_bytes, uint64_t 
     cap = nc;
 }
 
-// Every index counts the running servers that read it (engine mutex held):
-// while the count is nonzero, add and swap_remove fail and free is deferred
-// to the last server's stop, so no in-flight batch sees its rows move.
+// Every index counts the running servers and live two-stage batch plans that
+// read it (engine mutex held): while the count is nonzero, add and
+// swap_remove fail and free is deferred to the last server's stop or plan's
+// free, so no in-flight batch sees its rows move.
 struct Served {
     uint32_t serving = 0;
     bool free_pending = false;
@@ -313,8 +315,8 @@ struct Served {
 
 void check_not_serving(const Served& ix, const char* what) {
     if (ix.serving)
-        fail(kBadArgument, std::string(what) + " is not allowed while a running server reads this index "
-                                               "(stop the server first)");
+        fail(kBadArgument, std::string(what) + " is not allowed while a running server or a two-stage batch plan "
+                                               "reads this index (stop the server or free the plan first)");
 }
 
 struct RqIndex : Served {
@@ -650,10 +652,11 @@ size_t stage1_bytes(const SbIndex* ix, uint64_t nq, uint64_t m_eff) {
 
 Stage1Bufs stage1_dev(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t m_eff, uint32_t* cand,
                       uint32_t* cand_h, uint32_t* qstat, Carver& cv, const PrepFuse* prep = nullptr,
-                      bool select = true, hipEvent_t before_scan = nullptr) {
+                      bool select = true, hipEvent_t before_scan = nullptr, const FastTune* tune = nullptr) {
     Stage1Bufs b{};
     b.qbits = cv.take<uint64_t>(nq * ix->qpv);
     b.plan = plan_stage1_fast(ix->n, ix->dim, nq, m_eff);
+    if (tune) b.plan.tune = *tune;
     void* base = cv.take<char>(stage1_fast_bytes(b.plan, nq));
     carve_stage1_fast(b.plan, nq, base, &b.work);
     launch_stage1_fast(ix->words, ix->n, ix->dim, dq, nq, m_eff, b.plan, b.work, b.qbits, cand, cand_h, qstat,
@@ -1487,6 +1490,324 @@ void two_stage_noalloc(const SbIndex* sb, const RqIndex* ix, const float* querie
         OVX_HIP(hipStreamSynchronize(s));
         f0 += c;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Two-stage batch plans: one (sign bitmap, RankQuant, m, k) pairing with its
+// own context and a workspace sized once for its largest batch.  The launch
+// sequence of a batched two-stage call is captured in two HIP graphs per
+// (query count, query buffer, output buffers): stage 1 up to the copy of the
+// status words, then the rerank.  A call replays both, waits for the status
+// copy only and recomputes the queries stage 1 flagged behind the rerank, as
+// two_stage_run does; results are the same as two_stage_run's.  The stage-1
+// launch shapes (FastTune) and the side-stream query preparation are picked
+// by timing them on real queries.
+// ---------------------------------------------------------------------------
+struct TsShape {
+    FastTune t{};
+    bool side_prep = true;
+};
+
+struct TsTuneRow {
+    TsShape c;
+    double ms;
+};
+
+struct TsGraph {
+    uint64_t nq = 0;
+    const float* q = nullptr;
+    float* s = nullptr;
+    int64_t* i = nullptr;
+    hipGraphExec_t stage1 = nullptr, rerank = nullptr;
+    uint64_t used = 0;
+    void drop() {
+        if (stage1) (void)hipGraphExecDestroy(stage1);
+        if (rerank) (void)hipGraphExecDestroy(rerank);
+        stage1 = rerank = nullptr;
+    }
+};
+
+constexpr size_t kTsGraphs = 4;  // graph pairs kept per plan (least recently used one replaced)
+
+// Buffers of one call of nq queries, carved from the plan's workspace in a
+// fixed order (so the graphs and the eager fallback see the same addresses).
+struct TsBufs {
+    float* qhat;
+    float* off;
+    uint32_t* qstat;
+    uint32_t* qmap;
+    uint32_t* cand;
+    void* rwork;
+    uint64_t* qbits;
+    uint32_t nsplit;
+    Carver stage1;  // positioned at stage 1's buffers (the query sign words first)
+};
+
+struct TsBatch {
+    Ctx ctx;                   // its stream is the engine stream of each call
+    hipStream_t cap = nullptr; // capture stream
+    SbIndex* sb = nullptr;
+    RqIndex* rq = nullptr;
+    uint64_t max_nq = 0, m_eff = 0, out_k = 0;
+    TsShape shape;
+    std::vector<TsGraph> graphs;
+    uint64_t clock = 0;
+    std::vector<TsTuneRow> tuning;
+    char* ws = nullptr;        // staging for host-pointer calls, then the per-call buffers
+    size_t ws_bytes = 0;
+    float* hq = nullptr;
+    float* hs = nullptr;
+    int64_t* hi = nullptr;
+    uint32_t* hstat = nullptr; // pinned status words
+    hipEvent_t st_ev = nullptr;
+    uint64_t captures = 0, replays = 0, eager = 0;
+
+    int tier() const { return select_tier(rq->dim, rq->bits); }
+    float inv_norm() const { return rankquant_inv_norm(rq->dim, rq->bits); }
+    size_t stage_bytes() const { return al(max_nq * rq->dim * 4) + al(max_nq * out_k * 4) + al(max_nq * out_k * 8); }
+    size_t call_bytes(uint64_t nq) const {
+        const uint32_t nsplit = rerank_nsplit(nq, m_eff);
+        return al(nq * rq->dim * 4) + 4 * al(nq * 4) + al(nq * m_eff * 4) +
+               rerank_work_bytes(nq, (uint32_t)out_k, nsplit) + stage1_bytes(sb, nq, m_eff);
+    }
+    TsBufs carve(uint64_t nq) {
+        TsBufs u{};
+        Carver cv{ws + stage_bytes()};
+        u.qhat = cv.take<float>(nq * rq->dim);
+        u.off = cv.take<float>(nq);
+        u.qstat = cv.take<uint32_t>(nq);
+        u.qmap = cv.take<uint32_t>(nq);
+        u.cand = cv.take<uint32_t>(nq * m_eff);
+        u.nsplit = rerank_nsplit(nq, m_eff);
+        u.rwork = cv.take<char>(rerank_work_bytes(nq, (uint32_t)out_k, u.nsplit));
+        u.stage1 = cv;
+        Carver peek = cv;
+        u.qbits = peek.take<uint64_t>(nq * sb->qpv);  // stage1_dev takes these first
+        return u;
+    }
+    void drop_graphs() {
+        for (TsGraph& g : graphs) g.drop();
+        graphs.clear();
+    }
+    ~TsBatch() {
+        {
+            CtxScope scope(&ctx);
+            if (ctx.stream) (void)hipStreamSynchronize(ctx.stream);
+            if (cap) (void)hipStreamSynchronize(cap);
+            drop_graphs();
+            if (ws) (void)hipFree(ws);
+            if (st_ev) (void)hipEventDestroy(st_ev);
+            ctx.stream = nullptr;  // the engine's stream, not ours
+            ctx.release();
+        }
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
+// Stage 1 of a call on cur().stream (dq: device queries): the rerank's query
+// preparation (on the side stream for batches of kSidePrepMin+ when the shape
+// says so), sign words, sampled threshold, scan, select, then the status
+// words copied into the plan's pinned words.
+void ts_enqueue_stage1(TsBatch* b, const float* dq, uint64_t nq, TsBufs& u, const TsShape& sh) {
+    const uint32_t d = b->rq->dim;
+    const PrepFuse prep{dq, u.qhat, u.off, nullptr, d, b->rq->bits, b->tier(), b->inv_norm()};
+    const bool side = sh.side_prep && nq >= kSidePrepMin;
+    hipEvent_t prepped = nullptr;
+    if (side) {
+        launch_prepare_queries(dq, nq, d, b->rq->bits, b->tier(), b->inv_norm(), u.qhat, u.off, nullptr, cur().fork());
+        prepped = cur().join_event();
+    }
+    Carver cv = u.stage1;
+    Stage1Bufs s1 = stage1_dev(b->sb, dq, nq, b->m_eff, u.cand, nullptr, u.qstat, cv, side ? nullptr : &prep, true,
+                               prepped, &sh.t);
+    if (s1.qbits != u.qbits) fail(kInternal, "two-stage batch plan: buffer layout out of step");
+    OVX_HIP(hipMemcpyAsync(b->hstat, u.qstat, nq * 4, hipMemcpyDeviceToHost, cur().stream));
+}
+
+void ts_enqueue_rerank(TsBatch* b, const TsBufs& u, uint64_t nq, float* ds, int64_t* di) {
+    launch_rerank(b->rq->codes, b->rq->dim, b->rq->bits, b->tier(), b->inv_norm(), u.qhat, u.off, nq, nullptr, u.cand,
+                  b->m_eff, b->rq->n, nullptr, (uint32_t)b->out_k, ds, di, u.nsplit, u.rwork, cur().stream);
+}
+
+// The graphs for (nq, dq, ds, di), captured on first use on the plan's
+// capture stream (kernel-family events are not recorded inside graphs).
+TsGraph& ts_graph(TsBatch* b, uint64_t nq, const float* dq, float* ds, int64_t* di) {
+    for (TsGraph& g : b->graphs)
+        if (g.nq == nq && g.q == dq && g.s == ds && g.i == di) {
+            g.used = ++b->clock;
+            return g;
+        }
+    if (b->graphs.size() >= kTsGraphs) {
+        size_t lru = 0;
+        for (size_t j = 1; j < b->graphs.size(); j++)
+            if (b->graphs[j].used < b->graphs[lru].used) lru = j;
+        b->graphs[lru].drop();
+        b->graphs.erase(b->graphs.begin() + (ptrdiff_t)lru);
+    }
+    TsGraph g;
+    g.nq = nq;
+    g.q = dq;
+    g.s = ds;
+    g.i = di;
+    hipStream_t run = b->ctx.stream;
+    b->ctx.stream = b->cap;
+    NoEventsScope no_events;
+    // Two graphs: stage 1 with the status copy, then the rerank; the call
+    // records the status event between their launches and waits for it only
+    // (an event record captured inside one graph is refused by this HIP).
+    auto capture = [&](auto&& enqueue, hipGraphExec_t* out) {
+        OVX_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+        hipGraph_t graph = nullptr;
+        try {
+            enqueue();
+        } catch (...) {
+            (void)hipStreamEndCapture(b->cap, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        OVX_HIP(hipStreamEndCapture(b->cap, &graph));
+        const hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        OVX_HIP(e);
+    };
+    try {
+        TsBufs u = b->carve(nq);
+        capture([&] { ts_enqueue_stage1(b, dq, nq, u, b->shape); }, &g.stage1);
+        capture([&] { ts_enqueue_rerank(b, u, nq, ds, di); }, &g.rerank);
+    } catch (...) {
+        b->ctx.stream = run;
+        g.drop();
+        throw;
+    }
+    b->ctx.stream = run;
+    b->captures++;
+    g.used = ++b->clock;
+    b->graphs.push_back(g);
+    return b->graphs.back();
+}
+
+// One call: nq queries (host or device per flags) -> nq x out_k results.
+void ts_run(TsBatch* b, const float* queries, uint64_t nq, float* out_scores, int64_t* out_ids, uint64_t flags) {
+    const bool dev = flags & ORDVEC_FLAG_DEVICE_PTRS;
+    const uint32_t d = b->rq->dim;
+    host_finite_or_fail(queries, nq * d, flags, kNonfiniteQuery, "queries");
+    b->ctx.stream = eng().def.stream;
+    CtxScope scope(&b->ctx);
+    hipStream_t s = b->ctx.stream;
+    const float* dq = queries;
+    float* ds = out_scores;
+    int64_t* di = out_ids;
+    if (!dev) {
+        h2d(b->hq, queries, nq * d * 4);
+        dq = b->hq;
+        ds = b->hs;
+        di = b->hi;
+    }
+    TsBufs u = b->carve(nq);
+    if (flags & ORDVEC_FLAG_NO_GRAPH) {
+        ts_enqueue_stage1(b, dq, nq, u, b->shape);
+        OVX_HIP(hipEventRecord(b->st_ev, s));
+        ts_enqueue_rerank(b, u, nq, ds, di);
+        b->eager++;
+    } else {
+        TsGraph& g = ts_graph(b, nq, dq, ds, di);
+        OVX_HIP(hipGraphLaunch(g.stage1, s));
+        OVX_HIP(hipEventRecord(b->st_ev, s));
+        OVX_HIP(hipGraphLaunch(g.rerank, s));
+        b->replays++;
+    }
+    if (!dev) {
+        d2h(out_scores, ds, nq * b->out_k * 4);
+        d2h(out_ids, di, nq * b->out_k * 8);
+    }
+    OVX_HIP(hipEventSynchronize(b->st_ev));
+    const std::vector<uint32_t> st(b->hstat, b->hstat + nq);
+    finish_two_stage(b->sb, b->rq, u.qbits, st, nq, b->m_eff, b->out_k, b->tier(), b->inv_norm(), u.qhat, u.off,
+                     u.cand, u.qmap, ds, di, out_scores, out_ids, flags);
+    if (!dev) sync();
+}
+
+// Times candidate shapes on nq device queries (reps graph calls each after a
+// warm one, mean per call) and keeps the fastest: first the scan's
+// (tiles per workgroup, pairs, persistent or flat grid), then the sample's
+// workgroups and the side-stream preparation with that scan.
+void ts_tune(TsBatch* b, const float* dq, uint64_t nq, uint32_t reps) {
+    b->ctx.stream = eng().def.stream;
+    CtxScope scope(&b->ctx);
+    NoEventsScope no_events;
+    hipStream_t s = b->ctx.stream;
+    float* ds = b->hs;
+    int64_t* di = b->hi;
+    const TsShape before = b->shape;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    OVX_HIP(hipEventCreate(&e0));
+    OVX_HIP(hipEventCreate(&e1));
+    // A shape's time per call: its graphs replayed as run() replays them
+    // (status event between the two, waited for), after one warm call.
+    auto time_shape = [&](const TsShape& sh) {
+        b->drop_graphs();
+        b->shape = sh;
+        TsGraph& g = ts_graph(b, nq, dq, ds, di);
+        for (uint32_t r = 0; r <= reps; r++) {
+            if (r == 1) OVX_HIP(hipEventRecord(e0, s));
+            OVX_HIP(hipGraphLaunch(g.stage1, s));
+            OVX_HIP(hipEventRecord(b->st_ev, s));
+            OVX_HIP(hipGraphLaunch(g.rerank, s));
+            OVX_HIP(hipEventSynchronize(b->st_ev));
+        }
+        OVX_HIP(hipEventRecord(e1, s));
+        OVX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        OVX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        b->tuning.push_back(TsTuneRow{sh, (double)ms / reps});
+        return (double)ms / reps;
+    };
+    b->tuning.clear();
+    try {
+        // single queries run the vector-ALU scan, which these shapes do not touch
+        std::vector<uint32_t> nqts;
+        if (nq > 64) nqts = {2u, 4u};
+        else if (nq > 32) nqts = {1u, 2u};
+        else if (nq >= 2) nqts = {1u};
+        TsShape best;
+        double best_ms = time_shape(best);  // the built-in rules
+        for (uint32_t nqt : nqts)
+            for (uint32_t pairs : {512u, 1024u, 2048u})
+                for (uint32_t flat : {0u, 1u}) {
+                    TsShape sh;
+                    sh.t.scan_nqt = nqt;
+                    sh.t.scan_pairs = pairs;
+                    sh.t.scan_flat = flat;
+                    const double ms = time_shape(sh);
+                    if (ms < best_ms) best_ms = ms, best = sh;
+                }
+        TsShape base = best;
+        for (uint32_t wgs : {128u, 512u})
+            for (bool side : {true, false}) {
+                TsShape sh = base;
+                sh.t.sample_wgs = wgs;
+                sh.side_prep = side;
+                const double ms = time_shape(sh);
+                if (ms < best_ms) best_ms = ms, best = sh;
+            }
+        {
+            TsShape sh = base;
+            sh.side_prep = !base.side_prep;
+            const double ms = time_shape(sh);
+            if (ms < best_ms) best_ms = ms, best = sh;
+        }
+        b->shape = best;
+    } catch (...) {
+        b->shape = before;
+        b->drop_graphs();
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    OVX_HIP(hipStreamSynchronize(s));
+    b->drop_graphs();
 }
 
 // ---------------------------------------------------------------------------
@@ -2638,7 +2959,8 @@ KTimes& kt() {
 
 void ktime_begin(int family, hipStream_t s) {
     KTimes& t = kt();
-    if (!((t.mask >> family) & 1u) || t_ctx || t_no_events) return;  // serving slots are not event-timed
+    if (!((t.mask >> family) & 1u) || (t_ctx && !t_ctx->timed) || t_no_events) return;  // serving slots are not
+                                                                                         // event-timed
     hipEvent_t e = t.get();
     OVX_HIP(hipEventRecord(e, s));
     t.open[family] = e;
@@ -2646,7 +2968,7 @@ void ktime_begin(int family, hipStream_t s) {
 
 void ktime_end(int family, hipStream_t s) {
     KTimes& t = kt();
-    if (!((t.mask >> family) & 1u) || t_ctx || t_no_events || !t.open[family]) return;
+    if (!((t.mask >> family) & 1u) || (t_ctx && !t_ctx->timed) || t_no_events || !t.open[family]) return;
     hipEvent_t e = t.get();
     OVX_HIP(hipEventRecord(e, s));
     t.pending.push_back({t.open[family], e, family});
@@ -4392,6 +4714,193 @@ ordvec_status_t ordvec_fastscan_batch_tuning(ordvec_fastscan_batch_t* batch, uin
 void ordvec_fastscan_batch_free(ordvec_fastscan_batch_t* batch) {
     if (!batch) return;
     delete FB(batch);
+}
+
+// ---- two-stage batch plans -----------------------------------------------------
+static TsBatch* TB(const ordvec_two_stage_batch_t* p) {
+    return reinterpret_cast<TsBatch*>(const_cast<ordvec_two_stage_batch_t*>(p));
+}
+
+static void shape_out(const TsShape& sh, ordvec_two_stage_shape_t* o) {
+    *o = ordvec_two_stage_shape_t{};
+    o->scan_tiles = sh.t.scan_nqt;
+    o->scan_pairs = sh.t.scan_pairs;
+    o->scan_flat = sh.t.scan_flat;
+    o->sample_wgs = sh.t.sample_wgs;
+    o->side_prep = sh.side_prep ? 1u : 0u;
+}
+
+constexpr uint64_t kTsMaxQueries = 65536;
+
+ordvec_status_t ordvec_two_stage_batch_new(const ordvec_sign_bitmap_t* sign, const ordvec_rankquant_t* rq,
+                                           uint64_t max_queries, uint64_t m, uint64_t k,
+                                           ordvec_two_stage_batch_t** out) {
+    return boundary([&] {
+        NONNULL(out, "out");
+        *out = nullptr;
+        NONNULL(sign, "sign");
+        NONNULL(rq, "rq");
+        LOCK;
+        SbIndex* sb = SB(sign);
+        RqIndex* ix = RQ(rq);
+        if (sb->dim != ix->dim) fail(kBadArgument, "sign_probe.dim must match RankQuant dim " + std::to_string(ix->dim));
+        if (sb->n != ix->n) fail(kBadArgument, "sign_probe.len must match RankQuant len " + std::to_string(ix->n));
+        if (ix->n == 0) fail(kBadArgument, "a two-stage batch plan needs a non-empty index");
+        if (max_queries == 0 || max_queries > kTsMaxQueries)
+            fail(kBadArgument, "max_queries must be in [1, " + std::to_string(kTsMaxQueries) + "]");
+        if (m == 0 || k == 0) fail(kBadArgument, "m and k must be >= 1");
+        const uint64_t m_eff = std::min<uint64_t>(m, ix->n), out_k = std::min<uint64_t>(k, ix->n);
+        if (m_eff > kStage1MaxM || out_k > kTopKMax)
+            fail(kUnsupportedOperation, "two-stage batch plans take m <= " + std::to_string(kStage1MaxM) +
+                                            " and k <= " + std::to_string(kTopKMax) +
+                                            " (ordvec_two_stage_search takes larger ones)");
+        check_search_dim(ix->dim);
+        if (ix->dim > kSignMaxDim) fail(kUnsupportedOperation, "GPU sign scan supports dim <= 8192");
+        eng().init();
+        std::unique_ptr<TsBatch> b(new TsBatch());
+        b->sb = sb;
+        b->rq = ix;
+        b->max_nq = max_queries;
+        b->m_eff = m_eff;
+        b->out_k = out_k;
+        b->ctx.timed = true;
+        OVX_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+        OVX_HIP(hipEventCreateWithFlags(&b->st_ev, hipEventDisableTiming));
+        size_t need = b->call_bytes(max_queries);
+        for (uint64_t q = 1; q < std::min<uint64_t>(max_queries, 128); q++) need = std::max(need, b->call_bytes(q));
+        b->ws_bytes = b->stage_bytes() + need;
+        if (hipMalloc(&b->ws, b->ws_bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            b->ws = nullptr;
+            fail(kInternal, "device allocation of the plan's workspace (" + std::to_string(b->ws_bytes) +
+                                " bytes) failed");
+        }
+        Carver cv{b->ws};
+        b->hq = cv.take<float>(max_queries * ix->dim);
+        b->hs = cv.take<float>(max_queries * out_k);
+        b->hi = cv.take<int64_t>(max_queries * out_k);
+        {
+            b->ctx.stream = b->cap;
+            CtxScope scope(&b->ctx);
+            b->hstat = cur().pinned_u32(max_queries);
+            b->ctx.stream = nullptr;
+        }
+        sb->serving++;
+        ix->serving++;
+        *out = reinterpret_cast<ordvec_two_stage_batch_t*>(b.release());
+    });
+}
+
+ordvec_status_t ordvec_two_stage_batch_run(ordvec_two_stage_batch_t* batch, const float* queries, uint64_t nq,
+                                           float* out_scores, int64_t* out_ids, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        TsBatch* b = TB(batch);
+        if (nq > b->max_nq)
+            fail(kBadArgument, "the plan takes at most " + std::to_string(b->max_nq) + " queries per call");
+        if (nq == 0) return;
+        NONNULL(queries, "queries");
+        NONNULL(out_scores, "out_scores");
+        NONNULL(out_ids, "out_ids");
+        LOCK;
+        ts_run(b, queries, nq, out_scores, out_ids, flags);
+    });
+}
+
+ordvec_status_t ordvec_two_stage_batch_autotune(ordvec_two_stage_batch_t* batch, const float* queries, uint64_t nq,
+                                                uint32_t reps, uint64_t flags) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        NONNULL(queries, "queries");
+        TsBatch* b = TB(batch);
+        if (nq == 0 || nq > b->max_nq)
+            fail(kBadArgument, "autotune takes 1 to " + std::to_string(b->max_nq) + " queries");
+        if (reps == 0) reps = 3;
+        LOCK;
+        const uint32_t d = b->rq->dim;
+        host_finite_or_fail(queries, nq * d, flags, kNonfiniteQuery, "queries");
+        const float* dq = queries;
+        if (!(flags & ORDVEC_FLAG_DEVICE_PTRS)) {
+            b->ctx.stream = eng().def.stream;
+            CtxScope scope(&b->ctx);
+            h2d(b->hq, queries, nq * d * 4);
+            dq = b->hq;
+        }
+        ts_tune(b, dq, nq, reps);
+    });
+}
+
+ordvec_status_t ordvec_two_stage_batch_shape(const ordvec_two_stage_batch_t* batch, ordvec_two_stage_shape_t* shape) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        NONNULL(shape, "shape");
+        shape_out(TB(batch)->shape, shape);
+    });
+}
+
+ordvec_status_t ordvec_two_stage_batch_set_shape(ordvec_two_stage_batch_t* batch,
+                                                 const ordvec_two_stage_shape_t* shape) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        NONNULL(shape, "shape");
+        if (shape->scan_tiles != 0 && shape->scan_tiles != 1 && shape->scan_tiles != 2 && shape->scan_tiles != 4)
+            fail(kBadArgument, "scan_tiles must be 0, 1, 2 or 4");
+        if (shape->scan_flat > 1 || shape->side_prep > 1) fail(kBadArgument, "scan_flat and side_prep must be 0 or 1");
+        if (shape->reserved[0] || shape->reserved[1] || shape->reserved[2])
+            fail(kBadArgument, "reserved fields must be zero");
+        if (shape->scan_pairs > (1u << 20) || shape->sample_wgs > (1u << 16))
+            fail(kBadArgument, "scan_pairs must be <= 2^20 and sample_wgs <= 65536");
+        TsBatch* b = TB(batch);
+        LOCK;
+        TsShape sh;
+        sh.t.scan_nqt = shape->scan_tiles;
+        sh.t.scan_pairs = shape->scan_pairs;
+        sh.t.scan_flat = shape->scan_flat;
+        sh.t.sample_wgs = shape->sample_wgs;
+        sh.side_prep = shape->side_prep != 0;
+        if (eng().def.stream) OVX_HIP(hipStreamSynchronize(eng().def.stream));
+        else OVX_HIP(hipDeviceSynchronize());
+        b->drop_graphs();
+        b->shape = sh;
+    });
+}
+
+ordvec_status_t ordvec_two_stage_batch_tuning(const ordvec_two_stage_batch_t* batch, uint64_t i,
+                                              ordvec_two_stage_shape_t* shape, double* ms) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        const TsBatch* b = TB(batch);
+        if (i >= b->tuning.size()) fail(kBadArgument, "tuning row out of range");
+        if (shape) shape_out(b->tuning[i].c, shape);
+        if (ms) *ms = b->tuning[i].ms;
+    });
+}
+
+ordvec_status_t ordvec_two_stage_batch_stats(const ordvec_two_stage_batch_t* batch, uint64_t* graph_captures,
+                                             uint64_t* graph_calls, uint64_t* eager_calls, uint64_t* exact_queries,
+                                             uint64_t* tuned_shapes) {
+    return boundary([&] {
+        NONNULL(batch, "batch");
+        const TsBatch* b = TB(batch);
+        if (graph_captures) *graph_captures = b->captures;
+        if (graph_calls) *graph_calls = b->replays;
+        if (eager_calls) *eager_calls = b->eager;
+        if (exact_queries) *exact_queries = b->ctx.exact_queries;
+        if (tuned_shapes) *tuned_shapes = b->tuning.size();
+    });
+}
+
+void ordvec_two_stage_batch_free(ordvec_two_stage_batch_t* batch) {
+    if (!batch) return;
+    TsBatch* b = TB(batch);
+    LOCK;
+    if (eng().def.stream) (void)hipStreamSynchronize(eng().def.stream);
+    else (void)hipDeviceSynchronize();
+    SbIndex* sb = b->sb;
+    RqIndex* rq = b->rq;
+    delete b;
+    if (--sb->serving == 0 && sb->free_pending) delete sb;
+    if (--rq->serving == 0 && rq->free_pending) delete rq;
 }
 
 void ordvec_fastscan_free(ordvec_fastscan_t* index) {

@@ -202,6 +202,15 @@ void launch_stage1(const uint64_t* words, uint64_t n, uint32_t d, const uint64_t
 // survivor buffer cannot prove exactness get qstat bit 1 and are recomputed
 // by launch_stage1 (the exact histogram pipeline).
 constexpr uint32_t kFastCap = 8192;  // survivor keys per query (64 KB LDS sort)
+// Launch shapes of the batched stage 1 that do not change its results
+// (0 = the built-in rule); a two-stage batch plan times them and keeps the
+// fastest.
+struct FastTune {
+    uint32_t scan_nqt;    // matrix-core scan: 32-query tiles per workgroup (1, 2 or 4)
+    uint32_t scan_pairs;  // matrix-core scan: target (document segment, query tile) pairs (rule: 1024)
+    uint32_t scan_flat;   // 1: one workgroup per pair instead of the persistent XCD-grouped grid
+    uint32_t sample_wgs;  // matrix-core sample: target workgroups (rule: 256)
+};
 struct FastPlan {
     uint32_t qt, nbins, hwords, lds_bytes;
     uint32_t stride, nsample, sample_bps, nsseg;
@@ -209,6 +218,7 @@ struct FastPlan {
     uint32_t scan_wgl;  // VALU scan: survivor list entries per workgroup (0: global appends)
     uint32_t s_target;  // ~0u: admit every document
     uint32_t cap;
+    FastTune tune;      // zero: built-in rules
 };
 struct FastWork {
     uint32_t* hist;  // [nq][nbins]

@@ -3858,7 +3858,8 @@ static void launch_sample(const uint64_t* words, uint64_t n, uint32_t qpv, const
         // (one per wave) per workgroup, about kSampleWgs workgroups
         const uint32_t nblk = (uint32_t)((n + 63) / 64);
         const uint32_t tiles = (uint32_t)((nq + 31) / 32);
-        const uint64_t want = std::max<uint64_t>(1, (kSampleWgs + tiles - 1) / tiles);
+        const uint64_t wgs = p.tune.sample_wgs ? p.tune.sample_wgs : kSampleWgs;
+        const uint64_t want = std::max<uint64_t>(1, (wgs + tiles - 1) / tiles);
         const uint64_t nseg = std::max<uint64_t>(1, std::min<uint64_t>(want, (p.nsample + 7) / 8));
         const uint32_t bps = (uint32_t)std::min<uint64_t>(1023, (p.nsample + nseg - 1) / nseg);
         const uint32_t segs = (p.nsample + bps - 1) / bps;
@@ -3960,9 +3961,11 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
         // the smallest tile that holds the batch: every extra query tile
         // re-reads the codes, and a large corpus does not stay in L2 between
         // the tiles of a segment.
-        const uint32_t nqt = nq > 64 ? 4 : (nq > 32 ? 2 : 1);
+        uint32_t nqt = nq > 64 ? 4 : (nq > 32 ? 2 : 1);
+        if (p.tune.scan_nqt == 1 || p.tune.scan_nqt == 2 || p.tune.scan_nqt == 4) nqt = p.tune.scan_nqt;
         const uint32_t tiles_m = (uint32_t)((nq + 32 * nqt - 1) / (32 * nqt));
-        const uint64_t want = std::max<uint64_t>(1, (1024 + tiles_m - 1) / tiles_m);
+        const uint64_t pairs = p.tune.scan_pairs ? p.tune.scan_pairs : 1024;
+        const uint64_t want = std::max<uint64_t>(1, (pairs + tiles_m - 1) / tiles_m);
         uint64_t nseg = std::min<uint64_t>(std::max<uint64_t>(1, (nblk + 7) / 8), want);
         if (nseg >= 8) nseg &= ~7ull;  // XCD-grouped segments (see the kernel)
         const uint32_t bps = (uint32_t)((nblk + nseg - 1) / nseg);
@@ -3972,7 +3975,7 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
         // multiple of the tile count (each keeps one tile), at most the XCD's
         // resident capacity (32 CUs x workgroups per CU).
         uint32_t grid = tiles_m * segs;
-        if (segs % 8 == 0) {
+        if (segs % 8 == 0 && !p.tune.scan_flat) {
             const uint32_t per_cu = nqt == 4 ? 1u : 2u;
             const uint32_t c = std::max<uint32_t>(1, 32 * per_cu / tiles_m);
             const uint32_t pairs = tiles_m * (segs / 8);

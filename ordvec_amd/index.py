@@ -959,6 +959,99 @@ class FastscanBatch:
         return out
 
 
+class TwoStageBatch:
+    """A batched two-stage search plan over one (SignBitmap, RankQuant, m, k):
+    ``run(queries)`` equals ``two_stage_search(sign, rq, queries, m, k)``, with
+    the launch sequence replayed from HIP graphs captured per (query count,
+    buffers) and stage-1 launch shapes picked by ``autotune``.  While the plan
+    lives, ``add`` / ``swap_remove`` on the two indexes raise."""
+
+    _SHAPE_KEYS = ("scan_tiles", "scan_pairs", "scan_flat", "sample_wgs", "side_prep")
+
+    def __init__(self, sign: SignBitmap, rq: RankQuant, max_queries: int, m: int, k: int):
+        self._keep = (sign, rq)
+        h = C.c_void_p()
+        N.check(N.lib().ordvec_two_stage_batch_new(sign._h, rq._h, int(max_queries), int(m), int(k), C.byref(h)))
+        self._h = h
+        self.dim, _, n = rq._info()
+        self.max_queries = int(max_queries)
+        self.out_k = min(int(k), n)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and N._lib is not None:
+            N.lib().ordvec_two_stage_batch_free(h)
+            self._h = None
+
+    def close(self) -> None:
+        """Frees the plan now (its indexes accept add / swap_remove again)."""
+        self.__del__()
+
+    @classmethod
+    def _shape_dict(cls, sh) -> dict:
+        return {key: int(getattr(sh, key)) for key in cls._SHAPE_KEYS}
+
+    def shape(self) -> dict:
+        sh = N.TwoStageShape()
+        N.check(N.lib().ordvec_two_stage_batch_shape(self._h, C.byref(sh)))
+        return self._shape_dict(sh)
+
+    def set_shape(self, **shape) -> dict:
+        """Pins launch shapes (keys of ``shape()``; missing keys keep their
+        current value) and drops the captured graphs."""
+        cur = self.shape()
+        unknown = set(shape) - set(self._SHAPE_KEYS)
+        if unknown:
+            raise ValueError(f"unknown shape keys {sorted(unknown)}")
+        cur.update({k: int(v) for k, v in shape.items()})
+        sh = N.TwoStageShape(**cur)
+        _bind_stream()
+        N.check(N.lib().ordvec_two_stage_batch_set_shape(self._h, C.byref(sh)))
+        return self.shape()
+
+    def stats(self) -> dict:
+        v = [C.c_uint64() for _ in range(5)]
+        N.check(N.lib().ordvec_two_stage_batch_stats(self._h, *[C.byref(x) for x in v]))
+        keys = ("graph_captures", "graph_calls", "eager_calls", "exact_queries", "tuned_shapes")
+        return {key: x.value for key, x in zip(keys, v)}
+
+    def tuning(self) -> list:
+        rows = []
+        for i in range(self.stats()["tuned_shapes"]):
+            sh, ms = N.TwoStageShape(), C.c_double()
+            N.check(N.lib().ordvec_two_stage_batch_tuning(self._h, i, C.byref(sh), C.byref(ms)))
+            rows.append(dict(self._shape_dict(sh), ms=ms.value))
+        return rows
+
+    def autotune(self, queries, reps: int = 3) -> list:
+        q, nq, flag = _float_rows(queries, self.dim, "queries")
+        _bind_stream() if flag else _host_stream()
+        N.check(N.lib().ordvec_two_stage_batch_autotune(self._h, _ptr(q), nq, int(reps), flag))
+        return self.tuning()
+
+    def run(self, queries, use_graph: bool = True, assume_finite: bool = False, out=None):
+        """(scores f32[nq, min(k, n)], ids i64[nq, min(k, n)]); ``out`` may
+        pass preallocated (scores, ids) of that shape on the queries' device."""
+        q, nq, flag = _float_rows(queries, self.dim, "queries")
+        if out is None:
+            s = _out((nq, self.out_k), np.float32, flag)
+            i = _out((nq, self.out_k), np.int64, flag)
+        else:
+            s, i = out
+            for a, dt in ((s, "float32"), (i, "int64")):
+                ok = (_is_cuda_tensor(a) if flag else isinstance(a, np.ndarray) and a.flags.c_contiguous)
+                if not ok or tuple(a.shape) != (nq, self.out_k) or str(a.dtype).split(".")[-1] != dt:
+                    raise ValueError(f"out arrays must be contiguous ({nq}, {self.out_k}) float32 / int64 "
+                                     "on the queries' device")
+                if flag and not a.is_contiguous():
+                    raise ValueError("out tensors must be contiguous")
+        if nq:
+            _bind_stream() if flag else _host_stream()
+            f = flag | (0 if use_graph else N.FLAG_NO_GRAPH) | (N.FLAG_ASSUME_FINITE if assume_finite else 0)
+            N.check(N.lib().ordvec_two_stage_batch_run(self._h, _ptr(q), nq, _ptr(s), _ptr(i), f))
+        return s, i
+
+
 def two_stage_search(sign: SignBitmap, rq: RankQuant, queries, m: int, k: int,
                      assume_finite: bool = False):
     """Sign-bitmap top-m candidates then RankQuant asymmetric rerank to k
