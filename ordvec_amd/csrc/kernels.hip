@@ -1466,18 +1466,23 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
     }
 }
 
-// Full scan on the matrix cores.  Documents expand to d_k in {0, 1} and
-// queries to s_k = +1 / -1 for bit 1 / 0 (sign bitmaps) or to {0, 1}
-// (top-bucket bitmaps), both as FP4 (e2m1: 0x2 = 1.0, 0xA = -1.0, 0x0 = 0);
-// a 32 x 32 tile of (document, query) dot products is then
+// Full scan on the matrix cores.  Documents expand to w_k d_k with d_k in
+// {0, 1} and queries to s_k / w_k with s_k = +1 / -1 for bit 1 / 0 (sign
+// bitmaps) or s_k in {0, 1} (top-bucket bitmaps), both as FP4 (e2m1); a
+// 32 x 32 tile of (document, query) dot products is then
 // v_mfma_scale_f32_32x32x64_f8f6f4 (FP4 x FP4, unit scales) over one 64-bit
-// code word per instruction.  Every product is 0 or +-1 and every partial sum
-// an integer below 2^24, so the f32 accumulation is exact:
+// code word per instruction.  The weight w_k in {0.5, 1, 2} is whatever the
+// bit's position in its nibble gives for free: a document dword masked with
+// 0x11111111, 0x22222222 or 0x44444444 is already a vector of FP4 {0, 0.5},
+// {0, 1.0} or {0, 2.0} (one AND each; only the fourth bit of each nibble,
+// the sign position, needs a shift), and the query carries 1 / w_k (2.0, 1.0,
+// 0.5).  Every product w_k d_k * s_k / w_k is 0 or +-1 exactly and every
+// partial sum an integer below 2^24, so the f32 accumulation is exact:
 //   sign:   sum_k d_k s_k = 2|d & q| - |d|,  so  popcount(d ^ q) = |q| - dot
 //   bitmap: sum_k d_k q_k = |d & q|,         so  n_top - overlap = n_top - dot
 // A (documents): lane l holds row l%32 and the 32 coordinates of half l/32 of
 // the word as 4 dwords of 8 nibbles, coordinate 4i + j in nibble i of dword j
-// (four shift-and-mask ops, no table); B (queries) uses the same assignment,
+// (five mask-and-shift ops, no table); B (queries) uses the same assignment,
 // so the k order inside the instruction never matters.  D: lane l holds
 // column (query) l%32 and rows 8*(v/4) + 4*(l/32) + v%4, v = 0..15.
 // A workgroup (8 waves) holds NQT query tiles of 32 whose nibbles are
@@ -1489,26 +1494,33 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
 typedef int ovx_v8i __attribute__((ext_vector_type(8)));
 typedef float ovx_v16f __attribute__((ext_vector_type(16)));
 
-// 32 bits -> 32 FP4 values {0, 1.0}: nibble i of dword j is bit 4i + j.
-__device__ __forceinline__ ovx_v8i fp4_01(uint32_t x) {
+// Document side: 32 bits -> 32 weighted FP4 values, nibble i of dword j is
+// bit 4i + j times w_j, (w_0, w_1, w_2, w_3) = (0.5, 1.0, 2.0, 0.5)
+// (e2m1 0x1 = 0.5, 0x2 = 1.0, 0x4 = 2.0).
+__device__ __forceinline__ ovx_v8i fp4_doc(uint32_t x) {
     ovx_v8i r;
-    r[0] = (int)((x << 1) & 0x22222222u);
+    r[0] = (int)(x & 0x11111111u);
     r[1] = (int)(x & 0x22222222u);
-    r[2] = (int)((x >> 1) & 0x22222222u);
-    r[3] = (int)((x >> 2) & 0x22222222u);
+    r[2] = (int)(x & 0x44444444u);
+    r[3] = (int)((x >> 3) & 0x11111111u);
     r[4] = r[5] = r[6] = r[7] = 0;  // FP4 operands use the first four dwords
     return r;
 }
 
-// 32 bits -> 32 FP4 values: +1.0 (0x2) for a set bit, -1.0 (0xA) otherwise.
-__device__ __forceinline__ uint4 fp4_pm(uint32_t x) {
-    const ovx_v8i y = fp4_01(x);
-    uint4 o;
-    o.x = 0x22222222u | ((~(uint32_t)y[0] << 2) & 0x88888888u);
-    o.y = 0x22222222u | ((~(uint32_t)y[1] << 2) & 0x88888888u);
-    o.z = 0x22222222u | ((~(uint32_t)y[2] << 2) & 0x88888888u);
-    o.w = 0x22222222u | ((~(uint32_t)y[3] << 2) & 0x88888888u);
-    return o;
+// Query side, the same assignment with the inverse weights 1 / w_j = (2.0,
+// 1.0, 0.5, 2.0) (e2m1 0x4, 0x2, 0x1; bit 3 of a nibble is the sign):
+// PM: +1 / w_j for a set bit, -1 / w_j otherwise (sign bitmaps);
+// otherwise 1 / w_j for a set bit, 0 otherwise (top-bucket bitmaps).
+template <bool PM>
+__device__ __forceinline__ uint4 fp4_query(uint32_t x) {
+    const uint32_t mag[4] = {0x44444444u, 0x22222222u, 0x11111111u, 0x44444444u};
+    uint32_t o[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t b = (x >> j) & 0x11111111u;  // bit 4i + j at bit 4i
+        o[j] = PM ? (mag[j] | ((~b & 0x11111111u) << 3)) : (b * (mag[j] & 0xfu));
+    }
+    return make_uint4(o[0], o[1], o[2], o[3]);
 }
 
 constexpr uint32_t kMfmaThreads = 512;
@@ -1526,7 +1538,7 @@ __device__ __forceinline__ void mfma_word(uint32_t x0, uint32_t x1, int w, uint3
     uint4 bq[NQT];
 #pragma unroll
     for (int t = 0; t < NQT; t++) bq[t] = sB[(t * QPV + w) * 64 + lane];
-    const ovx_v8i a0 = fp4_01(x0), a1 = fp4_01(x1);
+    const ovx_v8i a0 = fp4_doc(x0), a1 = fp4_doc(x1);
 #pragma unroll
     for (int t = 0; t < NQT; t++) {
         const ovx_v8i bf = {(int)bq[t].x, (int)bq[t].y, (int)bq[t].z, (int)bq[t].w, 0, 0, 0, 0};
@@ -1645,12 +1657,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                 const uint32_t q = q0 + t * 32 + (ln & 31);
                 if (q < nq) {
                     const uint32_t bits = (uint32_t)(qbits[(uint64_t)q * QPV + w] >> (32 * (ln >> 5)));
-                    if (OP) {
-                        const ovx_v8i y = fp4_01(bits);
-                        sB[e] = make_uint4((uint32_t)y[0], (uint32_t)y[1], (uint32_t)y[2], (uint32_t)y[3]);
-                    } else {
-                        sB[e] = fp4_pm(bits);
-                    }
+                    sB[e] = fp4_query<!OP>(bits);
                 } else {
                     sB[e] = make_uint4(0, 0, 0, 0);
                 }
@@ -1789,12 +1796,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
         const uint32_t ln = e & 63, w = e >> 6, q = q0 + (ln & 31);
         if (q < nq) {
             const uint32_t bits = (uint32_t)(qbits[(uint64_t)q * QPV + w] >> (32 * (ln >> 5)));
-            if (OP) {
-                const ovx_v8i y = fp4_01(bits);
-                sB[e] = make_uint4((uint32_t)y[0], (uint32_t)y[1], (uint32_t)y[2], (uint32_t)y[3]);
-            } else {
-                sB[e] = fp4_pm(bits);
-            }
+            sB[e] = fp4_query<!OP>(bits);
         } else {
             sB[e] = make_uint4(0, 0, 0, 0);
         }
