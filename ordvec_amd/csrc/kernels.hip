@@ -1530,14 +1530,27 @@ constexpr int kUnitScale = 0x7f7f7f7f;      // E8M0 127 = 2^0 in every byte
 // One code word (64 coordinates) of documents col and col + 32 (this lane's
 // 32-bit halves x0, x1) against every query tile.
 // FIRST: the accumulators start from zero (inline constant), no clearing pass.
+// pf: with two or more tiles, the fragments of tiles 0 and 1 for this word,
+// read during the previous word; this word reads the next word's (word 0
+// after the last) so that their LDS latency is covered by this word's MFMAs.
 template <int NQT, int QPV, bool FIRST>
 __device__ __forceinline__ void mfma_word(uint32_t x0, uint32_t x1, int w, uint32_t lane,
-                                          const uint4* __restrict__ sB, ovx_v16f (&acc)[2][NQT]) {
+                                          const uint4* __restrict__ sB, ovx_v16f (&acc)[2][NQT], uint4 (&pf)[2]) {
     // All of the word's query fragments are requested before the document
     // expansion, so the VALU work covers part of the LDS latency.
     uint4 bq[NQT];
+    if constexpr (NQT >= 2) {
+        bq[0] = pf[0];
+        bq[1] = pf[1];
 #pragma unroll
-    for (int t = 0; t < NQT; t++) bq[t] = sB[(t * QPV + w) * 64 + lane];
+        for (int t = 2; t < NQT; t++) bq[t] = sB[(t * QPV + w) * 64 + lane];
+        const int wn = (w + 1) % QPV;
+        pf[0] = sB[wn * 64 + lane];
+        pf[1] = sB[(QPV + wn) * 64 + lane];
+    } else {
+#pragma unroll
+        for (int t = 0; t < NQT; t++) bq[t] = sB[(t * QPV + w) * 64 + lane];
+    }
     const ovx_v8i a0 = fp4_doc(x0), a1 = fp4_doc(x1);
 #pragma unroll
     for (int t = 0; t < NQT; t++) {
@@ -1557,10 +1570,10 @@ __device__ __forceinline__ void mfma_word(uint32_t x0, uint32_t x1, int w, uint3
     }
 }
 
-// Survivor lists: 256 entries per 32-query tile (1024 at four tiles) of
+// Survivor lists: 320 entries per 32-query tile (1280 at four tiles) of
 // (query slot << 56 | hamming << 32 | doc)
 // per wave (plus one spare slot per lane), drained to the per-query buffers
-// when half full and at the end.  A drain takes one global atomic per query
+// when all but 64 entries per tile are used, and at the end.  A drain takes one global atomic per query
 // slot rather than one per entry: the wave counts its entries per slot in
 // LDS (wcnt, QW counters), reserves that many positions with one atomicAdd
 // per slot, leaves the reserved base in wcnt, and then hands out positions
@@ -1615,8 +1628,14 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     __shared__ int s_base[QW];
     __shared__ uint32_t s_thr[QW];
     // Per-wave survivor lists (+ a spare slot per lane), sized by the tile
-    // count so one- and two-tile workgroups leave LDS for a second workgroup.
-    constexpr uint32_t kListCap = 256 * NQT;
+    // count so one- and two-tile workgroups leave LDS for a second workgroup
+    // (four tiles: 84 KiB of lists + 64 KiB of fragments + counters fill the
+    // 160 KiB).  A list is drained once it holds kListCap - kListHeadroom
+    // entries, about 8 per query slot, so each drain writes runs of about 64
+    // bytes per query; one block adds about 18 entries on average and never
+    // more than the headroom on ordinary data (more is caught and sends the
+    // tile to the exact path).
+    constexpr uint32_t kListCap = 320 * NQT, kListHeadroom = 64 * NQT;
     __shared__ uint64_t s_list[NWAVE * (kListCap + 64)];
     __shared__ uint32_t s_wcnt[NWAVE * QW];  // per-wave slot counters of the list drain
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1693,6 +1712,11 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                 r1[i] = first[i * 128 + 64];
             }
         }
+        uint4 pf[2] = {};  // tiles 0 and 1 of the next word (mfma_word reads them one word ahead)
+        if constexpr (NQT >= 2) {
+            pf[0] = sB[lane];
+            pf[1] = sB[QPV * 64 + lane];
+        }
         for (uint32_t b = b_begin + wave; b < b_end; b += NWAVE) {
             ovx_v16f acc[2][NQT];
             // The query fragments do not depend on b; an opaque lane index keeps
@@ -1719,8 +1743,8 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                         r0[i] = next[(wn - QPV) * 128];
                         r1[i] = next[(wn - QPV) * 128 + 64];
                     }
-                    if (w == 0) mfma_word<NQT, QPV, true>(x0, x1, 0, slane, sB, acc);
-                    else mfma_word<NQT, QPV, false>(x0, x1, w, slane, sB, acc);
+                    if (w == 0) mfma_word<NQT, QPV, true>(x0, x1, 0, slane, sB, acc, pf);
+                    else mfma_word<NQT, QPV, false>(x0, x1, w, slane, sB, acc, pf);
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }
@@ -1748,14 +1772,16 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
 #pragma unroll
                     for (int v = 0; v < 16; v++) {
                         if (__builtin_expect(m[v] != 0, 0)) {  // out of line: the common path falls through
-                            const uint32_t off = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[v] >> 32),
-                                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m[v], 0u));
+                            // list position = wbase + rank of this lane among the passing lanes
+                            const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
+                                (uint32_t)(m[v] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[v], wbase));
                             const bool p = acc[G][t][v] >= need[t];
-                            const uint32_t slot = min(p ? wbase + off : 0xffffffffu, kListCap + lane);
+                            const uint32_t slot = min(p ? pos : 0xffffffffu, kListCap + lane);
                             // (query slot << 24 | base) - dot = query slot << 24 | hamming
                             const uint32_t hi = hbase - (uint32_t)(int)acc[G][t][v];
                             wlist[slot] = ((uint64_t)hi << 32) | (dbase + (G * 32 + 8 * (v / 4) + (v % 4)));
-                            wbase += (uint32_t)__popcll(m[v]);
+                            // wave-uniform: kept in a scalar register
+                            wbase = __builtin_amdgcn_readfirstlane(wbase + (uint32_t)__popcll(m[v]));
                         }
                     }
                 }
@@ -1766,7 +1792,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                     if (q0 + t * 32 + col < nq) atomicAdd(cnt_col + t * 32, cap + 1);
                 wbase = kListCap;
             }
-            if (wbase >= kListCap / 2) {
+            if (wbase >= kListCap - kListHeadroom) {
                 s1_flush_list<QW>(wlist, wbase, lane, q0, n, cap, keys, cnt, s_wcnt + wave * QW);
                 wbase = 0;
             }
@@ -1822,6 +1848,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
         uint32_t slane = lane;
         asm volatile("" : "+v"(slane));
         ovx_v16f acc[2][1];
+        uint4 pf[2] = {};  // unused with one tile
         const uint32_t* base = words32 + ((uint64_t)b * QPV * 64 + col) * 2 + g;
         uint32_t x0[QPV], x1[QPV];
 #pragma unroll
@@ -1831,8 +1858,8 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
         }
 #pragma unroll
         for (int w = 0; w < QPV; w++) {
-            if (w == 0) mfma_word<1, QPV, true>(x0[w], x1[w], 0, slane, sB, acc);
-            else mfma_word<1, QPV, false>(x0[w], x1[w], w, slane, sB, acc);
+            if (w == 0) mfma_word<1, QPV, true>(x0[w], x1[w], 0, slane, sB, acc, pf);
+            else mfma_word<1, QPV, false>(x0[w], x1[w], w, slane, sB, acc, pf);
         }
         // The MFMAs read every lane's operands, so they must run with the full
         // exec mask: an empty asm reading the results keeps the compiler from
