@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 MFMA_FP4_PEAK_TFLOPS = 10100.0           # MI355X dense FP4 matrix peak (10.1 PFLOP/s)
 VALU_INT32_PEAK_OPS = 256 * 64 * 2.4e9  # CUs x lanes/clk x clock (estimate)
+LDS_LOOKUP_PEAK = 256 * 32 * 4 * 2.4e9   # CUs x LDS dwords/clk x queries per dword x clock
 
 
 def measured_traffic(kernel):
@@ -996,6 +997,14 @@ def multi_main(args, dev):
         "tuning": sorted(tuning, key=lambda r: r["ms"])[:8],
         "codes_read_gbs": round(code_bytes / (batch_ms / 1e3) / 1e9, 1),
         "lut_lookups_per_s": float(f"{lookups / (batch_ms / 1e3):.4g}"),
+        # The LUT tiles' bound: one 32-bit LDS read serves a coordinate's entry for
+        # four queries, so a (query, document, coordinate) lookup costs a quarter of
+        # an LDS dword read; MI355X's LDS returns 128 bytes per clock per CU.
+        "roofline": {"bound": "lds", "unit": "lookups/s", "achieved": float(f"{lookups / (batch_ms / 1e3):.4g}"),
+                     "peak": LDS_LOOKUP_PEAK, "frac": round(lookups / (batch_ms / 1e3) / LDS_LOOKUP_PEAK, 4),
+                     "peak_basis": "256 CUs x 32 LDS dwords/clock x 4 lookups per dword x 2.4 GHz",
+                     "hbm_achieved_gbs": round(code_bytes / (batch_ms / 1e3) / 1e9, 1),
+                     "hbm_frac": round(code_bytes / (batch_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "batch_equals_plain": same,
         "oracle_checked_indexes": checked,
         "oracle_equal": oracle_ok,

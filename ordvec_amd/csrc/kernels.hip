@@ -2340,6 +2340,10 @@ __device__ __forceinline__ void stage_q(float* qs, const float* qg, uint32_t d) 
 // writes the final row; otherwise it writes its slice's top-k keys to
 // `partial` and k_merge_topk finishes the row.  `qmap` (optional) maps grid
 // rows to query rows.  q_hat is staged in dynamic LDS (d floats).
+template <int KW>
+__device__ void wave_kmin(const uint64_t* a, uint32_t cnt, uint32_t k, uint64_t* out, uint32_t lane);
+constexpr uint32_t kRerankSelK = 32;  // largest k the rerank selects from per-candidate key slots
+
 template <int TIER, int BITS>
 __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ codes, uint32_t d, float inv_norm,
                                                 const float* __restrict__ qhat, const float* __restrict__ offset,
@@ -2362,6 +2366,12 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
     const float off = offset[q];
     const uint32_t bpv = d * BITS / 8;
     tk_init(tk, out_k);
+    // Rows of at most TK_CAP candidates with k <= kRerankSelK (the two-stage
+    // rerank: m = 1000, k = 10) keep every key at its candidate's slot of
+    // tk.keys (no atomics, no barrier per round) and select the k smallest at
+    // the end: each wave takes its quarter's k smallest (wave_kmin), one wave
+    // the k smallest of those.  Same keys and order as the streaming top-k.
+    const bool slots = TIER == 2 && se - sb <= TK_CAP && out_k <= kRerankSelK && blockDim.x == 256;
     if (TIER == 2) {
         // Two candidates per quad per round (quad and quad + 64), so every q
         // value read from LDS is used twice.
@@ -2371,7 +2381,7 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
 #pragma unroll
         for (uint32_t r = 0; r < NC; r++) id_next[r] = sb + quad + r * nquad < se ? cands[sb + quad + r * nquad] : 0u;
         for (uint64_t r0 = sb; r0 < se; r0 += per_round) {
-            tk_reserve(tk, out_k, per_round);
+            if (!slots) tk_reserve(tk, out_k, per_round);
             const uint8_t* rows[NC];
             uint32_t id[NC];
             bool ok[NC];
@@ -2385,9 +2395,19 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
             }
             float raw[NC];
             raw_avx512_quad_n<BITS, NC>(rows, d, qs, jl, raw);
+            if (slots) {
 #pragma unroll
-            for (uint32_t r = 0; r < NC; r++)
-                if (ok[r] && jl == 0) tk_offer(tk, out_k, score_key(__fadd_rn(__fmul_rn(raw[r], inv_norm), off), id[r]));
+                for (uint32_t r = 0; r < NC; r++) {
+                    const uint64_t j = r0 + quad + r * nquad;
+                    if (jl == 0 && j < se)
+                        tk.keys[j - sb] = ok[r] ? score_key(__fadd_rn(__fmul_rn(raw[r], inv_norm), off), id[r]) : kKeyMax;
+                }
+            } else {
+#pragma unroll
+                for (uint32_t r = 0; r < NC; r++)
+                    if (ok[r] && jl == 0)
+                        tk_offer(tk, out_k, score_key(__fadd_rn(__fmul_rn(raw[r], inv_norm), off), id[r]));
+            }
         }
     } else {
         for (uint64_t r0 = sb; r0 < se; r0 += blockDim.x) {
@@ -2401,18 +2421,32 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
         }
     }
     __syncthreads();
-    tk_flush(tk, out_k);
+    const uint64_t* best = tk.keys;  // the k smallest keys, ascending
+    if (slots) {
+        // four waves' k smallest (packed: wave w at w * out_k), then the result
+        __shared__ uint64_t s_sel[5 * kRerankSelK];
+        const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const uint32_t count = (uint32_t)(se - sb), quarter = (count + 3) / 4;  // <= 512 keys per wave
+        const uint32_t w0 = min(count, wave * quarter), w1 = min(count, w0 + quarter);
+        wave_kmin<8>(tk.keys + w0, w1 - w0, out_k, s_sel + wave * out_k, lane);
+        __syncthreads();
+        if (wave == 0) wave_kmin<2>(s_sel, 4 * out_k, out_k, s_sel + 4 * kRerankSelK, lane);
+        __syncthreads();
+        best = s_sel + 4 * kRerankSelK;
+    } else {
+        tk_flush(tk, out_k);
+    }
     if (nsplit == 1) {
         float* os = out_s + q * out_k;
         int64_t* oi = out_i + q * out_k;
         for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) {
-            uint64_t key = tk.keys[j];
+            uint64_t key = best[j];
             os[j] = key == kKeyMax ? -INFINITY : key_score(key);
             oi[j] = key == kKeyMax ? -1 : (int64_t)(key & 0xffffffffu);
         }
     } else {
         uint64_t* pp = partial + (qidx * nsplit + split) * out_k;
-        for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) pp[j] = tk.keys[j];
+        for (uint32_t j = threadIdx.x; j < out_k; j += blockDim.x) pp[j] = best[j];
     }
 }
 
