@@ -1624,6 +1624,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                                                                   uint32_t ntiles, uint32_t nsegs) {
     constexpr uint32_t QW = 32 * NQT, NWAVE = kMfmaThreads / 64;
     constexpr int RING = QPV % 8 == 0 ? 8 : (QPV % 4 == 0 ? 4 : QPV);  // words in flight
+    constexpr bool PAIR = QPV % 2 == 0;  // whole 64-bit words per lane (half the load instructions)
     extern __shared__ uint4 sB[];  // [NQT][QPV][64]
     __shared__ int s_base[QW];
     __shared__ uint32_t s_thr[QW];
@@ -1659,6 +1660,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     float need[NQT];
     int basev[NQT];
     const uint32_t* words32 = reinterpret_cast<const uint32_t*>(words);
+    const uint2* words2 = reinterpret_cast<const uint2*>(words);
     for (uint32_t pr = p0; pr < pend; pr += pstep) {
         const uint32_t tile = pr % ntiles;
         const uint32_t seg = grouped ? xcd + 8 * (pr / ntiles) : pr / ntiles;
@@ -1675,7 +1677,11 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                 const uint32_t ln = e & 63, w = (e >> 6) % QPV, t = (e >> 6) / QPV;
                 const uint32_t q = q0 + t * 32 + (ln & 31);
                 if (q < nq) {
-                    const uint32_t bits = (uint32_t)(qbits[(uint64_t)q * QPV + w] >> (32 * (ln >> 5)));
+                    // slot w: K half h = ln / 32 is dword h of word w (one word per
+                    // slot), or with paired loads dword w % 2 of word 2 (w / 2) + h
+                    const uint32_t word = PAIR ? 2 * (w / 2) + (ln >> 5) : w;
+                    const uint32_t half = PAIR ? (w & 1) : (ln >> 5);
+                    const uint32_t bits = (uint32_t)(qbits[(uint64_t)q * QPV + word] >> (32 * half));
                     sB[e] = fp4_query<!OP>(bits);
                 } else {
                     sB[e] = make_uint4(0, 0, 0, 0);
@@ -1702,14 +1708,26 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
         const uint32_t b_end = min(nblk, b_begin + blocks_per_seg);
         // dword g of documents b*64 + col (+32): word w at [w * 128] (+64).  The
         // ring runs across blocks: the last RING words of a block prefetch the
-        // first RING words of the wave's next block.
+        // first RING words of the wave's next block.  With paired loads (QPV
+        // even) a lane loads whole words instead, word 2p + g of documents col
+        // and col + 32 for pair p, feeding slots 2p (low dwords) and 2p + 1.
         uint32_t r0[RING], r1[RING];
+        uint2 p0[PAIR ? RING / 2 : 1], p1[PAIR ? RING / 2 : 1];
         if (b_begin + wave < b_end) {
-            const uint32_t* first = words32 + ((uint64_t)(b_begin + wave) * QPV * 64 + col) * 2 + g;
+            if constexpr (PAIR) {
+                const uint2* first = words2 + (uint64_t)(b_begin + wave) * QPV * 64 + (uint64_t)g * 64 + col;
 #pragma unroll
-            for (int i = 0; i < RING; i++) {
-                r0[i] = first[i * 128];
-                r1[i] = first[i * 128 + 64];
+                for (int i = 0; i < RING / 2; i++) {
+                    p0[i] = first[2 * i * 64];
+                    p1[i] = first[2 * i * 64 + 32];
+                }
+            } else {
+                const uint32_t* first = words32 + ((uint64_t)(b_begin + wave) * QPV * 64 + col) * 2 + g;
+#pragma unroll
+                for (int i = 0; i < RING; i++) {
+                    r0[i] = first[i * 128];
+                    r1[i] = first[i * 128 + 64];
+                }
             }
         }
         uint4 pf[2] = {};  // tiles 0 and 1 of the next word (mfma_word reads them one word ahead)
@@ -1724,28 +1742,57 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
             // not fit in registers).
             uint32_t slane = lane;
             asm volatile("" : "+v"(slane));
-            const uint32_t* base = words32 + ((uint64_t)b * QPV * 64 + col) * 2 + g;
-            // the wave's next block, or (after its last block) this block again:
-            // an in-bounds load whose value is never used, which keeps the
-            // unrolled word loop free of branches
-            const uint32_t* next = b + NWAVE < b_end ? base + (uint64_t)NWAVE * QPV * 128 : base;
+            if constexpr (PAIR) {
+                constexpr int RP = RING / 2, NP = QPV / 2;  // pairs in flight, pairs per block
+                const uint2* base = words2 + (uint64_t)b * QPV * 64 + (uint64_t)g * 64 + col;
+                // the wave's next block, or (after its last block) this block
+                // again: an in-bounds load whose value is never used
+                const uint2* next = b + NWAVE < b_end ? base + (uint64_t)NWAVE * QPV * 64 : base;
 #pragma unroll
-            for (int w0 = 0; w0 < QPV; w0 += RING) {
+                for (int j0 = 0; j0 < NP; j0 += RP) {
 #pragma unroll
-                for (int i = 0; i < RING; i++) {
-                    const int w = w0 + i;
-                    const uint32_t x0 = r0[i], x1 = r1[i];
-                    const int wn = w + RING;
-                    if (wn < QPV) {
-                        r0[i] = base[wn * 128];
-                        r1[i] = base[wn * 128 + 64];
-                    } else {
-                        r0[i] = next[(wn - QPV) * 128];
-                        r1[i] = next[(wn - QPV) * 128 + 64];
+                    for (int i = 0; i < RP; i++) {
+                        const int p = j0 + i;
+                        const uint2 u0 = p0[i], u1 = p1[i];
+                        const int pn = p + RP;
+                        if (pn < NP) {
+                            p0[i] = base[2 * pn * 64];
+                            p1[i] = base[2 * pn * 64 + 32];
+                        } else {
+                            p0[i] = next[2 * (pn - NP) * 64];
+                            p1[i] = next[2 * (pn - NP) * 64 + 32];
+                        }
+                        if (p == 0) mfma_word<NQT, QPV, true>(u0.x, u1.x, 0, slane, sB, acc, pf);
+                        else mfma_word<NQT, QPV, false>(u0.x, u1.x, 2 * p, slane, sB, acc, pf);
+                        __builtin_amdgcn_sched_barrier(0);
+                        mfma_word<NQT, QPV, false>(u0.y, u1.y, 2 * p + 1, slane, sB, acc, pf);
+                        __builtin_amdgcn_sched_barrier(0);
                     }
-                    if (w == 0) mfma_word<NQT, QPV, true>(x0, x1, 0, slane, sB, acc, pf);
-                    else mfma_word<NQT, QPV, false>(x0, x1, w, slane, sB, acc, pf);
-                    __builtin_amdgcn_sched_barrier(0);
+                }
+            } else {
+                const uint32_t* base = words32 + ((uint64_t)b * QPV * 64 + col) * 2 + g;
+                // the wave's next block, or (after its last block) this block again:
+                // an in-bounds load whose value is never used, which keeps the
+                // unrolled word loop free of branches
+                const uint32_t* next = b + NWAVE < b_end ? base + (uint64_t)NWAVE * QPV * 128 : base;
+#pragma unroll
+                for (int w0 = 0; w0 < QPV; w0 += RING) {
+#pragma unroll
+                    for (int i = 0; i < RING; i++) {
+                        const int w = w0 + i;
+                        const uint32_t x0 = r0[i], x1 = r1[i];
+                        const int wn = w + RING;
+                        if (wn < QPV) {
+                            r0[i] = base[wn * 128];
+                            r1[i] = base[wn * 128 + 64];
+                        } else {
+                            r0[i] = next[(wn - QPV) * 128];
+                            r1[i] = next[(wn - QPV) * 128 + 64];
+                        }
+                        if (w == 0) mfma_word<NQT, QPV, true>(x0, x1, 0, slane, sB, acc, pf);
+                        else mfma_word<NQT, QPV, false>(x0, x1, w, slane, sB, acc, pf);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
                 }
             }
             // Every accumulator is consumed here (an empty asm reading one element
