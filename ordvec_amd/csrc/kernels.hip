@@ -1530,23 +1530,21 @@ constexpr int kUnitScale = 0x7f7f7f7f;      // E8M0 127 = 2^0 in every byte
 // One code word (64 coordinates) of documents col and col + 32 (this lane's
 // 32-bit halves x0, x1) against every query tile.
 // FIRST: the accumulators start from zero (inline constant), no clearing pass.
-// pf: with two or more tiles, the fragments of tiles 0 and 1 for this word,
-// read during the previous word; this word reads the next word's (word 0
-// after the last) so that their LDS latency is covered by this word's MFMAs.
+// pf: with two or more tiles, every tile's fragment for this slot, read
+// during the previous slot; this slot reads the next slot's (slot 0 after the
+// last) so that their LDS latency is covered by this slot's MFMAs.
 template <int NQT, int QPV, bool FIRST>
 __device__ __forceinline__ void mfma_word(uint32_t x0, uint32_t x1, int w, uint32_t lane,
-                                          const uint4* __restrict__ sB, ovx_v16f (&acc)[2][NQT], uint4 (&pf)[2]) {
+                                          const uint4* __restrict__ sB, ovx_v16f (&acc)[2][NQT], uint4 (&pf)[4]) {
     // All of the word's query fragments are requested before the document
     // expansion, so the VALU work covers part of the LDS latency.
     uint4 bq[NQT];
     if constexpr (NQT >= 2) {
-        bq[0] = pf[0];
-        bq[1] = pf[1];
 #pragma unroll
-        for (int t = 2; t < NQT; t++) bq[t] = sB[(t * QPV + w) * 64 + lane];
+        for (int t = 0; t < NQT; t++) bq[t] = pf[t];
         const int wn = (w + 1) % QPV;
-        pf[0] = sB[wn * 64 + lane];
-        pf[1] = sB[(QPV + wn) * 64 + lane];
+#pragma unroll
+        for (int t = 0; t < NQT; t++) pf[t] = sB[(t * QPV + wn) * 64 + lane];
     } else {
 #pragma unroll
         for (int t = 0; t < NQT; t++) bq[t] = sB[(t * QPV + w) * 64 + lane];
@@ -1730,10 +1728,10 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
                 }
             }
         }
-        uint4 pf[2] = {};  // tiles 0 and 1 of the next word (mfma_word reads them one word ahead)
+        uint4 pf[4] = {};  // every tile's fragment of the next slot (mfma_word reads them one slot ahead)
         if constexpr (NQT >= 2) {
-            pf[0] = sB[lane];
-            pf[1] = sB[QPV * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < NQT; t++) pf[t] = sB[t * QPV * 64 + lane];
         }
         for (uint32_t b = b_begin + wave; b < b_end; b += NWAVE) {
             ovx_v16f acc[2][NQT];
@@ -1895,7 +1893,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
         uint32_t slane = lane;
         asm volatile("" : "+v"(slane));
         ovx_v16f acc[2][1];
-        uint4 pf[2] = {};  // unused with one tile
+        uint4 pf[4] = {};  // unused with one tile
         const uint32_t* base = words32 + ((uint64_t)b * QPV * 64 + col) * 2 + g;
         uint32_t x0[QPV], x1[QPV];
 #pragma unroll
