@@ -565,14 +565,16 @@ std::vector<uint32_t> read_qstat(const uint32_t* dstat, uint64_t nq) {
 
 constexpr uint64_t kEarlyStatusQueries = 64;
 
-// The same words, copied back at this point of the stream; qstat_wait blocks
-// until that copy (not the work enqueued after it) has finished.
+// The same words, copied back from this point of the stream on the side
+// stream, so that the work enqueued after it does not wait for the copy;
+// qstat_wait blocks until that copy has finished.
 uint32_t* qstat_copy_async(const uint32_t* dstat, uint64_t nq) {
     Ctx& c = cur();
     uint32_t* h = c.pinned_u32(nq);
-    d2h(h, dstat, nq * 4);
+    hipStream_t side = c.fork();
+    OVX_HIP(hipMemcpyAsync(h, dstat, nq * 4, hipMemcpyDeviceToHost, side));
     if (!c.status_ev) OVX_HIP(hipEventCreateWithFlags(&c.status_ev, hipEventDisableTiming));
-    OVX_HIP(hipEventRecord(c.status_ev, c.stream));
+    OVX_HIP(hipEventRecord(c.status_ev, side));
     return h;
 }
 
@@ -652,7 +654,8 @@ size_t stage1_bytes(const SbIndex* ix, uint64_t nq, uint64_t m_eff) {
 
 Stage1Bufs stage1_dev(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t m_eff, uint32_t* cand,
                       uint32_t* cand_h, uint32_t* qstat, Carver& cv, const PrepFuse* prep = nullptr,
-                      bool select = true, hipEvent_t before_scan = nullptr, const FastTune* tune = nullptr) {
+                      bool select = true, hipEvent_t before_scan = nullptr, const FastTune* tune = nullptr,
+                      bool as_set = false) {
     Stage1Bufs b{};
     b.qbits = cv.take<uint64_t>(nq * ix->qpv);
     b.plan = plan_stage1_fast(ix->n, ix->dim, nq, m_eff);
@@ -660,7 +663,7 @@ Stage1Bufs stage1_dev(const SbIndex* ix, const float* dq, uint64_t nq, uint64_t 
     void* base = cv.take<char>(stage1_fast_bytes(b.plan, nq));
     carve_stage1_fast(b.plan, nq, base, &b.work);
     launch_stage1_fast(ix->words, ix->n, ix->dim, dq, nq, m_eff, b.plan, b.work, b.qbits, cand, cand_h, qstat,
-                       ix->ntop, cur().stream, prep, select, before_scan);
+                       ix->ntop, cur().stream, prep, select, before_scan, as_set);
     return b;
 }
 
@@ -1236,7 +1239,7 @@ void two_stage_plain(const SbIndex* sb, const RqIndex* ix, const float* queries,
     hipStream_t s = c.stream;
     const PrepFuse prep{dq, qhat, off, nullptr, d, ix->bits, tier, inv_norm};
     launch_stage1_plain(sb->words, sb->n, d, dq, p, hist, done_head, thr, cnt, skeys, qbits, qstat, prep, s);
-    launch_s1_tail(skeys, cnt, p.cap, m_eff, ix->codes, d, ix->bits, tier, inv_norm, qhat, off, ix->n, 1,
+    launch_s1_tail(skeys, cnt, thr, p.cap, m_eff, ix->codes, d, ix->bits, tier, inv_norm, qhat, off, ix->n, 1,
                    (uint32_t)out_k, ds, di, nsplit, rwork, done_tail, qstat, hstat, s);
     if (!dev) {
         d2h(out_scores, ds, out_k * 4);
@@ -1322,7 +1325,7 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
         uint32_t* zw = c.zero_words(kZwTailDone + nq);
         uint32_t* hst = c.pinned_u32(nq);
         std::fill(hst, hst + nq, 0xffffffffu);
-        launch_s1_tail(b.work.keys, b.work.cnt, b.plan.cap, m_eff, ix->codes, d, ix->bits, tier, inv_norm, qhat, off,
+        launch_s1_tail(b.work.keys, b.work.cnt, b.work.thr, b.plan.cap, m_eff, ix->codes, d, ix->bits, tier, inv_norm, qhat, off,
                        ix->n, nq, (uint32_t)out_k, ds, di, nsplit, rwork, zw + kZwTailDone, qstat, hst, s);
         if (!dev) {
             d2h(out_scores, ds, nq * out_k * 4);
@@ -1342,7 +1345,8 @@ void two_stage_run(const SbIndex* sb, const RqIndex* ix, const float* queries, u
         launch_prepare_queries(dq, nq, d, ix->bits, tier, inv_norm, qhat, off, nullptr, cur().fork());
         prepped = cur().join_event();
     }
-    Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv, side_prep ? nullptr : &prep, true, prepped);
+    Stage1Bufs b = stage1_dev(sb, dq, nq, m_eff, cand, nullptr, qstat, cv, side_prep ? nullptr : &prep, true, prepped,
+                              nullptr, true);
     // Device pointers, batches of kEarlyStatusQueries or more: the status
     // words are final after stage 1, so the call waits for their copy only
     // and returns while the rerank runs (results complete in stream order,
@@ -1462,7 +1466,7 @@ void two_stage_noalloc(const SbIndex* sb, const RqIndex* ix, const float* querie
     hipStream_t s = cur().stream;
     if (cv.off + stage1_bytes(sb, nq, m_eff) != main_bytes) fail(kInternal, "allocation-free scratch plan out of step");
     const PrepFuse prep{queries, qhat, off, nullptr, d, ix->bits, tier, inv_norm};
-    Stage1Bufs b = stage1_dev(sb, queries, nq, m_eff, cand, nullptr, qstat, cv, &prep);
+    Stage1Bufs b = stage1_dev(sb, queries, nq, m_eff, cand, nullptr, qstat, cv, &prep, true, nullptr, nullptr, true);
     launch_rerank(ix->codes, d, ix->bits, tier, inv_norm, qhat, off, nq, nullptr, cand, m_eff, ix->n, nullptr,
                   (uint32_t)out_k, out_scores, out_ids, nsplit, rwork, s);
     OVX_HIP(hipMemcpyAsync(t_na_status, qstat, nq * 4, hipMemcpyDeviceToHost, s));
@@ -1619,7 +1623,7 @@ void ts_enqueue_stage1(TsBatch* b, const float* dq, uint64_t nq, TsBufs& u, cons
     }
     Carver cv = u.stage1;
     Stage1Bufs s1 = stage1_dev(b->sb, dq, nq, b->m_eff, u.cand, nullptr, u.qstat, cv, side ? nullptr : &prep, true,
-                               prepped, &sh.t);
+                               prepped, &sh.t, true);
     if (s1.qbits != u.qbits) fail(kInternal, "two-stage batch plan: buffer layout out of step");
     OVX_HIP(hipMemcpyAsync(b->hstat, u.qstat, nq * 4, hipMemcpyDeviceToHost, cur().stream));
 }

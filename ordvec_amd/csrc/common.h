@@ -242,10 +242,13 @@ struct PrepFuse {
     int tier;
     float inv_norm;
 };
+// as_set: the caller needs each query's top-m candidates as a set (cand in
+// no particular order, cand_h null), as the two-stage rerank does.
 void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
                         uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
                         uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s,
-                        const PrepFuse* prep = nullptr, bool select = true, hipEvent_t before_scan = nullptr);
+                        const PrepFuse* prep = nullptr, bool select = true, hipEvent_t before_scan = nullptr,
+                        bool as_set = false);
 
 // One-query two-stage path: stage 1 in two launches (head: sign words,
 // sampled histogram and threshold; then the vector-ALU scan with the rerank's
@@ -264,7 +267,8 @@ bool set_mfma_scan(int value);
 void launch_stage1_plain(const uint64_t* words, uint64_t n, uint32_t d, const float* query, const FastPlan& p,
                          uint32_t* hist, uint32_t* done, uint32_t* thr, uint32_t* cnt, uint64_t* keys,
                          uint64_t* qbits, uint32_t* qstat, const PrepFuse& prep, hipStream_t s);
-void launch_s1_tail(const uint64_t* keys, const uint32_t* cnt, uint32_t cap, uint64_t m_eff, const uint8_t* codes,
+void launch_s1_tail(const uint64_t* keys, const uint32_t* cnt, const uint32_t* thr, uint32_t cap, uint64_t m_eff,
+                    const uint8_t* codes,
                     uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat, const float* offset,
                     uint64_t n_docs, uint64_t nq, uint32_t out_k, float* out_s, int64_t* out_i, uint32_t nsplit,
                     void* work, uint32_t* done, uint32_t* qstat, uint32_t* hstat, hipStream_t s);

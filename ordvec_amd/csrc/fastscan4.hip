@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_fs4_scan(const Fs4IndexDesc* __restrict
                                                   uint32_t d, float inv_norm, uint32_t kcap,
                                                   uint64_t* __restrict__ partial) {
     constexpr int QT = 4 * QG;
-    extern __shared__ uint64_t smem[];
+    extern __shared__ __align__(16) uint64_t smem[];
     Fs4TopK* tk = reinterpret_cast<Fs4TopK*>(smem);
     uint32_t* slut = reinterpret_cast<uint32_t*>(tk + QT);
     __shared__ uint32_t s_tile;
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kFs4MfmaThreads, 1) void k_fs4_scan_mfma(
     const Fs4IndexDesc* __restrict__ indexes, const Fs4Tile* __restrict__ tiles, uint32_t ntiles,
     uint32_t* __restrict__ counter, const Fs4Group* __restrict__ groups, const uint4* __restrict__ frag,
     const float* __restrict__ bias_invq, uint32_t d, float inv_norm, uint32_t kcap, uint64_t* __restrict__ partial) {
-    extern __shared__ uint64_t smem[];
+    extern __shared__ __align__(16) uint64_t smem[];
     Fs4TopK* tk = reinterpret_cast<Fs4TopK*>(smem);
     uint4* sB = reinterpret_cast<uint4*>(tk + kFs4MfmaQT);
     __shared__ uint32_t s_tile;

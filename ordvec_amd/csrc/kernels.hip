@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_rank_encode(const float* __restrict__ x
                                                      uint16_t* __restrict__ ranks,
                                                      const uint32_t* __restrict__ list = nullptr,
                                                      const uint32_t* __restrict__ list_count = nullptr) {
-    extern __shared__ uint64_t smem[];
+    extern __shared__ __align__(16) uint64_t smem[];
     const uint32_t P = next_pow2(d);
     uint64_t* keys = smem;
     uint8_t* buck = reinterpret_cast<uint8_t*>(smem + P);
@@ -515,7 +515,7 @@ __global__ void k_rank_keys(const float* __restrict__ x, uint64_t count, uint32_
 __global__ __launch_bounds__(256) void k_rank_scatter(const uint64_t* __restrict__ sorted, uint64_t n, uint32_t d,
                                                       uint16_t* __restrict__ ranks, uint32_t bits,
                                                       uint8_t* __restrict__ codes) {
-    extern __shared__ uint8_t rowb[];
+    extern __shared__ __align__(16) uint8_t rowb[];
     for (uint64_t r = blockIdx.x; r < n; r += gridDim.x) {
         const uint64_t* sr = sorted + r * d;
         if (ranks) {
@@ -552,7 +552,7 @@ __global__ void k_eval_buckets(const uint16_t* __restrict__ ranks, uint64_t coun
 __global__ __launch_bounds__(256) void k_eval_keys(const uint8_t* __restrict__ buckets, uint64_t n, uint32_t d,
                                                    uint32_t bits, const float* __restrict__ qcent, float inv_norm_sq,
                                                    uint64_t* __restrict__ keys) {
-    extern __shared__ float eval_qc[];
+    extern __shared__ __align__(16) float eval_qc[];
     float* qc = eval_qc;
     const uint64_t q = blockIdx.y;
     const float half = (float)((1u << bits) - 1) / 2.0f;
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256) void k_sign_scan(const uint64_t* __restrict__ 
                                                    uint32_t blocks_per_seg, uint64_t n_pad, uint16_t* __restrict__ H,
                                                    uint16_t* __restrict__ hist_part, uint32_t nseg, uint32_t nbins,
                                                    uint32_t hwords, uint32_t ntop) {
-    extern __shared__ uint32_t lds_hist[];
+    extern __shared__ __align__(16) uint32_t lds_hist[];
     const uint32_t q0 = blockIdx.x * QT;
     const uint32_t tq = min((uint32_t)QT, nq - q0);
     const uint32_t seg = blockIdx.y;
@@ -687,7 +687,7 @@ __global__ __launch_bounds__(256) void k_sign_scan(const uint64_t* __restrict__ 
 __global__ __launch_bounds__(256) void k_stage1_threshold(const uint16_t* __restrict__ hist_part, uint32_t nseg,
                                                           uint32_t nbins, uint64_t m_eff, uint32_t* __restrict__ thr,
                                                           uint32_t* __restrict__ counters) {
-    extern __shared__ uint32_t cnt[];
+    extern __shared__ __align__(16) uint32_t cnt[];
     __shared__ uint32_t part[256];
     const uint64_t q = blockIdx.x;
     const uint16_t* hp = hist_part + q * nseg * nbins;
@@ -844,7 +844,7 @@ __global__ __launch_bounds__(256) void k_stage1_finalize(const uint16_t* __restr
 __global__ __launch_bounds__(256) void k_stage1_run_offsets(const uint16_t* __restrict__ hist_part, uint32_t nseg,
                                                             uint32_t nbins, const uint32_t* __restrict__ thr,
                                                             uint32_t* __restrict__ off) {
-    extern __shared__ uint32_t start[];
+    extern __shared__ __align__(16) uint32_t start[];
     __shared__ uint32_t part[256];
     const uint64_t q = blockIdx.x;
     const uint16_t* hp = hist_part + q * nseg * nbins;
@@ -898,7 +898,7 @@ __global__ __launch_bounds__(64) void k_stage1_scatter(const uint16_t* __restric
                                                        const uint32_t* __restrict__ thr,
                                                        const uint32_t* __restrict__ off, uint64_t m_eff,
                                                        uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_h) {
-    extern __shared__ uint32_t run[];
+    extern __shared__ __align__(16) uint32_t run[];
     const uint64_t q = blockIdx.y;
     const uint32_t seg = blockIdx.x;
     const uint32_t hstar = thr[q * 4 + 0];
@@ -1016,7 +1016,7 @@ __global__ __launch_bounds__(256) void k_topbits(const float* __restrict__ x, ui
                                                  uint64_t* __restrict__ out, uint64_t first_doc,
                                                  uint32_t* __restrict__ hist, uint32_t nbins,
                                                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ qstat) {
-    extern __shared__ uint64_t smem[];
+    extern __shared__ __align__(16) uint64_t smem[];
     const uint32_t P = next_pow2(d), qpv = d / 64;
     uint64_t* keys = smem;
     uint32_t* bits = reinterpret_cast<uint32_t*>(smem + P);
@@ -1126,7 +1126,7 @@ __global__ __launch_bounds__(256) void k_s1_sample(const uint64_t* __restrict__ 
                                                    uint32_t stride, uint32_t nsample, uint32_t blocks_per_seg,
                                                    uint32_t* __restrict__ hist, uint32_t nbins, uint32_t hwords,
                                                    uint32_t ntop) {
-    extern __shared__ uint32_t lds_hist[];
+    extern __shared__ __align__(16) uint32_t lds_hist[];
     const uint32_t q0 = blockIdx.x * QT;
     const uint32_t tq = min((uint32_t)QT, nq - q0);
     for (uint32_t i = threadIdx.x; i < QT * hwords; i += blockDim.x) lds_hist[i] = 0;
@@ -1228,7 +1228,7 @@ __global__ __launch_bounds__(256) void k_s1_head(const uint64_t* __restrict__ wo
                                                  uint32_t* __restrict__ thr, uint32_t* __restrict__ cnt,
                                                  uint64_t* __restrict__ qbits, uint32_t* __restrict__ qstat,
                                                  uint32_t* __restrict__ done) {
-    extern __shared__ uint32_t lds_h[];
+    extern __shared__ __align__(16) uint32_t lds_h[];
     __shared__ uint64_t s_qb[kSignMaxDim / 64];
     __shared__ int s_last;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1342,12 +1342,12 @@ __device__ __forceinline__ void s1_append_lds(bool pass, uint32_t acc, uint64_t 
 template <int WCH, int OP>
 __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ words, uint32_t nblk, uint32_t qpv,
                                                  uint64_t n, const uint64_t* __restrict__ qbits, uint32_t nq,
-                                                 uint32_t qt, uint32_t blocks_per_seg, const uint32_t* __restrict__ thr,
+                                                 uint32_t qt, uint32_t blocks_per_seg, uint32_t* __restrict__ thr,
                                                  uint32_t cap, uint64_t* __restrict__ keys, uint32_t* __restrict__ cnt,
                                                  uint32_t ntop, uint32_t wgl, PrepFuse pf, uint32_t prep_rows,
                                                  const uint32_t* __restrict__ hist, uint32_t nbins,
                                                  uint32_t s_target) {
-    extern __shared__ uint64_t s_wl[];
+    extern __shared__ __align__(16) uint64_t s_wl[];
     __shared__ uint32_t s_wn, s_slot[4], s_thr[4];
     // The first prep_rows rows of the grid prepare query y * gridDim.x + x
     // for the rerank (dispatched first, so they run beside the scan); the
@@ -1375,10 +1375,14 @@ __global__ __launch_bounds__(256) void k_s1_scan(const uint64_t* __restrict__ wo
     // at the end; otherwise each appending wave takes global positions.
     // hist != null (tiles of at most 4 queries): wave w derives query w's
     // threshold from the sample histogram (k_s1_threshold's rule) instead of
-    // reading thr.
+    // reading thr, and the first segment's workgroup stores it there for the
+    // select that follows.
     if (hist && wave < tq) {
         const uint32_t t = sample_threshold(hist + (uint64_t)(q0 + wave) * nbins, nbins, s_target, lane);
-        if (lane == 0) s_thr[wave] = t;
+        if (lane == 0) {
+            s_thr[wave] = t;
+            if (blockIdx.y == prep_rows) thr[q0 + wave] = t;
+        }
     }
     if (wgl) {
         if (threadIdx.x == 0) s_wn = 0;
@@ -1623,7 +1627,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_scan_mfma(const uint64_t
     constexpr uint32_t QW = 32 * NQT, NWAVE = kMfmaThreads / 64;
     constexpr int RING = QPV % 8 == 0 ? 8 : (QPV % 4 == 0 ? 4 : QPV);  // words in flight
     constexpr bool PAIR = QPV % 2 == 0;  // whole 64-bit words per lane (half the load instructions)
-    extern __shared__ uint4 sB[];  // [NQT][QPV][64]
+    extern __shared__ __align__(16) uint4 sB[];  // [NQT][QPV][64]
     __shared__ int s_base[QW];
     __shared__ uint32_t s_thr[QW];
     // Per-wave survivor lists (+ a spare slot per lane), sized by the tile
@@ -1859,7 +1863,7 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
     uint32_t stride, uint32_t nsample, uint32_t blocks_per_seg, uint32_t* __restrict__ hist, uint32_t nbins,
     uint32_t hwords, uint32_t s_target, uint32_t ntop) {
     constexpr uint32_t NWAVE = kMfmaThreads / 64;
-    extern __shared__ uint4 sB[];  // [QPV][64] query fragments, then [32][hwords] packed bins
+    extern __shared__ __align__(16) uint4 sB[];  // [QPV][64] query fragments, then [32][hwords] packed bins
     uint32_t* lds_hist = reinterpret_cast<uint32_t*>(sB + QPV * 64);
     __shared__ int s_base[32];
     const uint32_t q0 = blockIdx.x * 32;
@@ -2398,7 +2402,7 @@ __global__ __launch_bounds__(256) void k_rerank(const uint8_t* __restrict__ code
                                                 float* __restrict__ out_s, int64_t* __restrict__ out_i,
                                                 uint64_t* __restrict__ partial) {
     __shared__ TopKShared tk;
-    extern __shared__ float qs[];
+    extern __shared__ __align__(16) float qs[];
     const uint64_t qidx = blockIdx.x;
     const uint64_t q = qmap ? qmap[qidx] : qidx;
     const uint32_t split = blockIdx.y, nsplit = gridDim.y;
@@ -2545,46 +2549,207 @@ __device__ void wave_kmin(const uint64_t* a, uint32_t cnt, uint32_t k, uint64_t*
     }
 }
 
-// The take-th smallest (1-based) of cnt <= 256 keys that share their upper
-// 32 bits (one hamming value): a select over the low (document id) bits, most
-// significant first, by wave ballots.
-__device__ uint64_t wave_select_id(const uint64_t* a, uint32_t cnt, uint32_t take, uint32_t nbits, uint32_t lane) {
-    uint32_t id[4];
-    bool act[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const uint32_t i = lane + 64u * j;
-        act[j] = i < cnt;
-        id[j] = act[j] ? (uint32_t)(a[i] & 0xffffffffu) : 0u;
+// The top-m of one query's stage-1 survivors as a set (k_s1_select's rule,
+// without its order), for a 256-thread workgroup whose threads hold the keys
+// (hamming << 32 | id) in registers: thread t's slot j holds survivor
+// t + 256 j, kKeyMax past the count (count >= m_eff >= 1); slots from
+// `nslot` (wave-uniform) on hold no survivor and are not read; ids are below
+// 2^idbits and distances at most hcap (the scan's survivor threshold).  The
+// keys are histogrammed over kSelBins bins that follow the key order: bin =
+// (hamming - hmin) << s | id >> (idbits - s), clamped to kSelBins - 1, with
+// hmin the smallest survivor distance and s the most id bits for which the
+// distance range hmin..hcap still fits (so that a bin
+// holds one distance and a slice of the ids, and ties on one distance spread
+// over 2^s bins).  The cut bin is where the cumulative count reaches m_eff;
+// when it holds more keys than the top-m needs, its take-th smallest key is
+// the one whose rank among the bin's keys (gathered in tie_buf, each thread
+// counting the smaller ones) is take - 1.  A key is in the top-m iff its bin
+// is below the cut, or it is in the cut bin and not above that key: the
+// exact top-m by (hamming, id), as ids are unique.  Returns false (the query
+// takes the exact path) when the cut bin holds more than kTailTieCap keys.
+// Synchronises; every thread gets the same result.
+constexpr uint32_t kTailTieCap = 2048, kTailWaveK = 32;
+
+// Bits of the largest document id of an n-document index (1 to 32).
+__host__ __device__ inline uint32_t id_bits(uint64_t n) {
+    uint32_t b = 1;
+    while (b < 32 && (n - 1) >> b) b++;
+    return b;
+}
+
+struct S1CutShared {
+    uint32_t bh[kSelBins];
+    uint32_t wsum[4];
+    uint32_t cut, before, hmin, ntie, nout;
+    uint64_t cutkey;
+};
+
+struct S1Cut {
+    uint32_t hbase, sbits, ishift, bin;
+    uint64_t key;
+    __device__ uint32_t bin_of(uint64_t k) const {
+        const uint32_t dh = (uint32_t)(k >> 32) - hbase;
+        return dh >= (kSelBins >> sbits) ? kSelBins - 1 : (dh << sbits | (uint32_t)((k & 0xffffffffull) >> ishift));
     }
-    const uint64_t hi = a[0] & 0xffffffff00000000ull;
-    uint32_t r = take, prefix = 0;
-    for (int b = (int)nbits - 1; b >= 0; b--) {
-        uint32_t c0 = 0;
+    __device__ bool has(uint64_t k) const {
+        const uint32_t b = bin_of(k);
+        return k != kKeyMax && (b < bin || (b == bin && k <= key));
+    }
+};
+
+// Appends this wave's lanes with `in` set to list[*count ...] (one LDS
+// atomic per wave); returns the lane's slot (valid where `in`).
+__device__ __forceinline__ uint32_t wave_append(bool in, uint32_t* count, uint32_t lane) {
+    const uint64_t m = __ballot(in);
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    return base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <uint32_t KPT>
+__device__ bool s1_cut(const uint64_t (&kr)[KPT], uint32_t nslot, uint32_t m_eff, uint32_t idbits, uint32_t hcap,
+                       S1CutShared& sc, uint64_t* tie_buf, S1Cut& out) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x) sc.bh[i] = 0;
+    uint64_t kmin = kKeyMax;
 #pragma unroll
-        for (int j = 0; j < 4; j++) c0 += (uint32_t)__popcll(__ballot(act[j] && !((id[j] >> b) & 1u)));
-        const uint32_t want = r <= c0 ? 0u : 1u;
-        if (want) {
-            r -= c0;
-            prefix |= 1u << b;
+    for (uint32_t j = 0; j < KPT; j++)
+        if (j < nslot) kmin = min(kmin, kr[j]);
+    const uint32_t hmin = (uint32_t)(wave_min_u64(kmin) >> 32);
+    if (threadIdx.x == 0) {
+        sc.cutkey = kKeyMax;
+        sc.hmin = 0xffffffffu;
+        sc.ntie = 0;
+        sc.nout = 0;
+    }
+    __syncthreads();
+    if (lane == 0) atomicMin(&sc.hmin, hmin);
+    __syncthreads();
+    out.hbase = sc.hmin;  // count >= m_eff >= 1: some key is valid
+    // id bits per bin: the most with (hcap - hmin + 1) << s <= kSelBins
+    const uint32_t span = hcap >= out.hbase ? hcap - out.hbase + 1 : 1u;
+    uint32_t sb = 0;
+    while (sb < idbits && (span << (sb + 1)) <= kSelBins) sb++;
+    out.sbits = sb;
+    out.ishift = idbits - sb;
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++)
+        if (j < nslot && kr[j] != kKeyMax) atomicAdd(&sc.bh[out.bin_of(kr[j])], 1u);
+    __syncthreads();
+    // exclusive bin starts: 8 bins per thread, wave scans, then 4 wave totals
+    constexpr uint32_t BPT = kSelBins / 256;
+    uint32_t c[BPT], local = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < BPT; b++) {
+        c[b] = sc.bh[threadIdx.x * BPT + b];
+        local += c[b];
+    }
+    const uint32_t incl = wave_excl_scan(local, lane) + local;
+    if (lane == 63) sc.wsum[wave] = incl;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (uint32_t w = 0; w < wave; w++) wbase += sc.wsum[w];
+    uint32_t run = wbase + incl - local;
+    if (run < m_eff && run + local >= m_eff) {
+#pragma unroll
+        for (uint32_t b = 0; b < BPT; b++) {
+            if (run < m_eff && run + c[b] >= m_eff) {
+                sc.cut = threadIdx.x * BPT + b;
+                sc.before = run;
+            }
+            run += c[b];
         }
-#pragma unroll
-        for (int j = 0; j < 4; j++) act[j] = act[j] && ((id[j] >> b) & 1u) == want;
     }
-    return hi | prefix;
+    __syncthreads();
+    out.bin = sc.cut;
+    const uint32_t take = m_eff - sc.before, ncut = sc.bh[out.bin];
+    if (take < ncut) {
+        // the cut bin is split: its keys to LDS, and the take-th smallest
+        if (ncut > kTailTieCap) return false;
+#pragma unroll
+        for (uint32_t j = 0; j < KPT; j++)
+            if (j < nslot) {
+                const bool in = kr[j] != kKeyMax && out.bin_of(kr[j]) == out.bin;
+                const uint32_t pos = wave_append(in, &sc.ntie, lane);
+                if (in) tie_buf[pos] = kr[j];
+            }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < ncut; i += blockDim.x) {
+            const uint64_t k = tie_buf[i];
+            uint32_t rank = 0, t = 0;
+            for (; t + 4 <= ncut; t += 4)  // every lane reads the same words
+                rank += (tie_buf[t] < k) + (tie_buf[t + 1] < k) + (tie_buf[t + 2] < k) + (tie_buf[t + 3] < k);
+            for (; t < ncut; t++) rank += tie_buf[t] < k;
+            if (rank == take - 1) sc.cutkey = k;
+        }
+        __syncthreads();
+    }
+    out.key = sc.cutkey;
+    return true;
+}
+
+// Stage-1 select when the caller needs the top-m as a set (the two-stage
+// rerank, whose top-k breaks ties by document id, not by candidate
+// position): one 256-thread workgroup per query, s1_cut's rule, candidates
+// written in no particular order (each wave appends its keys as one run).
+// Queries whose survivors cannot prove exactness (fewer than m_eff, more
+// than cap, or a cut bin above kTailTieCap keys) get qstat bit 1, as from
+// k_s1_select.  Document ids are below 2^idbits.
+__global__ __launch_bounds__(256) void k_s1_pick(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ cnt,
+                                                 const uint32_t* __restrict__ thr, uint32_t cap, uint32_t m_eff,
+                                                 uint32_t idbits, uint32_t* __restrict__ cand,
+                                                 uint32_t* __restrict__ qstat) {
+    constexpr uint32_t KPT = kFastCap / 256;
+    __shared__ S1CutShared sc;
+    __shared__ uint64_t tie_buf[kTailTieCap];
+    const uint64_t q = blockIdx.x;
+    const uint32_t count = cnt[q];
+    if (count < m_eff || count > cap) {
+        if (threadIdx.x == 0) qstat[q] |= 2u;
+        return;
+    }
+    const uint32_t nslot = (count + 255) / 256, lane = threadIdx.x & 63;
+    const uint64_t* kq = keys + q * cap;
+    uint64_t kr[KPT];
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++) {
+        const uint32_t i = threadIdx.x + j * 256;
+        kr[j] = i < count ? kq[i] : kKeyMax;
+    }
+    S1Cut cut;
+    if (!s1_cut<KPT>(kr, nslot, m_eff, idbits, thr[q], sc, tie_buf, cut)) {
+        if (threadIdx.x == 0) qstat[q] |= 2u;
+        return;
+    }
+    // exactly m_eff keys qualify: each wave reserves its run, then writes it
+    uint32_t inm = 0, n = 0;  // inm bit j: slot j is in the top-m
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++)
+        if (j < nslot && cut.has(kr[j])) inm |= 1u << j;
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++)
+        if (j < nslot) n += (uint32_t)__popcll(__ballot((inm >> j) & 1u));
+    uint32_t base = 0;
+    if (lane == 0 && n) base = atomicAdd(&sc.nout, n);
+    base = __shfl(base, 0);
+    uint32_t* cq = cand + q * m_eff;
+#pragma unroll
+    for (uint32_t j = 0; j < KPT; j++)
+        if (j < nslot) {
+            const bool in = (inm >> j) & 1u;
+            const uint64_t m = __ballot(in);
+            const uint32_t pos =
+                base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (in && pos < m_eff) cq[pos] = (uint32_t)(kr[j] & 0xffffffffu);
+            base += (uint32_t)__popcll(m);
+        }
 }
 
 // Stage-1 select, stage-2 rerank and the top-k merge of a small two-stage
 // call in one launch: grid (nsplit, nq), 256 threads.
-//  * select (k_s1_select's rule): every workgroup of query q reads the
-//    survivor keys (hamming << 32 | id) into registers, histograms them by
-//    min(hamming - hmin, kSelBins - 1) (hmin: the smallest survivor
-//    distance, so that wide rows keep distinct bins), finds the cut bin where
-//    the cumulative count reaches m_eff, and, when the cut bin holds more keys
-//    than the top-m needs, the largest key of the bin that is still inside
-//    (a wave select over the ids, or a sort of the bin's keys in LDS).  A key
-//    is a candidate iff its bin is below the cut, or it is in the cut bin and
-//    not above that key: the exact top-m by (hamming, id).
+//  * select (s1_cut): every workgroup of query q reads the survivor keys
+//    into registers and finds the top-m set by (hamming, id).
 //  * rerank (k_rerank's scoring): workgroup `split` scores the candidates
 //    held by threads t with t mod nsplit == split, and keeps their top-k.
 //  * merge (k_merge_topk): the workgroup that arrives last at query q's
@@ -2599,11 +2764,10 @@ __device__ uint64_t wave_select_id(const uint64_t* a, uint32_t cnt, uint32_t tak
 // rows and counters pass between workgroups through device-scope atomics
 // (no device-scope fence, which would write back the XCD's L2).  Dynamic
 // LDS: s1_tail_lds(d) bytes (cut-bin keys, then q_hat).
-constexpr uint32_t kTailTieCap = 2048, kTailWaveK = 32;
 
 template <int TIER, int BITS>
 __global__ __launch_bounds__(256) void k_s1_tail(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ cnt,
-                                                 uint32_t cap, uint32_t m_eff, const uint8_t* __restrict__ codes,
+                                                 const uint32_t* __restrict__ thr, uint32_t cap, uint32_t m_eff, const uint8_t* __restrict__ codes,
                                                  uint32_t d, float inv_norm, const float* __restrict__ qhat,
                                                  const float* __restrict__ offset, uint64_t n_docs, uint32_t out_k,
                                                  float* __restrict__ out_s, int64_t* __restrict__ out_i,
@@ -2611,14 +2775,12 @@ __global__ __launch_bounds__(256) void k_s1_tail(const uint64_t* __restrict__ ke
                                                  uint32_t* __restrict__ qstat, uint32_t* __restrict__ hstat) {
     constexpr uint32_t KPT = kFastCap / 256;
     __shared__ TopKShared tk;  // block top-k (large k); otherwise the scored keys
-    __shared__ uint32_t bh[kSelBins];
+    __shared__ S1CutShared sc;
     __shared__ uint32_t s_list[kStage1MaxM];
     __shared__ uint64_t s_res[kTailWaveK];
-    __shared__ uint32_t wsum[4];
-    __shared__ uint32_t s_cut, s_before, s_nlist, s_hmin;
-    __shared__ uint64_t s_cutkey;
+    __shared__ uint32_t s_nlist;
     __shared__ int s_last;
-    extern __shared__ uint64_t tail_dyn[];  // [kTailTieCap] cut-bin keys, then q_hat (d floats)
+    extern __shared__ __align__(16) uint64_t tail_dyn[];  // [kTailTieCap] cut-bin keys, then q_hat (d floats)
     float* qs = reinterpret_cast<float*>(tail_dyn + kTailTieCap);
     const uint32_t split = blockIdx.x, nsplit = gridDim.x;
     const uint64_t q = blockIdx.y;
@@ -2626,7 +2788,7 @@ __global__ __launch_bounds__(256) void k_s1_tail(const uint64_t* __restrict__ ke
     // One round trip to memory: the count, every slot of the survivor
     // buffer (cap entries; those at or past the count are masked below) and
     // q_hat are requested together.
-    const uint32_t count = cnt[q];
+    const uint32_t count = cnt[q], hcap = thr[q];
     const uint64_t* kq = keys + q * cap;
     uint64_t kr[KPT];
 #pragma unroll
@@ -2638,97 +2800,21 @@ __global__ __launch_bounds__(256) void k_s1_tail(const uint64_t* __restrict__ ke
 #pragma unroll
         for (uint32_t j = 0; j < KPT; j++)
             if (threadIdx.x + j * 256 >= count) kr[j] = kKeyMax;
-        for (uint32_t i = threadIdx.x; i < kSelBins; i += blockDim.x) bh[i] = 0;
-        uint64_t kmin = kKeyMax;
-#pragma unroll
-        for (uint32_t j = 0; j < KPT; j++) kmin = min(kmin, kr[j]);
-        const uint32_t hmin = (uint32_t)(wave_min_u64(kmin) >> 32);
-        if (threadIdx.x == 0) {
-            s_nlist = 0;
-            s_cutkey = kKeyMax;
-            s_hmin = 0xffffffffu;
-        }
-        __syncthreads();
-        if (lane == 0) atomicMin(&s_hmin, hmin);
-        __syncthreads();
-        const uint32_t hbase = s_hmin;  // count >= m_eff >= 1: some key is valid
-        auto bin_of = [&](uint64_t key) { return min((uint32_t)(key >> 32) - hbase, kSelBins - 1); };
-#pragma unroll
-        for (uint32_t j = 0; j < KPT; j++)
-            if (kr[j] != kKeyMax) atomicAdd(&bh[bin_of(kr[j])], 1u);
-        __syncthreads();
-        // exclusive bin starts: 8 bins per thread, wave scans, then 4 wave totals
-        constexpr uint32_t BPT = kSelBins / 256;
-        uint32_t c[BPT], local = 0;
-#pragma unroll
-        for (uint32_t b = 0; b < BPT; b++) {
-            c[b] = bh[threadIdx.x * BPT + b];
-            local += c[b];
-        }
-        const uint32_t incl = wave_excl_scan(local, lane) + local;
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        uint32_t wbase = 0;
-        for (uint32_t w = 0; w < wave; w++) wbase += wsum[w];
-        uint32_t run = wbase + incl - local;
-        if (run < m_eff && run + local >= m_eff) {
-#pragma unroll
-            for (uint32_t b = 0; b < BPT; b++) {
-                if (run < m_eff && run + c[b] >= m_eff) {
-                    s_cut = threadIdx.x * BPT + b;
-                    s_before = run;
-                }
-                run += c[b];
-            }
-        }
-        __syncthreads();
-        const uint32_t cut = s_cut, take = m_eff - s_before, ncut = bh[cut];
-        if (take < ncut) {
-            // the cut bin is split: its keys to LDS, and the take-th smallest
-            if (ncut > kTailTieCap) {
-                fb = true;
-            } else {
-                // positions by a block scan of per-thread counts (no per-key atomics)
-                uint32_t mine = 0;
-#pragma unroll
-                for (uint32_t j = 0; j < KPT; j++) mine += (kr[j] != kKeyMax && bin_of(kr[j]) == cut) ? 1u : 0u;
-                uint32_t pos = block_excl_scan256(mine, wsum);
-#pragma unroll
-                for (uint32_t j = 0; j < KPT; j++)
-                    if (kr[j] != kKeyMax && bin_of(kr[j]) == cut) tail_dyn[pos++] = kr[j];
-                __syncthreads();
-                if (cut < kSelBins - 1 && ncut <= 256) {
-                    // one hamming value: select on the ids
-                    if (wave == 0) {
-                        const uint32_t nbits = n_docs > 1 ? 64u - (uint32_t)__clzll((long long)(n_docs - 1)) : 1u;
-                        const uint64_t ck = wave_select_id(tail_dyn, ncut, take, min(nbits, 32u), lane);
-                        if (lane == 0) s_cutkey = ck;
-                    }
-                } else {
-                    const uint32_t P = next_pow2(ncut);
-                    for (uint32_t i = ncut + threadIdx.x; i < P; i += blockDim.x) tail_dyn[i] = kKeyMax;
-                    __syncthreads();
-                    bitonic_sort(tail_dyn, P);
-                    if (threadIdx.x == 0) s_cutkey = tail_dyn[take - 1];
-                }
-                __syncthreads();
-            }
-        }
+        const uint32_t nslot = min(KPT, (count + 255) / 256);
+        S1Cut cut;
+        fb = !s1_cut<KPT>(kr, nslot, m_eff, id_bits(n_docs), hcap, sc, tail_dyn, cut);
         if (!fb) {
-            const uint64_t cutkey = s_cutkey;
             const bool take_mine = threadIdx.x % nsplit == split;
-            auto in = [&](uint64_t key) {
-                const uint32_t bn = bin_of(key);
-                return take_mine && key != kKeyMax && (bn < cut || (bn == cut && key <= cutkey));
-            };
-            uint32_t mine = 0;
-#pragma unroll
-            for (uint32_t j = 0; j < KPT; j++) mine += in(kr[j]) ? 1u : 0u;
-            uint32_t total = 0;
-            uint32_t pos = block_excl_scan256(mine, wsum, &total);
+            auto in = [&](uint64_t key) { return take_mine && cut.has(key); };
+            uint32_t inm = 0;  // bit j: slot j is a candidate of this workgroup
 #pragma unroll
             for (uint32_t j = 0; j < KPT; j++)
-                if (in(kr[j])) s_list[pos++] = (uint32_t)(kr[j] & 0xffffffffu);
+                if (j < nslot && in(kr[j])) inm |= 1u << j;
+            uint32_t total = 0;
+            uint32_t pos = block_excl_scan256((uint32_t)__popc(inm), sc.wsum, &total);
+#pragma unroll
+            for (uint32_t j = 0; j < KPT; j++)
+                if ((inm >> j) & 1u) s_list[pos++] = (uint32_t)(kr[j] & 0xffffffffu);
             if (threadIdx.x == 0) s_nlist = total;
         }
         __syncthreads();  // s_list and s_nlist complete (also orders qs)
@@ -2918,7 +3004,7 @@ __global__ __launch_bounds__(256) void k_rank_sym(const uint16_t* __restrict__ r
                                                   const uint16_t* __restrict__ qranks, float inv_norm_sq,
                                                   uint64_t per_seg, uint32_t k, uint64_t* __restrict__ partial) {
     __shared__ TopKShared tk;
-    extern __shared__ int32_t qc[];
+    extern __shared__ __align__(16) int32_t qc[];
     const uint64_t q = blockIdx.y, seg = blockIdx.x, nseg = gridDim.x;
     const int32_t m2 = (int32_t)d - 1;
     for (uint32_t p = threadIdx.x; p < d; p += blockDim.x) qc[p] = 2 * (int32_t)qranks[q * d + p] - m2;
@@ -2966,7 +3052,7 @@ __global__ __launch_bounds__(256) void k_rank_asym(const uint16_t* __restrict__ 
                                                    float inv_norm, float mean, uint64_t per_seg, uint32_t k,
                                                    uint64_t* __restrict__ partial) {
     __shared__ TopKShared tk;
-    extern __shared__ float qsh[];
+    extern __shared__ __align__(16) float qsh[];
     const uint64_t q = blockIdx.y, seg = blockIdx.x, nseg = gridDim.x;
     for (uint32_t p = threadIdx.x; p < d; p += blockDim.x) qsh[p] = qhat[q * d + p];
     const float shift = __fmul_rn(mean, qsum[q]);
@@ -3044,7 +3130,7 @@ __global__ void k_check_rq_rows(const uint8_t* __restrict__ codes, uint64_t n, u
 // a d-bit LDS bitmap; a value >= d or a bit already set marks the row.
 __global__ __launch_bounds__(256) void k_check_rank_rows(const uint16_t* __restrict__ ranks, uint64_t n, uint32_t d,
                                                          unsigned long long* __restrict__ first_bad) {
-    extern __shared__ uint32_t seen[];
+    extern __shared__ __align__(16) uint32_t seen[];
     __shared__ int s_bad;
     const uint32_t words = (d + 31) / 32;
     for (uint64_t r = blockIdx.x; r < n; r += gridDim.x) {
@@ -3181,7 +3267,7 @@ __global__ __launch_bounds__(256) void k_score_ids(const uint8_t* __restrict__ c
                                                    float inv_norm, const float* __restrict__ qhat,
                                                    const float* __restrict__ offset, const int64_t* __restrict__ ids,
                                                    uint32_t k, float* __restrict__ out, uint32_t* __restrict__ bad) {
-    extern __shared__ float qs[];
+    extern __shared__ __align__(16) float qs[];
     const uint64_t q = blockIdx.x;
     if (TIER == 2) stage_q_mix<BITS>(qs, qhat + q * d, d);
     else stage_q(qs, qhat + q * d, d);
@@ -3227,7 +3313,7 @@ __global__ __launch_bounds__(256) void k_score_keys(const uint8_t* __restrict__ 
                                                     const uint64_t* __restrict__ offsets,
                                                     const uint32_t* __restrict__ cands, uint64_t row_m,
                                                     uint64_t* __restrict__ keys) {
-    extern __shared__ float qs[];
+    extern __shared__ __align__(16) float qs[];
     const uint64_t q = blockIdx.y;
     if (TIER == 2) stage_q_mix<BITS>(qs, qhat + q * d, d);
     else stage_q(qs, qhat + q * d, d);
@@ -3314,7 +3400,7 @@ __global__ __launch_bounds__(256) void k_score_ids_bwd(const uint8_t* __restrict
                                                        uint32_t* __restrict__ bad) {
     constexpr uint32_t CPB = 8 / BITS, MASK = (1u << BITS) - 1;
     const float half = ((float)(1u << BITS) - 1.0f) / 2.0f;
-    extern __shared__ uint64_t smem[];
+    extern __shared__ __align__(16) uint64_t smem[];
     uint32_t* sid = reinterpret_cast<uint32_t*>(smem);
     float* sg = reinterpret_cast<float*>(sid + k);
     float* gs = sg + k;  // d floats
@@ -3414,7 +3500,7 @@ __global__ __launch_bounds__(256) void k_fullscan_fs(const uint8_t* __restrict__
                                                      float inv_norm, uint64_t blocks_per_seg, uint32_t k,
                                                      uint64_t* __restrict__ partial) {
     __shared__ TopKShared tk;
-    extern __shared__ uint8_t slut[];
+    extern __shared__ __align__(16) uint8_t slut[];
     const uint64_t q = blockIdx.y, seg = blockIdx.x, nseg = gridDim.x;
     const uint32_t pairs = d / 2;
     const uint8_t* lq = lut + q * (uint64_t)pairs * 16;
@@ -4141,7 +4227,7 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
 void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const float* queries, uint64_t nq,
                         uint64_t m_eff, const FastPlan& p, const FastWork& w, uint64_t* qbits, uint32_t* cand,
                         uint32_t* cand_h, uint32_t* qstat, uint32_t ntop, hipStream_t s, const PrepFuse* prep,
-                        bool select, hipEvent_t before_scan) {
+                        bool select, hipEvent_t before_scan, bool as_set) {
     if (nq == 0) return;
     const uint32_t qpv = d / 64;
     bool s1_prepped = false;
@@ -4192,8 +4278,12 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
     ktime_end(kFamSignScan, s);
     if (!select) return;  // the caller's tail selects from the survivor keys
     ktime_begin(kFamStage1Select, s);
-    hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(1024), 0, s, w.keys, w.cnt, p.cap, m_eff, cand, cand_h,
-                       qstat);
+    if (as_set && !cand_h)
+        hipLaunchKernelGGL(k_s1_pick, dim3((uint32_t)nq), dim3(256), 0, s, w.keys, w.cnt, w.thr, p.cap,
+                           (uint32_t)m_eff, id_bits(n), cand, qstat);
+    else
+        hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(1024), 0, s, w.keys, w.cnt, p.cap, m_eff, cand,
+                           cand_h, qstat);
     check_launch();
     ktime_end(kFamStage1Select, s);
 }
@@ -4250,7 +4340,8 @@ bool s1_tail_supported(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff, uint
 
 size_t s1_tail_lds(uint32_t d) { return (size_t)kTailTieCap * 8 + (size_t)d * 4; }
 
-void launch_s1_tail(const uint64_t* keys, const uint32_t* cnt, uint32_t cap, uint64_t m_eff, const uint8_t* codes,
+void launch_s1_tail(const uint64_t* keys, const uint32_t* cnt, const uint32_t* thr, uint32_t cap, uint64_t m_eff,
+                    const uint8_t* codes,
                     uint32_t d, uint32_t bits, int tier, float inv_norm, const float* qhat, const float* offset,
                     uint64_t n_docs, uint64_t nq, uint32_t out_k, float* out_s, int64_t* out_i, uint32_t nsplit,
                     void* work, uint32_t* done, uint32_t* qstat, uint32_t* hstat, hipStream_t s) {
@@ -4274,7 +4365,7 @@ void launch_s1_tail(const uint64_t* keys, const uint32_t* cnt, uint32_t cap, uin
     });
     ktime_begin(kFamRerank, s);
 #define OVX_TAIL(T, B, _)                                                                                          \
-    hipLaunchKernelGGL((k_s1_tail<T, B>), dim3(nsplit, (uint32_t)nq), dim3(256), lds, s, keys, cnt, cap,          \
+    hipLaunchKernelGGL((k_s1_tail<T, B>), dim3(nsplit, (uint32_t)nq), dim3(256), lds, s, keys, cnt, thr, cap,     \
                        (uint32_t)m_eff, codes, d, inv_norm, qhat, offset, n_docs, out_k, out_s, out_i,            \
                        static_cast<uint64_t*>(work), done, qstat, hstat)
     OVX_TIER_BITS_DISPATCH(OVX_TAIL, 0);
