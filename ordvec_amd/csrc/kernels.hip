@@ -4039,7 +4039,7 @@ static void launch_sample_wch(const uint64_t* words, uint64_t n, uint32_t qpv, c
     else launch_sample_qt<QT, 1, OP>(words, n, qpv, qbits, nq, p, w, ntop, s);
 }
 
-static bool use_mfma_scan(uint64_t nq, uint32_t qpv);
+static bool use_mfma_scan(uint64_t nq, uint32_t qpv, const FastPlan& p);
 // about one sample workgroup per CU: fewer leave CUs idle, more flush more
 // (each workgroup's local threshold sits higher) -- 256 measured best at 1M x 1024
 constexpr uint64_t kSampleWgs = 256;
@@ -4047,7 +4047,7 @@ constexpr uint64_t kSampleWgs = 256;
 template <int OP>
 static void launch_sample(const uint64_t* words, uint64_t n, uint32_t qpv, const uint64_t* qbits, uint64_t nq,
                           const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s) {
-    if (use_mfma_scan(nq, qpv) && qpv <= 16 && p.s_target != 0xffffffffu) {
+    if (use_mfma_scan(nq, qpv, p) && qpv <= 16) {
         // matrix-core sample: 32 queries per workgroup, >= 8 sampled blocks
         // (one per wave) per workgroup, about kSampleWgs workgroups
         const uint32_t nblk = (uint32_t)((n + 63) / 64);
@@ -4117,8 +4117,11 @@ static void set_sample_lds() {
 // beats the VALU scan (whose work grows with every query; at 20M x 1024 the
 // padded tile streams the codes at the same speed as a 1-query VALU pass).
 // Query fragments live in LDS (16 * d bytes per 32-query tile, <= 64 KiB).
+// Not when every document survives (an index of at most cap documents): a
+// block's survivors would overflow the workgroup's LDS lists, while the
+// vector-ALU scan appends to the query's buffer, which holds them all.
 constexpr uint64_t kMfmaMinQueries = 2;
-static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
+static bool use_mfma_scan(uint64_t nq, uint32_t qpv, const FastPlan& p) {
     // once per process, also when serving slots reach here concurrently
     static std::once_flag configured;
     std::call_once(configured, [] {
@@ -4130,19 +4133,20 @@ static bool use_mfma_scan(uint64_t nq, uint32_t qpv) {
         set_mfma_lds<2, 1>();
         (void)hipGetLastError();
     });
-    return g_mfma_scan && nq >= kMfmaMinQueries && (qpv == 16 || qpv == 12 || qpv == 8 || qpv == 4 || qpv == 2 || qpv == 1);
+    return g_mfma_scan && nq >= kMfmaMinQueries && p.s_target != 0xffffffffu &&
+           (qpv == 16 || qpv == 12 || qpv == 8 || qpv == 4 || qpv == 2 || qpv == 1);
 }
 
 // Whether the vector-ALU scan derives the thresholds itself (tiles of at most
 // 4 queries, one wave each), so that k_s1_threshold is not launched.
 static bool scan_fuses_threshold(uint64_t nq, uint32_t qpv, const FastPlan& p) {
-    return !use_mfma_scan(nq, qpv) && p.scan_qt <= 4;
+    return !use_mfma_scan(nq, qpv, p) && p.scan_qt <= 4;
 }
 
 // Whether launch_scan runs a PrepFuse on the scan's own workgroups.
 static bool scan_fuses_prep(uint64_t nq, uint32_t qpv, const FastPlan& p) {
     const uint64_t tiles = (nq + p.scan_qt - 1) / p.scan_qt;
-    return !use_mfma_scan(nq, qpv) && (uint64_t)p.nseg + (nq + tiles - 1) / tiles <= 65535;
+    return !use_mfma_scan(nq, qpv, p) && (uint64_t)p.nseg + (nq + tiles - 1) / tiles <= 65535;
 }
 
 template <int OP>
@@ -4150,7 +4154,7 @@ static void launch_scan(const uint64_t* words, uint64_t n, uint32_t qpv, const u
                         const FastPlan& p, const FastWork& w, uint32_t ntop, hipStream_t s,
                         const PrepFuse* prep = nullptr) {
     const uint32_t nblk = (uint32_t)((n + 63) / 64);
-    if (use_mfma_scan(nq, qpv)) {
+    if (use_mfma_scan(nq, qpv, p)) {
         // Matrix-core scan: 32, 64 or 128 queries per workgroup of 8 waves,
         // the smallest tile that holds the batch: every extra query tile
         // re-reads the codes, and a large corpus does not stay in L2 between
@@ -4330,7 +4334,7 @@ void launch_stage1_plain(const uint64_t* words, uint64_t n, uint32_t d, const fl
 bool stage1_plain_supported(uint64_t n, uint32_t d, uint64_t m_eff, uint64_t out_k) {
     if (d % 64 || d > kSignMaxDim || m_eff < 1 || m_eff > kStage1MaxM || out_k < 1 || out_k > kTopKMax) return false;
     const FastPlan p = plan_stage1_fast(n, d, 1, m_eff);
-    return p.cap == kFastCap && !use_mfma_scan(1, d / 64) && scan_fuses_prep(1, d / 64, p);
+    return p.cap == kFastCap && !use_mfma_scan(1, d / 64, p) && scan_fuses_prep(1, d / 64, p);
 }
 
 bool s1_tail_supported(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff, uint64_t out_k) {

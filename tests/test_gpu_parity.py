@@ -365,6 +365,31 @@ def test_fast_stage1_path_needs_no_exact_fallback_on_partial_tiles(engine, oracl
         assert np.array_equal(got, oracle.sign_top_m(words, oracle.sign_encode(qs[:nq]), 1000, 8)), nq
 
 
+@pytest.mark.parametrize("n,m", [(4096, 100), (8192, 1000), (3000, 3000)])
+def test_small_index_batches_stay_on_the_fast_path(engine, oracle, n, m):
+    """An index of at most 8192 documents (the survivor buffer's size) admits
+    every document in stage 1; batches must still finish on the fast path
+    (the matrix-core scan's LDS survivor lists cannot hold a whole block of
+    survivors, so such plans take the vector-ALU scan) and match the oracle,
+    also with m equal to the corpus size."""
+    d = 1024
+    gen = S.MixtureNP(d, seed=31)
+    docs = gen.docs(n)
+    qs, _ = gen.queries(docs, 200)
+    sb = engine.SignBitmap(d)
+    rq = engine.RankQuant(d, 2)
+    engine.two_stage_add(sb, rq, docs)
+    words = sb.words()
+    e0 = engine.stage1_exact_queries()
+    got = sb.top_m_candidates_batched(qs, m)
+    s, i = engine.two_stage_search(sb, rq, qs, m, 10)
+    assert engine.stage1_exact_queries() == e0
+    assert np.array_equal(got, oracle.sign_top_m(words, oracle.sign_encode(qs), m, 8))
+    ws, wi = _two_stage_oracle(oracle, rq.codes(), words, qs, m, 10)
+    assert np.array_equal(i, wi)
+    assert np.array_equal(s.view(np.uint32), ws.view(np.uint32))
+
+
 def _two_stage_oracle(oracle, docs_codes, words, qs, m, k):
     dim = qs.shape[1]
     cand = oracle.sign_top_m(words, oracle.sign_encode(qs), m, 8)
