@@ -3969,17 +3969,24 @@ FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
         p.s_target = 0xffffffffu;
         p.stride = 1;
         p.nsample = 0;
+        p.sparse = 0;
     } else {
         uint64_t stride = std::min<uint64_t>(64, std::max<uint64_t>(1, nblk / 256));
         p.stride = (uint32_t)stride;
         p.nsample = (uint32_t)((nblk + stride - 1) / stride);
+        double expect;  // survivors per query the threshold aims at
         if (stride == 1) {
             p.s_target = (uint32_t)m_eff;  // the sample is the corpus: exact threshold
+            expect = (double)m_eff;
         } else {
             double sample_docs = std::min<double>((double)n, (double)p.nsample * 64.0);
             double target_full = std::min(2.5 * (double)m_eff, 0.5 * (double)(m_eff + p.cap));
             p.s_target = (uint32_t)std::ceil(target_full * sample_docs / (double)n) + 8;
+            expect = target_full;
         }
+        // Measured at 1024 queries on mixture data: one survivor in 12
+        // documents stays on the fast path, one in 10 overflows the lists.
+        p.sparse = (double)n >= 12.0 * expect ? 1u : 0u;
         uint64_t want = std::max<uint64_t>(1, (1024 + tiles - 1) / tiles);
         uint64_t nsseg = std::min<uint64_t>(std::max<uint64_t>(1, p.nsample), want);
         uint64_t sbps = (p.nsample + nsseg - 1) / nsseg;
@@ -4117,9 +4124,10 @@ static void set_sample_lds() {
 // beats the VALU scan (whose work grows with every query; at 20M x 1024 the
 // padded tile streams the codes at the same speed as a 1-query VALU pass).
 // Query fragments live in LDS (16 * d bytes per 32-query tile, <= 64 KiB).
-// Not when every document survives (an index of at most cap documents): a
-// block's survivors would overflow the workgroup's LDS lists, while the
-// vector-ALU scan appends to the query's buffer, which holds them all.
+// Only when survivors are sparse (plan.sparse): with many survivors per block
+// (every document, in an index of at most cap documents) the workgroup's LDS
+// lists overflow and the tile goes to the exact path, while the vector-ALU
+// scan appends to the query's buffer, which holds up to cap of them.
 constexpr uint64_t kMfmaMinQueries = 2;
 static bool use_mfma_scan(uint64_t nq, uint32_t qpv, const FastPlan& p) {
     // once per process, also when serving slots reach here concurrently
@@ -4133,7 +4141,7 @@ static bool use_mfma_scan(uint64_t nq, uint32_t qpv, const FastPlan& p) {
         set_mfma_lds<2, 1>();
         (void)hipGetLastError();
     });
-    return g_mfma_scan && nq >= kMfmaMinQueries && p.s_target != 0xffffffffu &&
+    return g_mfma_scan && nq >= kMfmaMinQueries && p.sparse &&
            (qpv == 16 || qpv == 12 || qpv == 8 || qpv == 4 || qpv == 2 || qpv == 1);
 }
 

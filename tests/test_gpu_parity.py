@@ -112,10 +112,12 @@ def test_empty_and_zero_k_shapes(engine):
 
 @pytest.mark.parametrize("dim,n,nq,m", [(1024, 50_000, 16, 1000), (768, 20_011, 40, 300), (64, 5000, 3, 64),
                                         (128, 777, 9, 1000), (1024, 4096, 1, 4096), (192, 3001, 20, 1),
-                                        # matrix-core scan shapes (>= 32 queries): full and partial query
-                                        # tiles, a ragged last document block, tiny corpora
-                                        (1024, 30_001, 100, 1000), (64, 5000, 70, 64), (192, 7001, 33, 300),
-                                        (1024, 65, 64, 10), (512, 12_345, 32, 500),
+                                        # matrix-core scan shapes (>= 32 queries, over 8192 documents with
+                                        # sparse survivors): full and partial query tiles, a ragged last
+                                        # document block, one- and eight-word rows
+                                        (1024, 30_001, 100, 1000), (64, 40_000, 70, 64), (512, 12_345, 32, 500),
+                                        # batches over tiny corpora (every document survives: VALU scan)
+                                        (64, 5000, 70, 64), (192, 7001, 33, 300), (1024, 65, 64, 10),
                                         # VALU scan without a word-count specialisation
                                         (2048, 9000, 5, 500), (320, 10_000, 1, 700)])
 def test_sign_top_m_exact(engine, oracle, dim, n, nq, m):
@@ -365,13 +367,14 @@ def test_fast_stage1_path_needs_no_exact_fallback_on_partial_tiles(engine, oracl
         assert np.array_equal(got, oracle.sign_top_m(words, oracle.sign_encode(qs[:nq]), 1000, 8)), nq
 
 
-@pytest.mark.parametrize("n,m", [(4096, 100), (8192, 1000), (3000, 3000)])
-def test_small_index_batches_stay_on_the_fast_path(engine, oracle, n, m):
-    """An index of at most 8192 documents (the survivor buffer's size) admits
-    every document in stage 1; batches must still finish on the fast path
-    (the matrix-core scan's LDS survivor lists cannot hold a whole block of
-    survivors, so such plans take the vector-ALU scan) and match the oracle,
-    also with m equal to the corpus size."""
+@pytest.mark.parametrize("n,m", [(4096, 100), (8192, 1000), (3000, 3000), (9000, 1000), (50000, 2000)])
+def test_dense_survivor_batches_stay_on_the_fast_path(engine, oracle, n, m):
+    """Batches whose stage 1 keeps many survivors per document block: an index
+    of at most 8192 documents (the survivor buffer's size) admits every
+    document, and m near n / 10 keeps about one in ten.  The matrix-core
+    scan's LDS survivor lists cannot hold that many per block, so such plans
+    take the vector-ALU scan; every query must finish on the fast path and
+    match the oracle, also with m equal to the corpus size."""
     d = 1024
     gen = S.MixtureNP(d, seed=31)
     docs = gen.docs(n)
