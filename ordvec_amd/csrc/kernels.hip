@@ -1929,41 +1929,33 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
         }
     }
     __syncthreads();
-    // One wave per query: local threshold by a wave prefix over bin chunks,
-    // then the bins up to it go to the global histogram.
-    const uint32_t per = (nbins + 63) / 64;
+    // One wave per query: the packed bins in chunks of 128 (one word, two
+    // bins, per lane) from distance 0 up, each chunk's bins going to the
+    // global histogram, until the chunk where the local count reaches
+    // s_target; there only the bins up to that local threshold go.
     for (uint32_t qi = wave; qi < 32 && q0 + qi < nq; qi += NWAVE) {
         const uint32_t* hq = lds_hist + qi * hwords;
-        const uint32_t lo = min(nbins, lane * per), hi = min(nbins, lo + per);
-        uint32_t local = 0;
-        for (uint32_t bin = lo; bin < hi; bin++) local += (hq[bin >> 1] >> ((bin & 1) * 16)) & 0xffffu;
-        uint32_t incl = local;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t v = __shfl_up(incl, off);
-            if ((int)lane >= off) incl += v;
-        }
-        // first lane whose inclusive count reaches s_target (64 = never)
-        const uint64_t reach = __ballot(incl >= s_target);
-        uint32_t last = nbins - 1;
-        if (reach) {
-            const uint32_t l0 = (uint32_t)__builtin_ctzll(reach);
-            uint32_t cut = 0;
-            if (lane == l0) {
-                uint32_t cum = incl - local;
-                cut = hi - 1;
-                for (uint32_t bin = lo; bin < hi; bin++) {
-                    cum += (hq[bin >> 1] >> ((bin & 1) * 16)) & 0xffffu;
-                    if (cum >= s_target) {
-                        cut = bin;
-                        break;
-                    }
-                }
+        uint32_t* hg = hist + (uint64_t)(q0 + qi) * nbins;
+        uint32_t run = 0;  // local count below this chunk
+        for (uint32_t w0 = 0; w0 < hwords; w0 += 64) {
+            const uint32_t w = w0 + lane;
+            const uint32_t word = w < hwords ? hq[w] : 0u;
+            const uint32_t c0 = word & 0xffffu, c1 = word >> 16;  // bins 2w, 2w + 1
+            const uint32_t ex = wave_excl_scan(c0 + c1, lane);
+            const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)(ex + c0 + c1), 63);
+            uint32_t last = 0xffffffffu;  // every bin of the chunk
+            if (run + tot >= s_target) {
+                // the local threshold is in this chunk: the first lane whose
+                // count reaches s_target, then its first or second bin
+                const uint64_t m = __ballot(run + ex + c0 + c1 >= s_target);
+                const int l0 = (int)__builtin_ctzll(m);
+                const uint32_t cut = run + ex + c0 >= s_target ? 2 * w : 2 * w + 1;
+                last = (uint32_t)__shfl((int)cut, l0);
             }
-            last = __shfl(cut, (int)l0);
-        }
-        for (uint32_t bin = lane; bin <= last; bin += 64) {
-            const uint32_t c = (hq[bin >> 1] >> ((bin & 1) * 16)) & 0xffffu;
-            if (c) atomicAdd(&hist[(uint64_t)(q0 + qi) * nbins + bin], c);
+            if (c0 && 2 * w <= last) atomicAdd(&hg[2 * w], c0);
+            if (c1 && 2 * w + 1 <= last) atomicAdd(&hg[2 * w + 1], c1);
+            if (last != 0xffffffffu) break;
+            run += tot;
         }
     }
 }
