@@ -17,6 +17,7 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -230,6 +231,11 @@ def main():
     warm_ids = [ov.two_stage_search(sb, rq, plain[i:i + 1], m, k)[1] for i in range(P)]
     torch.cuda.synchronize()
     del warm_ids
+    # Python's cyclic garbage collector paused over the timed calls, as timeit
+    # does: one full collection in this process takes tens of ms and would
+    # land inside one of the 1000 calls
+    gc.collect()
+    gc.disable()
     tp0 = time.perf_counter()
     plain_ids = []
     for i in range(P):
@@ -239,6 +245,7 @@ def main():
         lat.append(time.perf_counter() - a)
         plain_ids.append(i_)
     tp1 = time.perf_counter()
+    gc.enable()
     lat_ms = np.array(lat) * 1e3
     plain_qps = P / (tp1 - tp0)
     log(f"plain: {plain_qps:,.0f} QPS, p50 {np.percentile(lat_ms, 50):.3f} ms, p99 {np.percentile(lat_ms, 99):.3f} ms, "
@@ -276,7 +283,7 @@ def main():
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "plain": {"qps": round(plain_qps, 1), "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
                   "p99_ms": round(float(np.percentile(lat_ms, 99)), 4), "mean_ms": round(float(lat_ms.mean()), 4),
-                  "max_ms": round(float(lat_ms.max()), 3), "queries": P,
+                  "max_ms": round(float(lat_ms.max()), 3), "queries": P, "python_gc": "paused over the timed calls",
                   "batched_over_plain": round(qps / plain_qps, 2)},
         "roofline": {"kernel": "k_s1_scan_mfma (stage-1 scan: Hamming distances as FP4 MFMA dot products)",
                      "bound": "mfma", "achieved": round(mfma_tops, 1), "peak": MFMA_FP4_PEAK_TFLOPS,
