@@ -393,6 +393,25 @@ def test_dense_survivor_batches_stay_on_the_fast_path(engine, oracle, n, m):
     assert np.array_equal(s.view(np.uint32), ws.view(np.uint32))
 
 
+def test_large_m_batches(engine, oracle, mixture_200k):
+    """m near the 4096 limit leaves little room between m and the 8192-key
+    survivor buffer; the sampled threshold aims at the geometric mean of the
+    two.  Up to m = 3000 every query of a 256-query batch finishes on the fast
+    path; at m = 4000 this corpus's Hamming bins near the threshold hold
+    thousands of documents each (the count jumps past m and past 8192 in one
+    distance step for most queries), so those queries take the exact path,
+    and the candidates must match the oracle either way."""
+    docs, qs, sb, _ = mixture_200k
+    words = sb.words()
+    qb = oracle.sign_encode(qs[:256])
+    for m in (2500, 3000, 4000):
+        e0 = engine.stage1_exact_queries()
+        got = sb.top_m_candidates_batched(qs[:256], m)
+        if m <= 3000:
+            assert engine.stage1_exact_queries() == e0, m
+        assert np.array_equal(got, oracle.sign_top_m(words, qb, m, 8)), m
+
+
 def _two_stage_oracle(oracle, docs_codes, words, qs, m, k):
     dim = qs.shape[1]
     cand = oracle.sign_top_m(words, oracle.sign_encode(qs), m, 8)
