@@ -1073,7 +1073,7 @@ void bm_subset(const SbIndex* ix, const float* queries, uint64_t nq, const uint6
 // Device bytes the exact recomputation of nf queries needs.
 size_t stage1_fix_bytes(const SbIndex* ix, uint64_t nf, uint64_t m_eff) {
     Stage1Plan plan = plan_stage1(ix->n, ix->dim, nf);
-    return al(nf * ix->qpv * 8) + al(stage1_work_bytes(plan, nf, m_eff)) + 2 * al(nf * m_eff * 4);
+    return al(nf * 4) + al(nf * ix->qpv * 8) + al(stage1_work_bytes(plan, nf, m_eff)) + 2 * al(nf * m_eff * 4);
 }
 
 // Exact recomputation of the nf queries listed in fl (histogram pipeline) in
@@ -1084,18 +1084,18 @@ void stage1_fix_into(const SbIndex* ix, const uint64_t* qbits, const uint32_t* f
     const uint64_t qpv = ix->qpv;
     Stage1Plan plan = plan_stage1(ix->n, ix->dim, nf);
     Carver cv{base};
+    uint32_t* didx = cv.take<uint32_t>(nf);
     uint64_t* qb = cv.take<uint64_t>(nf * qpv);
-    for (uint64_t i = 0; i < nf; i++) d2d(qb + i * qpv, qbits + (uint64_t)fl[i] * qpv, qpv * 8);
+    h2d(didx, fl, nf * 4);
+    gather_rows_u64(qb, qbits, didx, nf, (uint32_t)qpv, cur().stream);
     Stage1Work w{};
     carve_stage1(plan, nf, m_eff, cv.take<char>(stage1_work_bytes(plan, nf, m_eff)), &w);
     uint32_t* cc = cv.take<uint32_t>(nf * m_eff);
     uint32_t* ch = cv.take<uint32_t>(nf * m_eff);
     launch_stage1(ix->words, ix->n, ix->dim, qb, nf, m_eff, plan, w, cc, cand_h ? ch : nullptr, ix->ntop,
                   cur().stream);
-    for (uint64_t i = 0; i < nf; i++) {
-        d2d(cand + (uint64_t)fl[i] * m_eff, cc + i * m_eff, m_eff * 4);
-        if (cand_h) d2d(cand_h + (uint64_t)fl[i] * m_eff, ch + i * m_eff, m_eff * 4);
-    }
+    scatter_rows_u32(cand, cc, didx, nf, (uint32_t)m_eff, cur().stream);
+    if (cand_h) scatter_rows_u32(cand_h, ch, didx, nf, (uint32_t)m_eff, cur().stream);
 }
 
 // The same in the context's second workspace, in chunks of queries whose

@@ -95,6 +95,12 @@ void launch_score_ids(const uint8_t* codes, uint64_t n_docs, uint32_t d, uint32_
 void launch_score_ids_backward(const uint8_t* codes, uint64_t n_docs, uint32_t d, uint32_t bits, float inv_norm,
                                const float* x, uint64_t nq, const int64_t* ids, uint32_t k, const float* grad,
                                float* dx, uint32_t* bad, hipStream_t s);
+// dst row i = src row idx[i] / dst row idx[i] = src row i, rows of `row`
+// elements, in one launch.
+void gather_rows_u64(uint64_t* dst, const uint64_t* src, const uint32_t* idx, uint64_t nrows, uint32_t row,
+                     hipStream_t s);
+void scatter_rows_u32(uint32_t* dst, const uint32_t* src, const uint32_t* idx, uint64_t nrows, uint32_t row,
+                      hipStream_t s);
 void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bits, int tier, float inv_norm,
                             float* qhat, float* offset, uint32_t* qstat, hipStream_t s);
 void launch_sign_encode(const float* x, uint64_t n, uint32_t d, uint64_t* out, uint64_t first_doc, bool interleaved,

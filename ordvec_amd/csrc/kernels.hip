@@ -3549,6 +3549,41 @@ void launch_check_finite(const float* x, uint64_t count, uint32_t* flag, hipStre
     check_launch();
 }
 
+// Row gather / scatter by an index list, for the exact recomputation of
+// flagged queries (one launch each instead of one copy per row): gather
+// writes dst row i = src row idx[i], scatter writes dst row idx[i] = src row
+// i; rows of `row` elements.
+template <typename T, bool GATHER>
+__global__ __launch_bounds__(256) void k_rows_by_index(T* __restrict__ dst, const T* __restrict__ src,
+                                                       const uint32_t* __restrict__ idx, uint64_t nrows, uint32_t row) {
+    const uint64_t total = nrows * row;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e / row, j = e % row, o = (uint64_t)idx[i] * row + j;
+        if (GATHER) dst[e] = src[o];
+        else dst[o] = src[e];
+    }
+}
+
+template <typename T, bool GATHER>
+static void launch_rows_by_index(T* dst, const T* src, const uint32_t* idx, uint64_t nrows, uint32_t row,
+                                 hipStream_t s) {
+    const uint64_t total = nrows * row;
+    if (total == 0) return;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL((k_rows_by_index<T, GATHER>), dim3(grid), dim3(256), 0, s, dst, src, idx, nrows, row);
+    check_launch();
+}
+
+void gather_rows_u64(uint64_t* dst, const uint64_t* src, const uint32_t* idx, uint64_t nrows, uint32_t row,
+                     hipStream_t s) {
+    launch_rows_by_index<uint64_t, true>(dst, src, idx, nrows, row, s);
+}
+
+void scatter_rows_u32(uint32_t* dst, const uint32_t* src, const uint32_t* idx, uint64_t nrows, uint32_t row,
+                      hipStream_t s) {
+    launch_rows_by_index<uint32_t, false>(dst, src, idx, nrows, row, s);
+}
+
 void launch_prepare_queries(const float* q, uint64_t nq, uint32_t d, uint32_t bits, int tier, float inv_norm,
                             float* qhat, float* offset, uint32_t* qstat, hipStream_t s) {
     if (nq == 0) return;
