@@ -731,7 +731,9 @@ __global__ __launch_bounds__(256) void k_stage1_threshold(const uint16_t* __rest
 
 // K3: stream H[q] in segments; keys below h* go to `below` (order fixed later
 // by the sort), boundary-tier ids to `ties` (capacity kTieCap).  Appends are
-// wave-aggregated: one LDS-free global atomic per wave and element slot.
+// wave-aggregated over a thread's 8 elements: at most one global atomic per
+// wave and list per 2048 documents (one per element slot had serialised the
+// atomics' round trips).
 __global__ __launch_bounds__(256) void k_stage1_compact(const uint16_t* __restrict__ H, uint64_t n, uint64_t n_pad,
                                                         uint64_t per_seg, const uint32_t* __restrict__ thr,
                                                         uint32_t* __restrict__ counters, uint64_t m_eff,
@@ -748,28 +750,35 @@ __global__ __launch_bounds__(256) void k_stage1_compact(const uint16_t* __restri
     for (uint64_t base = begin + (uint64_t)threadIdx.x * 8; base < end + 0; base += (uint64_t)blockDim.x * 8) {
         uint4 v = *reinterpret_cast<const uint4*>(row + base);
         uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        uint64_t mb[8], mt[8];
+        uint32_t nb = 0, nt = 0;
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            uint64_t idx = base + e;
-            uint32_t h = (w[e >> 1] >> ((e & 1) * 16)) & 0xffffu;
-            bool ok = idx < end;
-            bool is_below = ok && h < hstar;
-            bool is_tie = ok && h == hstar;
-            uint64_t mb = __ballot(is_below);
-            uint64_t mt = __ballot(is_tie);
-            if (mb) {
-                uint32_t basepos = 0;
-                if (lane == __ffsll((long long)mb) - 1) basepos = atomicAdd(&counters[q * 2 + 0], (uint32_t)__popcll(mb));
-                basepos = __shfl(basepos, __ffsll((long long)mb) - 1);
-                if (is_below) bq[basepos + __popcll(mb & lt_mask)] = ((uint64_t)h << 32) | idx;
+            const uint32_t h = (w[e >> 1] >> ((e & 1) * 16)) & 0xffffu;
+            const bool ok = base + e < end;
+            mb[e] = __ballot(ok && h < hstar);
+            mt[e] = __ballot(ok && h == hstar);
+            nb += (uint32_t)__popcll(mb[e]);
+            nt += (uint32_t)__popcll(mt[e]);
+        }
+        uint32_t pb = 0, pt = 0;  // this wave's first positions in the two lists
+        if (lane == 0) {
+            if (nb) pb = atomicAdd(&counters[q * 2 + 0], nb);
+            if (nt) pt = atomicAdd(&counters[q * 2 + 1], nt);
+        }
+        pb = __shfl(pb, 0);
+        pt = __shfl(pt, 0);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const uint64_t idx = base + e;
+            const uint32_t h = (w[e >> 1] >> ((e & 1) * 16)) & 0xffffu;
+            if ((mb[e] >> lane) & 1) bq[pb + __popcll(mb[e] & lt_mask)] = ((uint64_t)h << 32) | idx;
+            if ((mt[e] >> lane) & 1) {
+                const uint32_t pos = pt + (uint32_t)__popcll(mt[e] & lt_mask);
+                if (pos < kTieCap) tq[pos] = (uint32_t)idx;
             }
-            if (mt) {
-                uint32_t basepos = 0;
-                if (lane == __ffsll((long long)mt) - 1) basepos = atomicAdd(&counters[q * 2 + 1], (uint32_t)__popcll(mt));
-                basepos = __shfl(basepos, __ffsll((long long)mt) - 1);
-                uint32_t pos = basepos + __popcll(mt & lt_mask);
-                if (is_tie && pos < kTieCap) tq[pos] = (uint32_t)idx;
-            }
+            pb += (uint32_t)__popcll(mb[e]);
+            pt += (uint32_t)__popcll(mt[e]);
         }
     }
 }
