@@ -691,9 +691,17 @@ __global__ __launch_bounds__(256) void k_stage1_threshold(const uint16_t* __rest
     __shared__ uint32_t part[256];
     const uint64_t q = blockIdx.x;
     const uint16_t* hp = hist_part + q * nseg * nbins;
+    // segment sums, eight segments' loads in flight per bin
     for (uint32_t bin = threadIdx.x; bin < nbins; bin += blockDim.x) {
-        uint32_t s = 0;
-        for (uint32_t g = 0; g < nseg; g++) s += hp[(uint64_t)g * nbins + bin];
+        uint32_t s = 0, g = 0;
+        for (; g + 8 <= nseg; g += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; k++) v[k] = hp[(uint64_t)(g + k) * nbins + bin];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; k++) s += v[k];
+        }
+        for (; g < nseg; g++) s += hp[(uint64_t)g * nbins + bin];
         cnt[bin] = s;
     }
     __syncthreads();
@@ -701,18 +709,14 @@ __global__ __launch_bounds__(256) void k_stage1_threshold(const uint16_t* __rest
     const uint32_t lo = min(nbins, threadIdx.x * per), hi = min(nbins, lo + per);
     uint32_t local = 0;
     for (uint32_t b = lo; b < hi; b++) local += cnt[b];
-    part[threadIdx.x] = local;
+    // exclusive prefix over the 256 threads: wave scans, then the 4 wave totals
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t wex = wave_excl_scan(local, lane);
+    if (lane == 63) part[wave] = wex + local;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (uint32_t t = 0; t < blockDim.x; t++) {
-            uint32_t v = part[t];
-            part[t] = run;
-            run += v;
-        }
-    }
-    __syncthreads();
-    uint64_t cum = part[threadIdx.x];
+    uint32_t wbase = 0;
+    for (uint32_t w = 0; w < wave; w++) wbase += part[w];
+    uint64_t cum = wbase + wex;
     for (uint32_t b = lo; b < hi; b++) {
         uint64_t c = cnt[b];
         if (cum < m_eff && cum + c >= m_eff) {
