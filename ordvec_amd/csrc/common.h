@@ -224,7 +224,7 @@ struct FastPlan {
     uint32_t scan_wgl;  // VALU scan: survivor list entries per workgroup (0: global appends)
     uint32_t s_target;  // ~0u: admit every document
     uint32_t cap;
-    uint32_t sparse;    // expected survivors at most 1 in 12 documents (the matrix-core scan's lists hold them)
+    uint32_t sparse;    // expected survivors at most 1 in 12-14 documents (the matrix-core scan's lists hold them)
     FastTune tune;      // zero: built-in rules
 };
 struct FastWork {

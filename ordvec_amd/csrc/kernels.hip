@@ -4011,7 +4011,7 @@ FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
         p.nsample = 0;
         p.sparse = 0;
     } else {
-        // Survivors to aim at: min(2.5 m, (m + cap) / 2); where that leaves
+        // Survivors to aim at: min(2 m, (m + cap) / 2); where that leaves
         // under 2x of room on either side (m above cap / 4) the geometric
         // mean of m and cap, the same factor below (count < m) and above
         // (count > cap), with a sample grown (a smaller stride) to about 200
@@ -4020,7 +4020,7 @@ FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
         // (2.87 ms per call), none do now (0.93 ms).
         const bool tight = 4 * m_eff > p.cap;
         const double target_full = tight ? std::sqrt((double)m_eff * (double)p.cap)
-                                         : std::min(2.5 * (double)m_eff, 0.5 * (double)(m_eff + p.cap));
+                                         : std::min(2.0 * (double)m_eff, 0.5 * (double)(m_eff + p.cap));
         uint64_t stride = std::min<uint64_t>(64, std::max<uint64_t>(1, nblk / 256));
         if (tight) stride = std::min<uint64_t>(stride, std::max<uint64_t>(1, (uint64_t)(target_full / 200.0)));
         p.stride = (uint32_t)stride;
@@ -4034,9 +4034,11 @@ FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
             p.s_target = (uint32_t)std::ceil(target_full * sample_docs / (double)n) + 8;
             expect = target_full;
         }
-        // Measured at 1024 queries on mixture data: one survivor in 12
-        // documents stays on the fast path, one in 10 overflows the lists.
-        p.sparse = (double)n >= 12.0 * expect ? 1u : 0u;
+        // Measured at 1024 and 200 queries on mixture data: an exact
+        // threshold (stride 1) keeping one survivor in 12 documents stays on
+        // the fast path; a sampled one, whose count overshoots its aim, needs
+        // one in 14 (one in 12.5 has overflowed the lists).
+        p.sparse = (double)n >= (stride == 1 ? 12.0 : 14.0) * expect ? 1u : 0u;
         uint64_t want = std::max<uint64_t>(1, (1024 + tiles - 1) / tiles);
         uint64_t nsseg = std::min<uint64_t>(std::max<uint64_t>(1, p.nsample), want);
         uint64_t sbps = (p.nsample + nsseg - 1) / nsseg;
