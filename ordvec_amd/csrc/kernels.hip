@@ -1981,7 +1981,8 @@ __global__ __launch_bounds__(kMfmaThreads, 1) void k_s1_sample_mfma(
 constexpr uint32_t kSelBins = 2048, kSelBuf = 4096;
 
 __global__ __launch_bounds__(1024) void k_s1_select(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ cnt,
-                                                    uint32_t cap, uint64_t m_eff, uint32_t* __restrict__ cand,
+                                                    const uint32_t* __restrict__ thr, uint32_t idbits, uint32_t cap,
+                                                    uint64_t m_eff, uint32_t* __restrict__ cand,
                                                     uint32_t* __restrict__ cand_h, uint32_t* __restrict__ qstat) {
     constexpr uint32_t KPT = kFastCap / 1024;  // keys per thread (buffer capacity / threads)
     __shared__ uint32_t hist[kSelBins];        // keys per bin
@@ -2023,7 +2024,17 @@ __global__ __launch_bounds__(1024) void k_s1_select(const uint64_t* __restrict__
     if ((threadIdx.x & 63) == 0) atomicMin(&s_hmin, hmin);
     __syncthreads();
     const uint32_t hbase = count ? s_hmin : 0u;
-    auto bin_of = [&](uint64_t key) { return min((uint32_t)(key >> 32) - hbase, kSelBins - 1); };
+    // bins follow the key order and split each distance by its leading id
+    // bits, as far as the distance range hbase..thr[q] leaves room (s1_cut's
+    // binning), so that a bin holds few keys to rank
+    const uint32_t hcap = thr[q], span = hcap >= hbase ? hcap - hbase + 1 : 1u;
+    uint32_t sbits = 0;
+    while (sbits < idbits && (span << (sbits + 1)) <= kSelBins) sbits++;
+    const uint32_t ishift = idbits - sbits;
+    auto bin_of = [&](uint64_t key) {
+        const uint32_t dh = (uint32_t)(key >> 32) - hbase;
+        return dh >= (kSelBins >> sbits) ? kSelBins - 1 : (dh << sbits | (uint32_t)((key & 0xffffffffull) >> ishift));
+    };
 #pragma unroll
     for (uint32_t j = 0; j < KPT; j++)
         if (kr[j] != kKeyMax) atomicAdd(&hist[bin_of(kr[j])], 1u);
@@ -4346,8 +4357,8 @@ void launch_stage1_fast(const uint64_t* words, uint64_t n, uint32_t d, const flo
         hipLaunchKernelGGL(k_s1_pick, dim3((uint32_t)nq), dim3(256), 0, s, w.keys, w.cnt, w.thr, p.cap,
                            (uint32_t)m_eff, id_bits(n), cand, qstat);
     else
-        hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(1024), 0, s, w.keys, w.cnt, p.cap, m_eff, cand,
-                           cand_h, qstat);
+        hipLaunchKernelGGL(k_s1_select, dim3((uint32_t)nq), dim3(1024), 0, s, w.keys, w.cnt, w.thr, id_bits(n), p.cap,
+                           m_eff, cand, cand_h, qstat);
     check_launch();
     ktime_end(kFamStage1Select, s);
 }
