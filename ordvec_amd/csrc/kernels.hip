@@ -4022,7 +4022,7 @@ FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
         p.nsample = 0;
         p.sparse = 0;
     } else {
-        // Survivors to aim at: min(1.6 m, (m + cap) / 2); where that leaves
+        // Survivors to aim at: min(2 m, (m + cap) / 2); where that leaves
         // under 2x of room on either side (m above cap / 4) the geometric
         // mean of m and cap, the same factor below (count < m) and above
         // (count > cap), with a sample grown (a smaller stride) to about 200
@@ -4031,7 +4031,7 @@ FastPlan plan_stage1_fast(uint64_t n, uint32_t d, uint64_t nq, uint64_t m_eff) {
         // (2.87 ms per call), none do now (0.93 ms).
         const bool tight = 4 * m_eff > p.cap;
         const double target_full = tight ? std::sqrt((double)m_eff * (double)p.cap)
-                                         : std::min(1.6 * (double)m_eff, 0.5 * (double)(m_eff + p.cap));
+                                         : std::min(2.0 * (double)m_eff, 0.5 * (double)(m_eff + p.cap));
         uint64_t stride = std::min<uint64_t>(64, std::max<uint64_t>(1, nblk / 256));
         if (tight) stride = std::min<uint64_t>(stride, std::max<uint64_t>(1, (uint64_t)(target_full / 200.0)));
         p.stride = (uint32_t)stride;
